@@ -1,0 +1,266 @@
+"""MI355X elastic RTM: Python mirror of the C ABI in include/erm.h.
+
+The library (liberm.so, built from csrc/ by `make` in this directory) holds
+the hot path: fused gfx950 HIP kernels for the 3D staggered-grid elastic
+velocity-stress step with vector P/S decomposition, excitation tracking and
+excitation-amplitude imaging, behind the entry points a user of the
+reference would call in place of its host drivers
+(3D_elastic_RTM/3D_forward_or_back_together.cu:9-453).
+
+This module is a thin ctypes binding: the parameter struct mirrors
+`erm_params`, `RTM3D` mirrors an `erm_ctx`, and every call raises
+`ErmError` with the library's message when the C call fails.  There is no
+CPU fallback: if liberm.so is missing, importing raises ImportError.
+
+Because the package directory name starts with a digit, import it with
+``importlib.import_module("2d_and_3d_elastic_reverse_time_migration_amd")``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liberm.so")
+
+from .params import *  # noqa: F401,F403
+from .params import PATH_MAX, Params, FIELD_ID, FIELDS  # noqa: F401
+
+
+class ErmError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"erm error {code}: {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C {_HERE}` (or __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    fp = P(ctypes.c_float)
+    vp = ctypes.c_void_p
+    sig = {
+        "erm_last_error": (ctypes.c_char_p, [vp]),
+        "erm_version": (ctypes.c_int, []),
+        "erm_params_size": (ctypes.c_size_t, []),
+        "erm_device_count": (ctypes.c_int, [P(ctypes.c_int)]),
+        "erm_params_default": (ctypes.c_int, [vp]),
+        "erm_parse_par": (ctypes.c_int, [ctypes.c_int, P(ctypes.c_char_p), vp]),
+        "erm_create": (ctypes.c_int, [vp, ctypes.c_int, P(vp)]),
+        "erm_destroy": (ctypes.c_int, [vp]),
+        "erm_get_dims": (ctypes.c_int, [vp, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]),
+        "erm_load_model": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, fp]),
+        "erm_load_model_files": (ctypes.c_int, [vp]),
+        "erm_set_shot": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "erm_zero_wavefield": (ctypes.c_int, [vp]),
+        "erm_zero_excitation": (ctypes.c_int, [vp]),
+        "erm_forward_real": (ctypes.c_int, [vp, fp]),
+        "erm_forward_smoothed": (ctypes.c_int, [vp]),
+        "erm_backward_smoothed": (ctypes.c_int, [vp, fp]),
+        "erm_set_record": (ctypes.c_int, [vp, fp]),
+        "erm_get_record": (ctypes.c_int, [vp, fp]),
+        "erm_rtm_step": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "erm_get_field": (ctypes.c_int, [vp, ctypes.c_int, fp]),
+        "erm_set_field": (ctypes.c_int, [vp, ctypes.c_int, fp]),
+        "erm_get_images": (ctypes.c_int, [vp, fp, fp, fp]),
+        "erm_write_results": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "erm_write_excitation": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "erm_set_profiling": (ctypes.c_int, [vp, ctypes.c_int]),
+        "erm_get_timing": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_long),
+                                          P(ctypes.c_double), P(ctypes.c_long)]),
+        "erm_algorithmic_bytes": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]),
+        "erm_synchronize": (ctypes.c_int, [vp]),
+        "erm_make_model3d": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
+                                            fp, fp, fp]),
+        "erm_write_raw": (ctypes.c_int, [ctypes.c_char_p, fp, ctypes.c_size_t]),
+        "erm_read_raw": (ctypes.c_int, [ctypes.c_char_p, fp, ctypes.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.erm_params_size() != ctypes.sizeof(Params):
+        raise ImportError(f"erm_params size mismatch: C {lib.erm_params_size()} vs Python {ctypes.sizeof(Params)}")
+    return lib
+
+
+lib = _load()
+
+
+def _fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "float32 C-contiguous array required"
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _pp(p):
+    """Address of an erm_params-layout struct (any Params class instance)."""
+    if ctypes.sizeof(p) != ctypes.sizeof(Params):
+        raise TypeError("not an erm_params struct")
+    return ctypes.addressof(p)
+
+
+def _check(rc: int, ctx=None):
+    if rc != 0:
+        raise ErmError(rc, lib.erm_last_error(ctx).decode(errors="replace"))
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    _check(lib.erm_params_default(_pp(p)))
+    return p.update(**kw)
+
+
+def parse_par(args) -> Params:
+    """getpar-compatible parsing of `key=value` / `par=file` arguments."""
+    p = Params()
+    arr = (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
+    _check(lib.erm_parse_par(len(args), arr, _pp(p)))
+    return p
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    lib.erm_device_count(ctypes.byref(n))
+    return n.value
+
+
+def make_model3d(kind, nx, ny, nz, dx, dy, dz, seed):
+    """Synthetic model, model region in disk order [nx][ny][nz]."""
+    vp = np.empty((nx, ny, nz), np.float32)
+    vs = np.empty_like(vp)
+    rho = np.empty_like(vp)
+    _check(lib.erm_make_model3d(kind, nx, ny, nz, dx, dy, dz, seed, _fptr(vp), _fptr(vs), _fptr(rho)))
+    return vp, vs, rho
+
+
+def write_raw(path, a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    _check(lib.erm_write_raw(path.encode(), _fptr(a), a.size))
+
+
+def read_raw(path, shape):
+    a = np.empty(shape, np.float32)
+    _check(lib.erm_read_raw(path.encode(), _fptr(a), a.size))
+    return a
+
+
+class RTM3D:
+    """One device-resident 3D elastic RTM context (≙ GPUdevice)."""
+
+    def __init__(self, params: Params, device: int = 0):
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(lib.erm_create(_pp(params), device, ctypes.byref(h)))
+        self._h = h
+        nnx, nny, nnz, wl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        lib.erm_get_dims(h, ctypes.byref(nnx), ctypes.byref(nny), ctypes.byref(nnz), ctypes.byref(wl))
+        self.shape = (nnz.value, nny.value, nnx.value)
+        self.wavelet_length = wl.value
+        self.nrec = params.receiver_num_x * params.receiver_num_y
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.erm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        _check(rc, self._h)
+
+    def load_model(self, which, vp, vs=None, rho=None):
+        self._c(lib.erm_load_model(self._h, which, _fptr(np.ascontiguousarray(vp, np.float32)),
+                                   _fptr(None if vs is None else np.ascontiguousarray(vs, np.float32)),
+                                   _fptr(None if rho is None else np.ascontiguousarray(rho, np.float32))))
+
+    def load_model_files(self):
+        self._c(lib.erm_load_model_files(self._h))
+
+    def set_shot(self, isx=0, isy=0, isz=0):
+        self._c(lib.erm_set_shot(self._h, isx, isy, isz))
+
+    def zero_wavefield(self):
+        self._c(lib.erm_zero_wavefield(self._h))
+
+    def zero_excitation(self):
+        self._c(lib.erm_zero_excitation(self._h))
+
+    def record_shape(self):
+        p = self.params
+        return (3, p.lt, p.receiver_num_y, p.receiver_num_x)
+
+    def forward_real(self, want_record=True):
+        rec = np.empty(self.record_shape(), np.float32) if want_record else None
+        self._c(lib.erm_forward_real(self._h, _fptr(rec)))
+        return rec
+
+    def forward_smoothed(self):
+        self._c(lib.erm_forward_smoothed(self._h))
+
+    def backward_smoothed(self, rec=None):
+        self._c(lib.erm_backward_smoothed(self._h, _fptr(None if rec is None else np.ascontiguousarray(rec, np.float32))))
+
+    def set_record(self, rec):
+        self._c(lib.erm_set_record(self._h, _fptr(np.ascontiguousarray(rec, np.float32))))
+
+    def get_record(self):
+        rec = np.empty(self.record_shape(), np.float32)
+        self._c(lib.erm_get_record(self._h, _fptr(rec)))
+        return rec
+
+    def step(self, pass_, it_begin, nsteps):
+        self._c(lib.erm_rtm_step(self._h, pass_, it_begin, nsteps))
+
+    def get_field(self, name):
+        out = np.empty(self.shape, np.float32)
+        self._c(lib.erm_get_field(self._h, FIELD_ID[name] if isinstance(name, str) else name, _fptr(out)))
+        return out
+
+    def set_field(self, name, a):
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.shape == self.shape
+        self._c(lib.erm_set_field(self._h, FIELD_ID[name] if isinstance(name, str) else name, _fptr(a)))
+
+    def get_images(self):
+        p = self.params
+        out = [np.empty((p.nx, p.ny, p.nz), np.float32) for _ in range(3)]
+        self._c(lib.erm_get_images(self._h, *[_fptr(o) for o in out]))
+        return out
+
+    def write_results(self, d):
+        self._c(lib.erm_write_results(self._h, d.encode()))
+
+    def write_excitation(self, d):
+        self._c(lib.erm_write_excitation(self._h, d.encode()))
+
+    def set_profiling(self, on=True):
+        self._c(lib.erm_set_profiling(self._h, 1 if on else 0))
+
+    def timing(self):
+        a, b, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        n1, n2 = ctypes.c_long(), ctypes.c_long()
+        self._c(lib.erm_get_timing(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n1),
+                                   ctypes.byref(d), ctypes.byref(n2)))
+        return {"pass_ms": a.value, "stress_ms": b.value, "stress_launches": n1.value,
+                "velocity_ms": d.value, "velocity_launches": n2.value}
+
+    def algorithmic_bytes(self, pass_):
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._c(lib.erm_algorithmic_bytes(self._h, pass_, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def synchronize(self):
+        self._c(lib.erm_synchronize(self._h))

@@ -1,0 +1,803 @@
+// rtm3d.hip - the 3D elastic RTM context behind the C ABI of include/erm.h.
+//
+// Replaces the reference's GPUdevice struct and allocate/zero helpers
+// (3D_elastic_RTM/3D_elastic_modeling_typedef_struct:1-293), the three host
+// time loops (3D_forward_or_back_together.cu:9-155,161-290,296-453) and the
+// result writers (3D_output_file.cu:155-250,309-428) with one device-resident
+// context per GPU.  Differences by design:
+//  - the whole padded grid lives on one GPU (no z-slab split, no halo
+//    exchange: the reference's exchange_device_nz_kernel_* are not needed);
+//  - the receiver record stays in HBM for the whole shot instead of one
+//    time slice crossing PCIe per step (:140,318);
+//  - two fused kernels per step, no device-wide synchronisation inside a pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels3d.hpp"
+
+namespace erm {
+
+static thread_local std::string g_err;
+
+std::string vformat(const char* fmt, va_list ap) {
+    char buf[1024];
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    return std::string(buf);
+}
+std::string format(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::string s = vformat(fmt, ap);
+    va_end(ap);
+    return s;
+}
+void set_global_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    g_err = vformat(fmt, ap);
+    va_end(ap);
+}
+const char* global_error() { return g_err.c_str(); }
+
+// Reference radius-6 optimized coefficients (3D_elastic_RTM/zzzzz:205-210)
+// and radius-4 staggered Taylor coefficients.
+static const double kCoeR6[7] = {0.0, 1.2213363647, -0.096931457519, 0.017447662353,
+                                 -0.0029672895159, 0.00035900539822, -0.00002184781161};
+static const double kCoeR4[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0};
+
+}  // namespace erm
+
+using namespace erm;
+
+struct erm_ctx {
+    erm_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int R = 6;
+    int nnx = 0, nny = 0, nnz = 0, px = 0;
+    long pxy = 0;
+    size_t ncell = 0;   // allocated floats per volume (nnz*pxy)
+    Coef coef{};
+    float cx = 0, cy = 0, cz = 0;
+    // device buffers
+    float* f[13] = {nullptr};
+    float* M[2] = {nullptr, nullptr};
+    float* MU[2] = {nullptr, nullptr};
+    float* BU[2] = {nullptr, nullptr};
+    bool have_model[2] = {false, false};
+    float* hxyz[3] = {nullptr, nullptr, nullptr};
+    float* wavelet = nullptr;
+    int wl = 0;
+    std::vector<float> wavelet_h;
+    int* ex_time = nullptr;
+    float* ex[5] = {nullptr};   // amp tp vxp vyp vzp
+    float* img[3] = {nullptr};  // tp pp ps
+    float* rec = nullptr;       // [3][lt][nr]
+    long* rec_idx = nullptr;
+    int nr = 0;
+    bool have_record = false;
+    int sxr = 0, syr = 0, szr = 0;  // shot position in model coordinates (sx_real, ...)
+    long src = -1;                  // padded index of the source
+    int tx = 64, ty = 8, kchunk = 32;
+    std::string err;
+    // timing
+    hipEvent_t pass_ev[2] = {nullptr, nullptr};
+    bool profiling = false;
+    std::vector<hipEvent_t> kev;  // 3 per step when profiling
+    double last_pass_ms = 0, last_stress_ms = 0, last_velocity_ms = 0;
+    long last_stress_n = 0, last_velocity_n = 0;
+
+    int fail(int code, const char* fmt, ...) {
+        va_list ap;
+        va_start(ap, fmt);
+        err = vformat(fmt, ap);
+        va_end(ap);
+        return code;
+    }
+};
+
+#define HIPC(ctx, call)                                                                          \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return (ctx)->fail(ERM_E_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                               __FILE__, __LINE__);                                              \
+    } while (0)
+
+namespace {
+
+Geo make_geo(const erm_ctx* c) {
+    Geo g;
+    g.nnx = c->nnx; g.nny = c->nny; g.nnz = c->nnz;
+    g.px = c->px; g.pxy = c->pxy;
+    g.cx = c->cx; g.cy = c->cy; g.cz = c->cz;
+    g.hx = c->hxyz[0]; g.hy = c->hxyz[1]; g.hz = c->hxyz[2];
+    g.kchunk = c->kchunk;
+    return g;
+}
+
+Wave make_wave(const erm_ctx* c) {
+    Wave w;
+    float* const* f = c->f;
+    w.vx = f[0]; w.vy = f[1]; w.vz = f[2];
+    w.txx = f[3]; w.tyy = f[4]; w.tzz = f[5]; w.txy = f[6]; w.txz = f[7]; w.tyz = f[8];
+    w.tp = f[9]; w.vxp = f[10]; w.vyp = f[11]; w.vzp = f[12];
+    return w;
+}
+
+Excite make_excite(const erm_ctx* c) {
+    Excite e;
+    e.time = c->ex_time;
+    e.amp = c->ex[0]; e.tp = c->ex[1]; e.vxp = c->ex[2]; e.vyp = c->ex[3]; e.vzp = c->ex[4];
+    e.img_tp = c->img[0]; e.img_pp = c->img[1]; e.img_ps = c->img[2];
+    return e;
+}
+
+// 1-cos face profile of coe_attenuation_3d_new2 (3D_zzzzz:284-313), pai = 3.1415926.
+void face_profile(int b, double cA, bool rising, std::vector<double>& d) {
+    const double pai = 3.1415926;
+    d.assign(b, 0.0);
+    for (int i = 0; i < b; ++i) {
+        double s = rising ? (double)i / (b - 1) : (double)(b - i - 1) / (b - 1);
+        d[i] = cA * (1.0 - std::cos(pai / 2 * s));
+    }
+}
+
+// make_ricker_new (3D_elastic_RTM/zzzzz:136-157), dt in seconds, cwp PI.
+std::vector<float> ricker(float freq, float dt_s) {
+    const double PI = 3.141592653589793;
+    if (freq == 0.0f) freq = 30.0f;
+    if (dt_s == 0.0f) dt_s = 0.004f;
+    float B = (float)(std::sqrt(6.0) / (PI * freq));
+    int N = (int)std::ceil(1.35 * (double)B / (double)dt_s);
+    std::vector<float> w(2 * N + 1);
+    w[N] = 1.0f;
+    for (int i = 1; i <= N; ++i) {
+        float t = dt_s * (float)i;
+        float u = (float)(2.0 * std::sqrt(6.0)) * t / B;
+        w[N + i] = w[N - i] = 0.5f * (2.0f - u * u) * std::exp(-u * u / 4.0f);
+    }
+    return w;
+}
+
+
+template <int R, int TX, int TY>
+void stress_launch(erm_ctx* c, int which, long src, int it) {
+    const int src_k = src < 0 ? -1 : (int)(src / c->pxy), src_col = src < 0 ? -1 : (int)(src % c->pxy);
+    Geo g = make_geo(c);
+    dim3 blk(TX, TY);
+    int nzi = c->nnz - 2 * R;
+    dim3 grd((c->nnx + TX - 1) / TX, (c->nny + TY - 1) / TY, (nzi + c->kchunk - 1) / c->kchunk);
+    hipLaunchKernelGGL((k_stress<R, TX, TY>), grd, blk, 0, c->stream, make_wave(c), c->M[which], c->MU[which], g,
+                       c->coef, src_k, src_col, c->wavelet, c->wl, it);
+}
+
+template <int R, int TX, int TY, int MODE>
+void velocity_launch(erm_ctx* c, int which, int it) {
+    Geo g = make_geo(c);
+    dim3 blk(TX, TY);
+    int nzi = c->nnz - 2 * R;
+    dim3 grd((c->nnx + TX - 1) / TX, (c->nny + TY - 1) / TY, (nzi + c->kchunk - 1) / c->kchunk);
+    hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE>), grd, blk, 0, c->stream, make_wave(c), c->BU[which], g,
+                       c->coef, make_excite(c), it);
+}
+
+void do_stress(erm_ctx* c, int which, long src, int it) {
+    if (c->R == 4) {
+        if (c->ty == 4) stress_launch<4, 64, 4>(c, which, src, it);
+        else stress_launch<4, 64, 8>(c, which, src, it);
+    } else {
+        if (c->ty == 4) stress_launch<6, 64, 4>(c, which, src, it);
+        else stress_launch<6, 64, 8>(c, which, src, it);
+    }
+}
+
+template <int MODE>
+void do_velocity_m(erm_ctx* c, int which, int it) {
+    if (c->R == 4) {
+        if (c->ty == 4) velocity_launch<4, 64, 4, MODE>(c, which, it);
+        else velocity_launch<4, 64, 8, MODE>(c, which, it);
+    } else {
+        if (c->ty == 4) velocity_launch<6, 64, 4, MODE>(c, which, it);
+        else velocity_launch<6, 64, 8, MODE>(c, which, it);
+    }
+}
+
+void do_velocity(erm_ctx* c, int which, int mode, int it) {
+    if (mode == MODE_EXCITE) do_velocity_m<MODE_EXCITE>(c, which, it);
+    else if (mode == MODE_IMAGE) do_velocity_m<MODE_IMAGE>(c, which, it);
+    else do_velocity_m<MODE_PLAIN>(c, which, it);
+}
+
+int fill(erm_ctx* c, float* p, size_t n, float v) {
+    if (!p) return 0;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(256), 0, c->stream, p, (long)n, v);
+    HIPC(c, hipGetLastError());
+    return 0;
+}
+int filli(erm_ctx* c, int* p, size_t n, int v) {
+    if (!p) return 0;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(256), 0, c->stream, p, (long)n, v);
+    HIPC(c, hipGetLastError());
+    return 0;
+}
+
+int check_grid(const erm_params* p, int R, std::string* err) {
+    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0) { *err = "nx, ny, nz must be positive"; return ERM_E_PARAM; }
+    if (p->dx <= 0 || p->dy <= 0 || p->dz <= 0 || p->dt <= 0) { *err = "dx, dy, dz, dt must be positive"; return ERM_E_PARAM; }
+    if (p->lt <= 0) { *err = "lt must be positive"; return ERM_E_PARAM; }
+    int b[6] = {p->bu, p->bd, p->bl, p->br, p->bf, p->bb};
+    for (int v : b)
+        if (v < 0 || v == 1) { *err = "boundary widths must be 0 or >= 2"; return ERM_E_PARAM; }
+    long nnx = p->nx + p->bl + p->br, nny = p->ny + p->bf + p->bb, nnz = p->nz + p->bu + p->bd;
+    if (nnx < 2 * R + 1 || nny < 2 * R + 1 || nnz < 2 * R + 1) { *err = "padded grid smaller than the stencil"; return ERM_E_SIZE; }
+    if (nnx * nny * nnz > (1L << 34)) { *err = "grid too large"; return ERM_E_SIZE; }
+    if (p->receiver_num_x < 0 || p->receiver_num_y < 0) { *err = "receiver counts must be >= 0"; return ERM_E_PARAM; }
+    if (p->boundary != ERM_BOUNDARY_DAMP) { *err = "boundary=1 (CPML) is not available in this build"; return ERM_E_PARAM; }
+    return 0;
+}
+
+template <typename T>
+int dmalloc(erm_ctx* c, T** ptr, size_t n) {
+    if (n == 0) { *ptr = nullptr; return 0; }
+    hipError_t e = hipMalloc((void**)ptr, n * sizeof(T));
+    if (e != hipSuccess) return c->fail(ERM_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
+void free_all(erm_ctx* c) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    for (auto& p : c->f) F(p);
+    for (int w = 0; w < 2; ++w) { F(c->M[w]); F(c->MU[w]); F(c->BU[w]); }
+    for (auto& p : c->hxyz) F(p);
+    F(c->wavelet); F(c->ex_time);
+    for (auto& p : c->ex) F(p);
+    for (auto& p : c->img) F(p);
+    F(c->rec); F(c->rec_idx);
+    for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->kev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// download a padded-grid volume into a dense nnx*nny*nnz host array
+int download(erm_ctx* c, const float* d, float* h) {
+    HIPC(c, hipMemcpy2DAsync(h, (size_t)c->nnx * sizeof(float), d, (size_t)c->px * sizeof(float),
+                             (size_t)c->nnx * sizeof(float), (size_t)c->nny * c->nnz, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+int upload(erm_ctx* c, float* d, const float* h) {
+    HIPC(c, hipMemcpy2DAsync(d, (size_t)c->px * sizeof(float), h, (size_t)c->nnx * sizeof(float),
+                             (size_t)c->nnx * sizeof(float), (size_t)c->nny * c->nnz, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// edge-replicated padding of a model-region array in disk order [nx][ny][nz]
+// into the dense padded x-fastest layout (add_pml_layers_v_h, 3D_zzzzz:141-259).
+void pad_model(const erm_params& p, int nnx, int nny, int nnz, const float* src, std::vector<float>& dst) {
+    dst.resize((size_t)nnx * nny * nnz);
+    for (int k = 0; k < nnz; ++k) {
+        int kz = std::min(std::max(k - p.bu, 0), p.nz - 1);
+        for (int j = 0; j < nny; ++j) {
+            int jy = std::min(std::max(j - p.bf, 0), p.ny - 1);
+            for (int i = 0; i < nnx; ++i) {
+                int ix = std::min(std::max(i - p.bl, 0), p.nx - 1);
+                dst[((size_t)k * nny + j) * nnx + i] = src[((size_t)ix * p.ny + jy) * p.nz + kz];
+            }
+        }
+    }
+}
+
+int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps) {
+    if (pass < ERM_PASS_MODEL || pass > ERM_PASS_RECEIVER) return c->fail(ERM_E_PARAM, "unknown pass %d", pass);
+    const int which = pass == ERM_PASS_MODEL ? 0 : 1;
+    if (!c->have_model[which]) return c->fail(ERM_E_STATE, "%s model not loaded", which ? "smoothed" : "true");
+    if (nsteps < 0) return c->fail(ERM_E_PARAM, "nsteps < 0");
+    if (pass == ERM_PASS_RECEIVER) {
+        if (!c->have_record) return c->fail(ERM_E_STATE, "no receiver record: run erm_forward_real or erm_set_record first");
+        if (it_begin >= c->p.lt || it_begin - nsteps + 1 < 0) return c->fail(ERM_E_PARAM, "receiver pass time range out of [0, lt)");
+    } else {
+        if (it_begin < 0) return c->fail(ERM_E_PARAM, "it_begin < 0");
+        if (pass == ERM_PASS_MODEL && it_begin + nsteps > c->p.lt) return c->fail(ERM_E_PARAM, "model pass beyond lt");
+    }
+    const long comp = (long)c->p.lt * c->nr;
+    const int rb = 256, rg = (c->nr + rb - 1) / rb;
+    if (c->profiling) {
+        size_t need = (size_t)3 * nsteps;
+        while (c->kev.size() < need) {
+            hipEvent_t e;
+            HIPC(c, hipEventCreate(&e));
+            c->kev.push_back(e);
+        }
+    }
+    HIPC(c, hipEventRecord(c->pass_ev[0], c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        const int it = pass == ERM_PASS_RECEIVER ? it_begin - s : it_begin + s;
+        if (pass == ERM_PASS_RECEIVER && c->nr > 0)
+            hipLaunchKernelGGL(k_inject, dim3(rg), dim3(rb), 0, c->stream, c->f[0], c->f[1], c->f[2], c->rec_idx, c->nr,
+                               c->rec, comp, (long)it * c->nr);
+        if (c->profiling) HIPC(c, hipEventRecord(c->kev[3 * s], c->stream));
+        do_stress(c, which, pass == ERM_PASS_RECEIVER ? -1L : c->src, it);
+        if (c->profiling) HIPC(c, hipEventRecord(c->kev[3 * s + 1], c->stream));
+        do_velocity(c, which, pass == ERM_PASS_SOURCE ? MODE_EXCITE : pass == ERM_PASS_RECEIVER ? MODE_IMAGE : MODE_PLAIN, it);
+        if (c->profiling) HIPC(c, hipEventRecord(c->kev[3 * s + 2], c->stream));
+        if (pass == ERM_PASS_MODEL && c->nr > 0)
+            hipLaunchKernelGGL(k_record, dim3(rg), dim3(rb), 0, c->stream, c->f[0], c->f[1], c->f[2], c->rec_idx, c->nr,
+                               c->rec, comp, (long)it * c->nr);
+    }
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipEventRecord(c->pass_ev[1], c->stream));
+    HIPC(c, hipEventSynchronize(c->pass_ev[1]));
+    float ms = 0;
+    HIPC(c, hipEventElapsedTime(&ms, c->pass_ev[0], c->pass_ev[1]));
+    c->last_pass_ms = ms;
+    c->last_stress_ms = c->last_velocity_ms = 0;
+    c->last_stress_n = c->last_velocity_n = 0;
+    if (c->profiling) {
+        for (int s = 0; s < nsteps; ++s) {
+            float a = 0, b = 0;
+            HIPC(c, hipEventElapsedTime(&a, c->kev[3 * s], c->kev[3 * s + 1]));
+            HIPC(c, hipEventElapsedTime(&b, c->kev[3 * s + 1], c->kev[3 * s + 2]));
+            c->last_stress_ms += a;
+            c->last_velocity_ms += b;
+        }
+        c->last_stress_n = c->last_velocity_n = nsteps;
+    }
+    if (pass == ERM_PASS_MODEL) c->have_record = true;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* erm_last_error(const erm_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
+int erm_version(void) { return 100; }
+size_t erm_params_size(void) { return sizeof(erm_params); }
+
+int erm_device_count(int* n) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        set_global_error("hipGetDeviceCount: %s", hipGetErrorString(e));
+        *n = 0;
+        return ERM_E_DEVICE;
+    }
+    *n = c;
+    return 0;
+}
+
+int erm_params_default(erm_params* p) {
+    if (!p) return ERM_E_PARAM;
+    params_default(p);
+    return 0;
+}
+
+int erm_parse_par(int argc, const char* const* argv, erm_params* p) {
+    if (!p) return ERM_E_PARAM;
+    std::string err;
+    int rc = parse_par(argc, argv, p, &err);
+    if (rc) set_global_error("%s", err.c_str());
+    return rc;
+}
+
+int erm_create(const erm_params* pin, int device, erm_ctx** out) {
+    if (!pin || !out) { set_global_error("null argument"); return ERM_E_PARAM; }
+    *out = nullptr;
+    int R = pin->fd_order == 8 ? 4 : pin->fd_order == 12 ? 6 : -1;
+    if (R < 0) { set_global_error("fd_order must be 8 or 12"); return ERM_E_PARAM; }
+    std::string err;
+    int rc = check_grid(pin, R, &err);
+    if (rc) { set_global_error("%s", err.c_str()); return rc; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_global_error("no HIP device"); return ERM_E_DEVICE; }
+    if (device < 0 || device >= ndev) { set_global_error("device %d out of range (%d devices)", device, ndev); return ERM_E_PARAM; }
+    erm_ctx* c = new erm_ctx();
+    c->p = *pin;
+    if (c->p.lt_forward < 0) c->p.lt_forward = 2 * c->p.lt / 3;
+    c->device = device;
+    c->R = R;
+    c->nnx = pin->nx + pin->bl + pin->br;
+    c->nny = pin->ny + pin->bf + pin->bb;
+    c->nnz = pin->nz + pin->bu + pin->bd;
+    c->px = c->nnx;
+    c->pxy = (long)c->px * c->nny;
+    c->ncell = (size_t)c->pxy * c->nnz;
+    for (int m = 0; m <= 6; ++m) c->coef.c[m] = 0.f;
+    for (int m = 0; m <= R; ++m) c->coef.c[m] = (float)(R == 6 ? kCoeR6[m] : kCoeR4[m]);
+    const double dts = pin->dt / 1000.0;
+    c->cx = (float)(dts / pin->dx);
+    c->cy = (float)(dts / pin->dy);
+    c->cz = (float)(dts / pin->dz);
+    c->ty = pin->tile_y == 4 ? 4 : 8;
+    {
+        int tiles = ((c->nnx + 63) / 64) * ((c->nny + c->ty - 1) / c->ty);
+        int nzi = c->nnz - 2 * R;
+        int chunks = std::max(1, (2048 + tiles - 1) / tiles);
+        int kc = (nzi + chunks - 1) / chunks;
+        kc = std::max(kc, std::min(nzi, 16));
+        c->kchunk = pin->zchunk > 0 ? std::min(pin->zchunk, nzi) : kc;
+    }
+#define CK(x)                         \
+    do {                              \
+        int r_ = (x);                 \
+        if (r_) {                     \
+            set_global_error("%s", c->err.c_str()); \
+            free_all(c);              \
+            delete c;                 \
+            return r_;                \
+        }                             \
+    } while (0)
+    if (hipSetDevice(device) != hipSuccess) { set_global_error("hipSetDevice failed"); delete c; return ERM_E_DEVICE; }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { set_global_error("stream create failed"); delete c; return ERM_E_DEVICE; }
+    for (auto& e : c->pass_ev)
+        if (hipEventCreate(&e) != hipSuccess) { set_global_error("event create failed"); free_all(c); delete c; return ERM_E_DEVICE; }
+    for (auto& p : c->f) CK(dmalloc(c, &p, c->ncell));
+    for (auto& p : c->ex) CK(dmalloc(c, &p, c->ncell));
+    for (auto& p : c->img) CK(dmalloc(c, &p, c->ncell));
+    CK(dmalloc(c, &c->ex_time, c->ncell));
+    // damping, h = d*dt/2 per axis (3D_zzzzz:261-369)
+    {
+        std::vector<double> du, dd, dl, dr, df, db;
+        double cA = pin->coe_attenuation;
+        face_profile(pin->bu, cA, false, du); face_profile(pin->bd, cA, true, dd);
+        face_profile(pin->bl, cA, false, dl); face_profile(pin->br, cA, true, dr);
+        face_profile(pin->bf, cA, false, df); face_profile(pin->bb, cA, true, db);
+        std::vector<float> ax(c->nnx, 0.f), ay(c->nny, 0.f), az(c->nnz, 0.f);
+        for (int i = 0; i < pin->bl; ++i) ax[i] += (float)dl[i];
+        for (int i = 0; i < pin->br; ++i) ax[pin->bl + pin->nx + i] += (float)dr[i];
+        for (int j = 0; j < pin->bf; ++j) ay[j] += (float)df[j];
+        for (int j = 0; j < pin->bb; ++j) ay[pin->bf + pin->ny + j] += (float)db[j];
+        for (int k = 0; k < pin->bu; ++k) az[k] += (float)du[k];
+        for (int k = 0; k < pin->bd; ++k) az[pin->bu + pin->nz + k] += (float)dd[k];
+        // the kernel forms h = hx+hy+hz; h = (ax+ay+az)*dt/2 up to rounding
+        const float half = (float)(dts / 2.0);
+        for (auto* v : {&ax, &ay, &az}) for (auto& x : *v) x *= half;
+        std::vector<float>* axes[3] = {&ax, &ay, &az};
+        for (int a = 0; a < 3; ++a) {
+            CK(dmalloc(c, &c->hxyz[a], axes[a]->size()));
+            if (hipMemcpy(c->hxyz[a], axes[a]->data(), axes[a]->size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+                set_global_error("hipMemcpy damping failed"); free_all(c); delete c; return ERM_E_DEVICE;
+            }
+        }
+    }
+    c->wavelet_h = ricker(pin->freq, (float)dts);
+    c->wl = (int)c->wavelet_h.size();
+    CK(dmalloc(c, &c->wavelet, c->wavelet_h.size()));
+    if (hipMemcpy(c->wavelet, c->wavelet_h.data(), c->wavelet_h.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        set_global_error("hipMemcpy wavelet failed"); free_all(c); delete c; return ERM_E_DEVICE;
+    }
+    // receivers on the surface grid (write_or_add_shot_3D_surface_three, :127)
+    c->nr = pin->receiver_num_x * pin->receiver_num_y;
+    if (c->nr > 0) {
+        std::vector<long> idx(c->nr);
+        for (int iry = 0; iry < pin->receiver_num_y; ++iry)
+            for (int irx = 0; irx < pin->receiver_num_x; ++irx) {
+                int i = pin->bl + pin->receiver_start_x + irx * pin->receiver_interval_x;
+                int j = pin->bf + pin->receiver_start_y + iry * pin->receiver_interval_y;
+                int k = pin->bu + pin->receiver_start_z;
+                if (i < 0 || i >= c->nnx || j < 0 || j >= c->nny || k < 0 || k >= c->nnz) {
+                    set_global_error("receiver (%d,%d) outside the grid", irx, iry);
+                    free_all(c); delete c; return ERM_E_PARAM;
+                }
+                idx[(size_t)iry * pin->receiver_num_x + irx] = (long)k * c->pxy + (long)j * c->px + i;
+            }
+        CK(dmalloc(c, &c->rec_idx, idx.size()));
+        if (hipMemcpy(c->rec_idx, idx.data(), idx.size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess) {
+            set_global_error("hipMemcpy receivers failed"); free_all(c); delete c; return ERM_E_DEVICE;
+        }
+        CK(dmalloc(c, &c->rec, (size_t)3 * pin->lt * c->nr));
+    }
+    CK(erm_zero_wavefield(c));
+    CK(erm_zero_excitation(c));
+    for (auto& p : c->img) CK(fill(c, p, c->ncell, 0.f));
+    CK(erm_set_shot(c, 0, 0, 0));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { set_global_error("init sync failed"); free_all(c); delete c; return ERM_E_DEVICE; }
+#undef CK
+    *out = c;
+    return 0;
+}
+
+int erm_destroy(erm_ctx* c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return 0;
+}
+
+int erm_get_dims(const erm_ctx* c, int* nnx, int* nny, int* nnz, int* wl) {
+    if (!c) return ERM_E_PARAM;
+    if (nnx) *nnx = c->nnx;
+    if (nny) *nny = c->nny;
+    if (nnz) *nnz = c->nnz;
+    if (wl) *wl = c->wl;
+    return 0;
+}
+
+int erm_load_model(erm_ctx* c, int which, const float* vp, const float* vs, const float* rho) {
+    if (!c) return ERM_E_PARAM;
+    if (which < 0 || which > 1 || !vp) return c->fail(ERM_E_PARAM, "erm_load_model: bad arguments");
+    HIPC(c, hipSetDevice(c->device));
+    std::vector<float> pvp, pvs, prho;
+    pad_model(c->p, c->nnx, c->nny, c->nnz, vp, pvp);
+    const size_t N = pvp.size();
+    if (vs) pad_model(c->p, c->nnx, c->nny, c->nnz, vs, pvs);
+    else { pvs.resize(N); for (size_t n = 0; n < N; ++n) pvs[n] = (float)(pvp[n] / std::sqrt(3.0)); }
+    if (rho) pad_model(c->p, c->nnx, c->nny, c->nnz, rho, prho);
+    else { prho.resize(N); for (size_t n = 0; n < N; ++n) prho[n] = (float)(1000 * 0.23 * std::sqrt(std::sqrt((double)pvp[n]))); }
+    for (size_t n = 0; n < N; ++n)
+        if (!(prho[n] > 0.f) || !(pvp[n] > 0.f) || pvs[n] < 0.f)
+            return c->fail(ERM_E_PARAM, "model has non-positive vp/rho or negative vs at padded cell %zu", n);
+    std::vector<float> hM(N), hU(N), hB(N);
+    for (size_t n = 0; n < N; ++n) {
+        float r = prho[n], a = pvp[n], b = pvs[n];
+        hM[n] = r * a * a;
+        hU[n] = r * b * b;
+        hB[n] = 1.0f / r;
+    }
+    if (!c->M[which]) {
+        if (dmalloc(c, &c->M[which], c->ncell) || dmalloc(c, &c->MU[which], c->ncell) || dmalloc(c, &c->BU[which], c->ncell))
+            return ERM_E_NOMEM;
+    }
+    if (upload(c, c->M[which], hM.data()) || upload(c, c->MU[which], hU.data()) || upload(c, c->BU[which], hB.data()))
+        return ERM_E_DEVICE;
+    c->have_model[which] = true;
+    return 0;
+}
+
+int erm_load_model_files(erm_ctx* c) {
+    if (!c) return ERM_E_PARAM;
+    const erm_params& p = c->p;
+    const size_t n = (size_t)p.nx * p.ny * p.nz;
+    std::vector<float> vp(n), vs(n), rho(n);
+    for (int which = 0; which < 2; ++which) {
+        const char* fv = which ? p.s_velocity : p.velocity;
+        const char* fs = which ? p.s_velocity1 : p.velocity1;
+        const char* fr = which ? p.s_density : p.density;
+        if (which == 1 && (p.smooth_time_vp != 0 || !fv[0]))
+            return c->fail(ERM_E_PARAM, "smoothed models must be supplied as files (smooth_time_*=0); "
+                                        "the SU smooth3d path is not part of this library");
+        if (read_raw(fv, vp.data(), n)) return c->fail(ERM_E_IO, "Can not open this file : %s", fv);
+        bool use_vs = p.join_vs != 0, use_rho = p.join_den != 0;
+        if (use_vs && read_raw(fs, vs.data(), n)) return c->fail(ERM_E_IO, "Can not open this file : %s", fs);
+        if (use_rho && read_raw(fr, rho.data(), n)) return c->fail(ERM_E_IO, "Can not open this file : %s", fr);
+        int rc = erm_load_model(c, which, vp.data(), use_vs ? vs.data() : nullptr, use_rho ? rho.data() : nullptr);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int erm_set_shot(erm_ctx* c, int isx, int isy, int isz) {
+    if (!c) return ERM_E_PARAM;
+    const erm_params& p = c->p;
+    int sxr = p.shot_start_x + isx * p.shot_interval_x;
+    int syr = p.shot_start_y + isy * p.shot_interval_y;
+    int szr = p.shot_start_z + isz * p.shot_interval_z;
+    int i = p.bl + sxr, j = p.bf + syr, k = p.bu + szr;
+    if (i < c->R || i >= c->nnx - c->R || j < c->R || j >= c->nny - c->R || k < c->R || k >= c->nnz - c->R)
+        return c->fail(ERM_E_PARAM, "source (%d,%d,%d) is not an updated cell of the padded grid", i, j, k);
+    c->sxr = sxr; c->syr = syr; c->szr = szr;
+    c->src = (long)k * c->pxy + (long)j * c->px + i;
+    return 0;
+}
+
+int erm_zero_wavefield(erm_ctx* c) {
+    if (!c) return ERM_E_PARAM;
+    for (auto& p : c->f) if (fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
+    return 0;
+}
+
+int erm_zero_excitation(erm_ctx* c) {
+    if (!c) return ERM_E_PARAM;
+    if (filli(c, c->ex_time, c->ncell, -1)) return ERM_E_DEVICE;
+    for (auto& p : c->ex) if (fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
+    return 0;
+}
+
+int erm_rtm_step(erm_ctx* c, int pass, int it_begin, int nsteps) {
+    if (!c) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    return run_pass(c, pass, it_begin, nsteps);
+}
+
+int erm_forward_real(erm_ctx* c, float* rec) {
+    if (!c) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    int rc = erm_zero_wavefield(c);
+    if (rc) return rc;
+    if ((rc = run_pass(c, ERM_PASS_MODEL, 0, c->p.lt))) return rc;
+    if (rec) return erm_get_record(c, rec);
+    return 0;
+}
+
+int erm_forward_smoothed(erm_ctx* c) {
+    if (!c) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) return rc;
+    return run_pass(c, ERM_PASS_SOURCE, 0, c->p.lt_forward);
+}
+
+int erm_backward_smoothed(erm_ctx* c, const float* rec) {
+    if (!c) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    int rc;
+    if (rec && (rc = erm_set_record(c, rec))) return rc;
+    if ((rc = erm_zero_wavefield(c))) return rc;
+    for (auto& p : c->img) if (fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
+    return run_pass(c, ERM_PASS_RECEIVER, c->p.lt - 1, c->p.lt);
+}
+
+int erm_set_record(erm_ctx* c, const float* rec) {
+    if (!c || !rec) return ERM_E_PARAM;
+    if (c->nr == 0) { c->have_record = true; return 0; }
+    HIPC(c, hipMemcpyAsync(c->rec, rec, (size_t)3 * c->p.lt * c->nr * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->have_record = true;
+    return 0;
+}
+
+int erm_get_record(erm_ctx* c, float* rec) {
+    if (!c || !rec) return ERM_E_PARAM;
+    if (c->nr == 0) return 0;
+    HIPC(c, hipMemcpyAsync(rec, c->rec, (size_t)3 * c->p.lt * c->nr * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int erm_get_field(erm_ctx* c, int field, float* dst) {
+    if (!c || !dst) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    const size_t N = (size_t)c->nnx * c->nny * c->nnz;
+    if (field >= 0 && field < 13) return download(c, c->f[field], dst);
+    if (field >= ERM_F_VXS && field <= ERM_F_VZS) {
+        int k = field - ERM_F_VXS;
+        std::vector<float> vp(N);
+        int rc = download(c, c->f[k], dst);
+        if (!rc) rc = download(c, c->f[10 + k], vp.data());
+        if (rc) return rc;
+        for (size_t n = 0; n < N; ++n) dst[n] -= vp[n];
+        return 0;
+    }
+    if (field == ERM_F_EX_TIME) {
+        std::vector<int> t(c->ncell);
+        HIPC(c, hipMemcpyAsync(t.data(), c->ex_time, c->ncell * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        for (int k = 0; k < c->nnz; ++k)
+            for (int j = 0; j < c->nny; ++j)
+                for (int i = 0; i < c->nnx; ++i)
+                    dst[((size_t)k * c->nny + j) * c->nnx + i] = (float)t[(size_t)k * c->pxy + (size_t)j * c->px + i];
+        return 0;
+    }
+    if (field >= ERM_F_EX_AMP && field <= ERM_F_EX_VZP) return download(c, c->ex[field - ERM_F_EX_AMP], dst);
+    if (field >= ERM_F_IMG_TP && field <= ERM_F_IMG_PS) return download(c, c->img[field - ERM_F_IMG_TP], dst);
+    return c->fail(ERM_E_PARAM, "unknown field %d", field);
+}
+
+int erm_set_field(erm_ctx* c, int field, const float* src) {
+    if (!c || !src) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    if (field >= 0 && field < 13) return upload(c, c->f[field], src);
+    return c->fail(ERM_E_PARAM, "field %d is not writable", field);
+}
+
+int erm_get_images(erm_ctx* c, float* tp, float* pp, float* ps) {
+    if (!c) return ERM_E_PARAM;
+    const erm_params& p = c->p;
+    const size_t N = (size_t)c->nnx * c->nny * c->nnz;
+    std::vector<float> full(N);
+    float* outs[3] = {tp, pp, ps};
+    for (int a = 0; a < 3; ++a) {
+        if (!outs[a]) continue;
+        int rc = erm_get_field(c, ERM_F_IMG_TP + a, full.data());
+        if (rc) return rc;
+        // output_file_xyz_boundary (3D_zzzzz:65-85): cut the model region, disk order
+        for (int ix = 0; ix < p.nx; ++ix)
+            for (int iy = 0; iy < p.ny; ++iy)
+                for (int iz = 0; iz < p.nz; ++iz)
+                    outs[a][((size_t)ix * p.ny + iy) * p.nz + iz] =
+                        full[((size_t)(iz + p.bu) * c->nny + iy + p.bf) * c->nnx + ix + p.bl];
+    }
+    return 0;
+}
+
+static int write_cut(erm_ctx* c, int field, const std::string& path) {
+    const erm_params& p = c->p;
+    std::vector<float> full((size_t)c->nnx * c->nny * c->nnz), cut((size_t)p.nx * p.ny * p.nz);
+    int rc = erm_get_field(c, field, full.data());
+    if (rc) return rc;
+    for (int ix = 0; ix < p.nx; ++ix)
+        for (int iy = 0; iy < p.ny; ++iy)
+            for (int iz = 0; iz < p.nz; ++iz)
+                cut[((size_t)ix * p.ny + iy) * p.nz + iz] =
+                    full[((size_t)(iz + p.bu) * c->nny + iy + p.bf) * c->nnx + ix + p.bl];
+    if (write_raw(path.c_str(), cut.data(), cut.size())) return c->fail(ERM_E_IO, "Can not oen this file : %s", path.c_str());
+    return 0;
+}
+
+int erm_write_results(erm_ctx* c, const char* dir) {
+    if (!c || !dir) return ERM_E_PARAM;
+    const char* names[3] = {"tp", "pp", "ps"};
+    for (int a = 0; a < 3; ++a) {
+        std::string path = c->p.vsp == 0 ? format("%s/result-%s-%d-%d", dir, names[a], c->syr, c->sxr)
+                                         : format("%s/result-%s-%d", dir, names[a], c->szr);
+        int rc = write_cut(c, ERM_F_IMG_TP + a, path);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int erm_write_excitation(erm_ctx* c, const char* dir) {
+    if (!c || !dir) return ERM_E_PARAM;
+    const char* names[6] = {"time", "amp", "tp", "vxp", "vyp", "vzp"};
+    for (int a = 0; a < 6; ++a) {
+        int rc = write_cut(c, ERM_F_EX_TIME + a, format("%s/cut-ex-%s-%d-%d", dir, names[a], c->syr, c->sxr));
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int erm_set_profiling(erm_ctx* c, int on) {
+    if (!c) return ERM_E_PARAM;
+    c->profiling = on != 0;
+    return 0;
+}
+
+int erm_get_timing(erm_ctx* c, double* pass_ms, double* stress_ms, long* stress_n, double* vel_ms, long* vel_n) {
+    if (!c) return ERM_E_PARAM;
+    if (pass_ms) *pass_ms = c->last_pass_ms;
+    if (stress_ms) *stress_ms = c->last_stress_ms;
+    if (stress_n) *stress_n = c->last_stress_n;
+    if (vel_ms) *vel_ms = c->last_velocity_ms;
+    if (vel_n) *vel_n = c->last_velocity_n;
+    return 0;
+}
+
+int erm_algorithmic_bytes(const erm_ctx* c, int pass, double* sb, double* vb) {
+    if (!c) return ERM_E_PARAM;
+    // stress: v(3) read + 7 stress/tp rw (14) + M, mu (2) = 19 floats
+    // velocity: 7 stress/tp read + 6 v/vP rw (12) + 1/rho (1) = 20 floats
+    //           + ex_amp (source pass) or ex_time (receiver pass)
+    if (sb) *sb = 19.0 * 4.0;
+    if (vb) *vb = 20.0 * 4.0 + ((pass == ERM_PASS_SOURCE || pass == ERM_PASS_RECEIVER) ? 4.0 : 0.0);
+    return 0;
+}
+
+int erm_synchronize(erm_ctx* c) {
+    if (!c) return ERM_E_PARAM;
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int erm_make_model3d(int kind, int nx, int ny, int nz, float dx, float dy, float dz, uint64_t seed, float* vp,
+                     float* vs, float* rho) {
+    int rc = make_model3d(kind, nx, ny, nz, dx, dy, dz, (unsigned long long)seed, vp, vs, rho);
+    if (rc) set_global_error("erm_make_model3d: bad arguments");
+    return rc;
+}
+
+int erm_write_raw(const char* path, const float* data, size_t n) {
+    int rc = write_raw(path, data, n);
+    if (rc) set_global_error("Can not oen this file : %s", path ? path : "(null)");
+    return rc;
+}
+
+int erm_read_raw(const char* path, float* data, size_t n) {
+    int rc = read_raw(path, data, n);
+    if (rc) set_global_error("Can not oen this file : %s", path ? path : "(null)");
+    return rc;
+}
+
+}  // extern "C"

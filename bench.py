@@ -1,0 +1,204 @@
+"""Benchmark: 3D elastic RTM on one MI355X (Gcell-updates/s).
+
+Workload (the 3D single-shot configuration): 256x256x128 model, d = 10 m,
+20-cell absorbing boundary on every face (296x296x168 padded grid), 8th-order
+staggered FD (radius 4), dt = 0.8 ms, nt = 2500, 9 wavefield components plus
+the vector-P split (tp, vxp, vyp, vzp), FP32.  Model: five dipping layers
+vp 1800-4200 m/s, vs = vp/1.732, density 230*vp^0.25, three random Gaussian
+anomalies (seed 2); migration model = the same smoothed by a 7-point box
+filter.  15 Hz Ricker at the centre of the surface, 128x128 receivers.
+
+One benchmark step = one migrated shot: the source pass (nt steps with
+excitation tracking) + the receiver pass (nt steps with record injection and
+PP/PS/tp excitation-amplitude imaging).  The receiver record is modelled once
+before timing and stays in HBM (inputs resident).  value = updated cells x
+time steps / wall time.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per
+launch / average launch duration from HIP events around every launch on the
+context's stream, against 8 TB/s), HBM traffic per launch from the committed
+rocprofv3 PMC summary (profiles/), and an FP32 CPU baseline (the oracle's
+FP32 instantiation, OpenMP) on a bounded sample of the same grid.
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+PKG = "2d_and_3d_elastic_reverse_time_migration_amd"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def smooth_box(a, passes=4):
+    a = a.astype(np.float32)
+    for _ in range(passes):
+        q = np.pad(a, 1, mode="edge")
+        a = (q[1:-1, 1:-1, 1:-1] * 2 + q[2:, 1:-1, 1:-1] + q[:-2, 1:-1, 1:-1] + q[1:-1, 2:, 1:-1]
+             + q[1:-1, :-2, 1:-1] + q[1:-1, 1:-1, 2:] + q[1:-1, 1:-1, :-2]) / 8.0
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def make_params(erm, nt, boundary):
+    p = erm.default_params()
+    p.update(nx=256, ny=256, nz=128, dx=10.0, dy=10.0, dz=10.0, dt=0.8, lt=nt, lt_forward=nt, freq=15.0,
+             fd_order=8, boundary=boundary, coe_attenuation=220.0,
+             bu=20, bd=20, bl=20, br=20, bf=20, bb=20,
+             shot_start_x=128, shot_start_y=128, shot_start_z=0,
+             receiver_num_x=128, receiver_num_y=128, receiver_num_z=1,
+             receiver_start_x=0, receiver_interval_x=2, receiver_start_y=0, receiver_interval_y=2,
+             receiver_start_z=0, add_receiver_bool=1, RTM=1, GPU_N=1)
+    return p
+
+
+def cpu_baseline(p, vp_s, vs_s, rho_s, budget_s):
+    """FP32 oracle (OpenMP) on the host: steps of the source pass over the full grid."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle3d
+    threads = min(os.cpu_count() or 1, 16)
+    oracle3d.set_threads(threads)
+    o = oracle3d.Oracle3D(p, 32)
+    o.load_model(1, vp_s, vs_s, rho_s)
+    o.step(1, 0, 1)  # warm the pages
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.step(1, 1 + steps, 2)
+        steps += 2
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 200:
+            break
+    R = 4
+    cells = (p.nx + p.bl + p.br - 2 * R) * (p.ny + p.bf + p.bb - 2 * R) * (p.nz + p.bu + p.bd - 2 * R)
+    o.close()
+    return {"value": round(cells * steps / el / 1e9, 4), "unit": "Gcell-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} source-pass time steps (stress+velocity+vP+excitation) of the same "
+                      f"296x296x168 grid, FP32 oracle, OpenMP over z; {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nt", type=int, default=2500, help="time steps per pass (profiling runs shorten it)")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-events", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the whole grid fits one MI355X); --gpus {args.gpus} is not "
+              f"supported in this build", file=sys.stderr)
+        sys.exit(2)
+
+    erm = importlib.import_module(PKG)
+    nt = args.nt
+    p = make_params(erm, nt, erm.BOUNDARY_DAMP)
+    vp, vs, rho = erm.make_model3d(1, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, 2)
+    vp_s = smooth_box(vp)
+    vs_s = (vp_s / 1.732).astype(np.float32)
+    rho_s = (230.0 * vp_s.astype(np.float64) ** 0.25).astype(np.float32)
+    log(f"bench: grid {tuple(p.padded_shape)} nt={nt} radius 4, building context")
+    g = erm.RTM3D(p, 0)
+    g.load_model(0, vp, vs, rho)
+    g.load_model(1, vp_s, vs_s, rho_s)
+    t0 = time.perf_counter()
+    g.forward_real(want_record=False)  # the record stays on the device
+    log(f"bench: modeled the receiver record in {time.perf_counter() - t0:.2f} s")
+
+    R = 4
+    nnz, nny, nnx = g.shape
+    cells = (nnx - 2 * R) * (nny - 2 * R) * (nnz - 2 * R)
+    sb, vb_src = g.algorithmic_bytes(1)
+    _, vb_rcv = g.algorithmic_bytes(2)
+
+    for w in range(args.warmup):
+        g.forward_smoothed()
+        g.backward_smoothed()
+        log(f"bench: warmup shot {w + 1} done")
+    g.set_profiling(not args.no_kernel_events)
+    acc = {"stress_ms": 0.0, "stress_n": 0, "vel_ms": 0.0, "vel_n": 0, "vel_bytes": 0.0, "pass_ms": 0.0}
+    g.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        for fn, vb in ((g.forward_smoothed, vb_src), (g.backward_smoothed, vb_rcv)):
+            fn()
+            t = g.timing()
+            acc["pass_ms"] += t["pass_ms"]
+            acc["stress_ms"] += t["stress_ms"]
+            acc["stress_n"] += t["stress_launches"]
+            acc["vel_ms"] += t["velocity_ms"]
+            acc["vel_n"] += t["velocity_launches"]
+            acc["vel_bytes"] += vb * cells * t["velocity_launches"]
+        log(f"bench: timed shot {s + 1}/{args.steps} done")
+    g.synchronize()
+    el = time.perf_counter() - t0
+    g.set_profiling(False)
+
+    updates = cells * 2 * nt * args.steps
+    value = updates / el / 1e9
+    roof = None
+    extra = {}
+    if acc["stress_n"] > 0 and acc["vel_n"] > 0:
+        st_avg = acc["stress_ms"] / acc["stress_n"]
+        ve_avg = acc["vel_ms"] / acc["vel_n"]
+        st_ach = sb * cells / (st_avg * 1e-3) / 1e9
+        ve_bytes = acc["vel_bytes"] / acc["vel_n"]
+        ve_ach = ve_bytes / (ve_avg * 1e-3) / 1e9
+        dom = "k_velocity" if acc["vel_ms"] >= acc["stress_ms"] else "k_stress"
+        ach, per_launch, avg = (ve_ach, ve_bytes, ve_avg) if dom == "k_velocity" else (st_ach, sb * cells, st_avg)
+        traffic = None
+        tf = os.path.join(ROOT, "profiles", "traffic_cfg2.json")
+        if os.path.exists(tf):
+            with open(tf) as f:
+                traffic = json.load(f).get(dom, {}).get("hbm_bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "algorithmic_bytes_per_launch": per_launch, "avg_launch_ms": round(avg, 5)}
+        extra = {"k_stress": {"avg_ms": round(st_avg, 5), "GB/s": round(st_ach, 1),
+                              "bytes_per_cell": sb},
+                 "k_velocity": {"avg_ms": round(ve_avg, 5), "GB/s": round(ve_ach, 1),
+                                "bytes_per_cell": round(ve_bytes / cells, 2)}}
+    g.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline (FP32 oracle)")
+        cpu = cpu_baseline(p, vp_s, vs_s, rho_s, args.cpu_budget)
+
+    line = {
+        "metric": "Gcell-updates/s 3D elastic RTM",
+        "value": round(value, 3),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded dipping-layer + anomaly model)",
+        "config": {"workload": "3D elastic RTM, one shot: 256x256x128 model + 20-cell damping boundary "
+                               "(296x296x168), 8th-order staggered FD, dt=0.8 ms, nt=%d, 15 Hz Ricker, "
+                               "128x128 receivers; step = source pass + receiver pass" % nt,
+                   "cells_per_time_step": cells, "time_steps_per_shot": 2 * nt,
+                   "parallelism": "single GPU"},
+        "roofline": roof,
+        "kernels": extra,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,512 @@
+/*
+ * oracle3d.cpp - CPU restatement of the reference's 3D elastic RTM hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the correctness oracle (FP64
+ * instantiation) and the CPU baseline (FP32 instantiation, OpenMP over z).
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+ * load it, and only as the checker or the timed CPU baseline; the product
+ * library (liberm.so) never links or calls it.
+ *
+ * It follows, step for step, the reference files:
+ *   step order of the three time loops  3D_elastic_RTM/3D_forward_or_back_together.cu:9-155,161-290,296-453
+ *   staggered operators / update form   2D_elastic_RTM/viscoelastic_equation.cu:137-176,294-323
+ *                                       (elastic limit: tao=strain so memory terms vanish, :43-56)
+ *   vector P (tp, vxp/vyp/vzp)          3D_forward_or_back_together.cu:83,118 (argument lists);
+ *                                       tp update as viscoelastic_equation.cu:763
+ *   damping profile 1-cos               3D_elastic_RTM/3D_zzzzz:261-369
+ *   model padding                       3D_elastic_RTM/3D_zzzzz:141-259
+ *   Ricker                              3D_elastic_RTM/zzzzz:136-157 (make_ricker_new)
+ *   FD coefficients (radius 6)          3D_elastic_RTM/zzzzz:202-219
+ *   vs, density from vp                 3D_elastic_RTM/zzzzz:1261-1287
+ *   excitation amplitude/time           2D_elastic_RTM/elastic_2D_kernel_3.cu:30-48,71-89
+ *   excitation-amplitude imaging        2D_elastic_RTM/elastic_2D_kernel_3.cu:542-580
+ * The 3D kernel bodies are absent from the reference; the 3D staggering
+ * (vy at (i,j-1/2,k), sxy at (i-1/2,j-1/2,k), syz at (i,j-1/2,k+1/2)) extends
+ * the 2D positions of viscoelastic_equation.cu by symmetry.
+ *
+ * How it is pinned (tests/test_oracle3d.py): coefficient sums and polynomial
+ * exactness, exact decay under uniform damping, x/y mirror symmetry of an
+ * explosive source, y<->x transposition symmetry, P arrival times against
+ * r/vp, S-energy leakage of the P/S split, excitation time against the
+ * travel time, and the FP32 instantiation against the FP64 one.
+ */
+#include "../include/erm.h"
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+/* Reference radius-6 optimized first-derivative coefficients, zzzzz:205-210. */
+const double kCoeR6[7] = {0.0, 1.2213363647, -0.096931457519, 0.017447662353,
+                          -0.0029672895159, 0.00035900539822, -0.00002184781161};
+/* Radius-4 staggered Taylor coefficients (exact for polynomials of degree <= 7). */
+const double kCoeR4[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0};
+
+/* make_ricker_new, zzzzz:136-157: dt in seconds, PI of cwp.h. */
+template <typename T>
+std::vector<T> ricker(double freq, double dt_s) {
+    const double PI = 3.141592653589793;
+    if (freq == 0.0) freq = 30.0;
+    if (dt_s == 0.0) dt_s = 0.004;
+    T B = (T)(std::sqrt(6.0) / (PI * freq));
+    int N = (int)std::ceil(1.35 * (double)B / dt_s);
+    std::vector<T> w(2 * N + 1);
+    w[N] = (T)1.0;
+    for (int i = 1; i <= N; ++i) {
+        T t = (T)dt_s * (T)i;
+        T u = (T)(2.0 * std::sqrt(6.0)) * t / B;
+        w[N + i] = w[N - i] = (T)0.5 * ((T)2.0 - u * u) * std::exp(-u * u / (T)4.0);
+    }
+    return w;
+}
+
+/* One face of coe_attenuation_3d_new2 (3D_zzzzz:284-313): d[i] for i in [0,b),
+ * `rising` = profile grows with i (down/right/back faces). pai = 3.1415926 as the reference. */
+void face_profile(int b, double c, bool rising, std::vector<double>& d) {
+    const double pai = 3.1415926;
+    d.assign(b, 0.0);
+    for (int i = 0; i < b; ++i) {
+        double s = rising ? (double)i / (b - 1) : (double)(b - i - 1) / (b - 1);
+        d[i] = c * (1.0 - std::cos(pai / 2 * s));
+    }
+}
+
+template <typename T>
+struct Oracle3D {
+    erm_params p;
+    int R = 6;
+    int nnx = 0, nny = 0, nnz = 0;
+    size_t N = 0;
+    T c[7] = {0};
+    T cx = 0, cy = 0, cz = 0, dtr = 0;
+    std::vector<T> f[13];                 /* vx vy vz txx tyy tzz txy txz tyz tp vxp vyp vzp */
+    std::vector<T> M[2], MU[2], B[2];     /* rho*vp^2, rho*vs^2, 1/rho; [0]=true [1]=smoothed */
+    bool have_model[2] = {false, false};
+    std::vector<T> ax, ay, az;            /* damping per axis; att(i,j,k) = ax[i]+ay[j]+az[k] */
+    std::vector<T> wavelet;
+    std::vector<int> ex_time;
+    std::vector<T> ex_amp, ex_tp, ex_vxp, ex_vyp, ex_vzp;
+    std::vector<T> img_tp, img_pp, img_ps;
+    std::vector<T> rec;                   /* [3][lt][ry][rx] */
+    int sx = 0, sy = 0, sz = 0;           /* padded-grid source position */
+
+    size_t id(int i, int j, int k) const { return ((size_t)k * nny + j) * nnx + i; }
+
+    int init(const erm_params* pp) {
+        p = *pp;
+        if (p.fd_order == 12) R = 6;
+        else if (p.fd_order == 8) R = 4;
+        else return ERM_E_PARAM;
+        if (p.boundary != ERM_BOUNDARY_DAMP) return ERM_E_PARAM; /* CPML: oracle3d_cpml */
+        nnx = p.nx + p.bl + p.br;
+        nny = p.ny + p.bf + p.bb;
+        nnz = p.nz + p.bu + p.bd;
+        if (nnx < 2 * R + 1 || nny < 2 * R + 1 || nnz < 2 * R + 1) return ERM_E_SIZE;
+        N = (size_t)nnx * nny * nnz;
+        for (int m = 0; m <= R; ++m) c[m] = (T)(R == 6 ? kCoeR6[m] : kCoeR4[m]);
+        dtr = (T)(p.dt / 1000.0);
+        cx = (T)((p.dt / 1000.0) / p.dx);
+        cy = (T)((p.dt / 1000.0) / p.dy);
+        cz = (T)((p.dt / 1000.0) / p.dz);
+        for (auto& v : f) v.assign(N, (T)0);
+        /* damping, 3D_zzzzz:261-369 (sum over the six faces) */
+        std::vector<double> du, dd, dl, dr, df, db;
+        double cA = p.coe_attenuation;
+        face_profile(p.bu, cA, false, du);
+        face_profile(p.bd, cA, true, dd);
+        face_profile(p.bl, cA, false, dl);
+        face_profile(p.br, cA, true, dr);
+        face_profile(p.bf, cA, false, df);
+        face_profile(p.bb, cA, true, db);
+        ax.assign(nnx, 0); ay.assign(nny, 0); az.assign(nnz, 0);
+        for (int i = 0; i < p.bl; ++i) ax[i] += (T)dl[i];
+        for (int i = 0; i < p.br; ++i) ax[p.bl + p.nx + i] += (T)dr[i];
+        for (int j = 0; j < p.bf; ++j) ay[j] += (T)df[j];
+        for (int j = 0; j < p.bb; ++j) ay[p.bf + p.ny + j] += (T)db[j];
+        for (int k = 0; k < p.bu; ++k) az[k] += (T)du[k];
+        for (int k = 0; k < p.bd; ++k) az[p.bu + p.nz + k] += (T)dd[k];
+        wavelet = ricker<T>(p.freq, p.dt / 1000.0);
+        ex_time.assign(N, -1);
+        ex_amp.assign(N, 0); ex_tp.assign(N, 0); ex_vxp.assign(N, 0); ex_vyp.assign(N, 0); ex_vzp.assign(N, 0);
+        img_tp.assign(N, 0); img_pp.assign(N, 0); img_ps.assign(N, 0);
+        rec.assign((size_t)3 * p.lt * p.receiver_num_y * p.receiver_num_x, 0);
+        return set_shot(0, 0, 0);
+    }
+
+    int set_shot(int isx, int isy, int isz) {
+        int sxr = p.shot_start_x + isx * p.shot_interval_x;
+        int syr = p.shot_start_y + isy * p.shot_interval_y;
+        int szr = p.shot_start_z + isz * p.shot_interval_z;
+        sx = p.bl + sxr; sy = p.bf + syr; sz = p.bu + szr;
+        if (sx < R || sx >= nnx - R || sy < R || sy >= nny - R || sz < R || sz >= nnz - R) return ERM_E_PARAM;
+        return ERM_OK;
+    }
+
+    /* add_pml_layers_v_h (3D_zzzzz:141-259): edge replication == clamping
+     * the padded index into the model region. */
+    void pad(const float* src, std::vector<double>& dst) const {
+        dst.resize(N);
+        for (int k = 0; k < nnz; ++k) {
+            int kz = std::min(std::max(k - p.bu, 0), p.nz - 1);
+            for (int j = 0; j < nny; ++j) {
+                int jy = std::min(std::max(j - p.bf, 0), p.ny - 1);
+                for (int i = 0; i < nnx; ++i) {
+                    int ix = std::min(std::max(i - p.bl, 0), p.nx - 1);
+                    dst[id(i, j, k)] = src[((size_t)ix * p.ny + jy) * p.nz + kz];
+                }
+            }
+        }
+    }
+
+    int load_model(int which, const float* vp, const float* vs, const float* rho) {
+        if (which < 0 || which > 1 || !vp) return ERM_E_PARAM;
+        std::vector<double> pvp, pvs, prho;
+        pad(vp, pvp);
+        if (vs) pad(vs, pvs);
+        else { pvs.resize(N); for (size_t n = 0; n < N; ++n) pvs[n] = (float)(pvp[n] / std::sqrt(3.0)); }
+        if (rho) pad(rho, prho);
+        else { prho.resize(N); for (size_t n = 0; n < N; ++n) prho[n] = (float)(1000 * 0.23 * std::sqrt(std::sqrt(pvp[n]))); }
+        M[which].resize(N); MU[which].resize(N); B[which].resize(N);
+        for (size_t n = 0; n < N; ++n) {
+            T r = (T)prho[n], a = (T)pvp[n], b = (T)pvs[n];
+            M[which][n] = r * a * a;
+            MU[which][n] = r * b * b;
+            B[which][n] = (T)1 / r;
+        }
+        have_model[which] = true;
+        return ERM_OK;
+    }
+
+    /* D+ f[i] = sum c_m (f[i+m] - f[i-m+1])  (centred at i+1/2),
+     * D- f[i] = sum c_m (f[i+m-1] - f[i-m])  (centred at i-1/2);
+     * viscoelastic_equation.cu:137-149. */
+    inline T dp(const T* q, ptrdiff_t s) const {
+        T a = 0;
+        for (int m = 1; m <= R; ++m) a += c[m] * (q[m * s] - q[-(m - 1) * s]);
+        return a;
+    }
+    inline T dm(const T* q, ptrdiff_t s) const {
+        T a = 0;
+        for (int m = 1; m <= R; ++m) a += c[m] * (q[(m - 1) * s] - q[-m * s]);
+        return a;
+    }
+
+    void add_source(int it) {  /* add_source_3D, 3D_forward_or_back_together.cu:36-38 */
+        if (it < 0 || it >= (int)wavelet.size()) return;
+        size_t s = id(sx, sy, sz);
+        f[3][s] += wavelet[it];
+        f[4][s] += wavelet[it];
+        f[5][s] += wavelet[it];
+    }
+
+    /* fwd_txxzzxzpp_3D: stress and P-stress from v (launch :83). */
+    void stress(int which) {
+        const T* vx = f[0].data(); const T* vy = f[1].data(); const T* vz = f[2].data();
+        T* txx = f[3].data(); T* tyy = f[4].data(); T* tzz = f[5].data();
+        T* txy = f[6].data(); T* txz = f[7].data(); T* tyz = f[8].data(); T* tp = f[9].data();
+        const T* Mm = M[which].data(); const T* Um = MU[which].data();
+        const ptrdiff_t sX = 1, sY = nnx, sZ = (ptrdiff_t)nnx * nny;
+        const T h = dtr / (T)2;
+#pragma omp parallel for schedule(static)
+        for (int k = R; k < nnz - R; ++k)
+            for (int j = R; j < nny - R; ++j)
+                for (int i = R; i < nnx - R; ++i) {
+                    size_t n = id(i, j, k);
+                    T exx = cx * dp(vx + n, sX), eyy = cy * dp(vy + n, sY), ezz = cz * dm(vz + n, sZ);
+                    T exy = cy * dm(vx + n, sY) + cx * dm(vy + n, sX);
+                    T exz = cz * dp(vx + n, sZ) + cx * dm(vz + n, sX);
+                    T eyz = cz * dp(vy + n, sZ) + cy * dm(vz + n, sY);
+                    T d = ax[i] + ay[j] + az[k];
+                    T a = (T)1 / ((T)1 + d * h), b = (T)1 - d * h;
+                    T Mc = Mm[n], mu = Um[n], lam = Mc - (T)2 * mu;
+                    txx[n] = a * (b * txx[n] + Mc * exx + lam * (eyy + ezz));
+                    tyy[n] = a * (b * tyy[n] + Mc * eyy + lam * (exx + ezz));
+                    tzz[n] = a * (b * tzz[n] + Mc * ezz + lam * (exx + eyy));
+                    tp[n] = a * (b * tp[n] + Mc * (exx + eyy + ezz));
+                    txy[n] = a * (b * txy[n] + mu * exy);
+                    txz[n] = a * (b * txz[n] + mu * exz);
+                    tyz[n] = a * (b * tyz[n] + mu * eyz);
+                }
+    }
+
+    /* fwd_vx_3D/fwd_vy_3D/fwd_vz_3D (:100-104) + fwd_vxp_vzp_3D (:118),
+     * followed by excitation (:282, mode 1) or imaging (:442-446, mode 2). */
+    void velocity(int which, int mode, int it) {
+        T* vx = f[0].data(); T* vy = f[1].data(); T* vz = f[2].data();
+        const T* txx = f[3].data(); const T* tyy = f[4].data(); const T* tzz = f[5].data();
+        const T* txy = f[6].data(); const T* txz = f[7].data(); const T* tyz = f[8].data();
+        const T* tp = f[9].data();
+        T* vxp = f[10].data(); T* vyp = f[11].data(); T* vzp = f[12].data();
+        const T* Bm = B[which].data();
+        const ptrdiff_t sX = 1, sY = nnx, sZ = (ptrdiff_t)nnx * nny;
+        const T h = dtr / (T)2;
+#pragma omp parallel for schedule(static)
+        for (int k = R; k < nnz - R; ++k)
+            for (int j = R; j < nny - R; ++j)
+                for (int i = R; i < nnx - R; ++i) {
+                    size_t n = id(i, j, k);
+                    T d = ax[i] + ay[j] + az[k];
+                    T a = (T)1 / ((T)1 + d * h), b = (T)1 - d * h;
+                    T bo = Bm[n];
+                    T fx = cx * dm(txx + n, sX) + cy * dp(txy + n, sY) + cz * dm(txz + n, sZ);
+                    T fy = cx * dp(txy + n, sX) + cy * dm(tyy + n, sY) + cz * dm(tyz + n, sZ);
+                    T fz = cx * dp(txz + n, sX) + cy * dp(tyz + n, sY) + cz * dp(tzz + n, sZ);
+                    T px = cx * dm(tp + n, sX), py = cy * dm(tp + n, sY), pz = cz * dp(tp + n, sZ);
+                    T nvx = a * (b * vx[n] + bo * fx);
+                    T nvy = a * (b * vy[n] + bo * fy);
+                    T nvz = a * (b * vz[n] + bo * fz);
+                    T nxp = a * (b * vxp[n] + bo * px);
+                    T nyp = a * (b * vyp[n] + bo * py);
+                    T nzp = a * (b * vzp[n] + bo * pz);
+                    vx[n] = nvx; vy[n] = nvy; vz[n] = nvz;
+                    vxp[n] = nxp; vyp[n] = nyp; vzp[n] = nzp;
+                    if (mode == 1) {
+                        /* caculate_ex_amp_time_new (elastic_2D_kernel_3.cu:30-48) in 3D:
+                         * amplitude |vP|, strict '<' so the first maximum wins. */
+                        T amp = std::sqrt(nxp * nxp + nyp * nyp + nzp * nzp);
+                        if (ex_amp[n] < amp) {
+                            ex_time[n] = it; ex_amp[n] = amp; ex_tp[n] = tp[n];
+                            ex_vxp[n] = nxp; ex_vyp[n] = nyp; ex_vzp[n] = nzp;
+                        }
+                    } else if (mode == 2) {
+                        if (ex_time[n] == it) {
+                            /* imaging_correlation_ex_2D (:542-558), imaging_inner_product_ex_2D
+                             * (:559-580); a zero denominator images 0. */
+                            T etp = ex_tp[n];
+                            img_tp[n] = etp != (T)0 ? tp[n] / etp : (T)0;
+                            T ex = ex_vxp[n], ey = ex_vyp[n], ez = ex_vzp[n];
+                            T den = ex * ex + ey * ey + ez * ez;
+                            T pp = ex * nxp + ey * nyp + ez * nzp;
+                            T ps = ex * (nvx - nxp) + ey * (nvy - nyp) + ez * (nvz - nzp);
+                            img_pp[n] = den > (T)0 ? pp / den : (T)0;
+                            img_ps[n] = den > (T)0 ? ps / den : (T)0;
+                        }
+                    }
+                }
+    }
+
+    size_t rec_index(int c3, int it, int iry, int irx) const {
+        return (((size_t)c3 * p.lt + it) * p.receiver_num_y + iry) * p.receiver_num_x + irx;
+    }
+    template <typename F>
+    void for_receivers(F&& fn) const {
+        for (int iry = 0; iry < p.receiver_num_y; ++iry)
+            for (int irx = 0; irx < p.receiver_num_x; ++irx) {
+                int i = p.bl + p.receiver_start_x + irx * p.receiver_interval_x;
+                int j = p.bf + p.receiver_start_y + iry * p.receiver_interval_y;
+                int k = p.bu + p.receiver_start_z;
+                fn(iry, irx, id(i, j, k));
+            }
+    }
+
+    int check_receivers() const {
+        int bad = 0;
+        for_receivers([&](int, int, size_t n) { if (n >= N) bad = 1; });
+        int i1 = p.bl + p.receiver_start_x + (p.receiver_num_x - 1) * p.receiver_interval_x;
+        int j1 = p.bf + p.receiver_start_y + (p.receiver_num_y - 1) * p.receiver_interval_y;
+        int k0 = p.bu + p.receiver_start_z;
+        if (p.bl + p.receiver_start_x < 0 || i1 >= nnx || p.bf + p.receiver_start_y < 0 || j1 >= nny ||
+            k0 < 0 || k0 >= nnz)
+            bad = 1;
+        return bad ? ERM_E_PARAM : ERM_OK;
+    }
+
+    int step(int pass, int it_begin, int nsteps) {
+        int which = pass == ERM_PASS_MODEL ? 0 : 1;
+        if (!have_model[which]) return ERM_E_STATE;
+        if (check_receivers() != ERM_OK) return ERM_E_PARAM;
+        for (int s = 0; s < nsteps; ++s) {
+            if (pass == ERM_PASS_RECEIVER) {
+                int it = it_begin - s;
+                if (it < 0 || it >= p.lt) return ERM_E_PARAM;
+                /* write_or_add_shot_3D_surface_three, mode add (:328) into v at level it */
+                for_receivers([&](int iry, int irx, size_t n) {
+                    f[0][n] += rec[rec_index(0, it, iry, irx)];
+                    f[1][n] += rec[rec_index(1, it, iry, irx)];
+                    f[2][n] += rec[rec_index(2, it, iry, irx)];
+                });
+                stress(which);
+                velocity(which, 2, it);
+            } else {
+                int it = it_begin + s;
+                add_source(it);
+                stress(which);
+                velocity(which, pass == ERM_PASS_SOURCE ? 1 : 0, it);
+                if (pass == ERM_PASS_MODEL && it < p.lt) {
+                    /* write_or_add_shot_3D_surface_three, mode 0 (:127) */
+                    for_receivers([&](int iry, int irx, size_t n) {
+                        rec[rec_index(0, it, iry, irx)] = f[0][n];
+                        rec[rec_index(1, it, iry, irx)] = f[1][n];
+                        rec[rec_index(2, it, iry, irx)] = f[2][n];
+                    });
+                }
+            }
+        }
+        return ERM_OK;
+    }
+
+    void zero_wavefield() { for (auto& v : f) std::fill(v.begin(), v.end(), (T)0); }
+    void zero_excitation() {
+        std::fill(ex_time.begin(), ex_time.end(), -1);
+        for (auto* v : {&ex_amp, &ex_tp, &ex_vxp, &ex_vyp, &ex_vzp}) std::fill(v->begin(), v->end(), (T)0);
+    }
+    void zero_images() { for (auto* v : {&img_tp, &img_pp, &img_ps}) std::fill(v->begin(), v->end(), (T)0); }
+
+    int get_field(int fid, double* dst) const {
+        if (fid >= 0 && fid < 13) { for (size_t n = 0; n < N; ++n) dst[n] = (double)f[fid][n]; return 0; }
+        if (fid >= ERM_F_VXS && fid <= ERM_F_VZS) {
+            int c3 = fid - ERM_F_VXS;
+            for (size_t n = 0; n < N; ++n) dst[n] = (double)(f[c3][n] - f[10 + c3][n]);
+            return 0;
+        }
+        const std::vector<T>* src = nullptr;
+        switch (fid) {
+            case ERM_F_EX_TIME: for (size_t n = 0; n < N; ++n) dst[n] = ex_time[n]; return 0;
+            case ERM_F_EX_AMP: src = &ex_amp; break;
+            case ERM_F_EX_TP: src = &ex_tp; break;
+            case ERM_F_EX_VXP: src = &ex_vxp; break;
+            case ERM_F_EX_VYP: src = &ex_vyp; break;
+            case ERM_F_EX_VZP: src = &ex_vzp; break;
+            case ERM_F_IMG_TP: src = &img_tp; break;
+            case ERM_F_IMG_PP: src = &img_pp; break;
+            case ERM_F_IMG_PS: src = &img_ps; break;
+            default: return ERM_E_PARAM;
+        }
+        for (size_t n = 0; n < N; ++n) dst[n] = (double)(*src)[n];
+        return 0;
+    }
+    int set_field(int fid, const double* src) {
+        if (fid >= 0 && fid < 13) { for (size_t n = 0; n < N; ++n) f[fid][n] = (T)src[n]; return 0; }
+        return ERM_E_PARAM;
+    }
+};
+
+struct Handle {
+    int prec;
+    Oracle3D<double>* d = nullptr;
+    Oracle3D<float>* s = nullptr;
+};
+
+template <typename Fn>
+int dispatch(void* h, Fn&& fn) {
+    if (!h) return ERM_E_PARAM;
+    Handle* H = (Handle*)h;
+    if (H->prec == 64) return fn(*H->d);
+    return fn(*H->s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+int oracle_coefficients(int radius, double* c) {
+    if (radius == 6) { for (int m = 0; m <= 6; ++m) c[m] = kCoeR6[m]; return 0; }
+    if (radius == 4) { for (int m = 0; m <= 4; ++m) c[m] = kCoeR4[m]; return 0; }
+    return ERM_E_PARAM;
+}
+
+int oracle_ricker(double freq, double dt_ms, double* w, int cap) {
+    std::vector<double> r = ricker<double>(freq, dt_ms / 1000.0);
+    if (w) for (int i = 0; i < (int)r.size() && i < cap; ++i) w[i] = r[i];
+    return (int)r.size();
+}
+
+int oracle_damping_axis(const erm_params* p, int axis, double* out) {
+    /* per-axis damping sum used by oracle3d (x: nnx, y: nny, z: nnz entries) */
+    Oracle3D<double> o;
+    int rc = o.init(p);
+    if (rc) return rc;
+    const std::vector<double>& v = axis == 0 ? o.ax : axis == 1 ? o.ay : o.az;
+    for (size_t i = 0; i < v.size(); ++i) out[i] = v[i];
+    return (int)v.size();
+}
+
+void* oracle3d_create(const erm_params* p, int prec) {
+    Handle* H = new Handle();
+    H->prec = prec == 32 ? 32 : 64;
+    int rc;
+    if (H->prec == 64) { H->d = new Oracle3D<double>(); rc = H->d->init(p); }
+    else { H->s = new Oracle3D<float>(); rc = H->s->init(p); }
+    if (rc) { delete H->d; delete H->s; delete H; return nullptr; }
+    return H;
+}
+
+void oracle3d_destroy(void* h) {
+    if (!h) return;
+    Handle* H = (Handle*)h;
+    delete H->d; delete H->s; delete H;
+}
+
+int oracle3d_load_model(void* h, int which, const float* vp, const float* vs, const float* rho) {
+    return dispatch(h, [&](auto& o) { return o.load_model(which, vp, vs, rho); });
+}
+int oracle3d_set_shot(void* h, int isx, int isy, int isz) {
+    return dispatch(h, [&](auto& o) { return o.set_shot(isx, isy, isz); });
+}
+int oracle3d_zero(void* h, int what) {
+    return dispatch(h, [&](auto& o) {
+        if (what & 1) o.zero_wavefield();
+        if (what & 2) o.zero_excitation();
+        if (what & 4) o.zero_images();
+        return 0;
+    });
+}
+int oracle3d_step(void* h, int pass, int it_begin, int nsteps) {
+    return dispatch(h, [&](auto& o) { return o.step(pass, it_begin, nsteps); });
+}
+int oracle3d_get_field(void* h, int fid, double* dst) {
+    return dispatch(h, [&](auto& o) { return o.get_field(fid, dst); });
+}
+int oracle3d_set_field(void* h, int fid, const double* src) {
+    return dispatch(h, [&](auto& o) { return o.set_field(fid, src); });
+}
+int oracle3d_get_record(void* h, double* dst) {
+    return dispatch(h, [&](auto& o) {
+        for (size_t n = 0; n < o.rec.size(); ++n) dst[n] = (double)o.rec[n];
+        return 0;
+    });
+}
+int oracle3d_set_record(void* h, const double* src) {
+    return dispatch(h, [&](auto& o) {
+        for (size_t n = 0; n < o.rec.size(); ++n) o.rec[n] = (decltype(o.rec[0] + 0))src[n];
+        return 0;
+    });
+}
+/* One full shot in the order of the reference driver: modeling in the true
+ * model (skipped when use_record != 0), source forward with excitation,
+ * receiver backward with imaging. */
+int oracle3d_rtm_shot(void* h, int use_record) {
+    return dispatch(h, [&](auto& o) {
+        int rc;
+        if (!use_record) {
+            o.zero_wavefield();
+            if ((rc = o.step(ERM_PASS_MODEL, 0, o.p.lt))) return rc;
+        }
+        o.zero_wavefield();
+        o.zero_excitation();
+        if ((rc = o.step(ERM_PASS_SOURCE, 0, o.p.lt_forward))) return rc;
+        o.zero_wavefield();
+        o.zero_images();
+        return o.step(ERM_PASS_RECEIVER, o.p.lt - 1, o.p.lt);
+    });
+}
+
+}  // extern "C"
