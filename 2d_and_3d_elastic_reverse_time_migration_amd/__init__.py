@@ -73,6 +73,7 @@ def _load():
                                           P(ctypes.c_double), P(ctypes.c_long)]),
         "erm_algorithmic_bytes": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]),
         "erm_synchronize": (ctypes.c_int, [vp]),
+        "erm_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, P(ctypes.c_double)]),
         "erm_make_model3d": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
                                             fp, fp, fp]),
@@ -128,6 +129,13 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     lib.erm_device_count(ctypes.byref(n))
     return n.value
+
+
+def copy_bandwidth(nbytes, reps=20, device=0):
+    """GB/s of a plain streaming copy (2*nbytes of HBM traffic per copy)."""
+    ms = ctypes.c_double()
+    _check(lib.erm_copy_bandwidth(device, nbytes, reps, ctypes.byref(ms)))
+    return 2.0 * nbytes / (ms.value * 1e-3) / 1e9, ms.value
 
 
 def make_model3d(kind, nx, ny, nz, dx, dy, dz, seed):

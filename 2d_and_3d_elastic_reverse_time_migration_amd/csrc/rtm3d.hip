@@ -85,7 +85,7 @@ struct erm_ctx {
     bool have_record = false;
     int sxr = 0, syr = 0, szr = 0;  // shot position in model coordinates (sx_real, ...)
     long src = -1;                  // padded index of the source
-    int tx = 64, ty = 8, kchunk = 32;
+    int tx = 64, ty = 8, kchunk = 32, xcd_map = 0;
     std::string err;
     // timing
     hipEvent_t pass_ev[2] = {nullptr, nullptr};
@@ -120,6 +120,12 @@ Geo make_geo(const erm_ctx* c) {
     g.cx = c->cx; g.cy = c->cy; g.cz = c->cz;
     g.hx = c->hxyz[0]; g.hy = c->hxyz[1]; g.hz = c->hxyz[2];
     g.kchunk = c->kchunk;
+    g.ntx = (c->nnx + 63) / 64;
+    g.nty = (c->nny + c->ty - 1) / c->ty;
+    const int nch = (c->nnz - 2 * c->R + c->kchunk - 1) / c->kchunk;
+    g.ntiles = g.ntx * g.nty * nch;
+    g.xcd_map = c->xcd_map;
+    g.per_xcd = (g.ntiles + 7) / 8;
     return g;
 }
 
@@ -168,13 +174,14 @@ std::vector<float> ricker(float freq, float dt_s) {
 }
 
 
+inline dim3 grid_of(const Geo& g) { return dim3(g.xcd_map ? 8 * g.per_xcd : g.ntiles); }
+
 template <int R, int TX, int TY>
 void stress_launch(erm_ctx* c, int which, long src, int it) {
     const int src_k = src < 0 ? -1 : (int)(src / c->pxy), src_col = src < 0 ? -1 : (int)(src % c->pxy);
     Geo g = make_geo(c);
     dim3 blk(TX, TY);
-    int nzi = c->nnz - 2 * R;
-    dim3 grd((c->nnx + TX - 1) / TX, (c->nny + TY - 1) / TY, (nzi + c->kchunk - 1) / c->kchunk);
+    dim3 grd = grid_of(g);
     hipLaunchKernelGGL((k_stress<R, TX, TY>), grd, blk, 0, c->stream, make_wave(c), c->M[which], c->MU[which], g,
                        c->coef, src_k, src_col, c->wavelet, c->wl, it);
 }
@@ -183,8 +190,7 @@ template <int R, int TX, int TY, int MODE>
 void velocity_launch(erm_ctx* c, int which, int it) {
     Geo g = make_geo(c);
     dim3 blk(TX, TY);
-    int nzi = c->nnz - 2 * R;
-    dim3 grd((c->nnx + TX - 1) / TX, (c->nny + TY - 1) / TY, (nzi + c->kchunk - 1) / c->kchunk);
+    dim3 grd = grid_of(g);
     hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE>), grd, blk, 0, c->stream, make_wave(c), c->BU[which], g,
                        c->coef, make_excite(c), it);
 }
@@ -410,7 +416,11 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     c->nnx = pin->nx + pin->bl + pin->br;
     c->nny = pin->ny + pin->bf + pin->bb;
     c->nnz = pin->nz + pin->bu + pin->bd;
-    c->px = c->nnx;
+    {
+        const int align = pin->pitch_align == 1 ? 1 : (pin->pitch_align > 1 ? pin->pitch_align : 32);
+        c->px = (c->nnx + align - 1) / align * align;  // 128-byte rows: a 64-wide tile row = 2 cache lines
+    }
+    c->xcd_map = pin->xcd_map == 1 ? 1 : 0;
     c->pxy = (long)c->px * c->nny;
     c->ncell = (size_t)c->pxy * c->nnz;
     for (int m = 0; m <= 6; ++m) c->coef.c[m] = 0.f;
@@ -423,9 +433,9 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     {
         int tiles = ((c->nnx + 63) / 64) * ((c->nny + c->ty - 1) / c->ty);
         int nzi = c->nnz - 2 * R;
-        int chunks = std::max(1, (2048 + tiles - 1) / tiles);
+        int chunks = std::max(1, (4096 + tiles - 1) / tiles);
         int kc = (nzi + chunks - 1) / chunks;
-        kc = std::max(kc, std::min(nzi, 16));
+        kc = std::max(kc, std::min(nzi, 8));
         c->kchunk = pin->zchunk > 0 ? std::min(pin->zchunk, nzi) : kc;
     }
 #define CK(x)                         \
@@ -778,6 +788,37 @@ int erm_algorithmic_bytes(const erm_ctx* c, int pass, double* sb, double* vb) {
 int erm_synchronize(erm_ctx* c) {
     if (!c) return ERM_E_PARAM;
     HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int erm_copy_bandwidth(int device, size_t bytes, int reps, double* ms_per_copy) {
+    if (bytes < 16 || reps <= 0 || !ms_per_copy) { set_global_error("erm_copy_bandwidth: bad arguments"); return ERM_E_PARAM; }
+    if (hipSetDevice(device) != hipSuccess) { set_global_error("hipSetDevice failed"); return ERM_E_DEVICE; }
+    const long n4 = (long)(bytes / 16);
+    float4 *a = nullptr, *b = nullptr;
+    if (hipMalloc((void**)&a, n4 * 16) != hipSuccess || hipMalloc((void**)&b, n4 * 16) != hipSuccess) {
+        if (a) (void)hipFree(a);
+        set_global_error("hipMalloc failed");
+        return ERM_E_NOMEM;
+    }
+    (void)hipMemset(a, 0, n4 * 16);
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int blocks = 256 * 8 * 4;
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n4);  // warm-up
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n4);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_per_copy = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) { set_global_error("copy kernel failed: %s", hipGetErrorString(e)); return ERM_E_DEVICE; }
     return 0;
 }
 

@@ -94,6 +94,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-events", action="store_true")
+    ap.add_argument("--tile-y", type=int, default=0, help="launch shape override (0: library default)")
+    ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0: library default)")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run the plain copy kernel over GIB GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the whole grid fits one MI355X); --gpus {args.gpus} is not "
@@ -101,8 +105,14 @@ def main():
         sys.exit(2)
 
     erm = importlib.import_module(PKG)
+    if args.copy_calibration > 0:
+        nbytes = int(args.copy_calibration * (1 << 30))
+        gbs, ms = erm.copy_bandwidth(nbytes, reps=10)
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": round(ms, 4), "GB/s": round(gbs, 1)}), flush=True)
+        return
     nt = args.nt
     p = make_params(erm, nt, erm.BOUNDARY_DAMP)
+    p.tile_y, p.zchunk = args.tile_y, args.zchunk
     vp, vs, rho = erm.make_model3d(1, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, 2)
     vp_s = smooth_box(vp)
     vs_s = (vp_s / 1.732).astype(np.float32)

@@ -91,7 +91,9 @@ typedef struct erm_params {
     float cpml_r;       /* CPML target reflection coefficient (default 1e-3) */
     float cpml_vmax;    /* CPML reference velocity in m/s (0: max vp of the model) */
     int   tile_x, tile_y, zchunk;   /* kernel launch shape (0: autotuned default) */
-    int   reserved[8];
+    int   pitch_align;  /* row pitch multiple in floats (0: 32 = 128 B; 1: dense rows) */
+    int   xcd_map;      /* 0: plain workgroup order (default, measured faster); 1: XCD-aware tile mapping */
+    int   reserved[6];
 } erm_params;
 
 /* field ids for erm_get_field / erm_set_field */
@@ -190,6 +192,11 @@ int   erm_get_timing(erm_ctx* ctx, double* pass_ms, double* stress_ms, long* str
  * for the given pass, as the kernels are built (see the kernel sources). */
 int   erm_algorithmic_bytes(const erm_ctx* ctx, int pass, double* stress_bytes, double* velocity_bytes);
 int   erm_synchronize(erm_ctx* ctx);
+/* Measurement utility (no reference counterpart): time `reps` launches of a
+ * plain float4 streaming copy of `bytes` bytes on `device`; it is the
+ * calibration kernel for HBM byte counters and the practical bandwidth
+ * ceiling (2*bytes of traffic per copy). */
+int   erm_copy_bandwidth(int device, size_t bytes, int reps, double* ms_per_copy);
 
 /* Synthetic models (seeded; std::mt19937_64).  Arrays are model-region,
  * disk order [nx][ny][nz].  kind 0: flat layers + Gaussian anomaly,
