@@ -94,6 +94,33 @@ struct Excite {  // excitation state and images
 
 enum { MODE_PLAIN = 0, MODE_EXCITE = 1, MODE_IMAGE = 2 };
 
+// C-PML state (no reference counterpart; the reference damps instead).
+// Coefficients a, b per axis at integer ("full") and staggered ("half":
+// x-1/2, y-1/2, z+1/2) positions; 21 memory variables stored only in the
+// absorbing slabs of their derivative's axis: along an axis of n cells the
+// slab is [0, lo+1) U [n-hi-1, n) (width w = lo+hi+2, covering the half
+// positions too).  Layouts: x slabs [nnz][nny][wx], y slabs [nnz][wy][nnx],
+// z slabs [wz][nny][nnx].  psi ids: stress 0-2 (x: exx, exy, exz), 3-5
+// (y: eyy, eyx, eyz), 6-8 (z: ezz, ezx, ezy); velocity 9-12 (x: sxx, sxy,
+// sxz, tp), 13-16 (y: sxy, syy, syz, tp), 17-20 (z: sxz, syz, szz, tp).
+struct Pml {
+    const float* a[3][2];
+    const float* b[3][2];
+    float* psi[21];
+    int lo[3], hi[3], w[3];
+};
+
+__device__ __forceinline__ int slab_of(int i, int n, int lo, int hi) {
+    return i <= lo ? i : (i >= n - hi - 1 ? lo + 1 + i - (n - hi - 1) : -1);
+}
+// e += (psi = b psi + a e) where the profile is non-zero
+__device__ __forceinline__ float pml_upd(float* psi, long q, float a, float b, float e) {
+    if (a == 0.f) return e;
+    const float v = b * psi[q] + a * e;
+    psi[q] = v;
+    return e + v;
+}
+
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Global accesses go through raw buffer instructions: a wave-uniform buffer
@@ -124,11 +151,11 @@ __device__ __forceinline__ int bldi(rsrc_t r, unsigned vo, unsigned so) {
 // before updating them.
 // HBM per cell-update: reads v(3), rw stress+tp(14), reads M, mu(2) = 76 B.
 // ---------------------------------------------------------------------------
-template <int R, int TX, int TY>
+template <int R, int TX, int TY, bool CPML>
 __global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restrict__ Mm,
                                                    const float* __restrict__ MUm, Geo g, Coef cf,
                                                    int src_k, int src_col, const float* __restrict__ wavelet,
-                                                   int wl, int it) {
+                                                   int wl, int it, Pml P) {
     constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
     constexpr int NYH = (2 * R + TY - 1) / TY;
     __shared__ float sx[LY][LX];
@@ -244,7 +271,35 @@ __global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restri
                 const float s = wavelet[it];
                 sxx += s; syy += s; szz += s;
             }
-            const float exx = g.cx * dxvx, eyy = g.cy * dyvy, ezz = g.cz * dzvz;
+            float exx = g.cx * dxvx, eyy = g.cy * dyvy, ezz = g.cz * dzvz;
+            float e_yx = g.cy * dyvx, e_xy = g.cx * dxvy, e_zx = g.cz * dzvx, e_xz = g.cx * dxvz;
+            float e_zy = g.cz * dzvy, e_yz = g.cy * dyvz;
+            if constexpr (CPML) {
+                const int sxi = slab_of(i, g.nnx, P.lo[0], P.hi[0]);
+                if (sxi >= 0) {
+                    const long q = ((long)k * g.nny + j) * P.w[0] + sxi;
+                    const float af = P.a[0][0][i], bf = P.b[0][0][i], ah = P.a[0][1][i], bh = P.b[0][1][i];
+                    exx = pml_upd(P.psi[0], q, af, bf, exx);
+                    e_xy = pml_upd(P.psi[1], q, ah, bh, e_xy);
+                    e_xz = pml_upd(P.psi[2], q, ah, bh, e_xz);
+                }
+                const int syi = slab_of(j, g.nny, P.lo[1], P.hi[1]);
+                if (syi >= 0) {
+                    const long q = ((long)k * P.w[1] + syi) * g.nnx + i;
+                    const float af = P.a[1][0][j], bf = P.b[1][0][j], ah = P.a[1][1][j], bh = P.b[1][1][j];
+                    eyy = pml_upd(P.psi[3], q, af, bf, eyy);
+                    e_yx = pml_upd(P.psi[4], q, ah, bh, e_yx);
+                    e_yz = pml_upd(P.psi[5], q, ah, bh, e_yz);
+                }
+                const int szi = slab_of(k, g.nnz, P.lo[2], P.hi[2]);
+                if (szi >= 0) {
+                    const long q = ((long)szi * g.nny + j) * g.nnx + i;
+                    const float af = P.a[2][0][k], bf = P.b[2][0][k], ah = P.a[2][1][k], bh = P.b[2][1][k];
+                    ezz = pml_upd(P.psi[6], q, af, bf, ezz);
+                    e_zx = pml_upd(P.psi[7], q, ah, bh, e_zx);
+                    e_zy = pml_upd(P.psi[8], q, ah, bh, e_zy);
+                }
+            }
             const float h = hxy + g.hz[k];
             const float a = 1.0f / (1.0f + h), b = 1.0f - h;
             const float lam = M - 2.0f * mu;
@@ -252,9 +307,9 @@ __global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restri
             bst(ryy, o_c, so, a * (b * syy + M * eyy + lam * (exx + ezz)));
             bst(rzz, o_c, so, a * (b * szz + M * ezz + lam * (exx + eyy)));
             bst(rtp, o_c, so, a * (b * stp + M * (exx + eyy + ezz)));
-            bst(rxy, o_c, so, a * (b * sxy + mu * (g.cy * dyvx + g.cx * dxvy)));
-            bst(rxz, o_c, so, a * (b * sxz + mu * (g.cz * dzvx + g.cx * dxvz)));
-            bst(ryz, o_c, so, a * (b * syz + mu * (g.cz * dzvy + g.cy * dyvz)));
+            bst(rxy, o_c, so, a * (b * sxy + mu * (e_yx + e_xy)));
+            bst(rxz, o_c, so, a * (b * sxz + mu * (e_zx + e_xz)));
+            bst(ryz, o_c, so, a * (b * syz + mu * (e_zy + e_yz)));
         }
     }
 }
@@ -273,9 +328,9 @@ __global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restri
 // HBM per cell-update: reads stress+tp(7), rw v+vP(12), reads 1/rho(1) = 80 B,
 // +4 B ex_amp (EXCITE) or ex_time (IMAGE).
 // ---------------------------------------------------------------------------
-template <int R, int TX, int TY, int MODE>
+template <int R, int TX, int TY, int MODE, bool CPML>
 __global__ __launch_bounds__(TX* TY) void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
-                                                     Coef cf, Excite ex, int it) {
+                                                     Coef cf, Excite ex, int it, Pml P) {
     constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
     constexpr int NYH = (2 * R + TY - 1) / TY;
     __shared__ float s_xx[TY][LX];   // x-halo only
@@ -401,14 +456,47 @@ __global__ __launch_bounds__(TX* TY) void k_velocity(Wave w, const float* __rest
                 p2 += c[m] * (s_tp[cj + m - 1][ci] - s_tp[cj - m][ci]);        // D-y tp
                 p3 += c[m] * (qtp[R + m] - qtp[R - m + 1]);                    // D+z tp
             }
+            float gxx = g.cx * a1, gyx = g.cy * a2, gzx = g.cz * a3;
+            float gxy = g.cx * b1, gyy = g.cy * b2, gzy = g.cz * b3;
+            float gxz = g.cx * d1, gyz = g.cy * d2, gzz = g.cz * d3;
+            float px = g.cx * p1, py = g.cy * p2, pz = g.cz * p3;
+            if constexpr (CPML) {
+                const int sxi = slab_of(i, g.nnx, P.lo[0], P.hi[0]);
+                if (sxi >= 0) {
+                    const long q = ((long)k * g.nny + j) * P.w[0] + sxi;
+                    const float af = P.a[0][0][i], bf = P.b[0][0][i], ah = P.a[0][1][i], bh = P.b[0][1][i];
+                    gxx = pml_upd(P.psi[9], q, ah, bh, gxx);
+                    gxy = pml_upd(P.psi[10], q, af, bf, gxy);
+                    gxz = pml_upd(P.psi[11], q, af, bf, gxz);
+                    px = pml_upd(P.psi[12], q, ah, bh, px);
+                }
+                const int syi = slab_of(j, g.nny, P.lo[1], P.hi[1]);
+                if (syi >= 0) {
+                    const long q = ((long)k * P.w[1] + syi) * g.nnx + i;
+                    const float af = P.a[1][0][j], bf = P.b[1][0][j], ah = P.a[1][1][j], bh = P.b[1][1][j];
+                    gyx = pml_upd(P.psi[13], q, af, bf, gyx);
+                    gyy = pml_upd(P.psi[14], q, ah, bh, gyy);
+                    gyz = pml_upd(P.psi[15], q, af, bf, gyz);
+                    py = pml_upd(P.psi[16], q, ah, bh, py);
+                }
+                const int szi = slab_of(k, g.nnz, P.lo[2], P.hi[2]);
+                if (szi >= 0) {
+                    const long q = ((long)szi * g.nny + j) * g.nnx + i;
+                    const float af = P.a[2][0][k], bf = P.b[2][0][k], ah = P.a[2][1][k], bh = P.b[2][1][k];
+                    gzx = pml_upd(P.psi[17], q, af, bf, gzx);
+                    gzy = pml_upd(P.psi[18], q, af, bf, gzy);
+                    gzz = pml_upd(P.psi[19], q, ah, bh, gzz);
+                    pz = pml_upd(P.psi[20], q, ah, bh, pz);
+                }
+            }
             const float h = hxy + g.hz[k];
             const float a = 1.0f / (1.0f + h), b = 1.0f - h;
-            const float nvx = a * (b * vx + bo * (g.cx * a1 + g.cy * a2 + g.cz * a3));
-            const float nvy = a * (b * vy + bo * (g.cx * b1 + g.cy * b2 + g.cz * b3));
-            const float nvz = a * (b * vz + bo * (g.cx * d1 + g.cy * d2 + g.cz * d3));
-            const float nxp = a * (b * vxp + bo * (g.cx * p1));
-            const float nyp = a * (b * vyp + bo * (g.cy * p2));
-            const float nzp = a * (b * vzp + bo * (g.cz * p3));
+            const float nvx = a * (b * vx + bo * (gxx + gyx + gzx));
+            const float nvy = a * (b * vy + bo * (gxy + gyy + gzy));
+            const float nvz = a * (b * vz + bo * (gxz + gyz + gzz));
+            const float nxp = a * (b * vxp + bo * px);
+            const float nyp = a * (b * vyp + bo * py);
+            const float nzp = a * (b * vzp + bo * pz);
             bst(rvx, o_c, so, nvx); bst(rvy, o_c, so, nvy); bst(rvz, o_c, so, nvz);
             bst(rpx, o_c, so, nxp); bst(rpy, o_c, so, nyp); bst(rpz, o_c, so, nzp);
             // excitation / imaging state: the per-cell test reads one word

@@ -86,6 +86,14 @@ struct erm_ctx {
     int sxr = 0, syr = 0, szr = 0;  // shot position in model coordinates (sx_real, ...)
     long src = -1;                  // padded index of the source
     int tx = 64, ty = 8, kchunk = 32, xcd_map = 0;
+    // CPML (boundary == ERM_BOUNDARY_CPML)
+    bool cpml = false;
+    float* pml_coef = nullptr;          // a,b x {full,half} per axis, packed
+    float* psi[21] = {nullptr};
+    int slab_lo[3] = {0, 0, 0}, slab_hi[3] = {0, 0, 0}, slab_w[3] = {0, 0, 0};
+    Pml pml{};
+    double vmax_loaded = 0.0;
+    double pml_extra_stress = 0.0, pml_extra_velocity = 0.0;  // avg extra bytes per cell-update
     std::string err;
     // timing
     hipEvent_t pass_ev[2] = {nullptr, nullptr};
@@ -174,6 +182,14 @@ std::vector<float> ricker(float freq, float dt_s) {
 }
 
 
+template <typename T>
+int dmalloc(erm_ctx* c, T** ptr, size_t n) {
+    if (n == 0) { *ptr = nullptr; return 0; }
+    hipError_t e = hipMalloc((void**)ptr, n * sizeof(T));
+    if (e != hipSuccess) return c->fail(ERM_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
 inline dim3 grid_of(const Geo& g) { return dim3(g.xcd_map ? 8 * g.per_xcd : g.ntiles); }
 
 template <int R, int TX, int TY>
@@ -182,8 +198,12 @@ void stress_launch(erm_ctx* c, int which, long src, int it) {
     Geo g = make_geo(c);
     dim3 blk(TX, TY);
     dim3 grd = grid_of(g);
-    hipLaunchKernelGGL((k_stress<R, TX, TY>), grd, blk, 0, c->stream, make_wave(c), c->M[which], c->MU[which], g,
-                       c->coef, src_k, src_col, c->wavelet, c->wl, it);
+    if (c->cpml)
+        hipLaunchKernelGGL((k_stress<R, TX, TY, true>), grd, blk, 0, c->stream, make_wave(c), c->M[which],
+                           c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml);
+    else
+        hipLaunchKernelGGL((k_stress<R, TX, TY, false>), grd, blk, 0, c->stream, make_wave(c), c->M[which],
+                           c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml);
 }
 
 template <int R, int TX, int TY, int MODE>
@@ -191,8 +211,75 @@ void velocity_launch(erm_ctx* c, int which, int it) {
     Geo g = make_geo(c);
     dim3 blk(TX, TY);
     dim3 grd = grid_of(g);
-    hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE>), grd, blk, 0, c->stream, make_wave(c), c->BU[which], g,
-                       c->coef, make_excite(c), it);
+    if (c->cpml)
+        hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, true>), grd, blk, 0, c->stream, make_wave(c), c->BU[which],
+                           g, c->coef, make_excite(c), it, c->pml);
+    else
+        hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, false>), grd, blk, 0, c->stream, make_wave(c),
+                           c->BU[which], g, c->coef, make_excite(c), it, c->pml);
+}
+
+// C-PML coefficients (Roden & Gedney 2000; Komatitsch & Martin 2007, kappa = 1):
+// b = exp(-(d+alpha) dt), a = d (b-1)/(d+alpha), d = d0 r^2,
+// d0 = -3 vmax ln(Rc) / (2 L), alpha = pi f0 (1 - r), r = depth into the layer / L,
+// L = layer width - R (the outer R cells are never updated),
+// at integer positions and at the staggered ones (x-1/2, y-1/2, z+1/2).
+// Also counts, per kernel, the memory-variable bytes the launch moves.
+int build_cpml(erm_ctx* c) {
+    const erm_params& p = c->p;
+    const double vmax = p.cpml_vmax > 0 ? p.cpml_vmax : c->vmax_loaded;
+    const double Rc = p.cpml_r > 0 ? p.cpml_r : 1e-3;
+    const double dts = p.dt / 1000.0;
+    const int n3[3] = {c->nnx, c->nny, c->nnz};
+    const int lo[3] = {p.bl, p.bf, p.bu}, hi[3] = {p.br, p.bb, p.bd};
+    const double hh[3] = {p.dx, p.dy, p.dz};
+    const double shift[3] = {-0.5, -0.5, +0.5};
+    std::vector<float> host;
+    size_t off[3][2][2];
+    for (int a = 0; a < 3; ++a)
+        for (int f = 0; f < 2; ++f) {
+            off[a][f][0] = host.size();
+            host.insert(host.end(), n3[a], 0.f);
+            off[a][f][1] = host.size();
+            host.insert(host.end(), n3[a], 1.f);
+            for (int i = 0; i < n3[a]; ++i) {
+                // the layer spans the updated cells only (the outer R never update)
+                double x = i + (f ? shift[a] : 0.0), s = 0, L = 0;
+                if (lo[a] > 0 && x < lo[a]) { s = lo[a] - x; L = std::max(lo[a] - c->R, 1); }
+                else if (hi[a] > 0 && x > n3[a] - hi[a] - 1) { s = x - (n3[a] - hi[a] - 1); L = std::max(hi[a] - c->R, 1); }
+                if (L <= 0) continue;
+                double r = s / L, d0 = -3.0 * vmax * std::log(Rc) / (2.0 * L * hh[a]);
+                double d = d0 * r * r, al = M_PI * p.freq * (1.0 - std::min(r, 1.0));
+                double bb = std::exp(-(d + al) * dts);
+                host[off[a][f][1] + i] = (float)bb;
+                host[off[a][f][0] + i] = (float)(d > 0 ? d * (bb - 1.0) / (d + al) : 0.0);
+            }
+        }
+    if (!c->pml_coef && dmalloc(c, &c->pml_coef, host.size())) return ERM_E_NOMEM;
+    HIPC(c, hipMemcpyAsync(c->pml_coef, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a)
+        for (int f = 0; f < 2; ++f) {
+            c->pml.a[a][f] = c->pml_coef + off[a][f][0];
+            c->pml.b[a][f] = c->pml_coef + off[a][f][1];
+        }
+    // memory-variable traffic: 8 B (read+write) per psi at every updated cell whose profile is non-zero
+    const int R = c->R;
+    double cnt[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // updated positions with a != 0 along each axis
+    for (int a = 0; a < 3; ++a)
+        for (int f = 0; f < 2; ++f)
+            for (int i = R; i < n3[a] - R; ++i)
+                if (host[off[a][f][0] + i] != 0.f) cnt[a][f] += 1;
+    const double span[3] = {(double)(c->nnx - 2 * R), (double)(c->nny - 2 * R), (double)(c->nnz - 2 * R)};
+    // stress psi per axis: 1 full + 2 half; velocity psi per axis: 2 full + 2 half
+    double st = 0, ve = 0;
+    for (int a = 0; a < 3; ++a) {
+        st += 8.0 * (cnt[a][0] + 2 * cnt[a][1]) / span[a];
+        ve += 8.0 * (2 * cnt[a][0] + 2 * cnt[a][1]) / span[a];
+    }
+    c->pml_extra_stress = st;
+    c->pml_extra_velocity = ve;
+    return 0;
 }
 
 void do_stress(erm_ctx* c, int which, long src, int it) {
@@ -246,17 +333,10 @@ int check_grid(const erm_params* p, int R, std::string* err) {
     if (nnx < 2 * R + 1 || nny < 2 * R + 1 || nnz < 2 * R + 1) { *err = "padded grid smaller than the stencil"; return ERM_E_SIZE; }
     if (nnx * nny * nnz > (1L << 34)) { *err = "grid too large"; return ERM_E_SIZE; }
     if (p->receiver_num_x < 0 || p->receiver_num_y < 0) { *err = "receiver counts must be >= 0"; return ERM_E_PARAM; }
-    if (p->boundary != ERM_BOUNDARY_DAMP) { *err = "boundary=1 (CPML) is not available in this build"; return ERM_E_PARAM; }
+    if (p->boundary != ERM_BOUNDARY_DAMP && p->boundary != ERM_BOUNDARY_CPML) { *err = "boundary must be 0 (damping) or 1 (CPML)"; return ERM_E_PARAM; }
     return 0;
 }
 
-template <typename T>
-int dmalloc(erm_ctx* c, T** ptr, size_t n) {
-    if (n == 0) { *ptr = nullptr; return 0; }
-    hipError_t e = hipMalloc((void**)ptr, n * sizeof(T));
-    if (e != hipSuccess) return c->fail(ERM_E_NOMEM, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T), hipGetErrorString(e));
-    return 0;
-}
 
 void free_all(erm_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
@@ -267,6 +347,8 @@ void free_all(erm_ctx* c) {
     for (auto& p : c->ex) F(p);
     for (auto& p : c->img) F(p);
     F(c->rec); F(c->rec_idx);
+    F(c->pml_coef);
+    for (auto& p : c->psi) F(p);
     for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -456,10 +538,36 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     for (auto& p : c->ex) CK(dmalloc(c, &p, c->ncell));
     for (auto& p : c->img) CK(dmalloc(c, &p, c->ncell));
     CK(dmalloc(c, &c->ex_time, c->ncell));
+    if (pin->boundary == ERM_BOUNDARY_CPML) {
+        c->cpml = true;
+        const int n3[3] = {c->nnx, c->nny, c->nnz};
+        const int lo[3] = {pin->bl, pin->bf, pin->bu}, hi[3] = {pin->br, pin->bb, pin->bd};
+        for (int a = 0; a < 3; ++a) {
+            c->slab_lo[a] = lo[a];
+            c->slab_hi[a] = hi[a];
+            c->slab_w[a] = lo[a] + hi[a] + 2;
+            if (c->slab_w[a] > n3[a]) {  // layers cover the axis: keep the slab map inside [0, n)
+                c->slab_hi[a] = std::max(0, n3[a] - lo[a] - 2);
+                c->slab_w[a] = lo[a] + c->slab_hi[a] + 2;
+            }
+            c->pml.lo[a] = c->slab_lo[a];
+            c->pml.hi[a] = c->slab_hi[a];
+            c->pml.w[a] = c->slab_w[a];
+        }
+        const size_t sz[3] = {(size_t)c->nnz * c->nny * c->slab_w[0], (size_t)c->nnz * c->slab_w[1] * c->nnx,
+                              (size_t)c->slab_w[2] * c->nny * c->nnx};
+        static const int axis_of[21] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+        for (int q = 0; q < 21; ++q) {
+            CK(dmalloc(c, &c->psi[q], sz[axis_of[q]]));
+            c->pml.psi[q] = c->psi[q];
+        }
+        c->vmax_loaded = 0.0;
+        CK(build_cpml(c));
+    }
     // damping, h = d*dt/2 per axis (3D_zzzzz:261-369)
     {
         std::vector<double> du, dd, dl, dr, df, db;
-        double cA = pin->coe_attenuation;
+        double cA = pin->boundary == ERM_BOUNDARY_CPML ? 0.0 : pin->coe_attenuation;  // CPML replaces damping
         face_profile(pin->bu, cA, false, du); face_profile(pin->bd, cA, true, dd);
         face_profile(pin->bl, cA, false, dl); face_profile(pin->br, cA, true, dr);
         face_profile(pin->bf, cA, false, df); face_profile(pin->bb, cA, true, db);
@@ -564,6 +672,10 @@ int erm_load_model(erm_ctx* c, int which, const float* vp, const float* vs, cons
     if (upload(c, c->M[which], hM.data()) || upload(c, c->MU[which], hU.data()) || upload(c, c->BU[which], hB.data()))
         return ERM_E_DEVICE;
     c->have_model[which] = true;
+    double vm = 0;
+    for (size_t n = 0; n < N; ++n) vm = std::max(vm, (double)pvp[n]);
+    c->vmax_loaded = std::max(c->vmax_loaded, vm);
+    if (c->cpml) return build_cpml(c);
     return 0;
 }
 
@@ -606,6 +718,12 @@ int erm_set_shot(erm_ctx* c, int isx, int isy, int isz) {
 int erm_zero_wavefield(erm_ctx* c) {
     if (!c) return ERM_E_PARAM;
     for (auto& p : c->f) if (fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
+    if (c->cpml) {
+        const size_t sz[3] = {(size_t)c->nnz * c->nny * c->slab_w[0], (size_t)c->nnz * c->slab_w[1] * c->nnx,
+                              (size_t)c->slab_w[2] * c->nny * c->nnx};
+        static const int axis_of[21] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
+        for (int q = 0; q < 21; ++q) if (fill(c, c->psi[q], sz[axis_of[q]], 0.f)) return ERM_E_DEVICE;
+    }
     return 0;
 }
 
@@ -780,8 +898,10 @@ int erm_algorithmic_bytes(const erm_ctx* c, int pass, double* sb, double* vb) {
     // stress: v(3) read + 7 stress/tp rw (14) + M, mu (2) = 19 floats
     // velocity: 7 stress/tp read + 6 v/vP rw (12) + 1/rho (1) = 20 floats
     //           + ex_amp (source pass) or ex_time (receiver pass)
-    if (sb) *sb = 19.0 * 4.0;
-    if (vb) *vb = 20.0 * 4.0 + ((pass == ERM_PASS_SOURCE || pass == ERM_PASS_RECEIVER) ? 4.0 : 0.0);
+    // + C-PML memory variables (8 B per psi at every updated cell inside its layer)
+    if (sb) *sb = 19.0 * 4.0 + (c->cpml ? c->pml_extra_stress : 0.0);
+    if (vb) *vb = 20.0 * 4.0 + ((pass == ERM_PASS_SOURCE || pass == ERM_PASS_RECEIVER) ? 4.0 : 0.0) +
+                  (c->cpml ? c->pml_extra_velocity : 0.0);
     return 0;
 }
 

@@ -1,7 +1,8 @@
 """Benchmark: 3D elastic RTM on one MI355X (Gcell-updates/s).
 
 Workload (the 3D single-shot configuration): 256x256x128 model, d = 10 m,
-20-cell absorbing boundary on every face (296x296x168 padded grid), 8th-order
+20-cell C-PML on every face (296x296x168 padded grid; --boundary damp selects
+the reference's damping instead), 8th-order
 staggered FD (radius 4), dt = 0.8 ms, nt = 2500, 9 wavefield components plus
 the vector-P split (tp, vxp, vyp, vzp), FP32.  Model: five dipping layers
 vp 1800-4200 m/s, vs = vp/1.732, density 230*vp^0.25, three random Gaussian
@@ -96,6 +97,7 @@ def main():
     ap.add_argument("--no-kernel-events", action="store_true")
     ap.add_argument("--tile-y", type=int, default=0, help="launch shape override (0: library default)")
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0: library default)")
+    ap.add_argument("--boundary", choices=["cpml", "damp"], default="cpml")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the plain copy kernel over GIB GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -111,7 +113,7 @@ def main():
         print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": round(ms, 4), "GB/s": round(gbs, 1)}), flush=True)
         return
     nt = args.nt
-    p = make_params(erm, nt, erm.BOUNDARY_DAMP)
+    p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
     p.tile_y, p.zchunk = args.tile_y, args.zchunk
     vp, vs, rho = erm.make_model3d(1, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, 2)
     vp_s = smooth_box(vp)
@@ -198,9 +200,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded dipping-layer + anomaly model)",
-        "config": {"workload": "3D elastic RTM, one shot: 256x256x128 model + 20-cell damping boundary "
+        "config": {"workload": "3D elastic RTM, one shot: 256x256x128 model + 20-cell %s "
                                "(296x296x168), 8th-order staggered FD, dt=0.8 ms, nt=%d, 15 Hz Ricker, "
-                               "128x128 receivers; step = source pass + receiver pass" % nt,
+                               "128x128 receivers; step = source pass + receiver pass"
+                               % ("C-PML" if args.boundary == "cpml" else "damping boundary", nt),
                    "cells_per_time_step": cells, "time_steps_per_shot": 2 * nt,
                    "parallelism": "single GPU"},
         "roofline": roof,

@@ -20,6 +20,10 @@
  *   vs, density from vp                 3D_elastic_RTM/zzzzz:1261-1287
  *   excitation amplitude/time           2D_elastic_RTM/elastic_2D_kernel_3.cu:30-48,71-89
  *   excitation-amplitude imaging        2D_elastic_RTM/elastic_2D_kernel_3.cu:542-580
+ * C-PML (boundary == ERM_BOUNDARY_CPML) has no reference counterpart: it is
+ * the standard convolutional PML (Roden & Gedney 2000, Komatitsch & Martin
+ * 2007) with kappa = 1, pinned by tests/test_cpml.py (exact no-op at Rc = 1,
+ * reflection against an enlarged domain).
  * The 3D kernel bodies are absent from the reference; the 3D staggering
  * (vy at (i,j-1/2,k), sxy at (i-1/2,j-1/2,k), syz at (i,j-1/2,k+1/2)) extends
  * the 2D positions of viscoelastic_equation.cu by symmetry.
@@ -96,6 +100,13 @@ struct Oracle3D {
     std::vector<T> ex_amp, ex_tp, ex_vxp, ex_vyp, ex_vzp;
     std::vector<T> img_tp, img_pp, img_ps;
     std::vector<T> rec;                   /* [3][lt][ry][rx] */
+    /* CPML (no reference counterpart): per axis a, b at integer ("full") and
+     * staggered ("half": x-1/2, y-1/2, z+1/2) positions, and 21 memory
+     * variables kept on the whole grid (zero outside the absorbing layers). */
+    bool cpml = false;
+    std::vector<T> ca[3][2], cb[3][2];    /* [axis][0 full, 1 half] */
+    std::vector<T> psi[21];
+    double vmax_loaded = 0.0;
     int sx = 0, sy = 0, sz = 0;           /* padded-grid source position */
 
     size_t id(int i, int j, int k) const { return ((size_t)k * nny + j) * nnx + i; }
@@ -105,7 +116,7 @@ struct Oracle3D {
         if (p.fd_order == 12) R = 6;
         else if (p.fd_order == 8) R = 4;
         else return ERM_E_PARAM;
-        if (p.boundary != ERM_BOUNDARY_DAMP) return ERM_E_PARAM; /* CPML: oracle3d_cpml */
+        if (p.boundary != ERM_BOUNDARY_DAMP && p.boundary != ERM_BOUNDARY_CPML) return ERM_E_PARAM;
         nnx = p.nx + p.bl + p.br;
         nny = p.ny + p.bf + p.bb;
         nnz = p.nz + p.bu + p.bd;
@@ -119,7 +130,7 @@ struct Oracle3D {
         for (auto& v : f) v.assign(N, (T)0);
         /* damping, 3D_zzzzz:261-369 (sum over the six faces) */
         std::vector<double> du, dd, dl, dr, df, db;
-        double cA = p.coe_attenuation;
+        double cA = p.boundary == ERM_BOUNDARY_CPML ? 0.0 : p.coe_attenuation;  /* CPML replaces damping */
         face_profile(p.bu, cA, false, du);
         face_profile(p.bd, cA, true, dd);
         face_profile(p.bl, cA, false, dl);
@@ -182,7 +193,56 @@ struct Oracle3D {
             B[which][n] = (T)1 / r;
         }
         have_model[which] = true;
+        double vm = 0;
+        for (size_t n = 0; n < N; ++n) vm = std::max(vm, pvp[n]);
+        vmax_loaded = std::max(vmax_loaded, vm);
+        if (p.boundary == ERM_BOUNDARY_CPML) build_cpml();
         return ERM_OK;
+    }
+
+    /* C-PML (Roden & Gedney 2000; Komatitsch & Martin 2007) with kappa = 1:
+     * psi' = b psi + a e,  e' = e + psi',  b = exp(-(d+alpha) dt),
+     * a = d (b-1)/(d+alpha);  d = d0 r^2, d0 = -3 vmax ln(Rc) / (2 L),
+     * alpha = pi f0 (1-r), r = distance into the layer / L, L = width - R. */
+    void build_cpml() {
+        cpml = true;
+        const double vmax = p.cpml_vmax > 0 ? p.cpml_vmax : vmax_loaded;
+        const double Rc = p.cpml_r > 0 ? p.cpml_r : 1e-3;
+        const double dts = p.dt / 1000.0;
+        const int n3[3] = {nnx, nny, nnz};
+        const int lo[3] = {p.bl, p.bf, p.bu}, hi[3] = {p.br, p.bb, p.bd};
+        const double h[3] = {p.dx, p.dy, p.dz};
+        const double half_shift[3] = {-0.5, -0.5, +0.5};
+        for (int a = 0; a < 3; ++a)
+            for (int f = 0; f < 2; ++f) {
+                ca[a][f].assign(n3[a], 0);
+                cb[a][f].assign(n3[a], 1);
+                for (int i = 0; i < n3[a]; ++i) {
+                    double x = i + (f ? half_shift[a] : 0.0);
+                    /* the layer spans the updated cells only: the outer R
+                     * cells are never updated, so L = width - R */
+                    double s = 0, L = 0;
+                    if (lo[a] > 0 && x < lo[a]) { s = lo[a] - x; L = std::max(lo[a] - R, 1); }
+                    else if (hi[a] > 0 && x > n3[a] - hi[a] - 1) { s = x - (n3[a] - hi[a] - 1); L = std::max(hi[a] - R, 1); }
+                    if (L <= 0) continue;
+                    double r = s / L;
+                    double d0 = -3.0 * vmax * std::log(Rc) / (2.0 * L * h[a]);
+                    double d = d0 * r * r;
+                    double al = M_PI * p.freq * (1.0 - std::min(r, 1.0));
+                    double bb = std::exp(-(d + al) * dts);
+                    cb[a][f][i] = (T)bb;
+                    ca[a][f][i] = (T)(d > 0 ? d * (bb - 1.0) / (d + al) : 0.0);
+                }
+            }
+        for (auto& v : psi) v.assign(N, (T)0);
+    }
+    /* one memory-variable update: e += (psi = b psi + a e) */
+    inline T pml(int q, size_t n, int axis, int half, int idx, T e) {
+        const T bq = cb[axis][half][idx], aq = ca[axis][half][idx];
+        if (aq == (T)0) return e;
+        T v = bq * psi[q][n] + aq * e;
+        psi[q][n] = v;
+        return e + v;
     }
 
     /* D+ f[i] = sum c_m (f[i+m] - f[i-m+1])  (centred at i+1/2),
@@ -221,9 +281,15 @@ struct Oracle3D {
                 for (int i = R; i < nnx - R; ++i) {
                     size_t n = id(i, j, k);
                     T exx = cx * dp(vx + n, sX), eyy = cy * dp(vy + n, sY), ezz = cz * dm(vz + n, sZ);
-                    T exy = cy * dm(vx + n, sY) + cx * dm(vy + n, sX);
-                    T exz = cz * dp(vx + n, sZ) + cx * dm(vz + n, sX);
-                    T eyz = cz * dp(vy + n, sZ) + cy * dm(vz + n, sY);
+                    T e_yx = cy * dm(vx + n, sY), e_xy = cx * dm(vy + n, sX);
+                    T e_zx = cz * dp(vx + n, sZ), e_xz = cx * dm(vz + n, sX);
+                    T e_zy = cz * dp(vy + n, sZ), e_yz = cy * dm(vz + n, sY);
+                    if (cpml) {
+                        exx = pml(0, n, 0, 0, i, exx); e_xy = pml(1, n, 0, 1, i, e_xy); e_xz = pml(2, n, 0, 1, i, e_xz);
+                        eyy = pml(3, n, 1, 0, j, eyy); e_yx = pml(4, n, 1, 1, j, e_yx); e_yz = pml(5, n, 1, 1, j, e_yz);
+                        ezz = pml(6, n, 2, 0, k, ezz); e_zx = pml(7, n, 2, 1, k, e_zx); e_zy = pml(8, n, 2, 1, k, e_zy);
+                    }
+                    T exy = e_yx + e_xy, exz = e_zx + e_xz, eyz = e_zy + e_yz;
                     T d = ax[i] + ay[j] + az[k];
                     T a = (T)1 / ((T)1 + d * h), b = (T)1 - d * h;
                     T Mc = Mm[n], mu = Um[n], lam = Mc - (T)2 * mu;
@@ -256,10 +322,21 @@ struct Oracle3D {
                     T d = ax[i] + ay[j] + az[k];
                     T a = (T)1 / ((T)1 + d * h), b = (T)1 - d * h;
                     T bo = Bm[n];
-                    T fx = cx * dm(txx + n, sX) + cy * dp(txy + n, sY) + cz * dm(txz + n, sZ);
-                    T fy = cx * dp(txy + n, sX) + cy * dm(tyy + n, sY) + cz * dm(tyz + n, sZ);
-                    T fz = cx * dp(txz + n, sX) + cy * dp(tyz + n, sY) + cz * dp(tzz + n, sZ);
+                    T gxx = cx * dm(txx + n, sX), gyx = cy * dp(txy + n, sY), gzx = cz * dm(txz + n, sZ);
+                    T gxy = cx * dp(txy + n, sX), gyy = cy * dm(tyy + n, sY), gzy = cz * dm(tyz + n, sZ);
+                    T gxz = cx * dp(txz + n, sX), gyz = cy * dp(tyz + n, sY), gzz = cz * dp(tzz + n, sZ);
                     T px = cx * dm(tp + n, sX), py = cy * dm(tp + n, sY), pz = cz * dp(tp + n, sZ);
+                    if (cpml) {
+                        gxx = pml(9, n, 0, 1, i, gxx); gxy = pml(10, n, 0, 0, i, gxy);
+                        gxz = pml(11, n, 0, 0, i, gxz); px = pml(12, n, 0, 1, i, px);
+                        gyx = pml(13, n, 1, 0, j, gyx); gyy = pml(14, n, 1, 1, j, gyy);
+                        gyz = pml(15, n, 1, 0, j, gyz); py = pml(16, n, 1, 1, j, py);
+                        gzx = pml(17, n, 2, 0, k, gzx); gzy = pml(18, n, 2, 0, k, gzy);
+                        gzz = pml(19, n, 2, 1, k, gzz); pz = pml(20, n, 2, 1, k, pz);
+                    }
+                    T fx = gxx + gyx + gzx;
+                    T fy = gxy + gyy + gzy;
+                    T fz = gxz + gyz + gzz;
                     T nvx = a * (b * vx[n] + bo * fx);
                     T nvy = a * (b * vy[n] + bo * fy);
                     T nvz = a * (b * vz[n] + bo * fz);
@@ -353,7 +430,10 @@ struct Oracle3D {
         return ERM_OK;
     }
 
-    void zero_wavefield() { for (auto& v : f) std::fill(v.begin(), v.end(), (T)0); }
+    void zero_wavefield() {
+        for (auto& v : f) std::fill(v.begin(), v.end(), (T)0);
+        for (auto& v : psi) std::fill(v.begin(), v.end(), (T)0);
+    }
     void zero_excitation() {
         std::fill(ex_time.begin(), ex_time.end(), -1);
         for (auto* v : {&ex_amp, &ex_tp, &ex_vxp, &ex_vyp, &ex_vzp}) std::fill(v->begin(), v->end(), (T)0);
