@@ -27,6 +27,7 @@ LIB_PATH = os.path.join(_HERE, "liberm.so")
 
 from .params import *  # noqa: F401,F403
 from .params import PATH_MAX, Params, FIELD_ID, FIELDS  # noqa: F401
+from .params2d import Params2D, FIELD2D_ID, FIELDS2D  # noqa: F401
 
 
 class ErmError(RuntimeError):
@@ -80,10 +81,32 @@ def _load():
         "erm_write_raw": (ctypes.c_int, [ctypes.c_char_p, fp, ctypes.c_size_t]),
         "erm_read_raw": (ctypes.c_int, [ctypes.c_char_p, fp, ctypes.c_size_t]),
     }
+    ip = P(ctypes.c_int)
+    sig.update({
+        "erm2d_last_error": (ctypes.c_char_p, [vp]),
+        "erm2d_params_size": (ctypes.c_size_t, []),
+        "erm2d_params_default": (ctypes.c_int, [vp]),
+        "erm2d_create": (ctypes.c_int, [vp, ctypes.c_int, P(vp)]),
+        "erm2d_destroy": (ctypes.c_int, [vp]),
+        "erm2d_load_model": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, fp, fp, fp]),
+        "erm2d_set_shot": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "erm2d_zero": (ctypes.c_int, [vp]),
+        "erm2d_step": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "erm2d_get_field": (ctypes.c_int, [vp, ctypes.c_int, fp]),
+        "erm2d_set_field": (ctypes.c_int, [vp, ctypes.c_int, fp]),
+        "erm2d_get_dims": (ctypes.c_int, [vp, ip, ip]),
+        "erm2d_forward": (ctypes.c_int, [vp, fp]),
+        "erm2d_rtm": (ctypes.c_int, [vp, fp, fp, fp, fp]),
+        "erm2d_born": (ctypes.c_int, [vp, fp, fp]),
+        "erm2d_adjoint": (ctypes.c_int, [vp, fp, fp]),
+        "erm2d_last_ms": (ctypes.c_int, [vp, P(ctypes.c_double)]),
+    })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    if lib.erm2d_params_size() != ctypes.sizeof(Params2D):
+        raise ImportError("erm2d_params size mismatch")
     if lib.erm_params_size() != ctypes.sizeof(Params):
         raise ImportError(f"erm_params size mismatch: C {lib.erm_params_size()} vs Python {ctypes.sizeof(Params)}")
     return lib
@@ -272,3 +295,95 @@ class RTM3D:
 
     def synchronize(self):
         self._c(lib.erm_synchronize(self._h))
+
+
+def _check2(rc, ctx=None):
+    if rc != 0:
+        raise ErmError(rc, lib.erm2d_last_error(ctx).decode(errors="replace"))
+
+
+class Elastic2D:
+    """2D elastic / viscoelastic context (include/erm2d.h): modeling, RTM,
+    and the LSRTM Born / adjoint operators."""
+
+    def __init__(self, p: Params2D, device: int = 0):
+        if ctypes.sizeof(p) != ctypes.sizeof(Params2D):
+            raise TypeError("not an erm2d_params struct")
+        self.p = p
+        h = ctypes.c_void_p()
+        _check2(lib.erm2d_create(ctypes.addressof(p), device, ctypes.byref(h)))
+        self._h = h
+        nx, nz = ctypes.c_int(), ctypes.c_int()
+        lib.erm2d_get_dims(h, ctypes.byref(nx), ctypes.byref(nz))
+        self.shape = (nx.value, nz.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.erm2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        _check2(rc, self._h)
+
+    def load_model(self, which, vp, vs, rho, qp=None, qs=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho, qp, qs)]
+        self._c(lib.erm2d_load_model(self._h, which, *[_fptr(a) for a in arrs]))
+
+    def set_shot(self, sx, sz):
+        self._c(lib.erm2d_set_shot(self._h, sx, sz))
+
+    def zero(self):
+        self._c(lib.erm2d_zero(self._h))
+
+    def step(self, pass_, it_begin, nsteps):
+        self._c(lib.erm2d_step(self._h, pass_, it_begin, nsteps))
+
+    def get_field(self, name):
+        out = np.empty(self.shape, np.float32)
+        self._c(lib.erm2d_get_field(self._h, FIELD2D_ID[name] if isinstance(name, str) else name, _fptr(out)))
+        return out
+
+    def set_field(self, name, a):
+        a = np.ascontiguousarray(a, np.float32)
+        assert a.shape == self.shape
+        self._c(lib.erm2d_set_field(self._h, FIELD2D_ID[name], _fptr(a)))
+
+    def record_shape(self):
+        return (2, self.p.lt, self.p.nrec)
+
+    def forward(self):
+        rec = np.empty(self.record_shape(), np.float32)
+        self._c(lib.erm2d_forward(self._h, _fptr(rec)))
+        return rec
+
+    def rtm(self, rec=None):
+        p = self.p
+        out = [np.empty((p.nx, p.nz), np.float32) for _ in range(3)]
+        r = None if rec is None else np.ascontiguousarray(rec, np.float32)
+        self._c(lib.erm2d_rtm(self._h, _fptr(r), *[_fptr(o) for o in out]))
+        return out
+
+    def born(self, dm):
+        d = np.empty(self.record_shape(), np.float32)
+        self._c(lib.erm2d_born(self._h, _fptr(np.ascontiguousarray(dm, np.float32)), _fptr(d)))
+        return d
+
+    def adjoint(self, r):
+        p = self.p
+        g = np.empty((3, p.nx, p.nz), np.float32)
+        self._c(lib.erm2d_adjoint(self._h, _fptr(np.ascontiguousarray(r, np.float32)), _fptr(g)))
+        return g
+
+    def last_ms(self):
+        v = ctypes.c_double()
+        self._c(lib.erm2d_last_ms(self._h, ctypes.byref(v)))
+        return v.value

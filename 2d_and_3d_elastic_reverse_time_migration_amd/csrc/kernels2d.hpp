@@ -1,0 +1,425 @@
+// kernels2d.hpp - gfx950 kernels of the 2D elastic / viscoelastic step, the
+// LSRTM Born sources and the exact discrete adjoint.
+//
+// Layout: z fastest, idx = i*pz + k (the reference's 2D order ix*nz+iz,
+// viscoelastic_equation.cu:90-92, with the z pitch rounded up to 32 floats).
+// A workgroup is 64 lanes along z (one wavefront per x row, coalesced) by
+// TXR rows along x; stencil inputs go through LDS tiles with R-cell halos in
+// both directions.  Only interior cells (R <= i < nnx-R, R <= k < nnz-R) are
+// updated.  Staggering (viscoelastic_equation.cu:137-176): sxx, szz, tp, r_p,
+// r_xx, r_zz at (i,k); vx (i-1/2,k); vz (i,k+1/2); sxz, r_xz (i-1/2,k+1/2).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace erm2 {
+
+constexpr int TZ = 64;  // lanes along z
+
+struct Coef2 {
+    float c[7];
+};
+
+struct G2 {
+    int nnx, nnz, pz;
+    float cx, cz, dtr;   // dt/dx, dt/dz (dt in s), dt
+    const float* att;    // damping d per cell
+    int bl, bu, nx, nz;  // model region
+};
+
+struct F2 {  // one wavefield state (forward, Born, or adjoint)
+    float *vx, *vz, *xx, *zz, *xz, *tp, *vxp, *vzp;
+    float* r[3][4];  // memory variables per mechanism: p, xx, zz, xz
+};
+
+struct M2 {  // model: unrelaxed moduli, buoyancy, SLS coefficients
+    const float *M, *U, *B;
+    const float* kP[3];
+    const float* kS[3];
+    const float* decay[3];  // dt / tau_sigma_l
+};
+
+struct Ex2 {
+    int* time;
+    float *amp, *tp, *vxp, *vzp, *img_tp, *img_pp, *img_ps;
+};
+
+__device__ __forceinline__ int clamp2(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// LDS tile [TXR+2R][TZ+2R] of `f` around the block's (i0, k0) corner; lanes
+// load the centre, the z halo (lanes k<R and k>=TZ-R) and the x halo rows.
+template <int R, int TXR>
+__device__ __forceinline__ void load_tile(float (*t)[TZ + 2 * R], const float* __restrict__ f, const G2& g, int i0,
+                                          int k0) {
+    const int tz = threadIdx.x, tx = threadIdx.y;
+    const int i = clamp2(i0 + tx, 0, g.nnx - 1);
+    const int k = clamp2(k0 + tz, 0, g.nnz - 1);
+    t[tx + R][tz + R] = f[(long)i * g.pz + k];
+    if (tz < R) t[tx + R][tz] = f[(long)i * g.pz + clamp2(k0 - R + tz, 0, g.nnz - 1)];
+    if (tz >= TZ - R) t[tx + R][tz + 2 * R] = f[(long)i * g.pz + clamp2(k0 + tz + R, 0, g.nnz - 1)];
+    for (int r = tx; r < 2 * R; r += TXR) {
+        const int ii = clamp2(r < R ? i0 - R + r : i0 + TXR + (r - R), 0, g.nnx - 1);
+        t[r < R ? r : TXR + r][tz + R] = f[(long)ii * g.pz + k];
+    }
+}
+
+// D+ f centred at +1/2 and D- f at -1/2 along x (rows) or z (columns) of an LDS tile
+template <int R>
+__device__ __forceinline__ float dpx(const float (*t)[TZ + 2 * R], const float* c, int a, int b) {
+    float s = 0.f;
+#pragma unroll
+    for (int m = 1; m <= R; ++m) s += c[m] * (t[a + m][b] - t[a - m + 1][b]);
+    return s;
+}
+template <int R>
+__device__ __forceinline__ float dmx(const float (*t)[TZ + 2 * R], const float* c, int a, int b) {
+    float s = 0.f;
+#pragma unroll
+    for (int m = 1; m <= R; ++m) s += c[m] * (t[a + m - 1][b] - t[a - m][b]);
+    return s;
+}
+template <int R>
+__device__ __forceinline__ float dpz(const float (*t)[TZ + 2 * R], const float* c, int a, int b) {
+    float s = 0.f;
+#pragma unroll
+    for (int m = 1; m <= R; ++m) s += c[m] * (t[a][b + m] - t[a][b - m + 1]);
+    return s;
+}
+template <int R>
+__device__ __forceinline__ float dmz(const float (*t)[TZ + 2 * R], const float* c, int a, int b) {
+    float s = 0.f;
+#pragma unroll
+    for (int m = 1; m <= R; ++m) s += c[m] * (t[a][b + m - 1] - t[a][b - m]);
+    return s;
+}
+
+__device__ __forceinline__ bool in_model(const G2& g, int i, int k, long* q) {
+    if (i < g.bl || i >= g.bl + g.nx || k < g.bu || k >= g.bu + g.nz) return false;
+    *q = (long)(i - g.bl) * g.nz + (k - g.bu);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Stress + memory + tp (fwd_txxzzxzpp_viscoelastic_and_memory_3parameterization,
+// viscoelastic_equation.cu:184-325; tp as :172), L mechanisms, explosive
+// source fused (src = cell index or -1).  With SAVE, the strain increments
+// E = (Vx, Vz, S) of the model region go to Eh ([3][nx*nz]) for LSRTM.
+// ---------------------------------------------------------------------------
+template <int R, int L, int TXR, bool SAVE>
+__global__ __launch_bounds__(TZ* TXR) void k2_stress(F2 f, M2 m, G2 g, Coef2 cf, long src, float srcval,
+                                                     float* __restrict__ Eh) {
+    __shared__ float tvx[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float tvz[TXR + 2 * R][TZ + 2 * R];
+    const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
+    load_tile<R, TXR>(tvx, f.vx, g, i0, k0);
+    load_tile<R, TXR>(tvz, f.vz, g, i0, k0);
+    __syncthreads();
+    const int k = k0 + threadIdx.x, i = i0 + threadIdx.y;
+    if (i < R || i >= g.nnx - R || k < R || k >= g.nnz - R) return;
+    float c[R + 1];
+#pragma unroll
+    for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
+    const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
+    const float E0 = g.cx * dpx<R>(tvx, c, a_, b_);
+    const float E1 = g.cz * dmz<R>(tvz, c, a_, b_);
+    const float E2 = g.cx * dmx<R>(tvz, c, a_, b_) + g.cz * dpz<R>(tvx, c, a_, b_);
+    const long n = (long)i * g.pz + k;
+    if constexpr (SAVE) {
+        long q;
+        if (in_model(g, i, k, &q)) {
+            const long mn = (long)g.nx * g.nz;
+            Eh[q] = E0; Eh[mn + q] = E1; Eh[2 * mn + q] = E2;
+        }
+    }
+    const float h = g.att[n] * g.dtr * 0.5f;
+    const float a = 1.0f / (1.0f + h), b = 1.0f - h;
+    const float M = m.M[n], U = m.U[n], La = M - 2.0f * U;
+    float sp = 0.f, sxx = 0.f, szz = 0.f, sxz = 0.f;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const float bl = b - m.decay[l][n], kP = m.kP[l][n], kS = m.kS[l][n];
+        const float rp = a * (bl * f.r[l][0][n] + kP * (E0 + E1));
+        const float rxx = a * (bl * f.r[l][1][n] - 2.0f * kS * E1);
+        const float rzz = a * (bl * f.r[l][2][n] - 2.0f * kS * E0);
+        const float rxz = a * (bl * f.r[l][3][n] + kS * E2);
+        f.r[l][0][n] = rp; f.r[l][1][n] = rxx; f.r[l][2][n] = rzz; f.r[l][3][n] = rxz;
+        sp += rp; sxx += rxx; szz += rzz; sxz += rxz;
+    }
+    float xx = f.xx[n], zz = f.zz[n];
+    if (n == src) { xx += srcval; zz += srcval; }
+    f.xx[n] = a * (b * xx + M * E0 + La * E1 + g.dtr * (sp + sxx));
+    f.zz[n] = a * (b * zz + La * E0 + M * E1 + g.dtr * (sp + szz));
+    f.tp[n] = a * (b * f.tp[n] + M * (E0 + E1) + g.dtr * sp);
+    f.xz[n] = a * (b * f.xz[n] + U * E2 + g.dtr * sxz);
+}
+
+// ---------------------------------------------------------------------------
+// Velocity + vector P (the 2D counterpart of fwd_vx/vz and fwd_vxp_vzp),
+// MODE 1: excitation (caculate_ex_amp_time_new, elastic_2D_kernel_3.cu:30-48),
+// MODE 2: imaging (imaging_correlation_ex_2D / imaging_inner_product_ex_2D,
+// :542-580).  With SAVE, the stress forces F = (fx, fz) of the model region
+// go to Fh ([2][nx*nz]).
+// ---------------------------------------------------------------------------
+template <int R, int TXR, int MODE, bool SAVE>
+__global__ __launch_bounds__(TZ* TXR) void k2_velocity(F2 f, M2 m, G2 g, Coef2 cf, Ex2 ex, int it,
+                                                       float* __restrict__ Fh) {
+    __shared__ float txx[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float tzz[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float txz[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float ttp[TXR + 2 * R][TZ + 2 * R];
+    const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
+    load_tile<R, TXR>(txx, f.xx, g, i0, k0);
+    load_tile<R, TXR>(tzz, f.zz, g, i0, k0);
+    load_tile<R, TXR>(txz, f.xz, g, i0, k0);
+    load_tile<R, TXR>(ttp, f.tp, g, i0, k0);
+    __syncthreads();
+    const int k = k0 + threadIdx.x, i = i0 + threadIdx.y;
+    if (i < R || i >= g.nnx - R || k < R || k >= g.nnz - R) return;
+    float c[R + 1];
+#pragma unroll
+    for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
+    const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
+    const float fx = g.cx * dmx<R>(txx, c, a_, b_) + g.cz * dmz<R>(txz, c, a_, b_);
+    const float fz = g.cx * dpx<R>(txz, c, a_, b_) + g.cz * dpz<R>(tzz, c, a_, b_);
+    const float px = g.cx * dmx<R>(ttp, c, a_, b_), pz = g.cz * dpz<R>(ttp, c, a_, b_);
+    const long n = (long)i * g.pz + k;
+    if constexpr (SAVE) {
+        long q;
+        if (in_model(g, i, k, &q)) {
+            const long mn = (long)g.nx * g.nz;
+            Fh[q] = fx; Fh[mn + q] = fz;
+        }
+    }
+    const float h = g.att[n] * g.dtr * 0.5f;
+    const float a = 1.0f / (1.0f + h), b = 1.0f - h, bo = m.B[n];
+    const float nvx = a * (b * f.vx[n] + bo * fx);
+    const float nvz = a * (b * f.vz[n] + bo * fz);
+    const float nxp = a * (b * f.vxp[n] + bo * px);
+    const float nzp = a * (b * f.vzp[n] + bo * pz);
+    f.vx[n] = nvx; f.vz[n] = nvz; f.vxp[n] = nxp; f.vzp[n] = nzp;
+    if constexpr (MODE == 1) {
+        const float amp = sqrtf(nxp * nxp + nzp * nzp);
+        if (ex.amp[n] < amp) {
+            ex.time[n] = it; ex.amp[n] = amp; ex.tp[n] = ttp[a_][b_];
+            ex.vxp[n] = nxp; ex.vzp[n] = nzp;
+        }
+    } else if constexpr (MODE == 2) {
+        if (ex.time[n] == it) {
+            const float etp = ex.tp[n];
+            ex.img_tp[n] = etp != 0.f ? ttp[a_][b_] / etp : 0.f;
+            const float e1 = ex.vxp[n], e2 = ex.vzp[n], den = e1 * e1 + e2 * e2;
+            ex.img_pp[n] = den > 0.f ? (e1 * nxp + e2 * nzp) / den : 0.f;
+            ex.img_ps[n] = den > 0.f ? (e1 * (nvx - nxp) + e2 * (nvz - nzp)) / den : 0.f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Born secondary sources (cuda_cal_dem_parameter_viscoelastic_media_new with
+// inversion_para = 2, viscoelastic_equation.cu:2398-2505), added after the
+// stress / velocity updates of the Born state (the updates are linear, so a
+// source inside the damping bracket equals a*source added afterwards).
+// dm = [3][nx*nz] relative perturbations (dvp/vp, dvs/vs, drho/rho).
+// ---------------------------------------------------------------------------
+template <int L>
+__global__ void k2_born_stress_src(F2 d, M2 m, G2 g, const float* __restrict__ dm, const float* __restrict__ Eh) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (k >= g.nnz) return;
+    long q;
+    if (!in_model(g, i, k, &q)) return;
+    const long n = (long)i * g.pz + k, mn = (long)g.nx * g.nz;
+    const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h);
+    const float E0 = Eh[q], E1 = Eh[mn + q], E2 = Eh[2 * mn + q];
+    const float sP = 2.0f * dm[q] + dm[2 * mn + q], sS = 2.0f * dm[mn + q] + dm[2 * mn + q];
+    const float dM = m.M[n] * sP, dU = m.U[n] * sS, dL = dM - 2.0f * dU;
+    float sp = 0.f, sxx = 0.f, szz = 0.f, sxz = 0.f;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const float dkP = m.kP[l][n] * sP, dkS = m.kS[l][n] * sS;
+        const float rp = a * dkP * (E0 + E1), rxx = a * (-2.0f * dkS * E1), rzz = a * (-2.0f * dkS * E0),
+                    rxz = a * dkS * E2;
+        d.r[l][0][n] += rp; d.r[l][1][n] += rxx; d.r[l][2][n] += rzz; d.r[l][3][n] += rxz;
+        sp += rp; sxx += rxx; szz += rzz; sxz += rxz;
+    }
+    d.xx[n] += a * (dM * E0 + dL * E1 + g.dtr * (sp + sxx));
+    d.zz[n] += a * (dL * E0 + dM * E1 + g.dtr * (sp + szz));
+    d.xz[n] += a * (dU * E2 + g.dtr * sxz);
+}
+
+__global__ void k2_born_velocity_src(F2 d, M2 m, G2 g, const float* __restrict__ dm, const float* __restrict__ Fh) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (k >= g.nnz) return;
+    long q;
+    if (!in_model(g, i, k, &q)) return;
+    const long n = (long)i * g.pz + k, mn = (long)g.nx * g.nz;
+    const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h);
+    const float s = -a * m.B[n] * dm[2 * mn + q];  // delta(1/rho) = -(drho/rho)/rho
+    d.vx[n] += s * Fh[q];
+    d.vz[n] += s * Fh[mn + q];
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint (exact transpose of one Born step, run backwards in time).
+// Transpose of the velocity update: ls += F^T(Z), Z = a*B*lv on interior
+// cells (0 elsewhere), with (D-)^T = -D+ and (D+)^T = -D-; the scaling
+// lv <- a*b*lv is deferred into k2a_s2.  Density gradient from the stored
+// forces F0 of the step.
+// ---------------------------------------------------------------------------
+template <int R, int TXR>
+__global__ __launch_bounds__(TZ* TXR) void k2a_v(F2 l, M2 m, G2 g, Coef2 cf, const float* __restrict__ Fh,
+                                                 float* __restrict__ grad) {
+    __shared__ float zx[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float zz[TXR + 2 * R][TZ + 2 * R];
+    const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
+    const int tz = threadIdx.x, tx = threadIdx.y;
+    auto Zat = [&](int ii, int kk, float* ox, float* oz) {
+        if (ii < R || ii >= g.nnx - R || kk < R || kk >= g.nnz - R) { *ox = 0.f; *oz = 0.f; return; }
+        const long n = (long)ii * g.pz + kk;
+        const float h = g.att[n] * g.dtr * 0.5f, s = m.B[n] / (1.0f + h);
+        *ox = s * l.vx[n];
+        *oz = s * l.vz[n];
+    };
+    {
+        float a, b;
+        Zat(i0 + tx, k0 + tz, &a, &b);
+        zx[tx + R][tz + R] = a; zz[tx + R][tz + R] = b;
+        if (tz < R) { Zat(i0 + tx, k0 - R + tz, &a, &b); zx[tx + R][tz] = a; zz[tx + R][tz] = b; }
+        if (tz >= TZ - R) { Zat(i0 + tx, k0 + tz + R, &a, &b); zx[tx + R][tz + 2 * R] = a; zz[tx + R][tz + 2 * R] = b; }
+        for (int r = tx; r < 2 * R; r += TXR) {
+            const int ii = r < R ? i0 - R + r : i0 + TXR + (r - R);
+            Zat(ii, k0 + tz, &a, &b);
+            zx[r < R ? r : TXR + r][tz + R] = a; zz[r < R ? r : TXR + r][tz + R] = b;
+        }
+    }
+    __syncthreads();
+    const int k = k0 + tz, i = i0 + tx;
+    if (i < R || i >= g.nnx - R || k < R || k >= g.nnz - R) return;
+    float c[R + 1];
+#pragma unroll
+    for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
+    const int a_ = tx + R, b_ = tz + R;
+    const long n = (long)i * g.pz + k;
+    l.xx[n] += -g.cx * dpx<R>(zx, c, a_, b_);
+    l.xz[n] += -g.cz * dpz<R>(zx, c, a_, b_) - g.cx * dmx<R>(zz, c, a_, b_);
+    l.zz[n] += -g.cz * dmz<R>(zz, c, a_, b_);
+    long q;
+    if (in_model(g, i, k, &q)) {
+        const long mn = (long)g.nx * g.nz;
+        grad[2 * mn + q] += -(Fh[q] * zx[a_][b_] + Fh[mn + q] * zz[a_][b_]);
+    }
+}
+
+// Transpose of the stress/memory update, pointwise part: mu_s = a ls,
+// mu_r = lr + dt P^T mu_s, W = C mu_s + a K^T mu_r (to scratch, interior
+// cells), ls <- b mu_s, lr <- a (b - dt/tau) mu_r, gradients (mp, ms, mr)
+// from the stored strain increments E0 of the step.
+template <int L>
+__global__ void k2a_s1(F2 l, M2 m, G2 g, const float* __restrict__ Eh, float* __restrict__ grad,
+                       float* __restrict__ W0, float* __restrict__ W1, float* __restrict__ W2, int R) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (k < R || k >= g.nnz - R || i < R || i >= g.nnx - R) return;
+    const long n = (long)i * g.pz + k;
+    const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h), b = 1.0f - h;
+    const float mxx = a * l.xx[n], mzz = a * l.zz[n], mxz = a * l.xz[n];
+    const float M = m.M[n], U = m.U[n], La = M - 2.0f * U;
+    float wx = M * mxx + La * mzz, wz = La * mxx + M * mzz, ws = U * mxz;
+    long q;
+    const bool inm = in_model(g, i, k, &q);
+    const long mn = (long)g.nx * g.nz;
+    float E0 = 0.f, E1 = 0.f, E2 = 0.f;
+    if (inm) { E0 = Eh[q]; E1 = Eh[mn + q]; E2 = Eh[2 * mn + q]; }
+    float gkp = 0.f, gks = 0.f;
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        const float rp = l.r[s][0][n] + g.dtr * (mxx + mzz), rxx = l.r[s][1][n] + g.dtr * mxx;
+        const float rzz = l.r[s][2][n] + g.dtr * mzz, rxz = l.r[s][3][n] + g.dtr * mxz;
+        const float kP = m.kP[s][n], kS = m.kS[s][n], bl = b - m.decay[s][n];
+        wx += a * (kP * rp - 2.0f * kS * rzz);
+        wz += a * (kP * rp - 2.0f * kS * rxx);
+        ws += a * kS * rxz;
+        if (inm) {
+            gkp += kP * (E0 + E1) * a * rp;
+            gks += kS * (-2.0f * E1 * a * rxx - 2.0f * E0 * a * rzz + E2 * a * rxz);
+        }
+        l.r[s][0][n] = a * bl * rp; l.r[s][1][n] = a * bl * rxx;
+        l.r[s][2][n] = a * bl * rzz; l.r[s][3][n] = a * bl * rxz;
+    }
+    W0[n] = wx; W1[n] = wz; W2[n] = ws;
+    if (inm) {
+        const float gM = (E0 + E1) * (mxx + mzz), gU = -2.0f * (E1 * mxx + E0 * mzz) + E2 * mxz;
+        grad[q] += 2.0f * M * gM + 2.0f * gkp;
+        grad[mn + q] += 2.0f * U * gU + 2.0f * gks;
+        grad[2 * mn + q] += M * gM + U * gU + gkp + gks;
+    }
+    l.xx[n] = b * mxx; l.zz[n] = b * mzz; l.xz[n] = b * mxz;
+}
+
+// Transpose of the strain operator: lv <- a b lv (deferred from k2a_v) + E^T W.
+template <int R, int TXR>
+__global__ __launch_bounds__(TZ* TXR) void k2a_s2(F2 l, G2 g, Coef2 cf, const float* __restrict__ W0,
+                                                  const float* __restrict__ W1, const float* __restrict__ W2) {
+    __shared__ float w0[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float w1[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float w2[TXR + 2 * R][TZ + 2 * R];
+    const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
+    load_tile<R, TXR>(w0, W0, g, i0, k0);
+    load_tile<R, TXR>(w1, W1, g, i0, k0);
+    load_tile<R, TXR>(w2, W2, g, i0, k0);
+    __syncthreads();
+    const int k = k0 + threadIdx.x, i = i0 + threadIdx.y;
+    if (i < R || i >= g.nnx - R || k < R || k >= g.nnz - R) return;
+    float c[R + 1];
+#pragma unroll
+    for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
+    const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
+    const long n = (long)i * g.pz + k;
+    const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h), b = 1.0f - h;
+    l.vx[n] = a * b * l.vx[n] - g.cx * dmx<R>(w0, c, a_, b_) - g.cz * dmz<R>(w2, c, a_, b_);
+    l.vz[n] = a * b * l.vz[n] - g.cz * dpz<R>(w1, c, a_, b_) - g.cx * dpx<R>(w2, c, a_, b_);
+}
+
+// div / curl decomposition (decom, elastic_2D_kernel_2.cu:621-666)
+template <int R, int TXR>
+__global__ __launch_bounds__(TZ* TXR) void k2_decom(const float* __restrict__ vx, const float* __restrict__ vz,
+                                                    float* __restrict__ dv, float* __restrict__ cu, G2 g, Coef2 cf,
+                                                    float idx, float idz) {
+    __shared__ float tvx[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float tvz[TXR + 2 * R][TZ + 2 * R];
+    const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
+    load_tile<R, TXR>(tvx, vx, g, i0, k0);
+    load_tile<R, TXR>(tvz, vz, g, i0, k0);
+    __syncthreads();
+    const int k = k0 + threadIdx.x, i = i0 + threadIdx.y;
+    if (i >= g.nnx || k >= g.nnz) return;
+    const long n = (long)i * g.pz + k;
+    if (i < R || i >= g.nnx - R || k < R || k >= g.nnz - R) { dv[n] = 0.f; cu[n] = 0.f; return; }
+    float c[R + 1];
+#pragma unroll
+    for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
+    const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
+    dv[n] = idx * dpx<R>(tvx, c, a_, b_) + idz * dmz<R>(tvz, c, a_, b_);
+    cu[n] = idz * dpz<R>(tvx, c, a_, b_) - idx * dmx<R>(tvz, c, a_, b_);
+}
+
+// receivers: rec [2][lt][nrec]
+__global__ void k2_record(const float* __restrict__ vx, const float* __restrict__ vz, const long* __restrict__ idx,
+                          int nr, float* __restrict__ rec, long comp, long off) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    rec[off + r] = vx[idx[r]];
+    rec[comp + off + r] = vz[idx[r]];
+}
+__global__ void k2_inject(float* __restrict__ vx, float* __restrict__ vz, const long* __restrict__ idx, int nr,
+                          const float* __restrict__ rec, long comp, long off) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nr) return;
+    atomicAdd(&vx[idx[r]], rec[off + r]);
+    atomicAdd(&vz[idx[r]], rec[comp + off + r]);
+}
+__global__ void k2_fill(float* __restrict__ p, long n, float v) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
+}
+__global__ void k2_filli(int* __restrict__ p, long n, int v) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
+}
+
+}  // namespace erm2

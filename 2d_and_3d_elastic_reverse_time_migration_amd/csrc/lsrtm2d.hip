@@ -1,0 +1,657 @@
+// lsrtm2d.hip - 2D elastic / viscoelastic context behind include/erm2d.h.
+//
+// The reference's 2D directory has the kernels but no host driver; this file
+// is the driver: allocation, model setup (cuda_cal_viscoelastic,
+// viscoelastic_equation.cu:22-58), the modeling / RTM time loops, and the
+// LSRTM operators (Born = background and Born states stepped together;
+// adjoint = background pass storing the strain increments and stress forces
+// of the model region for every step, then the exact transpose stepped back).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <string>
+#include <vector>
+
+#include "../../include/erm2d.h"
+#include "common.hpp"
+#include "kernels2d.hpp"
+
+using namespace erm2;
+
+namespace {
+thread_local std::string g_err2;
+const double kC6[7] = {0.0, 1.2213363647, -0.096931457519, 0.017447662353,
+                       -0.0029672895159, 0.00035900539822, -0.00002184781161};
+const double kC4[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0};
+constexpr int TXR = 4;  // rows per workgroup (64 x 4 = 256 threads)
+
+struct Sls {
+    double fM, fU, gP[3], gS[3], taus[3];
+};
+// cuda_cal_viscoelastic (viscoelastic_equation.cu:43-56) for one mechanism;
+// tau-method (Blanch et al. 1995) for three: tau_sig_l = 1/(2 pi f_l),
+// f_l = freq {1/4, 1, 4}, tau_eps_l = tau_sig_l (1 + tau), Q(2 pi freq) = Q.
+Sls sls_params(int L, double freq, double qp, double qs) {
+    Sls s{};
+    const double w = 2 * 3.1415926 * freq;
+    if (L == 0) { s.fM = s.fU = 1.0; return s; }
+    if (L == 1) {
+        double tao = 1.0 / w * (std::sqrt(1.0 + 1.0 / qp / qp) - 1.0 / qp);
+        double sp = 1.0 / (w * w * tao);
+        double ss = (1 + w * tao * qs) / (w * qs - w * w * tao);
+        s.taus[0] = tao;
+        s.fM = sp / tao;
+        s.fU = ss / tao;
+        s.gP[0] = (1.0 - sp / tao) / tao;
+        s.gS[0] = (1.0 - ss / tao) / tao;
+        return s;
+    }
+    const double f3[3] = {freq / 4, freq, freq * 4};
+    double S1 = 0, S2 = 0;
+    for (int l = 0; l < 3; ++l) {
+        s.taus[l] = 1.0 / (2 * M_PI * f3[l]);
+        double x = w * s.taus[l];
+        S1 += x / (1 + x * x);
+        S2 += x * x / (1 + x * x);
+    }
+    double tP = 1.0 / (qp * S1 - S2), tS = 1.0 / (qs * S1 - S2);
+    s.fM = 1 + 3 * tP;
+    s.fU = 1 + 3 * tS;
+    for (int l = 0; l < 3; ++l) { s.gP[l] = -tP / s.taus[l]; s.gS[l] = -tS / s.taus[l]; }
+    return s;
+}
+}  // namespace
+
+struct erm2d_ctx {
+    erm2d_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int R = 4, L = 0, nnx = 0, nnz = 0, pz = 0;
+    size_t ncell = 0, mn = 0;
+    Coef2 coef{};
+    float cx = 0, cz = 0, dtr = 0;
+    float* fA[8] = {nullptr};
+    float* rA[3][4] = {{nullptr}};
+    float* fB[8] = {nullptr};
+    float* rB[3][4] = {{nullptr}};
+    float* W[3] = {nullptr};
+    float* dv = nullptr;
+    float* cu = nullptr;
+    struct Mod { float *M = nullptr, *U = nullptr, *B = nullptr, *kP[3] = {0}, *kS[3] = {0}, *decay[3] = {0}; bool have = false; } mod[2];
+    float* att = nullptr;
+    std::vector<float> wavelet;
+    long src = -1;
+    long* rec_idx = nullptr;
+    float* rec = nullptr;   // [2][lt][nrec]
+    float* rec2 = nullptr;  // second record buffer (residual / Born data)
+    int* ex_time = nullptr;
+    float* ex[4] = {nullptr};  // amp tp vxp vzp
+    float* img[3] = {nullptr};
+    float* hist = nullptr;  // [lt][5][mn]: E (3) then F (2)
+    float* dm = nullptr;    // [3][mn]
+    float* grad = nullptr;  // [3][mn]
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double last_ms = 0;
+    std::string err;
+    int fail(int code, const char* fmt, ...) {
+        va_list ap;
+        va_start(ap, fmt);
+        err = erm::vformat(fmt, ap);
+        va_end(ap);
+        return code;
+    }
+};
+
+#define HC(c, call)                                                                                      \
+    do {                                                                                                 \
+        hipError_t e_ = (call);                                                                          \
+        if (e_ != hipSuccess) return (c)->fail(ERM_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+namespace {
+
+template <typename T>
+int dmal(erm2d_ctx* c, T** p, size_t n) {
+    if (n == 0) { *p = nullptr; return 0; }
+    if (hipMalloc((void**)p, n * sizeof(T)) != hipSuccess) return c->fail(ERM_E_NOMEM, "hipMalloc(%zu) failed", n * sizeof(T));
+    return 0;
+}
+int fillf(erm2d_ctx* c, float* p, size_t n, float v) {
+    if (!p) return 0;
+    hipLaunchKernelGGL(k2_fill, dim3(512), dim3(256), 0, c->stream, p, (long)n, v);
+    HC(c, hipGetLastError());
+    return 0;
+}
+
+G2 geo(const erm2d_ctx* c) {
+    G2 g;
+    g.nnx = c->nnx; g.nnz = c->nnz; g.pz = c->pz;
+    g.cx = c->cx; g.cz = c->cz; g.dtr = c->dtr;
+    g.att = c->att;
+    g.bl = c->p.bl; g.bu = c->p.bu; g.nx = c->p.nx; g.nz = c->p.nz;
+    return g;
+}
+F2 state(erm2d_ctx* c, bool B) {
+    F2 f;
+    float** a = B ? c->fB : c->fA;
+    f.vx = a[0]; f.vz = a[1]; f.xx = a[2]; f.zz = a[3]; f.xz = a[4]; f.tp = a[5]; f.vxp = a[6]; f.vzp = a[7];
+    for (int l = 0; l < 3; ++l) for (int q = 0; q < 4; ++q) f.r[l][q] = B ? c->rB[l][q] : c->rA[l][q];
+    return f;
+}
+M2 model(const erm2d_ctx* c, int which) {
+    M2 m;
+    const auto& d = c->mod[which];
+    m.M = d.M; m.U = d.U; m.B = d.B;
+    for (int l = 0; l < 3; ++l) { m.kP[l] = d.kP[l]; m.kS[l] = d.kS[l]; m.decay[l] = d.decay[l]; }
+    return m;
+}
+Ex2 excite(const erm2d_ctx* c) {
+    Ex2 e;
+    e.time = c->ex_time; e.amp = c->ex[0]; e.tp = c->ex[1]; e.vxp = c->ex[2]; e.vzp = c->ex[3];
+    e.img_tp = c->img[0]; e.img_pp = c->img[1]; e.img_ps = c->img[2];
+    return e;
+}
+dim3 grid2(const erm2d_ctx* c) { return dim3((c->nnz + TZ - 1) / TZ, (c->nnx + TXR - 1) / TXR); }
+dim3 grid_pt(const erm2d_ctx* c) { return dim3((c->nnz + 255) / 256, c->nnx); }
+
+template <int R, int L, bool SAVE>
+void stress_r(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
+    hipLaunchKernelGGL((k2_stress<R, L, TXR, SAVE>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, B), model(c, which),
+                       geo(c), c->coef, src, sv, Eh);
+}
+template <int R, bool SAVE>
+void stress_l(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
+    if (c->L == 0) stress_r<R, 0, SAVE>(c, B, which, src, sv, Eh);
+    else if (c->L == 1) stress_r<R, 1, SAVE>(c, B, which, src, sv, Eh);
+    else stress_r<R, 3, SAVE>(c, B, which, src, sv, Eh);
+}
+void do_stress(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
+    if (c->R == 4) { if (Eh) stress_l<4, true>(c, B, which, src, sv, Eh); else stress_l<4, false>(c, B, which, src, sv, Eh); }
+    else { if (Eh) stress_l<6, true>(c, B, which, src, sv, Eh); else stress_l<6, false>(c, B, which, src, sv, Eh); }
+}
+template <int R, int MODE, bool SAVE>
+void vel_r(erm2d_ctx* c, bool B, int which, int it, float* Fh) {
+    hipLaunchKernelGGL((k2_velocity<R, TXR, MODE, SAVE>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, B),
+                       model(c, which), geo(c), c->coef, excite(c), it, Fh);
+}
+template <int R>
+void vel_m(erm2d_ctx* c, bool B, int which, int mode, int it, float* Fh) {
+    if (Fh) vel_r<R, 0, true>(c, B, which, it, Fh);
+    else if (mode == 1) vel_r<R, 1, false>(c, B, which, it, nullptr);
+    else if (mode == 2) vel_r<R, 2, false>(c, B, which, it, nullptr);
+    else vel_r<R, 0, false>(c, B, which, it, nullptr);
+}
+void do_velocity(erm2d_ctx* c, bool B, int which, int mode, int it, float* Fh) {
+    if (c->R == 4) vel_m<4>(c, B, which, mode, it, Fh);
+    else vel_m<6>(c, B, which, mode, it, Fh);
+}
+void born_stress_src(erm2d_ctx* c, const float* Eh) {
+    dim3 g = grid_pt(c);
+    if (c->L == 0) hipLaunchKernelGGL((k2_born_stress_src<0>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh);
+    else if (c->L == 1) hipLaunchKernelGGL((k2_born_stress_src<1>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh);
+    else hipLaunchKernelGGL((k2_born_stress_src<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh);
+}
+void adj_v(erm2d_ctx* c, const float* Fh) {
+    if (c->R == 4) hipLaunchKernelGGL((k2a_v<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, c->grad);
+    else hipLaunchKernelGGL((k2a_v<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, c->grad);
+}
+void adj_s1(erm2d_ctx* c, const float* Eh) {
+    dim3 g = grid_pt(c);
+    if (c->L == 0) hipLaunchKernelGGL((k2a_s1<0>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, c->grad, c->W[0], c->W[1], c->W[2], c->R);
+    else if (c->L == 1) hipLaunchKernelGGL((k2a_s1<1>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, c->grad, c->W[0], c->W[1], c->W[2], c->R);
+    else hipLaunchKernelGGL((k2a_s1<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, c->grad, c->W[0], c->W[1], c->W[2], c->R);
+}
+void adj_s2(erm2d_ctx* c) {
+    if (c->R == 4) hipLaunchKernelGGL((k2a_s2<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, c->W[0], c->W[1], c->W[2]);
+    else hipLaunchKernelGGL((k2a_s2<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, c->W[0], c->W[1], c->W[2]);
+}
+
+int zero_state(erm2d_ctx* c, bool B) {
+    float** f = B ? c->fB : c->fA;
+    for (int q = 0; q < 8; ++q) if (fillf(c, f[q], c->ncell, 0.f)) return ERM_E_DEVICE;
+    for (int l = 0; l < c->L; ++l)
+        for (int q = 0; q < 4; ++q) if (fillf(c, B ? c->rB[l][q] : c->rA[l][q], c->ncell, 0.f)) return ERM_E_DEVICE;
+    return 0;
+}
+int zero_excitation(erm2d_ctx* c) {
+    hipLaunchKernelGGL(k2_filli, dim3(512), dim3(256), 0, c->stream, c->ex_time, (long)c->ncell, -1);
+    for (auto* e : c->ex) if (fillf(c, e, c->ncell, 0.f)) return ERM_E_DEVICE;
+    return 0;
+}
+
+float wavelet_at(const erm2d_ctx* c, int it) { return (it >= 0 && it < (int)c->wavelet.size()) ? c->wavelet[it] : 0.f; }
+
+int run(erm2d_ctx* c, int pass, int it_begin, int nsteps, float* recbuf) {
+    const int which = pass == ERM2D_PASS_MODEL ? 0 : 1;
+    if (!c->mod[which].have) return c->fail(ERM_E_STATE, "%s model not loaded", which ? "background" : "true");
+    const long comp = (long)c->p.lt * c->p.nrec;
+    const int rg = (c->p.nrec + 255) / 256;
+    for (int s = 0; s < nsteps; ++s) {
+        if (pass == ERM2D_PASS_RECEIVER) {
+            const int it = it_begin - s;
+            if (it < 0 || it >= c->p.lt) return c->fail(ERM_E_PARAM, "time index out of range");
+            if (c->p.nrec > 0)
+                hipLaunchKernelGGL(k2_inject, dim3(rg), dim3(256), 0, c->stream, c->fA[0], c->fA[1], c->rec_idx,
+                                   c->p.nrec, recbuf, comp, (long)it * c->p.nrec);
+            do_stress(c, false, which, -1, 0.f, nullptr);
+            do_velocity(c, false, which, 2, it, nullptr);
+        } else {
+            const int it = it_begin + s;
+            do_stress(c, false, which, c->src, wavelet_at(c, it), nullptr);
+            do_velocity(c, false, which, pass, it, nullptr);
+            if (pass == ERM2D_PASS_MODEL && it < c->p.lt && c->p.nrec > 0)
+                hipLaunchKernelGGL(k2_record, dim3(rg), dim3(256), 0, c->stream, c->fA[0], c->fA[1], c->rec_idx,
+                                   c->p.nrec, recbuf, comp, (long)it * c->p.nrec);
+        }
+    }
+    HC(c, hipGetLastError());
+    HC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ensure_lsrtm(erm2d_ctx* c) {
+    if (!c->hist && dmal(c, &c->hist, (size_t)c->p.lt * 5 * c->mn)) return ERM_E_NOMEM;
+    if (!c->dm && dmal(c, &c->dm, 3 * c->mn)) return ERM_E_NOMEM;
+    if (!c->grad && dmal(c, &c->grad, 3 * c->mn)) return ERM_E_NOMEM;
+    return 0;
+}
+
+void free2(erm2d_ctx* c) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    for (int q = 0; q < 8; ++q) { F(c->fA[q]); F(c->fB[q]); }
+    for (int l = 0; l < 3; ++l) for (int q = 0; q < 4; ++q) { F(c->rA[l][q]); F(c->rB[l][q]); }
+    for (auto& w : c->W) F(w);
+    F(c->dv); F(c->cu);
+    for (auto& m : c->mod) { F(m.M); F(m.U); F(m.B); for (int l = 0; l < 3; ++l) { F(m.kP[l]); F(m.kS[l]); F(m.decay[l]); } }
+    F(c->att); F(c->rec_idx); F(c->rec); F(c->rec2); F(c->ex_time);
+    for (auto& e : c->ex) F(e);
+    for (auto& e : c->img) F(e);
+    F(c->hist); F(c->dm); F(c->grad);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+int upload(erm2d_ctx* c, float* d, const std::vector<float>& h) {
+    HC(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+// dense [nnx][nnz] host <-> pitched device
+int up2(erm2d_ctx* c, float* d, const float* h) {
+    HC(c, hipMemcpy2DAsync(d, (size_t)c->pz * 4, h, (size_t)c->nnz * 4, (size_t)c->nnz * 4, c->nnx,
+                           hipMemcpyHostToDevice, c->stream));
+    HC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+int down2(erm2d_ctx* c, float* h, const float* d) {
+    HC(c, hipMemcpy2DAsync(h, (size_t)c->nnz * 4, d, (size_t)c->pz * 4, (size_t)c->nnz * 4, c->nnx,
+                           hipMemcpyDeviceToHost, c->stream));
+    HC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* erm2d_last_error(const erm2d_ctx* c) { return c ? c->err.c_str() : g_err2.c_str(); }
+size_t erm2d_params_size(void) { return sizeof(erm2d_params); }
+
+int erm2d_params_default(erm2d_params* p) {
+    if (!p) return ERM_E_PARAM;
+    *p = erm2d_params{};
+    p->fd_order = 8;
+    p->lt_forward = -1;
+    p->rec_dx = 1;
+    return 0;
+}
+
+int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
+    if (!pin || !out) { g_err2 = "null argument"; return ERM_E_PARAM; }
+    *out = nullptr;
+    const erm2d_params& p = *pin;
+    int R = p.fd_order == 8 ? 4 : p.fd_order == 12 ? 6 : -1;
+    if (R < 0) { g_err2 = "fd_order must be 8 or 12"; return ERM_E_PARAM; }
+    if (p.nsls != 0 && p.nsls != 1 && p.nsls != 3) { g_err2 = "nsls must be 0, 1 or 3"; return ERM_E_PARAM; }
+    if (p.nx <= 0 || p.nz <= 0 || p.lt <= 0 || p.dt <= 0 || p.dx <= 0 || p.dz <= 0) { g_err2 = "bad grid"; return ERM_E_PARAM; }
+    if (p.bu < 0 || p.bd < 0 || p.bl < 0 || p.br < 0) { g_err2 = "negative boundary width"; return ERM_E_PARAM; }
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) { g_err2 = "no HIP device"; return ERM_E_DEVICE; }
+    if (device < 0 || device >= nd) { g_err2 = "device out of range"; return ERM_E_PARAM; }
+    erm2d_ctx* c = new erm2d_ctx();
+    c->p = p;
+    if (c->p.lt_forward < 0) c->p.lt_forward = p.lt;
+    c->device = device;
+    c->R = R;
+    c->L = p.nsls;
+    c->nnx = p.nx + p.bl + p.br;
+    c->nnz = p.nz + p.bu + p.bd;
+    if (c->nnx < 2 * R + 1 || c->nnz < 2 * R + 1) { delete c; g_err2 = "padded grid smaller than the stencil"; return ERM_E_SIZE; }
+    c->pz = (c->nnz + 31) / 32 * 32;
+    c->ncell = (size_t)c->nnx * c->pz;
+    c->mn = (size_t)p.nx * p.nz;
+    for (int m = 0; m <= R; ++m) c->coef.c[m] = (float)(R == 6 ? kC6[m] : kC4[m]);
+    c->dtr = (float)(p.dt / 1000.0);
+    c->cx = (float)(p.dt / 1000.0 / p.dx);
+    c->cz = (float)(p.dt / 1000.0 / p.dz);
+#define CK2(x) do { int r_ = (x); if (r_) { g_err2 = c->err; free2(c); delete c; return r_; } } while (0)
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_err2 = "stream create failed"; delete c; return ERM_E_DEVICE;
+    }
+    for (auto& e : c->ev) if (hipEventCreate(&e) != hipSuccess) { g_err2 = "event create failed"; free2(c); delete c; return ERM_E_DEVICE; }
+    for (int q = 0; q < 8; ++q) { CK2(dmal(c, &c->fA[q], c->ncell)); CK2(dmal(c, &c->fB[q], c->ncell)); }
+    for (int l = 0; l < c->L; ++l)
+        for (int q = 0; q < 4; ++q) { CK2(dmal(c, &c->rA[l][q], c->ncell)); CK2(dmal(c, &c->rB[l][q], c->ncell)); }
+    for (auto& w : c->W) CK2(dmal(c, &w, c->ncell));
+    CK2(dmal(c, &c->dv, c->ncell));
+    CK2(dmal(c, &c->cu, c->ncell));
+    CK2(dmal(c, &c->att, c->ncell));
+    CK2(dmal(c, &c->ex_time, c->ncell));
+    for (auto& e : c->ex) CK2(dmal(c, &e, c->ncell));
+    for (auto& e : c->img) CK2(dmal(c, &e, c->ncell));
+    // damping: make_attenuation_new (zzzzz:238-325), coe * s^3
+    {
+        std::vector<double> ax(c->nnx, 0), az(c->nnz, 0);
+        auto face = [&](int b, bool rising, std::vector<double>& axis, int start) {
+            for (int t = 0; t < b; ++t) {
+                double s = rising ? t / (b - 1.0) : (b - 1.0 - t) / (b - 1.0);
+                axis[start + t] += p.coe_attenuation * std::pow(s, 3.0);
+            }
+        };
+        if (p.bu > 1) face(p.bu, false, az, 0);
+        if (p.bd > 1) face(p.bd, true, az, p.bu + p.nz);
+        if (p.bl > 1) face(p.bl, false, ax, 0);
+        if (p.br > 1) face(p.br, true, ax, p.bl + p.nx);
+        std::vector<float> h(c->ncell, 0.f);
+        for (int i = 0; i < c->nnx; ++i)
+            for (int k = 0; k < c->nnz; ++k) h[(size_t)i * c->pz + k] = (float)(ax[i] + az[k]);
+        CK2(upload(c, c->att, h));
+    }
+    // make_ricker_new (zzzzz:136-157)
+    {
+        const double PI = 3.141592653589793;
+        const float dts = (float)(p.dt / 1000.0);
+        float B = (float)(std::sqrt(6.0) / (PI * p.freq));
+        int N = (int)std::ceil(1.35 * (double)B / (double)dts);
+        c->wavelet.assign(2 * N + 1, 0.f);
+        c->wavelet[N] = 1.f;
+        for (int i = 1; i <= N; ++i) {
+            float t = dts * (float)i, u = (float)(2.0 * std::sqrt(6.0)) * t / B;
+            c->wavelet[N + i] = c->wavelet[N - i] = 0.5f * (2.0f - u * u) * std::exp(-u * u / 4.0f);
+        }
+    }
+    if (p.nrec > 0) {
+        std::vector<long> idx(p.nrec);
+        for (int q = 0; q < p.nrec; ++q) {
+            int i = p.bl + p.rec_x0 + q * p.rec_dx, k = p.bu + p.rec_z;
+            if (i < 0 || i >= c->nnx || k < 0 || k >= c->nnz) { g_err2 = "receiver outside the grid"; free2(c); delete c; return ERM_E_PARAM; }
+            idx[q] = (long)i * c->pz + k;
+        }
+        CK2(dmal(c, &c->rec_idx, idx.size()));
+        if (hipMemcpy(c->rec_idx, idx.data(), idx.size() * sizeof(long), hipMemcpyHostToDevice) != hipSuccess) {
+            g_err2 = "receiver upload failed"; free2(c); delete c; return ERM_E_DEVICE;
+        }
+        CK2(dmal(c, &c->rec, (size_t)2 * p.lt * p.nrec));
+        CK2(dmal(c, &c->rec2, (size_t)2 * p.lt * p.nrec));
+    }
+    CK2(zero_state(c, false));
+    CK2(zero_state(c, true));
+    for (auto& w : c->W) CK2(fillf(c, w, c->ncell, 0.f));
+    CK2(zero_excitation(c));
+    for (auto& e : c->img) CK2(fillf(c, e, c->ncell, 0.f));
+    CK2(erm2d_set_shot(c, p.shot_x, p.shot_z));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { g_err2 = "init failed"; free2(c); delete c; return ERM_E_DEVICE; }
+#undef CK2
+    *out = c;
+    return 0;
+}
+
+int erm2d_destroy(erm2d_ctx* c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free2(c);
+    delete c;
+    return 0;
+}
+
+int erm2d_get_dims(const erm2d_ctx* c, int* nnx, int* nnz) {
+    if (!c) return ERM_E_PARAM;
+    if (nnx) *nnx = c->nnx;
+    if (nnz) *nnz = c->nnz;
+    return 0;
+}
+
+int erm2d_load_model(erm2d_ctx* c, int which, const float* vp, const float* vs, const float* rho, const float* qp,
+                     const float* qs) {
+    if (!c || which < 0 || which > 1 || !vp || !vs || !rho) return c ? c->fail(ERM_E_PARAM, "bad arguments") : ERM_E_PARAM;
+    if (c->L > 0 && (!qp || !qs)) return c->fail(ERM_E_PARAM, "qp/qs required when nsls > 0");
+    HC(c, hipSetDevice(c->device));
+    const erm2d_params& p = c->p;
+    std::vector<float> hM(c->ncell, 0.f), hU(c->ncell, 0.f), hB(c->ncell, 0.f);
+    std::vector<float> hkP[3], hkS[3], hd[3];
+    for (int l = 0; l < c->L; ++l) { hkP[l].assign(c->ncell, 0.f); hkS[l].assign(c->ncell, 0.f); hd[l].assign(c->ncell, 0.f); }
+    for (int i = 0; i < c->nnx; ++i)
+        for (int k = 0; k < c->nnz; ++k) {
+            int ix = std::min(std::max(i - p.bl, 0), p.nx - 1), iz = std::min(std::max(k - p.bu, 0), p.nz - 1);
+            size_t s = (size_t)ix * p.nz + iz, n = (size_t)i * c->pz + k;
+            double a = vp[s], b = vs[s], r = rho[s];
+            if (!(a > 0) || b < 0 || !(r > 0)) return c->fail(ERM_E_PARAM, "non-physical model value at (%d,%d)", ix, iz);
+            Sls q = sls_params(c->L, p.freq, c->L ? qp[s] : 1.0, c->L ? qs[s] : 1.0);
+            if (c->L && (!(qp[s] > 0) || !(qs[s] > 0))) return c->fail(ERM_E_PARAM, "Q must be positive");
+            hM[n] = (float)(r * a * a * q.fM);
+            hU[n] = (float)(r * b * b * q.fU);
+            hB[n] = (float)(1.0 / r);
+            for (int l = 0; l < c->L; ++l) {
+                hkP[l][n] = (float)(r * a * a * q.gP[l]);
+                hkS[l][n] = (float)(r * b * b * q.gS[l]);
+                hd[l][n] = (float)(p.dt / 1000.0 / q.taus[l]);
+            }
+        }
+    auto& m = c->mod[which];
+    if (!m.M) {
+        if (dmal(c, &m.M, c->ncell) || dmal(c, &m.U, c->ncell) || dmal(c, &m.B, c->ncell)) return ERM_E_NOMEM;
+        for (int l = 0; l < c->L; ++l)
+            if (dmal(c, &m.kP[l], c->ncell) || dmal(c, &m.kS[l], c->ncell) || dmal(c, &m.decay[l], c->ncell)) return ERM_E_NOMEM;
+    }
+    if (upload(c, m.M, hM) || upload(c, m.U, hU) || upload(c, m.B, hB)) return ERM_E_DEVICE;
+    for (int l = 0; l < c->L; ++l)
+        if (upload(c, m.kP[l], hkP[l]) || upload(c, m.kS[l], hkS[l]) || upload(c, m.decay[l], hd[l])) return ERM_E_DEVICE;
+    m.have = true;
+    return 0;
+}
+
+int erm2d_set_shot(erm2d_ctx* c, int sx, int sz) {
+    if (!c) return ERM_E_PARAM;
+    int i = c->p.bl + sx, k = c->p.bu + sz;
+    if (i < c->R || i >= c->nnx - c->R || k < c->R || k >= c->nnz - c->R)
+        return c->fail(ERM_E_PARAM, "source (%d,%d) is not an updated cell", i, k);
+    c->p.shot_x = sx;
+    c->p.shot_z = sz;
+    c->src = (long)i * c->pz + k;
+    return 0;
+}
+
+int erm2d_zero(erm2d_ctx* c) {
+    if (!c) return ERM_E_PARAM;
+    if (zero_state(c, false)) return ERM_E_DEVICE;
+    HC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int erm2d_step(erm2d_ctx* c, int pass, int it_begin, int nsteps) {
+    if (!c) return ERM_E_PARAM;
+    HC(c, hipSetDevice(c->device));
+    if (pass < 0 || pass > 2) return c->fail(ERM_E_PARAM, "bad pass");
+    return run(c, pass, it_begin, nsteps, c->rec);
+}
+
+int erm2d_get_field(erm2d_ctx* c, int field, float* dst) {
+    if (!c || !dst) return ERM_E_PARAM;
+    HC(c, hipSetDevice(c->device));
+    if (field >= 0 && field < 8) return down2(c, dst, c->fA[field]);
+    if (field == ERM2D_DIV || field == ERM2D_CURL) {
+        G2 g = geo(c);
+        if (c->R == 4)
+            hipLaunchKernelGGL((k2_decom<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, c->fA[0], c->fA[1], c->dv,
+                               c->cu, g, c->coef, (float)(1.0 / c->p.dx), (float)(1.0 / c->p.dz));
+        else
+            hipLaunchKernelGGL((k2_decom<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, c->fA[0], c->fA[1], c->dv,
+                               c->cu, g, c->coef, (float)(1.0 / c->p.dx), (float)(1.0 / c->p.dz));
+        HC(c, hipGetLastError());
+        return down2(c, dst, field == ERM2D_DIV ? c->dv : c->cu);
+    }
+    if (field == ERM2D_EX_TIME) {
+        std::vector<int> t(c->ncell);
+        HC(c, hipMemcpyAsync(t.data(), c->ex_time, c->ncell * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HC(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < c->nnx; ++i)
+            for (int k = 0; k < c->nnz; ++k) dst[(size_t)i * c->nnz + k] = (float)t[(size_t)i * c->pz + k];
+        return 0;
+    }
+    if (field == ERM2D_EX_AMP) return down2(c, dst, c->ex[0]);
+    if (field >= ERM2D_IMG_TP && field <= ERM2D_IMG_PS) return down2(c, dst, c->img[field - ERM2D_IMG_TP]);
+    return c->fail(ERM_E_PARAM, "unknown field %d", field);
+}
+
+int erm2d_set_field(erm2d_ctx* c, int field, const float* src) {
+    if (!c || !src) return ERM_E_PARAM;
+    if (field < 0 || field >= 8) return c->fail(ERM_E_PARAM, "field %d is not writable", field);
+    return up2(c, c->fA[field], src);
+}
+
+int erm2d_forward(erm2d_ctx* c, float* rec) {
+    if (!c) return ERM_E_PARAM;
+    HC(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = zero_state(c, false))) return rc;
+    if ((rc = run(c, ERM2D_PASS_MODEL, 0, c->p.lt, c->rec))) return rc;
+    if (rec && c->p.nrec > 0) {
+        HC(c, hipMemcpyAsync(rec, c->rec, (size_t)2 * c->p.lt * c->p.nrec * 4, hipMemcpyDeviceToHost, c->stream));
+        HC(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+static int cut_model(erm2d_ctx* c, const float* d, float* out) {
+    std::vector<float> full((size_t)c->nnx * c->nnz);
+    int rc = down2(c, full.data(), d);
+    if (rc) return rc;
+    for (int ix = 0; ix < c->p.nx; ++ix)
+        for (int iz = 0; iz < c->p.nz; ++iz)
+            out[(size_t)ix * c->p.nz + iz] = full[(size_t)(ix + c->p.bl) * c->nnz + iz + c->p.bu];
+    return 0;
+}
+
+int erm2d_rtm(erm2d_ctx* c, const float* rec, float* tp, float* pp, float* ps) {
+    if (!c) return ERM_E_PARAM;
+    HC(c, hipSetDevice(c->device));
+    if (rec && c->p.nrec > 0)
+        HC(c, hipMemcpyAsync(c->rec, rec, (size_t)2 * c->p.lt * c->p.nrec * 4, hipMemcpyHostToDevice, c->stream));
+    HC(c, hipEventRecord(c->ev[0], c->stream));
+    int rc;
+    if ((rc = zero_state(c, false)) || (rc = zero_excitation(c))) return rc;
+    if ((rc = run(c, ERM2D_PASS_SOURCE, 0, c->p.lt_forward, c->rec))) return rc;
+    if ((rc = zero_state(c, false))) return rc;
+    for (auto& e : c->img) if (fillf(c, e, c->ncell, 0.f)) return ERM_E_DEVICE;
+    if ((rc = run(c, ERM2D_PASS_RECEIVER, c->p.lt - 1, c->p.lt, c->rec))) return rc;
+    HC(c, hipEventRecord(c->ev[1], c->stream));
+    HC(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0;
+    HC(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->last_ms = ms;
+    float* outs[3] = {tp, pp, ps};
+    for (int a = 0; a < 3; ++a)
+        if (outs[a] && (rc = cut_model(c, c->img[a], outs[a]))) return rc;
+    return 0;
+}
+
+int erm2d_born(erm2d_ctx* c, const float* dm, float* d) {
+    if (!c || !dm) return ERM_E_PARAM;
+    HC(c, hipSetDevice(c->device));
+    if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
+    int rc;
+    if ((rc = ensure_lsrtm(c))) return rc;
+    HC(c, hipMemcpyAsync(c->dm, dm, 3 * c->mn * 4, hipMemcpyHostToDevice, c->stream));
+    HC(c, hipEventRecord(c->ev[0], c->stream));
+    if ((rc = zero_state(c, false)) || (rc = zero_state(c, true))) return rc;
+    const long comp = (long)c->p.lt * c->p.nrec;
+    const int rg = (c->p.nrec + 255) / 256;
+    float* Eh = c->hist;          // one step of E and F suffices here
+    float* Fh = c->hist + 3 * c->mn;
+    for (int it = 0; it < c->p.lt; ++it) {
+        do_stress(c, false, 1, c->src, wavelet_at(c, it), Eh);
+        do_stress(c, true, 1, -1, 0.f, nullptr);
+        born_stress_src(c, Eh);
+        do_velocity(c, false, 1, 0, it, Fh);
+        do_velocity(c, true, 1, 0, it, nullptr);
+        hipLaunchKernelGGL(k2_born_velocity_src, grid_pt(c), dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c),
+                           c->dm, Fh);
+        if (c->p.nrec > 0)
+            hipLaunchKernelGGL(k2_record, dim3(rg), dim3(256), 0, c->stream, c->fB[0], c->fB[1], c->rec_idx, c->p.nrec,
+                               c->rec2, comp, (long)it * c->p.nrec);
+    }
+    HC(c, hipGetLastError());
+    HC(c, hipEventRecord(c->ev[1], c->stream));
+    HC(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0;
+    HC(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->last_ms = ms;
+    if (d && c->p.nrec > 0) {
+        HC(c, hipMemcpyAsync(d, c->rec2, (size_t)2 * c->p.lt * c->p.nrec * 4, hipMemcpyDeviceToHost, c->stream));
+        HC(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int erm2d_adjoint(erm2d_ctx* c, const float* r, float* gout) {
+    if (!c || !r) return ERM_E_PARAM;
+    HC(c, hipSetDevice(c->device));
+    if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
+    int rc;
+    if ((rc = ensure_lsrtm(c))) return rc;
+    const long comp = (long)c->p.lt * c->p.nrec;
+    const int rg = (c->p.nrec + 255) / 256;
+    if (c->p.nrec > 0)
+        HC(c, hipMemcpyAsync(c->rec2, r, (size_t)2 * c->p.lt * c->p.nrec * 4, hipMemcpyHostToDevice, c->stream));
+    HC(c, hipEventRecord(c->ev[0], c->stream));
+    if ((rc = zero_state(c, false)) || (rc = zero_state(c, true))) return rc;
+    if ((rc = fillf(c, c->grad, 3 * c->mn, 0.f))) return rc;
+    for (auto& w : c->W) if ((rc = fillf(c, w, c->ncell, 0.f))) return rc;
+    const size_t step = 5 * c->mn;
+    for (int it = 0; it < c->p.lt; ++it) {  // background, storing E and F per step
+        float* h = c->hist + (size_t)it * step;
+        do_stress(c, false, 1, c->src, wavelet_at(c, it), h);
+        do_velocity(c, false, 1, 0, it, h + 3 * c->mn);
+    }
+    for (int it = c->p.lt - 1; it >= 0; --it) {
+        const float* h = c->hist + (size_t)it * step;
+        if (c->p.nrec > 0)
+            hipLaunchKernelGGL(k2_inject, dim3(rg), dim3(256), 0, c->stream, c->fB[0], c->fB[1], c->rec_idx, c->p.nrec,
+                               c->rec2, comp, (long)it * c->p.nrec);
+        adj_v(c, h + 3 * c->mn);
+        adj_s1(c, h);
+        adj_s2(c);
+    }
+    HC(c, hipGetLastError());
+    HC(c, hipEventRecord(c->ev[1], c->stream));
+    HC(c, hipEventSynchronize(c->ev[1]));
+    float ms = 0;
+    HC(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->last_ms = ms;
+    if (gout) {
+        HC(c, hipMemcpyAsync(gout, c->grad, 3 * c->mn * 4, hipMemcpyDeviceToHost, c->stream));
+        HC(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int erm2d_last_ms(const erm2d_ctx* c, double* ms) {
+    if (!c || !ms) return ERM_E_PARAM;
+    *ms = c->last_ms;
+    return 0;
+}
+
+}  // extern "C"

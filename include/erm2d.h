@@ -1,0 +1,97 @@
+/*
+ * erm2d.h - C ABI of the 2D elastic / viscoelastic part of liberm.so:
+ * 2D propagation with standard-linear-solid attenuation, P/S decomposition,
+ * excitation-amplitude RTM, and the Born / adjoint operators of LSRTM with a
+ * host CG driver.
+ *
+ * Reference counterparts (GeophysicsLab/2D_and_3D_elastic_reverse_time_migration,
+ * 2D_elastic_RTM/): the reference has kernels but no host driver here; each
+ * entry point names the kernels it drives.  Conventions as in erm.h (return
+ * codes, erm2d_last_error, caller-owned host arrays, dt in ms).
+ *
+ * Layout: 2D volumes are z-fastest, idx = ix*nnz + iz, as the reference
+ * (viscoelastic_equation.cu:90-92); model/image arrays cover the model region
+ * [nx][nz].  Records are [2][lt][nrec] (vx, vz).
+ */
+#ifndef ERM2D_H
+#define ERM2D_H
+#include <stddef.h>
+#include "erm.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct erm2d_params {
+    int   nx, nz;
+    float dx, dz;
+    float dt;          /* ms */
+    float freq;        /* Ricker peak frequency (Hz); also the SLS reference frequency */
+    int   lt;          /* time steps */
+    int   lt_forward;  /* source-pass steps for RTM (<0: lt) */
+    int   bu, bd, bl, br;
+    float coe_attenuation;  /* damping (cubic profile, zzzzz:238-325) */
+    int   fd_order;    /* 8 (radius 4) or 12 (radius 6, reference coefficients) */
+    int   nsls;        /* 0: elastic; 1: reference SLS (viscoelastic_equation.cu:22-58); 3: three mechanisms */
+    int   shot_x, shot_z;              /* source, model coordinates (cells) */
+    int   nrec, rec_x0, rec_dx, rec_z; /* receiver line, model coordinates */
+    int   reserved[8];
+} erm2d_params;
+
+/* fields for erm2d_get_field / erm2d_set_field (padded grid, z fastest) */
+enum erm2d_field {
+    ERM2D_VX = 0, ERM2D_VZ, ERM2D_TXX, ERM2D_TZZ, ERM2D_TXZ, ERM2D_TP, ERM2D_VXP, ERM2D_VZP,
+    ERM2D_DIV, ERM2D_CURL,              /* decom (elastic_2D_kernel_2.cu:563-669), read only */
+    ERM2D_EX_TIME, ERM2D_EX_AMP, ERM2D_IMG_TP, ERM2D_IMG_PP, ERM2D_IMG_PS,
+    ERM2D_FIELD_COUNT
+};
+/* passes for erm2d_step */
+#define ERM2D_PASS_MODEL    0
+#define ERM2D_PASS_SOURCE   1
+#define ERM2D_PASS_RECEIVER 2
+
+typedef struct erm2d_ctx erm2d_ctx;
+
+const char* erm2d_last_error(const erm2d_ctx* ctx);
+size_t erm2d_params_size(void);
+int   erm2d_params_default(erm2d_params* p);
+
+/* create ≙ the reference's device allocations; which: 0 = true, 1 = background (smoothed) model.
+ * cuda_cal_viscoelastic (viscoelastic_equation.cu:22-58) runs inside erm2d_load_model.
+ * qp/qs may be NULL when nsls == 0. */
+int   erm2d_create(const erm2d_params* p, int device, erm2d_ctx** out);
+int   erm2d_destroy(erm2d_ctx* ctx);
+int   erm2d_load_model(erm2d_ctx* ctx, int which, const float* vp, const float* vs, const float* rho,
+                       const float* qp, const float* qs);
+int   erm2d_set_shot(erm2d_ctx* ctx, int shot_x, int shot_z);
+int   erm2d_zero(erm2d_ctx* ctx);
+
+/* Kernel-level stepping (parity tests) and field access. */
+int   erm2d_step(erm2d_ctx* ctx, int pass, int it_begin, int nsteps);
+int   erm2d_get_field(erm2d_ctx* ctx, int field, float* dst);
+int   erm2d_set_field(erm2d_ctx* ctx, int field, const float* src);
+int   erm2d_get_dims(const erm2d_ctx* ctx, int* nnx, int* nnz);
+
+/* Modeling in the true model (fwd_txxzzxzpp_viscoelastic_and_memory_3parameterization,
+ * viscoelastic_equation.cu:184-325, + the velocity update): rec [2][lt][nrec]. */
+int   erm2d_forward(erm2d_ctx* ctx, float* rec);
+/* Excitation-amplitude RTM in the background model: source pass
+ * (caculate_ex_amp_time_new, elastic_2D_kernel_3.cu:30-48) then receiver pass with
+ * imaging_correlation_ex_2D / imaging_inner_product_ex_2D (:542-580).
+ * Images over the model region [nx][nz]; any pointer may be NULL. */
+int   erm2d_rtm(erm2d_ctx* ctx, const float* rec, float* img_tp, float* img_pp, float* img_ps);
+
+/* LSRTM operators in the background model, model perturbation dm = [3][nx][nz]
+ * (dvp/vp, dvs/vs, drho/rho; the reference's inversion_para = 2,
+ * viscoelastic_equation.cu:2431-2489):
+ *   erm2d_born    d = L dm   (demig_fwd_... :328-477 with cuda_cal_dem_parameter_..._new :2398-2505)
+ *   erm2d_adjoint g = L^T r  (adjoint_fwd_vx/vz/memory :1827-2198, cal_gradient_..._new :2248-2295),
+ * where L^T is the exact transpose of the discrete L (dot-product exact). */
+int   erm2d_born(erm2d_ctx* ctx, const float* dm, float* d);
+int   erm2d_adjoint(erm2d_ctx* ctx, const float* r, float* g);
+/* Timing of the last operator call (HIP events): milliseconds. */
+int   erm2d_last_ms(const erm2d_ctx* ctx, double* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

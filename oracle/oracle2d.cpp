@@ -1,0 +1,605 @@
+/*
+ * oracle2d.cpp - CPU restatement of the reference's 2D elastic/viscoelastic
+ * kernels and of the LSRTM operators built on them.
+ *
+ * TEST INFRASTRUCTURE ONLY (FP64 = oracle, FP32 = CPU baseline); loaded by
+ * tests/ and bench.py's CPU leg, never by liberm.so.
+ *
+ * Follows (2D_elastic_RTM/):
+ *   SLS parameters, L = 1            viscoelastic_equation.cu:22-58 (cuda_cal_viscoelastic)
+ *   stress + memory, 3-param.        viscoelastic_equation.cu:184-325 (tp: :172, :763)
+ *   staggered operators              viscoelastic_equation.cu:137-176, adjoint :1923-1967
+ *   damping (cubic profile)          3D_elastic_RTM/zzzzz:238-325 (make_attenuation_new)
+ *   P/S div/curl decomposition       elastic_2D_kernel_2.cu:563-669 (decom)
+ *   excitation, imaging              elastic_2D_kernel_3.cu:30-48, 542-580
+ *   Born sources (inversion_para 2)  viscoelastic_equation.cu:2398-2505
+ *   adjoint / gradient (structure)   viscoelastic_equation.cu:1827-2295
+ * The velocity update (the reference's fwd_vx/fwd_vz kernels are absent) is
+ * the 2D form of the stress divergence with the positions of :137-176.
+ * L = 3 uses the tau-method (Blanch et al. 1995): tau_eps_l = tau_sig_l (1+tau),
+ * tau_sig_l = 1/(2 pi f_l), f_l = freq * {1/4, 1, 4}, tau fixed by Q(2 pi freq) = Q.
+ * The adjoint is the exact transpose of the discrete Born operator (pinned
+ * by the dot-product test in tests/test_oracle2d.py).
+ */
+#include "../include/erm2d.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+const double kC6[7] = {0.0, 1.2213363647, -0.096931457519, 0.017447662353,
+                       -0.0029672895159, 0.00035900539822, -0.00002184781161};
+const double kC4[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0};
+
+template <typename T>
+std::vector<T> ricker(double freq, double dt_s) {  /* make_ricker_new, zzzzz:136-157 */
+    const double PI = 3.141592653589793;
+    T B = (T)(std::sqrt(6.0) / (PI * freq));
+    int N = (int)std::ceil(1.35 * (double)B / dt_s);
+    std::vector<T> w(2 * N + 1);
+    w[N] = (T)1;
+    for (int i = 1; i <= N; ++i) {
+        T t = (T)dt_s * (T)i, u = (T)(2.0 * std::sqrt(6.0)) * t / B;
+        w[N + i] = w[N - i] = (T)0.5 * ((T)2 - u * u) * std::exp(-u * u / (T)4);
+    }
+    return w;
+}
+
+struct Sls {  /* per cell: unrelaxed-modulus factors and memory coefficients */
+    double fM, fU;             /* M_U = rho vp^2 fM, mu_U = rho vs^2 fU */
+    double gP[3], gS[3];       /* kP_l = rho vp^2 gP_l, kS_l = rho vs^2 gS_l */
+    double taus[3];
+};
+
+/* cuda_cal_viscoelastic (viscoelastic_equation.cu:43-56) for L = 1; tau-method for L = 3 */
+Sls sls_params(int L, double freq, double qp, double qs) {
+    Sls s{};
+    const double w = 2 * 3.1415926 * freq;  /* pai as the reference */
+    if (L == 0) { s.fM = s.fU = 1.0; return s; }
+    if (L == 1) {
+        double tao = 1.0 / w * (std::sqrt(1.0 + 1.0 / qp / qp) - 1.0 / qp);
+        double sp = 1.0 / (w * w * tao);
+        double ss = (1 + w * tao * qs) / (w * qs - w * w * tao);
+        s.taus[0] = tao;
+        s.fM = sp / tao;
+        s.fU = ss / tao;
+        s.gP[0] = (1.0 - sp / tao) / tao;
+        s.gS[0] = (1.0 - ss / tao) / tao;
+        return s;
+    }
+    const double f3[3] = {freq / 4, freq, freq * 4};
+    double S1 = 0, S2 = 0;
+    for (int l = 0; l < 3; ++l) {
+        s.taus[l] = 1.0 / (2 * M_PI * f3[l]);
+        double x = w * s.taus[l];
+        S1 += x / (1 + x * x);
+        S2 += x * x / (1 + x * x);
+    }
+    double tP = 1.0 / (qp * S1 - S2), tS = 1.0 / (qs * S1 - S2);
+    s.fM = 1 + 3 * tP;
+    s.fU = 1 + 3 * tS;
+    for (int l = 0; l < 3; ++l) {
+        s.gP[l] = -tP / s.taus[l];
+        s.gS[l] = -tS / s.taus[l];
+    }
+    return s;
+}
+
+template <typename T>
+struct Oracle2D {
+    erm2d_params p;
+    int R = 4, L = 0, nnx = 0, nnz = 0;
+    size_t N = 0;
+    T c[7] = {0};
+    T cx = 0, cz = 0, dtr = 0;
+    enum { VX, VZ, XX, ZZ, XZ, TP, VXP, VZP, NF };
+    std::vector<T> f[NF];
+    std::vector<T> r[3][4];  /* memory: p, xx, zz, xz per mechanism */
+    struct Model {
+        std::vector<T> M, U, B, kP[3], kS[3], decay[3];  /* decay_l = dt/tau_sig_l */
+        bool have = false;
+    } mod[2];
+    std::vector<T> att;
+    std::vector<T> wavelet;
+    long src = 0;
+    std::vector<long> recs;
+    std::vector<int> ex_time;
+    std::vector<T> ex_amp, ex_tp, ex_vxp, ex_vzp, img_tp, img_pp, img_ps;
+    std::vector<T> rec;  /* [2][lt][nrec] */
+
+    size_t id(int i, int k) const { return (size_t)i * nnz + k; }
+    bool interior(int i, int k) const { return i >= R && i < nnx - R && k >= R && k < nnz - R; }
+
+    int init(const erm2d_params* pp) {
+        p = *pp;
+        R = p.fd_order == 12 ? 6 : (p.fd_order == 8 ? 4 : -1);
+        if (R < 0) return -1;
+        L = p.nsls;
+        if (L != 0 && L != 1 && L != 3) return -1;
+        nnx = p.nx + p.bl + p.br;
+        nnz = p.nz + p.bu + p.bd;
+        if (nnx < 2 * R + 1 || nnz < 2 * R + 1) return -5;
+        N = (size_t)nnx * nnz;
+        for (int m = 0; m <= R; ++m) c[m] = (T)(R == 6 ? kC6[m] : kC4[m]);
+        dtr = (T)(p.dt / 1000.0);
+        cx = (T)(p.dt / 1000.0 / p.dx);
+        cz = (T)(p.dt / 1000.0 / p.dz);
+        for (auto& v : f) v.assign(N, 0);
+        for (int l = 0; l < L; ++l) for (auto& v : r[l]) v.assign(N, 0);
+        /* make_attenuation_new (zzzzz:238-325): coe * s^3, s = depth into the layer / (b-1) */
+        std::vector<double> ax(nnx, 0), az(nnz, 0);
+        auto face = [&](int b, bool rising, std::vector<double>& axis, int start) {
+            for (int t = 0; t < b; ++t) {
+                double s = rising ? t / (b - 1.0) : (b - 1.0 - t) / (b - 1.0);
+                axis[start + t] += p.coe_attenuation * std::pow(s, 3.0);
+            }
+        };
+        if (p.bu > 1) face(p.bu, false, az, 0);
+        if (p.bd > 1) face(p.bd, true, az, p.bu + p.nz);
+        if (p.bl > 1) face(p.bl, false, ax, 0);
+        if (p.br > 1) face(p.br, true, ax, p.bl + p.nx);
+        att.assign(N, 0);
+        for (int i = 0; i < nnx; ++i)
+            for (int k = 0; k < nnz; ++k) att[id(i, k)] = (T)(ax[i] + az[k]);
+        wavelet = ricker<T>(p.freq, p.dt / 1000.0);
+        recs.clear();
+        for (int q = 0; q < p.nrec; ++q) {
+            int i = p.bl + p.rec_x0 + q * p.rec_dx, k = p.bu + p.rec_z;
+            if (i < 0 || i >= nnx || k < 0 || k >= nnz) return -1;
+            recs.push_back((long)id(i, k));
+        }
+        rec.assign((size_t)2 * p.lt * p.nrec, 0);
+        ex_time.assign(N, -1);
+        for (auto* v : {&ex_amp, &ex_tp, &ex_vxp, &ex_vzp, &img_tp, &img_pp, &img_ps}) v->assign(N, 0);
+        return set_shot(p.shot_x, p.shot_z);
+    }
+
+    int set_shot(int sx, int sz) {
+        int i = p.bl + sx, k = p.bu + sz;
+        if (!interior(i, k)) return -1;
+        src = (long)id(i, k);
+        return 0;
+    }
+
+    int load_model(int which, const float* vp, const float* vs, const float* rho, const float* qp, const float* qs) {
+        Model& m = mod[which];
+        m.M.assign(N, 0); m.U.assign(N, 0); m.B.assign(N, 0);
+        for (int l = 0; l < L; ++l) { m.kP[l].assign(N, 0); m.kS[l].assign(N, 0); m.decay[l].assign(N, 0); }
+        for (int i = 0; i < nnx; ++i)
+            for (int k = 0; k < nnz; ++k) {
+                /* edge-replicated padding (read_velocity_new, zzzzz:327-357) */
+                int ix = std::min(std::max(i - p.bl, 0), p.nx - 1), iz = std::min(std::max(k - p.bu, 0), p.nz - 1);
+                size_t s = (size_t)ix * p.nz + iz, n = id(i, k);
+                double a = vp[s], b = vs[s], rr = rho[s];
+                if (!(a > 0) || b < 0 || !(rr > 0)) return -1;
+                Sls q = sls_params(L, p.freq, L ? qp[s] : 1.0, L ? qs[s] : 1.0);
+                m.M[n] = (T)(rr * a * a * q.fM);
+                m.U[n] = (T)(rr * b * b * q.fU);
+                m.B[n] = (T)(1.0 / rr);
+                for (int l = 0; l < L; ++l) {
+                    m.kP[l][n] = (T)(rr * a * a * q.gP[l]);
+                    m.kS[l][n] = (T)(rr * b * b * q.gS[l]);
+                    m.decay[l][n] = (T)(p.dt / 1000.0 / q.taus[l]);
+                }
+            }
+        m.have = true;
+        return 0;
+    }
+
+    /* staggered operators on a z-fastest grid: D+ centred at +1/2, D- at -1/2 */
+    inline T dp(const T* q, ptrdiff_t s) const { T a = 0; for (int m = 1; m <= R; ++m) a += c[m] * (q[m * s] - q[-(m - 1) * s]); return a; }
+    inline T dm(const T* q, ptrdiff_t s) const { T a = 0; for (int m = 1; m <= R; ++m) a += c[m] * (q[(m - 1) * s] - q[-m * s]); return a; }
+    T ab(size_t n, T* b) const { T h = att[n] * dtr / (T)2; *b = (T)1 - h; return (T)1 / ((T)1 + h); }
+
+    /* strain increments E = (Vx, Vz, S) of a velocity field at cell n */
+    void strain(const T* vx, const T* vz, size_t n, T* E) const {
+        const ptrdiff_t sX = nnz;
+        E[0] = cx * dp(vx + n, sX);
+        E[1] = cz * dm(vz + n, 1);
+        E[2] = cx * dm(vz + n, sX) + cz * dp(vx + n, 1);
+    }
+    /* stress forces F = (fx, fz) at cell n */
+    void force(const T* xx, const T* zz, const T* xz, size_t n, T* F) const {
+        const ptrdiff_t sX = nnz;
+        F[0] = cx * dm(xx + n, sX) + cz * dm(xz + n, 1);
+        F[1] = cx * dp(xz + n, sX) + cz * dp(zz + n, 1);
+    }
+
+    /* stress + memory update of state (fv, rv) with model m (viscoelastic_equation.cu:294-323) */
+    void stress(std::vector<T>* fv, std::vector<T> (*rv)[4], const Model& m, T* Ehist = nullptr) {
+#pragma omp parallel for schedule(static)
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) {
+                size_t n = id(i, k);
+                T E[3];
+                strain(fv[VX].data(), fv[VZ].data(), n, E);
+                if (Ehist && i >= p.bl && i < p.bl + p.nx && k >= p.bu && k < p.bu + p.nz) {
+                    size_t q = (size_t)(i - p.bl) * p.nz + (k - p.bu), mn = (size_t)p.nx * p.nz;
+                    Ehist[q] = E[0]; Ehist[mn + q] = E[1]; Ehist[2 * mn + q] = E[2];
+                }
+                T b, a = ab(n, &b);
+                T M = m.M[n], U = m.U[n], La = M - 2 * U;
+                T sp = 0, sxx = 0, szz = 0, sxz = 0;
+                for (int l = 0; l < L; ++l) {
+                    T bl = b - m.decay[l][n], kP = m.kP[l][n], kS = m.kS[l][n];
+                    T rp = a * (bl * rv[l][0][n] + kP * (E[0] + E[1]));
+                    T rxx = a * (bl * rv[l][1][n] - 2 * kS * E[1]);
+                    T rzz = a * (bl * rv[l][2][n] - 2 * kS * E[0]);
+                    T rxz = a * (bl * rv[l][3][n] + kS * E[2]);
+                    rv[l][0][n] = rp; rv[l][1][n] = rxx; rv[l][2][n] = rzz; rv[l][3][n] = rxz;
+                    sp += rp; sxx += rxx; szz += rzz; sxz += rxz;
+                }
+                fv[XX][n] = a * (b * fv[XX][n] + M * E[0] + La * E[1] + dtr * (sp + sxx));
+                fv[ZZ][n] = a * (b * fv[ZZ][n] + La * E[0] + M * E[1] + dtr * (sp + szz));
+                fv[TP][n] = a * (b * fv[TP][n] + M * (E[0] + E[1]) + dtr * sp);
+                fv[XZ][n] = a * (b * fv[XZ][n] + U * E[2] + dtr * sxz);
+            }
+    }
+
+    void velocity(std::vector<T>* fv, const Model& m, int mode, int it, T* Fhist = nullptr) {
+        const ptrdiff_t sX = nnz;
+#pragma omp parallel for schedule(static)
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) {
+                size_t n = id(i, k);
+                T F[2];
+                force(fv[XX].data(), fv[ZZ].data(), fv[XZ].data(), n, F);
+                if (Fhist && i >= p.bl && i < p.bl + p.nx && k >= p.bu && k < p.bu + p.nz) {
+                    size_t q = (size_t)(i - p.bl) * p.nz + (k - p.bu), mn = (size_t)p.nx * p.nz;
+                    Fhist[q] = F[0]; Fhist[mn + q] = F[1];
+                }
+                T b, a = ab(n, &b), bo = m.B[n];
+                T px = cx * dm(fv[TP].data() + n, sX), pz = cz * dp(fv[TP].data() + n, 1);
+                T nvx = a * (b * fv[VX][n] + bo * F[0]);
+                T nvz = a * (b * fv[VZ][n] + bo * F[1]);
+                T nxp = a * (b * fv[VXP][n] + bo * px);
+                T nzp = a * (b * fv[VZP][n] + bo * pz);
+                fv[VX][n] = nvx; fv[VZ][n] = nvz; fv[VXP][n] = nxp; fv[VZP][n] = nzp;
+                if (mode == 1) {  /* caculate_ex_amp_time_new: |vP| = sqrt(vxp^2 + vzp^2) */
+                    T amp = std::sqrt(nxp * nxp + nzp * nzp);
+                    if (ex_amp[n] < amp) {
+                        ex_time[n] = it; ex_amp[n] = amp; ex_tp[n] = fv[TP][n];
+                        ex_vxp[n] = nxp; ex_vzp[n] = nzp;
+                    }
+                } else if (mode == 2 && ex_time[n] == it) {
+                    T etp = ex_tp[n];
+                    img_tp[n] = etp != (T)0 ? fv[TP][n] / etp : (T)0;
+                    T ex = ex_vxp[n], ez = ex_vzp[n], den = ex * ex + ez * ez;
+                    img_pp[n] = den > 0 ? (ex * nxp + ez * nzp) / den : (T)0;
+                    img_ps[n] = den > 0 ? (ex * (nvx - nxp) + ez * (nvz - nzp)) / den : (T)0;
+                }
+            }
+    }
+
+    void add_source(std::vector<T>* fv, int it) {
+        if (it >= 0 && it < (int)wavelet.size()) { fv[XX][src] += wavelet[it]; fv[ZZ][src] += wavelet[it]; }
+    }
+
+    int step(int pass, int it_begin, int nsteps) {
+        const Model& m = mod[pass == 0 ? 0 : 1];
+        if (!m.have) return -4;
+        for (int s = 0; s < nsteps; ++s) {
+            if (pass == 2) {
+                int it = it_begin - s;
+                if (it < 0 || it >= p.lt) return -1;
+                for (int q = 0; q < p.nrec; ++q) {
+                    f[VX][recs[q]] += rec[((size_t)0 * p.lt + it) * p.nrec + q];
+                    f[VZ][recs[q]] += rec[((size_t)1 * p.lt + it) * p.nrec + q];
+                }
+                stress(f, r, m);
+                velocity(f, m, 2, it);
+            } else {
+                int it = it_begin + s;
+                add_source(f, it);
+                stress(f, r, m);
+                velocity(f, m, pass, it);
+                if (pass == 0 && it < p.lt)
+                    for (int q = 0; q < p.nrec; ++q) {
+                        rec[((size_t)0 * p.lt + it) * p.nrec + q] = f[VX][recs[q]];
+                        rec[((size_t)1 * p.lt + it) * p.nrec + q] = f[VZ][recs[q]];
+                    }
+            }
+        }
+        return 0;
+    }
+
+    void zero() {
+        for (auto& v : f) std::fill(v.begin(), v.end(), (T)0);
+        for (int l = 0; l < L; ++l) for (auto& v : r[l]) std::fill(v.begin(), v.end(), (T)0);
+    }
+
+    /* decom (elastic_2D_kernel_2.cu:621-666): divergence at the stress node,
+     * curl at the sxz node; interior cells. */
+    void decom(std::vector<T>& dv, std::vector<T>& cu) const {
+        dv.assign(N, 0); cu.assign(N, 0);
+        const ptrdiff_t sX = nnz;
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) {
+                size_t n = id(i, k);
+                dv[n] = (T)(1.0 / p.dx) * dp(f[VX].data() + n, sX) + (T)(1.0 / p.dz) * dm(f[VZ].data() + n, 1);
+                cu[n] = (T)(1.0 / p.dz) * dp(f[VX].data() + n, 1) - (T)(1.0 / p.dx) * dm(f[VZ].data() + n, sX);
+            }
+    }
+
+    /* ------------------------------------------------------------ LSRTM */
+    struct Pert { std::vector<T> mp, ms, mr; };
+    Pert unpack(const float* dm) const {
+        Pert P;
+        size_t mn = (size_t)p.nx * p.nz;
+        P.mp.assign(dm, dm + mn); P.ms.assign(dm + mn, dm + 2 * mn); P.mr.assign(dm + 2 * mn, dm + 3 * mn);
+        return P;
+    }
+    bool model_cell(int i, int k, size_t* q) const {
+        if (i < p.bl || i >= p.bl + p.nx || k < p.bu || k >= p.bu + p.nz) return false;
+        *q = (size_t)(i - p.bl) * p.nz + (k - p.bu);
+        return true;
+    }
+
+    /* d = L dm: background and Born wavefields stepped together. */
+    int born(const float* dmv, float* d) {
+        const Model& m = mod[1];
+        if (!m.have) return -4;
+        Pert P = unpack(dmv);
+        size_t mn = (size_t)p.nx * p.nz;
+        std::vector<T> Eh(3 * mn), Fh(2 * mn);
+        std::vector<T> g[NF];
+        std::vector<T> gr[3][4];
+        for (auto& v : g) v.assign(N, 0);
+        for (int l = 0; l < L; ++l) for (auto& v : gr[l]) v.assign(N, 0);
+        zero();
+        for (int it = 0; it < p.lt; ++it) {
+            add_source(f, it);
+            stress(f, r, m, Eh.data());
+            stress(g, gr, m);
+            /* Born stress/memory sources: cuda_cal_dem_parameter_viscoelastic_media_new (:2398-2505) */
+#pragma omp parallel for schedule(static)
+            for (int i = R; i < nnx - R; ++i)
+                for (int k = R; k < nnz - R; ++k) {
+                    size_t q;
+                    if (!model_cell(i, k, &q)) continue;
+                    size_t n = id(i, k);
+                    T b, a = ab(n, &b);
+                    T E0 = Eh[q], E1 = Eh[mn + q], E2 = Eh[2 * mn + q];
+                    T sP = 2 * P.mp[q] + P.mr[q], sS = 2 * P.ms[q] + P.mr[q];
+                    T dM = m.M[n] * sP, dU = m.U[n] * sS, dL = dM - 2 * dU;
+                    T sp = 0, sxx = 0, szz = 0, sxz = 0;
+                    for (int l = 0; l < L; ++l) {
+                        T dkP = m.kP[l][n] * sP, dkS = m.kS[l][n] * sS;
+                        T rp = a * dkP * (E0 + E1), rxx = a * (-2 * dkS * E1), rzz = a * (-2 * dkS * E0), rxz = a * dkS * E2;
+                        gr[l][0][n] += rp; gr[l][1][n] += rxx; gr[l][2][n] += rzz; gr[l][3][n] += rxz;
+                        sp += rp; sxx += rxx; szz += rzz; sxz += rxz;
+                    }
+                    g[XX][n] += a * (dM * E0 + dL * E1 + dtr * (sp + sxx));
+                    g[ZZ][n] += a * (dL * E0 + dM * E1 + dtr * (sp + szz));
+                    g[XZ][n] += a * (dU * E2 + dtr * sxz);
+                }
+            velocity(f, m, 0, it, Fh.data());
+            velocity(g, m, 0, it);
+#pragma omp parallel for schedule(static)
+            for (int i = R; i < nnx - R; ++i)
+                for (int k = R; k < nnz - R; ++k) {
+                    size_t q;
+                    if (!model_cell(i, k, &q)) continue;
+                    size_t n = id(i, k);
+                    T b, a = ab(n, &b);
+                    T s = -a * m.B[n] * P.mr[q];  /* delta(1/rho) = -mr/rho: -drho dv/dt (:2488-2489) */
+                    g[VX][n] += s * Fh[q];
+                    g[VZ][n] += s * Fh[mn + q];
+                }
+            for (int qq = 0; qq < p.nrec; ++qq) {
+                d[((size_t)0 * p.lt + it) * p.nrec + qq] = (float)g[VX][recs[qq]];
+                d[((size_t)1 * p.lt + it) * p.nrec + qq] = (float)g[VZ][recs[qq]];
+            }
+        }
+        return 0;
+    }
+
+    /* g = L^T r: exact transpose of born(), stepping back from lt-1. */
+    int adjoint(const float* rr, float* gout) {
+        const Model& m = mod[1];
+        if (!m.have) return -4;
+        size_t mn = (size_t)p.nx * p.nz;
+        /* background history: E(v0^n) and F(s0^{n+1}) over the model region */
+        std::vector<T> EH((size_t)p.lt * 3 * mn), FH((size_t)p.lt * 2 * mn);
+        zero();
+        for (int it = 0; it < p.lt; ++it) {
+            add_source(f, it);
+            stress(f, r, m, EH.data() + (size_t)it * 3 * mn);
+            velocity(f, m, 0, it, FH.data() + (size_t)it * 2 * mn);
+        }
+        std::vector<T> lv[2], ls[3], lr[3][4], W[3];
+        for (auto& v : lv) v.assign(N, 0);
+        for (auto& v : ls) v.assign(N, 0);
+        for (auto& v : W) v.assign(N, 0);
+        for (int l = 0; l < L; ++l) for (auto& v : lr[l]) v.assign(N, 0);
+        std::vector<T> gp(mn, 0), gs(mn, 0), gr_(mn, 0);
+        std::vector<T> Z[2];
+        for (auto& v : Z) v.assign(N, 0);
+        const ptrdiff_t sX = nnz;
+        for (int it = p.lt - 1; it >= 0; --it) {
+            const T* Eh = EH.data() + (size_t)it * 3 * mn;
+            const T* Fh = FH.data() + (size_t)it * 2 * mn;
+            for (int q = 0; q < p.nrec; ++q) {  /* record transpose */
+                lv[0][recs[q]] += (T)rr[((size_t)0 * p.lt + it) * p.nrec + q];
+                lv[1][recs[q]] += (T)rr[((size_t)1 * p.lt + it) * p.nrec + q];
+            }
+            /* V^T: ls += F^T(a B lv); lv <- a b lv; density gradient */
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < nnx; ++i)
+                for (int k = 0; k < nnz; ++k) {
+                    size_t n = id(i, k);
+                    if (!interior(i, k)) { Z[0][n] = Z[1][n] = 0; continue; }
+                    T b, a = ab(n, &b);
+                    Z[0][n] = a * m.B[n] * lv[0][n];
+                    Z[1][n] = a * m.B[n] * lv[1][n];
+                    size_t q;
+                    if (model_cell(i, k, &q)) gr_[q] += -(Fh[q] * Z[0][n] + Fh[mn + q] * Z[1][n]);
+                }
+#pragma omp parallel for schedule(static)
+            for (int i = R; i < nnx - R; ++i)
+                for (int k = R; k < nnz - R; ++k) {
+                    size_t n = id(i, k);
+                    ls[0][n] += -cx * dp(Z[0].data() + n, sX);                          /* (D-x)^T = -D+x */
+                    ls[2][n] += -cz * dp(Z[0].data() + n, 1) - cx * dm(Z[1].data() + n, sX);
+                    ls[1][n] += -cz * dm(Z[1].data() + n, 1);                           /* (D+z)^T = -D-z */
+                }
+#pragma omp parallel for schedule(static)
+            for (int i = R; i < nnx - R; ++i)
+                for (int k = R; k < nnz - R; ++k) {
+                    size_t n = id(i, k);
+                    T b, a = ab(n, &b);
+                    lv[0][n] *= a * b;
+                    lv[1][n] *= a * b;
+                }
+            /* S^T: mu_s = a ls, mu_r = lr + dt P^T mu_s; W = C mu_s + a K^T mu_r; gradients */
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < nnx; ++i)
+                for (int k = 0; k < nnz; ++k) {
+                    size_t n = id(i, k);
+                    if (!interior(i, k)) { W[0][n] = W[1][n] = W[2][n] = 0; continue; }
+                    T b, a = ab(n, &b);
+                    T mxx = a * ls[0][n], mzz = a * ls[1][n], mxz = a * ls[2][n];
+                    T M = m.M[n], U = m.U[n], La = M - 2 * U;
+                    T wx = M * mxx + La * mzz, wz = La * mxx + M * mzz, ws = U * mxz;
+                    size_t q;
+                    bool inm = model_cell(i, k, &q);
+                    T E0 = 0, E1 = 0, E2 = 0;
+                    if (inm) { E0 = Eh[q]; E1 = Eh[mn + q]; E2 = Eh[2 * mn + q]; }
+                    T gkp = 0, gks = 0;
+                    for (int l = 0; l < L; ++l) {
+                        T rp = lr[l][0][n] + dtr * (mxx + mzz), rxx = lr[l][1][n] + dtr * mxx;
+                        T rzz = lr[l][2][n] + dtr * mzz, rxz = lr[l][3][n] + dtr * mxz;
+                        T kP = m.kP[l][n], kS = m.kS[l][n], bl = b - m.decay[l][n];
+                        wx += a * (kP * rp - 2 * kS * rzz);
+                        wz += a * (kP * rp - 2 * kS * rxx);
+                        ws += a * kS * rxz;
+                        if (inm) {
+                            gkp += kP * (E0 + E1) * a * rp;
+                            gks += kS * (-2 * E1 * a * rxx - 2 * E0 * a * rzz + E2 * a * rxz);
+                        }
+                        lr[l][0][n] = a * bl * rp; lr[l][1][n] = a * bl * rxx;
+                        lr[l][2][n] = a * bl * rzz; lr[l][3][n] = a * bl * rxz;
+                    }
+                    W[0][n] = wx; W[1][n] = wz; W[2][n] = ws;
+                    if (inm) {
+                        T gM = (E0 + E1) * (mxx + mzz), gU = -2 * (E1 * mxx + E0 * mzz) + E2 * mxz;
+                        gp[q] += 2 * M * gM + 2 * gkp;
+                        gs[q] += 2 * U * gU + 2 * gks;
+                        gr_[q] += M * gM + U * gU + gkp + gks;
+                    }
+                    ls[0][n] = b * mxx; ls[1][n] = b * mzz; ls[2][n] = b * mxz;
+                }
+#pragma omp parallel for schedule(static)
+            for (int i = R; i < nnx - R; ++i)
+                for (int k = R; k < nnz - R; ++k) {
+                    size_t n = id(i, k);
+                    lv[0][n] += -cx * dm(W[0].data() + n, sX) - cz * dm(W[2].data() + n, 1);
+                    lv[1][n] += -cz * dp(W[1].data() + n, 1) - cx * dp(W[2].data() + n, sX);
+                }
+        }
+        for (size_t q = 0; q < mn; ++q) {
+            gout[q] = (float)gp[q]; gout[mn + q] = (float)gs[q]; gout[2 * mn + q] = (float)gr_[q];
+        }
+        return 0;
+    }
+};
+
+template <typename T>
+struct Box { Oracle2D<T> o; };
+
+struct H2 { int prec; Oracle2D<double>* d = nullptr; Oracle2D<float>* s = nullptr; };
+
+template <typename Fn>
+int disp(void* h, Fn&& fn) {
+    if (!h) return -1;
+    H2* H = (H2*)h;
+    return H->prec == 64 ? fn(*H->d) : fn(*H->s);
+}
+
+}  // namespace
+
+extern "C" {
+
+void* oracle2d_create(const erm2d_params* p, int prec) {
+    H2* H = new H2();
+    H->prec = prec == 32 ? 32 : 64;
+    int rc;
+    if (H->prec == 64) { H->d = new Oracle2D<double>(); rc = H->d->init(p); }
+    else { H->s = new Oracle2D<float>(); rc = H->s->init(p); }
+    if (rc) { delete H->d; delete H->s; delete H; return nullptr; }
+    return H;
+}
+void oracle2d_destroy(void* h) {
+    if (!h) return;
+    H2* H = (H2*)h;
+    delete H->d; delete H->s; delete H;
+}
+int oracle2d_load_model(void* h, int which, const float* vp, const float* vs, const float* rho, const float* qp,
+                        const float* qs) {
+    return disp(h, [&](auto& o) { return o.load_model(which, vp, vs, rho, qp, qs); });
+}
+int oracle2d_set_shot(void* h, int sx, int sz) { return disp(h, [&](auto& o) { return o.set_shot(sx, sz); }); }
+int oracle2d_zero(void* h, int what) {
+    return disp(h, [&](auto& o) {
+        if (what & 1) o.zero();
+        if (what & 2) {
+            std::fill(o.ex_time.begin(), o.ex_time.end(), -1);
+            for (auto* v : {&o.ex_amp, &o.ex_tp, &o.ex_vxp, &o.ex_vzp}) std::fill(v->begin(), v->end(), 0);
+        }
+        if (what & 4) for (auto* v : {&o.img_tp, &o.img_pp, &o.img_ps}) std::fill(v->begin(), v->end(), 0);
+        return 0;
+    });
+}
+int oracle2d_step(void* h, int pass, int it_begin, int nsteps) {
+    return disp(h, [&](auto& o) { return o.step(pass, it_begin, nsteps); });
+}
+int oracle2d_get_field(void* h, int fid, double* dst) {
+    return disp(h, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.c[0])>::type;
+        if (fid >= 0 && fid < 8) { for (size_t n = 0; n < o.N; ++n) dst[n] = o.f[fid][n]; return 0; }
+        if (fid == ERM2D_DIV || fid == ERM2D_CURL) {
+            std::vector<T> dv, cu;
+            o.decom(dv, cu);
+            const auto& s = fid == ERM2D_DIV ? dv : cu;
+            for (size_t n = 0; n < o.N; ++n) dst[n] = s[n];
+            return 0;
+        }
+        if (fid == ERM2D_EX_TIME) { for (size_t n = 0; n < o.N; ++n) dst[n] = o.ex_time[n]; return 0; }
+        const std::vector<T>* s = fid == ERM2D_EX_AMP ? &o.ex_amp : fid == ERM2D_IMG_TP ? &o.img_tp
+                                : fid == ERM2D_IMG_PP ? &o.img_pp : fid == ERM2D_IMG_PS ? &o.img_ps : nullptr;
+        if (!s) return -1;
+        for (size_t n = 0; n < o.N; ++n) dst[n] = (*s)[n];
+        return 0;
+    });
+}
+int oracle2d_set_field(void* h, int fid, const double* src) {
+    return disp(h, [&](auto& o) {
+        if (fid < 0 || fid >= 8) return -1;
+        for (size_t n = 0; n < o.N; ++n) o.f[fid][n] = src[n];
+        return 0;
+    });
+}
+int oracle2d_record(void* h, double* dst, const double* src) {
+    return disp(h, [&](auto& o) {
+        if (src) for (size_t n = 0; n < o.rec.size(); ++n) o.rec[n] = src[n];
+        if (dst) for (size_t n = 0; n < o.rec.size(); ++n) dst[n] = o.rec[n];
+        return 0;
+    });
+}
+int oracle2d_born(void* h, const float* dm, float* d) { return disp(h, [&](auto& o) { return o.born(dm, d); }); }
+int oracle2d_adjoint(void* h, const float* r, float* g) { return disp(h, [&](auto& o) { return o.adjoint(r, g); }); }
+/* SLS parameter check: returns fM, fU, gP[0..2], gS[0..2], taus[0..2] for one cell */
+int oracle2d_sls(int L, double freq, double qp, double qs, double* out) {
+    Sls s = sls_params(L, freq, qp, qs);
+    out[0] = s.fM; out[1] = s.fU;
+    for (int l = 0; l < 3; ++l) { out[2 + l] = s.gP[l]; out[5 + l] = s.gS[l]; out[8 + l] = s.taus[l]; }
+    return 0;
+}
+
+}  // extern "C"

@@ -20,6 +20,12 @@ def erm():
 
 
 @pytest.fixture(scope="session")
+def oracle2d_mod():
+    import oracle2d
+    return oracle2d
+
+
+@pytest.fixture(scope="session")
 def oracle():
     import oracle3d
     oracle3d.set_threads(min(8, os.cpu_count() or 1))

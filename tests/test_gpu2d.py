@@ -1,0 +1,118 @@
+"""GPU parity of the 2D elastic / viscoelastic module (include/erm2d.h)
+against the FP64 2D oracle, through the C ABI.
+
+Tolerances: wavefields and records rel L2 <= 1e-5 (vector P: 1e-4 or 10x
+the FP32 CPU instantiation's error, as in 3D);
+Born data <= 1e-5; adjoint (migrated gradient) <= 1e-4 (it sums products of
+two FP32 wavefields over every step); images <= 1e-4 on cells whose
+excitation time agrees; the GPU dot-product test <= 1e-4 (FP32).
+"""
+import numpy as np
+import pytest
+
+import cases
+import cases2d
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(m):
+    print(m, flush=True)
+
+
+@pytest.mark.parametrize("R", [4, 6])
+def test_gpu2d_closed_forms(erm, R):
+    import test_oracle2d as T
+    _say(f"2D closed forms on GPU R={R}")
+    for comp, axis, o, err in T.closed_form_2d(lambda p: erm.Elastic2D(p), R):
+        assert err < 2e-5, (comp, axis, o, err)
+
+
+@pytest.mark.parametrize("L,R", [(0, 4), (1, 4), (3, 4), (1, 6)])
+def test_gpu2d_wavefields_match_oracle(erm, oracle2d_mod, L, R):
+    p = cases2d.params(nx=90, nz=70, b=12, R=R, L=L, lt=260, freq=20.0)
+    vp, vs, rho, qp, qs = cases2d.model(p, seed=4)
+    _say(f"2D wavefield parity L={L} R={R} padded {p.padded_shape}")
+    g = erm.Elastic2D(p)
+    g.load_model(0, vp, vs, rho, qp, qs)
+    rec = g.forward()
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(0, vp, vs, rho, qp, qs)
+    o.step(0, 0, p.lt)
+    assert cases.rel_l2(rec, o.record()) < 1e-5
+    for k in ["vx", "vz", "txx", "tzz", "txz", "tp"]:
+        e = cases.rel_l2(g.get_field(k), o.get_field(k))
+        assert e < 1e-5, (k, e)
+    o32 = oracle2d_mod.Oracle2D(p, 32)
+    o32.load_model(0, vp, vs, rho, qp, qs)
+    o32.step(0, 0, p.lt)
+    for k in ["vxp", "vzp"]:
+        ref = o.get_field(k)
+        e, e32 = cases.rel_l2(g.get_field(k), ref), cases.rel_l2(o32.get_field(k), ref)
+        assert e < max(1e-4, 10 * e32), (k, e, e32)
+    for k in ["div", "curl"]:
+        e = cases.rel_l2(g.get_field(k), o.get_field(k))
+        assert e < 1e-5, (k, e)
+    g.close()
+
+
+def test_gpu2d_rtm_images_match_oracle(erm, oracle2d_mod):
+    nz, zi = 50, 32
+    p = cases2d.params(nx=60, nz=nz, b=14, R=4, L=0, lt=700, freq=25.0, coe=250.0, src=(30, 3), rec_dx=1, rec_z=2)
+    z = np.arange(nz)[None, :]
+    vp = (np.where(z < zi, 2000.0, 3000.0) * np.ones((p.nx, 1))).astype(np.float32)
+    rho = (np.where(z < zi, 2000.0, 2400.0) * np.ones((p.nx, 1))).astype(np.float32)
+    vs = (vp / 1.732).astype(np.float32)
+    s = np.full_like(vp, 2000.0)
+    bg = (s, (s / 1.732).astype(np.float32), np.full_like(vp, 2000.0))
+    g = erm.Elastic2D(p)
+    g.load_model(0, vp, vs, rho)
+    g.load_model(1, *bg)
+    rec = g.forward()
+    tp, pp, ps = g.rtm()
+    gt = g.get_field("ex_time")
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(0, vp, vs, rho)
+    o.load_model(1, *bg)
+    o.step(0, 0, p.lt)
+    assert cases.rel_l2(rec, o.record()) < 1e-5
+    o.zero(7)
+    o.step(1, 0, p.lt)
+    o.zero(1)
+    o.step(2, p.lt - 1, p.lt)
+    ot = o.get_field("ex_time")
+    oa = o.get_field("ex_amp")
+    mask = (oa > 1e-3 * oa.max()) & (gt == ot)
+    assert 1.0 - (gt == ot)[oa > 1e-3 * oa.max()].mean() < 0.02
+    cut = mask[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz]
+    otp, opp, ops = o.images()
+    for a, b in ((pp, opp), (ps, ops)):
+        assert cases.rel_l2(a[cut], b[cut]) < 1e-4
+    col = np.abs(pp[26:35, 10:]).sum(axis=0)
+    assert abs(int(np.argmax(col)) + 10 - zi) <= 2
+    g.close()
+
+
+@pytest.mark.parametrize("L", [0, 1, 3])
+def test_gpu2d_born_and_adjoint_match_oracle(erm, oracle2d_mod, L):
+    import test_oracle2d as T
+    p, bg = T._lsrtm_case(L, lt=200)
+    rng = np.random.default_rng(5)
+    dm = (rng.standard_normal((3, p.nx, p.nz)) * cases2d.taper_mask(p)).astype(np.float32)
+    r = rng.standard_normal((2, p.lt, p.nrec)).astype(np.float32)
+    _say(f"2D Born/adjoint parity L={L}")
+    g = erm.Elastic2D(p)
+    g.load_model(1, *bg)
+    d = g.born(dm)
+    gr = g.adjoint(r)
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, *bg)
+    assert cases.rel_l2(d, o.born(dm)) < 1e-5
+    og = o.adjoint(r)
+    for c in range(3):
+        e = cases.rel_l2(gr[c], og[c])
+        assert e < 1e-4, (c, e)
+    lhs = float(np.sum(d.astype(np.float64) * r))
+    rhs = float(np.sum(dm.astype(np.float64) * gr))
+    assert abs(lhs - rhs) / abs(lhs) < 1e-4, (lhs, rhs)
+    g.close()
