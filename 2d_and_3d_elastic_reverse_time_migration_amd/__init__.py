@@ -100,6 +100,25 @@ def _load():
         "erm2d_born": (ctypes.c_int, [vp, fp, fp]),
         "erm2d_adjoint": (ctypes.c_int, [vp, fp, fp]),
         "erm2d_last_ms": (ctypes.c_int, [vp, P(ctypes.c_double)]),
+        "erm2d_lsrtm_cg": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ctypes.c_int, ctypes.c_uint, fp, fp,
+                                          P(ctypes.c_double), vp, vp, ctypes.c_float]),
+    })
+    ull = ctypes.c_ulonglong
+    sig.update({
+        "erm_unet_last_error": (ctypes.c_char_p, []),
+        "erm_unet_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ull, P(vp)]),
+        "erm_unet_destroy": (ctypes.c_int, [vp]),
+        "erm_unet_num_params": (ctypes.c_size_t, [vp]),
+        "erm_unet_get_params": (ctypes.c_int, [vp, fp]),
+        "erm_unet_set_params": (ctypes.c_int, [vp, fp]),
+        "erm_unet_forward": (ctypes.c_int, [vp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, fp]),
+        "erm_unet_loss_grad": (ctypes.c_int, [vp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, fp]),
+        "erm_unet_train_step": (ctypes.c_int, [vp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                               fp]),
+        "erm_unet_train": (ctypes.c_int, [vp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_float, ull, fp]),
+        "erm_unet_last_stats": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
     })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -383,7 +402,118 @@ class Elastic2D:
         self._c(lib.erm2d_adjoint(self._h, _fptr(np.ascontiguousarray(r, np.float32)), _fptr(g)))
         return g
 
+    def lsrtm_cg(self, shots, dobs, iters, pmask=3, m0=None, tol=0.0, precond=None):
+        """CGLS over shots [(x, z), ...] with observed data [nshots][2][lt][nrec].
+        precond: optional python callable g -> z on [3][nx][nz] float32 arrays.
+        Returns (m, misfit history, seconds history, iterations)."""
+        p = self.p
+        shots_a = np.ascontiguousarray(np.asarray(shots, np.int32).reshape(-1, 2))
+        d = np.ascontiguousarray(dobs, np.float32)
+        m = np.zeros((3, p.nx, p.nz), np.float32) if m0 is None else np.ascontiguousarray(m0, np.float32).copy()
+        mis = np.zeros(iters + 1, np.float32)
+        sec = np.zeros(iters + 1, np.float64)
+        cb = None
+        if precond is not None:
+            shape = (3, p.nx, p.nz)
+            n = 3 * p.nx * p.nz
+
+            def _cb(gp, zp, user):
+                g = np.ctypeslib.as_array(ctypes.cast(gp, ctypes.POINTER(ctypes.c_float)), shape=(n,)).reshape(shape)
+                z = np.ctypeslib.as_array(ctypes.cast(zp, ctypes.POINTER(ctypes.c_float)), shape=(n,)).reshape(shape)
+                z[...] = precond(g.copy())
+                return 0
+            cb = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)(_cb)
+        it = lib.erm2d_lsrtm_cg(self._h, len(shots_a), shots_a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                _fptr(d), iters, pmask, _fptr(m), _fptr(mis),
+                                sec.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                ctypes.cast(cb, ctypes.c_void_p) if cb else None, None, tol)
+        if it < 0:
+            _check2(it, self._h)
+        return m, mis[:it + 1], sec[:it + 1], it
+
     def last_ms(self):
         v = ctypes.c_double()
         self._c(lib.erm2d_last_ms(self._h, ctypes.byref(v)))
         return v.value
+
+
+def _checku(rc):
+    if rc != 0:
+        raise ErmError(rc, lib.erm_unet_last_error().decode(errors="replace"))
+
+
+class UNet:
+    """U-Net preconditioner for LSRTM (include/erm_unet.h): BF16-MFMA
+    convolutions, FP32 master weights, Adam training on the device.
+    Images are [n][2][h][w] float32 with h, w multiples of 8."""
+
+    def __init__(self, channels=(16, 32, 64, 64), seed=0, device=0):
+        h = ctypes.c_void_p()
+        _checku(lib.erm_unet_create(device, *[int(c) for c in channels], seed, ctypes.byref(h)))
+        self._h = h
+        self.channels = tuple(channels)
+        self.num_params = lib.erm_unet_num_params(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.erm_unet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def get_params(self):
+        out = np.empty(self.num_params, np.float32)
+        _checku(lib.erm_unet_get_params(self._h, _fptr(out)))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.num_params
+        _checku(lib.erm_unet_set_params(self._h, _fptr(p)))
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        n, c, hh, ww = x.shape
+        assert c == 2
+        y = np.empty_like(x)
+        _checku(lib.erm_unet_forward(self._h, _fptr(x), n, hh, ww, _fptr(y)))
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        assert x.shape == t.shape and x.shape[1] == 2
+        n, _, hh, ww = x.shape
+        loss = np.zeros(1, np.float32)
+        g = np.empty(self.num_params, np.float32)
+        _checku(lib.erm_unet_loss_grad(self._h, _fptr(x), _fptr(t), n, hh, ww, _fptr(loss), _fptr(g)))
+        return float(loss[0]), g
+
+    def train_step(self, x, t, lr):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        n, _, hh, ww = x.shape
+        loss = np.zeros(1, np.float32)
+        _checku(lib.erm_unet_train_step(self._h, _fptr(x), _fptr(t), n, hh, ww, lr, _fptr(loss)))
+        return float(loss[0])
+
+    def train(self, x, t, batch, epochs, lr, seed=0):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        count, _, hh, ww = x.shape
+        losses = np.zeros(epochs, np.float32)
+        _checku(lib.erm_unet_train(self._h, _fptr(x), _fptr(t), count, hh, ww, batch, epochs, lr, seed,
+                                   _fptr(losses)))
+        return losses
+
+    def last_stats(self):
+        ms, fl = ctypes.c_double(), ctypes.c_double()
+        _checku(lib.erm_unet_last_stats(self._h, ctypes.byref(ms), ctypes.byref(fl)))
+        return ms.value, fl.value

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <string>
@@ -655,3 +656,88 @@ int erm2d_last_ms(const erm2d_ctx* c, double* ms) {
 }
 
 }  // extern "C"
+
+extern "C" int erm2d_lsrtm_cg(erm2d_ctx* c, int nshots, const int* shots, const float* dobs, int iters, unsigned pmask,
+                              float* m, float* misfit, double* seconds, erm2d_precond_fn precond, void* user,
+                              float tol) {
+    if (!c || nshots <= 0 || !shots || !dobs || iters < 0 || !m || !misfit) return c ? c->fail(ERM_E_PARAM, "bad arguments") : ERM_E_PARAM;
+    const size_t nm = 3 * c->mn, nd = (size_t)2 * c->p.lt * c->p.nrec;
+    const int sx0 = c->p.shot_x, sz0 = c->p.shot_z;
+    auto t0 = std::chrono::steady_clock::now();
+    auto now = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    auto mask = [&](std::vector<float>& v) {
+        for (int a = 0; a < 3; ++a)
+            if (!(pmask & (1u << a))) std::fill(v.begin() + a * c->mn, v.begin() + (a + 1) * c->mn, 0.f);
+    };
+    auto dot = [](const float* a, const float* b, size_t n) { double s = 0; for (size_t i = 0; i < n; ++i) s += (double)a[i] * b[i]; return s; };
+    std::vector<float> r((size_t)nshots * nd), q((size_t)nshots * nd), s(nm), z(nm), pvec(nm), sprev(nm), tmp(nm);
+    std::vector<float> mv(m, m + nm);
+    mask(mv);
+    // L over all shots
+    auto applyL = [&](const std::vector<float>& x, std::vector<float>& out) -> int {
+        for (int k = 0; k < nshots; ++k) {
+            int rc = erm2d_set_shot(c, shots[2 * k], shots[2 * k + 1]);
+            if (!rc) rc = erm2d_born(c, x.data(), out.data() + (size_t)k * nd);
+            if (rc) return rc;
+        }
+        return 0;
+    };
+    auto applyLT = [&](const std::vector<float>& y, std::vector<float>& out) -> int {
+        std::fill(out.begin(), out.end(), 0.f);
+        for (int k = 0; k < nshots; ++k) {
+            int rc = erm2d_set_shot(c, shots[2 * k], shots[2 * k + 1]);
+            if (!rc) rc = erm2d_adjoint(c, y.data() + (size_t)k * nd, tmp.data());
+            if (rc) return rc;
+            for (size_t i = 0; i < nm; ++i) out[i] += tmp[i];
+        }
+        mask(out);
+        return 0;
+    };
+    auto pre = [&](const std::vector<float>& g, std::vector<float>& zo) -> int {
+        if (!precond) { zo = g; return 0; }
+        int rc = precond(g.data(), zo.data(), user);
+        if (rc) return c->fail(ERM_E_PARAM, "preconditioner returned %d", rc);
+        mask(zo);
+        return 0;
+    };
+    const double dnorm = std::sqrt(dot(dobs, dobs, (size_t)nshots * nd));
+    if (dnorm == 0) return c->fail(ERM_E_PARAM, "observed data are zero");
+    int rc;
+    if ((rc = applyL(mv, q))) return rc;
+    for (size_t i = 0; i < r.size(); ++i) r[i] = dobs[i] - q[i];
+    misfit[0] = (float)(std::sqrt(dot(r.data(), r.data(), r.size())) / dnorm);
+    if (seconds) seconds[0] = now();
+    if ((rc = applyLT(r, s)) || (rc = pre(s, z))) return rc;
+    pvec = z;
+    double gamma = dot(s.data(), z.data(), nm);
+    int done = 0;
+    for (int it = 1; it <= iters; ++it) {
+        if (misfit[it - 1] <= tol || gamma <= 0) break;
+        if ((rc = applyL(pvec, q))) return rc;
+        const double qq = dot(q.data(), q.data(), q.size());
+        if (qq <= 0) break;
+        const double alpha = gamma / qq;
+        for (size_t i = 0; i < nm; ++i) mv[i] += (float)(alpha * pvec[i]);
+        for (size_t i = 0; i < r.size(); ++i) r[i] -= (float)(alpha * q[i]);
+        misfit[it] = (float)(std::sqrt(dot(r.data(), r.data(), r.size())) / dnorm);
+        if (seconds) seconds[it] = now();
+        done = it;
+        if (it == iters || misfit[it] <= tol) break;
+        sprev = s;
+        std::vector<float> zprev = z;
+        if ((rc = applyLT(r, s)) || (rc = pre(s, z))) return rc;
+        double gnew = dot(s.data(), z.data(), nm);
+        double beta;
+        if (precond) {  // flexible (Polak-Ribiere) for a nonlinear preconditioner
+            double cross = dot(sprev.data(), z.data(), nm);
+            beta = std::max(0.0, (gnew - cross) / gamma);
+        } else {
+            beta = gnew / gamma;
+        }
+        for (size_t i = 0; i < nm; ++i) pvec[i] = (float)(z[i] + beta * pvec[i]);
+        gamma = gnew;
+    }
+    std::copy(mv.begin(), mv.end(), m);
+    erm2d_set_shot(c, sx0, sz0);
+    return done;
+}

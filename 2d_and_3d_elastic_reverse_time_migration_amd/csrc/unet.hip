@@ -1,0 +1,633 @@
+// unet.hip - U-Net image-domain preconditioner for LSRTM (include/erm_unet.h).
+//
+// Convolutions are implicit GEMMs on BF16 MFMA with FP32 accumulation
+// (v_mfma_f32_16x16x16_bf16).  Operand layout of the 16x16x16 instruction for
+// a 64-lane wavefront (lane l, element j = 0..3 of the 4-wide operands):
+//   A[m][k]: m = l % 16, k = 4*(l/16) + j
+//   B[k][n]: n = l % 16, k = 4*(l/16) + j
+//   D[m][n]: n = l % 16, m = 4*(l/16) + j
+// One wave owns a 16x16 output tile; a workgroup of 4 waves covers 16 x 64.
+//   forward        Y[co][p]  = sum_{ci,t} W[co][ci][t] X[ci][p+o(t)]   (M=Cout, N=pixels, K=Cin*9)
+//   backward data  dX[ci][p] = sum_{co,t} W[co][ci][8-t] dY[co][p+o(t)] (M=Cin,  N=pixels, K=Cout*9)
+//   backward wts   dW[co][ci*9+t] = sum_p dY[co][p] X[ci][p+o(t)]       (M=Cout, N=Cin*9,  K=pixels, split-K)
+// with o(t) = (t/3-1, t%3-1) and zero padding; dY is masked by the ReLU of
+// the layer output where the layer has one.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/erm.h"
+#include "../../include/erm_unet.h"
+
+namespace {
+
+thread_local std::string g_uerr;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// round to nearest even
+__device__ __forceinline__ short to_bf16(float x) {
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+
+enum { FWD = 0, BWD_DATA = 1, BWD_W = 2 };
+
+struct ConvArgs {
+    int n, h, w, cin, cout;
+    const float* x;      // [n][cin][h][w]
+    const float* dy;     // [n][cout][h][w]
+    const float* y;      // layer output for the ReLU mask
+    const short* wb;     // [cout][cin*9] bf16
+    const float* bias;   // [cout]
+    float* out;
+    int relu;
+    int accumulate;
+    int kchunk;          // BWD_W: K per workgroup
+};
+
+__device__ __forceinline__ float sample(const float* t, int c, int C, int b, int yy, int xx, int h, int w) {
+    if (yy < 0 || yy >= h || xx < 0 || xx >= w) return 0.f;
+    return t[(((long)b * C + c) * h + yy) * w + xx];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int hw = a.h * a.w;
+    int M, N, K;
+    if (MODE == FWD) { M = a.cout; N = a.n * hw; K = a.cin * 9; }
+    else if (MODE == BWD_DATA) { M = a.cin; N = a.n * hw; K = a.cout * 9; }
+    else { M = a.cout; N = a.cin * 9; K = a.n * hw; }
+    const int m0 = blockIdx.y * 16, n0 = (blockIdx.x * 4 + wave) * 16;
+    const int kb = MODE == BWD_W ? blockIdx.z * a.kchunk : 0;
+    const int ke = MODE == BWD_W ? min(K, kb + a.kchunk) : K;
+    const int li = lane & 15, lk = 4 * (lane >> 4);
+    const int am = m0 + li;  // A row of this lane
+    const int bn = n0 + li;  // B column of this lane
+    // decode this lane's B column once (pixel for FWD/BWD_DATA, (ci,t) for BWD_W)
+    int pb = 0, py = 0, px = 0, bci = 0, bt = 0;
+    if (MODE != BWD_W) {
+        if (bn < N) { pb = bn / hw; const int r = bn - pb * hw; py = r / a.w; px = r - py * a.w; }
+    } else if (bn < N) { bci = bn / 9; bt = bn - bci * 9; }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kb; k0 < ke; k0 += 16) {
+        bf16x4 av, bv;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + lk + j;
+            float va = 0.f, vb = 0.f;
+            if (k < ke) {
+                if (MODE == FWD) {
+                    if (am < M) va = __int_as_float(((int)(unsigned short)a.wb[(long)am * K + k]) << 16);
+                    if (bn < N) {
+                        const int ci = k / 9, t = k - ci * 9;
+                        vb = sample(a.x, ci, a.cin, pb, py + t / 3 - 1, px + t % 3 - 1, a.h, a.w);
+                    }
+                } else if (MODE == BWD_DATA) {
+                    const int co = k / 9, t = k - co * 9;
+                    if (am < M) va = __int_as_float(((int)(unsigned short)a.wb[((long)co * a.cin + am) * 9 + (8 - t)]) << 16);
+                    if (bn < N) {
+                        const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
+                        float g = sample(a.dy, co, a.cout, pb, yy, xx, a.h, a.w);
+                        if (a.relu && g != 0.f && sample(a.y, co, a.cout, pb, yy, xx, a.h, a.w) <= 0.f) g = 0.f;
+                        vb = g;
+                    }
+                } else {
+                    const int b = k / hw, r = k - b * hw, yy = r / a.w, xx = r - yy * a.w;
+                    if (am < M) {
+                        float g = a.dy[(((long)b * a.cout + am) * a.h + yy) * a.w + xx];
+                        if (a.relu && a.y[(((long)b * a.cout + am) * a.h + yy) * a.w + xx] <= 0.f) g = 0.f;
+                        va = g;
+                    }
+                    if (bn < N) vb = sample(a.x, bci, a.cin, b, yy + bt / 3 - 1, xx + bt % 3 - 1, a.h, a.w);
+                }
+            }
+            av[j] = MODE == FWD || MODE == BWD_DATA ? (short)(__float_as_uint(va) >> 16) : to_bf16(va);
+            bv[j] = to_bf16(vb);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int m = m0 + lk + j, n = n0 + li;
+        if (m >= M || n >= N) continue;
+        if (MODE == FWD) {
+            const int b = n / hw, r = n - b * hw;
+            float v = acc[j] + a.bias[m];
+            if (a.relu) v = fmaxf(v, 0.f);
+            a.out[((long)b * a.cout + m) * hw + r] = v;
+        } else if (MODE == BWD_DATA) {
+            const int b = n / hw, r = n - b * hw;
+            float* o = &a.out[((long)b * a.cin + m) * hw + r];
+            *o = a.accumulate ? *o + acc[j] : acc[j];
+        } else {
+            atomicAdd(&a.out[(long)m * N + n], acc[j]);
+        }
+    }
+}
+
+// d bias[co] = sum over batch and pixels of the (ReLU-masked) output gradient
+__global__ void k_bias_grad(const float* dy, const float* y, int relu, int n, int c, int hw, float* db) {
+    const int co = blockIdx.x;
+    float s = 0.f;
+    for (int b = 0; b < n; ++b)
+        for (int p = threadIdx.x; p < hw; p += blockDim.x) {
+            const long i = ((long)b * c + co) * hw + p;
+            float g = dy[i];
+            if (relu && y[i] <= 0.f) g = 0.f;
+            s += g;
+        }
+    __shared__ float red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] += red[0];
+}
+
+// 2x2 average pooling; backward accumulates
+__global__ void k_pool(const float* in, float* out, int nc, int h, int w) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int ho = h / 2, wo = w / 2;
+    if (i >= (long)nc * ho * wo) return;
+    const int x = i % wo, y = (i / wo) % ho;
+    const long c = i / ((long)ho * wo);
+    const float* s = in + c * h * w + (2 * y) * w + 2 * x;
+    out[i] = 0.25f * (s[0] + s[1] + s[w] + s[w + 1]);
+}
+__global__ void k_pool_bwd(const float* dout, float* din, int nc, int h, int w) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= (long)nc * h * w) return;
+    const int x = i % w, y = (i / w) % h;
+    const long c = i / ((long)h * w);
+    din[i] += 0.25f * dout[c * (h / 2) * (w / 2) + (y / 2) * (w / 2) + x / 2];
+}
+// nearest 2x upsampling of [n][c][h][w] into channels [c0, c0+c) of a [n][ctot][2h][2w] buffer
+__global__ void k_up(const float* in, float* out, int n, int c, int h, int w, int ctot, int c0) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int H = 2 * h, W = 2 * w;
+    if (i >= (long)n * c * H * W) return;
+    const int x = i % W, y = (i / W) % H;
+    const int ch = (i / ((long)H * W)) % c, b = i / ((long)c * H * W);
+    out[(((long)b * ctot + c0 + ch) * H + y) * W + x] = in[(((long)b * c + ch) * h + y / 2) * w + x / 2];
+}
+__global__ void k_up_bwd(const float* dout, float* din, int n, int c, int h, int w, int ctot, int c0) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= (long)n * c * h * w) return;
+    const int x = i % w, y = (i / w) % h;
+    const int ch = (i / ((long)h * w)) % c, b = i / ((long)c * h * w);
+    const int W = 2 * w;
+    const float* s = dout + (((long)b * ctot + c0 + ch) * (2 * h) + 2 * y) * W + 2 * x;
+    din[i] = s[0] + s[1] + s[W] + s[W + 1];
+}
+// copy channels: dst[:, d0:d0+c] (+)= src[:, s0:s0+c]
+__global__ void k_chan_copy(const float* src, int sct, int s0, float* dst, int dct, int d0, int n, int c, int hw,
+                            int add) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= (long)n * c * hw) return;
+    const int p = i % hw, ch = (i / hw) % c, b = i / ((long)c * hw);
+    const float v = src[((long)b * sct + s0 + ch) * hw + p];
+    float* d = &dst[((long)b * dct + d0 + ch) * hw + p];
+    *d = add ? *d + v : v;
+}
+__global__ void k_add(const float* a, const float* b, float* o, long n) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) o[i] = a[i] + b[i];
+}
+// MSE: grad = 2 (y - t) / n, loss partial sums
+__global__ void k_mse(const float* y, const float* t, float* g, long n, float* loss) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    float d = 0.f;
+    if (i < n) {
+        d = y[i] - t[i];
+        g[i] = 2.f * d / (float)n;
+    }
+    __shared__ float red[256];
+    red[threadIdx.x] = d * d;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] / (float)n);
+}
+__global__ void k_adam(float* p, const float* g, float* m, float* v, short* pb, long n, float lr, float c1, float c2) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi, vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+    p[i] = pi;
+    if (pb) pb[i] = to_bf16(pi);
+}
+__global__ void k_to_bf16(const float* p, short* pb, long n) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) pb[i] = to_bf16(p[i]);
+}
+__global__ void k_zero(float* p, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.f;
+}
+
+inline dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+struct Conv {
+    int cin, cout, relu;
+    size_t woff, boff;  // offsets into the parameter vector
+};
+
+}  // namespace
+
+struct erm_unet {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int c[4];
+    std::vector<Conv> conv;   // 16 layers in forward order
+    size_t nparam = 0;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    short* pb = nullptr;
+    long adam_t = 0;
+    // activation / gradient buffers for the current (n, h, w)
+    int bn = 0, bh = 0, bw = 0;
+    std::vector<float*> act, grad;
+    float* xin = nullptr;
+    float* tgt = nullptr;
+    float* loss = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double last_ms = 0, last_flops = 0, flops_acc = 0;
+    std::vector<size_t> act_size;
+
+    ~erm_unet() {
+        auto F = [](void* q) { if (q) (void)hipFree(q); };
+        for (auto* a : act) F(a);
+        for (auto* a : grad) F(a);
+        F(p); F(g); F(m); F(v); F(pb); F(xin); F(tgt); F(loss);
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// buffer ids
+enum {
+    E1A, E1, P1, E2A, E2, P2, E3A, E3, P3, BA, BB, C3, D3A, D3, C2, D2A, D2, C1, D1A, D1, OUT, Y, NBUF
+};
+
+struct Shape { int c, lvl; };
+
+void shapes(const erm_unet* u, Shape* s) {
+    const int c1 = u->c[0], c2 = u->c[1], c3 = u->c[2], c4 = u->c[3];
+    s[E1A] = {c1, 0}; s[E1] = {c1, 0}; s[P1] = {c1, 1};
+    s[E2A] = {c2, 1}; s[E2] = {c2, 1}; s[P2] = {c2, 2};
+    s[E3A] = {c3, 2}; s[E3] = {c3, 2}; s[P3] = {c3, 3};
+    s[BA] = {c4, 3}; s[BB] = {c4, 3};
+    s[C3] = {c4 + c3, 2}; s[D3A] = {c3, 2}; s[D3] = {c3, 2};
+    s[C2] = {c3 + c2, 1}; s[D2A] = {c2, 1}; s[D2] = {c2, 1};
+    s[C1] = {c2 + c1, 0}; s[D1A] = {c1, 0}; s[D1] = {c1, 0};
+    s[OUT] = {2, 0}; s[Y] = {2, 0};
+}
+
+int ensure(erm_unet* u, int n, int h, int w) {
+    if (h % 8 || w % 8 || n <= 0) { g_uerr = "h and w must be multiples of 8 and n > 0"; return ERM_E_PARAM; }
+    if (u->bn == n && u->bh == h && u->bw == w) return 0;
+    for (auto* a : u->act) if (a) (void)hipFree(a);
+    for (auto* a : u->grad) if (a) (void)hipFree(a);
+    if (u->xin) (void)hipFree(u->xin);
+    if (u->tgt) (void)hipFree(u->tgt);
+    u->act.assign(NBUF, nullptr);
+    u->grad.assign(NBUF, nullptr);
+    u->act_size.assign(NBUF, 0);
+    Shape s[NBUF];
+    shapes(u, s);
+    for (int b = 0; b < NBUF; ++b) {
+        size_t sz = (size_t)n * s[b].c * (h >> s[b].lvl) * (w >> s[b].lvl);
+        u->act_size[b] = sz;
+        if (hipMalloc(&u->act[b], sz * 4) != hipSuccess || hipMalloc(&u->grad[b], sz * 4) != hipSuccess) {
+            g_uerr = "hipMalloc failed";
+            return ERM_E_NOMEM;
+        }
+    }
+    size_t xs = (size_t)n * 2 * h * w * 4;
+    if (hipMalloc(&u->xin, xs) != hipSuccess || hipMalloc(&u->tgt, xs) != hipSuccess) { g_uerr = "hipMalloc failed"; return ERM_E_NOMEM; }
+    u->bn = n; u->bh = h; u->bw = w;
+    return 0;
+}
+
+void conv_fwd(erm_unet* u, int li, const float* x, float* y, int n, int h, int w) {
+    const Conv& L = u->conv[li];
+    ConvArgs a{};
+    a.n = n; a.h = h; a.w = w; a.cin = L.cin; a.cout = L.cout;
+    a.x = x; a.wb = u->pb + L.woff; a.bias = u->p + L.boff; a.out = y; a.relu = L.relu;
+    const long N = (long)n * h * w;
+    dim3 grd((unsigned)((N + 63) / 64), (unsigned)((L.cout + 15) / 16));
+    hipLaunchKernelGGL(k_conv_mfma<FWD>, grd, dim3(256), 0, u->stream, a);
+    u->flops_acc += 2.0 * ((L.cout + 15) / 16 * 16) * (double)((N + 63) / 64 * 64) * ((L.cin * 9 + 15) / 16 * 16);
+}
+
+// backward of conv li: dy is the gradient w.r.t. its output y; adds dW/db into u->g,
+// writes (or accumulates) dx when dx != nullptr
+void conv_bwd(erm_unet* u, int li, const float* x, const float* y, const float* dy, float* dx, int acc_dx, int n,
+              int h, int w) {
+    const Conv& L = u->conv[li];
+    ConvArgs a{};
+    a.n = n; a.h = h; a.w = w; a.cin = L.cin; a.cout = L.cout;
+    a.x = x; a.y = y; a.dy = dy; a.wb = u->pb + L.woff; a.relu = L.relu;
+    const long P = (long)n * h * w;
+    if (dx) {
+        a.out = dx;
+        a.accumulate = acc_dx;
+        dim3 grd((unsigned)((P + 63) / 64), (unsigned)((L.cin + 15) / 16));
+        hipLaunchKernelGGL(k_conv_mfma<BWD_DATA>, grd, dim3(256), 0, u->stream, a);
+        u->flops_acc += 2.0 * ((L.cin + 15) / 16 * 16) * (double)((P + 63) / 64 * 64) * ((L.cout * 9 + 15) / 16 * 16);
+    }
+    {
+        a.out = u->g + L.woff;
+        const int Nw = L.cin * 9;
+        const int kchunk = 1024;  // pixels per workgroup (multiple of 16)
+        dim3 grd((unsigned)((Nw + 63) / 64), (unsigned)((L.cout + 15) / 16), (unsigned)((P + kchunk - 1) / kchunk));
+        a.kchunk = kchunk;
+        hipLaunchKernelGGL(k_conv_mfma<BWD_W>, grd, dim3(256), 0, u->stream, a);
+        u->flops_acc += 2.0 * ((L.cout + 15) / 16 * 16) * (double)((Nw + 63) / 64 * 64) * ((P + 15) / 16 * 16);
+        hipLaunchKernelGGL(k_bias_grad, dim3(L.cout), dim3(256), 0, u->stream, dy, y, L.relu, n, L.cout, h * w,
+                           u->g + L.boff);
+    }
+}
+
+void forward(erm_unet* u, int n, int h, int w) {
+    auto& A = u->act;
+    const int c1 = u->c[0], c2 = u->c[1], c3 = u->c[2], c4 = u->c[3];
+    int hh[4] = {h, h / 2, h / 4, h / 8}, ww[4] = {w, w / 2, w / 4, w / 8};
+    conv_fwd(u, 0, u->xin, A[E1A], n, hh[0], ww[0]);
+    conv_fwd(u, 1, A[E1A], A[E1], n, hh[0], ww[0]);
+    hipLaunchKernelGGL(k_pool, g1((long)n * c1 * hh[1] * ww[1]), dim3(256), 0, u->stream, A[E1], A[P1], n * c1, hh[0], ww[0]);
+    conv_fwd(u, 2, A[P1], A[E2A], n, hh[1], ww[1]);
+    conv_fwd(u, 3, A[E2A], A[E2], n, hh[1], ww[1]);
+    hipLaunchKernelGGL(k_pool, g1((long)n * c2 * hh[2] * ww[2]), dim3(256), 0, u->stream, A[E2], A[P2], n * c2, hh[1], ww[1]);
+    conv_fwd(u, 4, A[P2], A[E3A], n, hh[2], ww[2]);
+    conv_fwd(u, 5, A[E3A], A[E3], n, hh[2], ww[2]);
+    hipLaunchKernelGGL(k_pool, g1((long)n * c3 * hh[3] * ww[3]), dim3(256), 0, u->stream, A[E3], A[P3], n * c3, hh[2], ww[2]);
+    conv_fwd(u, 6, A[P3], A[BA], n, hh[3], ww[3]);
+    conv_fwd(u, 7, A[BA], A[BB], n, hh[3], ww[3]);
+    // decoder level 3
+    hipLaunchKernelGGL(k_up, g1((long)n * c4 * hh[2] * ww[2]), dim3(256), 0, u->stream, A[BB], A[C3], n, c4, hh[3], ww[3], c4 + c3, 0);
+    hipLaunchKernelGGL(k_chan_copy, g1((long)n * c3 * hh[2] * ww[2]), dim3(256), 0, u->stream, A[E3], c3, 0, A[C3], c4 + c3, c4, n, c3, hh[2] * ww[2], 0);
+    conv_fwd(u, 8, A[C3], A[D3A], n, hh[2], ww[2]);
+    conv_fwd(u, 9, A[D3A], A[D3], n, hh[2], ww[2]);
+    hipLaunchKernelGGL(k_up, g1((long)n * c3 * hh[1] * ww[1]), dim3(256), 0, u->stream, A[D3], A[C2], n, c3, hh[2], ww[2], c3 + c2, 0);
+    hipLaunchKernelGGL(k_chan_copy, g1((long)n * c2 * hh[1] * ww[1]), dim3(256), 0, u->stream, A[E2], c2, 0, A[C2], c3 + c2, c3, n, c2, hh[1] * ww[1], 0);
+    conv_fwd(u, 10, A[C2], A[D2A], n, hh[1], ww[1]);
+    conv_fwd(u, 11, A[D2A], A[D2], n, hh[1], ww[1]);
+    hipLaunchKernelGGL(k_up, g1((long)n * c2 * hh[0] * ww[0]), dim3(256), 0, u->stream, A[D2], A[C1], n, c2, hh[1], ww[1], c2 + c1, 0);
+    hipLaunchKernelGGL(k_chan_copy, g1((long)n * c1 * hh[0] * ww[0]), dim3(256), 0, u->stream, A[E1], c1, 0, A[C1], c2 + c1, c2, n, c1, hh[0] * ww[0], 0);
+    conv_fwd(u, 12, A[C1], A[D1A], n, hh[0], ww[0]);
+    conv_fwd(u, 13, A[D1A], A[D1], n, hh[0], ww[0]);
+    conv_fwd(u, 14, A[D1], A[OUT], n, hh[0], ww[0]);
+    hipLaunchKernelGGL(k_add, g1((long)n * 2 * h * w), dim3(256), 0, u->stream, A[OUT], u->xin, A[Y], (long)n * 2 * h * w);
+}
+
+// gradient w.r.t. Y must be in G[Y]; fills u->g
+void backward(erm_unet* u, int n, int h, int w) {
+    auto& A = u->act;
+    auto& G = u->grad;
+    const int c1 = u->c[0], c2 = u->c[1], c3 = u->c[2], c4 = u->c[3];
+    int hh[4] = {h, h / 2, h / 4, h / 8}, ww[4] = {w, w / 2, w / 4, w / 8};
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, u->stream, u->g, (long)u->nparam);
+    // Y = OUT + x: dOUT = dY
+    conv_bwd(u, 14, A[D1], A[OUT], G[Y], G[D1], 0, n, hh[0], ww[0]);
+    conv_bwd(u, 13, A[D1A], A[D1], G[D1], G[D1A], 0, n, hh[0], ww[0]);
+    conv_bwd(u, 12, A[C1], A[D1A], G[D1A], G[C1], 0, n, hh[0], ww[0]);
+    // C1 = [up(D2) | E1]
+    hipLaunchKernelGGL(k_up_bwd, g1((long)n * c2 * hh[1] * ww[1]), dim3(256), 0, u->stream, G[C1], G[D2], n, c2, hh[1], ww[1], c2 + c1, 0);
+    hipLaunchKernelGGL(k_chan_copy, g1((long)n * c1 * hh[0] * ww[0]), dim3(256), 0, u->stream, G[C1], c2 + c1, c2, G[E1], c1, 0, n, c1, hh[0] * ww[0], 0);
+    conv_bwd(u, 11, A[D2A], A[D2], G[D2], G[D2A], 0, n, hh[1], ww[1]);
+    conv_bwd(u, 10, A[C2], A[D2A], G[D2A], G[C2], 0, n, hh[1], ww[1]);
+    hipLaunchKernelGGL(k_up_bwd, g1((long)n * c3 * hh[2] * ww[2]), dim3(256), 0, u->stream, G[C2], G[D3], n, c3, hh[2], ww[2], c3 + c2, 0);
+    hipLaunchKernelGGL(k_chan_copy, g1((long)n * c2 * hh[1] * ww[1]), dim3(256), 0, u->stream, G[C2], c3 + c2, c3, G[E2], c2, 0, n, c2, hh[1] * ww[1], 0);
+    conv_bwd(u, 9, A[D3A], A[D3], G[D3], G[D3A], 0, n, hh[2], ww[2]);
+    conv_bwd(u, 8, A[C3], A[D3A], G[D3A], G[C3], 0, n, hh[2], ww[2]);
+    hipLaunchKernelGGL(k_up_bwd, g1((long)n * c4 * hh[3] * ww[3]), dim3(256), 0, u->stream, G[C3], G[BB], n, c4, hh[3], ww[3], c4 + c3, 0);
+    hipLaunchKernelGGL(k_chan_copy, g1((long)n * c3 * hh[2] * ww[2]), dim3(256), 0, u->stream, G[C3], c4 + c3, c4, G[E3], c3, 0, n, c3, hh[2] * ww[2], 0);
+    conv_bwd(u, 7, A[BA], A[BB], G[BB], G[BA], 0, n, hh[3], ww[3]);
+    conv_bwd(u, 6, A[P3], A[BA], G[BA], G[P3], 0, n, hh[3], ww[3]);
+    // E3 feeds the skip (already in G[E3]) and the pool
+    hipLaunchKernelGGL(k_pool_bwd, g1((long)n * c3 * hh[2] * ww[2]), dim3(256), 0, u->stream, G[P3], G[E3], n * c3, hh[2], ww[2]);
+    conv_bwd(u, 5, A[E3A], A[E3], G[E3], G[E3A], 0, n, hh[2], ww[2]);
+    conv_bwd(u, 4, A[P2], A[E3A], G[E3A], G[P2], 0, n, hh[2], ww[2]);
+    hipLaunchKernelGGL(k_pool_bwd, g1((long)n * c2 * hh[1] * ww[1]), dim3(256), 0, u->stream, G[P2], G[E2], n * c2, hh[1], ww[1]);
+    conv_bwd(u, 3, A[E2A], A[E2], G[E2], G[E2A], 0, n, hh[1], ww[1]);
+    conv_bwd(u, 2, A[P1], A[E2A], G[E2A], G[P1], 0, n, hh[1], ww[1]);
+    hipLaunchKernelGGL(k_pool_bwd, g1((long)n * c1 * hh[0] * ww[0]), dim3(256), 0, u->stream, G[P1], G[E1], n * c1, hh[0], ww[0]);
+    conv_bwd(u, 1, A[E1A], A[E1], G[E1], G[E1A], 0, n, hh[0], ww[0]);
+    conv_bwd(u, 0, u->xin, A[E1A], G[E1A], nullptr, 0, n, hh[0], ww[0]);
+}
+
+int upload(erm_unet* u, float* d, const float* h, size_t n) {
+    if (hipMemcpyAsync(d, h, n * 4, hipMemcpyHostToDevice, u->stream) != hipSuccess) { g_uerr = "upload failed"; return ERM_E_DEVICE; }
+    return 0;
+}
+int sync(erm_unet* u) {
+    hipError_t e = hipStreamSynchronize(u->stream);
+    if (e != hipSuccess) { g_uerr = std::string("device error: ") + hipGetErrorString(e); return ERM_E_DEVICE; }
+    return 0;
+}
+int loss_and_grad(erm_unet* u, int n, int h, int w, float* loss_out) {
+    const long ne = (long)n * 2 * h * w;
+    hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, u->stream, u->loss, 1L);
+    forward(u, n, h, w);
+    hipLaunchKernelGGL(k_mse, g1(ne), dim3(256), 0, u->stream, u->act[Y], u->tgt, u->grad[Y], ne, u->loss);
+    backward(u, n, h, w);
+    if (loss_out) {
+        if (hipMemcpyAsync(loss_out, u->loss, 4, hipMemcpyDeviceToHost, u->stream) != hipSuccess) { g_uerr = "copy failed"; return ERM_E_DEVICE; }
+    }
+    return sync(u);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* erm_unet_last_error(void) { return g_uerr.c_str(); }
+
+int erm_unet_create(int device, int c1, int c2, int c3, int c4, unsigned long long seed, erm_unet** out) {
+    if (!out || c1 <= 0 || c2 <= 0 || c3 <= 0 || c4 <= 0) { g_uerr = "bad arguments"; return ERM_E_PARAM; }
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) { g_uerr = "hipSetDevice failed"; return ERM_E_DEVICE; }
+    erm_unet* u = new erm_unet();
+    u->device = device;
+    u->c[0] = c1; u->c[1] = c2; u->c[2] = c3; u->c[3] = c4;
+    const int spec[15][3] = {{2, c1, 1}, {c1, c1, 1}, {c1, c2, 1}, {c2, c2, 1}, {c2, c3, 1}, {c3, c3, 1},
+                             {c3, c4, 1}, {c4, c4, 1}, {c4 + c3, c3, 1}, {c3, c3, 1}, {c3 + c2, c2, 1},
+                             {c2, c2, 1}, {c2 + c1, c1, 1}, {c1, c1, 1}, {c1, 2, 0}};
+    size_t off = 0;
+    for (auto& s : spec) {
+        Conv L;
+        L.cin = s[0]; L.cout = s[1]; L.relu = s[2];
+        L.woff = off;
+        off += (size_t)L.cin * L.cout * 9;
+        L.boff = off;
+        off += L.cout;
+        u->conv.push_back(L);
+    }
+    u->nparam = off;
+    std::vector<float> init(off, 0.f);
+    std::mt19937_64 rng(seed);
+    for (size_t li = 0; li < u->conv.size(); ++li) {
+        const Conv& L = u->conv[li];
+        // He initialisation; the output layer starts at 1% scale so the net starts near identity
+        const double sd = std::sqrt(2.0 / (L.cin * 9.0)) * (li + 1 == u->conv.size() ? 0.01 : 1.0);
+        std::normal_distribution<double> N(0.0, sd);
+        for (size_t i = 0; i < (size_t)L.cin * L.cout * 9; ++i) init[L.woff + i] = (float)N(rng);
+    }
+    bool ok = hipStreamCreateWithFlags(&u->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc(&u->p, off * 4) == hipSuccess && hipMalloc(&u->g, off * 4) == hipSuccess &&
+              hipMalloc(&u->m, off * 4) == hipSuccess && hipMalloc(&u->v, off * 4) == hipSuccess &&
+              hipMalloc(&u->pb, off * 2) == hipSuccess && hipMalloc(&u->loss, 4) == hipSuccess &&
+              hipEventCreate(&u->ev[0]) == hipSuccess && hipEventCreate(&u->ev[1]) == hipSuccess;
+    if (!ok) { g_uerr = "allocation failed"; delete u; return ERM_E_NOMEM; }
+    int rc = erm_unet_set_params(u, init.data());
+    if (rc) { delete u; return rc; }
+    *out = u;
+    return 0;
+}
+
+int erm_unet_destroy(erm_unet* u) {
+    if (!u) return 0;
+    (void)hipSetDevice(u->device);
+    if (u->stream) (void)hipStreamSynchronize(u->stream);
+    delete u;
+    return 0;
+}
+
+size_t erm_unet_num_params(const erm_unet* u) { return u ? u->nparam : 0; }
+
+int erm_unet_get_params(erm_unet* u, float* dst) {
+    if (!u || !dst) return ERM_E_PARAM;
+    if (hipMemcpyAsync(dst, u->p, u->nparam * 4, hipMemcpyDeviceToHost, u->stream) != hipSuccess) { g_uerr = "copy failed"; return ERM_E_DEVICE; }
+    return sync(u);
+}
+
+int erm_unet_set_params(erm_unet* u, const float* src) {
+    if (!u || !src) return ERM_E_PARAM;
+    int rc = upload(u, u->p, src, u->nparam);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_to_bf16, g1((long)u->nparam), dim3(256), 0, u->stream, u->p, u->pb, (long)u->nparam);
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, u->stream, u->m, (long)u->nparam);
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, u->stream, u->v, (long)u->nparam);
+    u->adam_t = 0;
+    return sync(u);
+}
+
+int erm_unet_forward(erm_unet* u, const float* x, int n, int h, int w, float* y) {
+    if (!u || !x || !y) return ERM_E_PARAM;
+    (void)hipSetDevice(u->device);
+    int rc = ensure(u, n, h, w);
+    if (rc) return rc;
+    if ((rc = upload(u, u->xin, x, (size_t)n * 2 * h * w))) return rc;
+    u->flops_acc = 0;
+    (void)hipEventRecord(u->ev[0], u->stream);
+    forward(u, n, h, w);
+    (void)hipEventRecord(u->ev[1], u->stream);
+    if (hipMemcpyAsync(y, u->act[Y], (size_t)n * 2 * h * w * 4, hipMemcpyDeviceToHost, u->stream) != hipSuccess) { g_uerr = "copy failed"; return ERM_E_DEVICE; }
+    if ((rc = sync(u))) return rc;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, u->ev[0], u->ev[1]);
+    u->last_ms = ms;
+    u->last_flops = u->flops_acc;
+    return 0;
+}
+
+int erm_unet_loss_grad(erm_unet* u, const float* x, const float* t, int n, int h, int w, float* loss, float* grad) {
+    if (!u || !x || !t) return ERM_E_PARAM;
+    (void)hipSetDevice(u->device);
+    int rc = ensure(u, n, h, w);
+    if (rc) return rc;
+    const size_t ne = (size_t)n * 2 * h * w;
+    if ((rc = upload(u, u->xin, x, ne)) || (rc = upload(u, u->tgt, t, ne))) return rc;
+    if ((rc = loss_and_grad(u, n, h, w, loss))) return rc;
+    if (grad) {
+        if (hipMemcpyAsync(grad, u->g, u->nparam * 4, hipMemcpyDeviceToHost, u->stream) != hipSuccess) { g_uerr = "copy failed"; return ERM_E_DEVICE; }
+        return sync(u);
+    }
+    return 0;
+}
+
+static int adam(erm_unet* u, float lr) {
+    u->adam_t += 1;
+    const float c1 = (float)(1.0 - std::pow(0.9, (double)u->adam_t)), c2 = (float)(1.0 - std::pow(0.999, (double)u->adam_t));
+    hipLaunchKernelGGL(k_adam, g1((long)u->nparam), dim3(256), 0, u->stream, u->p, u->g, u->m, u->v, u->pb,
+                       (long)u->nparam, lr, c1, c2);
+    return 0;
+}
+
+int erm_unet_train_step(erm_unet* u, const float* x, const float* t, int n, int h, int w, float lr, float* loss) {
+    if (!u || !x || !t) return ERM_E_PARAM;
+    (void)hipSetDevice(u->device);
+    int rc = ensure(u, n, h, w);
+    if (rc) return rc;
+    const size_t ne = (size_t)n * 2 * h * w;
+    if ((rc = upload(u, u->xin, x, ne)) || (rc = upload(u, u->tgt, t, ne))) return rc;
+    if ((rc = loss_and_grad(u, n, h, w, loss))) return rc;
+    adam(u, lr);
+    return sync(u);
+}
+
+int erm_unet_train(erm_unet* u, const float* x, const float* t, int count, int h, int w, int batch, int epochs,
+                   float lr, unsigned long long seed, float* losses) {
+    if (!u || !x || !t || count <= 0 || batch <= 0 || epochs <= 0) { g_uerr = "bad arguments"; return ERM_E_PARAM; }
+    (void)hipSetDevice(u->device);
+    const size_t per = (size_t)2 * h * w;
+    std::vector<int> order(count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    std::mt19937_64 rng(seed);
+    std::vector<float> bx(per * batch), bt(per * batch);
+    u->flops_acc = 0;
+    (void)hipEventRecord(u->ev[0], u->stream);
+    for (int e = 0; e < epochs; ++e) {
+        std::shuffle(order.begin(), order.end(), rng);
+        double sum = 0;
+        int nb = 0;
+        for (int s = 0; s + batch <= count; s += batch) {
+            for (int b = 0; b < batch; ++b) {
+                std::memcpy(&bx[b * per], x + (size_t)order[s + b] * per, per * 4);
+                std::memcpy(&bt[b * per], t + (size_t)order[s + b] * per, per * 4);
+            }
+            int rc = ensure(u, batch, h, w);
+            if (rc) return rc;
+            if ((rc = upload(u, u->xin, bx.data(), per * batch)) || (rc = upload(u, u->tgt, bt.data(), per * batch))) return rc;
+            float l = 0;
+            if ((rc = loss_and_grad(u, batch, h, w, &l))) return rc;
+            adam(u, lr);
+            sum += l;
+            ++nb;
+        }
+        if (losses) losses[e] = nb ? (float)(sum / nb) : 0.f;
+    }
+    (void)hipEventRecord(u->ev[1], u->stream);
+    int rc = sync(u);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, u->ev[0], u->ev[1]);
+    u->last_ms = ms;
+    u->last_flops = u->flops_acc;
+    return rc;
+}
+
+int erm_unet_last_stats(const erm_unet* u, double* ms, double* flops) {
+    if (!u) return ERM_E_PARAM;
+    if (ms) *ms = u->last_ms;
+    if (flops) *flops = u->last_flops;
+    return 0;
+}
+
+}  // extern "C"

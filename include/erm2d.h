@@ -88,6 +88,23 @@ int   erm2d_rtm(erm2d_ctx* ctx, const float* rec, float* img_tp, float* img_pp, 
  * where L^T is the exact transpose of the discrete L (dot-product exact). */
 int   erm2d_born(erm2d_ctx* ctx, const float* dm, float* d);
 int   erm2d_adjoint(erm2d_ctx* ctx, const float* r, float* g);
+/* Least-squares RTM by CGLS (conjugate gradients on the normal equations,
+ * Paige & Saunders 1982) over nshots shots: minimise sum_s |L_s m - d_s|^2.
+ * The reference ships the operators but no CG loop.
+ *   shots   [nshots][2]  source (x, z), model coordinates
+ *   dobs    [nshots][2][lt][nrec] observed data
+ *   pmask   bit mask of inverted parameters (1: dvp/vp, 2: dvs/vs, 4: drho/rho)
+ *   m       [3][nx][nz] initial model in, solution out
+ *   misfit  [iters+1]   |L m - d| / |d| before iteration 1 and after each iteration
+ *   seconds [iters+1]   wall time at the same points (may be NULL)
+ *   precond optional z = P(g) over the model vector (NULL: identity); with a
+ *           preconditioner the update is the flexible (Polak-Ribiere) form.
+ * Returns the number of iterations run (stops early when misfit <= tol), or
+ * a negative error code. */
+typedef int (*erm2d_precond_fn)(const float* g, float* z, void* user);
+int   erm2d_lsrtm_cg(erm2d_ctx* ctx, int nshots, const int* shots, const float* dobs, int iters, unsigned pmask,
+                     float* m, float* misfit, double* seconds, erm2d_precond_fn precond, void* user, float tol);
+
 /* Timing of the last operator call (HIP events): milliseconds. */
 int   erm2d_last_ms(const erm2d_ctx* ctx, double* ms);
 
