@@ -119,6 +119,10 @@ def _load():
         "erm_unet_train": (ctypes.c_int, [vp, fp, fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_float, ull, fp]),
         "erm_unet_last_stats": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
+        "erm_unet_bind_model": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+        "erm_unet_precond": (ctypes.c_int, [fp, fp, vp]),
+        "erm2d_set_workers": (ctypes.c_int, [vp, ctypes.c_int]),
+        "erm2d_set_cg_monitor": (ctypes.c_int, [vp, vp, vp]),
     })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -402,9 +406,15 @@ class Elastic2D:
         self._c(lib.erm2d_adjoint(self._h, _fptr(np.ascontiguousarray(r, np.float32)), _fptr(g)))
         return g
 
-    def lsrtm_cg(self, shots, dobs, iters, pmask=3, m0=None, tol=0.0, precond=None):
+    def set_workers(self, n):
+        """Run the shots of lsrtm_cg on n concurrent contexts/streams."""
+        self._c(lib.erm2d_set_workers(self._h, n))
+
+    def lsrtm_cg(self, shots, dobs, iters, pmask=3, m0=None, tol=0.0, precond=None, monitor=None):
         """CGLS over shots [(x, z), ...] with observed data [nshots][2][lt][nrec].
-        precond: optional python callable g -> z on [3][nx][nz] float32 arrays.
+        precond: a UNet (applied through erm_unet_precond) or a python callable
+        g -> z on [3][nx][nz] float32 arrays.  monitor: optional callable
+        (it, m, g, misfit) called with copies of the model and gradient.
         Returns (m, misfit history, seconds history, iterations)."""
         p = self.p
         shots_a = np.ascontiguousarray(np.asarray(shots, np.int32).reshape(-1, 2))
@@ -412,23 +422,42 @@ class Elastic2D:
         m = np.zeros((3, p.nx, p.nz), np.float32) if m0 is None else np.ascontiguousarray(m0, np.float32).copy()
         mis = np.zeros(iters + 1, np.float32)
         sec = np.zeros(iters + 1, np.float64)
-        cb = None
-        if precond is not None:
-            shape = (3, p.nx, p.nz)
-            n = 3 * p.nx * p.nz
+        shape = (3, p.nx, p.nz)
+        n = 3 * p.nx * p.nz
 
-            def _cb(gp, zp, user):
-                g = np.ctypeslib.as_array(ctypes.cast(gp, ctypes.POINTER(ctypes.c_float)), shape=(n,)).reshape(shape)
-                z = np.ctypeslib.as_array(ctypes.cast(zp, ctypes.POINTER(ctypes.c_float)), shape=(n,)).reshape(shape)
-                z[...] = precond(g.copy())
+        def view(ptr):
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ctypes.c_float)), shape=(n,)).reshape(shape)
+
+        cb, user = None, None
+        if isinstance(precond, UNet):
+            _checku(lib.erm_unet_bind_model(precond._h, p.nx, p.nz))
+            cb, user = ctypes.cast(lib.erm_unet_precond, ctypes.c_void_p), precond._h
+        elif precond is not None:
+            def _cb(gp, zp, _user):
+                view(zp)[...] = precond(view(gp).copy())
                 return 0
             cb = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)(_cb)
-        it = lib.erm2d_lsrtm_cg(self._h, len(shots_a), shots_a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                _fptr(d), iters, pmask, _fptr(m), _fptr(mis),
-                                sec.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                ctypes.cast(cb, ctypes.c_void_p) if cb else None, None, tol)
+        mon = None
+        if monitor is not None:
+            def _mon(it, mp, gp, mis_, _user):
+                monitor(it, view(mp).copy(), view(gp).copy(), mis_)
+                return 0
+            mon = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                   ctypes.c_void_p)(_mon)
+        self._c(lib.erm2d_set_cg_monitor(self._h, ctypes.cast(mon, ctypes.c_void_p) if mon else None, None))
+        try:
+            it = lib.erm2d_lsrtm_cg(self._h, len(shots_a), shots_a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                    _fptr(d), iters, pmask, _fptr(m), _fptr(mis),
+                                    sec.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    ctypes.cast(cb, ctypes.c_void_p) if isinstance(cb, ctypes._CFuncPtr) else cb,
+                                    user, tol)
+        finally:
+            lib.erm2d_set_cg_monitor(self._h, None, None)
         if it < 0:
-            _check2(it, self._h)
+            msg = lib.erm2d_last_error(self._h).decode(errors="replace")
+            if isinstance(precond, UNet):
+                msg += " / " + lib.erm_unet_last_error().decode(errors="replace")
+            raise ErmError(it, msg)
         return m, mis[:it + 1], sec[:it + 1], it
 
     def last_ms(self):

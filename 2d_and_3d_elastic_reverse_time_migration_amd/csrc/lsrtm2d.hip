@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/erm2d.h"
@@ -95,6 +96,10 @@ struct erm2d_ctx {
     float* grad = nullptr;  // [3][mn]
     hipEvent_t ev[2] = {nullptr, nullptr};
     double last_ms = 0;
+    std::vector<float> bg_host[5];          // background model as loaded (vp vs rho qp qs), for workers
+    std::vector<erm2d_ctx*> workers;        // extra contexts (own streams) for concurrent shots
+    erm2d_cg_monitor_fn monitor = nullptr;
+    void* monitor_user = nullptr;
     std::string err;
     int fail(int code, const char* fmt, ...) {
         va_list ap;
@@ -411,6 +416,8 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
 
 int erm2d_destroy(erm2d_ctx* c) {
     if (!c) return 0;
+    for (auto* w : c->workers) erm2d_destroy(w);
+    c->workers.clear();
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free2(c);
@@ -461,6 +468,44 @@ int erm2d_load_model(erm2d_ctx* c, int which, const float* vp, const float* vs, 
     for (int l = 0; l < c->L; ++l)
         if (upload(c, m.kP[l], hkP[l]) || upload(c, m.kS[l], hkS[l]) || upload(c, m.decay[l], hd[l])) return ERM_E_DEVICE;
     m.have = true;
+    if (which == 1) {
+        const float* src[5] = {vp, vs, rho, c->L ? qp : nullptr, c->L ? qs : nullptr};
+        for (int a = 0; a < 5; ++a) {
+            if (src[a]) c->bg_host[a].assign(src[a], src[a] + c->mn);
+            else c->bg_host[a].clear();
+        }
+        for (auto* w : c->workers) {
+            int rc = erm2d_load_model(w, 1, vp, vs, rho, qp, qs);
+            if (rc) return c->fail(rc, "worker: %s", w->err.c_str());
+        }
+    }
+    return 0;
+}
+
+int erm2d_set_workers(erm2d_ctx* c, int n) {
+    if (!c || n < 1 || n > 16) return c ? c->fail(ERM_E_PARAM, "workers must be 1..16") : ERM_E_PARAM;
+    while ((int)c->workers.size() > n - 1) {
+        erm2d_destroy(c->workers.back());
+        c->workers.pop_back();
+    }
+    while ((int)c->workers.size() < n - 1) {
+        erm2d_ctx* w = nullptr;
+        int rc = erm2d_create(&c->p, c->device, &w);
+        if (rc) return c->fail(rc, "worker create: %s", g_err2.c_str());
+        c->workers.push_back(w);
+        if (c->mod[1].have) {
+            auto P = [&](int a) { return c->bg_host[a].empty() ? nullptr : c->bg_host[a].data(); };
+            rc = erm2d_load_model(w, 1, P(0), P(1), P(2), P(3), P(4));
+            if (rc) return c->fail(rc, "worker load: %s", w->err.c_str());
+        }
+    }
+    return 0;
+}
+
+int erm2d_set_cg_monitor(erm2d_ctx* c, erm2d_cg_monitor_fn fn, void* user) {
+    if (!c) return ERM_E_PARAM;
+    c->monitor = fn;
+    c->monitor_user = user;
     return 0;
 }
 
@@ -670,26 +715,48 @@ extern "C" int erm2d_lsrtm_cg(erm2d_ctx* c, int nshots, const int* shots, const 
             if (!(pmask & (1u << a))) std::fill(v.begin() + a * c->mn, v.begin() + (a + 1) * c->mn, 0.f);
     };
     auto dot = [](const float* a, const float* b, size_t n) { double s = 0; for (size_t i = 0; i < n; ++i) s += (double)a[i] * b[i]; return s; };
-    std::vector<float> r((size_t)nshots * nd), q((size_t)nshots * nd), s(nm), z(nm), pvec(nm), sprev(nm), tmp(nm);
+    std::vector<float> r((size_t)nshots * nd), q((size_t)nshots * nd), s(nm), z(nm), pvec(nm);
     std::vector<float> mv(m, m + nm);
     mask(mv);
-    // L over all shots
-    auto applyL = [&](const std::vector<float>& x, std::vector<float>& out) -> int {
-        for (int k = 0; k < nshots; ++k) {
-            int rc = erm2d_set_shot(c, shots[2 * k], shots[2 * k + 1]);
-            if (!rc) rc = erm2d_born(c, x.data(), out.data() + (size_t)k * nd);
-            if (rc) return rc;
-        }
+    // Shots are spread over the context and its workers; each worker runs its
+    // shots on its own stream from its own host thread, so small shots overlap.
+    std::vector<erm2d_ctx*> pool{c};
+    for (auto* w : c->workers) pool.push_back(w);
+    const int nw = std::min((int)pool.size(), nshots);
+    std::vector<std::vector<float>> part(nw, std::vector<float>(nm)), tmp(nw, std::vector<float>(nm));
+    auto parallel = [&](auto&& job) -> int {
+        std::vector<int> rcs(nw, 0);
+        std::vector<std::thread> th;
+        for (int w = 1; w < nw; ++w)
+            th.emplace_back([&, w]() {
+                for (int k = w; k < nshots && !rcs[w]; k += nw) rcs[w] = job(pool[w], w, k);
+            });
+        for (int k = 0; k < nshots && !rcs[0]; k += nw) rcs[0] = job(pool[0], 0, k);
+        for (auto& t : th) t.join();
+        for (int w = 0; w < nw; ++w)
+            if (rcs[w]) return w ? c->fail(rcs[w], "worker %d: %s", w, pool[w]->err.c_str()) : rcs[w];
         return 0;
     };
+    // L over all shots
+    auto applyL = [&](const std::vector<float>& x, std::vector<float>& out) -> int {
+        return parallel([&](erm2d_ctx* cw, int, int k) {
+            int rc = erm2d_set_shot(cw, shots[2 * k], shots[2 * k + 1]);
+            if (!rc) rc = erm2d_born(cw, x.data(), out.data() + (size_t)k * nd);
+            return rc;
+        });
+    };
     auto applyLT = [&](const std::vector<float>& y, std::vector<float>& out) -> int {
+        for (auto& pp : part) std::fill(pp.begin(), pp.end(), 0.f);
+        int rc = parallel([&](erm2d_ctx* cw, int w, int k) {
+            int rc2 = erm2d_set_shot(cw, shots[2 * k], shots[2 * k + 1]);
+            if (!rc2) rc2 = erm2d_adjoint(cw, y.data() + (size_t)k * nd, tmp[w].data());
+            if (!rc2) for (size_t i = 0; i < nm; ++i) part[w][i] += tmp[w][i];
+            return rc2;
+        });
+        if (rc) return rc;
         std::fill(out.begin(), out.end(), 0.f);
-        for (int k = 0; k < nshots; ++k) {
-            int rc = erm2d_set_shot(c, shots[2 * k], shots[2 * k + 1]);
-            if (!rc) rc = erm2d_adjoint(c, y.data() + (size_t)k * nd, tmp.data());
-            if (rc) return rc;
-            for (size_t i = 0; i < nm; ++i) out[i] += tmp[i];
-        }
+        for (int w = 0; w < nw; ++w)
+            for (size_t i = 0; i < nm; ++i) out[i] += part[w][i];
         mask(out);
         return 0;
     };
@@ -707,35 +774,70 @@ extern "C" int erm2d_lsrtm_cg(erm2d_ctx* c, int nshots, const int* shots, const 
     for (size_t i = 0; i < r.size(); ++i) r[i] = dobs[i] - q[i];
     misfit[0] = (float)(std::sqrt(dot(r.data(), r.data(), r.size())) / dnorm);
     if (seconds) seconds[0] = now();
-    if ((rc = applyLT(r, s)) || (rc = pre(s, z))) return rc;
-    pvec = z;
-    double gamma = dot(s.data(), z.data(), nm);
+    auto notify = [&](int it) -> int {
+        if (!c->monitor) return 0;
+        int rc2 = c->monitor(it, mv.data(), s.data(), misfit[it], c->monitor_user);
+        return rc2 ? c->fail(ERM_E_PARAM, "monitor returned %d", rc2) : 0;
+    };
+    if ((rc = applyLT(r, s)) || (rc = notify(0)) || (rc = pre(s, z))) return rc;
     int done = 0;
-    for (int it = 1; it <= iters; ++it) {
-        if (misfit[it - 1] <= tol || gamma <= 0) break;
-        if ((rc = applyL(pvec, q))) return rc;
-        const double qq = dot(q.data(), q.data(), q.size());
-        if (qq <= 0) break;
-        const double alpha = gamma / qq;
-        for (size_t i = 0; i < nm; ++i) mv[i] += (float)(alpha * pvec[i]);
-        for (size_t i = 0; i < r.size(); ++i) r[i] -= (float)(alpha * q[i]);
-        misfit[it] = (float)(std::sqrt(dot(r.data(), r.data(), r.size())) / dnorm);
-        if (seconds) seconds[it] = now();
-        done = it;
-        if (it == iters || misfit[it] <= tol) break;
-        sprev = s;
-        std::vector<float> zprev = z;
-        if ((rc = applyLT(r, s)) || (rc = pre(s, z))) return rc;
-        double gnew = dot(s.data(), z.data(), nm);
-        double beta;
-        if (precond) {  // flexible (Polak-Ribiere) for a nonlinear preconditioner
-            double cross = dot(sprev.data(), z.data(), nm);
-            beta = std::max(0.0, (gnew - cross) / gamma);
-        } else {
-            beta = gnew / gamma;
+    if (!precond) {
+        // CGLS (Paige & Saunders 1982)
+        pvec = z;
+        double gamma = dot(s.data(), z.data(), nm);
+        for (int it = 1; it <= iters; ++it) {
+            if (misfit[it - 1] <= tol || gamma <= 0) break;
+            if ((rc = applyL(pvec, q))) return rc;
+            const double qq = dot(q.data(), q.data(), q.size());
+            if (qq <= 0) break;
+            const double alpha = gamma / qq;
+            for (size_t i = 0; i < nm; ++i) mv[i] += (float)(alpha * pvec[i]);
+            for (size_t i = 0; i < r.size(); ++i) r[i] -= (float)(alpha * q[i]);
+            misfit[it] = (float)(std::sqrt(dot(r.data(), r.data(), r.size())) / dnorm);
+            if (seconds) seconds[it] = now();
+            done = it;
+            if (it == iters || misfit[it] <= tol) break;
+            if ((rc = applyLT(r, s)) || (rc = notify(it)) || (rc = pre(s, z))) return rc;
+            const double gnew = dot(s.data(), z.data(), nm);
+            const double beta = gnew / gamma;
+            for (size_t i = 0; i < nm; ++i) pvec[i] = (float)(z[i] + beta * pvec[i]);
+            gamma = gnew;
         }
-        for (size_t i = 0; i < nm; ++i) pvec[i] = (float)(z[i] + beta * pvec[i]);
-        gamma = gnew;
+    } else {
+        // Flexible variant for a nonlinear preconditioner (GCR-type, as in
+        // Notay 2000 / Eisenstat-Elman-Schultz 1983 applied to the normal
+        // equations): the new direction p = z is made L^T L-orthogonal to the
+        // last kMem directions through their data-space images q_j = L p_j
+        // (kept mutually orthogonal), then the step is the exact minimiser of
+        // |r - a L p|.  The misfit cannot increase whatever z is, and with
+        // z = L^T r the iterates are those of CGLS.
+        constexpr int kMem = 4;
+        std::vector<std::vector<float>> P, Q;
+        std::vector<double> QQ;
+        for (int it = 1; it <= iters; ++it) {
+            if (misfit[it - 1] <= tol) break;
+            std::vector<float> pn(z), qn(r.size());
+            if ((rc = applyL(pn, qn))) return rc;
+            for (size_t j = 0; j < Q.size(); ++j) {
+                const double b = dot(qn.data(), Q[j].data(), qn.size()) / QQ[j];
+                for (size_t i = 0; i < nm; ++i) pn[i] -= (float)(b * P[j][i]);
+                for (size_t i = 0; i < qn.size(); ++i) qn[i] -= (float)(b * Q[j][i]);
+            }
+            const double qq = dot(qn.data(), qn.data(), qn.size());
+            if (!(qq > 0)) break;
+            const double alpha = dot(r.data(), qn.data(), r.size()) / qq;
+            for (size_t i = 0; i < nm; ++i) mv[i] += (float)(alpha * pn[i]);
+            for (size_t i = 0; i < r.size(); ++i) r[i] -= (float)(alpha * qn[i]);
+            misfit[it] = (float)(std::sqrt(dot(r.data(), r.data(), r.size())) / dnorm);
+            if (seconds) seconds[it] = now();
+            done = it;
+            if ((int)P.size() == kMem) { P.erase(P.begin()); Q.erase(Q.begin()); QQ.erase(QQ.begin()); }
+            P.push_back(std::move(pn));
+            Q.push_back(std::move(qn));
+            QQ.push_back(qq);
+            if (it == iters || misfit[it] <= tol) break;
+            if ((rc = applyLT(r, s)) || (rc = notify(it)) || (rc = pre(s, z))) return rc;
+        }
     }
     std::copy(mv.begin(), mv.end(), m);
     erm2d_set_shot(c, sx0, sz0);

@@ -267,6 +267,8 @@ struct erm_unet {
     hipEvent_t ev[2] = {nullptr, nullptr};
     double last_ms = 0, last_flops = 0, flops_acc = 0;
     std::vector<size_t> act_size;
+    int img_nx = 0, img_nz = 0;            // model image shape bound for erm_unet_precond
+    std::vector<float> img_x, img_y;       // padded [1][2][H][W] staging
 
     ~erm_unet() {
         auto F = [](void* q) { if (q) (void)hipFree(q); };
@@ -621,6 +623,40 @@ int erm_unet_train(erm_unet* u, const float* x, const float* t, int count, int h
     u->last_ms = ms;
     u->last_flops = u->flops_acc;
     return rc;
+}
+
+int erm_unet_bind_model(erm_unet* u, int nx, int nz) {
+    if (!u || nx <= 0 || nz <= 0) { g_uerr = "bad arguments"; return ERM_E_PARAM; }
+    u->img_nx = nx;
+    u->img_nz = nz;
+    const int H = (nx + 7) / 8 * 8, W = (nz + 7) / 8 * 8;
+    u->img_x.assign((size_t)2 * H * W, 0.f);
+    u->img_y.assign((size_t)2 * H * W, 0.f);
+    return 0;
+}
+
+int erm_unet_precond(const float* g, float* z, void* user) {
+    erm_unet* u = (erm_unet*)user;
+    if (!u || !g || !z || u->img_nx <= 0) { g_uerr = "erm_unet_precond: net not bound to a model shape"; return ERM_E_STATE; }
+    const int nx = u->img_nx, nz = u->img_nz, H = (nx + 7) / 8 * 8, W = (nz + 7) / 8 * 8;
+    const size_t mn = (size_t)nx * nz;
+    double ss = 0;
+    for (size_t i = 0; i < 2 * mn; ++i) ss += (double)g[i] * g[i];
+    const double rms = std::sqrt(ss / (2.0 * mn));
+    std::copy(g, g + 3 * mn, z);
+    if (rms == 0) return 0;
+    std::fill(u->img_x.begin(), u->img_x.end(), 0.f);
+    for (int ch = 0; ch < 2; ++ch)
+        for (int i = 0; i < nx; ++i)
+            for (int k = 0; k < nz; ++k)
+                u->img_x[((size_t)ch * H + i) * W + k] = (float)(g[ch * mn + (size_t)i * nz + k] / rms);
+    int rc = erm_unet_forward(u, u->img_x.data(), 1, H, W, u->img_y.data());
+    if (rc) return rc;
+    for (int ch = 0; ch < 2; ++ch)
+        for (int i = 0; i < nx; ++i)
+            for (int k = 0; k < nz; ++k)
+                z[ch * mn + (size_t)i * nz + k] = (float)(u->img_y[((size_t)ch * H + i) * W + k] * rms);
+    return 0;
 }
 
 int erm_unet_last_stats(const erm_unet* u, double* ms, double* flops) {

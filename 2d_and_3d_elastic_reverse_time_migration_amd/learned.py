@@ -1,0 +1,300 @@
+"""Learned image-domain preconditioner for 2D elastic LSRTM.
+
+The workflow has no counterpart in the reference (which ships the LSRTM
+operators only); every number it produces comes from the library:
+
+1. random layered-plus-anomaly models (`random_model`), a smooth background
+   and the relative perturbations (dvp/vp, dvs/vs) the Born operator is
+   linearised in (erm2d.h, inversion_para 2 of the reference);
+2. Born data of each model and plain CG-LSRTM on it (`erm2d_lsrtm_cg`); a CG
+   monitor records, at selected iterations k, the gradient g_k = L^T(d - L m_k)
+   and the model m_k, and the training pair is (g_k, dm - m_k): the update
+   that would take the iterate to the model the noise-free data were made
+   from, which converged LSRTM approaches (d = L dm, so a perfect net makes
+   one step of the flexible CG exact).  Plain CG on the training models is
+   run for a fixed, unconverged number of iterations only to supply the
+   gradients g_k that the preconditioner meets along the way;
+3. 64x64 patches, normalised by the RMS of the gradient image (targets scaled
+   by one global constant so both sides are O(1)), train the U-Net on the GPU
+   (erm_unet_train: BF16-MFMA forward/backward, Adam);
+4. on a held-out model, flexible CG preconditioned by the net
+   (erm_unet_precond) against plain CG: iterations and seconds to a data
+   misfit |L m - d| / |d| <= tol, and the model error of both against the
+   plain converged image.
+
+The test suite runs a reduced version of the same steps
+(tests/test_gpu_learned.py).
+"""
+import json
+import time
+
+import numpy as np
+
+from . import Elastic2D, UNet
+from .params2d import Params2D
+
+
+def _smooth(a, sigma):
+    """Separable Gaussian smoothing with edge replication (FP64)."""
+    a = np.asarray(a, np.float64)
+    r = max(1, int(3 * sigma))
+    k = np.exp(-0.5 * (np.arange(-r, r + 1) / sigma) ** 2)
+    k /= k.sum()
+    for ax in (0, 1):
+        pad = [(0, 0), (0, 0)]
+        pad[ax] = (r, r)
+        ap = np.pad(a, pad, mode="edge")
+        a = np.apply_along_axis(lambda v: np.convolve(v, k, "valid"), ax, ap)
+    return a
+
+
+def random_model(nx, nz, seed, smooth_sigma=6.0, top=6):
+    """Layered model with curved interfaces and Gaussian anomalies.
+
+    Returns (background vp, vs, rho) as float32 [nx][nz] (z fastest) and the
+    relative perturbation dm [3][nx][nz] (dvp/vp, dvs/vs, 0) of the true
+    model against the background; dm is tapered to zero on the top `top`
+    cells (sources and receivers) and on the side and bottom 4 cells."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx)[:, None].astype(np.float64)
+    z = np.arange(nz)[None, :].astype(np.float64)
+    nl = int(rng.integers(3, 7))
+    depths = np.sort(rng.uniform(0.12, 0.95, nl)) * nz
+    vel = 1900.0 + np.cumsum(rng.uniform(80.0, 450.0, nl + 1))
+    vp = np.full((nx, nz), vel[0])
+    for i, d0 in enumerate(depths):
+        slope = rng.uniform(-0.15, 0.15)
+        amp = rng.uniform(0.0, 0.06) * nz
+        lam = rng.uniform(0.5, 2.0) * nx
+        phase = rng.uniform(0, 2 * np.pi)
+        zi = d0 + slope * (x - nx / 2) + amp * np.sin(2 * np.pi * x / lam + phase)
+        vp = np.where(z >= zi, vel[i + 1], vp)
+    for _ in range(int(rng.integers(1, 4))):
+        cx, cz = rng.uniform(0.1, 0.9) * nx, rng.uniform(0.25, 0.9) * nz
+        s = rng.uniform(3.0, 10.0)
+        vp += rng.uniform(-350.0, 350.0) * np.exp(-((x - cx) ** 2 + (z - cz) ** 2) / (2 * s * s))
+    n = _smooth(rng.standard_normal((nx, nz)), 8.0)
+    ratio = np.clip(1.732 * (1.0 + 0.05 * n / n.std()), 1.55, 2.0)
+    vs = vp / ratio
+    bvp, bvs = _smooth(vp, smooth_sigma), _smooth(vs, smooth_sigma)
+    brho = 230.0 * bvp ** 0.25
+    w = np.zeros((nx, nz))
+    w[4:nx - 4, top:nz - 4] = 1.0
+    w = _smooth(w, 2.0) * (w > 0)
+    dm = np.zeros((3, nx, nz))
+    dm[0] = (vp / bvp - 1.0) * w
+    dm[1] = (vs / bvs - 1.0) * w
+    f = lambda a: np.ascontiguousarray(a, np.float32)
+    return f(bvp), f(bvs), f(brho), f(dm)
+
+
+def survey_params(nx, nz, lt, freq=15.0, d=10.0, dt=1.0, b=20, fd_order=8):
+    p = Params2D()
+    p.nx, p.nz, p.dx, p.dz, p.dt, p.freq, p.lt = nx, nz, d, d, dt, freq, lt
+    p.lt_forward = lt
+    p.bu = p.bd = p.bl = p.br = b
+    p.coe_attenuation = 200.0
+    p.fd_order = fd_order
+    p.nsls = 0
+    p.shot_x, p.shot_z = nx // 2, 2
+    p.nrec, p.rec_x0, p.rec_dx, p.rec_z = nx - 2, 1, 1, 2
+    return p
+
+
+def shot_positions(nx, nshots):
+    xs = np.linspace(4, nx - 5, nshots).round().astype(int)
+    return [(int(v), 2) for v in xs]
+
+
+def born_data(g, shots, dm):
+    out = []
+    for sx, sz in shots:
+        g.set_shot(sx, sz)
+        out.append(g.born(dm))
+    return np.stack(out)
+
+
+def patches(img, size=64, stride=32):
+    """[c][nx][nz] -> [n][c][size][size] with the given stride (both axes)."""
+    c, nx, nz = img.shape
+    xs = list(range(0, max(nx - size, 0) + 1, stride))
+    zs = list(range(0, max(nz - size, 0) + 1, stride))
+    return np.stack([img[:, i:i + size, k:k + size] for i in xs for k in zs])
+
+
+def training_pairs(g, shots, dm, iters, keep, pmask=3):
+    """Plain CG on Born data of dm; returns the list of (g_k, dm - m_k) for
+    k in keep (2-channel images) and the CG misfit history."""
+    d = born_data(g, shots, dm)
+    rec = {}
+
+    def mon(it, m, grad, mis):
+        if it in keep:
+            rec[it] = (grad[:2].copy(), m[:2].copy())
+
+    m, mis, sec, it = g.lsrtm_cg(shots, d, iters, pmask=pmask, monitor=mon)
+    pairs = [(rec[k][0], dm[:2] - rec[k][1]) for k in sorted(rec)]
+    return pairs, mis
+
+
+def normalise_pairs(pairs, scale=None):
+    """x = g / rms(g); t = scale * (dm - m_k) / rms(g) with one global scale
+    (the median over pairs of rms(g) / rms(target), unless given)."""
+    rg = np.array([np.sqrt(np.mean(p[0].astype(np.float64) ** 2)) for p in pairs])
+    rt = np.array([np.sqrt(np.mean(p[1].astype(np.float64) ** 2)) for p in pairs])
+    if scale is None:
+        scale = float(np.median(rg / np.maximum(rt, 1e-30)))
+    xs = [(p[0] / r).astype(np.float32) for p, r in zip(pairs, rg)]
+    ts = [(scale * p[1] / r).astype(np.float32) for p, r in zip(pairs, rg)]
+    return xs, ts, scale
+
+
+def make_patch_set(xs, ts, size=64, stride=32, flip=True):
+    X, T = [], []
+    for x, t in zip(xs, ts):
+        px, pt = patches(x, size, stride), patches(t, size, stride)
+        X.append(px)
+        T.append(pt)
+        if flip:  # mirror in x: the survey (evenly spaced shots, full receiver line) is symmetric
+            X.append(px[:, :, ::-1, :])
+            T.append(pt[:, :, ::-1, :])
+    return np.ascontiguousarray(np.concatenate(X), np.float32), np.ascontiguousarray(np.concatenate(T), np.float32)
+
+
+def time_to(mis, sec, tol):
+    """(iterations, seconds) at which the misfit first reaches tol, or (None, None)."""
+    idx = np.nonzero(np.asarray(mis) <= tol)[0]
+    if len(idx) == 0:
+        return None, None
+    return int(idx[0]), float(sec[idx[0]])
+
+
+def generate(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7,
+             keep=(0, 1, 2, 4, 6), workers=4, seed0=1000, log=print):
+    """Training pairs from plain CG on random models (seeds seed0, seed0+1, ...)."""
+    nx, nz = train_shape
+    g = Elastic2D(survey_params(nx, nz, train_lt))
+    keep = set(keep)
+    pairs = []
+    shots = shot_positions(nx, train_shots)
+    for i in range(train_models):
+        bvp, bvs, brho, dm = random_model(nx, nz, seed0 + i)
+        g.load_model(1, bvp, bvs, brho)
+        if i == 0:
+            g.set_workers(workers)
+        pr, mis = training_pairs(g, shots, dm, cg_iters, keep)
+        pairs += pr
+        log(f"train model {i}: plain CG misfit {mis[0]:.3f} -> {mis[-1]:.4f} ({len(pr)} pairs)")
+    g.close()
+    return pairs
+
+
+def train(pairs, channels=(16, 32, 64, 64), epochs=60, batch=32, lr=1e-3, seed=0, log=print):
+    """Normalise, cut patches and train a fresh U-Net; returns (net, stats)."""
+    xs, ts, scale = normalise_pairs(pairs)
+    X, T = make_patch_set(xs, ts)
+    net = UNet(channels, seed=seed)
+    t0 = time.perf_counter()
+    losses = net.train(X, T, batch=batch, epochs=epochs, lr=lr, seed=seed)
+    st = {"patches": int(X.shape[0]), "target_scale": scale, "train_s": time.perf_counter() - t0}
+    ms, flops = net.last_stats()
+    st["train_device_ms"] = ms
+    st["train_mfma_tflops"] = flops / ms / 1e9 if ms > 0 else None
+    st["loss_first_last"] = [float(losses[0]), float(losses[-1])]
+    st["loss_identity"] = float(np.mean((X - T) ** 2))  # loss of the identity map, for scale
+    log(f"trained {X.shape[0]} patches, loss {losses[0]:.4f} -> {losses[-1]:.4f} (identity {st['loss_identity']:.4f}),"
+        f" {st['train_s']:.1f} s, {st['train_mfma_tflops']:.2f} TFLOP/s")
+    return net, st
+
+
+def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, workers=4,
+             test_seed=5000, log=print, every=25, warm=5, flexible=True):
+    """Plain CG against CG preconditioned by `net` on a held-out model."""
+    nx, nz = test_shape
+    g = Elastic2D(survey_params(nx, nz, test_lt))
+    bvp, bvs, brho, dm = random_model(nx, nz, test_seed)
+    g.load_model(1, bvp, bvs, brho)
+    g.set_workers(workers)
+    shots = shot_positions(nx, test_shots)
+    d = born_data(g, shots, dm)
+
+    def progress(name):
+        def mon(it, m, grad, mis):
+            if it % every == 0:
+                log(f"{name} CG iteration {it}: misfit {mis:.4f}")
+        return mon
+
+    res = {}
+    m_plain, mis_p, sec_p, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, monitor=progress("plain"))
+    ip, tp = time_to(mis_p, sec_p, tol)
+    res["plain"] = dict(misfit=[float(v) for v in mis_p], seconds=[float(v) for v in sec_p], iters_to_tol=ip,
+                        s_to_tol=tp)
+    nrm = lambda a: float(np.linalg.norm(a[:2] - dm[:2]) / np.linalg.norm(dm[:2]))
+    res["model_err_vs_true"] = {"plain": nrm(m_plain)}
+    if flexible:
+        m_learn, mis_l, sec_l, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, precond=net,
+                                              monitor=progress("learned"))
+        il, tl = time_to(mis_l, sec_l, tol)
+        res["learned"] = dict(misfit=[float(v) for v in mis_l], seconds=[float(v) for v in sec_l], iters_to_tol=il,
+                              s_to_tol=tl, model_err_vs_plain=float(np.linalg.norm(m_learn[:2] - m_plain[:2])
+                                                                    / np.linalg.norm(m_plain[:2])))
+        res["model_err_vs_true"]["learned"] = nrm(m_learn)
+        if tp and tl:
+            res["time_ratio"] = tl / tp
+        log(f"held-out: plain {ip} it / {tp} s, learned {il} it / {tl} s to misfit {tol}")
+    if warm:
+        # learned warm start: `warm` preconditioned iterations, then plain CG from there
+        m_w, mis_w1, sec_w1, it_w = g.lsrtm_cg(shots, d, warm, pmask=3, tol=tol, precond=net)
+        m_w, mis_w2, sec_w2, _ = g.lsrtm_cg(shots, d, test_iters - it_w, pmask=3, m0=m_w, tol=tol,
+                                            monitor=progress("warm-start"))
+        mis_w = np.concatenate([mis_w1, mis_w2[1:]])
+        sec_w = np.concatenate([sec_w1, sec_w1[-1] + sec_w2[1:]])
+        iw, tw = time_to(mis_w, sec_w, tol)
+        res["warm_start"] = dict(warm_iters=warm, misfit=[float(v) for v in mis_w], seconds=[float(v) for v in sec_w],
+                                 iters_to_tol=iw, s_to_tol=tw)
+        res["model_err_vs_true"]["warm_start"] = nrm(m_w)
+        if tp and tw:
+            res["time_ratio_warm_start"] = tw / tp
+        log(f"held-out: plain {ip} it / {tp} s, warm start ({warm} learned iterations) {iw} it / {tw} s"
+            f" to misfit {tol}")
+    g.close()
+    return res
+
+
+def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7, test_shape=(256, 96),
+        test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, epochs=20, batch=32, lr=1e-3,
+        channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, log=print):
+    """The whole workflow; returns a dict of results."""
+    res = {"config": dict(train_models=train_models, train_shape=train_shape, train_shots=train_shots,
+                          train_lt=train_lt, cg_iters=cg_iters, test_shape=test_shape, test_shots=test_shots,
+                          test_lt=test_lt, tol=tol, epochs=epochs, batch=batch, lr=lr, channels=channels)}
+    t0 = time.perf_counter()
+    pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, workers=workers, seed0=seed0, log=log)
+    res["datagen_s"] = time.perf_counter() - t0
+    net, st = train(pairs, channels, epochs, batch, lr, seed=seed0, log=log)
+    res.update(st)
+    res.update(evaluate(net, test_shape, test_shots, test_lt, test_iters, tol, workers, test_seed, log=log))
+    net.close()
+    return res
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="learned-preconditioner LSRTM workflow")
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--train-models", type=int, default=160)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--test-iters", type=int, default=400)
+    ap.add_argument("--workers", type=int, default=4)
+    a = ap.parse_args(argv)
+    res = run(train_models=a.train_models, epochs=a.epochs, test_iters=a.test_iters, workers=a.workers,
+              log=lambda s: print(s, flush=True))
+    txt = json.dumps(res)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(txt)
+    print(txt)
+
+
+if __name__ == "__main__":
+    main()

@@ -90,6 +90,10 @@ int   erm2d_born(erm2d_ctx* ctx, const float* dm, float* d);
 int   erm2d_adjoint(erm2d_ctx* ctx, const float* r, float* g);
 /* Least-squares RTM by CGLS (conjugate gradients on the normal equations,
  * Paige & Saunders 1982) over nshots shots: minimise sum_s |L_s m - d_s|^2.
+ * With a preconditioner the flexible form is used: each preconditioned
+ * gradient is made L^T L-orthogonal to the last 4 search directions and the
+ * step is the exact line minimiser, so the misfit never increases even for a
+ * nonlinear (learned) preconditioner; with the identity it equals CGLS.
  * The reference ships the operators but no CG loop.
  *   shots   [nshots][2]  source (x, z), model coordinates
  *   dobs    [nshots][2][lt][nrec] observed data
@@ -97,13 +101,28 @@ int   erm2d_adjoint(erm2d_ctx* ctx, const float* r, float* g);
  *   m       [3][nx][nz] initial model in, solution out
  *   misfit  [iters+1]   |L m - d| / |d| before iteration 1 and after each iteration
  *   seconds [iters+1]   wall time at the same points (may be NULL)
- *   precond optional z = P(g) over the model vector (NULL: identity); with a
- *           preconditioner the update is the flexible (Polak-Ribiere) form.
+ *   precond optional z = P(g) over the model vector (NULL: identity).
  * Returns the number of iterations run (stops early when misfit <= tol), or
  * a negative error code. */
 typedef int (*erm2d_precond_fn)(const float* g, float* z, void* user);
 int   erm2d_lsrtm_cg(erm2d_ctx* ctx, int nshots, const int* shots, const float* dobs, int iters, unsigned pmask,
                      float* m, float* misfit, double* seconds, erm2d_precond_fn precond, void* user, float tol);
+
+/* Shot concurrency for erm2d_lsrtm_cg: n contexts in total (this one plus
+ * n-1 workers created here with the same parameters and background model),
+ * each with its own HIP stream and host thread, take the shots round-robin so
+ * that shots too small to fill the GPU run side by side.  n = 1 (default)
+ * runs the shots one after another.  Not part of the reference, which runs
+ * one shot per GPU. */
+int   erm2d_set_workers(erm2d_ctx* ctx, int n);
+/* Optional per-iteration callback of erm2d_lsrtm_cg, called with the current
+ * model m [3][nx][nz], the (masked) gradient g = L^T (d - L m) at that model
+ * and the misfit, before iteration 1 (it = 0) and after each iteration that
+ * is followed by another; a non-zero return aborts the solve.  Used to record
+ * (gradient, remaining update) pairs for training the learned
+ * preconditioner. */
+typedef int (*erm2d_cg_monitor_fn)(int it, const float* m, const float* g, float misfit, void* user);
+int   erm2d_set_cg_monitor(erm2d_ctx* ctx, erm2d_cg_monitor_fn fn, void* user);
 
 /* Timing of the last operator call (HIP events): milliseconds. */
 int   erm2d_last_ms(const erm2d_ctx* ctx, double* ms);

@@ -41,6 +41,14 @@ int   erm_unet_train_step(erm_unet* u, const float* x, const float* t, int n, in
  * (shuffled with `seed`); losses[epochs] receives the mean loss per epoch. */
 int   erm_unet_train(erm_unet* u, const float* x, const float* t, int count, int h, int w, int batch, int epochs,
                      float lr, unsigned long long seed, float* losses);
+/* Preconditioner adaptor for erm2d_lsrtm_cg (erm2d_precond_fn, user = the
+ * net): bind the model image shape [3][nx][nz] (z fastest) once, then
+ * z = s * net(g / s) on the (dvp/vp, dvs/vs) planes, with s the RMS of g over
+ * those planes and the image zero-padded to multiples of 8 (h = x, w = z);
+ * the drho/rho plane passes through.  Degree-1 homogeneous, so CG's line
+ * search sees the same directions for any scaling of g. */
+int   erm_unet_bind_model(erm_unet* u, int nx, int nz);
+int   erm_unet_precond(const float* g, float* z, void* user);
 /* Milliseconds of device time of the last forward / train call and the
  * number of MFMA floating-point operations it issued (2*M*N*K per GEMM,
  * padded shapes), for utilisation reporting. */

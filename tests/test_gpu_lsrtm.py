@@ -84,3 +84,40 @@ def test_cg_matches_dense_cgls(erm):
     assert it_p == 4 and len(calls) == 4
     np.testing.assert_allclose(mis_p, ref[:5], rtol=1e-4)
     g.close()
+
+
+def test_workers_monitor_and_unet_precond(erm):
+    """Shots spread over 3 concurrent contexts give the same CG iterates as one
+    context; the monitor sees every gradient; the U-Net adaptor plugs in as
+    the preconditioner and keeps the misfit decreasing."""
+    p = cases2d.params(nx=40, nz=24, b=10, R=4, L=0, lt=160, freq=25.0, src=(20, 1), rec_dx=1, rec_z=1, nrec=38,
+                       rec_x0=1)
+    vp, vs, rho, _, _ = cases2d.model(p, seed=4, q=False)
+    bg = (cases2d.smooth2(vp), cases2d.smooth2(vs), cases2d.smooth2(rho))
+    rng = np.random.default_rng(2)
+    dm = np.zeros((3, p.nx, p.nz), np.float32)
+    dm[:2, 4:-4, 6:-4] = 0.05 * rng.standard_normal((2, p.nx - 8, p.nz - 10))
+    shots = [(6, 1), (14, 1), (26, 1), (34, 1), (20, 1)]
+    with erm.Elastic2D(p) as g:
+        g.load_model(1, *bg)
+        d = []
+        for sx, sz in shots:
+            g.set_shot(sx, sz)
+            d.append(g.born(dm))
+        d = np.stack(d)
+        m1, mis1, _, it1 = g.lsrtm_cg(shots, d, 5, pmask=3)
+        g.set_workers(3)
+        seen = []
+        m3, mis3, sec3, it3 = g.lsrtm_cg(shots, d, 5, pmask=3,
+                                         monitor=lambda it, m, gr, mis: seen.append((it, m.shape, float(np.abs(gr[2]).max()), mis)))
+        assert it1 == it3 == 5
+        np.testing.assert_allclose(mis3, mis1, rtol=1e-5)
+        np.testing.assert_allclose(m3, m1, rtol=1e-4, atol=1e-4 * np.abs(m1).max())
+        assert [s[0] for s in seen] == [0, 1, 2, 3, 4]
+        assert all(s[1] == (3, p.nx, p.nz) and s[2] == 0.0 for s in seen)  # drho masked
+        np.testing.assert_allclose([s[3] for s in seen], mis3[:5], rtol=1e-6)
+        assert np.all(np.diff(sec3) > 0)
+        with erm.UNet((8, 8, 16, 16), seed=3) as net:
+            mu, misu, _, itu = g.lsrtm_cg(shots, d, 4, pmask=3, precond=net)
+        assert itu == 4 and np.all(np.diff(misu) <= 1e-6) and misu[-1] < 0.9
+        g.set_workers(1)
