@@ -242,20 +242,22 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
         if tp and tl:
             res["time_ratio"] = tl / tp
         log(f"held-out: plain {ip} it / {tp} s, learned {il} it / {tl} s to misfit {tol}")
-    if warm:
-        # learned warm start: `warm` preconditioned iterations, then plain CG from there
-        m_w, mis_w1, sec_w1, it_w = g.lsrtm_cg(shots, d, warm, pmask=3, tol=tol, precond=net)
+    warms = [warm] if isinstance(warm, int) else list(warm or [])
+    for wk in [w for w in warms if w]:
+        # learned warm start: `wk` preconditioned iterations, then plain CG from there
+        m_w, mis_w1, sec_w1, it_w = g.lsrtm_cg(shots, d, wk, pmask=3, tol=tol, precond=net)
         m_w, mis_w2, sec_w2, _ = g.lsrtm_cg(shots, d, test_iters - it_w, pmask=3, m0=m_w, tol=tol,
-                                            monitor=progress("warm-start"))
+                                            monitor=progress(f"warm-start {wk}"))
         mis_w = np.concatenate([mis_w1, mis_w2[1:]])
         sec_w = np.concatenate([sec_w1, sec_w1[-1] + sec_w2[1:]])
         iw, tw = time_to(mis_w, sec_w, tol)
-        res["warm_start"] = dict(warm_iters=warm, misfit=[float(v) for v in mis_w], seconds=[float(v) for v in sec_w],
-                                 iters_to_tol=iw, s_to_tol=tw)
-        res["model_err_vs_true"]["warm_start"] = nrm(m_w)
+        key = "warm_start" if len(warms) == 1 else f"warm_start_{wk}"
+        res[key] = dict(warm_iters=wk, misfit=[float(v) for v in mis_w], seconds=[float(v) for v in sec_w],
+                        iters_to_tol=iw, s_to_tol=tw)
+        res["model_err_vs_true"][key] = nrm(m_w)
         if tp and tw:
-            res["time_ratio_warm_start"] = tw / tp
-        log(f"held-out: plain {ip} it / {tp} s, warm start ({warm} learned iterations) {iw} it / {tw} s"
+            res["time_ratio_" + key] = tw / tp
+        log(f"held-out: plain {ip} it / {tp} s, warm start ({wk} learned iterations) {iw} it / {tw} s"
             f" to misfit {tol}")
     g.close()
     return res
@@ -263,7 +265,7 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
 
 def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7, test_shape=(256, 96),
         test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, epochs=20, batch=32, lr=1e-3,
-        channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, log=print):
+        channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, warm=5, flexible=True, log=print):
     """The whole workflow; returns a dict of results."""
     res = {"config": dict(train_models=train_models, train_shape=train_shape, train_shots=train_shots,
                           train_lt=train_lt, cg_iters=cg_iters, test_shape=test_shape, test_shots=test_shots,
@@ -273,7 +275,8 @@ def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, c
     res["datagen_s"] = time.perf_counter() - t0
     net, st = train(pairs, channels, epochs, batch, lr, seed=seed0, log=log)
     res.update(st)
-    res.update(evaluate(net, test_shape, test_shots, test_lt, test_iters, tol, workers, test_seed, log=log))
+    res.update(evaluate(net, test_shape, test_shots, test_lt, test_iters, tol, workers, test_seed, log=log,
+                        warm=warm, flexible=flexible))
     net.close()
     return res
 
@@ -286,9 +289,12 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--test-iters", type=int, default=400)
     ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--warm", type=int, nargs="*", default=[5],
+                    help="learned warm-start lengths to compare (net iterations before plain CG)")
+    ap.add_argument("--no-flexible", action="store_true", help="skip the every-iteration learned CG run")
     a = ap.parse_args(argv)
     res = run(train_models=a.train_models, epochs=a.epochs, test_iters=a.test_iters, workers=a.workers,
-              log=lambda s: print(s, flush=True))
+              warm=a.warm, flexible=not a.no_flexible, log=lambda s: print(s, flush=True))
     txt = json.dumps(res)
     if a.out:
         with open(a.out, "w") as f:
