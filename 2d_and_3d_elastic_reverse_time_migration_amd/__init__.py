@@ -122,6 +122,20 @@ def _load():
         "erm_unet_bind_model": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "erm_unet_precond": (ctypes.c_int, [fp, fp, vp]),
         "erm2d_set_workers": (ctypes.c_int, [vp, ctypes.c_int]),
+        "erm_batch_last_error": (ctypes.c_char_p, [vp]),
+        "erm_batch_create": (ctypes.c_int, [vp, ctypes.c_int, P(vp)]),
+        "erm_batch_destroy": (ctypes.c_int, [vp]),
+        "erm_batch_load_model": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, fp]),
+        "erm_batch_set_shots": (ctypes.c_int, [vp, vp, ctypes.c_int]),
+        "erm_batch_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "erm_batch_autotune": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                              P(ctypes.c_double)]),
+        "erm_batch_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
+        "erm_batch_set_record": (ctypes.c_int, [vp, ctypes.c_int, fp]),
+        "erm_batch_run": (ctypes.c_int, [vp, fp, fp, fp]),
+        "erm_batch_shot_context": (ctypes.c_int, [vp, ctypes.c_int, P(vp)]),
+        "erm_batch_timing": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
+                                            P(ctypes.c_double), P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]),
         "erm2d_set_cg_monitor": (ctypes.c_int, [vp, vp, vp]),
     })
     for name, (res, args) in sig.items():
@@ -207,10 +221,14 @@ def read_raw(path, shape):
 class RTM3D:
     """One device-resident 3D elastic RTM context (≙ GPUdevice)."""
 
-    def __init__(self, params: Params, device: int = 0):
+    def __init__(self, params: Params, device: int = 0, _borrow=None):
         self.params = params
-        h = ctypes.c_void_p()
-        _check(lib.erm_create(_pp(params), device, ctypes.byref(h)))
+        self._owned = _borrow is None
+        if _borrow is None:
+            h = ctypes.c_void_p()
+            _check(lib.erm_create(_pp(params), device, ctypes.byref(h)))
+        else:
+            h = _borrow
         self._h = h
         nnx, nny, nnz, wl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         lib.erm_get_dims(h, ctypes.byref(nnx), ctypes.byref(nny), ctypes.byref(nnz), ctypes.byref(wl))
@@ -220,7 +238,8 @@ class RTM3D:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib.erm_destroy(self._h)
+            if getattr(self, "_owned", True):
+                lib.erm_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -546,3 +565,93 @@ class UNet:
         ms, fl = ctypes.c_double(), ctypes.c_double()
         _checku(lib.erm_unet_last_stats(self._h, ctypes.byref(ms), ctypes.byref(fl)))
         return ms.value, fl.value
+
+
+class ShotWindow(ctypes.Structure):
+    """erm_shot_window: source and lateral window in full-model cells."""
+    _fields_ = [(n, ctypes.c_int) for n in ("sx", "sy", "sz", "x0", "y0", "wx", "wy")]
+
+
+def _checkb(rc, h=None):
+    if rc != 0:
+        raise ErmError(rc, lib.erm_batch_last_error(h).decode(errors="replace"))
+
+
+class Batch3D:
+    """Batched multi-shot 3D RTM (include/erm_batch.h): all shots' windows
+    step together, one launch per kernel per step, captured in a hipGraph."""
+
+    def __init__(self, params: Params, device: int = 0):
+        self.params = params
+        h = ctypes.c_void_p()
+        _checkb(lib.erm_batch_create(_pp(params), device, ctypes.byref(h)))
+        self._h = h
+        self.windows = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.erm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _c(self, rc):
+        _checkb(rc, self._h)
+
+    def load_model(self, which, vp, vs, rho):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho)]
+        self._c(lib.erm_batch_load_model(self._h, which, *[_fptr(a) for a in arrs]))
+
+    def set_shots(self, windows):
+        """windows: iterable of (sx, sy, sz, x0, y0, wx, wy)."""
+        arr = (ShotWindow * len(windows))(*[ShotWindow(*map(int, w)) for w in windows])
+        self._c(lib.erm_batch_set_shots(self._h, ctypes.cast(arr, ctypes.c_void_p), len(windows)))
+        self.windows = [tuple(map(int, w)) for w in windows]
+
+    def set_launch(self, tile_y, zchunk, grid_blocks=0):
+        self._c(lib.erm_batch_set_launch(self._h, tile_y, zchunk, grid_blocks))
+
+    def autotune(self, steps=4):
+        ty, kc, gr, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._c(lib.erm_batch_autotune(self._h, steps, ctypes.byref(ty), ctypes.byref(kc), ctypes.byref(gr),
+                                       ctypes.byref(ms)))
+        return {"tile_y": ty.value, "zchunk": kc.value, "grid_blocks": gr.value, "ms_per_step": ms.value}
+
+    def use_graph(self, on=True):
+        self._c(lib.erm_batch_use_graph(self._h, 1 if on else 0))
+
+    def set_record(self, shot, rec):
+        self._c(lib.erm_batch_set_record(self._h, shot, _fptr(np.ascontiguousarray(rec, np.float32))))
+
+    def run(self, want_images=True):
+        p = self.params
+        out = [np.empty((p.nx, p.ny, p.nz), np.float32) for _ in range(3)] if want_images else [None] * 3
+        self._c(lib.erm_batch_run(self._h, *[_fptr(o) for o in out]))
+        return out
+
+    def shot(self, s):
+        """RTM3D view (not owned) of shot s's window context."""
+        h = ctypes.c_void_p()
+        self._c(lib.erm_batch_shot_context(self._h, s, ctypes.byref(h)))
+        sx, sy, sz, x0, y0, wx, wy = self.windows[s]
+        q = Params()
+        ctypes.memmove(ctypes.addressof(q), ctypes.addressof(self.params), ctypes.sizeof(Params))
+        q.nx, q.ny = wx, wy
+        rix, riy = max(1, q.receiver_interval_x), max(1, q.receiver_interval_y)
+        q.receiver_num_x, q.receiver_num_y = (wx + rix - 1) // rix, (wy + riy - 1) // riy
+        return RTM3D(q, _borrow=h)
+
+    def timing(self):
+        d = [ctypes.c_double() for _ in range(4)]
+        n = [ctypes.c_int() for _ in range(3)]
+        self._c(lib.erm_batch_timing(self._h, *[ctypes.byref(v) for v in d], *[ctypes.byref(v) for v in n]))
+        return {"ms_model": d[0].value, "ms_source": d[1].value, "ms_receiver": d[2].value,
+                "cells_per_step": d[3].value, "steps_model": n[0].value, "steps_source": n[1].value,
+                "steps_receiver": n[2].value}

@@ -1,0 +1,495 @@
+// batch3d.hip - batched multi-shot 3D RTM (include/erm_batch.h).
+//
+// The reference runs one shot per pass (3D_forward_or_back_together.cu:9-453
+// per shot; per-shot windows and stacking in the 2D code,
+// viscoelastic_equation.cu:2606-2754).  Here the shots of a batch step
+// together: per time step one stress launch and one velocity launch cover the
+// tiles of every window (k_stress_batch / k_velocity_batch over a block
+// table), one launch serves all receivers, and the step is captured once in a
+// hipGraph and replayed with the step index in device memory.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/erm_batch.h"
+#include "common.hpp"
+#include "ctx3d.hpp"
+#include "kernels3d.hpp"
+
+struct erm_batch {
+    erm_params p;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int R = 4;
+    int cus = 256;
+    std::vector<float> model[2][3];  // full-model host copies (vp vs rho), disk order
+    bool have_model[2] = {false, false};
+    std::vector<erm_shot_window> win;
+    std::vector<erm_ctx*> shots;
+    std::vector<char> have_rec;
+    // device tables
+    ShotStep* d_steps[2] = {nullptr, nullptr};
+    int2* d_table = nullptr;
+    size_t table_cap = 0;
+    int total_blocks = 0;
+    RecShot* d_rec = nullptr;
+    int* d_shot_of = nullptr;
+    int total_rec = 0;
+    int* d_it = nullptr;
+    float* d_stack = nullptr;
+    // launch shape
+    int ty = 8, kchunk = 8, grid = 0;
+    bool use_graph = true;
+    bool tables_dirty = true;
+    // timing
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    double ms[3] = {0, 0, 0};
+    int steps[3] = {0, 0, 0};
+    double cells_per_step = 0;
+    std::string err;
+    int fail(int code, const char* fmt, ...) {
+        va_list ap;
+        va_start(ap, fmt);
+        err = erm::vformat(fmt, ap);
+        va_end(ap);
+        return code;
+    }
+};
+
+#define BHC(b, call)                                                                                  \
+    do {                                                                                              \
+        hipError_t e_ = (call);                                                                       \
+        if (e_ != hipSuccess) return (b)->fail(ERM_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+thread_local std::string g_berr;
+
+void free_tables(erm_batch* b) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    F(b->d_steps[0]); F(b->d_steps[1]); F(b->d_table); F(b->d_rec); F(b->d_shot_of);
+    b->d_steps[0] = b->d_steps[1] = nullptr;
+    b->d_table = nullptr;
+    b->d_rec = nullptr;
+    b->d_shot_of = nullptr;
+    b->table_cap = 0;
+}
+
+void free_shots(erm_batch* b) {
+    for (auto* c : b->shots) erm_destroy(c);
+    b->shots.clear();
+    b->have_rec.clear();
+}
+
+int load_window(erm_batch* b, int s, int which) {
+    const erm_shot_window& w = b->win[s];
+    const erm_params& p = b->p;
+    std::vector<float> cut[3];
+    for (int a = 0; a < 3; ++a) {
+        cut[a].resize((size_t)w.wx * w.wy * p.nz);
+        const float* src = b->model[which][a].data();
+        for (int i = 0; i < w.wx; ++i)
+            for (int j = 0; j < w.wy; ++j)
+                std::memcpy(&cut[a][((size_t)i * w.wy + j) * p.nz], &src[((size_t)(w.x0 + i) * p.ny + (w.y0 + j)) * p.nz],
+                            (size_t)p.nz * sizeof(float));
+    }
+    int rc = erm_load_model(b->shots[s], which, cut[0].data(), cut[1].data(), cut[2].data());
+    if (rc) return b->fail(rc, "shot %d: %s", s, erm_last_error(b->shots[s]));
+    return 0;
+}
+
+// Device tables for the current shots and launch shape.
+int build_tables(erm_batch* b) {
+    if (!b->tables_dirty) return 0;
+    BHC(b, hipSetDevice(b->device));
+    const int ns = (int)b->shots.size();
+    std::vector<ShotStep> st[2];
+    std::vector<int2> table;
+    std::vector<RecShot> rs;
+    std::vector<int> shot_of;
+    double cells = 0;
+    for (int s = 0; s < ns; ++s) {
+        erm_ctx* c = b->shots[s];
+        c->ty = b->ty;
+        c->kchunk = std::max(1, std::min(b->kchunk, c->nnz - 2 * c->R));
+        c->xcd_map = 0;
+        const Geo g = make_geo(c);
+        for (int which = 0; which < 2; ++which) {
+            ShotStep d{};
+            d.w = make_wave(c);
+            d.ex = make_excite(c);
+            d.P = c->pml;
+            d.g = g;
+            d.M = c->M[which];
+            d.MU = c->MU[which];
+            d.B = c->BU[which];
+            d.src_k = c->src < 0 ? -1 : (int)(c->src / c->pxy);
+            d.src_col = c->src < 0 ? -1 : (int)(c->src % c->pxy);
+            st[which].push_back(d);
+        }
+        for (int l = 0; l < g.ntiles; ++l) table.push_back(make_int2(s, l));
+        RecShot r{};
+        r.vx = c->f[0]; r.vy = c->f[1]; r.vz = c->f[2];
+        r.idx = c->rec_idx;
+        r.rec = c->rec;
+        r.nr = c->nr;
+        r.first = (int)shot_of.size();
+        r.comp = (long)c->p.lt * c->nr;
+        rs.push_back(r);
+        for (int q = 0; q < c->nr; ++q) shot_of.push_back(s);
+        cells += (double)(c->nnx - 2 * c->R) * (c->nny - 2 * c->R) * (c->nnz - 2 * c->R);
+    }
+    free_tables(b);
+    for (int which = 0; which < 2; ++which) {
+        BHC(b, hipMalloc(&b->d_steps[which], std::max<size_t>(1, st[which].size()) * sizeof(ShotStep)));
+        if (!st[which].empty())
+            BHC(b, hipMemcpy(b->d_steps[which], st[which].data(), st[which].size() * sizeof(ShotStep), hipMemcpyHostToDevice));
+    }
+    BHC(b, hipMalloc(&b->d_table, std::max<size_t>(1, table.size()) * sizeof(int2)));
+    if (!table.empty()) BHC(b, hipMemcpy(b->d_table, table.data(), table.size() * sizeof(int2), hipMemcpyHostToDevice));
+    BHC(b, hipMalloc(&b->d_rec, std::max<size_t>(1, rs.size()) * sizeof(RecShot)));
+    if (!rs.empty()) BHC(b, hipMemcpy(b->d_rec, rs.data(), rs.size() * sizeof(RecShot), hipMemcpyHostToDevice));
+    BHC(b, hipMalloc(&b->d_shot_of, std::max<size_t>(1, shot_of.size()) * sizeof(int)));
+    if (!shot_of.empty()) BHC(b, hipMemcpy(b->d_shot_of, shot_of.data(), shot_of.size() * sizeof(int), hipMemcpyHostToDevice));
+    b->total_blocks = (int)table.size();
+    b->total_rec = (int)shot_of.size();
+    b->cells_per_step = cells;
+    b->tables_dirty = false;
+    return 0;
+}
+
+template <int R, int TY, bool CPML>
+void launch_stress(erm_batch* b, int which, int source_on, const Coef& cf, const float* wavelet, int wl) {
+    const int grid = b->grid > 0 ? std::min(b->grid, b->total_blocks) : b->total_blocks;
+    hipLaunchKernelGGL((k_stress_batch<R, 64, TY, CPML>), dim3(grid), dim3(64, TY), 0, b->stream, b->d_steps[which],
+                       b->d_table, b->total_blocks, cf, wavelet, wl, (const int*)b->d_it, source_on);
+}
+template <int R, int TY, int MODE, bool CPML>
+void launch_velocity(erm_batch* b, int which, const Coef& cf) {
+    const int grid = b->grid > 0 ? std::min(b->grid, b->total_blocks) : b->total_blocks;
+    hipLaunchKernelGGL((k_velocity_batch<R, 64, TY, MODE, CPML>), dim3(grid), dim3(64, TY), 0, b->stream,
+                       b->d_steps[which], b->d_table, b->total_blocks, cf, (const int*)b->d_it);
+}
+template <int R, int TY, bool CPML>
+void launch_step_rt(erm_batch* b, int pass) {
+    erm_ctx* c0 = b->shots[0];
+    const int which = pass == ERM_PASS_MODEL ? 0 : 1;
+    if (pass == ERM_PASS_RECEIVER && b->total_rec > 0)
+        hipLaunchKernelGGL(k_receivers_batch, dim3((b->total_rec + 255) / 256), dim3(256), 0, b->stream, b->d_rec,
+                           b->d_shot_of, b->total_rec, (const int*)b->d_it, 1);
+    launch_stress<R, TY, CPML>(b, which, pass != ERM_PASS_RECEIVER, c0->coef, c0->wavelet, c0->wl);
+    if (pass == ERM_PASS_SOURCE) launch_velocity<R, TY, MODE_EXCITE, CPML>(b, which, c0->coef);
+    else if (pass == ERM_PASS_RECEIVER) launch_velocity<R, TY, MODE_IMAGE, CPML>(b, which, c0->coef);
+    else launch_velocity<R, TY, MODE_PLAIN, CPML>(b, which, c0->coef);
+    if (pass == ERM_PASS_MODEL && b->total_rec > 0)
+        hipLaunchKernelGGL(k_receivers_batch, dim3((b->total_rec + 255) / 256), dim3(256), 0, b->stream, b->d_rec,
+                           b->d_shot_of, b->total_rec, (const int*)b->d_it, 0);
+    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, b->stream, b->d_it, pass == ERM_PASS_RECEIVER ? -1 : 1);
+}
+void launch_step(erm_batch* b, int pass) {
+    const bool cpml = b->shots[0]->cpml;
+    if (b->R == 4) {
+        if (b->ty == 4) { if (cpml) launch_step_rt<4, 4, true>(b, pass); else launch_step_rt<4, 4, false>(b, pass); }
+        else { if (cpml) launch_step_rt<4, 8, true>(b, pass); else launch_step_rt<4, 8, false>(b, pass); }
+    } else {
+        if (b->ty == 4) { if (cpml) launch_step_rt<6, 4, true>(b, pass); else launch_step_rt<6, 4, false>(b, pass); }
+        else { if (cpml) launch_step_rt<6, 8, true>(b, pass); else launch_step_rt<6, 8, false>(b, pass); }
+    }
+}
+
+// Run nsteps of `pass` for all shots starting at it_begin (receiver pass counts down).
+int run_steps(erm_batch* b, int pass, int it_begin, int nsteps, bool graph, float* ms_out) {
+    int rc;
+    if ((rc = build_tables(b))) return rc;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, b->stream, b->d_it, 1L, it_begin);
+    BHC(b, hipGetLastError());
+    BHC(b, hipEventRecord(b->ev[0], b->stream));
+    if (graph && nsteps > 0) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        BHC(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        launch_step(b, pass);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (e != hipSuccess) return b->fail(ERM_E_DEVICE, "graph capture: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) { (void)hipGraphDestroy(g); return b->fail(ERM_E_DEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
+        for (int s = 0; s < nsteps && e == hipSuccess; ++s) e = hipGraphLaunch(ge, b->stream);
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return b->fail(ERM_E_DEVICE, "graph launch: %s", hipGetErrorString(e));
+    } else {
+        for (int s = 0; s < nsteps; ++s) launch_step(b, pass);
+    }
+    BHC(b, hipGetLastError());
+    BHC(b, hipEventRecord(b->ev[1], b->stream));
+    BHC(b, hipEventSynchronize(b->ev[1]));
+    float ms = 0;
+    BHC(b, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    if (ms_out) *ms_out = ms;
+    return 0;
+}
+
+int zero_shots(erm_batch* b, bool wave, bool excite, bool images) {
+    for (size_t s = 0; s < b->shots.size(); ++s) {
+        erm_ctx* c = b->shots[s];
+        int rc = 0;
+        if (wave && (rc = erm_zero_wavefield(c))) return b->fail(rc, "shot %zu: %s", s, erm_last_error(c));
+        if (excite && (rc = erm_zero_excitation(c))) return b->fail(rc, "shot %zu: %s", s, erm_last_error(c));
+        if (images)
+            for (auto* im : c->img)
+                hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(256), 0, c->stream, im, (long)c->ncell, 0.f);
+        BHC(b, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* erm_batch_last_error(const erm_batch* b) { return b ? b->err.c_str() : g_berr.c_str(); }
+
+int erm_batch_create(const erm_params* p, int device, erm_batch** out) {
+    if (!p || !out) { g_berr = "null argument"; return ERM_E_PARAM; }
+    *out = nullptr;
+    const int R = p->fd_order == 8 ? 4 : p->fd_order == 12 ? 6 : -1;
+    if (R < 0) { g_berr = "fd_order must be 8 or 12"; return ERM_E_PARAM; }
+    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0 || p->lt <= 0) { g_berr = "bad grid"; return ERM_E_PARAM; }
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) { g_berr = "no HIP device"; return ERM_E_DEVICE; }
+    if (device < 0 || device >= nd) { g_berr = "device out of range"; return ERM_E_PARAM; }
+    erm_batch* b = new erm_batch();
+    b->p = *p;
+    if (b->p.lt_forward < 0) b->p.lt_forward = 2 * b->p.lt / 3;
+    b->device = device;
+    b->R = R;
+    hipDeviceProp_t prop;
+    if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess ||
+        hipMalloc(&b->d_it, sizeof(int)) != hipSuccess) {
+        g_berr = "device setup failed";
+        erm_batch_destroy(b);
+        return ERM_E_DEVICE;
+    }
+    b->cus = prop.multiProcessorCount;
+    if (p->tile_y == 4) b->ty = 4;
+    if (p->zchunk > 0) b->kchunk = p->zchunk;
+    *out = b;
+    return 0;
+}
+
+int erm_batch_destroy(erm_batch* b) {
+    if (!b) return 0;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    free_shots(b);
+    free_tables(b);
+    if (b->d_it) (void)hipFree(b->d_it);
+    if (b->d_stack) (void)hipFree(b->d_stack);
+    for (auto& e : b->ev) if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int erm_batch_load_model(erm_batch* b, int which, const float* vp, const float* vs, const float* rho) {
+    if (!b || which < 0 || which > 1 || !vp || !vs || !rho) return b ? b->fail(ERM_E_PARAM, "bad arguments") : ERM_E_PARAM;
+    const size_t n = (size_t)b->p.nx * b->p.ny * b->p.nz;
+    b->model[which][0].assign(vp, vp + n);
+    b->model[which][1].assign(vs, vs + n);
+    b->model[which][2].assign(rho, rho + n);
+    b->have_model[which] = true;
+    for (size_t s = 0; s < b->shots.size(); ++s) {
+        int rc = load_window(b, (int)s, which);
+        if (rc) return rc;
+    }
+    b->tables_dirty = true;
+    return 0;
+}
+
+int erm_batch_set_shots(erm_batch* b, const erm_shot_window* shots, int nshots) {
+    if (!b || !shots || nshots <= 0) return b ? b->fail(ERM_E_PARAM, "bad arguments") : ERM_E_PARAM;
+    const erm_params& P = b->p;
+    for (int s = 0; s < nshots; ++s) {
+        const erm_shot_window& w = shots[s];
+        if (w.wx <= 0 || w.wy <= 0 || w.x0 < 0 || w.y0 < 0 || w.x0 + w.wx > P.nx || w.y0 + w.wy > P.ny)
+            return b->fail(ERM_E_PARAM, "shot %d: window outside the model", s);
+        if (w.sx < w.x0 || w.sx >= w.x0 + w.wx || w.sy < w.y0 || w.sy >= w.y0 + w.wy || w.sz < 0 || w.sz >= P.nz)
+            return b->fail(ERM_E_PARAM, "shot %d: source outside its window", s);
+    }
+    BHC(b, hipSetDevice(b->device));
+    free_shots(b);
+    b->win.assign(shots, shots + nshots);
+    for (int s = 0; s < nshots; ++s) {
+        const erm_shot_window& w = shots[s];
+        erm_params q = P;
+        q.nx = w.wx;
+        q.ny = w.wy;
+        q.shot_start_x = w.sx - w.x0; q.shot_interval_x = 0;
+        q.shot_start_y = w.sy - w.y0; q.shot_interval_y = 0;
+        q.shot_start_z = w.sz; q.shot_interval_z = 0;
+        const int rix = std::max(1, P.receiver_interval_x), riy = std::max(1, P.receiver_interval_y);
+        q.receiver_start_x = 0; q.receiver_interval_x = rix; q.receiver_num_x = (w.wx + rix - 1) / rix;
+        q.receiver_start_y = 0; q.receiver_interval_y = riy; q.receiver_num_y = (w.wy + riy - 1) / riy;
+        q.tile_y = b->ty;
+        q.zchunk = b->kchunk;
+        q.xcd_map = 0;
+        erm_ctx* c = nullptr;
+        int rc = erm_create(&q, b->device, &c);
+        if (rc) { free_shots(b); return b->fail(rc, "shot %d: %s", s, erm_last_error(nullptr)); }
+        b->shots.push_back(c);
+        b->have_rec.push_back(0);
+        if ((rc = erm_set_shot(c, 0, 0, 0))) return b->fail(rc, "shot %d: %s", s, erm_last_error(c));
+        for (int which = 0; which < 2; ++which)
+            if (b->have_model[which] && (rc = load_window(b, s, which))) return rc;
+    }
+    b->tables_dirty = true;
+    return 0;
+}
+
+int erm_batch_set_launch(erm_batch* b, int tile_y, int zchunk, int grid_blocks) {
+    if (!b || (tile_y != 4 && tile_y != 8) || zchunk < 1 || grid_blocks < 0)
+        return b ? b->fail(ERM_E_PARAM, "tile_y must be 4 or 8, zchunk >= 1, grid_blocks >= 0") : ERM_E_PARAM;
+    b->ty = tile_y;
+    b->kchunk = zchunk;
+    b->grid = grid_blocks;
+    b->tables_dirty = true;
+    return 0;
+}
+
+int erm_batch_use_graph(erm_batch* b, int on) {
+    if (!b) return ERM_E_PARAM;
+    b->use_graph = on != 0;
+    return 0;
+}
+
+int erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* grid_blocks, double* ms_per_step) {
+    if (!b || steps <= 0) return b ? b->fail(ERM_E_PARAM, "steps must be positive") : ERM_E_PARAM;
+    if (b->shots.empty()) return b->fail(ERM_E_STATE, "no shots");
+    if (!b->have_model[1]) return b->fail(ERM_E_STATE, "smoothed model not loaded");
+    BHC(b, hipSetDevice(b->device));
+    const int ty0 = b->ty, kc0 = b->kchunk, gr0 = b->grid;
+    double best = 1e300;
+    int bt = ty0, bk = kc0, bg = gr0;
+    const int tys[2] = {4, 8};
+    const int kcs[3] = {4, 8, 16};
+    for (int ty : tys)
+        for (int kc : kcs) {
+            // resident workgroups: 16 waves per CU at 4 waves/SIMD
+            const int per_cu = std::max(1, 16 / ty);
+            const int grids[3] = {0, b->cus * per_cu, 2 * b->cus * per_cu};
+            for (int gr : grids) {
+                b->ty = ty; b->kchunk = kc; b->grid = gr;
+                b->tables_dirty = true;
+                int rc;
+                if ((rc = zero_shots(b, true, true, false))) return rc;
+                float ms = 0;
+                if ((rc = run_steps(b, ERM_PASS_SOURCE, 0, 1, false, &ms))) return rc;       // warm-up
+                if ((rc = run_steps(b, ERM_PASS_SOURCE, 1, steps, false, &ms))) return rc;
+                const double per = ms / steps;
+                if (per < best) { best = per; bt = ty; bk = kc; bg = gr; }
+            }
+        }
+    b->ty = bt; b->kchunk = bk; b->grid = bg;
+    b->tables_dirty = true;
+    int rc = zero_shots(b, true, true, false);
+    if (rc) return rc;
+    if (tile_y) *tile_y = bt;
+    if (zchunk) *zchunk = bk;
+    if (grid_blocks) *grid_blocks = bg;
+    if (ms_per_step) *ms_per_step = best;
+    return 0;
+}
+
+int erm_batch_set_record(erm_batch* b, int shot, const float* rec) {
+    if (!b || shot < 0 || shot >= (int)b->shots.size() || !rec) return b ? b->fail(ERM_E_PARAM, "bad arguments") : ERM_E_PARAM;
+    int rc = erm_set_record(b->shots[shot], rec);
+    if (rc) return b->fail(rc, "shot %d: %s", shot, erm_last_error(b->shots[shot]));
+    b->have_rec[shot] = 1;
+    return 0;
+}
+
+int erm_batch_run(erm_batch* b, float* stp, float* spp, float* sps) {
+    if (!b) return ERM_E_PARAM;
+    if (b->shots.empty()) return b->fail(ERM_E_STATE, "no shots");
+    if (!b->have_model[1]) return b->fail(ERM_E_STATE, "smoothed model not loaded");
+    BHC(b, hipSetDevice(b->device));
+    int rc;
+    bool need_model = false;
+    for (char h : b->have_rec) need_model |= !h;
+    b->ms[0] = b->ms[1] = b->ms[2] = 0;
+    b->steps[0] = b->steps[1] = b->steps[2] = 0;
+    float ms = 0;
+    if (need_model) {
+        // shots that already hold a record keep it: model all, then restore is not
+        // possible in place, so require all-or-nothing
+        for (char h : b->have_rec)
+            if (h) return b->fail(ERM_E_STATE, "either every shot has a record or none");
+        if (!b->have_model[0]) return b->fail(ERM_E_STATE, "true model not loaded and no records set");
+        if ((rc = zero_shots(b, true, false, false))) return rc;
+        if ((rc = run_steps(b, ERM_PASS_MODEL, 0, b->p.lt, b->use_graph, &ms))) return rc;
+        b->ms[0] = ms;
+        b->steps[0] = b->p.lt;
+        for (size_t s = 0; s < b->shots.size(); ++s) b->shots[s]->have_record = true;
+    }
+    if ((rc = zero_shots(b, true, true, false))) return rc;
+    if ((rc = run_steps(b, ERM_PASS_SOURCE, 0, b->p.lt_forward, b->use_graph, &ms))) return rc;
+    b->ms[1] = ms;
+    b->steps[1] = b->p.lt_forward;
+    if ((rc = zero_shots(b, true, false, true))) return rc;
+    if ((rc = run_steps(b, ERM_PASS_RECEIVER, b->p.lt - 1, b->p.lt, b->use_graph, &ms))) return rc;
+    b->ms[2] = ms;
+    b->steps[2] = b->p.lt;
+    // stack the window images into the full model
+    const erm_params& P = b->p;
+    const size_t nfull = (size_t)P.nx * P.ny * P.nz;
+    if (!b->d_stack) BHC(b, hipMalloc(&b->d_stack, 3 * nfull * sizeof(float)));
+    hipLaunchKernelGGL(k_fill_f32, dim3(1024), dim3(256), 0, b->stream, b->d_stack, (long)(3 * nfull), 0.f);
+    for (size_t s = 0; s < b->shots.size(); ++s) {
+        erm_ctx* c = b->shots[s];
+        const erm_shot_window& w = b->win[s];
+        const long n = (long)w.wx * w.wy * P.nz;
+        for (int q = 0; q < 3; ++q)
+            hipLaunchKernelGGL(k_stack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, c->img[q], w.wx, w.wy,
+                               P.nz, P.bl, P.bf, P.bu, c->px, c->pxy, b->d_stack + q * nfull, P.nx, P.ny, w.x0, w.y0);
+    }
+    BHC(b, hipGetLastError());
+    float* outs[3] = {stp, spp, sps};
+    std::vector<float> tmp(nfull);
+    for (int q = 0; q < 3; ++q) {
+        if (!outs[q]) continue;
+        BHC(b, hipMemcpyAsync(tmp.data(), b->d_stack + q * nfull, nfull * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+        BHC(b, hipStreamSynchronize(b->stream));
+        for (int k = 0; k < P.nz; ++k)
+            for (int j = 0; j < P.ny; ++j)
+                for (int i = 0; i < P.nx; ++i)
+                    outs[q][((size_t)i * P.ny + j) * P.nz + k] = tmp[((size_t)k * P.ny + j) * P.nx + i];
+    }
+    BHC(b, hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int erm_batch_shot_context(erm_batch* b, int shot, erm_ctx** ctx) {
+    if (!b || !ctx || shot < 0 || shot >= (int)b->shots.size()) return b ? b->fail(ERM_E_PARAM, "bad shot") : ERM_E_PARAM;
+    *ctx = b->shots[shot];
+    return 0;
+}
+
+int erm_batch_timing(const erm_batch* b, double* m0, double* m1, double* m2, double* cells, int* s0, int* s1, int* s2) {
+    if (!b) return ERM_E_PARAM;
+    if (m0) *m0 = b->ms[0];
+    if (m1) *m1 = b->ms[1];
+    if (m2) *m2 = b->ms[2];
+    if (cells) *cells = b->cells_per_step;
+    if (s0) *s0 = b->steps[0];
+    if (s1) *s1 = b->steps[1];
+    if (s2) *s2 = b->steps[2];
+    return 0;
+}
+
+}  // extern "C"

@@ -70,8 +70,8 @@ struct TileId {
     int bx, by, bz;
     bool valid;
 };
-__device__ __forceinline__ TileId tile_of_block(const Geo& g) {
-    int l = (int)blockIdx.x;
+__device__ __forceinline__ TileId tile_of_block(const Geo& g, int blk) {
+    int l = blk;
     if (g.xcd_map) l = (l & 7) * g.per_xcd + (l >> 3);
     TileId t;
     t.valid = l < g.ntiles;
@@ -151,18 +151,20 @@ __device__ __forceinline__ int bldi(rsrc_t r, unsigned vo, unsigned so) {
 // before updating them.
 // HBM per cell-update: reads v(3), rw stress+tp(14), reads M, mu(2) = 76 B.
 // ---------------------------------------------------------------------------
+// Body of one workgroup's tile (block id `blk` of the shot's tiling), shared by
+// the one-shot launch and the batched launch.
 template <int R, int TX, int TY, bool CPML>
-__global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restrict__ Mm,
-                                                   const float* __restrict__ MUm, Geo g, Coef cf,
-                                                   int src_k, int src_col, const float* __restrict__ wavelet,
-                                                   int wl, int it, Pml P) {
+__device__ __forceinline__ void stress_tile(const Wave& w, const float* __restrict__ Mm,
+                                            const float* __restrict__ MUm, const Geo& g, const Coef& cf,
+                                            int src_k, int src_col, const float* __restrict__ wavelet,
+                                            int wl, int it, const Pml& P, int blk) {
     constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
     constexpr int NYH = (2 * R + TY - 1) / TY;
     __shared__ float sx[LY][LX];
     __shared__ float sy[LY][LX];
     __shared__ float sz[LY][LX];
 
-    const TileId tid = tile_of_block(g);
+    const TileId tid = tile_of_block(g, blk);
     if (!tid.valid) return;  // uniform per block
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = tid.bx * TX, j0 = tid.by * TY;
@@ -314,6 +316,14 @@ __global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restri
     }
 }
 
+template <int R, int TX, int TY, bool CPML>
+__global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restrict__ Mm,
+                                                   const float* __restrict__ MUm, Geo g, Coef cf,
+                                                   int src_k, int src_col, const float* __restrict__ wavelet,
+                                                   int wl, int it, Pml P) {
+    stress_tile<R, TX, TY, CPML>(w, Mm, MUm, g, cf, src_k, src_col, wavelet, wl, it, P, (int)blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // k_velocity: fwd_vx_3D + fwd_vy_3D + fwd_vz_3D (:100-104) + fwd_vxp_vzp_3D
 // (:118), with vector S = v - vP left implicit (computed where it is used),
@@ -329,8 +339,8 @@ __global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restri
 // +4 B ex_amp (EXCITE) or ex_time (IMAGE).
 // ---------------------------------------------------------------------------
 template <int R, int TX, int TY, int MODE, bool CPML>
-__global__ __launch_bounds__(TX* TY) void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
-                                                     Coef cf, Excite ex, int it, Pml P) {
+__device__ __forceinline__ void velocity_tile(const Wave& w, const float* __restrict__ Bm, const Geo& g,
+                                              const Coef& cf, const Excite& ex, int it, const Pml& P, int blk) {
     constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
     constexpr int NYH = (2 * R + TY - 1) / TY;
     __shared__ float s_xx[TY][LX];   // x-halo only
@@ -340,7 +350,7 @@ __global__ __launch_bounds__(TX* TY) void k_velocity(Wave w, const float* __rest
     __shared__ float s_yz[LY][TX];   // y-halo only
     __shared__ float s_tp[LY][LX];   // x and y halo
 
-    const TileId tid = tile_of_block(g);
+    const TileId tid = tile_of_block(g, blk);
     if (!tid.valid) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = tid.bx * TX, j0 = tid.by * TY;
@@ -524,11 +534,104 @@ __global__ __launch_bounds__(TX* TY) void k_velocity(Wave w, const float* __rest
     }
 }
 
+template <int R, int TX, int TY, int MODE, bool CPML>
+__global__ __launch_bounds__(TX* TY) void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
+                                                     Coef cf, Excite ex, int it, Pml P) {
+    velocity_tile<R, TX, TY, MODE, CPML>(w, Bm, g, cf, ex, it, P, (int)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: many shots (each its own window grid, of any size) in one
+// launch per kernel.  table[b] = (shot, block of that shot's tiling); the
+// grid either covers the table once or is persistent (fewer workgroups, each
+// striding over the table).  The time step is read from device memory so that
+// one captured step can be replayed (k_step_advance moves it).
+// ---------------------------------------------------------------------------
+struct ShotStep {
+    Wave w;
+    Excite ex;
+    Pml P;
+    Geo g;
+    const float* M;
+    const float* MU;
+    const float* B;
+    int src_k, src_col;
+};
+
+template <int R, int TX, int TY, bool CPML>
+__global__ __launch_bounds__(TX* TY) void k_stress_batch(const ShotStep* __restrict__ shots,
+                                                         const int2* __restrict__ table, int total, Coef cf,
+                                                         const float* __restrict__ wavelet, int wl,
+                                                         const int* __restrict__ itp, int source_on) {
+    const int it = *itp;
+    for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
+        const int2 t = table[b];
+        const ShotStep& s = shots[t.x];
+        stress_tile<R, TX, TY, CPML>(s.w, s.M, s.MU, s.g, cf, source_on ? s.src_k : -1, source_on ? s.src_col : -1,
+                                     wavelet, wl, it, s.P, t.y);
+    }
+}
+
+template <int R, int TX, int TY, int MODE, bool CPML>
+__global__ __launch_bounds__(TX* TY) void k_velocity_batch(const ShotStep* __restrict__ shots,
+                                                           const int2* __restrict__ table, int total, Coef cf,
+                                                           const int* __restrict__ itp) {
+    const int it = *itp;
+    for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
+        const int2 t = table[b];
+        const ShotStep& s = shots[t.x];
+        velocity_tile<R, TX, TY, MODE, CPML>(s.w, s.B, s.g, cf, s.ex, it, s.P, t.y);
+    }
+}
+
+// Receivers of all shots: shot_of[r] names the shot of global receiver r,
+// RecShot holds that shot's fields, record and the first global index.
+struct RecShot {
+    float *vx, *vy, *vz;
+    const long* idx;
+    float* rec;   // [3][lt][nr]
+    int nr, first;
+    long comp;    // lt * nr
+};
+[[maybe_unused]] static __global__ void k_receivers_batch(const RecShot* __restrict__ shots, const int* __restrict__ shot_of, int total,
+                                  const int* __restrict__ itp, int inject) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= total) return;
+    const RecShot& s = shots[shot_of[r]];
+    const int q = r - s.first;
+    const long n = s.idx[q], off = (long)(*itp) * s.nr + q;
+    if (inject) {
+        atomicAdd(&s.vx[n], s.rec[off]);
+        atomicAdd(&s.vy[n], s.rec[s.comp + off]);
+        atomicAdd(&s.vz[n], s.rec[2 * s.comp + off]);
+    } else {
+        s.rec[off] = s.vx[n];
+        s.rec[s.comp + off] = s.vy[n];
+        s.rec[2 * s.comp + off] = s.vz[n];
+    }
+}
+
+// it += delta (one lane; a plain vector store).
+[[maybe_unused]] static __global__ void k_step_advance(int* itp, int delta) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *itp += delta;
+}
+
+// Stack a window's model-region image into the full-model image (x fastest,
+// dense [nz][ny][nx]) at lateral offset (x0, y0); img is the window's padded
+// volume with model origin (ox, oy, oz) and pitches px, pxy.
+[[maybe_unused]] static __global__ void k_stack(const float* __restrict__ img, int wx, int wy, int nz, int ox, int oy, int oz, int px,
+                        long pxy, float* __restrict__ out, int nx, int ny, int x0, int y0) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)wx * wy * nz) return;
+    const int i = (int)(t % wx), j = (int)((t / wx) % wy), k = (int)(t / ((long)wx * wy));
+    out[((long)k * ny + (y0 + j)) * nx + (x0 + i)] += img[(long)(k + oz) * pxy + (long)(j + oy) * px + (i + ox)];
+}
+
 // ---------------------------------------------------------------------------
 // Receivers: write_or_add_shot_3D_surface_three (:127 record, :328 inject).
 // rec layout [3][lt][nr]; idx[r] is the padded-grid index of receiver r.
 // ---------------------------------------------------------------------------
-__global__ void k_record(const float* __restrict__ vx, const float* __restrict__ vy,
+[[maybe_unused]] static __global__ void k_record(const float* __restrict__ vx, const float* __restrict__ vy,
                          const float* __restrict__ vz, const long* __restrict__ idx, int nr,
                          float* __restrict__ rec, long comp_stride, long it_off) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -539,7 +642,7 @@ __global__ void k_record(const float* __restrict__ vx, const float* __restrict__
     rec[2 * comp_stride + it_off + r] = vz[n];
 }
 
-__global__ void k_inject(float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
+[[maybe_unused]] static __global__ void k_inject(float* __restrict__ vx, float* __restrict__ vy, float* __restrict__ vz,
                          const long* __restrict__ idx, int nr, const float* __restrict__ rec,
                          long comp_stride, long it_off) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -552,16 +655,16 @@ __global__ void k_inject(float* __restrict__ vx, float* __restrict__ vy, float* 
 
 // Plain streaming copy (float4), the calibration kernel for HBM counters and
 // the practical bandwidth ceiling: reads and writes n4*16 bytes.
-__global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, long n4) {
+[[maybe_unused]] static __global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, long n4) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n4; t += (long)gridDim.x * blockDim.x)
         dst[t] = src[t];
 }
 
 // Fill kernels (used instead of hipMemsetAsync so the step can be graph-captured).
-__global__ void k_fill_f32(float* __restrict__ p, long n, float v) {
+[[maybe_unused]] static __global__ void k_fill_f32(float* __restrict__ p, long n, float v) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
 }
-__global__ void k_fill_i32(int* __restrict__ p, long n, int v) {
+[[maybe_unused]] static __global__ void k_fill_i32(int* __restrict__ p, long n, int v) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
 }
 

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "ctx3d.hpp"
 #include "kernels3d.hpp"
 
 namespace erm {
@@ -56,103 +57,7 @@ static const double kCoeR4[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5
 
 using namespace erm;
 
-struct erm_ctx {
-    erm_params p;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int R = 6;
-    int nnx = 0, nny = 0, nnz = 0, px = 0;
-    long pxy = 0;
-    size_t ncell = 0;   // allocated floats per volume (nnz*pxy)
-    Coef coef{};
-    float cx = 0, cy = 0, cz = 0;
-    // device buffers
-    float* f[13] = {nullptr};
-    float* M[2] = {nullptr, nullptr};
-    float* MU[2] = {nullptr, nullptr};
-    float* BU[2] = {nullptr, nullptr};
-    bool have_model[2] = {false, false};
-    float* hxyz[3] = {nullptr, nullptr, nullptr};
-    float* wavelet = nullptr;
-    int wl = 0;
-    std::vector<float> wavelet_h;
-    int* ex_time = nullptr;
-    float* ex[5] = {nullptr};   // amp tp vxp vyp vzp
-    float* img[3] = {nullptr};  // tp pp ps
-    float* rec = nullptr;       // [3][lt][nr]
-    long* rec_idx = nullptr;
-    int nr = 0;
-    bool have_record = false;
-    int sxr = 0, syr = 0, szr = 0;  // shot position in model coordinates (sx_real, ...)
-    long src = -1;                  // padded index of the source
-    int tx = 64, ty = 8, kchunk = 32, xcd_map = 0;
-    // CPML (boundary == ERM_BOUNDARY_CPML)
-    bool cpml = false;
-    float* pml_coef = nullptr;          // a,b x {full,half} per axis, packed
-    float* psi[21] = {nullptr};
-    int slab_lo[3] = {0, 0, 0}, slab_hi[3] = {0, 0, 0}, slab_w[3] = {0, 0, 0};
-    Pml pml{};
-    double vmax_loaded = 0.0;
-    double pml_extra_stress = 0.0, pml_extra_velocity = 0.0;  // avg extra bytes per cell-update
-    std::string err;
-    // timing
-    hipEvent_t pass_ev[2] = {nullptr, nullptr};
-    bool profiling = false;
-    std::vector<hipEvent_t> kev;  // 3 per step when profiling
-    double last_pass_ms = 0, last_stress_ms = 0, last_velocity_ms = 0;
-    long last_stress_n = 0, last_velocity_n = 0;
-
-    int fail(int code, const char* fmt, ...) {
-        va_list ap;
-        va_start(ap, fmt);
-        err = vformat(fmt, ap);
-        va_end(ap);
-        return code;
-    }
-};
-
-#define HIPC(ctx, call)                                                                          \
-    do {                                                                                         \
-        hipError_t e_ = (call);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return (ctx)->fail(ERM_E_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
-                               __FILE__, __LINE__);                                              \
-    } while (0)
-
 namespace {
-
-Geo make_geo(const erm_ctx* c) {
-    Geo g;
-    g.nnx = c->nnx; g.nny = c->nny; g.nnz = c->nnz;
-    g.px = c->px; g.pxy = c->pxy;
-    g.cx = c->cx; g.cy = c->cy; g.cz = c->cz;
-    g.hx = c->hxyz[0]; g.hy = c->hxyz[1]; g.hz = c->hxyz[2];
-    g.kchunk = c->kchunk;
-    g.ntx = (c->nnx + 63) / 64;
-    g.nty = (c->nny + c->ty - 1) / c->ty;
-    const int nch = (c->nnz - 2 * c->R + c->kchunk - 1) / c->kchunk;
-    g.ntiles = g.ntx * g.nty * nch;
-    g.xcd_map = c->xcd_map;
-    g.per_xcd = (g.ntiles + 7) / 8;
-    return g;
-}
-
-Wave make_wave(const erm_ctx* c) {
-    Wave w;
-    float* const* f = c->f;
-    w.vx = f[0]; w.vy = f[1]; w.vz = f[2];
-    w.txx = f[3]; w.tyy = f[4]; w.tzz = f[5]; w.txy = f[6]; w.txz = f[7]; w.tyz = f[8];
-    w.tp = f[9]; w.vxp = f[10]; w.vyp = f[11]; w.vzp = f[12];
-    return w;
-}
-
-Excite make_excite(const erm_ctx* c) {
-    Excite e;
-    e.time = c->ex_time;
-    e.amp = c->ex[0]; e.tp = c->ex[1]; e.vxp = c->ex[2]; e.vyp = c->ex[3]; e.vzp = c->ex[4];
-    e.img_tp = c->img[0]; e.img_pp = c->img[1]; e.img_ps = c->img[2];
-    return e;
-}
 
 // 1-cos face profile of coe_attenuation_3d_new2 (3D_zzzzz:284-313), pai = 3.1415926.
 void face_profile(int b, double cA, bool rising, std::vector<double>& d) {
