@@ -74,6 +74,7 @@ def _load():
                                           P(ctypes.c_double), P(ctypes.c_long)]),
         "erm_algorithmic_bytes": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]),
         "erm_synchronize": (ctypes.c_int, [vp]),
+        "erm_autotune": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_double)]),
         "erm_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, P(ctypes.c_double)]),
         "erm_make_model3d": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
@@ -337,6 +338,12 @@ class RTM3D:
 
     def synchronize(self):
         self._c(lib.erm_synchronize(self._h))
+
+    def autotune(self, steps=4):
+        """Pick the fastest tile height / z sweep depth (clobbers the wavefield)."""
+        ty, kc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._c(lib.erm_autotune(self._h, steps, ctypes.byref(ty), ctypes.byref(kc), ctypes.byref(ms)))
+        return {"tile_y": ty.value, "zchunk": kc.value, "ms_per_step": ms.value}
 
 
 def _check2(rc, ctx=None):

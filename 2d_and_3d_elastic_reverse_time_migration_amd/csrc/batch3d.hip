@@ -32,9 +32,10 @@ struct erm_batch {
     std::vector<char> have_rec;
     // device tables
     ShotStep* d_steps[2] = {nullptr, nullptr};
-    int2* d_table = nullptr;
+    int2* d_table = nullptr;      // tiles that touch no C-PML memory slab (plain kernel)
+    int2* d_table_pml = nullptr;  // tiles with C-PML cells (CPML kernel)
     size_t table_cap = 0;
-    int total_blocks = 0;
+    int total_blocks = 0, total_pml = 0;
     RecShot* d_rec = nullptr;
     int* d_shot_of = nullptr;
     int total_rec = 0;
@@ -71,9 +72,10 @@ thread_local std::string g_berr;
 
 void free_tables(erm_batch* b) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(b->d_steps[0]); F(b->d_steps[1]); F(b->d_table); F(b->d_rec); F(b->d_shot_of);
+    F(b->d_steps[0]); F(b->d_steps[1]); F(b->d_table); F(b->d_table_pml); F(b->d_rec); F(b->d_shot_of);
     b->d_steps[0] = b->d_steps[1] = nullptr;
     b->d_table = nullptr;
+    b->d_table_pml = nullptr;
     b->d_rec = nullptr;
     b->d_shot_of = nullptr;
     b->table_cap = 0;
@@ -108,7 +110,7 @@ int build_tables(erm_batch* b) {
     BHC(b, hipSetDevice(b->device));
     const int ns = (int)b->shots.size();
     std::vector<ShotStep> st[2];
-    std::vector<int2> table;
+    std::vector<int2> table, table_pml;
     std::vector<RecShot> rs;
     std::vector<int> shot_of;
     double cells = 0;
@@ -131,7 +133,21 @@ int build_tables(erm_batch* b) {
             d.src_col = c->src < 0 ? -1 : (int)(c->src % c->pxy);
             st[which].push_back(d);
         }
-        for (int l = 0; l < g.ntiles; ++l) table.push_back(make_int2(s, l));
+        // same classification as the one-shot region split: plain kernel where no
+        // updated cell of the tile lies in a memory-variable slab
+        auto inside = [](int a, int z, int n, int lo, int hi) { return a > lo && z < n - hi - 1; };
+        const int R = c->R;
+        for (int l = 0; l < g.ntiles; ++l) {
+            const int bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / (g.ntx * g.nty);
+            const int i0 = std::max(bx * 64, R), i1 = std::min(bx * 64 + 64, c->nnx - R) - 1;
+            const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
+            const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
+            if (i0 > i1 || j0 > j1 || k0 > k1) continue;
+            const bool in = !c->cpml || (c->region_split && inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) &&
+                                         inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) &&
+                                         inside(k0, k1, c->nnz, c->slab_lo[2], c->slab_hi[2]));
+            (in ? table : table_pml).push_back(make_int2(s, l));
+        }
         RecShot r{};
         r.vx = c->f[0]; r.vy = c->f[1]; r.vz = c->f[2];
         r.idx = c->rec_idx;
@@ -151,28 +167,51 @@ int build_tables(erm_batch* b) {
     }
     BHC(b, hipMalloc(&b->d_table, std::max<size_t>(1, table.size()) * sizeof(int2)));
     if (!table.empty()) BHC(b, hipMemcpy(b->d_table, table.data(), table.size() * sizeof(int2), hipMemcpyHostToDevice));
+    BHC(b, hipMalloc(&b->d_table_pml, std::max<size_t>(1, table_pml.size()) * sizeof(int2)));
+    if (!table_pml.empty())
+        BHC(b, hipMemcpy(b->d_table_pml, table_pml.data(), table_pml.size() * sizeof(int2), hipMemcpyHostToDevice));
     BHC(b, hipMalloc(&b->d_rec, std::max<size_t>(1, rs.size()) * sizeof(RecShot)));
     if (!rs.empty()) BHC(b, hipMemcpy(b->d_rec, rs.data(), rs.size() * sizeof(RecShot), hipMemcpyHostToDevice));
     BHC(b, hipMalloc(&b->d_shot_of, std::max<size_t>(1, shot_of.size()) * sizeof(int)));
     if (!shot_of.empty()) BHC(b, hipMemcpy(b->d_shot_of, shot_of.data(), shot_of.size() * sizeof(int), hipMemcpyHostToDevice));
     b->total_blocks = (int)table.size();
+    b->total_pml = (int)table_pml.size();
     b->total_rec = (int)shot_of.size();
     b->cells_per_step = cells;
     b->tables_dirty = false;
     return 0;
 }
 
+// The plain-tile table runs the plain kernel, the C-PML table (empty without
+// CPML) the CPML kernel; a persistent grid is split between the two in
+// proportion to their sizes.
+inline void grids(const erm_batch* b, int* g_in, int* g_pml) {
+    const int tot = b->total_blocks + b->total_pml;
+    if (b->grid <= 0 || b->grid >= tot) { *g_in = b->total_blocks; *g_pml = b->total_pml; return; }
+    *g_pml = b->total_pml ? std::max(1, (int)((long)b->grid * b->total_pml / tot)) : 0;
+    *g_in = b->total_blocks ? std::max(1, b->grid - *g_pml) : 0;
+}
 template <int R, int TY, bool CPML>
 void launch_stress(erm_batch* b, int which, int source_on, const Coef& cf, const float* wavelet, int wl) {
-    const int grid = b->grid > 0 ? std::min(b->grid, b->total_blocks) : b->total_blocks;
-    hipLaunchKernelGGL((k_stress_batch<R, 64, TY, CPML>), dim3(grid), dim3(64, TY), 0, b->stream, b->d_steps[which],
-                       b->d_table, b->total_blocks, cf, wavelet, wl, (const int*)b->d_it, source_on);
+    int gi, gp;
+    grids(b, &gi, &gp);
+    if (gi)
+        hipLaunchKernelGGL((k_stress_batch<R, 64, TY, false>), dim3(gi), dim3(64, TY), 0, b->stream, b->d_steps[which],
+                           b->d_table, b->total_blocks, cf, wavelet, wl, (const int*)b->d_it, source_on);
+    if (CPML && gp)
+        hipLaunchKernelGGL((k_stress_batch<R, 64, TY, true>), dim3(gp), dim3(64, TY), 0, b->stream, b->d_steps[which],
+                           b->d_table_pml, b->total_pml, cf, wavelet, wl, (const int*)b->d_it, source_on);
 }
 template <int R, int TY, int MODE, bool CPML>
 void launch_velocity(erm_batch* b, int which, const Coef& cf) {
-    const int grid = b->grid > 0 ? std::min(b->grid, b->total_blocks) : b->total_blocks;
-    hipLaunchKernelGGL((k_velocity_batch<R, 64, TY, MODE, CPML>), dim3(grid), dim3(64, TY), 0, b->stream,
-                       b->d_steps[which], b->d_table, b->total_blocks, cf, (const int*)b->d_it);
+    int gi, gp;
+    grids(b, &gi, &gp);
+    if (gi)
+        hipLaunchKernelGGL((k_velocity_batch<R, 64, TY, MODE, false>), dim3(gi), dim3(64, TY), 0, b->stream,
+                           b->d_steps[which], b->d_table, b->total_blocks, cf, (const int*)b->d_it);
+    if (CPML && gp)
+        hipLaunchKernelGGL((k_velocity_batch<R, 64, TY, MODE, true>), dim3(gp), dim3(64, TY), 0, b->stream,
+                           b->d_steps[which], b->d_table_pml, b->total_pml, cf, (const int*)b->d_it);
 }
 template <int R, int TY, bool CPML>
 void launch_step_rt(erm_batch* b, int pass) {
@@ -307,6 +346,7 @@ int erm_batch_load_model(erm_batch* b, int which, const float* vp, const float* 
     for (size_t s = 0; s < b->shots.size(); ++s) {
         int rc = load_window(b, (int)s, which);
         if (rc) return rc;
+        if (which == 0) b->have_rec[s] = 0;  // records of the old true model are stale
     }
     b->tables_dirty = true;
     return 0;
@@ -426,8 +466,7 @@ int erm_batch_run(erm_batch* b, float* stp, float* spp, float* sps) {
     b->steps[0] = b->steps[1] = b->steps[2] = 0;
     float ms = 0;
     if (need_model) {
-        // shots that already hold a record keep it: model all, then restore is not
-        // possible in place, so require all-or-nothing
+        // the model pass runs for all shots at once: all or none must hold records
         for (char h : b->have_rec)
             if (h) return b->fail(ERM_E_STATE, "either every shot has a record or none");
         if (!b->have_model[0]) return b->fail(ERM_E_STATE, "true model not loaded and no records set");
@@ -435,7 +474,10 @@ int erm_batch_run(erm_batch* b, float* stp, float* spp, float* sps) {
         if ((rc = run_steps(b, ERM_PASS_MODEL, 0, b->p.lt, b->use_graph, &ms))) return rc;
         b->ms[0] = ms;
         b->steps[0] = b->p.lt;
-        for (size_t s = 0; s < b->shots.size(); ++s) b->shots[s]->have_record = true;
+        for (size_t s = 0; s < b->shots.size(); ++s) {
+            b->shots[s]->have_record = true;
+            b->have_rec[s] = 1;  // kept for later runs until the true model or the shots change
+        }
     }
     if ((rc = zero_shots(b, true, true, false))) return rc;
     if ((rc = run_steps(b, ERM_PASS_SOURCE, 0, b->p.lt_forward, b->use_graph, &ms))) return rc;

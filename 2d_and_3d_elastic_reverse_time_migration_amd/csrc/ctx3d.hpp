@@ -48,6 +48,13 @@ struct erm_ctx {
     float* psi[21] = {nullptr};
     int slab_lo[3] = {0, 0, 0}, slab_hi[3] = {0, 0, 0}, slab_w[3] = {0, 0, 0};
     Pml pml{};
+    // region split of the CPML launches: tiles that touch no memory-variable
+    // slab run the plain kernel, the others the CPML kernel (built for ty/kchunk)
+    int* tiles_in = nullptr;
+    int* tiles_pml = nullptr;
+    int n_in = 0, n_pml = 0;
+    int split_ty = 0, split_kchunk = 0;
+    bool region_split = true;
     double vmax_loaded = 0.0;
     double pml_extra_stress = 0.0, pml_extra_velocity = 0.0;  // avg extra bytes per cell-update
     std::string err;

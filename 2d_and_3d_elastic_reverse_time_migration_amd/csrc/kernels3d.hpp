@@ -320,8 +320,10 @@ template <int R, int TX, int TY, bool CPML>
 __global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restrict__ Mm,
                                                    const float* __restrict__ MUm, Geo g, Coef cf,
                                                    int src_k, int src_col, const float* __restrict__ wavelet,
-                                                   int wl, int it, Pml P) {
-    stress_tile<R, TX, TY, CPML>(w, Mm, MUm, g, cf, src_k, src_col, wavelet, wl, it, P, (int)blockIdx.x);
+                                                   int wl, int it, Pml P, const int* __restrict__ blist) {
+    // blist (optional): tile ids of this launch (region-split launches); else one tile per workgroup
+    const int blk = blist ? blist[blockIdx.x] : (int)blockIdx.x;
+    stress_tile<R, TX, TY, CPML>(w, Mm, MUm, g, cf, src_k, src_col, wavelet, wl, it, P, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -536,8 +538,10 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
 
 template <int R, int TX, int TY, int MODE, bool CPML>
 __global__ __launch_bounds__(TX* TY) void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
-                                                     Coef cf, Excite ex, int it, Pml P) {
-    velocity_tile<R, TX, TY, MODE, CPML>(w, Bm, g, cf, ex, it, P, (int)blockIdx.x);
+                                                     Coef cf, Excite ex, int it, Pml P,
+                                                     const int* __restrict__ blist) {
+    const int blk = blist ? blist[blockIdx.x] : (int)blockIdx.x;
+    velocity_tile<R, TX, TY, MODE, CPML>(w, Bm, g, cf, ex, it, P, blk);
 }
 
 // ---------------------------------------------------------------------------

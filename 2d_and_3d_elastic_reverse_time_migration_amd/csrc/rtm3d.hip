@@ -97,31 +97,87 @@ int dmalloc(erm_ctx* c, T** ptr, size_t n) {
 
 inline dim3 grid_of(const Geo& g) { return dim3(g.xcd_map ? 8 * g.per_xcd : g.ntiles); }
 
+// Tile lists of the CPML region split for the current launch shape: a tile is
+// "interior" when none of its updated cells lies in a memory-variable slab
+// (then the CPML kernel would do exactly what the plain kernel does); tiles
+// with no updated cell are dropped.
+int build_split(erm_ctx* c) {
+    if (c->split_ty == c->ty && c->split_kchunk == c->kchunk && c->tiles_in) return 0;
+    Geo g = make_geo(c);
+    const int R = c->R;
+    std::vector<int> tin, tpml;
+    auto inside = [](int lo_c, int hi_c, int n, int lo, int hi) {  // [lo_c, hi_c] avoids both slabs
+        return lo_c > lo && hi_c < n - hi - 1;
+    };
+    for (int l = 0; l < g.ntiles; ++l) {
+        const int bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / (g.ntx * g.nty);
+        const int i0 = std::max(bx * 64, R), i1 = std::min(bx * 64 + 64, c->nnx - R) - 1;
+        const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
+        const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
+        if (i0 > i1 || j0 > j1 || k0 > k1) continue;
+        const bool in = inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) &&
+                        inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) &&
+                        inside(k0, k1, c->nnz, c->slab_lo[2], c->slab_hi[2]);
+        (in ? tin : tpml).push_back(l);
+    }
+    if (c->tiles_in) (void)hipFree(c->tiles_in);
+    if (c->tiles_pml) (void)hipFree(c->tiles_pml);
+    c->tiles_in = c->tiles_pml = nullptr;
+    if (dmalloc(c, &c->tiles_in, std::max<size_t>(1, tin.size())) ||
+        dmalloc(c, &c->tiles_pml, std::max<size_t>(1, tpml.size())))
+        return ERM_E_NOMEM;
+    if (!tin.empty()) HIPC(c, hipMemcpy(c->tiles_in, tin.data(), tin.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!tpml.empty()) HIPC(c, hipMemcpy(c->tiles_pml, tpml.data(), tpml.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->n_in = (int)tin.size();
+    c->n_pml = (int)tpml.size();
+    c->split_ty = c->ty;
+    c->split_kchunk = c->kchunk;
+    return 0;
+}
+
 template <int R, int TX, int TY>
 void stress_launch(erm_ctx* c, int which, long src, int it) {
     const int src_k = src < 0 ? -1 : (int)(src / c->pxy), src_col = src < 0 ? -1 : (int)(src % c->pxy);
     Geo g = make_geo(c);
     dim3 blk(TX, TY);
-    dim3 grd = grid_of(g);
-    if (c->cpml)
-        hipLaunchKernelGGL((k_stress<R, TX, TY, true>), grd, blk, 0, c->stream, make_wave(c), c->M[which],
-                           c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml);
-    else
-        hipLaunchKernelGGL((k_stress<R, TX, TY, false>), grd, blk, 0, c->stream, make_wave(c), c->M[which],
-                           c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml);
+    if (c->cpml && c->region_split && build_split(c) == 0) {
+        g.xcd_map = 0;
+        if (c->n_in)
+            hipLaunchKernelGGL((k_stress<R, TX, TY, false>), dim3(c->n_in), blk, 0, c->stream, make_wave(c),
+                               c->M[which], c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml,
+                               (const int*)c->tiles_in);
+        if (c->n_pml)
+            hipLaunchKernelGGL((k_stress<R, TX, TY, true>), dim3(c->n_pml), blk, 0, c->stream, make_wave(c),
+                               c->M[which], c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml,
+                               (const int*)c->tiles_pml);
+    } else if (c->cpml) {
+        hipLaunchKernelGGL((k_stress<R, TX, TY, true>), grid_of(g), blk, 0, c->stream, make_wave(c), c->M[which],
+                           c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml, nullptr);
+    } else {
+        hipLaunchKernelGGL((k_stress<R, TX, TY, false>), grid_of(g), blk, 0, c->stream, make_wave(c), c->M[which],
+                           c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml, nullptr);
+    }
 }
 
 template <int R, int TX, int TY, int MODE>
 void velocity_launch(erm_ctx* c, int which, int it) {
     Geo g = make_geo(c);
     dim3 blk(TX, TY);
-    dim3 grd = grid_of(g);
-    if (c->cpml)
-        hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, true>), grd, blk, 0, c->stream, make_wave(c), c->BU[which],
-                           g, c->coef, make_excite(c), it, c->pml);
-    else
-        hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, false>), grd, blk, 0, c->stream, make_wave(c),
-                           c->BU[which], g, c->coef, make_excite(c), it, c->pml);
+    if (c->cpml && c->region_split && build_split(c) == 0) {
+        g.xcd_map = 0;
+        if (c->n_in)
+            hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, false>), dim3(c->n_in), blk, 0, c->stream, make_wave(c),
+                               c->BU[which], g, c->coef, make_excite(c), it, c->pml, (const int*)c->tiles_in);
+        if (c->n_pml)
+            hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, true>), dim3(c->n_pml), blk, 0, c->stream, make_wave(c),
+                               c->BU[which], g, c->coef, make_excite(c), it, c->pml, (const int*)c->tiles_pml);
+    } else if (c->cpml) {
+        hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, true>), grid_of(g), blk, 0, c->stream, make_wave(c),
+                           c->BU[which], g, c->coef, make_excite(c), it, c->pml, nullptr);
+    } else {
+        hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, false>), grid_of(g), blk, 0, c->stream, make_wave(c),
+                           c->BU[which], g, c->coef, make_excite(c), it, c->pml, nullptr);
+    }
 }
 
 // C-PML coefficients (Roden & Gedney 2000; Komatitsch & Martin 2007, kappa = 1):
@@ -254,6 +310,7 @@ void free_all(erm_ctx* c) {
     F(c->rec); F(c->rec_idx);
     F(c->pml_coef);
     for (auto& p : c->psi) F(p);
+    F(c->tiles_in); F(c->tiles_pml);
     for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -408,6 +465,7 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
         c->px = (c->nnx + align - 1) / align * align;  // 128-byte rows: a 64-wide tile row = 2 cache lines
     }
     c->xcd_map = pin->xcd_map == 1 ? 1 : 0;
+    c->region_split = pin->cpml_split != 1;
     c->pxy = (long)c->px * c->nny;
     c->ncell = (size_t)c->pxy * c->nnz;
     for (int m = 0; m <= 6; ++m) c->coef.c[m] = 0.f;
@@ -813,6 +871,40 @@ int erm_algorithmic_bytes(const erm_ctx* c, int pass, double* sb, double* vb) {
 int erm_synchronize(erm_ctx* c) {
     if (!c) return ERM_E_PARAM;
     HIPC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int erm_autotune(erm_ctx* c, int steps, int* tile_y, int* zchunk, double* ms_per_step) {
+    if (!c || steps <= 0) return c ? c->fail(ERM_E_PARAM, "steps must be positive") : ERM_E_PARAM;
+    if (!c->have_model[1]) return c->fail(ERM_E_STATE, "smoothed model not loaded");
+    HIPC(c, hipSetDevice(c->device));
+    const bool prof = c->profiling;
+    c->profiling = false;
+    const int nzi = c->nnz - 2 * c->R;
+    double best = 1e300;
+    int bt = c->ty, bk = c->kchunk;
+    const int tys[2] = {4, 8};
+    const int kcs[4] = {4, 8, 16, 32};
+    int rc = 0;
+    for (int ty : tys)
+        for (int kc : kcs) {
+            if (kc > nzi && kc != kcs[0]) continue;
+            c->ty = ty;
+            c->kchunk = std::min(kc, nzi);
+            if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) break;
+            if ((rc = run_pass(c, ERM_PASS_SOURCE, 0, 1))) break;  // warm-up
+            if ((rc = run_pass(c, ERM_PASS_SOURCE, 1, steps))) break;
+            const double per = c->last_pass_ms / steps;
+            if (per < best) { best = per; bt = c->ty; bk = c->kchunk; }
+        }
+    c->ty = bt;
+    c->kchunk = bk;
+    c->profiling = prof;
+    if (rc) return rc;
+    if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) return rc;
+    if (tile_y) *tile_y = bt;
+    if (zchunk) *zchunk = bk;
+    if (ms_per_step) *ms_per_step = best;
     return 0;
 }
 

@@ -38,8 +38,8 @@ _PARAM_FIELDS = (
                                     "vsp", "vsp_precon", "precon", "precon_z1", "precon_z2",
                                     "fd_order", "boundary", "lt_forward"]]
     + [("cpml_r", ctypes.c_float), ("cpml_vmax", ctypes.c_float)]
-    + [(k, ctypes.c_int) for k in ["tile_x", "tile_y", "zchunk", "pitch_align", "xcd_map"]]
-    + [("reserved", ctypes.c_int * 6)]
+    + [(k, ctypes.c_int) for k in ["tile_x", "tile_y", "zchunk", "pitch_align", "xcd_map", "cpml_split"]]
+    + [("reserved", ctypes.c_int * 5)]
 )
 
 

@@ -93,7 +93,9 @@ typedef struct erm_params {
     int   tile_x, tile_y, zchunk;   /* kernel launch shape (0: autotuned default) */
     int   pitch_align;  /* row pitch multiple in floats (0: 32 = 128 B; 1: dense rows) */
     int   xcd_map;      /* 0: plain workgroup order (default, measured faster); 1: XCD-aware tile mapping */
-    int   reserved[6];
+    int   cpml_split;   /* 0 (default): C-PML steps launch the plain kernel on tiles without C-PML cells and
+                           the C-PML kernel on the rest; 1: the C-PML kernel on every tile (same results) */
+    int   reserved[5];
 } erm_params;
 
 /* field ids for erm_get_field / erm_set_field */
@@ -192,6 +194,12 @@ int   erm_get_timing(erm_ctx* ctx, double* pass_ms, double* stress_ms, long* str
  * for the given pass, as the kernels are built (see the kernel sources). */
 int   erm_algorithmic_bytes(const erm_ctx* ctx, int pass, double* stress_bytes, double* velocity_bytes);
 int   erm_synchronize(erm_ctx* ctx);
+/* Launch-shape autotuner (no reference counterpart; the reference launches
+ * one block per z plane): times `steps` source-pass steps (smoothed model)
+ * for tile heights {4, 8} x z sweep depths {4, 8, 16, 32} and keeps the
+ * fastest for later passes.  Results do not depend on the shape.  Zeroes the
+ * wavefield and the excitation state afterwards. */
+int   erm_autotune(erm_ctx* ctx, int steps, int* tile_y, int* zchunk, double* ms_per_step);
 /* Measurement utility (no reference counterpart): time `reps` launches of a
  * plain float4 streaming copy of `bytes` bytes on `device`; it is the
  * calibration kernel for HBM byte counters and the practical bandwidth

@@ -56,10 +56,12 @@ int   erm_batch_set_launch(erm_batch* b, int tile_y, int zchunk, int grid_blocks
 int   erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* grid_blocks, double* ms_per_step);
 /* 1 (default): replay a captured hipGraph per step; 0: plain launches. */
 int   erm_batch_use_graph(erm_batch* b, int on);
-/* Observed data of shot s ([3][lt][nr_s], see erm_batch_shot_context for nr);
- * shots without a record are modelled in the true model by erm_batch_run. */
+/* Observed data of shot s ([3][lt][nr_s], see erm_batch_shot_context for nr).
+ * When no shot holds a record, erm_batch_run models them all in the true
+ * model first and keeps them for later runs until the true model or the
+ * shots change; records must be set for all shots or for none. */
 int   erm_batch_set_record(erm_batch* b, int shot, const float* rec);
-/* Migrate every shot: model pass (true model) for shots without a record,
+/* Migrate every shot: model pass (true model) when there are no records,
  * source pass with excitation, receiver pass with imaging; then stack.
  * Stacked images over the model region in disk order [nx][ny][nz]; any may be NULL. */
 int   erm_batch_run(erm_batch* b, float* stacked_tp, float* stacked_pp, float* stacked_ps);

@@ -91,3 +91,32 @@ def test_gpu_cpml_matches_fp64_oracle(erm, oracle, R):
     o.step(PASS_MODEL, 0, 30)
     assert cases.rel_l2(g.get_field("vz"), o.get_field("vz")) < 1e-5
     g.close()
+
+
+@pytest.mark.gpu
+def test_gpu_cpml_region_split_and_autotune_are_exact(erm):
+    """The region-split launches (plain kernel on tiles without C-PML cells)
+    and every autotuned launch shape give bit-identical shots to one C-PML
+    kernel over all tiles with the default shape."""
+    p = cases.params(nx=150, ny=40, nz=44, b=12, R=4, lt=80, dt=1.0, freq=25.0, src=(75, 20, 3))
+    p.boundary = 1
+    vp, vs, rho = cases.layered(p, seed=5)
+    sm = [cases.smooth(a, 3) for a in (vp, vs, rho)]
+    out = []
+    for split_off, tune in ((1, False), (0, False), (0, True)):
+        p.cpml_split = split_off
+        with erm.RTM3D(p) as g:
+            g.load_model(0, vp, vs, rho)
+            g.load_model(1, *sm)
+            rec = g.forward_real()
+            if tune:
+                t = g.autotune(steps=2)
+                assert t["tile_y"] in (4, 8) and t["zchunk"] in (4, 8, 16, 32)
+            g.forward_smoothed()
+            g.backward_smoothed()
+            out.append((rec, g.get_images(), g.get_field("ex_time")))
+    for r in out[1:]:
+        np.testing.assert_array_equal(r[0], out[0][0])
+        np.testing.assert_array_equal(r[2], out[0][2])
+        for q in range(3):
+            np.testing.assert_array_equal(r[1][q], out[0][1][q])

@@ -3,14 +3,12 @@
 Each window of a batch is an erm_ctx, so the reference result for shot s is
 the same window migrated alone through the one-shot path (erm_forward_real,
 erm_forward_smoothed, erm_backward_smoothed).  The batched kernels run the
-same per-cell arithmetic, but they are separate instantiations of the tile
-bodies and the compiler may contract a multiply-add differently, so records
-agree to FP32 rounding (rel L2 <= 1e-6; the CPML runs are bit-identical) and
-images agree on the cells whose excitation step agrees (>= 99% of the
-excited cells, rel L2 <= 1e-5 there); the stacked image must equal the sum
-of the window images placed at their offsets; graph replay, plain launches,
-persistent grids and every autotuner candidate give identical images (one
-kernel instantiation per shape, so exactly).
+same per-cell arithmetic (the library is built with expression-level FP
+contraction only, so every instantiation of a tile body rounds alike), so
+records and images must be bit-identical; the stacked image must equal the
+sum of the window images placed at their offsets; graph replay, plain
+launches, persistent grids and every autotuner candidate give identical
+images.
 """
 import numpy as np
 import pytest
@@ -81,17 +79,10 @@ def test_batch_matches_single_shots(erm, boundary, R):
             c.backward_smoothed()
             imgs = c.get_images()
             ext = c.get_field("ex_time")
-        assert np.linalg.norm(per[s][1] - rec) <= 1e-6 * np.linalg.norm(rec)
-        if boundary == 1:
-            np.testing.assert_array_equal(per[s][1], rec)
-        # images over the model region; cells whose excitation step agrees
-        bz, by, bx = p.bu, p.bf, p.bl
-        same = (ext == per[s][2])[bz:bz + p.nz, by:by + w[6], bx:bx + w[5]].transpose(2, 1, 0)
-        excited = (ext >= 0)[bz:bz + p.nz, by:by + w[6], bx:bx + w[5]].transpose(2, 1, 0)
-        assert same[excited].mean() >= 0.99
+        np.testing.assert_array_equal(per[s][1], rec)
+        np.testing.assert_array_equal(per[s][2], ext)
         for q_ in range(3):
-            d = (per[s][0][q_] - imgs[q_])[same]
-            assert np.linalg.norm(d) <= 1e-5 * np.linalg.norm(imgs[q_][same])
+            np.testing.assert_array_equal(per[s][0][q_], imgs[q_])
             sx, sy, sz, x0, y0, wx, wy = w
             expect[q_][x0:x0 + wx, y0:y0 + wy, :] += per[s][0][q_]
         assert np.abs(imgs[1]).max() > 0
