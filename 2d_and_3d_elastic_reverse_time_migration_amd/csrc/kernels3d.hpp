@@ -113,10 +113,11 @@ struct Pml {
 __device__ __forceinline__ int slab_of(int i, int n, int lo, int hi) {
     return i <= lo ? i : (i >= n - hi - 1 ? lo + 1 + i - (n - hi - 1) : -1);
 }
-// e += (psi = b psi + a e) where the profile is non-zero
-__device__ __forceinline__ float pml_upd(float* psi, long q, float a, float b, float e) {
+// e += (psi = b psi + a e) where the profile is non-zero; `old` is psi[q],
+// loaded early (with the plane's other loads) so its latency overlaps theirs
+__device__ __forceinline__ float pml_upd(float* psi, unsigned q, float a, float b, float e, float old) {
     if (a == 0.f) return e;
-    const float v = b * psi[q] + a * e;
+    const float v = b * old + a * e;
     psi[q] = v;
     return e + v;
 }
@@ -192,6 +193,9 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
         l_y[t] = r < R ? r : TY + r;
     }
     const bool is_src = active && col == src_col;
+    // C-PML: x/y slab positions are fixed per column
+    const int sxi = CPML && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
+    const int syi = CPML && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
 
     float c[R + 1];
 #pragma unroll
@@ -241,6 +245,25 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
             stp = bld(rtp, o_c, so);
             M = bld(rM, o_c, so); mu = bld(rU, o_c, so);
         }
+        // C-PML memory variables of this cell, prefetched likewise
+        float ps[9];
+        unsigned pqx = 0, pqy = 0, pqz = 0;  // slab offsets (< 2^32 floats)
+        int szi = -1;
+        if constexpr (CPML) {
+            szi = active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
+            if (sxi >= 0) {
+                pqx = ((unsigned)k * g.nny + j) * P.w[0] + sxi;
+                ps[0] = P.psi[0][pqx]; ps[1] = P.psi[1][pqx]; ps[2] = P.psi[2][pqx];
+            }
+            if (syi >= 0) {
+                pqy = ((unsigned)k * P.w[1] + syi) * g.nnx + i;
+                ps[3] = P.psi[3][pqy]; ps[4] = P.psi[4][pqy]; ps[5] = P.psi[5][pqy];
+            }
+            if (szi >= 0) {
+                pqz = ((unsigned)szi * g.nny + j) * g.nnx + i;
+                ps[6] = P.psi[6][pqz]; ps[7] = P.psi[7][pqz]; ps[8] = P.psi[8][pqz];
+            }
+        }
         __syncthreads();  // previous plane's LDS reads are done
         sx[ty + R][tx + R] = qx[R];
         sy[ty + R][tx + R] = qy[R];
@@ -277,29 +300,23 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
             float e_yx = g.cy * dyvx, e_xy = g.cx * dxvy, e_zx = g.cz * dzvx, e_xz = g.cx * dxvz;
             float e_zy = g.cz * dzvy, e_yz = g.cy * dyvz;
             if constexpr (CPML) {
-                const int sxi = slab_of(i, g.nnx, P.lo[0], P.hi[0]);
                 if (sxi >= 0) {
-                    const long q = ((long)k * g.nny + j) * P.w[0] + sxi;
                     const float af = P.a[0][0][i], bf = P.b[0][0][i], ah = P.a[0][1][i], bh = P.b[0][1][i];
-                    exx = pml_upd(P.psi[0], q, af, bf, exx);
-                    e_xy = pml_upd(P.psi[1], q, ah, bh, e_xy);
-                    e_xz = pml_upd(P.psi[2], q, ah, bh, e_xz);
+                    exx = pml_upd(P.psi[0], pqx, af, bf, exx, ps[0]);
+                    e_xy = pml_upd(P.psi[1], pqx, ah, bh, e_xy, ps[1]);
+                    e_xz = pml_upd(P.psi[2], pqx, ah, bh, e_xz, ps[2]);
                 }
-                const int syi = slab_of(j, g.nny, P.lo[1], P.hi[1]);
                 if (syi >= 0) {
-                    const long q = ((long)k * P.w[1] + syi) * g.nnx + i;
                     const float af = P.a[1][0][j], bf = P.b[1][0][j], ah = P.a[1][1][j], bh = P.b[1][1][j];
-                    eyy = pml_upd(P.psi[3], q, af, bf, eyy);
-                    e_yx = pml_upd(P.psi[4], q, ah, bh, e_yx);
-                    e_yz = pml_upd(P.psi[5], q, ah, bh, e_yz);
+                    eyy = pml_upd(P.psi[3], pqy, af, bf, eyy, ps[3]);
+                    e_yx = pml_upd(P.psi[4], pqy, ah, bh, e_yx, ps[4]);
+                    e_yz = pml_upd(P.psi[5], pqy, ah, bh, e_yz, ps[5]);
                 }
-                const int szi = slab_of(k, g.nnz, P.lo[2], P.hi[2]);
                 if (szi >= 0) {
-                    const long q = ((long)szi * g.nny + j) * g.nnx + i;
                     const float af = P.a[2][0][k], bf = P.b[2][0][k], ah = P.a[2][1][k], bh = P.b[2][1][k];
-                    ezz = pml_upd(P.psi[6], q, af, bf, ezz);
-                    e_zx = pml_upd(P.psi[7], q, ah, bh, e_zx);
-                    e_zy = pml_upd(P.psi[8], q, ah, bh, e_zy);
+                    ezz = pml_upd(P.psi[6], pqz, af, bf, ezz, ps[6]);
+                    e_zx = pml_upd(P.psi[7], pqz, ah, bh, e_zx, ps[7]);
+                    e_zy = pml_upd(P.psi[8], pqz, ah, bh, e_zy, ps[8]);
                 }
             }
             const float h = hxy + g.hz[k];
@@ -381,6 +398,8 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
 #pragma unroll
     for (int m = 1; m <= R; ++m) c[m] = cf.c[m];
     const float hxy = active ? g.hx[ic] + g.hy[jc] : 0.f;
+    const int sxi = CPML && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
+    const int syi = CPML && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
 
     const long base = (long)(kb - R) * g.pxy;
     const long span = ((long)g.nnz * g.pxy - base) * 4;
@@ -431,6 +450,24 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             vxp = bld(rpx, o_c, so); vyp = bld(rpy, o_c, so); vzp = bld(rpz, o_c, so);
             bo = bld(rB, o_c, so);
         }
+        float ps[12];
+        unsigned pqx = 0, pqy = 0, pqz = 0;  // slab offsets (< 2^32 floats)
+        int szi = -1;
+        if constexpr (CPML) {
+            szi = active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
+            if (sxi >= 0) {
+                pqx = ((unsigned)k * g.nny + j) * P.w[0] + sxi;
+                ps[0] = P.psi[9][pqx]; ps[1] = P.psi[10][pqx]; ps[2] = P.psi[11][pqx]; ps[3] = P.psi[12][pqx];
+            }
+            if (syi >= 0) {
+                pqy = ((unsigned)k * P.w[1] + syi) * g.nnx + i;
+                ps[4] = P.psi[13][pqy]; ps[5] = P.psi[14][pqy]; ps[6] = P.psi[15][pqy]; ps[7] = P.psi[16][pqy];
+            }
+            if (szi >= 0) {
+                pqz = ((unsigned)szi * g.nny + j) * g.nnx + i;
+                ps[8] = P.psi[17][pqz]; ps[9] = P.psi[18][pqz]; ps[10] = P.psi[19][pqz]; ps[11] = P.psi[20][pqz];
+            }
+        }
         __syncthreads();
         s_xx[ty][tx + R] = cxx;
         s_yy[ty + R][tx] = cyy;
@@ -473,32 +510,26 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             float gxz = g.cx * d1, gyz = g.cy * d2, gzz = g.cz * d3;
             float px = g.cx * p1, py = g.cy * p2, pz = g.cz * p3;
             if constexpr (CPML) {
-                const int sxi = slab_of(i, g.nnx, P.lo[0], P.hi[0]);
                 if (sxi >= 0) {
-                    const long q = ((long)k * g.nny + j) * P.w[0] + sxi;
                     const float af = P.a[0][0][i], bf = P.b[0][0][i], ah = P.a[0][1][i], bh = P.b[0][1][i];
-                    gxx = pml_upd(P.psi[9], q, ah, bh, gxx);
-                    gxy = pml_upd(P.psi[10], q, af, bf, gxy);
-                    gxz = pml_upd(P.psi[11], q, af, bf, gxz);
-                    px = pml_upd(P.psi[12], q, ah, bh, px);
+                    gxx = pml_upd(P.psi[9], pqx, ah, bh, gxx, ps[0]);
+                    gxy = pml_upd(P.psi[10], pqx, af, bf, gxy, ps[1]);
+                    gxz = pml_upd(P.psi[11], pqx, af, bf, gxz, ps[2]);
+                    px = pml_upd(P.psi[12], pqx, ah, bh, px, ps[3]);
                 }
-                const int syi = slab_of(j, g.nny, P.lo[1], P.hi[1]);
                 if (syi >= 0) {
-                    const long q = ((long)k * P.w[1] + syi) * g.nnx + i;
                     const float af = P.a[1][0][j], bf = P.b[1][0][j], ah = P.a[1][1][j], bh = P.b[1][1][j];
-                    gyx = pml_upd(P.psi[13], q, af, bf, gyx);
-                    gyy = pml_upd(P.psi[14], q, ah, bh, gyy);
-                    gyz = pml_upd(P.psi[15], q, af, bf, gyz);
-                    py = pml_upd(P.psi[16], q, ah, bh, py);
+                    gyx = pml_upd(P.psi[13], pqy, af, bf, gyx, ps[4]);
+                    gyy = pml_upd(P.psi[14], pqy, ah, bh, gyy, ps[5]);
+                    gyz = pml_upd(P.psi[15], pqy, af, bf, gyz, ps[6]);
+                    py = pml_upd(P.psi[16], pqy, ah, bh, py, ps[7]);
                 }
-                const int szi = slab_of(k, g.nnz, P.lo[2], P.hi[2]);
                 if (szi >= 0) {
-                    const long q = ((long)szi * g.nny + j) * g.nnx + i;
                     const float af = P.a[2][0][k], bf = P.b[2][0][k], ah = P.a[2][1][k], bh = P.b[2][1][k];
-                    gzx = pml_upd(P.psi[17], q, af, bf, gzx);
-                    gzy = pml_upd(P.psi[18], q, af, bf, gzy);
-                    gzz = pml_upd(P.psi[19], q, ah, bh, gzz);
-                    pz = pml_upd(P.psi[20], q, ah, bh, pz);
+                    gzx = pml_upd(P.psi[17], pqz, af, bf, gzx, ps[8]);
+                    gzy = pml_upd(P.psi[18], pqz, af, bf, gzy, ps[9]);
+                    gzz = pml_upd(P.psi[19], pqz, ah, bh, gzz, ps[10]);
+                    pz = pml_upd(P.psi[20], pqz, ah, bh, pz, ps[11]);
                 }
             }
             const float h = hxy + g.hz[k];
@@ -536,8 +567,12 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
     }
 }
 
+// The radius-4 C-PML variant is held to 3 waves per SIMD (<= 168 VGPRs): its
+// prefetched memory variables would otherwise push it just past the 3-wave
+// limit (radius 6 would spill, so it keeps its natural 2 waves).
 template <int R, int TX, int TY, int MODE, bool CPML>
-__global__ __launch_bounds__(TX* TY) void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(CPML && R == 4 ? 3 : 1)))
+void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
                                                      Coef cf, Excite ex, int it, Pml P,
                                                      const int* __restrict__ blist) {
     const int blk = blist ? blist[blockIdx.x] : (int)blockIdx.x;
@@ -577,7 +612,8 @@ __global__ __launch_bounds__(TX* TY) void k_stress_batch(const ShotStep* __restr
 }
 
 template <int R, int TX, int TY, int MODE, bool CPML>
-__global__ __launch_bounds__(TX* TY) void k_velocity_batch(const ShotStep* __restrict__ shots,
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(CPML && R == 4 ? 3 : 1)))
+void k_velocity_batch(const ShotStep* __restrict__ shots,
                                                            const int2* __restrict__ table, int total, Coef cf,
                                                            const int* __restrict__ itp) {
     const int it = *itp;
