@@ -75,6 +75,12 @@ def _load():
         "erm_algorithmic_bytes": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]),
         "erm_synchronize": (ctypes.c_int, [vp]),
         "erm_autotune": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_double)]),
+        "erm_strip_bytes": (ctypes.c_int, [vp, P(ctypes.c_double)]),
+        "erm_forward_smoothed_strips": (ctypes.c_int, [vp]),
+        "erm_backward_smoothed_xcorr": (ctypes.c_int, [vp, fp]),
+        "erm_get_xcorr_images": (ctypes.c_int, [vp, fp, fp, fp]),
+        "erm_reconstruct": (ctypes.c_int, [vp, ctypes.c_int]),
+        "erm_get_reconstructed_field": (ctypes.c_int, [vp, ctypes.c_int, fp]),
         "erm_copy_bandwidth": (ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, P(ctypes.c_double)]),
         "erm_make_model3d": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
@@ -338,6 +344,33 @@ class RTM3D:
 
     def synchronize(self):
         self._c(lib.erm_synchronize(self._h))
+
+    def strip_bytes(self):
+        v = ctypes.c_double()
+        self._c(lib.erm_strip_bytes(self._h, ctypes.byref(v)))
+        return v.value
+
+    def forward_smoothed_strips(self):
+        """Source pass that also saves the boundary strips for reconstruction."""
+        self._c(lib.erm_forward_smoothed_strips(self._h))
+
+    def backward_smoothed_xcorr(self, rec=None):
+        """Receiver pass with source reconstruction and cross-correlation imaging."""
+        self._c(lib.erm_backward_smoothed_xcorr(self._h, _fptr(None if rec is None else np.ascontiguousarray(rec, np.float32))))
+
+    def get_xcorr_images(self):
+        p = self.params
+        out = [np.empty((p.nx, p.ny, p.nz), np.float32) for _ in range(3)]
+        self._c(lib.erm_get_xcorr_images(self._h, *[_fptr(o) for o in out]))
+        return out
+
+    def reconstruct(self, it_end):
+        self._c(lib.erm_reconstruct(self._h, it_end))
+
+    def get_reconstructed_field(self, name):
+        out = np.empty(self.shape, np.float32)
+        self._c(lib.erm_get_reconstructed_field(self._h, FIELD_ID[name] if isinstance(name, str) else name, _fptr(out)))
+        return out
 
     def autotune(self, steps=4):
         """Pick the fastest tile height / z sweep depth (clobbers the wavefield)."""

@@ -32,10 +32,11 @@ struct erm_batch {
     std::vector<char> have_rec;
     // device tables
     ShotStep* d_steps[2] = {nullptr, nullptr};
-    int2* d_table = nullptr;      // tiles that touch no C-PML memory slab (plain kernel)
-    int2* d_table_pml = nullptr;  // tiles with C-PML cells (CPML kernel)
-    size_t table_cap = 0;
-    int total_blocks = 0, total_pml = 0;
+    // block tables per C-PML axis group (see build_split in rtm3d.hip): group m
+    // runs the kernel variant that updates the memory variables of the axes in m
+    int2* d_table[8] = {nullptr};
+    int total[8] = {0};
+    int total_blocks = 0;  // all groups
     RecShot* d_rec = nullptr;
     int* d_shot_of = nullptr;
     int total_rec = 0;
@@ -72,13 +73,11 @@ thread_local std::string g_berr;
 
 void free_tables(erm_batch* b) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    F(b->d_steps[0]); F(b->d_steps[1]); F(b->d_table); F(b->d_table_pml); F(b->d_rec); F(b->d_shot_of);
+    F(b->d_steps[0]); F(b->d_steps[1]); F(b->d_rec); F(b->d_shot_of);
+    for (auto& t : b->d_table) { F(t); t = nullptr; }
     b->d_steps[0] = b->d_steps[1] = nullptr;
-    b->d_table = nullptr;
-    b->d_table_pml = nullptr;
     b->d_rec = nullptr;
     b->d_shot_of = nullptr;
-    b->table_cap = 0;
 }
 
 void free_shots(erm_batch* b) {
@@ -110,7 +109,7 @@ int build_tables(erm_batch* b) {
     BHC(b, hipSetDevice(b->device));
     const int ns = (int)b->shots.size();
     std::vector<ShotStep> st[2];
-    std::vector<int2> table, table_pml;
+    std::vector<int2> table[8];
     std::vector<RecShot> rs;
     std::vector<int> shot_of;
     double cells = 0;
@@ -143,10 +142,14 @@ int build_tables(erm_batch* b) {
             const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
             const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
             if (i0 > i1 || j0 > j1 || k0 > k1) continue;
-            const bool in = !c->cpml || (c->region_split && inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) &&
-                                         inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) &&
-                                         inside(k0, k1, c->nnz, c->slab_lo[2], c->slab_hi[2]));
-            (in ? table : table_pml).push_back(make_int2(s, l));
+            int m = 0;
+            if (c->cpml) {
+                m = (inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) ? 0 : 1) |
+                    (inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) ? 0 : 2) |
+                    (inside(k0, k1, c->nnz, c->slab_lo[2], c->slab_hi[2]) ? 0 : 4);
+                if (!c->region_split || m != 0) m = 7;  // two groups: plain and all-axes C-PML
+            }
+            table[m].push_back(make_int2(s, l));
         }
         RecShot r{};
         r.vx = c->f[0]; r.vy = c->f[1]; r.vz = c->f[2];
@@ -165,53 +168,59 @@ int build_tables(erm_batch* b) {
         if (!st[which].empty())
             BHC(b, hipMemcpy(b->d_steps[which], st[which].data(), st[which].size() * sizeof(ShotStep), hipMemcpyHostToDevice));
     }
-    BHC(b, hipMalloc(&b->d_table, std::max<size_t>(1, table.size()) * sizeof(int2)));
-    if (!table.empty()) BHC(b, hipMemcpy(b->d_table, table.data(), table.size() * sizeof(int2), hipMemcpyHostToDevice));
-    BHC(b, hipMalloc(&b->d_table_pml, std::max<size_t>(1, table_pml.size()) * sizeof(int2)));
-    if (!table_pml.empty())
-        BHC(b, hipMemcpy(b->d_table_pml, table_pml.data(), table_pml.size() * sizeof(int2), hipMemcpyHostToDevice));
+    b->total_blocks = 0;
+    for (int m = 0; m < 8; ++m) {
+        b->total[m] = (int)table[m].size();
+        b->total_blocks += b->total[m];
+        if (table[m].empty()) continue;
+        BHC(b, hipMalloc(&b->d_table[m], table[m].size() * sizeof(int2)));
+        BHC(b, hipMemcpy(b->d_table[m], table[m].data(), table[m].size() * sizeof(int2), hipMemcpyHostToDevice));
+    }
     BHC(b, hipMalloc(&b->d_rec, std::max<size_t>(1, rs.size()) * sizeof(RecShot)));
     if (!rs.empty()) BHC(b, hipMemcpy(b->d_rec, rs.data(), rs.size() * sizeof(RecShot), hipMemcpyHostToDevice));
     BHC(b, hipMalloc(&b->d_shot_of, std::max<size_t>(1, shot_of.size()) * sizeof(int)));
     if (!shot_of.empty()) BHC(b, hipMemcpy(b->d_shot_of, shot_of.data(), shot_of.size() * sizeof(int), hipMemcpyHostToDevice));
-    b->total_blocks = (int)table.size();
-    b->total_pml = (int)table_pml.size();
+
     b->total_rec = (int)shot_of.size();
     b->cells_per_step = cells;
     b->tables_dirty = false;
     return 0;
 }
 
-// The plain-tile table runs the plain kernel, the C-PML table (empty without
-// CPML) the CPML kernel; a persistent grid is split between the two in
+// Group 0 (tiles without C-PML cells) runs the plain kernel, group 7 the
+// all-axes C-PML kernel; a persistent grid is split between them in
 // proportion to their sizes.
-inline void grids(const erm_batch* b, int* g_in, int* g_pml) {
-    const int tot = b->total_blocks + b->total_pml;
-    if (b->grid <= 0 || b->grid >= tot) { *g_in = b->total_blocks; *g_pml = b->total_pml; return; }
-    *g_pml = b->total_pml ? std::max(1, (int)((long)b->grid * b->total_pml / tot)) : 0;
-    *g_in = b->total_blocks ? std::max(1, b->grid - *g_pml) : 0;
+inline int group_grid(const erm_batch* b, int m) {
+    if (b->grid <= 0 || b->grid >= b->total_blocks) return b->total[m];
+    return b->total[m] ? std::max(1, (int)((long)b->grid * b->total[m] / b->total_blocks)) : 0;
+}
+template <int R, int TY, int AX>
+void stress_group(erm_batch* b, int which, int source_on, const Coef& cf, const float* wavelet, int wl) {
+    const int gr = group_grid(b, AX);
+    if (gr)
+        hipLaunchKernelGGL((k_stress_batch<R, 64, TY, AX>), dim3(gr), dim3(64, TY), 0, b->stream, b->d_steps[which],
+                           b->d_table[AX], b->total[AX], cf, wavelet, wl, (const int*)b->d_it, source_on);
+}
+template <int R, int TY, int MODE, int AX>
+void velocity_group(erm_batch* b, int which, const Coef& cf) {
+    const int gr = group_grid(b, AX);
+    if (gr)
+        hipLaunchKernelGGL((k_velocity_batch<R, 64, TY, MODE, AX>), dim3(gr), dim3(64, TY), 0, b->stream,
+                           b->d_steps[which], b->d_table[AX], b->total[AX], cf, (const int*)b->d_it);
 }
 template <int R, int TY, bool CPML>
 void launch_stress(erm_batch* b, int which, int source_on, const Coef& cf, const float* wavelet, int wl) {
-    int gi, gp;
-    grids(b, &gi, &gp);
-    if (gi)
-        hipLaunchKernelGGL((k_stress_batch<R, 64, TY, false>), dim3(gi), dim3(64, TY), 0, b->stream, b->d_steps[which],
-                           b->d_table, b->total_blocks, cf, wavelet, wl, (const int*)b->d_it, source_on);
-    if (CPML && gp)
-        hipLaunchKernelGGL((k_stress_batch<R, 64, TY, true>), dim3(gp), dim3(64, TY), 0, b->stream, b->d_steps[which],
-                           b->d_table_pml, b->total_pml, cf, wavelet, wl, (const int*)b->d_it, source_on);
+    stress_group<R, TY, 0>(b, which, source_on, cf, wavelet, wl);
+    if constexpr (CPML) {
+        stress_group<R, TY, 7>(b, which, source_on, cf, wavelet, wl);
+    }
 }
 template <int R, int TY, int MODE, bool CPML>
 void launch_velocity(erm_batch* b, int which, const Coef& cf) {
-    int gi, gp;
-    grids(b, &gi, &gp);
-    if (gi)
-        hipLaunchKernelGGL((k_velocity_batch<R, 64, TY, MODE, false>), dim3(gi), dim3(64, TY), 0, b->stream,
-                           b->d_steps[which], b->d_table, b->total_blocks, cf, (const int*)b->d_it);
-    if (CPML && gp)
-        hipLaunchKernelGGL((k_velocity_batch<R, 64, TY, MODE, true>), dim3(gp), dim3(64, TY), 0, b->stream,
-                           b->d_steps[which], b->d_table_pml, b->total_pml, cf, (const int*)b->d_it);
+    velocity_group<R, TY, MODE, 0>(b, which, cf);
+    if constexpr (CPML) {
+        velocity_group<R, TY, MODE, 7>(b, which, cf);
+    }
 }
 template <int R, int TY, bool CPML>
 void launch_step_rt(erm_batch* b, int pass) {

@@ -48,13 +48,24 @@ struct erm_ctx {
     float* psi[21] = {nullptr};
     int slab_lo[3] = {0, 0, 0}, slab_hi[3] = {0, 0, 0}, slab_w[3] = {0, 0, 0};
     Pml pml{};
-    // region split of the CPML launches: tiles that touch no memory-variable
-    // slab run the plain kernel, the others the CPML kernel (built for ty/kchunk)
-    int* tiles_in = nullptr;
-    int* tiles_pml = nullptr;
-    int n_in = 0, n_pml = 0;
+    // region split of the CPML launches: tiles are grouped by the set of axes
+    // whose memory-variable slabs they touch (bit 0 x, 1 y, 2 z); each group
+    // runs the kernel variant that updates only those memory variables
+    // (group 0: the plain kernel).  Built for the current ty/kchunk.
+    int* tiles[8] = {nullptr};
+    int ntl[8] = {0};
     int split_ty = 0, split_kchunk = 0;
+    bool split_built = false;
     bool region_split = true;
+    // source-wavefield reconstruction from boundary strips
+    float* fr[13] = {nullptr};       // reconstructed source wavefield
+    int* strip_idx = nullptr;        // padded indices of the face slabs around the model region
+    int n_strip = 0;
+    float* strips = nullptr;         // [steps][10][n_strip]: sigma+tp (7) after the stress update, v (3) after the velocity update
+    int strip_steps = 0;
+    bool have_strips = false;
+    float* wavelet_neg = nullptr;    // -wavelet, for the time-reversed source
+    float* ximg[3] = {nullptr};      // cross-correlation images tp, pp, ps (padded grid)
     double vmax_loaded = 0.0;
     double pml_extra_stress = 0.0, pml_extra_velocity = 0.0;  // avg extra bytes per cell-update
     std::string err;

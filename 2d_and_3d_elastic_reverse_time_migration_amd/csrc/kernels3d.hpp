@@ -154,7 +154,7 @@ __device__ __forceinline__ int bldi(rsrc_t r, unsigned vo, unsigned so) {
 // ---------------------------------------------------------------------------
 // Body of one workgroup's tile (block id `blk` of the shot's tiling), shared by
 // the one-shot launch and the batched launch.
-template <int R, int TX, int TY, bool CPML>
+template <int R, int TX, int TY, int AX>
 __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restrict__ Mm,
                                             const float* __restrict__ MUm, const Geo& g, const Coef& cf,
                                             int src_k, int src_col, const float* __restrict__ wavelet,
@@ -194,8 +194,8 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
     }
     const bool is_src = active && col == src_col;
     // C-PML: x/y slab positions are fixed per column
-    const int sxi = CPML && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
-    const int syi = CPML && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
+    const int sxi = (AX & 1) && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
+    const int syi = (AX & 2) && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
 
     float c[R + 1];
 #pragma unroll
@@ -249,17 +249,17 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
         float ps[9];
         unsigned pqx = 0, pqy = 0, pqz = 0;  // slab offsets (< 2^32 floats)
         int szi = -1;
-        if constexpr (CPML) {
-            szi = active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
-            if (sxi >= 0) {
+        if constexpr (AX != 0) {
+            szi = (AX & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
+            if ((AX & 1) && sxi >= 0) {
                 pqx = ((unsigned)k * g.nny + j) * P.w[0] + sxi;
                 ps[0] = P.psi[0][pqx]; ps[1] = P.psi[1][pqx]; ps[2] = P.psi[2][pqx];
             }
-            if (syi >= 0) {
+            if ((AX & 2) && syi >= 0) {
                 pqy = ((unsigned)k * P.w[1] + syi) * g.nnx + i;
                 ps[3] = P.psi[3][pqy]; ps[4] = P.psi[4][pqy]; ps[5] = P.psi[5][pqy];
             }
-            if (szi >= 0) {
+            if ((AX & 4) && szi >= 0) {
                 pqz = ((unsigned)szi * g.nny + j) * g.nnx + i;
                 ps[6] = P.psi[6][pqz]; ps[7] = P.psi[7][pqz]; ps[8] = P.psi[8][pqz];
             }
@@ -299,20 +299,20 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
             float exx = g.cx * dxvx, eyy = g.cy * dyvy, ezz = g.cz * dzvz;
             float e_yx = g.cy * dyvx, e_xy = g.cx * dxvy, e_zx = g.cz * dzvx, e_xz = g.cx * dxvz;
             float e_zy = g.cz * dzvy, e_yz = g.cy * dyvz;
-            if constexpr (CPML) {
-                if (sxi >= 0) {
+            if constexpr (AX != 0) {
+                if ((AX & 1) && sxi >= 0) {
                     const float af = P.a[0][0][i], bf = P.b[0][0][i], ah = P.a[0][1][i], bh = P.b[0][1][i];
                     exx = pml_upd(P.psi[0], pqx, af, bf, exx, ps[0]);
                     e_xy = pml_upd(P.psi[1], pqx, ah, bh, e_xy, ps[1]);
                     e_xz = pml_upd(P.psi[2], pqx, ah, bh, e_xz, ps[2]);
                 }
-                if (syi >= 0) {
+                if ((AX & 2) && syi >= 0) {
                     const float af = P.a[1][0][j], bf = P.b[1][0][j], ah = P.a[1][1][j], bh = P.b[1][1][j];
                     eyy = pml_upd(P.psi[3], pqy, af, bf, eyy, ps[3]);
                     e_yx = pml_upd(P.psi[4], pqy, ah, bh, e_yx, ps[4]);
                     e_yz = pml_upd(P.psi[5], pqy, ah, bh, e_yz, ps[5]);
                 }
-                if (szi >= 0) {
+                if ((AX & 4) && szi >= 0) {
                     const float af = P.a[2][0][k], bf = P.b[2][0][k], ah = P.a[2][1][k], bh = P.b[2][1][k];
                     ezz = pml_upd(P.psi[6], pqz, af, bf, ezz, ps[6]);
                     e_zx = pml_upd(P.psi[7], pqz, ah, bh, e_zx, ps[7]);
@@ -333,14 +333,19 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
     }
 }
 
-template <int R, int TX, int TY, bool CPML>
-__global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restrict__ Mm,
+// AX: C-PML axis group of the tiles this launch covers (bit 0 x, 1 y, 2 z;
+// 0: no memory variables).  Radius-4 occupancy targets: 4 waves per SIMD for
+// the single-axis groups (they fit 128 VGPRs without spilling), 3 for the
+// multi-axis groups.
+template <int R, int TX, int TY, int AX>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(R != 4 || AX == 0 ? 1 : (AX == 1 || AX == 2 || AX == 4) ? 4 : 3)))
+void k_stress(Wave w, const float* __restrict__ Mm,
                                                    const float* __restrict__ MUm, Geo g, Coef cf,
                                                    int src_k, int src_col, const float* __restrict__ wavelet,
                                                    int wl, int it, Pml P, const int* __restrict__ blist) {
     // blist (optional): tile ids of this launch (region-split launches); else one tile per workgroup
     const int blk = blist ? blist[blockIdx.x] : (int)blockIdx.x;
-    stress_tile<R, TX, TY, CPML>(w, Mm, MUm, g, cf, src_k, src_col, wavelet, wl, it, P, blk);
+    stress_tile<R, TX, TY, AX>(w, Mm, MUm, g, cf, src_k, src_col, wavelet, wl, it, P, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -357,7 +362,7 @@ __global__ __launch_bounds__(TX* TY) void k_stress(Wave w, const float* __restri
 // HBM per cell-update: reads stress+tp(7), rw v+vP(12), reads 1/rho(1) = 80 B,
 // +4 B ex_amp (EXCITE) or ex_time (IMAGE).
 // ---------------------------------------------------------------------------
-template <int R, int TX, int TY, int MODE, bool CPML>
+template <int R, int TX, int TY, int MODE, int AX>
 __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __restrict__ Bm, const Geo& g,
                                               const Coef& cf, const Excite& ex, int it, const Pml& P, int blk) {
     constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
@@ -398,8 +403,8 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
 #pragma unroll
     for (int m = 1; m <= R; ++m) c[m] = cf.c[m];
     const float hxy = active ? g.hx[ic] + g.hy[jc] : 0.f;
-    const int sxi = CPML && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
-    const int syi = CPML && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
+    const int sxi = (AX & 1) && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
+    const int syi = (AX & 2) && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
 
     const long base = (long)(kb - R) * g.pxy;
     const long span = ((long)g.nnz * g.pxy - base) * 4;
@@ -453,17 +458,17 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
         float ps[12];
         unsigned pqx = 0, pqy = 0, pqz = 0;  // slab offsets (< 2^32 floats)
         int szi = -1;
-        if constexpr (CPML) {
-            szi = active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
-            if (sxi >= 0) {
+        if constexpr (AX != 0) {
+            szi = (AX & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
+            if ((AX & 1) && sxi >= 0) {
                 pqx = ((unsigned)k * g.nny + j) * P.w[0] + sxi;
                 ps[0] = P.psi[9][pqx]; ps[1] = P.psi[10][pqx]; ps[2] = P.psi[11][pqx]; ps[3] = P.psi[12][pqx];
             }
-            if (syi >= 0) {
+            if ((AX & 2) && syi >= 0) {
                 pqy = ((unsigned)k * P.w[1] + syi) * g.nnx + i;
                 ps[4] = P.psi[13][pqy]; ps[5] = P.psi[14][pqy]; ps[6] = P.psi[15][pqy]; ps[7] = P.psi[16][pqy];
             }
-            if (szi >= 0) {
+            if ((AX & 4) && szi >= 0) {
                 pqz = ((unsigned)szi * g.nny + j) * g.nnx + i;
                 ps[8] = P.psi[17][pqz]; ps[9] = P.psi[18][pqz]; ps[10] = P.psi[19][pqz]; ps[11] = P.psi[20][pqz];
             }
@@ -509,22 +514,22 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             float gxy = g.cx * b1, gyy = g.cy * b2, gzy = g.cz * b3;
             float gxz = g.cx * d1, gyz = g.cy * d2, gzz = g.cz * d3;
             float px = g.cx * p1, py = g.cy * p2, pz = g.cz * p3;
-            if constexpr (CPML) {
-                if (sxi >= 0) {
+            if constexpr (AX != 0) {
+                if ((AX & 1) && sxi >= 0) {
                     const float af = P.a[0][0][i], bf = P.b[0][0][i], ah = P.a[0][1][i], bh = P.b[0][1][i];
                     gxx = pml_upd(P.psi[9], pqx, ah, bh, gxx, ps[0]);
                     gxy = pml_upd(P.psi[10], pqx, af, bf, gxy, ps[1]);
                     gxz = pml_upd(P.psi[11], pqx, af, bf, gxz, ps[2]);
                     px = pml_upd(P.psi[12], pqx, ah, bh, px, ps[3]);
                 }
-                if (syi >= 0) {
+                if ((AX & 2) && syi >= 0) {
                     const float af = P.a[1][0][j], bf = P.b[1][0][j], ah = P.a[1][1][j], bh = P.b[1][1][j];
                     gyx = pml_upd(P.psi[13], pqy, af, bf, gyx, ps[4]);
                     gyy = pml_upd(P.psi[14], pqy, ah, bh, gyy, ps[5]);
                     gyz = pml_upd(P.psi[15], pqy, af, bf, gyz, ps[6]);
                     py = pml_upd(P.psi[16], pqy, ah, bh, py, ps[7]);
                 }
-                if (szi >= 0) {
+                if ((AX & 4) && szi >= 0) {
                     const float af = P.a[2][0][k], bf = P.b[2][0][k], ah = P.a[2][1][k], bh = P.b[2][1][k];
                     gzx = pml_upd(P.psi[17], pqz, af, bf, gzx, ps[8]);
                     gzy = pml_upd(P.psi[18], pqz, af, bf, gzy, ps[9]);
@@ -567,16 +572,17 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
     }
 }
 
-// The radius-4 C-PML variant is held to 3 waves per SIMD (<= 168 VGPRs): its
-// prefetched memory variables would otherwise push it just past the 3-wave
-// limit (radius 6 would spill, so it keeps its natural 2 waves).
-template <int R, int TX, int TY, int MODE, bool CPML>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(CPML && R == 4 ? 3 : 1)))
+// Occupancy target of the radius-4 C-PML velocity variants: 3 waves per SIMD
+// (<= 168 VGPRs; the prefetched memory variables would push the all-axes
+// group just past it, and 4 waves would spill).  Radius 6 keeps its natural
+// occupancy (forcing it would spill).
+template <int R, int TX, int TY, int MODE, int AX>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(R != 4 || AX == 0 ? 1 : 3)))
 void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
                                                      Coef cf, Excite ex, int it, Pml P,
                                                      const int* __restrict__ blist) {
     const int blk = blist ? blist[blockIdx.x] : (int)blockIdx.x;
-    velocity_tile<R, TX, TY, MODE, CPML>(w, Bm, g, cf, ex, it, P, blk);
+    velocity_tile<R, TX, TY, MODE, AX>(w, Bm, g, cf, ex, it, P, blk);
 }
 
 // ---------------------------------------------------------------------------
@@ -597,8 +603,9 @@ struct ShotStep {
     int src_k, src_col;
 };
 
-template <int R, int TX, int TY, bool CPML>
-__global__ __launch_bounds__(TX* TY) void k_stress_batch(const ShotStep* __restrict__ shots,
+template <int R, int TX, int TY, int AX>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(R != 4 || AX == 0 ? 1 : (AX == 1 || AX == 2 || AX == 4) ? 4 : 3)))
+void k_stress_batch(const ShotStep* __restrict__ shots,
                                                          const int2* __restrict__ table, int total, Coef cf,
                                                          const float* __restrict__ wavelet, int wl,
                                                          const int* __restrict__ itp, int source_on) {
@@ -606,13 +613,13 @@ __global__ __launch_bounds__(TX* TY) void k_stress_batch(const ShotStep* __restr
     for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
         const int2 t = table[b];
         const ShotStep& s = shots[t.x];
-        stress_tile<R, TX, TY, CPML>(s.w, s.M, s.MU, s.g, cf, source_on ? s.src_k : -1, source_on ? s.src_col : -1,
+        stress_tile<R, TX, TY, AX>(s.w, s.M, s.MU, s.g, cf, source_on ? s.src_k : -1, source_on ? s.src_col : -1,
                                      wavelet, wl, it, s.P, t.y);
     }
 }
 
-template <int R, int TX, int TY, int MODE, bool CPML>
-__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(CPML && R == 4 ? 3 : 1)))
+template <int R, int TX, int TY, int MODE, int AX>
+__global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(R != 4 || AX == 0 ? 1 : 3)))
 void k_velocity_batch(const ShotStep* __restrict__ shots,
                                                            const int2* __restrict__ table, int total, Coef cf,
                                                            const int* __restrict__ itp) {
@@ -620,7 +627,7 @@ void k_velocity_batch(const ShotStep* __restrict__ shots,
     for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
         const int2 t = table[b];
         const ShotStep& s = shots[t.x];
-        velocity_tile<R, TX, TY, MODE, CPML>(s.w, s.B, s.g, cf, s.ex, it, s.P, t.y);
+        velocity_tile<R, TX, TY, MODE, AX>(s.w, s.B, s.g, cf, s.ex, it, s.P, t.y);
     }
 }
 
@@ -649,6 +656,41 @@ struct RecShot {
         s.rec[s.comp + off] = s.vy[n];
         s.rec[2 * s.comp + off] = s.vz[n];
     }
+}
+
+// Boundary strips for source-wavefield reconstruction: the R-cell face slabs
+// just outside the model region, where the stencils of model-region cells
+// reach.  Gather (save) or scatter (restore) nf fields at the listed padded
+// indices; buf layout [nf][n].
+struct StripFields {
+    float* f[7];
+};
+[[maybe_unused]] static __global__ void k_strip(StripFields F, int nf, const int* __restrict__ idx, int n,
+                                                float* __restrict__ buf, int restore) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int q = idx[t];
+    for (int a = 0; a < nf; ++a) {
+        if (restore) F.f[a][q] = buf[(long)a * n + t];
+        else buf[(long)a * n + t] = F.f[a][q];
+    }
+}
+
+// Zero-lag cross-correlation imaging of a source and a receiver wavefield over
+// the model region (origin (ox, oy, oz), size nx x ny x nz of the padded grid):
+//   I_tp += tp_s tp_r,  I_PP += vP_s . vP_r,  I_PS += vP_s . (v_r - vP_r).
+[[maybe_unused]] static __global__ void k_xcorr(Wave s, Wave r, float* __restrict__ itp, float* __restrict__ ipp,
+                                                float* __restrict__ ips, int nx, int ny, int nz, int ox, int oy,
+                                                int oz, int px, long pxy) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)nx * ny * nz) return;
+    const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((long)nx * ny));
+    const long n = (long)(k + oz) * pxy + (long)(j + oy) * px + (i + ox);
+    const float sx = s.vxp[n], sy = s.vyp[n], sz = s.vzp[n];
+    const float rx = r.vxp[n], ry = r.vyp[n], rz = r.vzp[n];
+    itp[n] += s.tp[n] * r.tp[n];
+    ipp[n] += sx * rx + sy * ry + sz * rz;
+    ips[n] += sx * (r.vx[n] - rx) + sy * (r.vy[n] - ry) + sz * (r.vz[n] - rz);
 }
 
 // it += delta (one lane; a plain vector store).

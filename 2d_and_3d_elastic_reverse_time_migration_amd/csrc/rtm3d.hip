@@ -97,15 +97,17 @@ int dmalloc(erm_ctx* c, T** ptr, size_t n) {
 
 inline dim3 grid_of(const Geo& g) { return dim3(g.xcd_map ? 8 * g.per_xcd : g.ntiles); }
 
-// Tile lists of the CPML region split for the current launch shape: a tile is
-// "interior" when none of its updated cells lies in a memory-variable slab
-// (then the CPML kernel would do exactly what the plain kernel does); tiles
-// with no updated cell are dropped.
+// Tile groups of the CPML region split for the current launch shape: group 0
+// holds the tiles none of whose updated cells lies in a memory-variable slab
+// (they run the plain kernel), group 7 all others (the all-axes C-PML kernel).
+// Tiles with no updated cell are dropped.  The kernels also build for the
+// single-axis groups 1..6 (bit 0 x, 1 y, 2 z), but launching the groups one
+// after another measured slower (launch tails) than the two-group split.
 int build_split(erm_ctx* c) {
-    if (c->split_ty == c->ty && c->split_kchunk == c->kchunk && c->tiles_in) return 0;
+    if (c->split_built && c->split_ty == c->ty && c->split_kchunk == c->kchunk) return 0;
     Geo g = make_geo(c);
     const int R = c->R;
-    std::vector<int> tin, tpml;
+    std::vector<int> grp[8];
     auto inside = [](int lo_c, int hi_c, int n, int lo, int hi) {  // [lo_c, hi_c] avoids both slabs
         return lo_c > lo && hi_c < n - hi - 1;
     };
@@ -115,69 +117,143 @@ int build_split(erm_ctx* c) {
         const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
         const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
         if (i0 > i1 || j0 > j1 || k0 > k1) continue;
-        const bool in = inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) &&
-                        inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) &&
-                        inside(k0, k1, c->nnz, c->slab_lo[2], c->slab_hi[2]);
-        (in ? tin : tpml).push_back(l);
+        int m = (inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) ? 0 : 1) |
+                (inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) ? 0 : 2) |
+                (inside(k0, k1, c->nnz, c->slab_lo[2], c->slab_hi[2]) ? 0 : 4);
+        if (m != 0) m = 7;
+        grp[m].push_back(l);
     }
-    if (c->tiles_in) (void)hipFree(c->tiles_in);
-    if (c->tiles_pml) (void)hipFree(c->tiles_pml);
-    c->tiles_in = c->tiles_pml = nullptr;
-    if (dmalloc(c, &c->tiles_in, std::max<size_t>(1, tin.size())) ||
-        dmalloc(c, &c->tiles_pml, std::max<size_t>(1, tpml.size())))
-        return ERM_E_NOMEM;
-    if (!tin.empty()) HIPC(c, hipMemcpy(c->tiles_in, tin.data(), tin.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!tpml.empty()) HIPC(c, hipMemcpy(c->tiles_pml, tpml.data(), tpml.size() * sizeof(int), hipMemcpyHostToDevice));
-    c->n_in = (int)tin.size();
-    c->n_pml = (int)tpml.size();
+    for (int m = 0; m < 8; ++m) {
+        if (c->tiles[m]) (void)hipFree(c->tiles[m]);
+        c->tiles[m] = nullptr;
+        c->ntl[m] = (int)grp[m].size();
+        if (grp[m].empty()) continue;
+        if (dmalloc(c, &c->tiles[m], grp[m].size())) return ERM_E_NOMEM;
+        HIPC(c, hipMemcpy(c->tiles[m], grp[m].data(), grp[m].size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     c->split_ty = c->ty;
     c->split_kchunk = c->kchunk;
+    c->split_built = true;
     return 0;
+}
+
+template <int R, int TX, int TY, int AX>
+void stress_one(erm_ctx* c, int which, int src_k, int src_col, int it, const Geo& g, dim3 grd, const int* list) {
+    hipLaunchKernelGGL((k_stress<R, TX, TY, AX>), grd, dim3(TX, TY), 0, c->stream, make_wave(c), c->M[which],
+                       c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml, list);
+}
+template <int R, int TX, int TY, int MODE, int AX>
+void velocity_one(erm_ctx* c, int which, int it, const Geo& g, dim3 grd, const int* list) {
+    hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, AX>), grd, dim3(TX, TY), 0, c->stream, make_wave(c),
+                       c->BU[which], g, c->coef, make_excite(c), it, c->pml, list);
 }
 
 template <int R, int TX, int TY>
 void stress_launch(erm_ctx* c, int which, long src, int it) {
     const int src_k = src < 0 ? -1 : (int)(src / c->pxy), src_col = src < 0 ? -1 : (int)(src % c->pxy);
     Geo g = make_geo(c);
-    dim3 blk(TX, TY);
     if (c->cpml && c->region_split && build_split(c) == 0) {
         g.xcd_map = 0;
-        if (c->n_in)
-            hipLaunchKernelGGL((k_stress<R, TX, TY, false>), dim3(c->n_in), blk, 0, c->stream, make_wave(c),
-                               c->M[which], c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml,
-                               (const int*)c->tiles_in);
-        if (c->n_pml)
-            hipLaunchKernelGGL((k_stress<R, TX, TY, true>), dim3(c->n_pml), blk, 0, c->stream, make_wave(c),
-                               c->M[which], c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml,
-                               (const int*)c->tiles_pml);
+        for (int m = 0; m < 8; ++m) {
+            if (!c->ntl[m]) continue;
+            const dim3 grd(c->ntl[m]);
+            const int* L = c->tiles[m];
+            switch (m) {
+                case 0: stress_one<R, TX, TY, 0>(c, which, src_k, src_col, it, g, grd, L); break;
+                default: stress_one<R, TX, TY, 7>(c, which, src_k, src_col, it, g, grd, L); break;
+            }
+        }
     } else if (c->cpml) {
-        hipLaunchKernelGGL((k_stress<R, TX, TY, true>), grid_of(g), blk, 0, c->stream, make_wave(c), c->M[which],
-                           c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml, nullptr);
+        stress_one<R, TX, TY, 7>(c, which, src_k, src_col, it, g, grid_of(g), nullptr);
     } else {
-        hipLaunchKernelGGL((k_stress<R, TX, TY, false>), grid_of(g), blk, 0, c->stream, make_wave(c), c->M[which],
-                           c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml, nullptr);
+        stress_one<R, TX, TY, 0>(c, which, src_k, src_col, it, g, grid_of(g), nullptr);
     }
 }
 
 template <int R, int TX, int TY, int MODE>
 void velocity_launch(erm_ctx* c, int which, int it) {
     Geo g = make_geo(c);
-    dim3 blk(TX, TY);
     if (c->cpml && c->region_split && build_split(c) == 0) {
         g.xcd_map = 0;
-        if (c->n_in)
-            hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, false>), dim3(c->n_in), blk, 0, c->stream, make_wave(c),
-                               c->BU[which], g, c->coef, make_excite(c), it, c->pml, (const int*)c->tiles_in);
-        if (c->n_pml)
-            hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, true>), dim3(c->n_pml), blk, 0, c->stream, make_wave(c),
-                               c->BU[which], g, c->coef, make_excite(c), it, c->pml, (const int*)c->tiles_pml);
+        for (int m = 0; m < 8; ++m) {
+            if (!c->ntl[m]) continue;
+            const dim3 grd(c->ntl[m]);
+            const int* L = c->tiles[m];
+            switch (m) {
+                case 0: velocity_one<R, TX, TY, MODE, 0>(c, which, it, g, grd, L); break;
+                default: velocity_one<R, TX, TY, MODE, 7>(c, which, it, g, grd, L); break;
+            }
+        }
     } else if (c->cpml) {
-        hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, true>), grid_of(g), blk, 0, c->stream, make_wave(c),
-                           c->BU[which], g, c->coef, make_excite(c), it, c->pml, nullptr);
+        velocity_one<R, TX, TY, MODE, 7>(c, which, it, g, grid_of(g), nullptr);
     } else {
-        hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, false>), grid_of(g), blk, 0, c->stream, make_wave(c),
-                           c->BU[which], g, c->coef, make_excite(c), it, c->pml, nullptr);
+        velocity_one<R, TX, TY, MODE, 0>(c, which, it, g, grid_of(g), nullptr);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Source-wavefield reconstruction.  Inside the model region shrunk by one
+// cell (the first model cells of some faces still see a non-zero C-PML
+// profile at their staggered positions) the update has no damping and no
+// memory variables, so a step is inverted by the same kernels with dt
+// negated: v^n = v^{n+1} - dt b D sigma^{n+1/2}, then
+// sigma^{n-1/2} = sigma^{n+1/2} - dt C D v^n - wavelet[n] (the source enters
+// before the update).  The stencils of those cells reach R cells out, into
+// the face slabs whose forward values (sigma + tp after the stress update;
+// v + vP after the velocity update) were saved every step; they are restored
+// before each reversed half-step, so every model-region cell holds the
+// forward value when the images are formed.  Cells further out are not
+// needed (they are updated without absorption and stay bounded).
+// Strip layout: slot s holds 13 fields x n_strip; slot it+1 is forward step
+// it (7 stress-side fields, then 6 velocity-side fields); slot 0 is zero.
+// ---------------------------------------------------------------------------
+constexpr int kStripFields = 13;
+inline Wave wave_of(float* const* f) {
+    Wave w;
+    w.vx = f[0]; w.vy = f[1]; w.vz = f[2];
+    w.txx = f[3]; w.tyy = f[4]; w.tzz = f[5]; w.txy = f[6]; w.txz = f[7]; w.tyz = f[8];
+    w.tp = f[9]; w.vxp = f[10]; w.vyp = f[11]; w.vzp = f[12];
+    return w;
+}
+inline Geo reversed_geo(const erm_ctx* c) {
+    Geo g = make_geo(c);
+    g.cx = -g.cx; g.cy = -g.cy; g.cz = -g.cz;
+    return g;
+}
+inline StripFields stress_side(float* const* f) { return StripFields{{f[3], f[4], f[5], f[6], f[7], f[8], f[9]}}; }
+inline StripFields velocity_side(float* const* f) { return StripFields{{f[0], f[1], f[2], f[10], f[11], f[12], nullptr}}; }
+// restore (or save) the stress-side slab values of level it+1/2 = forward step it
+void strip_stress(erm_ctx* c, float* const* f, int it, int restore) {
+    const int n = c->n_strip, rb = 256, rg = (n + rb - 1) / rb;
+    hipLaunchKernelGGL(k_strip, dim3(rg), dim3(rb), 0, c->stream, stress_side(f), 7, (const int*)c->strip_idx, n,
+                       c->strips + (long)(it + 1) * kStripFields * n, restore);
+}
+// restore (or save) the velocity-side slab values of level lev (= forward step lev-1; lev 0 is zero)
+void strip_velocity(erm_ctx* c, float* const* f, int lev, int restore) {
+    const int n = c->n_strip, rb = 256, rg = (n + rb - 1) / rb;
+    hipLaunchKernelGGL(k_strip, dim3(rg), dim3(rb), 0, c->stream, velocity_side(f), 6, (const int*)c->strip_idx, n,
+                       c->strips + (long)lev * kStripFields * n + 7L * n, restore);
+}
+// reverse step it after the stress-side slabs of level it+1/2 are in place:
+// state {sigma^{it+1/2}, v^{it+1}} -> {sigma^{it-1/2}, v^{it}}
+template <int R, int TY>
+void reverse_rest_rt(erm_ctx* c, int it) {
+    const Geo g = reversed_geo(c);
+    const Wave w = wave_of(c->fr);
+    hipLaunchKernelGGL((k_velocity<R, 64, TY, MODE_PLAIN, 0>), grid_of(g), dim3(64, TY), 0, c->stream, w, c->BU[1], g,
+                       c->coef, make_excite(c), it, c->pml, nullptr);
+    strip_velocity(c, c->fr, it, 1);
+    const int src_k = c->src < 0 ? -1 : (int)(c->src / c->pxy), src_col = c->src < 0 ? -1 : (int)(c->src % c->pxy);
+    hipLaunchKernelGGL((k_stress<R, 64, TY, 0>), grid_of(g), dim3(64, TY), 0, c->stream, w, c->M[1], c->MU[1], g,
+                       c->coef, src_k, src_col, (const float*)c->wavelet_neg, c->wl, it, c->pml, nullptr);
+}
+void reverse_rest(erm_ctx* c, int it) {
+    if (c->R == 4) { if (c->ty == 4) reverse_rest_rt<4, 4>(c, it); else reverse_rest_rt<4, 8>(c, it); }
+    else { if (c->ty == 4) reverse_rest_rt<6, 4>(c, it); else reverse_rest_rt<6, 8>(c, it); }
+}
+void reverse_step(erm_ctx* c, int it) {
+    strip_stress(c, c->fr, it, 1);
+    reverse_rest(c, it);
 }
 
 // C-PML coefficients (Roden & Gedney 2000; Komatitsch & Martin 2007, kappa = 1):
@@ -310,7 +386,10 @@ void free_all(erm_ctx* c) {
     F(c->rec); F(c->rec_idx);
     F(c->pml_coef);
     for (auto& p : c->psi) F(p);
-    F(c->tiles_in); F(c->tiles_pml);
+    for (auto& t : c->tiles) F(t);
+    for (auto& q : c->fr) F(q);
+    F(c->strip_idx); F(c->strips); F(c->wavelet_neg);
+    for (auto& q : c->ximg) F(q);
     for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -348,7 +427,7 @@ void pad_model(const erm_params& p, int nnx, int nny, int nnz, const float* src,
     }
 }
 
-int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps) {
+int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false) {
     if (pass < ERM_PASS_MODEL || pass > ERM_PASS_RECEIVER) return c->fail(ERM_E_PARAM, "unknown pass %d", pass);
     const int which = pass == ERM_PASS_MODEL ? 0 : 1;
     if (!c->have_model[which]) return c->fail(ERM_E_STATE, "%s model not loaded", which ? "smoothed" : "true");
@@ -379,8 +458,20 @@ int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps) {
         if (c->profiling) HIPC(c, hipEventRecord(c->kev[3 * s], c->stream));
         do_stress(c, which, pass == ERM_PASS_RECEIVER ? -1L : c->src, it);
         if (c->profiling) HIPC(c, hipEventRecord(c->kev[3 * s + 1], c->stream));
+        if (strips && pass == ERM_PASS_SOURCE) strip_stress(c, c->f, it, 0);
         do_velocity(c, which, pass == ERM_PASS_SOURCE ? MODE_EXCITE : pass == ERM_PASS_RECEIVER ? MODE_IMAGE : MODE_PLAIN, it);
         if (c->profiling) HIPC(c, hipEventRecord(c->kev[3 * s + 2], c->stream));
+        if (strips && pass == ERM_PASS_SOURCE) strip_velocity(c, c->f, it + 1, 0);
+        if (strips && pass == ERM_PASS_RECEIVER) {
+            // source state {sigma^{it+1/2}, v^{it+1}}: complete the slab cells, image, step back
+            strip_stress(c, c->fr, it, 1);
+            const erm_params& P = c->p;
+            const long nm = (long)P.nx * P.ny * P.nz;
+            hipLaunchKernelGGL(k_xcorr, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, c->stream, wave_of(c->fr),
+                               make_wave(c), c->ximg[0], c->ximg[1], c->ximg[2], P.nx, P.ny, P.nz, P.bl, P.bf, P.bu,
+                               c->px, c->pxy);
+            reverse_rest(c, it);
+        }
         if (pass == ERM_PASS_MODEL && c->nr > 0)
             hipLaunchKernelGGL(k_record, dim3(rg), dim3(rb), 0, c->stream, c->f[0], c->f[1], c->f[2], c->rec_idx, c->nr,
                                c->rec, comp, (long)it * c->nr);
@@ -865,6 +956,147 @@ int erm_algorithmic_bytes(const erm_ctx* c, int pass, double* sb, double* vb) {
     if (sb) *sb = 19.0 * 4.0 + (c->cpml ? c->pml_extra_stress : 0.0);
     if (vb) *vb = 20.0 * 4.0 + ((pass == ERM_PASS_SOURCE || pass == ERM_PASS_RECEIVER) ? 4.0 : 0.0) +
                   (c->cpml ? c->pml_extra_velocity : 0.0);
+    return 0;
+}
+
+static int build_strip_index(erm_ctx* c) {
+    if (c->strip_idx) return 0;
+    const erm_params& P = c->p;
+    const int R = c->R;
+    if (P.bl < R - 1 || P.br < R - 1 || P.bf < R - 1 || P.bb < R - 1 || P.bu < R - 1 || P.bd < R - 1)
+        return c->fail(ERM_E_PARAM, "boundary strips need every boundary width >= %d", R - 1);
+    if (c->ncell >= (1UL << 31)) return c->fail(ERM_E_SIZE, "grid too large for 32-bit strip indices");
+    if (P.nx < 3 || P.ny < 3 || P.nz < 3) return c->fail(ERM_E_SIZE, "model region too small for reconstruction");
+    // reconstructed region I: the model region shrunk by one cell on every face.
+    // Strip cells: the R-cell face slabs around I (where the stencils of I
+    // reach) and the rest of the model region's border shell (its edges and
+    // corners), so every model-region cell is either reconstructed or restored.
+    const int x0 = P.bl + 1, x1 = P.bl + P.nx - 1, y0 = P.bf + 1, y1 = P.bf + P.ny - 1, z0 = P.bu + 1, z1 = P.bu + P.nz - 1;
+    std::vector<char> mark((size_t)c->nnx * c->nny * c->nnz, 0);
+    auto set = [&](int i, int j, int k) { mark[((size_t)k * c->nny + j) * c->nnx + i] = 1; };
+    for (int k = z0; k < z1; ++k)
+        for (int j = y0; j < y1; ++j)
+            for (int r = 1; r <= R; ++r) { set(x0 - r, j, k); set(x1 - 1 + r, j, k); }
+    for (int k = z0; k < z1; ++k)
+        for (int r = 1; r <= R; ++r)
+            for (int i = x0; i < x1; ++i) { set(i, y0 - r, k); set(i, y1 - 1 + r, k); }
+    for (int r = 1; r <= R; ++r)
+        for (int j = y0; j < y1; ++j)
+            for (int i = x0; i < x1; ++i) { set(i, j, z0 - r); set(i, j, z1 - 1 + r); }
+    for (int k = P.bu; k < P.bu + P.nz; ++k)
+        for (int j = P.bf; j < P.bf + P.ny; ++j)
+            for (int i = P.bl; i < P.bl + P.nx; ++i)
+                if (i < x0 || i >= x1 || j < y0 || j >= y1 || k < z0 || k >= z1) set(i, j, k);
+    std::vector<int> idx;
+    for (int k = 0; k < c->nnz; ++k)
+        for (int j = 0; j < c->nny; ++j)
+            for (int i = 0; i < c->nnx; ++i)
+                if (mark[((size_t)k * c->nny + j) * c->nnx + i]) idx.push_back((int)((long)k * c->pxy + (long)j * c->px + i));
+    c->n_strip = (int)idx.size();
+    if (dmalloc(c, &c->strip_idx, idx.size())) return ERM_E_NOMEM;
+    HIPC(c, hipMemcpy(c->strip_idx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int erm_strip_bytes(const erm_ctx* c, double* bytes) {
+    if (!c || !bytes) return ERM_E_PARAM;
+    const erm_params& P = c->p;
+    const double ax = P.nx - 2, ay = P.ny - 2, az = P.nz - 2;
+    // face slabs around the shrunk region plus the model region's border edges (upper bound)
+    const double n = 2.0 * c->R * (ay * az + ax * az + ax * ay) + 4.0 * ((double)P.nx + P.ny + P.nz);
+    *bytes = 4.0 * kStripFields * n * (c->p.lt_forward + 1);
+    return 0;
+}
+
+int erm_forward_smoothed_strips(erm_ctx* c) {
+    if (!c) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    if (c->p.lt_forward != c->p.lt) return c->fail(ERM_E_PARAM, "reconstruction needs lt_forward == lt");
+    int rc;
+    if ((rc = build_strip_index(c))) return rc;
+    const long need = (long)kStripFields * c->n_strip * (c->p.lt + 1);
+    if (c->strip_steps != c->p.lt) {
+        if (c->strips) (void)hipFree(c->strips);
+        c->strips = nullptr;
+        c->strip_steps = 0;
+        size_t fr = 0, tot = 0;
+        HIPC(c, hipMemGetInfo(&fr, &tot));
+        if ((size_t)need * 4 + (64UL << 20) > fr)
+            return c->fail(ERM_E_NOMEM, "boundary strips need %.1f GB, %.1f GB free", need * 4e-9, fr * 1e-9);
+        if (dmalloc(c, &c->strips, (size_t)need)) return ERM_E_NOMEM;
+        c->strip_steps = c->p.lt;
+    }
+    if (!c->wavelet_neg) {
+        std::vector<float> wn(c->wavelet_h.size());
+        for (size_t q = 0; q < wn.size(); ++q) wn[q] = -c->wavelet_h[q];
+        if (dmalloc(c, &c->wavelet_neg, wn.size())) return ERM_E_NOMEM;
+        HIPC(c, hipMemcpy(c->wavelet_neg, wn.data(), wn.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    for (auto& q : c->fr) if (!q && dmalloc(c, &q, c->ncell)) return ERM_E_NOMEM;
+    if ((rc = fill(c, c->strips, (size_t)kStripFields * c->n_strip, 0.f))) return rc;  // slot 0: the zero initial state
+    if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) return rc;
+    c->have_strips = false;
+    if ((rc = run_pass(c, ERM_PASS_SOURCE, 0, c->p.lt, true))) return rc;
+    for (int q = 0; q < 13; ++q)
+        HIPC(c, hipMemcpyAsync(c->fr[q], c->f[q], c->ncell * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->have_strips = true;
+    return 0;
+}
+
+int erm_reconstruct(erm_ctx* c, int it_end) {
+    if (!c) return ERM_E_PARAM;
+    if (!c->have_strips) return c->fail(ERM_E_STATE, "no boundary strips: run erm_forward_smoothed_strips first");
+    if (it_end < 0 || it_end > c->p.lt) return c->fail(ERM_E_PARAM, "it_end out of [0, lt]");
+    HIPC(c, hipSetDevice(c->device));
+    for (int it = c->p.lt - 1; it >= it_end; --it) reverse_step(c, it);
+    // complete the slab cells at level it_end - 1/2 (v, vP are already in place)
+    if (it_end > 0) strip_stress(c, c->fr, it_end - 1, 1);
+    else for (int q = 3; q <= 9; ++q) if (int rc = fill(c, c->fr[q], c->ncell, 0.f)) return rc;
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->have_strips = false;  // the reconstruction wavefield has moved off the final state
+    return 0;
+}
+
+int erm_get_reconstructed_field(erm_ctx* c, int field, float* dst) {
+    if (!c || !dst || field < 0 || field >= 13 || !c->fr[field]) return c ? c->fail(ERM_E_PARAM, "bad field") : ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    return download(c, c->fr[field], dst);
+}
+
+int erm_backward_smoothed_xcorr(erm_ctx* c, const float* rec) {
+    if (!c) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    if (!c->have_strips) return c->fail(ERM_E_STATE, "no boundary strips: run erm_forward_smoothed_strips first");
+    int rc;
+    if (rec && (rc = erm_set_record(c, rec))) return rc;
+    for (auto& q : c->ximg) if (!q && dmalloc(c, &q, c->ncell)) return ERM_E_NOMEM;
+    if ((rc = erm_zero_wavefield(c))) return rc;
+    for (auto& q : c->img) if ((rc = fill(c, q, c->ncell, 0.f))) return rc;
+    for (auto& q : c->ximg) if ((rc = fill(c, q, c->ncell, 0.f))) return rc;
+    rc = run_pass(c, ERM_PASS_RECEIVER, c->p.lt - 1, c->p.lt, true);
+    c->have_strips = false;
+    return rc;
+}
+
+int erm_get_xcorr_images(erm_ctx* c, float* tp, float* pp, float* ps) {
+    if (!c) return ERM_E_PARAM;
+    if (!c->ximg[0]) return c->fail(ERM_E_STATE, "no cross-correlation images: run erm_backward_smoothed_xcorr first");
+    const erm_params& p = c->p;
+    const size_t N = (size_t)c->nnx * c->nny * c->nnz;
+    std::vector<float> full(N);
+    float* outs[3] = {tp, pp, ps};
+    for (int a = 0; a < 3; ++a) {
+        if (!outs[a]) continue;
+        int rc = download(c, c->ximg[a], full.data());
+        if (rc) return rc;
+        for (int ix = 0; ix < p.nx; ++ix)
+            for (int iy = 0; iy < p.ny; ++iy)
+                for (int iz = 0; iz < p.nz; ++iz)
+                    outs[a][((size_t)ix * p.ny + iy) * p.nz + iz] =
+                        full[((size_t)(iz + p.bu) * c->nny + iy + p.bf) * c->nnx + ix + p.bl];
+    }
     return 0;
 }
 

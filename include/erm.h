@@ -194,6 +194,30 @@ int   erm_get_timing(erm_ctx* ctx, double* pass_ms, double* stress_ms, long* str
  * for the given pass, as the kernels are built (see the kernel sources). */
 int   erm_algorithmic_bytes(const erm_ctx* ctx, int pass, double* stress_bytes, double* velocity_bytes);
 int   erm_synchronize(erm_ctx* ctx);
+/* Source-wavefield reconstruction from boundary strips (no reference
+ * counterpart; the reference images with the excitation amplitude only, and
+ * its 2D code stores full histories, elastic_2D_kernel_2.cu:793-827).
+ * erm_forward_smoothed_strips runs the source pass like erm_forward_smoothed
+ * (excitation kept) and also saves, every step, sigma+tp and v+vP on the
+ * R-cell face slabs around the model region shrunk by one cell (whose
+ * update is free of damping and C-PML) plus the final wavefield, in HBM
+ * (erm_strip_bytes tells how much: 52 B x slab cells x (lt+1)).
+ * erm_backward_smoothed_xcorr then runs the receiver pass (excitation imaging
+ * as erm_backward_smoothed) while reconstructing the source wavefield
+ * backward in time in the model region (the step inverted with dt negated,
+ * the slabs restored from the strips) and accumulates the zero-lag
+ * cross-correlation images I_tp = sum_t tp_s tp_r, I_PP = sum_t vP_s.vP_r,
+ * I_PS = sum_t vP_s.(v_r - vP_r), returned by erm_get_xcorr_images over the
+ * model region in disk order.  Requires lt_forward == lt and every boundary
+ * width >= the stencil radius - 1.  erm_reconstruct / erm_get_reconstructed_field
+ * expose the reconstruction itself (step the saved final state back to the
+ * start of step it_end; fields of the padded grid as erm_get_field). */
+int   erm_strip_bytes(const erm_ctx* ctx, double* bytes);
+int   erm_forward_smoothed_strips(erm_ctx* ctx);
+int   erm_backward_smoothed_xcorr(erm_ctx* ctx, const float* rec);
+int   erm_get_xcorr_images(erm_ctx* ctx, float* tp, float* pp, float* ps);
+int   erm_reconstruct(erm_ctx* ctx, int it_end);
+int   erm_get_reconstructed_field(erm_ctx* ctx, int field, float* dst);
 /* Launch-shape autotuner (no reference counterpart; the reference launches
  * one block per z plane): times `steps` source-pass steps (smoothed model)
  * for tile heights {4, 8} x z sweep depths {4, 8, 16, 32} and keeps the

@@ -41,6 +41,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -586,6 +587,58 @@ int oracle3d_rtm_shot(void* h, int use_record) {
         o.zero_wavefield();
         o.zero_images();
         return o.step(ERM_PASS_RECEIVER, o.p.lt - 1, o.p.lt);
+    });
+}
+
+/* Zero-lag cross-correlation imaging with the source wavefield kept in full
+ * (the full-history strategy of elastic_2D_kernel_2.cu:793-827, here in
+ * memory): source pass storing vP and tp over the model region after every
+ * step, then the receiver pass accumulating, after every step it,
+ *   I_tp += tp_s(it) tp_r,  I_PP += vP_s(it).vP_r,  I_PS += vP_s(it).(v_r - vP_r).
+ * Outputs over the model region in disk order [nx][ny][nz]; requires
+ * lt_forward == lt and a record (modelled here when use_record == 0). */
+int oracle3d_xcorr_shot(void* h, int use_record, double* itp, double* ipp, double* ips) {
+    return dispatch(h, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.rec[0])>::type;
+        const erm_params& p = o.p;
+        if (p.lt_forward != p.lt) return ERM_E_PARAM;
+        int rc;
+        if (!use_record) {
+            o.zero_wavefield();
+            if ((rc = o.step(ERM_PASS_MODEL, 0, p.lt))) return rc;
+        }
+        const size_t nm = (size_t)p.nx * p.ny * p.nz;
+        auto cell = [&](size_t q) {  // model-region cell q (disk order) -> padded index
+            const int iz = (int)(q % p.nz), iy = (int)((q / p.nz) % p.ny), ix = (int)(q / ((size_t)p.nz * p.ny));
+            return ((size_t)(iz + p.bu) * o.nny + (iy + p.bf)) * o.nnx + (ix + p.bl);
+        };
+        std::vector<T> hist((size_t)p.lt * 4 * nm);
+        o.zero_wavefield();
+        o.zero_excitation();
+        for (int it = 0; it < p.lt; ++it) {
+            if ((rc = o.step(ERM_PASS_SOURCE, it, 1))) return rc;
+            T* h = &hist[(size_t)it * 4 * nm];
+            for (size_t q = 0; q < nm; ++q) {
+                const size_t n = cell(q);
+                h[q] = o.f[9][n]; h[nm + q] = o.f[10][n]; h[2 * nm + q] = o.f[11][n]; h[3 * nm + q] = o.f[12][n];
+            }
+        }
+        for (size_t q = 0; q < nm; ++q) itp[q] = ipp[q] = ips[q] = 0.0;
+        o.zero_wavefield();
+        o.zero_images();
+        for (int it = p.lt - 1; it >= 0; --it) {
+            if ((rc = o.step(ERM_PASS_RECEIVER, it, 1))) return rc;
+            const T* h = &hist[(size_t)it * 4 * nm];
+            for (size_t q = 0; q < nm; ++q) {
+                const size_t n = cell(q);
+                const double sx = h[nm + q], sy = h[2 * nm + q], sz = h[3 * nm + q];
+                const double rx = o.f[10][n], ry = o.f[11][n], rz = o.f[12][n];
+                itp[q] += (double)h[q] * (double)o.f[9][n];
+                ipp[q] += sx * rx + sy * ry + sz * rz;
+                ips[q] += sx * ((double)o.f[0][n] - rx) + sy * ((double)o.f[1][n] - ry) + sz * ((double)o.f[2][n] - rz);
+            }
+        }
+        return 0;
     });
 }
 

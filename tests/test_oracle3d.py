@@ -391,3 +391,28 @@ def test_rtm_images_reflector_depth(oracle):
     col = np.abs(pp[18:23, 18:23, 10:]).sum(axis=(0, 1))
     k = int(np.argmax(col)) + 10
     assert abs(k - zi) <= 2, (k, zi)
+
+
+def test_xcorr_images_reflector_depth(oracle):
+    """Cross-correlation PP image (full source history) of a flat reflector:
+    below the shot its energy peaks at the interface depth.  Pins the zero-lag
+    correlation restatement (a wrong time pairing of source and receiver
+    fields moves or smears the peak)."""
+    nz, zi = 36, 22
+    p = cases.params(nx=28, ny=28, nz=nz, b=8, R=4, lt=260, dt=1.0, freq=25.0, coe=250.0,
+                     src=(14, 14, 3), rec=(13, 13, 1, 1, 2, 2, 2))
+    p.lt_forward = p.lt
+    shape = (p.nx, p.ny, p.nz)
+    z = np.arange(nz)[None, None, :]
+    vp = (np.where(z < zi, 2000.0, 3000.0) * np.ones(shape)).astype(np.float32)
+    rho = (np.where(z < zi, 2000.0, 2400.0) * np.ones(shape)).astype(np.float32)
+    vs = (vp / 1.732).astype(np.float32)
+    o = oracle.Oracle3D(p)
+    o.load_model(0, vp, vs, rho)
+    s = np.full(shape, 2000.0, np.float32)
+    o.load_model(1, s, (s / 1.732).astype(np.float32), np.full(shape, 2000.0, np.float32))
+    tp, pp, ps = o.xcorr_shot()
+    col = np.abs(pp[12:17, 12:17, 12:]).sum(axis=(0, 1))
+    k = int(np.argmax(col)) + 12
+    assert abs(k - zi) <= 2, (k, zi)
+    assert np.isfinite(ps).all() and np.abs(ps).max() > 0
