@@ -182,6 +182,114 @@ def run_batch(erm, args):
     print(json.dumps(line), flush=True)
 
 
+def visco_model(nx, nz, seed):
+    """Layered viscoelastic model with smooth random Q (Qp 30..200, Qs = 0.6 Qp):
+    background (smoothed) model and the relative perturbation dm (dvp/vp, dvs/vs)."""
+    learned = importlib.import_module(PKG + ".learned")
+    bvp, bvs, brho, dm = learned.random_model(nx, nz, seed, smooth_sigma=8.0)
+    rng = np.random.default_rng(seed + 1)
+    q = learned._smooth(rng.standard_normal((nx, nz)), 25.0)
+    q = (q - q.min()) / max(q.max() - q.min(), 1e-12)
+    qp = (30.0 + 170.0 * q).astype(np.float32)
+    qs = (0.6 * qp).astype(np.float32)
+    return bvp, bvs, brho, qp, qs, dm
+
+
+def run_visco(erm, args):
+    """2D viscoelastic LSRTM with plain CG: 600x300 model, d = 8 m, 3 SLS
+    mechanisms, 60 shots, 600 receivers, 20 Hz, nt = 3000 (dt 0.8 ms), 40-cell
+    damping boundary, Born data of a seeded layered-plus-anomaly perturbation;
+    20 CG iterations (dvp/vp, dvs/vs).  One step = one CG iteration."""
+    learned = importlib.import_module(PKG + ".learned")
+    nx, nz = 600, 300
+    p = learned.survey_params(nx, nz, args.nt if args.nt != 2500 else 3000, freq=20.0, d=8.0, dt=0.8, b=40)
+    p.nsls = 3
+    p.nrec, p.rec_x0 = 600, 0
+    bvp, bvs, brho, qp, qs, dm = visco_model(nx, nz, 4)
+    g = erm.Elastic2D(p)
+    g.load_model(1, bvp, bvs, brho, qp, qs)
+    g.set_workers(args.workers)
+    shots = learned.shot_positions(nx, args.shots)
+    t0 = time.perf_counter()
+    d = learned.born_data(g, shots, dm)
+    log(f"bench visco: Born data of {len(shots)} shots in {time.perf_counter() - t0:.1f} s")
+
+    def mon(it, m, grad, mis):
+        log(f"bench visco: CG iteration {it}: misfit {mis:.4f}")
+
+    t0 = time.perf_counter()
+    m, mis, sec, it = g.lsrtm_cg(shots, d, args.iters, pmask=3, monitor=mon)
+    el = time.perf_counter() - t0
+    err = float(np.linalg.norm(m[:2] - dm[:2]) / np.linalg.norm(dm[:2]))
+    nnx, nnz = g.shape
+    g.close()
+    oracle_check = None
+    if args.oracle_check:
+        oracle_check = visco_oracle_check(erm, learned, p, (bvp, bvs, brho, qp, qs), dm)
+    # operator throughput: per shot and iteration one Born (2 states) and one adjoint (forward + backward)
+    updates = float(nnx * nnz) * p.lt * 4 * len(shots) * it
+    line = {
+        "metric": "2D viscoelastic LSRTM (plain CG): s per CG iteration",
+        "value": round(el / max(it, 1), 3), "unit": "s/iteration", "n_gpus": 1, "steps": it, "warmup": 0,
+        "ms_per_step": round(el / max(it, 1) * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded layered model, smooth random Q, Born data)",
+        "config": {"workload": f"2D visco LSRTM, {nx}x{nz}, d=8 m, L=3 SLS, {len(shots)} shots, 600 receivers, "
+                               f"20 Hz, nt={p.lt}, dt=0.8 ms, 40-cell damping, {args.iters} CG iterations",
+                   "workers": args.workers, "parallelism": "single GPU"},
+        "misfit": [round(float(v), 5) for v in mis], "seconds": [round(float(v), 2) for v in sec],
+        "model_error_vs_true": round(err, 4),
+        "operator_Gcell_updates_per_s": round(updates / el / 1e9, 3),
+        "oracle_check": oracle_check,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def visco_oracle_check(erm, learned, p, bg, dm, iters=2, nt=500):
+    """GPU CG against CGLS on the FP64 oracle's operators for one shot, on the
+    full grid with nt shortened: relative model difference per iteration."""
+    import ctypes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle2d
+    q = type(p)()
+    ctypes.memmove(ctypes.addressof(q), ctypes.addressof(p), ctypes.sizeof(q))
+    q.lt = q.lt_forward = nt
+    shot = (p.nx // 2, 2)
+    g = erm.Elastic2D(q)
+    g.load_model(1, *bg)
+    g.set_shot(*shot)
+    d = g.born(dm)
+    m_gpu, mis_gpu, _, _ = g.lsrtm_cg([shot], d[None], iters, pmask=3)
+    g.close()
+    o = oracle2d.Oracle2D(q, 64)
+    o.load_model(1, *bg)
+    o.set_shot(*shot)
+    mask = np.zeros((3, 1, 1), np.float32)
+    mask[:2] = 1
+    x = np.zeros((3, p.nx, p.nz), np.float64)
+    r = d.astype(np.float64)
+    s = o.adjoint(r.astype(np.float32)).astype(np.float64) * mask
+    pv = s.copy()
+    gam = float(np.sum(s * s))
+    mis = [1.0]
+    log("bench visco: oracle check: FP64 adjoint done")
+    for k in range(iters):
+        qd = o.born(pv.astype(np.float32)).astype(np.float64)
+        log(f"bench visco: oracle check: FP64 Born of iteration {k + 1} done")
+        al = gam / float(np.sum(qd * qd))
+        x += al * pv
+        r -= al * qd
+        mis.append(float(np.linalg.norm(r) / np.linalg.norm(d)))
+        s = o.adjoint(r.astype(np.float32)).astype(np.float64) * mask
+        gn = float(np.sum(s * s))
+        pv = s + gn / gam * pv
+        gam = gn
+    o.close()
+    e = float(np.linalg.norm(m_gpu[:2] - x[:2]) / np.linalg.norm(x[:2]))
+    return {"shot": shot, "nt": nt, "iterations": iters, "model_rel_diff_gpu_vs_fp64": e,
+            "misfit_gpu": [float(v) for v in mis_gpu], "misfit_fp64": mis}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,7 +304,11 @@ def main():
     ap.add_argument("--boundary", choices=["cpml", "damp"], default="cpml")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the plain copy kernel over GIB GiB (HBM counter calibration) and exit")
-    ap.add_argument("--workload", choices=["shot", "batch"], default="shot",
+    ap.add_argument("--shots", type=int, default=60, help="visco workload: shots")
+    ap.add_argument("--iters", type=int, default=20, help="visco workload: CG iterations")
+    ap.add_argument("--workers", type=int, default=4, help="visco workload: concurrent shot contexts")
+    ap.add_argument("--oracle-check", action="store_true", help="visco workload: FP64 oracle CG check (1 shot)")
+    ap.add_argument("--workload", choices=["shot", "batch", "visco"], default="shot",
                     help="shot: the one-shot 3D configuration (default); batch: 24 windowed shots over a "
                          "600x600x250 model in one batched, graph-captured launch per kernel per step")
     args = ap.parse_args()
@@ -213,6 +325,8 @@ def main():
         return
     if args.workload == "batch":
         return run_batch(erm, args)
+    if args.workload == "visco":
+        return run_visco(erm, args)
     nt = args.nt
     p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
     p.tile_y, p.zchunk = args.tile_y, args.zchunk

@@ -82,6 +82,9 @@ class Oracle2D:
         arrs = [_f(a) for a in (vp, vs, rho, qp, qs)]
         self._c(lib.oracle2d_load_model(self._h, which, *[_ptr(a, ctypes.c_float) for a in arrs]))
 
+    def set_shot(self, sx, sz):
+        self._c(lib.oracle2d_set_shot(self._h, sx, sz))
+
     def zero(self, what=1):
         self._c(lib.oracle2d_zero(self._h, what))
 
