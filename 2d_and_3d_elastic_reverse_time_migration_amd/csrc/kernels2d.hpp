@@ -24,6 +24,7 @@ struct G2 {
     float cx, cz, dtr;   // dt/dx, dt/dz (dt in s), dt
     const float* att;    // damping d per cell
     int bl, bu, nx, nz;  // model region
+    float qsign;         // sign of the memory-variable term in the stress update (-1: Q-compensated)
 };
 
 struct F2 {  // one wavefield state (forward, Born, or adjoint)
@@ -146,10 +147,13 @@ __global__ __launch_bounds__(TZ* TXR) void k2_stress(F2 f, M2 m, G2 g, Coef2 cf,
     }
     float xx = f.xx[n], zz = f.zz[n];
     if (n == src) { xx += srcval; zz += srcval; }
-    f.xx[n] = a * (b * xx + M * E0 + La * E1 + g.dtr * (sp + sxx));
-    f.zz[n] = a * (b * zz + La * E0 + M * E1 + g.dtr * (sp + szz));
-    f.tp[n] = a * (b * f.tp[n] + M * (E0 + E1) + g.dtr * sp);
-    f.xz[n] = a * (b * f.xz[n] + U * E2 + g.dtr * sxz);
+    // memory term dt * r; with qsign = -1 its sign is flipped (Q-compensated
+    // propagation, rfwd_txxzzxzpp_viscoelastic_and_memory, viscoelastic_equation.cu:1541-1569)
+    const float dq = g.dtr * g.qsign;
+    f.xx[n] = a * (b * xx + M * E0 + La * E1 + dq * (sp + sxx));
+    f.zz[n] = a * (b * zz + La * E0 + M * E1 + dq * (sp + szz));
+    f.tp[n] = a * (b * f.tp[n] + M * (E0 + E1) + dq * sp);
+    f.xz[n] = a * (b * f.xz[n] + U * E2 + dq * sxz);
 }
 
 // ---------------------------------------------------------------------------

@@ -96,6 +96,7 @@ struct erm2d_ctx {
     float* grad = nullptr;  // [3][mn]
     hipEvent_t ev[2] = {nullptr, nullptr};
     double last_ms = 0;
+    float qsign = 1.f;                      // -1 during a Q-compensated receiver pass
     std::vector<float> bg_host[5];          // background model as loaded (vp vs rho qp qs), for workers
     std::vector<erm2d_ctx*> workers;        // extra contexts (own streams) for concurrent shots
     erm2d_cg_monitor_fn monitor = nullptr;
@@ -137,6 +138,7 @@ G2 geo(const erm2d_ctx* c) {
     g.cx = c->cx; g.cz = c->cz; g.dtr = c->dtr;
     g.att = c->att;
     g.bl = c->p.bl; g.bu = c->p.bu; g.nx = c->p.nx; g.nz = c->p.nz;
+    g.qsign = c->qsign;
     return g;
 }
 F2 state(erm2d_ctx* c, bool B) {
@@ -234,6 +236,7 @@ int run(erm2d_ctx* c, int pass, int it_begin, int nsteps, float* recbuf) {
     if (!c->mod[which].have) return c->fail(ERM_E_STATE, "%s model not loaded", which ? "background" : "true");
     const long comp = (long)c->p.lt * c->p.nrec;
     const int rg = (c->p.nrec + 255) / 256;
+    c->qsign = pass == ERM2D_PASS_RECEIVER && c->p.qcomp ? -1.f : 1.f;
     for (int s = 0; s < nsteps; ++s) {
         if (pass == ERM2D_PASS_RECEIVER) {
             const int it = it_begin - s;
@@ -252,6 +255,7 @@ int run(erm2d_ctx* c, int pass, int it_begin, int nsteps, float* recbuf) {
                                    c->p.nrec, recbuf, comp, (long)it * c->p.nrec);
         }
     }
+    c->qsign = 1.f;
     HC(c, hipGetLastError());
     HC(c, hipStreamSynchronize(c->stream));
     return 0;

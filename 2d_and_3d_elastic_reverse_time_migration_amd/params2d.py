@@ -15,7 +15,7 @@ class Params2D(ctypes.Structure):
                 + [("coe_attenuation", ctypes.c_float)]
                 + [(k, ctypes.c_int) for k in ["fd_order", "nsls", "shot_x", "shot_z", "nrec", "rec_x0", "rec_dx",
                                                 "rec_z"]]
-                + [("reserved", ctypes.c_int * 8)])
+                + [("qcomp", ctypes.c_int), ("reserved", ctypes.c_int * 7)])
 
     def update(self, **kw):
         for k, v in kw.items():

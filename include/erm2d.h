@@ -34,7 +34,9 @@ typedef struct erm2d_params {
     int   nsls;        /* 0: elastic; 1: reference SLS (viscoelastic_equation.cu:22-58); 3: three mechanisms */
     int   shot_x, shot_z;              /* source, model coordinates (cells) */
     int   nrec, rec_x0, rec_dx, rec_z; /* receiver line, model coordinates */
-    int   reserved[8];
+    int   qcomp;       /* 1: Q-compensated receiver pass of erm2d_rtm (memory-term sign flipped,
+                          rfwd_/receiver_fwd_ kernels, viscoelastic_equation.cu:1438-1822) */
+    int   reserved[7];
 } erm2d_params;
 
 /* fields for erm2d_get_field / erm2d_set_field (padded grid, z fastest) */

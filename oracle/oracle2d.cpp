@@ -98,6 +98,7 @@ struct Oracle2D {
     size_t N = 0;
     T c[7] = {0};
     T cx = 0, cz = 0, dtr = 0;
+    T qsign = 1;  // -1 during a Q-compensated receiver pass
     enum { VX, VZ, XX, ZZ, XZ, TP, VXP, VZP, NF };
     std::vector<T> f[NF];
     std::vector<T> r[3][4];  /* memory: p, xx, zz, xz per mechanism */
@@ -235,10 +236,13 @@ struct Oracle2D {
                     rv[l][0][n] = rp; rv[l][1][n] = rxx; rv[l][2][n] = rzz; rv[l][3][n] = rxz;
                     sp += rp; sxx += rxx; szz += rzz; sxz += rxz;
                 }
-                fv[XX][n] = a * (b * fv[XX][n] + M * E[0] + La * E[1] + dtr * (sp + sxx));
-                fv[ZZ][n] = a * (b * fv[ZZ][n] + La * E[0] + M * E[1] + dtr * (sp + szz));
-                fv[TP][n] = a * (b * fv[TP][n] + M * (E[0] + E[1]) + dtr * sp);
-                fv[XZ][n] = a * (b * fv[XZ][n] + U * E[2] + dtr * sxz);
+                // qsign = -1: Q-compensated receiver propagation, the memory term's sign
+                // flipped as in rfwd_txxzzxzpp_viscoelastic_and_memory (viscoelastic_equation.cu:1541-1569)
+                const T dq = dtr * qsign;
+                fv[XX][n] = a * (b * fv[XX][n] + M * E[0] + La * E[1] + dq * (sp + sxx));
+                fv[ZZ][n] = a * (b * fv[ZZ][n] + La * E[0] + M * E[1] + dq * (sp + szz));
+                fv[TP][n] = a * (b * fv[TP][n] + M * (E[0] + E[1]) + dq * sp);
+                fv[XZ][n] = a * (b * fv[XZ][n] + U * E[2] + dq * sxz);
             }
     }
 
@@ -292,7 +296,9 @@ struct Oracle2D {
                     f[VX][recs[q]] += rec[((size_t)0 * p.lt + it) * p.nrec + q];
                     f[VZ][recs[q]] += rec[((size_t)1 * p.lt + it) * p.nrec + q];
                 }
+                qsign = p.qcomp ? (T)-1 : (T)1;
                 stress(f, r, m);
+                qsign = 1;
                 velocity(f, m, 2, it);
             } else {
                 int it = it_begin + s;
