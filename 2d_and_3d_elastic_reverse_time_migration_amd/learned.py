@@ -265,7 +265,7 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
 
 def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7, test_shape=(256, 96),
         test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, epochs=20, batch=32, lr=1e-3,
-        channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, warm=5, flexible=True, log=print):
+        channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, warm=20, flexible=True, log=print):
     """The whole workflow; returns a dict of results."""
     res = {"config": dict(train_models=train_models, train_shape=train_shape, train_shots=train_shots,
                           train_lt=train_lt, cg_iters=cg_iters, test_shape=test_shape, test_shots=test_shots,
@@ -289,7 +289,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--test-iters", type=int, default=400)
     ap.add_argument("--workers", type=int, default=4)
-    ap.add_argument("--warm", type=int, nargs="*", default=[5],
+    ap.add_argument("--warm", type=int, nargs="*", default=[20],
                     help="learned warm-start lengths to compare (net iterations before plain CG)")
     ap.add_argument("--no-flexible", action="store_true", help="skip the every-iteration learned CG run")
     a = ap.parse_args(argv)
