@@ -6,12 +6,13 @@
 //   A[m][k]: m = l % 16, k = 4*(l/16) + j
 //   B[k][n]: n = l % 16, k = 4*(l/16) + j
 //   D[m][n]: n = l % 16, m = 4*(l/16) + j
-// One wave owns a 16x16 output tile; a workgroup of 4 waves covers 16 x 64.
 //   forward        Y[co][p]  = sum_{ci,t} W[co][ci][t] X[ci][p+o(t)]   (M=Cout, N=pixels, K=Cin*9)
 //   backward data  dX[ci][p] = sum_{co,t} W[co][ci][8-t] dY[co][p+o(t)] (M=Cin,  N=pixels, K=Cout*9)
-//   backward wts   dW[co][ci*9+t] = sum_p dY[co][p] X[ci][p+o(t)]       (M=Cout, N=Cin*9,  K=pixels, split-K)
+//   backward wts   dW[co][ci][t] = sum_p dY[co][p] X[ci][p+o(t)]        (M=Cout, N=Cin*9,  K=pixels, split-K)
 // with o(t) = (t/3-1, t%3-1) and zero padding; dY is masked by the ReLU of
-// the layer output where the layer has one.
+// the layer output where the layer has one.  Operands are staged in LDS as
+// bf16 (round to nearest even) so each lane reads its 4 k-values as one
+// 8-byte word.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,101 +51,195 @@ struct ConvArgs {
     float* out;
     int relu;
     int accumulate;
-    int kchunk;          // BWD_W: K per workgroup
 };
 
-__device__ __forceinline__ float sample(const float* t, int c, int C, int b, int yy, int xx, int h, int w) {
-    if (yy < 0 || yy >= h || xx < 0 || xx >= w) return 0.f;
-    return t[(((long)b * C + c) * h + yy) * w + xx];
-}
+// LDS-tiled implicit GEMM for the forward and backward-data convolutions.
+// A workgroup owns an 8x8 pixel tile of one image and up to 64 output
+// channels; the K loop runs over 16-channel chunks, and inside a chunk over
+// the 9 taps, so one MFMA k-step is (tap t, 16 channels).  Per chunk the
+// input patch (10x10 pixels with the 1-pixel halo, channel fastest) and the
+// weights ([64 rows][9 taps][16 channels]) are converted to bf16 into LDS
+// once; every lane then reads its 4 k-values of A and of B as one 8-byte LDS
+// word.  Waves take the (16-row, 16-pixel) output subtiles round-robin.
+//   FWD      : in = x (cin channels), A[m][t][c] = W[m][c][t]
+//   BWD_DATA : in = dy masked by y > 0 (cout channels), A[m][t][c] = W[c][m][8-t]
+constexpr int TP = 8;             // pixel tile edge
+constexpr int TPH = TP + 2;       // with halo
+constexpr int CK = 16;            // channels per chunk
+constexpr int MT = 64;            // output channels per workgroup
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_conv_mfma(ConvArgs a) {
+__global__ __launch_bounds__(256) void k_conv_tiled(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) short patch[TPH * TPH * CK];
+    __shared__ __attribute__((aligned(16))) short wl[MT * 9 * CK];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int hw = a.h * a.w;
-    int M, N, K;
-    if (MODE == FWD) { M = a.cout; N = a.n * hw; K = a.cin * 9; }
-    else if (MODE == BWD_DATA) { M = a.cin; N = a.n * hw; K = a.cout * 9; }
-    else { M = a.cout; N = a.cin * 9; K = a.n * hw; }
-    const int m0 = blockIdx.y * 16, n0 = (blockIdx.x * 4 + wave) * 16;
-    const int kb = MODE == BWD_W ? blockIdx.z * a.kchunk : 0;
-    const int ke = MODE == BWD_W ? min(K, kb + a.kchunk) : K;
+    const int Cin = MODE == FWD ? a.cin : a.cout;    // channels of the GEMM's input side
+    const int M = MODE == FWD ? a.cout : a.cin;      // output channels
+    const int tx = (a.w + TP - 1) / TP, ty = (a.h + TP - 1) / TP;
+    const int tile = blockIdx.x, b = tile / (tx * ty), r = tile % (tx * ty);
+    const int y0 = (r / tx) * TP, x0 = (r % tx) * TP;
+    const int m0 = blockIdx.y * MT;
+    const int Mb = min(4, (M - m0 + 15) / 16);       // 16-row blocks in this workgroup
+    const long hw = (long)a.h * a.w;
+    const float* in = MODE == FWD ? a.x : a.dy;
     const int li = lane & 15, lk = 4 * (lane >> 4);
-    const int am = m0 + li;  // A row of this lane
-    const int bn = n0 + li;  // B column of this lane
-    // decode this lane's B column once (pixel for FWD/BWD_DATA, (ci,t) for BWD_W)
-    int pb = 0, py = 0, px = 0, bci = 0, bt = 0;
-    if (MODE != BWD_W) {
-        if (bn < N) { pb = bn / hw; const int r = bn - pb * hw; py = r / a.w; px = r - py * a.w; }
-    } else if (bn < N) { bci = bn / 9; bt = bn - bci * 9; }
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kb; k0 < ke; k0 += 16) {
-        bf16x4 av, bv;
+    f32x4 acc[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = k0 + lk + j;
-            float va = 0.f, vb = 0.f;
-            if (k < ke) {
-                if (MODE == FWD) {
-                    if (am < M) va = __int_as_float(((int)(unsigned short)a.wb[(long)am * K + k]) << 16);
-                    if (bn < N) {
-                        const int ci = k / 9, t = k - ci * 9;
-                        vb = sample(a.x, ci, a.cin, pb, py + t / 3 - 1, px + t % 3 - 1, a.h, a.w);
-                    }
-                } else if (MODE == BWD_DATA) {
-                    const int co = k / 9, t = k - co * 9;
-                    if (am < M) va = __int_as_float(((int)(unsigned short)a.wb[((long)co * a.cin + am) * 9 + (8 - t)]) << 16);
-                    if (bn < N) {
-                        const int yy = py + t / 3 - 1, xx = px + t % 3 - 1;
-                        float g = sample(a.dy, co, a.cout, pb, yy, xx, a.h, a.w);
-                        if (a.relu && g != 0.f && sample(a.y, co, a.cout, pb, yy, xx, a.h, a.w) <= 0.f) g = 0.f;
-                        vb = g;
-                    }
-                } else {
-                    const int b = k / hw, r = k - b * hw, yy = r / a.w, xx = r - yy * a.w;
-                    if (am < M) {
-                        float g = a.dy[(((long)b * a.cout + am) * a.h + yy) * a.w + xx];
-                        if (a.relu && a.y[(((long)b * a.cout + am) * a.h + yy) * a.w + xx] <= 0.f) g = 0.f;
-                        va = g;
-                    }
-                    if (bn < N) vb = sample(a.x, bci, a.cin, b, yy + bt / 3 - 1, xx + bt % 3 - 1, a.h, a.w);
-                }
+    for (int q = 0; q < 4; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c0 = 0; c0 < Cin; c0 += CK) {
+        __syncthreads();  // previous chunk's LDS reads are done
+        for (int e = threadIdx.x; e < TPH * TPH * CK; e += 256) {
+            const int px = e % TPH, py = (e / TPH) % TPH, c = e / (TPH * TPH);
+            const int yy = y0 - 1 + py, xx = x0 - 1 + px, ch = c0 + c;
+            float v = 0.f;
+            if (ch < Cin && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) {
+                const long gi = ((long)b * Cin + ch) * hw + (long)yy * a.w + xx;
+                v = in[gi];
+                if (MODE == BWD_DATA && a.relu && v != 0.f && a.y[gi] <= 0.f) v = 0.f;
             }
-            av[j] = MODE == FWD || MODE == BWD_DATA ? (short)(__float_as_uint(va) >> 16) : to_bf16(va);
-            bv[j] = to_bf16(vb);
+            patch[(py * TPH + px) * CK + c] = to_bf16(v);
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+        for (int e = threadIdx.x; e < MT * 9 * CK; e += 256) {
+            int m, c, t;
+            short v = 0;
+            if (MODE == FWD) {  // global [co][ci][t]: (m, c, t) with t fastest
+                t = e % 9; c = (e / 9) % CK; m = e / (9 * CK);
+                if (m0 + m < M && c0 + c < Cin) v = a.wb[((long)(m0 + m) * Cin + c0 + c) * 9 + t];
+            } else {            // A[m][t][c] = W[c0+c][m0+m][8-t]; global [co][ci][t] with m = ci
+                t = e % 9; m = (e / 9) % MT; c = e / (9 * MT);
+                if (m0 + m < M && c0 + c < Cin) v = a.wb[((long)(c0 + c) * a.cin + m0 + m) * 9 + (8 - t)];
+            }
+            wl[(m * 9 + t) * CK + c] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int dy = t / 3, dx = t % 3;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int st = wave + 4 * q;  // subtile: 16-row block st/4, 16-pixel block st%4
+                if (st >= Mb * 4) break;
+                const int mb = st >> 2, nb = st & 3;
+                const int n = nb * 16 + li, py = n / TP, px = n % TP;
+                const bf16x4 av = *reinterpret_cast<const bf16x4*>(&wl[((mb * 16 + li) * 9 + t) * CK + lk]);
+                const bf16x4 bv = *reinterpret_cast<const bf16x4*>(&patch[((py + dy) * TPH + px + dx) * CK + lk]);
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+            }
+        }
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int m = m0 + lk + j, n = n0 + li;
-        if (m >= M || n >= N) continue;
-        if (MODE == FWD) {
-            const int b = n / hw, r = n - b * hw;
-            float v = acc[j] + a.bias[m];
-            if (a.relu) v = fmaxf(v, 0.f);
-            a.out[((long)b * a.cout + m) * hw + r] = v;
-        } else if (MODE == BWD_DATA) {
-            const int b = n / hw, r = n - b * hw;
-            float* o = &a.out[((long)b * a.cin + m) * hw + r];
-            *o = a.accumulate ? *o + acc[j] : acc[j];
-        } else {
-            atomicAdd(&a.out[(long)m * N + n], acc[j]);
+    for (int q = 0; q < 4; ++q) {
+        const int st = wave + 4 * q;
+        if (st >= Mb * 4) break;
+        const int mb = st >> 2, nb = st & 3;
+        const int n = nb * 16 + li, yy = y0 + n / TP, xx = x0 + n % TP;
+        if (yy >= a.h || xx >= a.w) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = m0 + mb * 16 + lk + j;
+            if (m >= M) continue;
+            const long o = ((long)b * M + m) * hw + (long)yy * a.w + xx;
+            if (MODE == FWD) {
+                float v = acc[q][j] + a.bias[m];
+                a.out[o] = a.relu ? fmaxf(v, 0.f) : v;
+            } else {
+                a.out[o] = a.accumulate ? a.out[o] + acc[q][j] : acc[q][j];
+            }
         }
     }
 }
 
-// d bias[co] = sum over batch and pixels of the (ReLU-masked) output gradient
+// Weight gradient dW[co][ci][t] = sum_p dY[co][p] X[ci][p + o(t)] as a GEMM
+// with K = pixels: a workgroup owns up to 64 output channels x one 16-channel
+// input chunk (144 (tap, channel) columns = 9 subtiles of 16) and strides over
+// 8x8 pixel tiles (split-K over gridDim.z), accumulating in registers, with one
+// atomic add per output at the end.  Per pixel tile, dY (ReLU-masked) goes to
+// LDS as [co][64 pixels] and the input patch as three copies shifted by
+// dx = 0, 1, 2 ([c][10 rows][8]), so every lane reads its 4 consecutive
+// pixels of A and of B as one 8-byte LDS word.
+__global__ __launch_bounds__(256) void k_conv_wgrad(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) short dyl[MT * TP * TP];
+    __shared__ __attribute__((aligned(16))) short xs[3 * CK * TPH * TP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = 4 * (lane >> 4);
+    const int c0 = blockIdx.x * CK, m0 = blockIdx.y * MT;
+    const int M = a.cout, Cin = a.cin;
+    const int Mb = min(4, (M - m0 + 15) / 16);
+    const int tx = (a.w + TP - 1) / TP, ty = (a.h + TP - 1) / TP, ntile = a.n * tx * ty;
+    const long hw = (long)a.h * a.w;
+    f32x4 acc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int tile = blockIdx.z; tile < ntile; tile += gridDim.z) {
+        const int b = tile / (tx * ty), r = tile % (tx * ty);
+        const int y0 = (r / tx) * TP, x0 = (r % tx) * TP;
+        __syncthreads();
+        for (int e = threadIdx.x; e < MT * TP * TP; e += 256) {
+            const int px = e % TP, py = (e / TP) % TP, m = e / (TP * TP);
+            const int yy = y0 + py, xx = x0 + px;
+            float v = 0.f;
+            if (m0 + m < M && yy < a.h && xx < a.w) {
+                const long gi = ((long)b * M + m0 + m) * hw + (long)yy * a.w + xx;
+                v = a.dy[gi];
+                if (a.relu && v != 0.f && a.y[gi] <= 0.f) v = 0.f;
+            }
+            dyl[m * TP * TP + py * TP + px] = to_bf16(v);
+        }
+        for (int e = threadIdx.x; e < CK * TPH * TPH; e += 256) {
+            const int px = e % TPH, py = (e / TPH) % TPH, c = e / (TPH * TPH);
+            const int yy = y0 - 1 + py, xx = x0 - 1 + px, ch = c0 + c;
+            float v = 0.f;
+            if (ch < Cin && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w) v = a.x[((long)b * Cin + ch) * hw + (long)yy * a.w + xx];
+            const short hv = to_bf16(v);
+            // copy d holds column px - d of the tile's x range: xs[d][c][py][px - d]
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (px - d >= 0 && px - d < TP) xs[((d * CK + c) * TPH + py) * TP + px - d] = hv;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {             // 16 pixels per k-step: rows 2ks, 2ks+1
+            const int pk = ks * 16 + lk;             // this lane's first pixel of the k-step
+            const int py = pk / TP, px = pk % TP;    // 4 consecutive pixels in one row
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const int st = wave + 4 * q;         // subtile = (16-row block, tap)
+                if (st >= Mb * 9) break;
+                const int mb = st / 9, t = st % 9, dy = t / 3, dx = t % 3;
+                const bf16x4 av = *reinterpret_cast<const bf16x4*>(&dyl[(mb * 16 + li) * TP * TP + pk]);
+                const bf16x4 bv = *reinterpret_cast<const bf16x4*>(&xs[((dx * CK + li) * TPH + py + dy) * TP + px]);
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int st = wave + 4 * q;
+        if (st >= Mb * 9) break;
+        const int mb = st / 9, t = st % 9;
+        const int c = c0 + li;
+        if (c >= Cin) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int m = m0 + mb * 16 + lk + j;
+            if (m < M) atomicAdd(&a.out[((long)m * Cin + c) * 9 + t], acc[q][j]);
+        }
+    }
+}
+
+// d bias[co] = sum over batch and pixels of the (ReLU-masked) output gradient;
+// grid (channels, pixel chunks of 4096), one atomic add per block
 __global__ void k_bias_grad(const float* dy, const float* y, int relu, int n, int c, int hw, float* db) {
     const int co = blockIdx.x;
+    const long total = (long)n * hw;
     float s = 0.f;
-    for (int b = 0; b < n; ++b)
-        for (int p = threadIdx.x; p < hw; p += blockDim.x) {
-            const long i = ((long)b * c + co) * hw + p;
-            float g = dy[i];
-            if (relu && y[i] <= 0.f) g = 0.f;
-            s += g;
-        }
+    for (long e = (long)blockIdx.y * 4096 + threadIdx.x; e < min(total, (long)(blockIdx.y + 1) * 4096); e += blockDim.x) {
+        const long b = e / hw, p = e % hw;
+        const long i = (b * c + co) * hw + p;
+        float g = dy[i];
+        if (relu && y[i] <= 0.f) g = 0.f;
+        s += g;
+    }
     __shared__ float red[256];
     red[threadIdx.x] = s;
     __syncthreads();
@@ -152,7 +247,7 @@ __global__ void k_bias_grad(const float* dy, const float* y, int relu, int n, in
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[co] += red[0];
+    if (threadIdx.x == 0) atomicAdd(&db[co], red[0]);
 }
 
 // 2x2 average pooling; backward accumulates
@@ -332,10 +427,10 @@ void conv_fwd(erm_unet* u, int li, const float* x, float* y, int n, int h, int w
     ConvArgs a{};
     a.n = n; a.h = h; a.w = w; a.cin = L.cin; a.cout = L.cout;
     a.x = x; a.wb = u->pb + L.woff; a.bias = u->p + L.boff; a.out = y; a.relu = L.relu;
-    const long N = (long)n * h * w;
-    dim3 grd((unsigned)((N + 63) / 64), (unsigned)((L.cout + 15) / 16));
-    hipLaunchKernelGGL(k_conv_mfma<FWD>, grd, dim3(256), 0, u->stream, a);
-    u->flops_acc += 2.0 * ((L.cout + 15) / 16 * 16) * (double)((N + 63) / 64 * 64) * ((L.cin * 9 + 15) / 16 * 16);
+    const long tiles = (long)n * ((h + TP - 1) / TP) * ((w + TP - 1) / TP);
+    dim3 grd((unsigned)tiles, (unsigned)((L.cout + MT - 1) / MT));
+    hipLaunchKernelGGL(k_conv_tiled<FWD>, grd, dim3(256), 0, u->stream, a);
+    u->flops_acc += 2.0 * ((L.cout + 15) / 16 * 16) * (double)(tiles * TP * TP) * ((L.cin + CK - 1) / CK * CK * 9);
 }
 
 // backward of conv li: dy is the gradient w.r.t. its output y; adds dW/db into u->g,
@@ -350,20 +445,22 @@ void conv_bwd(erm_unet* u, int li, const float* x, const float* y, const float* 
     if (dx) {
         a.out = dx;
         a.accumulate = acc_dx;
-        dim3 grd((unsigned)((P + 63) / 64), (unsigned)((L.cin + 15) / 16));
-        hipLaunchKernelGGL(k_conv_mfma<BWD_DATA>, grd, dim3(256), 0, u->stream, a);
-        u->flops_acc += 2.0 * ((L.cin + 15) / 16 * 16) * (double)((P + 63) / 64 * 64) * ((L.cout * 9 + 15) / 16 * 16);
+        const long tiles = (long)n * ((h + TP - 1) / TP) * ((w + TP - 1) / TP);
+        dim3 grd((unsigned)tiles, (unsigned)((L.cin + MT - 1) / MT));
+        hipLaunchKernelGGL(k_conv_tiled<BWD_DATA>, grd, dim3(256), 0, u->stream, a);
+        u->flops_acc += 2.0 * ((L.cin + 15) / 16 * 16) * (double)(tiles * TP * TP) * ((L.cout + CK - 1) / CK * CK * 9);
     }
     {
         a.out = u->g + L.woff;
-        const int Nw = L.cin * 9;
-        const int kchunk = 1024;  // pixels per workgroup (multiple of 16)
-        dim3 grd((unsigned)((Nw + 63) / 64), (unsigned)((L.cout + 15) / 16), (unsigned)((P + kchunk - 1) / kchunk));
-        a.kchunk = kchunk;
-        hipLaunchKernelGGL(k_conv_mfma<BWD_W>, grd, dim3(256), 0, u->stream, a);
-        u->flops_acc += 2.0 * ((L.cout + 15) / 16 * 16) * (double)((Nw + 63) / 64 * 64) * ((P + 15) / 16 * 16);
-        hipLaunchKernelGGL(k_bias_grad, dim3(L.cout), dim3(256), 0, u->stream, dy, y, L.relu, n, L.cout, h * w,
-                           u->g + L.boff);
+        const int cchunks = (L.cin + CK - 1) / CK, mblocks = (L.cout + MT - 1) / MT;
+        const long tiles = (long)n * ((h + TP - 1) / TP) * ((w + TP - 1) / TP);
+        // split-K: about 1024 workgroups in all, at least 4 pixel tiles each
+        const int ks = (int)std::max(1L, std::min(tiles / 4, 1024L / (cchunks * mblocks)));
+        hipLaunchKernelGGL(k_conv_wgrad, dim3(cchunks, mblocks, ks), dim3(256), 0, u->stream, a);
+        u->flops_acc += 2.0 * ((L.cout + 15) / 16 * 16) * (double)(cchunks * CK * 9) * (double)(tiles * TP * TP);
+        const long chunks = (P + 4095) / 4096;
+        hipLaunchKernelGGL(k_bias_grad, dim3(L.cout, (unsigned)chunks), dim3(256), 0, u->stream, dy, y, L.relu, n,
+                           L.cout, h * w, u->g + L.boff);
     }
 }
 
