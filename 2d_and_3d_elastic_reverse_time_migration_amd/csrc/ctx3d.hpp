@@ -44,8 +44,7 @@ struct erm_ctx {
     int tx = 64, ty = 8, kchunk = 32, xcd_map = 0;
     // CPML (boundary == ERM_BOUNDARY_CPML)
     bool cpml = false;
-    float* pml_coef = nullptr;          // a,b x {full,half} per axis, packed
-    float* psi[21] = {nullptr};
+    float* psi_buf = nullptr;           // C-PML state: 21 memory variables, then coefficients (layout: Pml)
     int slab_lo[3] = {0, 0, 0}, slab_hi[3] = {0, 0, 0}, slab_w[3] = {0, 0, 0};
     Pml pml{};
     // region split of the CPML launches: tiles are grouped by the set of axes

@@ -96,30 +96,57 @@ enum { MODE_PLAIN = 0, MODE_EXCITE = 1, MODE_IMAGE = 2 };
 
 // C-PML state (no reference counterpart; the reference damps instead).
 // Coefficients a, b per axis at integer ("full") and staggered ("half":
-// x-1/2, y-1/2, z+1/2) positions; 21 memory variables stored only in the
-// absorbing slabs of their derivative's axis: along an axis of n cells the
-// slab is [0, lo+1) U [n-hi-1, n) (width w = lo+hi+2, covering the half
-// positions too).  Layouts: x slabs [nnz][nny][wx], y slabs [nnz][wy][nnx],
-// z slabs [wz][nny][nnx].  psi ids: stress 0-2 (x: exx, exy, exz), 3-5
-// (y: eyy, eyx, eyz), 6-8 (z: ezz, ezx, ezy); velocity 9-12 (x: sxx, sxy,
-// sxz, tp), 13-16 (y: sxy, syy, syz, tp), 17-20 (z: sxz, syz, szz, tp).
+// x-1/2, y-1/2, z+1/2) positions, interleaved per index in one array: axis a
+// starts at float 4*(n_0 + .. + n_{a-1}) and holds n_a quadruples
+// (a_full, b_full, a_half, b_half), so one 16-byte load fetches a position's
+// four coefficients.
+// 21 memory variables stored only in the absorbing slabs of their
+// derivative's axis: along an axis of n cells the slab is [0, lo+1) U
+// [n-hi-1, n) (width w = lo+hi+2, covering the half positions too).  Slab
+// cell layouts: x slabs [nnz][nny][wx], y slabs [nnz][wy][nnx], z slabs
+// [wz][nny][nnx]; sz[a] = slab cells rounded up to a multiple of 4.  The
+// variables of one slab cell are interleaved: per axis a stress-side block
+// of sz[a] triples at psi_stress(P, a) (x: exx, exy, exz; y: eyy, eyx, eyz;
+// z: ezz, ezx, ezy) followed by a velocity-side block of sz[a] quadruples at
+// psi_velocity(P, a) (x: sxx, sxy, sxz, tp; y: sxy, syy, syz, tp; z: sxz,
+// syz, szz, tp).  The coefficients follow the memory variables in the same
+// buffer (at float psi_total).  One load and one store per axis and cell,
+// and one buffer resource for the whole state, keep the C-PML kernels'
+// scalar registers from spilling.
 struct Pml {
-    const float* a[3][2];
-    const float* b[3][2];
-    float* psi[21];
+    float* psi;
     int lo[3], hi[3], w[3];
+    unsigned sz[3];
 };
+// float offsets of the stress-side and velocity-side blocks of axis a
+__host__ __device__ __forceinline__ unsigned psi_stress(const Pml& P, int a) {
+    return a == 0 ? 0u : a == 1 ? 7u * P.sz[0] : 7u * (P.sz[0] + P.sz[1]);
+}
+__host__ __device__ __forceinline__ unsigned psi_velocity(const Pml& P, int a) {
+    return psi_stress(P, a) + 3u * P.sz[a];
+}
+__host__ __device__ __forceinline__ unsigned psi_total(const Pml& P) { return 7u * (P.sz[0] + P.sz[1] + P.sz[2]); }
+// float offset of axis a's coefficient quadruples (in the coefficient array)
+__host__ __device__ __forceinline__ unsigned coef_base(int nnx, int nny, int a) {
+    return a == 0 ? 0u : a == 1 ? 4u * nnx : 4u * (nnx + nny);
+}
+// the same, in the state buffer
+__host__ __device__ __forceinline__ unsigned pml_coef(const Pml& P, int nnx, int nny, int a) {
+    return psi_total(P) + coef_base(nnx, nny, a);
+}
+__host__ __device__ __forceinline__ size_t pml_floats(const Pml& P, int nnx, int nny, int nnz) {
+    return (size_t)psi_total(P) + 4 * (size_t)(nnx + nny + nnz);
+}
 
 __device__ __forceinline__ int slab_of(int i, int n, int lo, int hi) {
     return i <= lo ? i : (i >= n - hi - 1 ? lo + 1 + i - (n - hi - 1) : -1);
 }
-// e += (psi = b psi + a e) where the profile is non-zero; `old` is psi[q],
-// loaded early (with the plane's other loads) so its latency overlaps theirs
-__device__ __forceinline__ float pml_upd(float* psi, unsigned q, float a, float b, float e, float old) {
-    if (a == 0.f) return e;
-    const float v = b * old + a * e;
-    psi[q] = v;
-    return e + v;
+// e += (psi = b psi + a e).  Where a position's profile is zero (a = 0)
+// psi stays at its initial 0 and e passes unchanged, so every slab cell can
+// be updated and stored unconditionally.
+__device__ __forceinline__ float pml_upd(float a, float b, float e, float& psi) {
+    psi = b * psi + a * e;
+    return e + psi;
 }
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -143,6 +170,20 @@ __device__ __forceinline__ void bst(rsrc_t r, unsigned vo, unsigned so, float v)
 }
 __device__ __forceinline__ int bldi(rsrc_t r, unsigned vo, unsigned so) {
     return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0);
+}
+// N consecutive floats per lane.  Written as dword accesses at adjacent
+// offsets, which the compiler merges into one dwordx3 / dwordx4 access (the
+// vector-typed buffer builtins are avoided: this toolchain lowers
+// element-wise use of their results to a single-dword load).
+template <int N>
+__device__ __forceinline__ void bldn(rsrc_t r, unsigned vo, unsigned so, float* v) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) v[c] = bld(r, vo + 4u * c, so);
+}
+template <int N>
+__device__ __forceinline__ void bstn(rsrc_t r, unsigned vo, unsigned so, const float* v) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) bst(r, vo + 4u * c, so, v[c]);
 }
 
 // ---------------------------------------------------------------------------
@@ -210,6 +251,8 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
     const rsrc_t rxx = mkrs(w.txx + base, span), ryy = mkrs(w.tyy + base, span), rzz = mkrs(w.tzz + base, span);
     const rsrc_t rxy = mkrs(w.txy + base, span), rxz = mkrs(w.txz + base, span), ryz = mkrs(w.tyz + base, span);
     const rsrc_t rtp = mkrs(w.tp + base, span), rM = mkrs(Mm + base, span), rU = mkrs(MUm + base, span);
+    // C-PML state (unused, and removed, when AX == 0)
+    [[maybe_unused]] const rsrc_t rpsi = mkrs(P.psi, 4L * (long)pml_floats(P, g.nnx, g.nny, g.nnz));
 
     float qx[2 * R + 1], qy[2 * R + 1], qz[2 * R + 1];
 #pragma unroll
@@ -253,15 +296,15 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
             szi = (AX & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
             if ((AX & 1) && sxi >= 0) {
                 pqx = ((unsigned)k * g.nny + j) * P.w[0] + sxi;
-                ps[0] = P.psi[0][pqx]; ps[1] = P.psi[1][pqx]; ps[2] = P.psi[2][pqx];
+                bldn<3>(rpsi, 12u * pqx, 4u * psi_stress(P, 0), ps);
             }
             if ((AX & 2) && syi >= 0) {
                 pqy = ((unsigned)k * P.w[1] + syi) * g.nnx + i;
-                ps[3] = P.psi[3][pqy]; ps[4] = P.psi[4][pqy]; ps[5] = P.psi[5][pqy];
+                bldn<3>(rpsi, 12u * pqy, 4u * psi_stress(P, 1), ps + 3);
             }
             if ((AX & 4) && szi >= 0) {
                 pqz = ((unsigned)szi * g.nny + j) * g.nnx + i;
-                ps[6] = P.psi[6][pqz]; ps[7] = P.psi[7][pqz]; ps[8] = P.psi[8][pqz];
+                bldn<3>(rpsi, 12u * pqz, 4u * psi_stress(P, 2), ps + 6);
             }
         }
         __syncthreads();  // previous plane's LDS reads are done
@@ -301,22 +344,27 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
             float e_zy = g.cz * dzvy, e_yz = g.cy * dyvz;
             if constexpr (AX != 0) {
                 if ((AX & 1) && sxi >= 0) {
-                    const float af = P.a[0][0][i], bf = P.b[0][0][i], ah = P.a[0][1][i], bh = P.b[0][1][i];
-                    exx = pml_upd(P.psi[0], pqx, af, bf, exx, ps[0]);
-                    e_xy = pml_upd(P.psi[1], pqx, ah, bh, e_xy, ps[1]);
-                    e_xz = pml_upd(P.psi[2], pqx, ah, bh, e_xz, ps[2]);
+                    float cq[4];
+                    bldn<4>(rpsi, 16u * i, 4u * pml_coef(P, g.nnx, g.nny, 0), cq);  // (a_full, b_full, a_half, b_half) of x
+                    exx = pml_upd(cq[0], cq[1], exx, ps[0]);
+                    e_xy = pml_upd(cq[2], cq[3], e_xy, ps[1]);
+                    e_xz = pml_upd(cq[2], cq[3], e_xz, ps[2]);
+                    bstn<3>(rpsi, 12u * pqx, 4u * psi_stress(P, 0), ps);
                 }
                 if ((AX & 2) && syi >= 0) {
-                    const float af = P.a[1][0][j], bf = P.b[1][0][j], ah = P.a[1][1][j], bh = P.b[1][1][j];
-                    eyy = pml_upd(P.psi[3], pqy, af, bf, eyy, ps[3]);
-                    e_yx = pml_upd(P.psi[4], pqy, ah, bh, e_yx, ps[4]);
-                    e_yz = pml_upd(P.psi[5], pqy, ah, bh, e_yz, ps[5]);
+                    float cq[4];
+                    bldn<4>(rpsi, 16u * j, 4u * pml_coef(P, g.nnx, g.nny, 1), cq);
+                    eyy = pml_upd(cq[0], cq[1], eyy, ps[3]);
+                    e_yx = pml_upd(cq[2], cq[3], e_yx, ps[4]);
+                    e_yz = pml_upd(cq[2], cq[3], e_yz, ps[5]);
+                    bstn<3>(rpsi, 12u * pqy, 4u * psi_stress(P, 1), ps + 3);
                 }
                 if ((AX & 4) && szi >= 0) {
-                    const float af = P.a[2][0][k], bf = P.b[2][0][k], ah = P.a[2][1][k], bh = P.b[2][1][k];
-                    ezz = pml_upd(P.psi[6], pqz, af, bf, ezz, ps[6]);
-                    e_zx = pml_upd(P.psi[7], pqz, ah, bh, e_zx, ps[7]);
-                    e_zy = pml_upd(P.psi[8], pqz, ah, bh, e_zy, ps[8]);
+                    const float4 cq = reinterpret_cast<const float4*>(P.psi + pml_coef(P, g.nnx, g.nny, 2))[k];
+                    ezz = pml_upd(cq.x, cq.y, ezz, ps[6]);
+                    e_zx = pml_upd(cq.z, cq.w, e_zx, ps[7]);
+                    e_zy = pml_upd(cq.z, cq.w, e_zy, ps[8]);
+                    bstn<3>(rpsi, 12u * pqz, 4u * psi_stress(P, 2), ps + 6);
                 }
             }
             const float h = hxy + g.hz[k];
@@ -414,6 +462,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
     const rsrc_t rxx = mkrs(w.txx + base, span), ryy = mkrs(w.tyy + base, span), rzz = mkrs(w.tzz + base, span);
     const rsrc_t rxy = mkrs(w.txy + base, span), rxz = mkrs(w.txz + base, span), ryz = mkrs(w.tyz + base, span);
     const rsrc_t rtp = mkrs(w.tp + base, span), rB = mkrs(Bm + base, span);
+    [[maybe_unused]] const rsrc_t rpsi = mkrs(P.psi, 4L * (long)pml_floats(P, g.nnx, g.nny, g.nnz));
 
     float qxz[2 * R + 1], qyz[2 * R + 1], qzz[2 * R + 1], qtp[2 * R + 1];
 #pragma unroll
@@ -462,15 +511,15 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             szi = (AX & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
             if ((AX & 1) && sxi >= 0) {
                 pqx = ((unsigned)k * g.nny + j) * P.w[0] + sxi;
-                ps[0] = P.psi[9][pqx]; ps[1] = P.psi[10][pqx]; ps[2] = P.psi[11][pqx]; ps[3] = P.psi[12][pqx];
+                bldn<4>(rpsi, 16u * pqx, 4u * psi_velocity(P, 0), ps);
             }
             if ((AX & 2) && syi >= 0) {
                 pqy = ((unsigned)k * P.w[1] + syi) * g.nnx + i;
-                ps[4] = P.psi[13][pqy]; ps[5] = P.psi[14][pqy]; ps[6] = P.psi[15][pqy]; ps[7] = P.psi[16][pqy];
+                bldn<4>(rpsi, 16u * pqy, 4u * psi_velocity(P, 1), ps + 4);
             }
             if ((AX & 4) && szi >= 0) {
                 pqz = ((unsigned)szi * g.nny + j) * g.nnx + i;
-                ps[8] = P.psi[17][pqz]; ps[9] = P.psi[18][pqz]; ps[10] = P.psi[19][pqz]; ps[11] = P.psi[20][pqz];
+                bldn<4>(rpsi, 16u * pqz, 4u * psi_velocity(P, 2), ps + 8);
             }
         }
         __syncthreads();
@@ -516,25 +565,30 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             float px = g.cx * p1, py = g.cy * p2, pz = g.cz * p3;
             if constexpr (AX != 0) {
                 if ((AX & 1) && sxi >= 0) {
-                    const float af = P.a[0][0][i], bf = P.b[0][0][i], ah = P.a[0][1][i], bh = P.b[0][1][i];
-                    gxx = pml_upd(P.psi[9], pqx, ah, bh, gxx, ps[0]);
-                    gxy = pml_upd(P.psi[10], pqx, af, bf, gxy, ps[1]);
-                    gxz = pml_upd(P.psi[11], pqx, af, bf, gxz, ps[2]);
-                    px = pml_upd(P.psi[12], pqx, ah, bh, px, ps[3]);
+                    float cq[4];
+                    bldn<4>(rpsi, 16u * i, 4u * pml_coef(P, g.nnx, g.nny, 0), cq);  // (a_full, b_full, a_half, b_half) of x
+                    gxx = pml_upd(cq[2], cq[3], gxx, ps[0]);
+                    gxy = pml_upd(cq[0], cq[1], gxy, ps[1]);
+                    gxz = pml_upd(cq[0], cq[1], gxz, ps[2]);
+                    px = pml_upd(cq[2], cq[3], px, ps[3]);
+                    bstn<4>(rpsi, 16u * pqx, 4u * psi_velocity(P, 0), ps);
                 }
                 if ((AX & 2) && syi >= 0) {
-                    const float af = P.a[1][0][j], bf = P.b[1][0][j], ah = P.a[1][1][j], bh = P.b[1][1][j];
-                    gyx = pml_upd(P.psi[13], pqy, af, bf, gyx, ps[4]);
-                    gyy = pml_upd(P.psi[14], pqy, ah, bh, gyy, ps[5]);
-                    gyz = pml_upd(P.psi[15], pqy, af, bf, gyz, ps[6]);
-                    py = pml_upd(P.psi[16], pqy, ah, bh, py, ps[7]);
+                    float cq[4];
+                    bldn<4>(rpsi, 16u * j, 4u * pml_coef(P, g.nnx, g.nny, 1), cq);
+                    gyx = pml_upd(cq[0], cq[1], gyx, ps[4]);
+                    gyy = pml_upd(cq[2], cq[3], gyy, ps[5]);
+                    gyz = pml_upd(cq[0], cq[1], gyz, ps[6]);
+                    py = pml_upd(cq[2], cq[3], py, ps[7]);
+                    bstn<4>(rpsi, 16u * pqy, 4u * psi_velocity(P, 1), ps + 4);
                 }
                 if ((AX & 4) && szi >= 0) {
-                    const float af = P.a[2][0][k], bf = P.b[2][0][k], ah = P.a[2][1][k], bh = P.b[2][1][k];
-                    gzx = pml_upd(P.psi[17], pqz, af, bf, gzx, ps[8]);
-                    gzy = pml_upd(P.psi[18], pqz, af, bf, gzy, ps[9]);
-                    gzz = pml_upd(P.psi[19], pqz, ah, bh, gzz, ps[10]);
-                    pz = pml_upd(P.psi[20], pqz, ah, bh, pz, ps[11]);
+                    const float4 cq = reinterpret_cast<const float4*>(P.psi + pml_coef(P, g.nnx, g.nny, 2))[k];
+                    gzx = pml_upd(cq.x, cq.y, gzx, ps[8]);
+                    gzy = pml_upd(cq.x, cq.y, gzy, ps[9]);
+                    gzz = pml_upd(cq.z, cq.w, gzz, ps[10]);
+                    pz = pml_upd(cq.z, cq.w, pz, ps[11]);
+                    bstn<4>(rpsi, 16u * pqz, 4u * psi_velocity(P, 2), ps + 8);
                 }
             }
             const float h = hxy + g.hz[k];

@@ -271,14 +271,14 @@ int build_cpml(erm_ctx* c) {
     const int lo[3] = {p.bl, p.bf, p.bu}, hi[3] = {p.br, p.bb, p.bd};
     const double hh[3] = {p.dx, p.dy, p.dz};
     const double shift[3] = {-0.5, -0.5, +0.5};
-    std::vector<float> host;
-    size_t off[3][2][2];
+    // per axis, per index: (a_full, b_full, a_half, b_half); a = 0, b = 1 outside the layers
+    std::vector<float> host(4 * (size_t)(n3[0] + n3[1] + n3[2]));
+    auto at = [&](int a, int f, int ab, int i) -> float& {
+        return host[coef_base(c->nnx, c->nny, a) + 4 * (size_t)i + 2 * f + ab];
+    };
     for (int a = 0; a < 3; ++a)
         for (int f = 0; f < 2; ++f) {
-            off[a][f][0] = host.size();
-            host.insert(host.end(), n3[a], 0.f);
-            off[a][f][1] = host.size();
-            host.insert(host.end(), n3[a], 1.f);
+            for (int i = 0; i < n3[a]; ++i) { at(a, f, 0, i) = 0.f; at(a, f, 1, i) = 1.f; }
             for (int i = 0; i < n3[a]; ++i) {
                 // the layer spans the updated cells only (the outer R never update)
                 double x = i + (f ? shift[a] : 0.0), s = 0, L = 0;
@@ -288,25 +288,20 @@ int build_cpml(erm_ctx* c) {
                 double r = s / L, d0 = -3.0 * vmax * std::log(Rc) / (2.0 * L * hh[a]);
                 double d = d0 * r * r, al = M_PI * p.freq * (1.0 - std::min(r, 1.0));
                 double bb = std::exp(-(d + al) * dts);
-                host[off[a][f][1] + i] = (float)bb;
-                host[off[a][f][0] + i] = (float)(d > 0 ? d * (bb - 1.0) / (d + al) : 0.0);
+                at(a, f, 1, i) = (float)bb;
+                at(a, f, 0, i) = (float)(d > 0 ? d * (bb - 1.0) / (d + al) : 0.0);
             }
         }
-    if (!c->pml_coef && dmalloc(c, &c->pml_coef, host.size())) return ERM_E_NOMEM;
-    HIPC(c, hipMemcpyAsync(c->pml_coef, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    float* dcoef = c->psi_buf + psi_total(c->pml);
+    HIPC(c, hipMemcpyAsync(dcoef, host.data(), host.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
-    for (int a = 0; a < 3; ++a)
-        for (int f = 0; f < 2; ++f) {
-            c->pml.a[a][f] = c->pml_coef + off[a][f][0];
-            c->pml.b[a][f] = c->pml_coef + off[a][f][1];
-        }
     // memory-variable traffic: 8 B (read+write) per psi at every updated cell whose profile is non-zero
     const int R = c->R;
     double cnt[3][2] = {{0, 0}, {0, 0}, {0, 0}};  // updated positions with a != 0 along each axis
     for (int a = 0; a < 3; ++a)
         for (int f = 0; f < 2; ++f)
             for (int i = R; i < n3[a] - R; ++i)
-                if (host[off[a][f][0] + i] != 0.f) cnt[a][f] += 1;
+                if (at(a, f, 0, i) != 0.f) cnt[a][f] += 1;
     const double span[3] = {(double)(c->nnx - 2 * R), (double)(c->nny - 2 * R), (double)(c->nnz - 2 * R)};
     // stress psi per axis: 1 full + 2 half; velocity psi per axis: 2 full + 2 half
     double st = 0, ve = 0;
@@ -384,8 +379,7 @@ void free_all(erm_ctx* c) {
     for (auto& p : c->ex) F(p);
     for (auto& p : c->img) F(p);
     F(c->rec); F(c->rec_idx);
-    F(c->pml_coef);
-    for (auto& p : c->psi) F(p);
+    F(c->psi_buf);
     for (auto& t : c->tiles) F(t);
     for (auto& q : c->fr) F(q);
     F(c->strip_idx); F(c->strips); F(c->wavelet_neg);
@@ -610,11 +604,14 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
         }
         const size_t sz[3] = {(size_t)c->nnz * c->nny * c->slab_w[0], (size_t)c->nnz * c->slab_w[1] * c->nnx,
                               (size_t)c->slab_w[2] * c->nny * c->nnx};
-        static const int axis_of[21] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
-        for (int q = 0; q < 21; ++q) {
-            CK(dmalloc(c, &c->psi[q], sz[axis_of[q]]));
-            c->pml.psi[q] = c->psi[q];
+        for (int a = 0; a < 3; ++a) c->pml.sz[a] = (unsigned)((sz[a] + 3) / 4 * 4);  // 16-B aligned blocks
+        const size_t nstate = pml_floats(c->pml, c->nnx, c->nny, c->nnz);
+        if (nstate * 4 >= (1UL << 32)) {
+            set_global_error("C-PML memory variables exceed 4 GB (32-bit buffer offsets)");
+            free_all(c); delete c; return ERM_E_SIZE;
         }
+        CK(dmalloc(c, &c->psi_buf, nstate));
+        c->pml.psi = c->psi_buf;
         c->vmax_loaded = 0.0;
         CK(build_cpml(c));
     }
@@ -772,12 +769,7 @@ int erm_set_shot(erm_ctx* c, int isx, int isy, int isz) {
 int erm_zero_wavefield(erm_ctx* c) {
     if (!c) return ERM_E_PARAM;
     for (auto& p : c->f) if (fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
-    if (c->cpml) {
-        const size_t sz[3] = {(size_t)c->nnz * c->nny * c->slab_w[0], (size_t)c->nnz * c->slab_w[1] * c->nnx,
-                              (size_t)c->slab_w[2] * c->nny * c->nnx};
-        static const int axis_of[21] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 0, 0, 0, 0, 1, 1, 1, 1, 2, 2, 2, 2};
-        for (int q = 0; q < 21; ++q) if (fill(c, c->psi[q], sz[axis_of[q]], 0.f)) return ERM_E_DEVICE;
-    }
+    if (c->cpml && fill(c, c->psi_buf, psi_total(c->pml), 0.f)) return ERM_E_DEVICE;
     return 0;
 }
 
