@@ -32,6 +32,7 @@ struct erm_ctx {
     float* wavelet = nullptr;
     int wl = 0;
     std::vector<float> wavelet_h;
+    float* exbuf = nullptr;     // excitation state and images (layout: Excite); the views below point into it
     int* ex_time = nullptr;
     float* ex[5] = {nullptr};   // amp tp vxp vyp vzp
     float* img[3] = {nullptr};  // tp pp ps
@@ -119,8 +120,7 @@ inline Wave make_wave(const erm_ctx* c) {
 
 inline Excite make_excite(const erm_ctx* c) {
     Excite e;
-    e.time = c->ex_time;
-    e.amp = c->ex[0]; e.tp = c->ex[1]; e.vxp = c->ex[2]; e.vyp = c->ex[3]; e.vzp = c->ex[4];
-    e.img_tp = c->img[0]; e.img_pp = c->img[1]; e.img_ps = c->img[2];
+    e.base = c->exbuf;
+    e.n = (long)c->ncell;
     return e;
 }

@@ -35,6 +35,7 @@
 // coe_attenuation_3d_new2 (3D_zzzzz:261-369); the kernels take h per axis.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 namespace erm {
 
@@ -86,11 +87,16 @@ struct Wave {  // the 13 propagated fields
     float *vx, *vy, *vz, *txx, *tyy, *tzz, *txy, *txz, *tyz, *tp, *vxp, *vyp, *vzp;
 };
 
-struct Excite {  // excitation state and images
-    int* time;
-    float *amp, *tp, *vxp, *vyp, *vzp;
-    float *img_tp, *img_pp, *img_ps;
+// Excitation state and images: nine arrays of n cells in one allocation, in
+// the order of the EX_ ids (time holds ints).  One base pointer keeps the
+// velocity kernels' rarely taken excitation/imaging branches from holding
+// nine pointers in scalar registers across the z sweep.
+struct Excite {
+    float* base;
+    long n;
 };
+enum { EX_TIME = 0, EX_AMP, EX_TP, EX_VXP, EX_VYP, EX_VZP, EX_IMG_TP, EX_IMG_PP, EX_IMG_PS, EX_COUNT };
+__host__ __device__ __forceinline__ float* ex_array(const Excite& E, int q) { return E.base + q * E.n; }
 
 enum { MODE_PLAIN = 0, MODE_EXCITE = 1, MODE_IMAGE = 2 };
 
@@ -111,7 +117,7 @@ enum { MODE_PLAIN = 0, MODE_EXCITE = 1, MODE_IMAGE = 2 };
 // psi_velocity(P, a) (x: sxx, sxy, sxz, tp; y: sxy, syy, syz, tp; z: sxz,
 // syz, szz, tp).  The coefficients follow the memory variables in the same
 // buffer (at float psi_total).  One load and one store per axis and cell,
-// and one buffer resource for the whole state, keep the C-PML kernels'
+// and one base pointer for the whole state, keep the C-PML kernels'
 // scalar registers from spilling.
 struct Pml {
     float* psi;
@@ -151,39 +157,68 @@ __device__ __forceinline__ float pml_upd(float a, float b, float e, float& psi) 
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// Global accesses go through raw buffer instructions: a wave-uniform buffer
-// resource (SGPRs) whose base is the first plane a workgroup touches, a
-// per-lane 32-bit byte offset (VGPR, loop invariant) and a per-plane byte
-// offset (SGPR).  This keeps per-lane 64-bit addresses out of the VGPR budget
-// and makes out-of-range loads return 0 (range = num_records).
-typedef __amdgpu_buffer_rsrc_t rsrc_t;
-__device__ __forceinline__ rsrc_t mkrs(const void* base, long bytes) {
+// Global accesses: a wave-uniform handle to an array at the first plane a
+// workgroup touches, plus 32-bit byte offsets: per lane (in-plane, loop
+// invariant) and per plane (uniform).  Two handle types with one interface:
+//  - Rsrc, a raw buffer resource (four SGPRs; the per-plane offset goes in
+//    the instruction's SGPR offset, so it costs no VGPRs).  Used by the tiles
+//    without C-PML memory variables, which have scalar registers to spare and
+//    keep their VGPR count, hence occupancy, lower this way.
+//  - Base, a plain pointer (two SGPRs) addressed as base + zext(offset),
+//    which compiles to global_load/store with a scalar base and a 32-bit
+//    vector offset.  Used by the C-PML tiles: they touch 15 arrays, whose
+//    buffer resources alone would take 60 of the 102 SGPRs and spill.
+// Every access is in range by construction (halo indices are clamped, z
+// planes stay within [0, nnz)).
+struct Rsrc {
+    __amdgpu_buffer_rsrc_t r;
+};
+struct Base {
+    char* p;
+};
+template <class A>
+__device__ __forceinline__ A mkacc(const void* base, long bytes);
+template <>
+__device__ __forceinline__ Rsrc mkacc<Rsrc>(const void* base, long bytes) {
     const unsigned n = bytes >= 0xFFFFFFFFL ? 0xFFFFFFFFu : (unsigned)bytes;
     // dword3 = 0x00020000: DATA_FORMAT = 32 bit, as required by offen raw loads on gfx9
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)n, 0x00020000);
+    return Rsrc{__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)n, 0x00020000)};
 }
-__device__ __forceinline__ float bld(rsrc_t r, unsigned vo, unsigned so) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0));
+template <>
+__device__ __forceinline__ Base mkacc<Base>(const void* base, long) {
+    return Base{(char*)const_cast<void*>(base)};
 }
-__device__ __forceinline__ void bst(rsrc_t r, unsigned vo, unsigned so, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)vo, (int)so, 0);
+__device__ __forceinline__ float bld(Rsrc r, unsigned vo, unsigned so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r.r, (int)vo, (int)so, 0));
 }
-__device__ __forceinline__ int bldi(rsrc_t r, unsigned vo, unsigned so) {
-    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)vo, (int)so, 0);
+__device__ __forceinline__ void bst(Rsrc r, unsigned vo, unsigned so, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r.r, (int)vo, (int)so, 0);
 }
-// N consecutive floats per lane.  Written as dword accesses at adjacent
-// offsets, which the compiler merges into one dwordx3 / dwordx4 access (the
-// vector-typed buffer builtins are avoided: this toolchain lowers
-// element-wise use of their results to a single-dword load).
+__device__ __forceinline__ int bldi(Rsrc r, unsigned vo, unsigned so) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r.r, (int)vo, (int)so, 0);
+}
+__device__ __forceinline__ float bld(Base r, unsigned vo, unsigned so) {
+    return *reinterpret_cast<const float*>(r.p + (vo + so));
+}
+__device__ __forceinline__ void bst(Base r, unsigned vo, unsigned so, float v) {
+    *reinterpret_cast<float*>(r.p + (vo + so)) = v;
+}
+__device__ __forceinline__ int bldi(Base r, unsigned vo, unsigned so) {
+    return *reinterpret_cast<const int*>(r.p + (vo + so));
+}
+// N consecutive floats per lane (C-PML state, Base handles only; the
+// compiler merges them into one dwordx3 / dwordx4 access)
 template <int N>
-__device__ __forceinline__ void bldn(rsrc_t r, unsigned vo, unsigned so, float* v) {
+__device__ __forceinline__ void bldn(Base r, unsigned vo, unsigned so, float* v) {
+    const float* q = reinterpret_cast<const float*>(r.p + (vo + so));
 #pragma unroll
-    for (int c = 0; c < N; ++c) v[c] = bld(r, vo + 4u * c, so);
+    for (int c = 0; c < N; ++c) v[c] = q[c];
 }
 template <int N>
-__device__ __forceinline__ void bstn(rsrc_t r, unsigned vo, unsigned so, const float* v) {
+__device__ __forceinline__ void bstn(Base r, unsigned vo, unsigned so, const float* v) {
+    float* q = reinterpret_cast<float*>(r.p + (vo + so));
 #pragma unroll
-    for (int c = 0; c < N; ++c) bst(r, vo + 4u * c, so, v[c]);
+    for (int c = 0; c < N; ++c) q[c] = v[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -243,16 +278,17 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
     for (int m = 1; m <= R; ++m) c[m] = cf.c[m];
     const float hxy = active ? g.hx[ic] + g.hy[jc] : 0.f;
 
-    // buffer resources based at plane kb-R
+    // array bases at plane kb-R
     const long base = (long)(kb - R) * g.pxy;
     const long span = ((long)g.nnz * g.pxy - base) * 4;
     const unsigned pb = 4u * (unsigned)g.pxy;  // plane pitch in bytes
-    const rsrc_t rvx = mkrs(w.vx + base, span), rvy = mkrs(w.vy + base, span), rvz = mkrs(w.vz + base, span);
-    const rsrc_t rxx = mkrs(w.txx + base, span), ryy = mkrs(w.tyy + base, span), rzz = mkrs(w.tzz + base, span);
-    const rsrc_t rxy = mkrs(w.txy + base, span), rxz = mkrs(w.txz + base, span), ryz = mkrs(w.tyz + base, span);
-    const rsrc_t rtp = mkrs(w.tp + base, span), rM = mkrs(Mm + base, span), rU = mkrs(MUm + base, span);
+    using Acc = std::conditional_t<AX == 0, Rsrc, Base>;
+    const Acc rvx = mkacc<Acc>(w.vx + base, span), rvy = mkacc<Acc>(w.vy + base, span), rvz = mkacc<Acc>(w.vz + base, span);
+    const Acc rxx = mkacc<Acc>(w.txx + base, span), ryy = mkacc<Acc>(w.tyy + base, span), rzz = mkacc<Acc>(w.tzz + base, span);
+    const Acc rxy = mkacc<Acc>(w.txy + base, span), rxz = mkacc<Acc>(w.txz + base, span), ryz = mkacc<Acc>(w.tyz + base, span);
+    const Acc rtp = mkacc<Acc>(w.tp + base, span), rM = mkacc<Acc>(Mm + base, span), rU = mkacc<Acc>(MUm + base, span);
     // C-PML state (unused, and removed, when AX == 0)
-    [[maybe_unused]] const rsrc_t rpsi = mkrs(P.psi, 4L * (long)pml_floats(P, g.nnx, g.nny, g.nnz));
+    [[maybe_unused]] const Base rpsi = mkacc<Base>(P.psi, 0);
 
     float qx[2 * R + 1], qy[2 * R + 1], qz[2 * R + 1];
 #pragma unroll
@@ -457,12 +493,13 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
     const long base = (long)(kb - R) * g.pxy;
     const long span = ((long)g.nnz * g.pxy - base) * 4;
     const unsigned pb = 4u * (unsigned)g.pxy;
-    const rsrc_t rvx = mkrs(w.vx + base, span), rvy = mkrs(w.vy + base, span), rvz = mkrs(w.vz + base, span);
-    const rsrc_t rpx = mkrs(w.vxp + base, span), rpy = mkrs(w.vyp + base, span), rpz = mkrs(w.vzp + base, span);
-    const rsrc_t rxx = mkrs(w.txx + base, span), ryy = mkrs(w.tyy + base, span), rzz = mkrs(w.tzz + base, span);
-    const rsrc_t rxy = mkrs(w.txy + base, span), rxz = mkrs(w.txz + base, span), ryz = mkrs(w.tyz + base, span);
-    const rsrc_t rtp = mkrs(w.tp + base, span), rB = mkrs(Bm + base, span);
-    [[maybe_unused]] const rsrc_t rpsi = mkrs(P.psi, 4L * (long)pml_floats(P, g.nnx, g.nny, g.nnz));
+    using Acc = std::conditional_t<AX == 0, Rsrc, Base>;
+    const Acc rvx = mkacc<Acc>(w.vx + base, span), rvy = mkacc<Acc>(w.vy + base, span), rvz = mkacc<Acc>(w.vz + base, span);
+    const Acc rpx = mkacc<Acc>(w.vxp + base, span), rpy = mkacc<Acc>(w.vyp + base, span), rpz = mkacc<Acc>(w.vzp + base, span);
+    const Acc rxx = mkacc<Acc>(w.txx + base, span), ryy = mkacc<Acc>(w.tyy + base, span), rzz = mkacc<Acc>(w.tzz + base, span);
+    const Acc rxy = mkacc<Acc>(w.txy + base, span), rxz = mkacc<Acc>(w.txz + base, span), ryz = mkacc<Acc>(w.tyz + base, span);
+    const Acc rtp = mkacc<Acc>(w.tp + base, span), rB = mkacc<Acc>(Bm + base, span);
+    [[maybe_unused]] const Base rpsi = mkacc<Base>(P.psi, 0);
 
     float qxz[2 * R + 1], qyz[2 * R + 1], qzz[2 * R + 1], qtp[2 * R + 1];
 #pragma unroll
@@ -606,20 +643,21 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             const long n = base + (long)(so / 4u) + (long)(o_c / 4u);
             if constexpr (MODE == MODE_EXCITE) {
                 const float amp = sqrtf(nxp * nxp + nyp * nyp + nzp * nzp);
-                if (bld(mkrs(ex.amp + base, span), o_c, so) < amp) {
-                    ex.time[n] = it; ex.amp[n] = amp; ex.tp[n] = qtp[R];
-                    ex.vxp[n] = nxp; ex.vyp[n] = nyp; ex.vzp[n] = nzp;
+                if (bld(mkacc<Acc>(ex_array(ex, EX_AMP) + base, span), o_c, so) < amp) {
+                    reinterpret_cast<int*>(ex_array(ex, EX_TIME))[n] = it;
+                    ex_array(ex, EX_AMP)[n] = amp; ex_array(ex, EX_TP)[n] = qtp[R];
+                    ex_array(ex, EX_VXP)[n] = nxp; ex_array(ex, EX_VYP)[n] = nyp; ex_array(ex, EX_VZP)[n] = nzp;
                 }
             } else if constexpr (MODE == MODE_IMAGE) {
-                if (bldi(mkrs(ex.time + base, span), o_c, so) == it) {
-                    const float etp = ex.tp[n];
-                    ex.img_tp[n] = etp != 0.f ? qtp[R] / etp : 0.f;
-                    const float ex_ = ex.vxp[n], ey_ = ex.vyp[n], ez_ = ex.vzp[n];
+                if (bldi(mkacc<Acc>(ex_array(ex, EX_TIME) + base, span), o_c, so) == it) {
+                    const float etp = ex_array(ex, EX_TP)[n];
+                    ex_array(ex, EX_IMG_TP)[n] = etp != 0.f ? qtp[R] / etp : 0.f;
+                    const float ex_ = ex_array(ex, EX_VXP)[n], ey_ = ex_array(ex, EX_VYP)[n], ez_ = ex_array(ex, EX_VZP)[n];
                     const float den = ex_ * ex_ + ey_ * ey_ + ez_ * ez_;
                     const float pp = ex_ * nxp + ey_ * nyp + ez_ * nzp;
                     const float ps = ex_ * (nvx - nxp) + ey_ * (nvy - nyp) + ez_ * (nvz - nzp);
-                    ex.img_pp[n] = den > 0.f ? pp / den : 0.f;
-                    ex.img_ps[n] = den > 0.f ? ps / den : 0.f;
+                    ex_array(ex, EX_IMG_PP)[n] = den > 0.f ? pp / den : 0.f;
+                    ex_array(ex, EX_IMG_PS)[n] = den > 0.f ? ps / den : 0.f;
                 }
             }
         }

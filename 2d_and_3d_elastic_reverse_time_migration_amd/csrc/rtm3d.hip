@@ -375,9 +375,7 @@ void free_all(erm_ctx* c) {
     for (auto& p : c->f) F(p);
     for (int w = 0; w < 2; ++w) { F(c->M[w]); F(c->MU[w]); F(c->BU[w]); }
     for (auto& p : c->hxyz) F(p);
-    F(c->wavelet); F(c->ex_time);
-    for (auto& p : c->ex) F(p);
-    for (auto& p : c->img) F(p);
+    F(c->wavelet); F(c->exbuf);
     F(c->rec); F(c->rec_idx);
     F(c->psi_buf);
     for (auto& t : c->tiles) F(t);
@@ -583,9 +581,10 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     for (auto& e : c->pass_ev)
         if (hipEventCreate(&e) != hipSuccess) { set_global_error("event create failed"); free_all(c); delete c; return ERM_E_DEVICE; }
     for (auto& p : c->f) CK(dmalloc(c, &p, c->ncell));
-    for (auto& p : c->ex) CK(dmalloc(c, &p, c->ncell));
-    for (auto& p : c->img) CK(dmalloc(c, &p, c->ncell));
-    CK(dmalloc(c, &c->ex_time, c->ncell));
+    CK(dmalloc(c, &c->exbuf, (size_t)EX_COUNT * c->ncell));
+    c->ex_time = reinterpret_cast<int*>(c->exbuf + (size_t)EX_TIME * c->ncell);
+    for (int q = 0; q < 5; ++q) c->ex[q] = c->exbuf + (size_t)(EX_AMP + q) * c->ncell;
+    for (int q = 0; q < 3; ++q) c->img[q] = c->exbuf + (size_t)(EX_IMG_TP + q) * c->ncell;
     if (pin->boundary == ERM_BOUNDARY_CPML) {
         c->cpml = true;
         const int n3[3] = {c->nnx, c->nny, c->nnz};
