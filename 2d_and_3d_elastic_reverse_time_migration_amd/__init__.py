@@ -128,6 +128,9 @@ def _load():
         "erm_unet_last_stats": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
         "erm_unet_bind_model": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
         "erm_unet_precond": (ctypes.c_int, [fp, fp, vp]),
+        "erm_unet_bind_weights": (ctypes.c_int, [vp, fp]),
+        "erm2d_illumination": (ctypes.c_int, [vp, ctypes.c_int, ip, fp]),
+        "erm2d_precond_diag": (ctypes.c_int, [fp, fp, vp]),
         "erm2d_set_workers": (ctypes.c_int, [vp, ctypes.c_int]),
         "erm_batch_last_error": (ctypes.c_char_p, [vp]),
         "erm_batch_create": (ctypes.c_int, [vp, ctypes.c_int, P(vp)]),
@@ -384,6 +387,19 @@ def _check2(rc, ctx=None):
         raise ErmError(rc, lib.erm2d_last_error(ctx).decode(errors="replace"))
 
 
+class _DiagPrecondS(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("nz", ctypes.c_int), ("w", ctypes.POINTER(ctypes.c_float))]
+
+
+class DiagPrecond:
+    """Diagonal LSRTM preconditioner z = w * g (erm2d_precond_diag), w [nx][nz]."""
+
+    def __init__(self, w):
+        self.w = np.ascontiguousarray(w, np.float32)
+        self.shape = self.w.shape
+        self._s = _DiagPrecondS(self.shape[0], self.shape[1], self.w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+
+
 class Elastic2D:
     """2D elastic / viscoelastic context (include/erm2d.h): modeling, RTM,
     and the LSRTM Born / adjoint operators."""
@@ -469,12 +485,23 @@ class Elastic2D:
         """Run the shots of lsrtm_cg on n concurrent contexts/streams."""
         self._c(lib.erm2d_set_workers(self._h, n))
 
-    def lsrtm_cg(self, shots, dobs, iters, pmask=3, m0=None, tol=0.0, precond=None, monitor=None):
+    def illumination(self, shots):
+        """Source illumination of the background model summed over shots
+        [(x, z), ...] (erm2d_illumination): [nx][nz] float32."""
+        p = self.p
+        shots_a = np.ascontiguousarray(np.asarray(shots, np.int32).reshape(-1, 2))
+        out = np.empty((p.nx, p.nz), np.float32)
+        self._c(lib.erm2d_illumination(self._h, len(shots_a), shots_a.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       _fptr(out)))
+        return out
+
+    def lsrtm_cg(self, shots, dobs, iters, pmask=3, m0=None, tol=0.0, precond=None, monitor=None, weights=None):
         """CGLS over shots [(x, z), ...] with observed data [nshots][2][lt][nrec].
-        precond: a UNet (applied through erm_unet_precond) or a python callable
-        g -> z on [3][nx][nz] float32 arrays.  monitor: optional callable
-        (it, m, g, misfit) called with copies of the model and gradient.
-        Returns (m, misfit history, seconds history, iterations)."""
+        precond: a UNet (applied through erm_unet_precond, to weights * g when
+        weights [nx][nz] are given), a DiagPrecond (erm2d_precond_diag) or a
+        python callable g -> z on [3][nx][nz] float32 arrays.  monitor:
+        optional callable (it, m, g, misfit) called with copies of the model
+        and gradient.  Returns (m, misfit history, seconds history, iterations)."""
         p = self.p
         shots_a = np.ascontiguousarray(np.asarray(shots, np.int32).reshape(-1, 2))
         d = np.ascontiguousarray(dobs, np.float32)
@@ -490,7 +517,15 @@ class Elastic2D:
         cb, user = None, None
         if isinstance(precond, UNet):
             _checku(lib.erm_unet_bind_model(precond._h, p.nx, p.nz))
+            if weights is not None:
+                w = np.ascontiguousarray(weights, np.float32)
+                assert w.shape == (p.nx, p.nz)
+                _checku(lib.erm_unet_bind_weights(precond._h, _fptr(w)))
             cb, user = ctypes.cast(lib.erm_unet_precond, ctypes.c_void_p), precond._h
+        elif isinstance(precond, DiagPrecond):
+            if precond.shape != (p.nx, p.nz):
+                raise ValueError("DiagPrecond shape does not match the model")
+            cb, user = ctypes.cast(lib.erm2d_precond_diag, ctypes.c_void_p), ctypes.addressof(precond._s)
         elif precond is not None:
             def _cb(gp, zp, _user):
                 view(zp)[...] = precond(view(gp).copy())

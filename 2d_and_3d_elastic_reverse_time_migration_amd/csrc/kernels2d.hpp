@@ -419,6 +419,15 @@ __global__ void k2_inject(float* __restrict__ vx, float* __restrict__ vz, const 
     atomicAdd(&vx[idx[r]], rec[off + r]);
     atomicAdd(&vz[idx[r]], rec[comp + off + r]);
 }
+// source illumination: W += vx^2 + vz^2 at every grid point (each component
+// at its own staggered position)
+__global__ void k2_illum_acc(const float* __restrict__ vx, const float* __restrict__ vz, float* __restrict__ W,
+                             long n) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const float a = vx[t], b = vz[t];
+        W[t] += a * a + b * b;
+    }
+}
 __global__ void k2_fill(float* __restrict__ p, long n, float v) {
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) p[t] = v;
 }

@@ -94,6 +94,7 @@ struct erm2d_ctx {
     float* hist = nullptr;  // [lt][5][mn]: E (3) then F (2)
     float* dm = nullptr;    // [3][mn]
     float* grad = nullptr;  // [3][mn]
+    float* illum = nullptr; // source illumination accumulator (padded grid)
     hipEvent_t ev[2] = {nullptr, nullptr};
     double last_ms = 0;
     float qsign = 1.f;                      // -1 during a Q-compensated receiver pass
@@ -278,7 +279,7 @@ void free2(erm2d_ctx* c) {
     F(c->att); F(c->rec_idx); F(c->rec); F(c->rec2); F(c->ex_time);
     for (auto& e : c->ex) F(e);
     for (auto& e : c->img) F(e);
-    F(c->hist); F(c->dm); F(c->grad);
+    F(c->hist); F(c->dm); F(c->grad); F(c->illum);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -695,6 +696,39 @@ int erm2d_adjoint(erm2d_ctx* c, const float* r, float* gout) {
         HC(c, hipMemcpyAsync(gout, c->grad, 3 * c->mn * 4, hipMemcpyDeviceToHost, c->stream));
         HC(c, hipStreamSynchronize(c->stream));
     }
+    return 0;
+}
+
+int erm2d_illumination(erm2d_ctx* c, int nshots, const int* shots, float* out) {
+    if (!c) return ERM_E_PARAM;
+    if (nshots <= 0 || !shots || !out) return c->fail(ERM_E_PARAM, "bad arguments");
+    HC(c, hipSetDevice(c->device));
+    if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
+    if (!c->illum && dmal(c, &c->illum, c->ncell)) return ERM_E_NOMEM;
+    int rc = fillf(c, c->illum, c->ncell, 0.f);
+    const int sx0 = c->p.shot_x, sz0 = c->p.shot_z;
+    const long n = (long)c->ncell;
+    for (int s = 0; s < nshots && !rc; ++s) {
+        if ((rc = erm2d_set_shot(c, shots[2 * s], shots[2 * s + 1])) || (rc = zero_state(c, false))) break;
+        for (int it = 0; it < c->p.lt; ++it) {
+            do_stress(c, false, 1, c->src, wavelet_at(c, it), nullptr);
+            do_velocity(c, false, 1, ERM2D_PASS_MODEL, it, nullptr);
+            hipLaunchKernelGGL(k2_illum_acc, dim3((unsigned)std::min<long>((n + 255) / 256, 2048)), dim3(256), 0,
+                               c->stream, c->fA[0], c->fA[1], c->illum, n);
+        }
+        HC(c, hipGetLastError());
+    }
+    if (!rc) rc = cut_model(c, c->illum, out);
+    const int rc2 = erm2d_set_shot(c, sx0, sz0);
+    return rc ? rc : rc2;
+}
+
+int erm2d_precond_diag(const float* g, float* z, void* user) {
+    const erm2d_diag_precond* d = (const erm2d_diag_precond*)user;
+    if (!d || !d->w || d->nx <= 0 || d->nz <= 0 || !g || !z) return ERM_E_PARAM;
+    const size_t mn = (size_t)d->nx * d->nz;
+    for (int ch = 0; ch < 3; ++ch)
+        for (size_t i = 0; i < mn; ++i) z[ch * mn + i] = d->w[i] * g[ch * mn + i];
     return 0;
 }
 

@@ -364,6 +364,7 @@ struct erm_unet {
     std::vector<size_t> act_size;
     int img_nx = 0, img_nz = 0;            // model image shape bound for erm_unet_precond
     std::vector<float> img_x, img_y;       // padded [1][2][H][W] staging
+    std::vector<float> img_w;              // optional input weights [nx][nz] (erm_unet_bind_weights)
 
     ~erm_unet() {
         auto F = [](void* q) { if (q) (void)hipFree(q); };
@@ -729,6 +730,15 @@ int erm_unet_bind_model(erm_unet* u, int nx, int nz) {
     const int H = (nx + 7) / 8 * 8, W = (nz + 7) / 8 * 8;
     u->img_x.assign((size_t)2 * H * W, 0.f);
     u->img_y.assign((size_t)2 * H * W, 0.f);
+    u->img_w.clear();
+    return 0;
+}
+
+int erm_unet_bind_weights(erm_unet* u, const float* w) {
+    if (!u) { g_uerr = "bad arguments"; return ERM_E_PARAM; }
+    if (!w) { u->img_w.clear(); return 0; }
+    if (u->img_nx <= 0) { g_uerr = "erm_unet_bind_weights: bind a model shape first"; return ERM_E_STATE; }
+    u->img_w.assign(w, w + (size_t)u->img_nx * u->img_nz);
     return 0;
 }
 
@@ -737,16 +747,19 @@ int erm_unet_precond(const float* g, float* z, void* user) {
     if (!u || !g || !z || u->img_nx <= 0) { g_uerr = "erm_unet_precond: net not bound to a model shape"; return ERM_E_STATE; }
     const int nx = u->img_nx, nz = u->img_nz, H = (nx + 7) / 8 * 8, W = (nz + 7) / 8 * 8;
     const size_t mn = (size_t)nx * nz;
+    // the net's input: w * g (w = 1 unless weights are bound)
+    const bool wt = !u->img_w.empty();
+    for (int ch = 0; ch < 3; ++ch)
+        for (size_t i = 0; i < mn; ++i) z[ch * mn + i] = wt ? u->img_w[i] * g[ch * mn + i] : g[ch * mn + i];
     double ss = 0;
-    for (size_t i = 0; i < 2 * mn; ++i) ss += (double)g[i] * g[i];
+    for (size_t i = 0; i < 2 * mn; ++i) ss += (double)z[i] * z[i];
     const double rms = std::sqrt(ss / (2.0 * mn));
-    std::copy(g, g + 3 * mn, z);
     if (rms == 0) return 0;
     std::fill(u->img_x.begin(), u->img_x.end(), 0.f);
     for (int ch = 0; ch < 2; ++ch)
         for (int i = 0; i < nx; ++i)
             for (int k = 0; k < nz; ++k)
-                u->img_x[((size_t)ch * H + i) * W + k] = (float)(g[ch * mn + (size_t)i * nz + k] / rms);
+                u->img_x[((size_t)ch * H + i) * W + k] = (float)(z[ch * mn + (size_t)i * nz + k] / rms);
     int rc = erm_unet_forward(u, u->img_x.data(), 1, H, W, u->img_y.data());
     if (rc) return rc;
     for (int ch = 0; ch < 2; ++ch)

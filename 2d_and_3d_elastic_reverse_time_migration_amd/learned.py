@@ -1,26 +1,32 @@
 """Learned image-domain preconditioner for 2D elastic LSRTM.
 
 The workflow has no counterpart in the reference (which ships the LSRTM
-operators only); every number it produces comes from the library:
+operators only); every number it produces comes from the library.  The
+approximate inverse Hessian is z = net(w * g): w is the diagonal from the
+source illumination of the background model (erm2d_illumination,
+w = 1 / (W + eps max W), mean 1) and the U-Net learns the rest.
 
 1. random layered-plus-anomaly models (`random_model`), a smooth background
    and the relative perturbations (dvp/vp, dvs/vs) the Born operator is
    linearised in (erm2d.h, inversion_para 2 of the reference);
-2. Born data of each model and plain CG-LSRTM on it (`erm2d_lsrtm_cg`); a CG
-   monitor records, at selected iterations k, the gradient g_k = L^T(d - L m_k)
-   and the model m_k, and the training pair is (g_k, dm - m_k): the update
-   that would take the iterate to the model the noise-free data were made
-   from, which converged LSRTM approaches (d = L dm, so a perfect net makes
-   one step of the flexible CG exact).  Plain CG on the training models is
-   run for a fixed, unconverged number of iterations only to supply the
-   gradients g_k that the preconditioner meets along the way;
-3. 64x64 patches, normalised by the RMS of the gradient image (targets scaled
+2. Born data of each model and CG-LSRTM on it (`erm2d_lsrtm_cg`, with the
+   diagonal w as preconditioner; or plain when no eps is given); a CG
+   monitor records, at selected iterations k, the gradient
+   g_k = L^T(d - L m_k) and the model m_k, and the training pair is
+   (w g_k, dm - m_k): the update that would take the iterate to the model
+   the noise-free data were made from (d = L dm, so a perfect net makes one
+   step of the flexible CG exact).  CG on the training models runs for a
+   fixed, unconverged number of iterations only to supply the gradients the
+   preconditioner meets along the way;
+3. 64x64 patches, normalised by the RMS of the input image (targets scaled
    by one global constant so both sides are O(1)), train the U-Net on the GPU
    (erm_unet_train: BF16-MFMA forward/backward, Adam);
-4. on a held-out model, flexible CG preconditioned by the net
-   (erm_unet_precond) against plain CG: iterations and seconds to a data
-   misfit |L m - d| / |d| <= tol, and the model error of both against the
-   plain converged image.
+4. on a held-out model: plain CG, CG with the diagonal alone, flexible CG
+   with the learned preconditioner at every iteration, and learned warm
+   starts (k learned iterations, then CG with the diagonal): iterations and
+   seconds to a data misfit |L m - d| / |d| <= tol (the illumination's cost
+   included), and model errors against the plain converged image and the
+   true perturbation.
 
 The test suite runs a reduced version of the same steps
 (tests/test_gpu_learned.py).
@@ -30,7 +36,7 @@ import time
 
 import numpy as np
 
-from . import Elastic2D, UNet
+from . import DiagPrecond, Elastic2D, UNet
 from .params2d import Params2D
 
 
@@ -122,9 +128,21 @@ def patches(img, size=64, stride=32):
     return np.stack([img[:, i:i + size, k:k + size] for i in xs for k in zs])
 
 
-def training_pairs(g, shots, dm, iters, keep, pmask=3):
-    """Plain CG on Born data of dm; returns the list of (g_k, dm - m_k) for
-    k in keep (2-channel images) and the CG misfit history."""
+def illum_weights(g, shots, eps=1e-3):
+    """Diagonal preconditioner weights from the source illumination W of the
+    loaded background model (erm2d_illumination): w = 1 / (W + eps max W),
+    normalised to mean 1; returns (w, seconds)."""
+    t0 = time.perf_counter()
+    W = g.illumination(shots).astype(np.float64)
+    w = 1.0 / (W + eps * W.max())
+    return (w / w.mean()).astype(np.float32), time.perf_counter() - t0
+
+
+def training_pairs(g, shots, dm, iters, keep, pmask=3, w=None):
+    """CG on Born data of dm (plain, or preconditioned by the diagonal w);
+    returns the list of (x_k, dm - m_k) for k in keep, x_k = g_k (plain) or
+    w * g_k (the net's input in the weighted preconditioner), 2-channel
+    images, and the misfit history."""
     d = born_data(g, shots, dm)
     rec = {}
 
@@ -132,8 +150,10 @@ def training_pairs(g, shots, dm, iters, keep, pmask=3):
         if it in keep:
             rec[it] = (grad[:2].copy(), m[:2].copy())
 
-    m, mis, sec, it = g.lsrtm_cg(shots, d, iters, pmask=pmask, monitor=mon)
-    pairs = [(rec[k][0], dm[:2] - rec[k][1]) for k in sorted(rec)]
+    pre = None if w is None else DiagPrecond(w)
+    m, mis, sec, it = g.lsrtm_cg(shots, d, iters, pmask=pmask, monitor=mon, precond=pre)
+    scale = 1.0 if w is None else w[None]
+    pairs = [((scale * rec[k][0]).astype(np.float32), dm[:2] - rec[k][1]) for k in sorted(rec)]
     return pairs, mis
 
 
@@ -170,8 +190,10 @@ def time_to(mis, sec, tol):
 
 
 def generate(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7,
-             keep=(0, 1, 2, 4, 6), workers=4, seed0=1000, log=print):
-    """Training pairs from plain CG on random models (seeds seed0, seed0+1, ...)."""
+             keep=(0, 1, 2, 4, 6), workers=4, seed0=1000, illum_eps=1e-3, log=print):
+    """Training pairs from CG on random models (seeds seed0, seed0+1, ...):
+    plain CG, or with illum_eps set, CG preconditioned by the illumination
+    diagonal (pairs (w g_k, dm - m_k))."""
     nx, nz = train_shape
     g = Elastic2D(survey_params(nx, nz, train_lt))
     keep = set(keep)
@@ -182,9 +204,11 @@ def generate(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=11
         g.load_model(1, bvp, bvs, brho)
         if i == 0:
             g.set_workers(workers)
-        pr, mis = training_pairs(g, shots, dm, cg_iters, keep)
+        w = None if illum_eps is None else illum_weights(g, shots, illum_eps)[0]
+        pr, mis = training_pairs(g, shots, dm, cg_iters, keep, w=w)
         pairs += pr
-        log(f"train model {i}: plain CG misfit {mis[0]:.3f} -> {mis[-1]:.4f} ({len(pr)} pairs)")
+        log(f"train model {i}: {'plain' if w is None else 'illumination-preconditioned'} CG misfit"
+            f" {mis[0]:.3f} -> {mis[-1]:.4f} ({len(pr)} pairs)")
     g.close()
     return pairs
 
@@ -208,8 +232,15 @@ def train(pairs, channels=(16, 32, 64, 64), epochs=60, batch=32, lr=1e-3, seed=0
 
 
 def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, workers=4,
-             test_seed=5000, log=print, every=25, warm=5, flexible=True):
-    """Plain CG against CG preconditioned by `net` on a held-out model."""
+             test_seed=5000, log=print, every=25, warm=5, flexible=True, illum_eps=1e-3):
+    """Plain CG against CG preconditioned by `net` on a held-out model.
+
+    With illum_eps set, the approximate inverse Hessian is z = net(w g) with
+    the illumination diagonal w (the net was trained on w g); the diagonal
+    alone (z = w g) is run as well, so the net's own contribution is
+    visible, and the illumination's cost is included in the times of both.
+    Warm starts run `warm` learned iterations and then continue with the
+    net-free preconditioner (plain CG, or the diagonal)."""
     nx, nz = test_shape
     g = Elastic2D(survey_params(nx, nz, test_lt))
     bvp, bvs, brho, dm = random_model(nx, nz, test_seed)
@@ -225,15 +256,31 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
         return mon
 
     res = {}
+    nrm = lambda a: float(np.linalg.norm(a[:2] - dm[:2]) / np.linalg.norm(dm[:2]))
     m_plain, mis_p, sec_p, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, monitor=progress("plain"))
     ip, tp = time_to(mis_p, sec_p, tol)
     res["plain"] = dict(misfit=[float(v) for v in mis_p], seconds=[float(v) for v in sec_p], iters_to_tol=ip,
                         s_to_tol=tp)
-    nrm = lambda a: float(np.linalg.norm(a[:2] - dm[:2]) / np.linalg.norm(dm[:2]))
     res["model_err_vs_true"] = {"plain": nrm(m_plain)}
+    w, t_w, base = None, 0.0, None
+    if illum_eps is not None:
+        w, t_w = illum_weights(g, shots, illum_eps)
+        base = DiagPrecond(w)
+        res["illumination_s"] = t_w
+        m_d, mis_d, sec_d, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, precond=base,
+                                          monitor=progress("illumination"))
+        sec_d = sec_d + t_w
+        i_d, t_d = time_to(mis_d, sec_d, tol)
+        res["illumination"] = dict(eps=illum_eps, misfit=[float(v) for v in mis_d], seconds=[float(v) for v in sec_d],
+                                   iters_to_tol=i_d, s_to_tol=t_d)
+        res["model_err_vs_true"]["illumination"] = nrm(m_d)
+        if tp and t_d:
+            res["time_ratio_illumination"] = t_d / tp
+        log(f"held-out: plain {ip} it / {tp} s, illumination diagonal {i_d} it / {t_d} s to misfit {tol}")
     if flexible:
-        m_learn, mis_l, sec_l, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, precond=net,
+        m_learn, mis_l, sec_l, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, precond=net, weights=w,
                                               monitor=progress("learned"))
+        sec_l = sec_l + t_w
         il, tl = time_to(mis_l, sec_l, tol)
         res["learned"] = dict(misfit=[float(v) for v in mis_l], seconds=[float(v) for v in sec_l], iters_to_tol=il,
                               s_to_tol=tl, model_err_vs_plain=float(np.linalg.norm(m_learn[:2] - m_plain[:2])
@@ -243,17 +290,17 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
             res["time_ratio"] = tl / tp
         log(f"held-out: plain {ip} it / {tp} s, learned {il} it / {tl} s to misfit {tol}")
     warms = [warm] if isinstance(warm, int) else list(warm or [])
-    for wk in [w for w in warms if w]:
-        # learned warm start: `wk` preconditioned iterations, then plain CG from there
-        m_w, mis_w1, sec_w1, it_w = g.lsrtm_cg(shots, d, wk, pmask=3, tol=tol, precond=net)
-        m_w, mis_w2, sec_w2, _ = g.lsrtm_cg(shots, d, test_iters - it_w, pmask=3, m0=m_w, tol=tol,
+    for wk in [v for v in warms if v]:
+        m_w, mis_w1, sec_w1, it_w = g.lsrtm_cg(shots, d, wk, pmask=3, tol=tol, precond=net, weights=w)
+        m_w, mis_w2, sec_w2, _ = g.lsrtm_cg(shots, d, test_iters - it_w, pmask=3, m0=m_w, tol=tol, precond=base,
                                             monitor=progress(f"warm-start {wk}"))
         mis_w = np.concatenate([mis_w1, mis_w2[1:]])
-        sec_w = np.concatenate([sec_w1, sec_w1[-1] + sec_w2[1:]])
+        sec_w = np.concatenate([sec_w1, sec_w1[-1] + sec_w2[1:]]) + t_w
         iw, tw = time_to(mis_w, sec_w, tol)
         key = "warm_start" if len(warms) == 1 else f"warm_start_{wk}"
         res[key] = dict(warm_iters=wk, misfit=[float(v) for v in mis_w], seconds=[float(v) for v in sec_w],
-                        iters_to_tol=iw, s_to_tol=tw)
+                        iters_to_tol=iw, s_to_tol=tw, model_err_vs_plain=float(np.linalg.norm(m_w[:2] - m_plain[:2])
+                                                                               / np.linalg.norm(m_plain[:2])))
         res["model_err_vs_true"][key] = nrm(m_w)
         if tp and tw:
             res["time_ratio_" + key] = tw / tp
@@ -265,18 +312,21 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
 
 def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7, test_shape=(256, 96),
         test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, epochs=20, batch=32, lr=1e-3,
-        channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, warm=20, flexible=True, log=print):
+        channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, warm=5, flexible=True, illum_eps=1e-3,
+        log=print):
     """The whole workflow; returns a dict of results."""
     res = {"config": dict(train_models=train_models, train_shape=train_shape, train_shots=train_shots,
                           train_lt=train_lt, cg_iters=cg_iters, test_shape=test_shape, test_shots=test_shots,
-                          test_lt=test_lt, tol=tol, epochs=epochs, batch=batch, lr=lr, channels=channels)}
+                          test_lt=test_lt, tol=tol, epochs=epochs, batch=batch, lr=lr, channels=channels,
+                          illum_eps=illum_eps)}
     t0 = time.perf_counter()
-    pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, workers=workers, seed0=seed0, log=log)
+    pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, workers=workers, seed0=seed0,
+                     illum_eps=illum_eps, log=log)
     res["datagen_s"] = time.perf_counter() - t0
     net, st = train(pairs, channels, epochs, batch, lr, seed=seed0, log=log)
     res.update(st)
     res.update(evaluate(net, test_shape, test_shots, test_lt, test_iters, tol, workers, test_seed, log=log,
-                        warm=warm, flexible=flexible))
+                        warm=warm, flexible=flexible, illum_eps=illum_eps))
     net.close()
     return res
 
@@ -289,12 +339,16 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--test-iters", type=int, default=400)
     ap.add_argument("--workers", type=int, default=4)
-    ap.add_argument("--warm", type=int, nargs="*", default=[20],
-                    help="learned warm-start lengths to compare (net iterations before plain CG)")
+    ap.add_argument("--warm", type=int, nargs="*", default=[5],
+                    help="learned warm-start lengths to compare (net iterations before the net-free CG)")
     ap.add_argument("--no-flexible", action="store_true", help="skip the every-iteration learned CG run")
+    ap.add_argument("--illum-eps", type=float, default=1e-3,
+                    help="illumination diagonal w = 1/(W + eps max W) composed with the net (train and test)")
+    ap.add_argument("--no-illum", action="store_true", help="no diagonal: the net alone on plain-CG gradients")
     a = ap.parse_args(argv)
     res = run(train_models=a.train_models, epochs=a.epochs, test_iters=a.test_iters, workers=a.workers,
-              warm=a.warm, flexible=not a.no_flexible, log=lambda s: print(s, flush=True))
+              warm=a.warm, flexible=not a.no_flexible, illum_eps=None if a.no_illum else a.illum_eps,
+              log=lambda s: print(s, flush=True))
     txt = json.dumps(res)
     if a.out:
         with open(a.out, "w") as f:

@@ -110,6 +110,23 @@ typedef int (*erm2d_precond_fn)(const float* g, float* z, void* user);
 int   erm2d_lsrtm_cg(erm2d_ctx* ctx, int nshots, const int* shots, const float* dobs, int iters, unsigned pmask,
                      float* m, float* misfit, double* seconds, erm2d_precond_fn precond, void* user, float tol);
 
+/* Source illumination of the background model: for each shot, the sum over
+ * the time steps it = 0 .. lt-1 of vx^2 + vz^2 of the source wavefield
+ * (after the step's velocity update, each component at its own staggered
+ * point), summed over the shots; out [nx][nz] over the model region (z
+ * fastest).  The reference only stacks per-shot illumination images
+ * (cuda_sum_new_acqusition_illum, viscoelastic_equation.cu:2736-2754); here it
+ * is the diagonal, physics-side part of the LSRTM preconditioner. */
+int   erm2d_illumination(erm2d_ctx* ctx, int nshots, const int* shots, float* out);
+/* Diagonal preconditioner for erm2d_lsrtm_cg (precond = erm2d_precond_diag,
+ * user = erm2d_diag_precond*): z = w * g on each of the three parameter
+ * planes, w [nx][nz] (e.g. 1 / (illumination + eps max illumination)). */
+typedef struct erm2d_diag_precond {
+    int nx, nz;
+    const float* w;
+} erm2d_diag_precond;
+int   erm2d_precond_diag(const float* g, float* z, void* user);
+
 /* Shot concurrency for erm2d_lsrtm_cg: n contexts in total (this one plus
  * n-1 workers created here with the same parameters and background model),
  * each with its own HIP stream and host thread, take the shots round-robin so

@@ -45,10 +45,17 @@ int   erm_unet_train(erm_unet* u, const float* x, const float* t, int count, int
  * net): bind the model image shape [3][nx][nz] (z fastest) once, then
  * z = s * net(g / s) on the (dvp/vp, dvs/vs) planes, with s the RMS of g over
  * those planes and the image zero-padded to multiples of 8 (h = x, w = z);
- * the drho/rho plane passes through.  Degree-1 homogeneous, so CG's line
+ * the drho/rho plane passes through (times w when weights are bound, see
+ * erm_unet_bind_weights).  Degree-1 homogeneous, so CG's line
  * search sees the same directions for any scaling of g. */
 int   erm_unet_bind_model(erm_unet* u, int nx, int nz);
 int   erm_unet_precond(const float* g, float* z, void* user);
+/* Optional per-cell input weights w [nx][nz] for erm_unet_precond (after
+ * erm_unet_bind_model, which clears them; w = NULL clears): the net is
+ * applied to w * g instead of g, so a diagonal preconditioner (e.g.
+ * erm2d_illumination-based) and the learned correction compose into one
+ * approximate inverse Hessian z = net(w * g).  Still degree-1 homogeneous. */
+int   erm_unet_bind_weights(erm_unet* u, const float* w);
 /* Milliseconds of device time of the last forward / train call and the
  * number of MFMA floating-point operations it issued (2*M*N*K per GEMM,
  * padded shapes), for utilisation reporting. */

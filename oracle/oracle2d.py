@@ -122,6 +122,27 @@ class Oracle2D:
         self._c(lib.oracle2d_adjoint(self._h, _ptr(r, ctypes.c_float), _ptr(g, ctypes.c_float)))
         return g
 
+    def illumination(self, shots):
+        """Source illumination of the background model (the quantity
+        erm2d_illumination returns): for each shot, zero state, then after each
+        source-pass step it = 0 .. lt-1 add vx^2 + vz^2 (each component at its
+        own staggered point); summed over shots; model region [nx][nz], FP64.
+        The reference stacks per-shot illumination images the same way
+        (2D_elastic_RTM/viscoelastic_equation.cu:2736-2754,
+        cuda_sum_new_acqusition_illum).  Pinned by additivity over shots,
+        independence from the true model and positivity
+        (tests/test_oracle2d.py)."""
+        p = self.p
+        W = np.zeros(self.shape)
+        for sx, sz in shots:
+            self.set_shot(sx, sz)
+            self.zero(1)
+            for it in range(p.lt):
+                self.step(1, it, 1)
+                vx, vz = self.get_field("vx"), self.get_field("vz")
+                W += vx * vx + vz * vz
+        return np.ascontiguousarray(W[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz])
+
     def images(self):
         p = self.p
         out = []

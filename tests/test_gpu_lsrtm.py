@@ -11,6 +11,7 @@ CGLS, and the dense adjoint agrees with the operator's adjoint.
 import numpy as np
 import pytest
 
+import cases
 import cases2d
 
 pytestmark = pytest.mark.gpu
@@ -121,3 +122,60 @@ def test_workers_monitor_and_unet_precond(erm):
             mu, misu, _, itu = g.lsrtm_cg(shots, d, 4, pmask=3, precond=net)
         assert itu == 4 and np.all(np.diff(misu) <= 1e-6) and misu[-1] < 0.9
         g.set_workers(1)
+
+
+def test_illumination_diag_and_weighted_unet_preconditioners(erm, oracle2d_mod):
+    """erm2d_illumination against the FP64 oracle (rel L2 <= 1e-4: FP32 sums
+    over lt steps) and additive over shots; the C diagonal preconditioner
+    gives the same CG iterates as the same scaling through a python
+    callable; a U-Net with bound input weights w equals the unweighted net
+    applied to w * g."""
+    import test_oracle2d as T
+    p, true, bg = T._illum_case()
+    shots = [(10, 2), (30, 3)]
+    with erm.Elastic2D(p) as g:
+        g.load_model(1, *bg)
+        W = g.illumination(shots)
+        W1, W2 = g.illumination(shots[:1]), g.illumination(shots[1:])
+        o = oracle2d_mod.Oracle2D(p)
+        o.load_model(1, *bg)
+        Wo = o.illumination(shots)
+        assert cases.rel_l2(W, Wo) < 1e-4, cases.rel_l2(W, Wo)
+        assert cases.rel_l2(W1 + W2, W) < 1e-6
+        w = (1.0 / (W + 1e-2 * W.max())).astype(np.float32)
+        w /= w.mean()
+        rng = np.random.default_rng(4)
+        dm = np.zeros((3, p.nx, p.nz), np.float32)
+        dm[:2, 4:-4, 6:-4] = 0.05 * rng.standard_normal((2, p.nx - 8, p.nz - 10))
+        d = []
+        for sx, sz in shots:
+            g.set_shot(sx, sz)
+            d.append(g.born(dm))
+        d = np.stack(d)
+        md, misd, _, itd = g.lsrtm_cg(shots, d, 4, pmask=3, precond=erm.DiagPrecond(w))
+        mc, misc, _, itc = g.lsrtm_cg(shots, d, 4, pmask=3, precond=lambda gr: (w[None] * gr).astype(np.float32))
+        assert itd == itc == 4 and np.all(np.diff(misd) <= 1e-6)
+        np.testing.assert_allclose(misd, misc, rtol=1e-5)
+        np.testing.assert_allclose(md, mc, rtol=1e-4, atol=1e-4 * np.abs(mc).max())
+        with erm.UNet((8, 8, 16, 16), seed=5) as net:
+            seen = {}
+            mw, misw, _, _ = g.lsrtm_cg(shots, d, 2, pmask=3, precond=net, weights=w,
+                                        monitor=lambda it, m, gr, mis: seen.setdefault(it, gr))
+            mu, misu, _, _ = g.lsrtm_cg(shots, d, 2, pmask=3,
+                                        precond=lambda gr: _net_apply(net, w[None] * gr))
+        np.testing.assert_allclose(misw, misu, rtol=1e-5)
+        assert misw[-1] < misw[0]
+
+
+def _net_apply(net, x):
+    """z = s * net(x / s) on the two parameter planes (s = RMS over them),
+    the third plane passing through: erm_unet_precond's contract."""
+    nx, nz = x.shape[1:]
+    s = float(np.sqrt(np.mean(x[:2].astype(np.float64) ** 2)))
+    H, W = (nx + 7) // 8 * 8, (nz + 7) // 8 * 8
+    xin = np.zeros((1, 2, H, W), np.float32)
+    xin[0, :, :nx, :nz] = x[:2] / s
+    y = net.forward(xin)
+    z = x.astype(np.float32).copy()
+    z[:2] = y[0, :, :nx, :nz] * s
+    return z

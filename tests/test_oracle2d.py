@@ -247,3 +247,39 @@ def test_q_compensated_receiver_pass_restores_amplitude(oracle2d_mod):
     d4 = abs(energy_after_receiver_pass(1e4, 1) / energy_after_receiver_pass(1e4, 0) - 1)
     d6 = abs(energy_after_receiver_pass(1e6, 1) / energy_after_receiver_pass(1e6, 0) - 1)
     assert d6 < 1e-3 and 30 < d4 / d6 < 300, (d4, d6)
+
+
+def _illum_case():
+    p = cases2d.params(nx=40, nz=30, b=10, R=4, L=0, lt=150, freq=25.0, src=(20, 2))
+    vp, vs, rho, _, _ = cases2d.model(p, seed=21, q=False)
+    bg = [cases2d.smooth2(a) for a in (vp, vs, rho)]
+    return p, (vp, vs, rho), bg
+
+
+def test_illumination_additive_background_only_and_peaked_at_source(oracle2d_mod):
+    """Oracle source illumination: the sum over shots is the sum of the
+    per-shot images (the state is reset between shots), only the background
+    model enters (a different true model changes nothing), it is
+    non-negative and peaks next to the source."""
+    p, true, bg = _illum_case()
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(0, *true)
+    o.load_model(1, *bg)
+    s1, s2 = [(10, 2)], [(30, 3)]
+    w1, w2, w12 = o.illumination(s1), o.illumination(s2), o.illumination(s1 + s2)
+    np.testing.assert_allclose(w12, w1 + w2, rtol=1e-12, atol=1e-30)
+    assert w1.min() >= 0 and w1.max() > 0
+    i, k = np.unravel_index(np.argmax(w1), w1.shape)
+    assert abs(i - 10) <= 3 and abs(k - 2) <= 3, (i, k)
+    o2 = oracle2d_mod.Oracle2D(p)
+    o2.load_model(0, *[(a * 1.3).astype(np.float32) for a in true])
+    o2.load_model(1, *bg)
+    np.testing.assert_array_equal(o2.illumination(s1), w1)
+    o3 = oracle2d_mod.Oracle2D(p)
+    o3.load_model(0, *true)
+    o3.load_model(1, *[(a * 1.1).astype(np.float32) for a in bg])
+    assert cases2d_rel(o3.illumination(s1), w1) > 1e-3
+
+
+def cases2d_rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
