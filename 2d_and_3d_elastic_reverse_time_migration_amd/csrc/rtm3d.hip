@@ -103,6 +103,21 @@ inline dim3 grid_of(const Geo& g) { return dim3(g.xcd_map ? 8 * g.per_xcd : g.nt
 // Tiles with no updated cell are dropped.  The kernels also build for the
 // single-axis groups 1..6 (bit 0 x, 1 y, 2 z), but launching the groups one
 // after another measured slower (launch tails) than the two-group split.
+// XCD-aware order of a tile list: workgroup b runs on XCD b % 8, each XCD with
+// its own L2, so XCD x gets the x-th contiguous chunk of the (x-fastest) list
+// -- its tiles' x, y and z neighbours, whose halo planes and rows it re-reads,
+// then sit in the same L2 -- at positions x, x + 8, x + 16, ... (chunk sizes
+// differ by at most one; the map is a bijection of [0, n)).
+std::vector<int> xcd_order(const std::vector<int>& t) {
+    const int n = (int)t.size(), s = n / 8, r = n % 8;
+    std::vector<int> out(n);
+    for (int b = 0; b < n; ++b) {
+        const int x = b % 8, j = b / 8;
+        out[b] = t[x * s + std::min(x, r) + j];
+    }
+    return out;
+}
+
 int build_split(erm_ctx* c) {
     if (c->split_built && c->split_ty == c->ty && c->split_kchunk == c->kchunk) return 0;
     Geo g = make_geo(c);
@@ -128,6 +143,7 @@ int build_split(erm_ctx* c) {
         c->tiles[m] = nullptr;
         c->ntl[m] = (int)grp[m].size();
         if (grp[m].empty()) continue;
+        if (c->xcd_map) grp[m] = xcd_order(grp[m]);
         if (dmalloc(c, &c->tiles[m], grp[m].size())) return ERM_E_NOMEM;
         HIPC(c, hipMemcpy(c->tiles[m], grp[m].data(), grp[m].size() * sizeof(int), hipMemcpyHostToDevice));
     }
@@ -1145,15 +1161,21 @@ int erm_copy_bandwidth(int device, size_t bytes, int reps, double* ms_per_copy) 
     hipEvent_t e0, e1;
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
-    const int blocks = 256 * 8 * 4;
-    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n4);  // warm-up
-    (void)hipEventRecord(e0, 0);
-    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n4);
-    (void)hipEventRecord(e1, 0);
-    hipError_t e = hipEventSynchronize(e1);
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    *ms_per_copy = ms / reps;
+    // the grid-stride copy's rate depends on the grid (DRAM page locality of the
+    // concurrently streamed window vs latency hiding): report the best of a sweep
+    hipError_t e = hipSuccess;
+    double best = 0;
+    for (int blocks = 1024; blocks <= 16384 && e == hipSuccess; blocks *= 2) {
+        hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n4);  // warm-up
+        (void)hipEventRecord(e0, 0);
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, 0, a, b, n4);
+        (void)hipEventRecord(e1, 0);
+        e = hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (best == 0 || ms / reps < best) best = ms / reps;
+    }
+    *ms_per_copy = best;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(a);

@@ -302,6 +302,8 @@ def main():
     ap.add_argument("--tile-y", type=int, default=0, help="launch shape override (0: library default)")
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0: library default)")
     ap.add_argument("--boundary", choices=["cpml", "damp"], default="cpml")
+    ap.add_argument("--xcd-map", type=int, choices=[0, 1], default=0,
+                    help="XCD-aware tile order (each XCD a contiguous chunk of tiles)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the plain copy kernel over GIB GiB (HBM counter calibration) and exit")
     ap.add_argument("--shots", type=int, default=60, help="visco workload: shots")
@@ -330,6 +332,7 @@ def main():
     nt = args.nt
     p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
     p.tile_y, p.zchunk = args.tile_y, args.zchunk
+    p.xcd_map = args.xcd_map
     vp, vs, rho = erm.make_model3d(1, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, 2)
     vp_s = smooth_box(vp)
     vs_s = (vp_s / 1.732).astype(np.float32)

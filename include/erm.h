@@ -225,9 +225,10 @@ int   erm_get_reconstructed_field(erm_ctx* ctx, int field, float* dst);
  * wavefield and the excitation state afterwards. */
 int   erm_autotune(erm_ctx* ctx, int steps, int* tile_y, int* zchunk, double* ms_per_step);
 /* Measurement utility (no reference counterpart): time `reps` launches of a
- * plain float4 streaming copy of `bytes` bytes on `device`; it is the
- * calibration kernel for HBM byte counters and the practical bandwidth
- * ceiling (2*bytes of traffic per copy). */
+ * plain float4 grid-stride copy of `bytes` bytes on `device` for each grid of
+ * 1024, 2048, .. 16384 workgroups and return the best time per copy; it is
+ * the calibration kernel for HBM byte counters (every launch moves the same
+ * bytes) and the practical bandwidth ceiling (2*bytes of traffic per copy). */
 int   erm_copy_bandwidth(int device, size_t bytes, int reps, double* ms_per_copy);
 
 /* Synthetic models (seeded; std::mt19937_64).  Arrays are model-region,
