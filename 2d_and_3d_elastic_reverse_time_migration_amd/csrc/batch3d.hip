@@ -483,7 +483,10 @@ int erm_batch_run(erm_batch* b, float* stp, float* spp, float* sps) {
         if ((rc = run_steps(b, ERM_PASS_MODEL, 0, b->p.lt, b->use_graph, &ms))) return rc;
         b->ms[0] = ms;
         b->steps[0] = b->p.lt;
+        if (hipStreamSynchronize(b->stream) != hipSuccess) return b->fail(ERM_E_DEVICE, "model pass failed");
         for (size_t s = 0; s < b->shots.size(); ++s) {
+            if ((rc = mute_direct(b->shots[s])))  // as erm_forward_real: the window's own direct wave
+                return b->fail(rc, "shot %zu: %s", s, b->shots[s]->err.c_str());
             b->shots[s]->have_record = true;
             b->have_rec[s] = 1;  // kept for later runs until the true model or the shots change
         }

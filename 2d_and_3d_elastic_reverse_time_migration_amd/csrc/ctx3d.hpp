@@ -124,3 +124,8 @@ inline Excite make_excite(const erm_ctx* c) {
     e.n = (long)c->ncell;
     return e;
 }
+
+// direct-wave mute of the modelled record when p.cut_direct_wave > 0 and
+// p.vsp == 0 (defined in rtm3d.hip, used by the one-shot and batched model passes)
+int mute_direct(erm_ctx* c);
+int mute_cut_samples(const erm_ctx* c, double vp_src, std::vector<int>& cut);

@@ -805,6 +805,15 @@ struct StripFields {
 // Receivers: write_or_add_shot_3D_surface_three (:127 record, :328 inject).
 // rec layout [3][lt][nr]; idx[r] is the padded-grid index of receiver r.
 // ---------------------------------------------------------------------------
+// direct-wave mute of a record [3][lt][nr]: samples it < cut[r] become 0
+[[maybe_unused]] static __global__ void k_mute(float* __restrict__ rec, int nr, long lt, const int* __restrict__ cut) {
+    const long n = 3 * lt * nr;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(t % nr);
+        const long it = (t / nr) % lt;
+        if (it < cut[r]) rec[t] = 0.f;
+    }
+}
 [[maybe_unused]] static __global__ void k_record(const float* __restrict__ vx, const float* __restrict__ vy,
                          const float* __restrict__ vz, const long* __restrict__ idx, int nr,
                          float* __restrict__ rec, long comp_stride, long it_off) {

@@ -508,6 +508,54 @@ int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false
 
 }  // namespace
 
+// Direct-wave mute of the modelled record (the reference calls
+// cut_direct_shot_3D_surface_three_new after recording each step when vsp == 0,
+// 3D_forward_or_back_together.cu:133-136; its body is not in the reference, so the
+// window is defined here): at receiver r the samples
+//   it < ceil(|x_r - x_s| / (vp_s dt) - 1e-6) + wl + cut_direct_wave
+// of all three components are zeroed -- the direct P arrival from the source at
+// the true model's P velocity at the source cell, the wavelet's duration (wl
+// samples, the Ricker starts at it = 0) and cut_direct_wave samples of margin.
+int mute_cut_samples(const erm_ctx* c, double vp_src, std::vector<int>& cut) {
+    const erm_params& p = c->p;
+    cut.assign(c->nr, 0);
+    const double dts = p.dt / 1000.0;
+    for (int iry = 0; iry < p.receiver_num_y; ++iry)
+        for (int irx = 0; irx < p.receiver_num_x; ++irx) {
+            const double ddx = (p.receiver_start_x + irx * p.receiver_interval_x - c->sxr) * (double)p.dx;
+            const double ddy = (p.receiver_start_y + iry * p.receiver_interval_y - c->syr) * (double)p.dy;
+            const double ddz = (p.receiver_start_z - c->szr) * (double)p.dz;
+            const double t = std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz) / (vp_src * dts);
+            // 1e-6 sample of slack: a round t does not hinge on the last bit of vp_src
+            const double n = std::ceil(t - 1e-6) + c->wl + p.cut_direct_wave;
+            cut[(size_t)iry * p.receiver_num_x + irx] = (int)std::min<double>(n, p.lt);
+        }
+    return 0;
+}
+
+int mute_direct(erm_ctx* c) {
+    if (c->p.cut_direct_wave <= 0 || c->p.vsp != 0 || c->nr <= 0) return 0;
+    if (!c->have_model[0]) return c->fail(ERM_E_STATE, "true model not loaded");
+    float mb[2];  // M = rho vp^2 and b = 1/rho at the source cell
+    HIPC(c, hipMemcpyAsync(&mb[0], c->M[0] + c->src, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(&mb[1], c->BU[0] + c->src, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    std::vector<int> cut;
+    mute_cut_samples(c, std::sqrt((double)mb[0] * (double)mb[1]), cut);
+    int* dcut = nullptr;
+    if (dmalloc(c, &dcut, cut.size())) return ERM_E_NOMEM;
+    hipError_t e = hipMemcpyAsync(dcut, cut.data(), cut.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    const long n = 3L * c->p.lt * c->nr;
+    if (e == hipSuccess)
+        hipLaunchKernelGGL(k_mute, dim3((unsigned)std::min<long>((n + 255) / 256, 4096)), dim3(256), 0, c->stream,
+                           c->rec, c->nr, (long)c->p.lt, (const int*)dcut);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(dcut);
+    if (e != hipSuccess) return c->fail(ERM_E_DEVICE, "direct-wave mute: %s", hipGetErrorString(e));
+    return 0;
+}
+
 extern "C" {
 
 const char* erm_last_error(const erm_ctx* ctx) { return ctx ? ctx->err.c_str() : global_error(); }
@@ -806,7 +854,7 @@ int erm_forward_real(erm_ctx* c, float* rec) {
     HIPC(c, hipSetDevice(c->device));
     int rc = erm_zero_wavefield(c);
     if (rc) return rc;
-    if ((rc = run_pass(c, ERM_PASS_MODEL, 0, c->p.lt))) return rc;
+    if ((rc = run_pass(c, ERM_PASS_MODEL, 0, c->p.lt)) || (rc = mute_direct(c))) return rc;
     if (rec) return erm_get_record(c, rec);
     return 0;
 }

@@ -157,7 +157,13 @@ int   erm_zero_excitation(erm_ctx* ctx);
 /* The three time loops.
  * rec layout: [3][lt][receiver_num_y][receiver_num_x] (vx, vy, vz).
  * erm_forward_real  ≙ forward_together_using_real_model     (3D_forward_or_back_together.cu:9-155);
- *                     rec may be NULL (the record stays on the device).
+ *                     rec may be NULL (the record stays on the device).  With
+ *                     cut_direct_wave > 0 and vsp == 0 the direct wave is muted
+ *                     (cut_direct_shot_3D_surface_three_new, :133-136, body not in
+ *                     the reference): at each receiver the samples
+ *                     it < ceil(offset / (vp_source dt)) + wavelet length +
+ *                     cut_direct_wave are zeroed, offset = source-receiver distance
+ *                     in metres, vp_source = the true model's vp at the source cell.
  * erm_forward_smoothed ≙ forward_together_using_smoothed_model (:161-290).
  * erm_backward_smoothed ≙ backward_together_using_smoothed_model (:296-453);
  *                     rec == NULL uses the device-resident record. */

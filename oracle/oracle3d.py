@@ -112,6 +112,37 @@ def damping_axis(p: Params, axis: int) -> np.ndarray:
     return out
 
 
+def mute_direct(p: Params, rec, vp, shot=(0, 0, 0)):
+    """Direct-wave mute of a record [3][lt][ny_rec][nx_rec] (the library applies it
+    in erm_forward_real when cut_direct_wave > 0 and vsp == 0), FP64: at each
+    receiver the samples it < ceil(offset / (vp_s dt) - 1e-6) + wl + cut_direct_wave
+    become 0, with offset the source-receiver distance in metres, vp_s the true
+    model's vp at the source cell (vp: model region [nx][ny][nz]) and wl the
+    Ricker length in samples.  The reference calls
+    cut_direct_shot_3D_surface_three_new (3D_forward_or_back_together.cu:133-136)
+    without its body, so the window is the library's definition, restated;
+    pinned by tests/test_oracle3d.py (zero-offset cut = wl + margin, monotone in
+    offset, only the direct wave's samples removed in a homogeneous medium)."""
+    out = np.array(rec, np.float64, copy=True)
+    if p.cut_direct_wave <= 0 or p.vsp != 0:
+        return out
+    sx = p.shot_start_x + shot[0] * p.shot_interval_x
+    sy = p.shot_start_y + shot[1] * p.shot_interval_y
+    sz = p.shot_start_z + shot[2] * p.shot_interval_z
+    vps = float(np.asarray(vp)[sx, sy, sz])
+    wl = len(ricker(p.freq, p.dt))
+    dts = p.dt / 1000.0
+    for iry in range(p.receiver_num_y):
+        for irx in range(p.receiver_num_x):
+            ddx = (p.receiver_start_x + irx * p.receiver_interval_x - sx) * p.dx
+            ddy = (p.receiver_start_y + iry * p.receiver_interval_y - sy) * p.dy
+            ddz = (p.receiver_start_z - sz) * p.dz
+            t = np.sqrt(ddx * ddx + ddy * ddy + ddz * ddz) / (vps * dts)
+            n = min(int(np.ceil(t - 1e-6)) + wl + p.cut_direct_wave, p.lt)
+            out[:, :n, iry, irx] = 0.0
+    return out
+
+
 class Oracle3D:
     """FP64 (prec=64) or FP32 (prec=32) CPU restatement of the 3D path."""
 

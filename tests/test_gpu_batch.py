@@ -24,9 +24,10 @@ WINDOWS = [  # (sx, sy, sz, x0, y0, wx, wy): uneven apertures
 ]
 
 
-def _full(boundary, R=4, lt=70):
+def _full(boundary, R=4, lt=70, cut=0):
     p = cases.params(nx=64, ny=52, nz=30, b=8 if R == 4 else 10, R=R, lt=lt, rec=(1, 1, 0, 0, 3, 3, 2))
     p.boundary = boundary
+    p.cut_direct_wave = cut
     p.receiver_interval_x, p.receiver_interval_y = 3, 2
     p.receiver_start_z = 2
     vp, vs, rho = cases.layered(p, seed=11)
@@ -52,9 +53,11 @@ def _cut(a, w):
     return np.ascontiguousarray(a[x0:x0 + wx, y0:y0 + wy, :])
 
 
-@pytest.mark.parametrize("boundary,R", [(0, 4), (1, 4), (1, 6)])
-def test_batch_matches_single_shots(erm, boundary, R):
-    p, true, sm = _full(boundary, R)
+@pytest.mark.parametrize("boundary,R,cut", [(0, 4, 0), (1, 4, 3), (1, 6, 0)])
+def test_batch_matches_single_shots(erm, boundary, R, cut):
+    """cut > 0: the records are direct-wave muted, per window, in both paths
+    (lt long enough for data to survive the 87-sample wavelet window)."""
+    p, true, sm = _full(boundary, R, lt=170 if cut else 70, cut=cut)
     with erm.Batch3D(p) as b:
         b.load_model(0, *true)
         b.load_model(1, *sm)

@@ -416,3 +416,32 @@ def test_xcorr_images_reflector_depth(oracle):
     k = int(np.argmax(col)) + 12
     assert abs(k - zi) <= 2, (k, zi)
     assert np.isfinite(ps).all() and np.abs(ps).max() > 0
+
+
+def test_direct_wave_mute_window(oracle):
+    """The oracle's direct-wave mute: zero-offset cut = wavelet length + margin,
+    the cut grows with offset, it covers the peak of the direct arrival in a
+    homogeneous medium, and samples at and after the cut are untouched."""
+    p = cases.params(nx=24, ny=20, nz=18, b=8, R=4, lt=160, dt=1.0, freq=25.0, src=(12, 10, 2),
+                     rec=(6, 1, 0, 10, 4, 1, 2))
+    p.cut_direct_wave = 3
+    vp, vs, rho = cases.homogeneous(p)
+    o = oracle.Oracle3D(p, 64)
+    o.load_model(0, vp, vs, rho)
+    o.step(0, 0, p.lt)
+    rec = o.get_record()
+    m = oracle.mute_direct(p, rec, vp)
+    wl = len(oracle.ricker(p.freq, p.dt))
+    cut = [int(np.argmax(np.any(m[:, :, 0, r] != 0, axis=0))) for r in range(p.receiver_num_x)]
+    offs = [abs(p.receiver_start_x + r * p.receiver_interval_x - 12) for r in range(p.receiver_num_x)]
+    zero = offs.index(0)
+    assert cut[zero] == wl + 3, (cut, wl)
+    order = np.argsort(offs)
+    assert all(cut[order[i]] <= cut[order[i + 1]] for i in range(len(order) - 1)), (offs, cut)
+    for r in range(p.receiver_num_x):
+        peak = int(np.argmax(np.abs(rec[:, :, 0, r]).max(axis=0)))
+        assert peak < cut[r], (r, peak, cut[r])
+        np.testing.assert_array_equal(m[:, cut[r]:, 0, r], rec[:, cut[r]:, 0, r])
+        assert not np.any(m[:, :cut[r], 0, r])
+    q = cases.params(nx=24, ny=20, nz=18, b=8, R=4, lt=160, dt=1.0, freq=25.0, src=(12, 10, 2))
+    np.testing.assert_array_equal(oracle.mute_direct(q, rec[:, :, :1, :1], vp), rec[:, :, :1, :1])  # off by default
