@@ -130,6 +130,7 @@ def _load():
         "erm_unet_precond": (ctypes.c_int, [fp, fp, vp]),
         "erm_unet_bind_weights": (ctypes.c_int, [vp, fp]),
         "erm2d_illumination": (ctypes.c_int, [vp, ctypes.c_int, ip, fp]),
+        "erm2d_angle_gathers": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_float, fp, fp, fp, fp, fp]),
         "erm2d_precond_diag": (ctypes.c_int, [fp, fp, vp]),
         "erm2d_set_workers": (ctypes.c_int, [vp, ctypes.c_int]),
         "erm_batch_last_error": (ctypes.c_char_p, [vp]),
@@ -484,6 +485,21 @@ class Elastic2D:
     def set_workers(self, n):
         """Run the shots of lsrtm_cg on n concurrent contexts/streams."""
         self._c(lib.erm2d_set_workers(self._h, n))
+
+    def angle_gathers(self, nang=30, dang=3.0):
+        """Angle-domain gathers of the last rtm() (erm2d_angle_gathers):
+        (pp [nang][nx][nz], ps [nang][nx][nz], angle-compensated pp and ps
+        [nx][nz], reflection angle [nx][nz] in degrees); bins of dang degrees
+        centred on the vertical."""
+        p = self.p
+        pp = np.empty((nang, p.nx, p.nz), np.float32)
+        ps = np.empty_like(pp)
+        ppc = np.empty((p.nx, p.nz), np.float32)
+        psc = np.empty_like(ppc)
+        gam = np.empty_like(ppc)
+        self._c(lib.erm2d_angle_gathers(self._h, nang, dang, _fptr(pp), _fptr(ps), _fptr(ppc), _fptr(psc),
+                                        _fptr(gam)))
+        return pp, ps, ppc, psc, gam
 
     def illumination(self, shots):
         """Source illumination of the background model summed over shots

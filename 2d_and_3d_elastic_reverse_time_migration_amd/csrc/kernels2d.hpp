@@ -42,6 +42,7 @@ struct M2 {  // model: unrelaxed moduli, buoyancy, SLS coefficients
 struct Ex2 {
     int* time;
     float *amp, *tp, *vxp, *vzp, *img_tp, *img_pp, *img_ps;
+    float *rvxp, *rvzp;  // receiver vector P at the imaging step (angle gathers)
 };
 
 __device__ __forceinline__ int clamp2(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -213,6 +214,7 @@ __global__ __launch_bounds__(TZ* TXR) void k2_velocity(F2 f, M2 m, G2 g, Coef2 c
             const float e1 = ex.vxp[n], e2 = ex.vzp[n], den = e1 * e1 + e2 * e2;
             ex.img_pp[n] = den > 0.f ? (e1 * nxp + e2 * nzp) / den : 0.f;
             ex.img_ps[n] = den > 0.f ? (e1 * (nvx - nxp) + e2 * (nvz - nzp)) / den : 0.f;
+            ex.rvxp[n] = nxp; ex.rvzp[n] = nzp;
         }
     }
 }
@@ -426,6 +428,78 @@ __global__ void k2_illum_acc(const float* __restrict__ vx, const float* __restri
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) {
         const float a = vx[t], b = vz[t];
         W[t] += a * a + b * b;
+    }
+}
+// Angle-domain imaging of the excitation images.  The reference has the
+// pieces -- P Poynting directions, least-squares angles over a 5x5 window
+// (caculate_ex_angle_pp_only_RTM, elastic_2D_kernel_3.cu:132-208), opening
+// angles (caculate_angle_open, elastic_2D_kernel_2.cu:1236-1256), angle bins
+// (set_adcigs_pp_new / _ps_new, :1751-1819) -- but no driver calls them; here,
+// per model cell with an excitation time:
+//   source incidence  ts = atan(sum Sx Sz / sum Sz^2), S = -tp vP at excitation
+//                     (Poynting vector of the P wave), 5x5 window, degrees from
+//                     the vertical (z down), +ve toward +x;
+//   receiver          tr = atan(sum rvxp rvzp / sum rvzp^2) over the same
+//                     window, the receiver P motion at each cell's imaging step
+//                     (as caculate_ex_angle_rp_only_RTM, :209-285; sign-free:
+//                     P motion is along the ray);
+//   reflection angle  g = (ts - tr) / 2 (the bisector of the two rays is the
+//                     reflector normal);
+//   gathers           bin floor((g + nang dang / 2) / dang) of pp / ps
+//                     [nang][nx][nz] receives img_pp / img_ps;
+//   compensation      the reference's angle-dependent compensation with the
+//                     opening angle 2g (cuda_ex_com_pp_ps_sign and
+//                     imaging_pp/ps_compensate_dependent_angle_2D_new,
+//                     elastic_2D_kernel_3.cu:698-770):
+//                     ppc = s(cos 2g) img_pp / (|cos 2g| + 0.03), s(x) = +1 for
+//                     x > 0 else -1; psc = img_ps / (|sin 2g| + 0.1) (the
+//                     reference also negates both; the images keep their sign).
+//                     The PS image of this vector inner-product condition does
+//                     not reverse polarity across the source (both factors are
+//                     true vectors), so no sign correction is applied.
+// Cells without an excitation time get g = 0, no gather entry and 0.
+__device__ __forceinline__ float deg_atan_ratio(float num, float den) {
+    if (den != 0.f) return atanf(num / den) * (180.f / 3.14159265358979f);
+    return num > 0.f ? 90.f : (num < 0.f ? -90.f : 0.f);
+}
+__global__ void k2_adcig(Ex2 ex, G2 g, int nang, float dang, float* __restrict__ pp, float* __restrict__ ps,
+                         float* __restrict__ ppc, float* __restrict__ psc, float* __restrict__ gam) {
+    const long mn = (long)g.nx * g.nz;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < mn; q += (long)gridDim.x * blockDim.x) {
+        const int ix = (int)(q / g.nz), iz = (int)(q % g.nz);
+        const int i = ix + g.bl, k = iz + g.bu;
+        const long n = (long)i * g.pz + k;
+        float gm = 0.f;
+        if (ex.time[n] >= 0) {
+            float sxz = 0.f, szz = 0.f, rxz = 0.f, rzz = 0.f;
+            for (int di = -2; di <= 2; ++di)
+                for (int dk = -2; dk <= 2; ++dk) {
+                    const int ii = min(max(i + di, 0), g.nnx - 1), kk = min(max(k + dk, 0), g.nnz - 1);
+                    const long m = (long)ii * g.pz + kk;
+                    const float t = ex.tp[m], sx = -t * ex.vxp[m], sz = -t * ex.vzp[m];
+                    sxz += sx * sz;
+                    szz += sz * sz;
+                    const float rx = ex.rvxp[m], rz = ex.rvzp[m];
+                    rxz += rx * rz;
+                    rzz += rz * rz;
+                }
+            const float ts = deg_atan_ratio(sxz, szz), tr = deg_atan_ratio(rxz, rzz);
+            gm = 0.5f * (ts - tr);
+            const int b = (int)floorf((gm + 0.5f * nang * dang) / dang);
+            if (b >= 0 && b < nang) {
+                if (pp) pp[(long)b * mn + q] = ex.img_pp[n];
+                if (ps) ps[(long)b * mn + q] = ex.img_ps[n];
+            }
+        }
+        float cp = 0.f, cs = 0.f;
+        if (ex.time[n] >= 0) {
+            const float o = gm * (2.f * 3.14159265358979f / 180.f), co = cosf(o);
+            cp = (co > 0.f ? ex.img_pp[n] : -ex.img_pp[n]) / (fabsf(co) + 0.03f);
+            cs = ex.img_ps[n] / (fabsf(sinf(o)) + 0.1f);
+        }
+        if (ppc) ppc[q] = cp;
+        if (psc) psc[q] = cs;
+        if (gam) gam[q] = gm;
     }
 }
 __global__ void k2_fill(float* __restrict__ p, long n, float v) {

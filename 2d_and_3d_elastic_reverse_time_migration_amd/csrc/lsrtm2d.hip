@@ -89,7 +89,7 @@ struct erm2d_ctx {
     float* rec = nullptr;   // [2][lt][nrec]
     float* rec2 = nullptr;  // second record buffer (residual / Born data)
     int* ex_time = nullptr;
-    float* ex[4] = {nullptr};  // amp tp vxp vzp
+    float* ex[6] = {nullptr};  // amp tp vxp vzp, receiver vxp vzp at the imaging step
     float* img[3] = {nullptr};
     float* hist = nullptr;  // [lt][5][mn]: E (3) then F (2)
     float* dm = nullptr;    // [3][mn]
@@ -159,6 +159,7 @@ M2 model(const erm2d_ctx* c, int which) {
 Ex2 excite(const erm2d_ctx* c) {
     Ex2 e;
     e.time = c->ex_time; e.amp = c->ex[0]; e.tp = c->ex[1]; e.vxp = c->ex[2]; e.vzp = c->ex[3];
+    e.rvxp = c->ex[4]; e.rvzp = c->ex[5];
     e.img_tp = c->img[0]; e.img_pp = c->img[1]; e.img_ps = c->img[2];
     return e;
 }
@@ -563,6 +564,7 @@ int erm2d_get_field(erm2d_ctx* c, int field, float* dst) {
         return 0;
     }
     if (field == ERM2D_EX_AMP) return down2(c, dst, c->ex[0]);
+    if (field >= ERM2D_EX_TP && field <= ERM2D_R_VZP) return down2(c, dst, c->ex[1 + field - ERM2D_EX_TP]);
     if (field >= ERM2D_IMG_TP && field <= ERM2D_IMG_PS) return down2(c, dst, c->img[field - ERM2D_IMG_TP]);
     return c->fail(ERM_E_PARAM, "unknown field %d", field);
 }
@@ -697,6 +699,34 @@ int erm2d_adjoint(erm2d_ctx* c, const float* r, float* gout) {
         HC(c, hipStreamSynchronize(c->stream));
     }
     return 0;
+}
+
+int erm2d_angle_gathers(erm2d_ctx* c, int nang, float dang, float* pp, float* ps, float* pp_comp,
+                        float* ps_comp, float* gamma) {
+    if (!c) return ERM_E_PARAM;
+    if (nang <= 0 || !(dang > 0.f) || nang * dang > 180.f) return c->fail(ERM_E_PARAM, "bad angle bins");
+    HC(c, hipSetDevice(c->device));
+    const size_t mn = c->mn;
+    float* d = nullptr;  // [nang][mn] pp, [nang][mn] ps, [mn] ppc, [mn] psc, [mn] gamma
+    const size_t nd = 2 * (size_t)nang * mn + 3 * mn;
+    if (dmal(c, &d, nd)) return ERM_E_NOMEM;
+    int rc = fillf(c, d, nd, 0.f);
+    if (!rc) {
+        const G2 g = geo(c);
+        hipLaunchKernelGGL(k2_adcig, dim3((unsigned)std::min<size_t>((mn + 255) / 256, 2048)), dim3(256), 0,
+                           c->stream, excite(c), g, nang, dang, d, d + nang * mn, d + 2 * nang * mn,
+                           d + 2 * nang * mn + mn, d + 2 * nang * mn + 2 * mn);
+        hipError_t e = hipGetLastError();
+        float* outs[5] = {pp, ps, pp_comp, ps_comp, gamma};
+        const size_t off[5] = {0, nang * mn, 2 * nang * mn, 2 * nang * mn + mn, 2 * nang * mn + 2 * mn};
+        const size_t len[5] = {nang * mn, nang * mn, mn, mn, mn};
+        for (int a = 0; a < 5 && e == hipSuccess; ++a)
+            if (outs[a]) e = hipMemcpyAsync(outs[a], d + off[a], len[a] * 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = c->fail(ERM_E_DEVICE, "angle gathers: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(d);
+    return rc;
 }
 
 int erm2d_illumination(erm2d_ctx* c, int nshots, const int* shots, float* out) {

@@ -2,7 +2,7 @@
 import ctypes
 
 FIELDS2D = ["vx", "vz", "txx", "tzz", "txz", "tp", "vxp", "vzp", "div", "curl",
-            "ex_time", "ex_amp", "img_tp", "img_pp", "img_ps"]
+            "ex_time", "ex_amp", "img_tp", "img_pp", "img_ps", "ex_tp", "ex_vxp", "ex_vzp", "r_vxp", "r_vzp"]
 FIELD2D_ID = {n: i for i, n in enumerate(FIELDS2D)}
 
 

@@ -44,6 +44,8 @@ enum erm2d_field {
     ERM2D_VX = 0, ERM2D_VZ, ERM2D_TXX, ERM2D_TZZ, ERM2D_TXZ, ERM2D_TP, ERM2D_VXP, ERM2D_VZP,
     ERM2D_DIV, ERM2D_CURL,              /* decom (elastic_2D_kernel_2.cu:563-669), read only */
     ERM2D_EX_TIME, ERM2D_EX_AMP, ERM2D_IMG_TP, ERM2D_IMG_PP, ERM2D_IMG_PS,
+    ERM2D_EX_TP, ERM2D_EX_VXP, ERM2D_EX_VZP,  /* excitation state, read only */
+    ERM2D_R_VXP, ERM2D_R_VZP,                /* receiver vector P at each cell's imaging step, read only */
     ERM2D_FIELD_COUNT
 };
 /* passes for erm2d_step */
@@ -110,6 +112,27 @@ typedef int (*erm2d_precond_fn)(const float* g, float* z, void* user);
 int   erm2d_lsrtm_cg(erm2d_ctx* ctx, int nshots, const int* shots, const float* dobs, int iters, unsigned pmask,
                      float* m, float* misfit, double* seconds, erm2d_precond_fn precond, void* user, float tol);
 
+/* Angle-domain common-image gathers and PS polarity correction of the last
+ * erm2d_rtm (excitation imaging).  The reference has the building blocks
+ * (Poynting vectors, least-squares angles, opening angles, angle bins:
+ * elastic_2D_kernel_2.cu:8-150,1236-1819, elastic_2D_kernel_3.cu:111-364) but
+ * no driver calls them; here, per model cell with an excitation time:
+ * source incidence ts = atan(sum Sx Sz / sum Sz^2) over a 5x5 window of the
+ * excitation-time P Poynting vector S = -tp vP (degrees from the vertical,
+ * +ve toward +x), receiver direction tr = atan(sum rvxp rvzp / sum rvzp^2) of
+ * the receiver P motion at the imaging step over the same window, reflection
+ * angle g = (ts - tr) / 2 (reliable below the critical angle).  Gathers
+ * pp, ps [nang][nx][nz]: bin floor((g + nang dang / 2) / dang) receives the
+ * cell's PP / PS image value.  pp_comp, ps_comp [nx][nz]: the reference's
+ * angle-dependent compensation with the opening angle 2g
+ * (elastic_2D_kernel_3.cu:698-770): PP * s(cos 2g) / (|cos 2g| + 0.03),
+ * s = +1 for a positive argument else -1, and PS / (|sin 2g| + 0.1) (the
+ * reference also negates both; here the images keep their sign).  The PS
+ * image of the vector inner-product condition does not reverse polarity
+ * across the source, so it needs no sign correction.  gamma [nx][nz] = g.
+ * Any output may be NULL; nang * dang <= 180. */
+int   erm2d_angle_gathers(erm2d_ctx* ctx, int nang, float dang, float* pp, float* ps, float* pp_comp,
+                          float* ps_comp, float* gamma);
 /* Source illumination of the background model: for each shot, the sum over
  * the time steps it = 0 .. lt-1 of vx^2 + vz^2 of the source wavefield
  * (after the step's velocity update, each component at its own staggered

@@ -112,6 +112,7 @@ struct Oracle2D {
     std::vector<long> recs;
     std::vector<int> ex_time;
     std::vector<T> ex_amp, ex_tp, ex_vxp, ex_vzp, img_tp, img_pp, img_ps;
+    std::vector<T> r_vxp, r_vzp;  // receiver vector P at each cell's imaging step
     std::vector<T> rec;  /* [2][lt][nrec] */
 
     size_t id(int i, int k) const { return (size_t)i * nnz + k; }
@@ -157,7 +158,7 @@ struct Oracle2D {
         }
         rec.assign((size_t)2 * p.lt * p.nrec, 0);
         ex_time.assign(N, -1);
-        for (auto* v : {&ex_amp, &ex_tp, &ex_vxp, &ex_vzp, &img_tp, &img_pp, &img_ps}) v->assign(N, 0);
+        for (auto* v : {&ex_amp, &ex_tp, &ex_vxp, &ex_vzp, &img_tp, &img_pp, &img_ps, &r_vxp, &r_vzp}) v->assign(N, 0);
         return set_shot(p.shot_x, p.shot_z);
     }
 
@@ -277,6 +278,7 @@ struct Oracle2D {
                     T ex = ex_vxp[n], ez = ex_vzp[n], den = ex * ex + ez * ez;
                     img_pp[n] = den > 0 ? (ex * nxp + ez * nzp) / den : (T)0;
                     img_ps[n] = den > 0 ? (ex * (nvx - nxp) + ez * (nvz - nzp)) / den : (T)0;
+                    r_vxp[n] = nxp; r_vzp[n] = nzp;
                 }
             }
     }
@@ -556,7 +558,7 @@ int oracle2d_zero(void* h, int what) {
         if (what & 1) o.zero();
         if (what & 2) {
             std::fill(o.ex_time.begin(), o.ex_time.end(), -1);
-            for (auto* v : {&o.ex_amp, &o.ex_tp, &o.ex_vxp, &o.ex_vzp}) std::fill(v->begin(), v->end(), 0);
+            for (auto* v : {&o.ex_amp, &o.ex_tp, &o.ex_vxp, &o.ex_vzp, &o.r_vxp, &o.r_vzp}) std::fill(v->begin(), v->end(), 0);
         }
         if (what & 4) for (auto* v : {&o.img_tp, &o.img_pp, &o.img_ps}) std::fill(v->begin(), v->end(), 0);
         return 0;
@@ -578,7 +580,10 @@ int oracle2d_get_field(void* h, int fid, double* dst) {
         }
         if (fid == ERM2D_EX_TIME) { for (size_t n = 0; n < o.N; ++n) dst[n] = o.ex_time[n]; return 0; }
         const std::vector<T>* s = fid == ERM2D_EX_AMP ? &o.ex_amp : fid == ERM2D_IMG_TP ? &o.img_tp
-                                : fid == ERM2D_IMG_PP ? &o.img_pp : fid == ERM2D_IMG_PS ? &o.img_ps : nullptr;
+                                : fid == ERM2D_IMG_PP ? &o.img_pp : fid == ERM2D_IMG_PS ? &o.img_ps
+                                : fid == ERM2D_EX_TP ? &o.ex_tp : fid == ERM2D_EX_VXP ? &o.ex_vxp
+                                : fid == ERM2D_EX_VZP ? &o.ex_vzp : fid == ERM2D_R_VXP ? &o.r_vxp
+                                : fid == ERM2D_R_VZP ? &o.r_vzp : nullptr;
         if (!s) return -1;
         for (size_t n = 0; n < o.N; ++n) dst[n] = (*s)[n];
         return 0;

@@ -143,6 +143,58 @@ class Oracle2D:
                 W += vx * vx + vz * vz
         return np.ascontiguousarray(W[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz])
 
+    def angle_gathers(self, nang, dang):
+        """Angle-domain gathers and PS polarity correction of the last RTM
+        (the quantities erm2d_angle_gathers returns), FP64, from the oracle's
+        excitation state, images and receiver vector P at the imaging step.
+        The reference's building blocks are the least-squares Poynting angle
+        over a 5x5 window (caculate_ex_angle_pp_only_RTM,
+        2D_elastic_RTM/elastic_2D_kernel_3.cu:132-208), the opening angle
+        (caculate_angle_open, elastic_2D_kernel_2.cu:1236-1256) and the angle
+        bins (set_adcigs_pp_new / _ps_new, :1751-1819); no driver of the
+        reference calls them, so the combination is the library's definition:
+        ts = atan(sum Sx Sz / sum Sz^2), S = -tp vP at excitation (window
+        clamped at the grid edge); tr = atan(sum rvxp rvzp / sum rvzp^2) over
+        the same window (caculate_ex_angle_rp_only_RTM, :209-285); g = (ts - tr) / 2;
+        bin floor((g + nang dang / 2) / dang).  Pinned by a flat reflector
+        (g = the geometric incidence angle below the critical angle) in
+        tests/test_oracle2d.py.  Compensation as the reference's
+        imaging_pp/ps_compensate_dependent_angle_2D_new
+        (elastic_2D_kernel_3.cu:698-770) without its overall sign flip.
+        Returns (pp, ps, pp_comp, ps_comp, gamma)."""
+        p = self.p
+        f = {k: self.get_field(k) for k in ("ex_time", "ex_tp", "ex_vxp", "ex_vzp", "r_vxp", "r_vzp",
+                                            "img_pp", "img_ps")}
+        nnx, nnz = self.shape
+        sx, sz = -f["ex_tp"] * f["ex_vxp"], -f["ex_tp"] * f["ex_vzp"]
+        pxz = np.pad(sx * sz, 2, mode="edge")
+        pzz = np.pad(sz * sz, 2, mode="edge")
+        win = lambda a: sum(a[2 + di:2 + di + nnx, 2 + dk:2 + dk + nnz] for di in range(-2, 3) for dk in range(-2, 3))
+        sxz, szz = win(pxz), win(pzz)
+        rxz = win(np.pad(f["r_vxp"] * f["r_vzp"], 2, mode="edge"))
+        rzz = win(np.pad(f["r_vzp"] * f["r_vzp"], 2, mode="edge"))
+
+        def deg(num, den):
+            with np.errstate(divide="ignore", invalid="ignore"):
+                a = np.degrees(np.arctan(num / den))
+            return np.where(den != 0, a, np.where(num > 0, 90.0, np.where(num < 0, -90.0, 0.0)))
+
+        ts, tr = deg(sxz, szz), deg(rxz, rzz)
+        cut = lambda a: a[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz]
+        ex = cut(f["ex_time"]) >= 0
+        g = np.where(ex, 0.5 * (cut(ts) - cut(tr)), 0.0)
+        b = np.floor((g + 0.5 * nang * dang) / dang).astype(int)
+        pp = np.zeros((nang, p.nx, p.nz))
+        ps = np.zeros((nang, p.nx, p.nz))
+        ok = ex & (b >= 0) & (b < nang)
+        ix, iz = np.nonzero(ok)
+        pp[b[ok], ix, iz] = cut(f["img_pp"])[ok]
+        ps[b[ok], ix, iz] = cut(f["img_ps"])[ok]
+        o = np.radians(2.0 * g)
+        ppc = np.where(ex, np.where(np.cos(o) > 0, 1.0, -1.0) * cut(f["img_pp"]) / (np.abs(np.cos(o)) + 0.03), 0.0)
+        psc = np.where(ex, cut(f["img_ps"]) / (np.abs(np.sin(o)) + 0.1), 0.0)
+        return pp, ps, ppc, psc, g
+
     def images(self):
         p = self.p
         out = []

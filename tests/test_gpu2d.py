@@ -150,3 +150,42 @@ def test_gpu2d_q_compensated_rtm_matches_oracle(erm, oracle2d_mod):
     otp, opp, ops = o.images()
     for a, b in ((pp, opp), (ps, ops)):
         assert cases.rel_l2(a[cut], b[cut]) < 1e-4
+
+
+def test_gpu2d_angle_gathers_match_oracle(erm, oracle2d_mod):
+    """erm2d_angle_gathers against the oracle's restatement on the flat-
+    reflector shot: on cells imaged at the same excitation time (and whose
+    5x5 window is too), the reflection angle agrees to 0.05 deg on 95% of the
+    cells, the bin of every cell farther than 0.2 deg from a bin edge agrees,
+    every imaged cell sits in one bin, and the compensated images agree to
+    rel L2 1e-3 (FP32 atan of FP32 window sums)."""
+    import test_oracle2d as T
+    p, true, bg, o, zi = T._flat_reflector_rtm(oracle2d_mod)
+    nang, dang = 30, 3.0
+    og = o.angle_gathers(nang, dang)
+    with erm.Elastic2D(p) as g:
+        g.load_model(0, *true)
+        g.load_model(1, *bg)
+        g.forward()
+        tp, pp, ps = g.rtm()
+        gg = g.angle_gathers(nang, dang)
+        gt = g.get_field("ex_time")
+    ot, oa = o.get_field("ex_time"), o.get_field("ex_amp")
+    same = gt == ot
+    win = np.ones_like(same)
+    for di in range(-2, 3):
+        for dk in range(-2, 3):
+            win &= np.roll(np.roll(same, di, 0), dk, 1)
+    cut = lambda a: a[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz]
+    mask = cut(win & (oa > 1e-3 * oa.max()))
+    assert mask.mean() > 0.3
+    d = np.abs(gg[4] - og[4])[mask]
+    assert np.quantile(d, 0.95) <= 0.05, np.quantile(d, [0.5, 0.95, 1.0])
+    edge = np.abs(((og[4] + 0.5 * nang * dang) / dang) - np.round((og[4] + 0.5 * nang * dang) / dang)) * dang
+    far = mask & (edge > 0.2) & (np.abs(og[4]) < 0.5 * nang * dang)
+    np.testing.assert_array_equal(np.argmax(np.abs(gg[0]), axis=0)[far & (np.abs(pp) > 0)],
+                                  np.argmax(np.abs(og[0]), axis=0)[far & (np.abs(pp) > 0)])
+    inr = np.abs(gg[4]) < 0.5 * nang * dang - dang
+    np.testing.assert_array_equal(gg[0].sum(axis=0)[inr], pp[inr])
+    for a, b in ((gg[2], og[2]), (gg[3], og[3])):
+        assert cases.rel_l2(a[mask], b[mask]) < 1e-3, cases.rel_l2(a[mask], b[mask])
