@@ -16,6 +16,8 @@ struct erm_ctx {
     erm_params p;
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;               // second stream of the region launches (region_streams)
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     int R = 6;
     int nnx = 0, nny = 0, nnz = 0, px = 0;
     long pxy = 0;

@@ -188,11 +188,20 @@ template <>
 __device__ __forceinline__ Base mkacc<Base>(const void* base, long) {
     return Base{(char*)const_cast<void*>(base)};
 }
+// Streaming cache hint (gfx950 "nt", buffer cache-policy bit 1) for the
+// accesses nothing else in the launch touches again: the updated-field
+// stores, the cell's own-state loads (bldo) and the C-PML state.  Keeps L2
+// for the stencil inputs that neighbouring tiles re-read (measured +1% on the
+// 296x296x168 C-PML workload).
+constexpr int kNtStore = 2, kNtOwn = 2;
 __device__ __forceinline__ float bld(Rsrc r, unsigned vo, unsigned so) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r.r, (int)vo, (int)so, 0));
 }
+__device__ __forceinline__ float bldo(Rsrc r, unsigned vo, unsigned so) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r.r, (int)vo, (int)so, kNtOwn));
+}
 __device__ __forceinline__ void bst(Rsrc r, unsigned vo, unsigned so, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r.r, (int)vo, (int)so, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r.r, (int)vo, (int)so, kNtStore);
 }
 __device__ __forceinline__ int bldi(Rsrc r, unsigned vo, unsigned so) {
     return (int)__builtin_amdgcn_raw_buffer_load_b32(r.r, (int)vo, (int)so, 0);
@@ -200,8 +209,15 @@ __device__ __forceinline__ int bldi(Rsrc r, unsigned vo, unsigned so) {
 __device__ __forceinline__ float bld(Base r, unsigned vo, unsigned so) {
     return *reinterpret_cast<const float*>(r.p + (vo + so));
 }
+__device__ __forceinline__ float bldo(Base r, unsigned vo, unsigned so) {
+    const float* q = reinterpret_cast<const float*>(r.p + (vo + so));
+    if constexpr (kNtOwn != 0) return __builtin_nontemporal_load(q);
+    else return *q;
+}
 __device__ __forceinline__ void bst(Base r, unsigned vo, unsigned so, float v) {
-    *reinterpret_cast<float*>(r.p + (vo + so)) = v;
+    float* q = reinterpret_cast<float*>(r.p + (vo + so));
+    if constexpr (kNtStore != 0) __builtin_nontemporal_store(v, q);
+    else *q = v;
 }
 __device__ __forceinline__ int bldi(Base r, unsigned vo, unsigned so) {
     return *reinterpret_cast<const int*>(r.p + (vo + so));
@@ -212,13 +228,19 @@ template <int N>
 __device__ __forceinline__ void bldn(Base r, unsigned vo, unsigned so, float* v) {
     const float* q = reinterpret_cast<const float*>(r.p + (vo + so));
 #pragma unroll
-    for (int c = 0; c < N; ++c) v[c] = q[c];
+    for (int c = 0; c < N; ++c) {
+        if constexpr (kNtOwn != 0) v[c] = __builtin_nontemporal_load(q + c);
+        else v[c] = q[c];
+    }
 }
 template <int N>
 __device__ __forceinline__ void bstn(Base r, unsigned vo, unsigned so, const float* v) {
     float* q = reinterpret_cast<float*>(r.p + (vo + so));
 #pragma unroll
-    for (int c = 0; c < N; ++c) q[c] = v[c];
+    for (int c = 0; c < N; ++c) {
+        if constexpr (kNtOwn != 0) __builtin_nontemporal_store(v[c], q + c);
+        else q[c] = v[c];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -319,10 +341,10 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
         // own stress state (issued before the barriers to overlap latency)
         float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f, stp = 0.f, M = 0.f, mu = 0.f;
         if (active) {
-            sxx = bld(rxx, o_c, so); syy = bld(ryy, o_c, so); szz = bld(rzz, o_c, so);
-            sxy = bld(rxy, o_c, so); sxz = bld(rxz, o_c, so); syz = bld(ryz, o_c, so);
-            stp = bld(rtp, o_c, so);
-            M = bld(rM, o_c, so); mu = bld(rU, o_c, so);
+            sxx = bldo(rxx, o_c, so); syy = bldo(ryy, o_c, so); szz = bldo(rzz, o_c, so);
+            sxy = bldo(rxy, o_c, so); sxz = bldo(rxz, o_c, so); syz = bldo(ryz, o_c, so);
+            stp = bldo(rtp, o_c, so);
+            M = bldo(rM, o_c, so); mu = bldo(rU, o_c, so);
         }
         // C-PML memory variables of this cell, prefetched likewise
         float ps[9];
@@ -537,9 +559,9 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
         }
         float vx = 0.f, vy = 0.f, vz = 0.f, vxp = 0.f, vyp = 0.f, vzp = 0.f, bo = 0.f;
         if (active) {
-            vx = bld(rvx, o_c, so); vy = bld(rvy, o_c, so); vz = bld(rvz, o_c, so);
-            vxp = bld(rpx, o_c, so); vyp = bld(rpy, o_c, so); vzp = bld(rpz, o_c, so);
-            bo = bld(rB, o_c, so);
+            vx = bldo(rvx, o_c, so); vy = bldo(rvy, o_c, so); vz = bldo(rvz, o_c, so);
+            vxp = bldo(rpx, o_c, so); vyp = bldo(rpy, o_c, so); vzp = bldo(rpz, o_c, so);
+            bo = bldo(rB, o_c, so);
         }
         float ps[12];
         unsigned pqx = 0, pqy = 0, pqz = 0;  // slab offsets (< 2^32 floats)

@@ -154,14 +154,29 @@ int build_split(erm_ctx* c) {
 }
 
 template <int R, int TX, int TY, int AX>
-void stress_one(erm_ctx* c, int which, int src_k, int src_col, int it, const Geo& g, dim3 grd, const int* list) {
-    hipLaunchKernelGGL((k_stress<R, TX, TY, AX>), grd, dim3(TX, TY), 0, c->stream, make_wave(c), c->M[which],
-                       c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml, list);
+void stress_one(erm_ctx* c, int which, int src_k, int src_col, int it, const Geo& g, dim3 grd, const int* list,
+                hipStream_t st = nullptr) {
+    hipLaunchKernelGGL((k_stress<R, TX, TY, AX>), grd, dim3(TX, TY), 0, st ? st : c->stream, make_wave(c),
+                       c->M[which], c->MU[which], g, c->coef, src_k, src_col, c->wavelet, c->wl, it, c->pml, list);
 }
 template <int R, int TX, int TY, int MODE, int AX>
-void velocity_one(erm_ctx* c, int which, int it, const Geo& g, dim3 grd, const int* list) {
-    hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, AX>), grd, dim3(TX, TY), 0, c->stream, make_wave(c),
+void velocity_one(erm_ctx* c, int which, int it, const Geo& g, dim3 grd, const int* list, hipStream_t st = nullptr) {
+    hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE, AX>), grd, dim3(TX, TY), 0, st ? st : c->stream, make_wave(c),
                        c->BU[which], g, c->coef, make_excite(c), it, c->pml, list);
+}
+// region launches of one step: the C-PML group on the second stream (when
+// region_streams), concurrent with the plain group on the context's stream;
+// fork before, join after, so the step is ordered like one launch
+hipStream_t region_fork(erm_ctx* c) {
+    if (!c->stream2) return c->stream;
+    (void)hipEventRecord(c->fork_ev, c->stream);
+    (void)hipStreamWaitEvent(c->stream2, c->fork_ev, 0);
+    return c->stream2;
+}
+void region_join(erm_ctx* c) {
+    if (!c->stream2) return;
+    (void)hipEventRecord(c->join_ev, c->stream2);
+    (void)hipStreamWaitEvent(c->stream, c->join_ev, 0);
 }
 
 template <int R, int TX, int TY>
@@ -170,15 +185,10 @@ void stress_launch(erm_ctx* c, int which, long src, int it) {
     Geo g = make_geo(c);
     if (c->cpml && c->region_split && build_split(c) == 0) {
         g.xcd_map = 0;
-        for (int m = 0; m < 8; ++m) {
-            if (!c->ntl[m]) continue;
-            const dim3 grd(c->ntl[m]);
-            const int* L = c->tiles[m];
-            switch (m) {
-                case 0: stress_one<R, TX, TY, 0>(c, which, src_k, src_col, it, g, grd, L); break;
-                default: stress_one<R, TX, TY, 7>(c, which, src_k, src_col, it, g, grd, L); break;
-            }
-        }
+        const hipStream_t s2 = region_fork(c);
+        if (c->ntl[7]) stress_one<R, TX, TY, 7>(c, which, src_k, src_col, it, g, dim3(c->ntl[7]), c->tiles[7], s2);
+        if (c->ntl[0]) stress_one<R, TX, TY, 0>(c, which, src_k, src_col, it, g, dim3(c->ntl[0]), c->tiles[0]);
+        region_join(c);
     } else if (c->cpml) {
         stress_one<R, TX, TY, 7>(c, which, src_k, src_col, it, g, grid_of(g), nullptr);
     } else {
@@ -191,15 +201,10 @@ void velocity_launch(erm_ctx* c, int which, int it) {
     Geo g = make_geo(c);
     if (c->cpml && c->region_split && build_split(c) == 0) {
         g.xcd_map = 0;
-        for (int m = 0; m < 8; ++m) {
-            if (!c->ntl[m]) continue;
-            const dim3 grd(c->ntl[m]);
-            const int* L = c->tiles[m];
-            switch (m) {
-                case 0: velocity_one<R, TX, TY, MODE, 0>(c, which, it, g, grd, L); break;
-                default: velocity_one<R, TX, TY, MODE, 7>(c, which, it, g, grd, L); break;
-            }
-        }
+        const hipStream_t s2 = region_fork(c);
+        if (c->ntl[7]) velocity_one<R, TX, TY, MODE, 7>(c, which, it, g, dim3(c->ntl[7]), c->tiles[7], s2);
+        if (c->ntl[0]) velocity_one<R, TX, TY, MODE, 0>(c, which, it, g, dim3(c->ntl[0]), c->tiles[0]);
+        region_join(c);
     } else if (c->cpml) {
         velocity_one<R, TX, TY, MODE, 7>(c, which, it, g, grid_of(g), nullptr);
     } else {
@@ -401,6 +406,9 @@ void free_all(erm_ctx* c) {
     for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
 }
 
 // download a padded-grid volume into a dense nnx*nny*nnz host array
@@ -642,6 +650,12 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     } while (0)
     if (hipSetDevice(device) != hipSuccess) { set_global_error("hipSetDevice failed"); delete c; return ERM_E_DEVICE; }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { set_global_error("stream create failed"); delete c; return ERM_E_DEVICE; }
+    if (pin->region_streams == 1 &&
+        (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+         hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+         hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess)) {
+        set_global_error("second stream create failed"); free_all(c); delete c; return ERM_E_DEVICE;
+    }
     for (auto& e : c->pass_ev)
         if (hipEventCreate(&e) != hipSuccess) { set_global_error("event create failed"); free_all(c); delete c; return ERM_E_DEVICE; }
     for (auto& p : c->f) CK(dmalloc(c, &p, c->ncell));

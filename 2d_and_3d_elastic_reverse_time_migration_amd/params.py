@@ -38,8 +38,9 @@ _PARAM_FIELDS = (
                                     "vsp", "vsp_precon", "precon", "precon_z1", "precon_z2",
                                     "fd_order", "boundary", "lt_forward"]]
     + [("cpml_r", ctypes.c_float), ("cpml_vmax", ctypes.c_float)]
-    + [(k, ctypes.c_int) for k in ["tile_x", "tile_y", "zchunk", "pitch_align", "xcd_map", "cpml_split"]]
-    + [("reserved", ctypes.c_int * 5)]
+    + [(k, ctypes.c_int) for k in ["tile_x", "tile_y", "zchunk", "pitch_align", "xcd_map", "cpml_split",
+                                   "region_streams"]]
+    + [("reserved", ctypes.c_int * 4)]
 )
 
 

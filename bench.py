@@ -305,6 +305,8 @@ def main():
     ap.add_argument("--imaging", choices=["excitation", "xcorr"], default="excitation",
                     help="shot workload: excitation-amplitude imaging (default), or zero-lag cross-correlation "
                          "with the source wavefield reconstructed backward from boundary strips")
+    ap.add_argument("--region-streams", type=int, choices=[0, 1], default=0,
+                    help="run the plain-tile and C-PML-tile launches of a step concurrently on two streams")
     ap.add_argument("--xcd-map", type=int, choices=[0, 1], default=0,
                     help="XCD-aware tile order (each XCD a contiguous chunk of tiles)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -336,6 +338,7 @@ def main():
     p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
     p.tile_y, p.zchunk = args.tile_y, args.zchunk
     p.xcd_map = args.xcd_map
+    p.region_streams = args.region_streams
     vp, vs, rho = erm.make_model3d(1, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, 2)
     vp_s = smooth_box(vp)
     vs_s = (vp_s / 1.732).astype(np.float32)

@@ -95,7 +95,10 @@ typedef struct erm_params {
     int   xcd_map;      /* 0: plain workgroup order (default, measured faster); 1: XCD-aware tile mapping */
     int   cpml_split;   /* 0 (default): C-PML steps launch the plain kernel on tiles without C-PML cells and
                            the C-PML kernel on the rest; 1: the C-PML kernel on every tile (same results) */
-    int   reserved[5];
+    int   region_streams; /* 1: the two region launches of a step run concurrently on two streams (they
+                             write disjoint cells and neither writes what the other reads; same results);
+                             0 (default, measured faster: 18.8 vs 18.3 Gcell/s): one stream */
+    int   reserved[4];
 } erm_params;
 
 /* field ids for erm_get_field / erm_set_field */
