@@ -310,17 +310,20 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
     return res
 
 
-def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7, test_shape=(256, 96),
+def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7, keep=(0, 1, 2, 4, 6),
+        test_shape=(256, 96),
         test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, epochs=20, batch=32, lr=1e-3,
         channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, warm=5, flexible=True, illum_eps=1e-3,
         log=print):
     """The whole workflow; returns a dict of results."""
     res = {"config": dict(train_models=train_models, train_shape=train_shape, train_shots=train_shots,
-                          train_lt=train_lt, cg_iters=cg_iters, test_shape=test_shape, test_shots=test_shots,
+                          train_lt=train_lt, cg_iters=cg_iters, keep=list(keep), test_shape=test_shape,
+                          test_shots=test_shots,
                           test_lt=test_lt, tol=tol, epochs=epochs, batch=batch, lr=lr, channels=channels,
                           illum_eps=illum_eps)}
     t0 = time.perf_counter()
-    pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, workers=workers, seed0=seed0,
+    pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, keep=keep, workers=workers,
+                     seed0=seed0,
                      illum_eps=illum_eps, log=log)
     res["datagen_s"] = time.perf_counter() - t0
     net, st = train(pairs, channels, epochs, batch, lr, seed=seed0, log=log)
@@ -342,11 +345,17 @@ def main(argv=None):
     ap.add_argument("--warm", type=int, nargs="*", default=[5],
                     help="learned warm-start lengths to compare (net iterations before the net-free CG)")
     ap.add_argument("--no-flexible", action="store_true", help="skip the every-iteration learned CG run")
+    ap.add_argument("--cg-iters", type=int, default=7, help="CG iterations per training model")
+    ap.add_argument("--keep", type=int, nargs="*", default=[0, 1, 2, 4, 6],
+                    help="CG iterations whose (gradient, remaining update) pairs are kept for training")
     ap.add_argument("--illum-eps", type=float, default=1e-3,
                     help="illumination diagonal w = 1/(W + eps max W) composed with the net (train and test)")
     ap.add_argument("--no-illum", action="store_true", help="no diagonal: the net alone on plain-CG gradients")
     a = ap.parse_args(argv)
-    res = run(train_models=a.train_models, epochs=a.epochs, test_iters=a.test_iters, workers=a.workers,
+    if any(k >= a.cg_iters or k < 0 for k in a.keep):
+        ap.error("--keep iterations must lie in [0, --cg-iters)")
+    res = run(train_models=a.train_models, cg_iters=a.cg_iters, keep=a.keep, epochs=a.epochs,
+              test_iters=a.test_iters, workers=a.workers,
               warm=a.warm, flexible=not a.no_flexible, illum_eps=None if a.no_illum else a.illum_eps,
               log=lambda s: print(s, flush=True))
     txt = json.dumps(res)
