@@ -405,7 +405,7 @@ def main():
         dom = "k_velocity" if acc["vel_ms"] >= acc["stress_ms"] else "k_stress"
         ach, per_launch, avg = (ve_ach, ve_bytes, ve_avg) if dom == "k_velocity" else (st_ach, sb * cells, st_avg)
         traffic = None
-        tf = os.path.join(ROOT, "profiles", "traffic_cfg2.json")
+        tf = os.path.join(ROOT, "profiles", "traffic_k_stress_k_velocity_296x296x168_cpml.json")
         if os.path.exists(tf):
             with open(tf) as f:
                 traffic = json.load(f).get(dom, {}).get("hbm_bytes_per_launch")
