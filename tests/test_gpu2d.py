@@ -189,3 +189,22 @@ def test_gpu2d_angle_gathers_match_oracle(erm, oracle2d_mod):
     np.testing.assert_array_equal(gg[0].sum(axis=0)[inr], pp[inr])
     for a, b in ((gg[2], og[2]), (gg[3], og[3])):
         assert cases.rel_l2(a[mask], b[mask]) < 1e-3, cases.rel_l2(a[mask], b[mask])
+
+
+def test_gpu2d_new_entry_points_reject_bad_input(erm):
+    """erm2d_angle_gathers rejects empty or over-wide bins, erm2d_illumination
+    needs the background model and a shot list."""
+    p = cases2d.params(nx=40, nz=30, b=10, R=4, L=0, lt=60, freq=25.0, src=(20, 2))
+    with erm.Elastic2D(p) as g:
+        with pytest.raises(erm.ErmError):
+            g.illumination([(10, 2)])               # no background model
+        vp, vs, rho, _, _ = cases2d.model(p, seed=2, q=False)
+        g.load_model(1, vp, vs, rho)
+        with pytest.raises(erm.ErmError):
+            g.illumination([])                      # no shots
+        with pytest.raises(erm.ErmError):
+            g.angle_gathers(0, 3.0)
+        with pytest.raises(erm.ErmError):
+            g.angle_gathers(90, 3.0)                # 270 degrees > 180
+        with pytest.raises(erm.ErmError):
+            g.angle_gathers(10, -1.0)
