@@ -426,7 +426,7 @@ int erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* g
     double best = 1e300;
     int bt = ty0, bk = kc0, bg = gr0;
     const int tys[2] = {4, 8};
-    const int kcs[3] = {4, 8, 16};
+    const int kcs[4] = {4, 6, 8, 16};
     for (int ty : tys)
         for (int kc : kcs) {
             // resident workgroups: 16 waves per CU at 4 waves/SIMD

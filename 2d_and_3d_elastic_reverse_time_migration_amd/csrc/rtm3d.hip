@@ -1185,7 +1185,7 @@ int erm_autotune(erm_ctx* c, int steps, int* tile_y, int* zchunk, double* ms_per
     double best = 1e300;
     int bt = c->ty, bk = c->kchunk;
     const int tys[2] = {4, 8};
-    const int kcs[4] = {4, 8, 16, 32};
+    const int kcs[8] = {4, 5, 6, 7, 8, 12, 16, 32};  // z sweep: parallelism vs re-read z halos
     int rc = 0;
     for (int ty : tys)
         for (int kc : kcs) {

@@ -229,7 +229,7 @@ int   erm_reconstruct(erm_ctx* ctx, int it_end);
 int   erm_get_reconstructed_field(erm_ctx* ctx, int field, float* dst);
 /* Launch-shape autotuner (no reference counterpart; the reference launches
  * one block per z plane): times `steps` source-pass steps (smoothed model)
- * for tile heights {4, 8} x z sweep depths {4, 8, 16, 32} and keeps the
+ * for tile heights {4, 8} x z sweep depths {4, 5, 6, 7, 8, 12, 16, 32} and keeps the
  * fastest for later passes.  Results do not depend on the shape.  Zeroes the
  * wavefield and the excitation state afterwards. */
 int   erm_autotune(erm_ctx* ctx, int steps, int* tile_y, int* zchunk, double* ms_per_step);

@@ -107,7 +107,7 @@ def test_graph_persistent_and_autotune_identical(erm):
         g1 = b.run()
         best = b.autotune(steps=2)
         g2 = b.run()
-    assert best["tile_y"] in (4, 8) and best["zchunk"] in (4, 8, 16) and best["ms_per_step"] > 0
+    assert best["tile_y"] in (4, 8) and best["zchunk"] in (4, 6, 8, 16) and best["ms_per_step"] > 0
     for q in range(3):
         np.testing.assert_array_equal(g1[q], ref[q])
         np.testing.assert_array_equal(g2[q], ref[q])
