@@ -20,6 +20,15 @@
  * tau_sig_l = 1/(2 pi f_l), f_l = freq * {1/4, 1, 4}, tau fixed by Q(2 pi freq) = Q.
  * The adjoint is the exact transpose of the discrete Born operator (pinned
  * by the dot-product test in tests/test_oracle2d.py).
+ * The imaging step also keeps the receiver vector P of each cell at its
+ * imaging step (r_vxp, r_vzp) for the angle gathers that oracle2d.py restates
+ * (pinned by a flat reflector); the source illumination is composed in
+ * oracle2d.py from this file's steps (pinned by additivity over shots and its
+ * dependence on the background model only).  How each piece is pinned:
+ * closed-form derivatives, SLS Q at the reference frequency and over the band,
+ * the Q -> infinity limit, attenuation, Born vs finite differences, the
+ * adjoint dot product, a flat reflector imaged at its depth, Q-compensation
+ * amplitude restoration and 1/Q scaling (tests/test_oracle2d.py).
  */
 #include "../include/erm2d.h"
 
