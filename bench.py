@@ -195,6 +195,48 @@ def visco_model(nx, nz, seed):
     return bvp, bvs, brho, qp, qs, dm
 
 
+def run_learned(erm, args):
+    """LSRTM seconds to a data-misfit target, learned preconditioner vs plain
+    CG (the learned workflow of learned.py: training pairs, U-Net training on
+    the GPU, held-out comparison).  Full: 160 training models 128x96, held-out
+    256x96 with 12 shots, misfit 1e-2 (about 15 minutes).  --quick: 48 models
+    64x48, held-out 128x48 with 8 shots, misfit 0.03 (about 90 s).  One step
+    = one whole workflow; value = learned seconds / plain seconds."""
+    learned = importlib.import_module(PKG + ".learned")
+    if args.quick:
+        kw = dict(train_models=48, train_shape=(64, 48), train_shots=4, train_lt=700, cg_iters=7,
+                  test_shape=(128, 48), test_shots=8, test_lt=700, test_iters=200, tol=0.03, epochs=30,
+                  batch=16, warm=3, flexible=False)
+    else:
+        kw = dict(warm=5, flexible=False)
+    t0 = time.perf_counter()
+    res = learned.run(log=log, **kw)
+    el = time.perf_counter() - t0
+    p, d = res["plain"], res["illumination"]
+    w = res["warm_start"]
+    ratio = w["s_to_tol"] / p["s_to_tol"] if (w["s_to_tol"] and p["s_to_tol"]) else None
+    line = {
+        "metric": "LSRTM seconds to data misfit %g, learned vs plain CG" % res["config"]["tol"],
+        "value": None if ratio is None else round(ratio, 4),
+        "unit": "x plain CG time",
+        "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(el * 1e3, 1),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 operators, bf16 U-Net (fp32 accumulate)",
+        "data": "synthetic (seeded random layered-plus-anomaly models; Born data)",
+        "config": {"workload": "learned-preconditioner LSRTM (2D elastic, dvp/vp and dvs/vs), %s"
+                               % ("reduced test case" if args.quick else "scaled held-out case"),
+                   **{k: (list(v) if isinstance(v, tuple) else v) for k, v in res["config"].items()}},
+        "plain": {"iters": p["iters_to_tol"], "s": p["s_to_tol"]},
+        "illumination_diagonal": {"iters": d["iters_to_tol"], "s": d["s_to_tol"]},
+        "learned_warm_start": {"learned_iters": w["warm_iters"], "iters": w["iters_to_tol"], "s": w["s_to_tol"]},
+        "model_err_vs_true": res["model_err_vs_true"],
+        "unet_train_mfma_tflops": res.get("train_mfma_tflops"),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_visco(erm, args):
     """2D viscoelastic LSRTM with plain CG: 600x300 model, d = 8 m, 3 SLS
     mechanisms, 60 shots, 600 receivers, 20 Hz, nt = 3000 (dt 0.8 ms), 40-cell
@@ -315,9 +357,11 @@ def main():
     ap.add_argument("--iters", type=int, default=20, help="visco workload: CG iterations")
     ap.add_argument("--workers", type=int, default=4, help="visco workload: concurrent shot contexts")
     ap.add_argument("--oracle-check", action="store_true", help="visco workload: FP64 oracle CG check (1 shot)")
-    ap.add_argument("--workload", choices=["shot", "batch", "visco"], default="shot",
+    ap.add_argument("--workload", choices=["shot", "batch", "visco", "learned"], default="shot",
                     help="shot: the one-shot 3D configuration (default); batch: 24 windowed shots over a "
-                         "600x600x250 model in one batched, graph-captured launch per kernel per step")
+                         "600x600x250 model in one batched, graph-captured launch per kernel per step; "
+                         "learned: LSRTM time to misfit, learned preconditioner vs plain CG")
+    ap.add_argument("--quick", action="store_true", help="learned workload: the reduced test case")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the whole grid fits one MI355X); --gpus {args.gpus} is not "
@@ -334,6 +378,8 @@ def main():
         return run_batch(erm, args)
     if args.workload == "visco":
         return run_visco(erm, args)
+    if args.workload == "learned":
+        return run_learned(erm, args)
     nt = args.nt
     p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
     p.tile_y, p.zchunk = args.tile_y, args.zchunk
