@@ -130,3 +130,30 @@ def test_model_generator_is_seeded_and_consistent(erm):
     assert abs(vp0[0, 0, 58] - 3500.0) < 1.0
     # +300 m/s Gaussian anomaly at half width, 0.55 depth, on the 2800 m/s layer
     assert abs(vp0[100, 2, 33] - 3100.0) < 1e-3      # the anomaly centre (1000, 20, 330) m
+
+
+def test_headers_are_c99_and_link(erm, tmp_path):
+    """The public headers are plain C: a C99 translation unit that includes all
+    of them and takes the address of every declared function compiles with
+    -pedantic and links against liberm.so (the binding a C, cgo or FFI caller
+    makes)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    names = header_functions()
+    src = tmp_path / "abi.c"
+    src.write_text(
+        "#include \"erm.h\"\n#include \"erm2d.h\"\n#include \"erm_unet.h\"\n#include \"erm_batch.h\"\n"
+        "#include <stdio.h>\n"
+        "typedef void (*fn)(void);\n"
+        "int main(void) {\n    fn f[] = {\n"
+        + "".join(f"        (fn)&{n},\n" for n in names)
+        + "    };\n    printf(\"%d\\n\", (int)(sizeof f / sizeof f[0]));\n    return 0;\n}\n")
+    libdir = os.path.join(ROOT, "2d_and_3d_elastic_reverse_time_migration_amd")
+    exe = tmp_path / "abi"
+    r = subprocess.run([cc, "-std=c99", "-Wall", "-pedantic", "-Werror", "-I", os.path.join(ROOT, "include"),
+                        str(src), "-o", str(exe), "-L", libdir, "-lerm", "-Wl,-rpath," + libdir],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
