@@ -117,6 +117,9 @@ int build_tables(erm_batch* b) {
         erm_ctx* c = b->shots[s];
         c->ty = b->ty;
         c->kchunk = std::max(1, std::min(b->kchunk, c->nnz - 2 * c->R));
+        if (!sweep_fits(c, c->kchunk))
+            return b->fail(ERM_E_PARAM, "zchunk %d too large for shot %d's %ld-cell planes (at most %d: 32-bit offsets)",
+                           c->kchunk, s, c->pxy, max_sweep(c));
         c->xcd_map = 0;
         const Geo g = make_geo(c);
         for (int which = 0; which < 2; ++which) {
@@ -432,6 +435,9 @@ int erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* g
             // resident workgroups: 16 waves per CU at 4 waves/SIMD
             const int per_cu = std::max(1, 16 / ty);
             const int grids[3] = {0, b->cus * per_cu, 2 * b->cus * per_cu};
+            bool fits = true;
+            for (erm_ctx* c : b->shots) fits = fits && sweep_fits(c, std::max(1, std::min(kc, c->nnz - 2 * c->R)));
+            if (!fits) continue;
             for (int gr : grids) {
                 b->ty = ty; b->kchunk = kc; b->grid = gr;
                 b->tables_dirty = true;
@@ -446,7 +452,15 @@ int erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* g
         }
     b->ty = bt; b->kchunk = bk; b->grid = bg;
     b->tables_dirty = true;
-    int rc = zero_shots(b, true, true, false);
+    int rc;
+    if (best == 1e300) {  // no candidate fits the 32-bit span: time the current shape
+        float ms = 0;
+        if ((rc = zero_shots(b, true, true, false)) || (rc = run_steps(b, ERM_PASS_SOURCE, 0, 1, false, &ms)) ||
+            (rc = run_steps(b, ERM_PASS_SOURCE, 1, steps, false, &ms)))
+            return rc;
+        best = ms / steps;
+    }
+    rc = zero_shots(b, true, true, false);
     if (rc) return rc;
     if (tile_y) *tile_y = bt;
     if (zchunk) *zchunk = bk;

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <string>
 #include <vector>
@@ -63,7 +64,7 @@ struct erm_ctx {
     float* fr[13] = {nullptr};       // reconstructed source wavefield
     int* strip_idx = nullptr;        // padded indices of the face slabs around the model region
     int n_strip = 0;
-    float* strips = nullptr;         // [steps][10][n_strip]: sigma+tp (7) after the stress update, v (3) after the velocity update
+    float* strips = nullptr;         // [steps+1][13][n_strip]: sigma+tp (7) after the stress update, v+vP (6) after the velocity update
     int strip_steps = 0;
     bool have_strips = false;
     float* wavelet_neg = nullptr;    // -wavelet, for the time-reversed source
@@ -94,6 +95,18 @@ struct erm_ctx {
             return (ctx)->fail(ERM_E_DEVICE, "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
                                __FILE__, __LINE__);                                              \
     } while (0)
+
+// The 3D kernels address every array through 32-bit byte offsets from the
+// first plane a workgroup reads: its z sweep of kchunk planes plus R halo
+// planes on either side must span less than 4 GiB.
+inline bool sweep_fits(const erm_ctx* c, int kchunk) {
+    return (unsigned long long)(kchunk + 2 * c->R) * (unsigned long long)c->pxy * 4ull < (1ull << 32);
+}
+// the largest z sweep that fits (0: not even one plane does)
+inline int max_sweep(const erm_ctx* c) {
+    const long long k = (long long)(((1ull << 32) - 1) / (4ull * (unsigned long long)c->pxy)) - 2 * c->R;
+    return k < 1 ? 0 : (int)std::min<long long>(k, 1 << 30);
+}
 
 inline Geo make_geo(const erm_ctx* c) {
     Geo g;

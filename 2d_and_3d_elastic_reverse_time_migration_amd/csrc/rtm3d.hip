@@ -637,6 +637,22 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
         int kc = (nzi + chunks - 1) / chunks;
         kc = std::max(kc, std::min(nzi, 8));
         c->kchunk = pin->zchunk > 0 ? std::min(pin->zchunk, nzi) : kc;
+        const int kmax = max_sweep(c);
+        if (kmax == 0) {
+            set_global_error("x-y plane of %ld cells too large: %d planes exceed the 4 GiB span of 32-bit offsets",
+                             c->pxy, 2 * R + 1);
+            delete c;
+            return ERM_E_SIZE;
+        }
+        if (c->kchunk > kmax) {
+            if (pin->zchunk > 0) {
+                set_global_error("zchunk %d too large for %ld-cell planes (at most %d: 32-bit offsets)", pin->zchunk,
+                                 c->pxy, kmax);
+                delete c;
+                return ERM_E_PARAM;
+            }
+            c->kchunk = kmax;
+        }
     }
 #define CK(x)                         \
     do {                              \
@@ -1190,6 +1206,7 @@ int erm_autotune(erm_ctx* c, int steps, int* tile_y, int* zchunk, double* ms_per
     for (int ty : tys)
         for (int kc : kcs) {
             if (kc > nzi && kc != kcs[0]) continue;
+            if (!sweep_fits(c, std::min(kc, nzi))) continue;
             c->ty = ty;
             c->kchunk = std::min(kc, nzi);
             if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) break;
@@ -1198,6 +1215,13 @@ int erm_autotune(erm_ctx* c, int steps, int* tile_y, int* zchunk, double* ms_per
             const double per = c->last_pass_ms / steps;
             if (per < best) { best = per; bt = c->ty; bk = c->kchunk; }
         }
+    if (!rc && best == 1e300) {  // no candidate fits the 32-bit span: time the current shape
+        c->ty = bt;
+        c->kchunk = bk;
+        if (!(rc = erm_zero_wavefield(c)) && !(rc = erm_zero_excitation(c)) && !(rc = run_pass(c, ERM_PASS_SOURCE, 0, 1)) &&
+            !(rc = run_pass(c, ERM_PASS_SOURCE, 1, steps)))
+            best = c->last_pass_ms / steps;
+    }
     c->ty = bt;
     c->kchunk = bk;
     c->profiling = prof;

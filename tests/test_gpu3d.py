@@ -221,6 +221,17 @@ def test_gpu_error_paths(erm):
     with pytest.raises(erm.ErmError):
         g.set_field("ex_time", np.zeros(g.shape, np.float32))
     g.close()
+    # one z sweep plus its 2R halo planes must span < 4 GiB (32-bit offsets);
+    # both cases are rejected before anything is allocated
+    big = cases.params(nx=4000, ny=4000, nz=80, b=0, R=4, lt=10)
+    big.zchunk = 60                        # 64 MB planes: at most 4 GiB / 64 MB - 8 = 59
+    with pytest.raises(erm.ErmError) as ei:
+        erm.RTM3D(big)
+    assert ei.value.code == erm.E_PARAM and "zchunk" in str(ei.value)
+    huge = cases.params(nx=11000, ny=11000, nz=9, b=0, R=4, lt=10)  # 9 planes of 484 MB > 4 GiB
+    with pytest.raises(erm.ErmError) as ei:
+        erm.RTM3D(huge)
+    assert ei.value.code == erm.E_SIZE
 
 
 def test_gpu_write_results_format(erm, tmp_path):
