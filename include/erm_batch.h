@@ -49,7 +49,9 @@ int   erm_batch_load_model(erm_batch* b, int which, const float* vp, const float
 /* (Re)define the shots; windows must lie inside the model and contain their source. */
 int   erm_batch_set_shots(erm_batch* b, const erm_shot_window* shots, int nshots);
 /* Launch shape: tile_y 4 or 8, zchunk >= 1, grid_blocks 0 = one workgroup per
- * table entry, > 0 = persistent grid of that many workgroups. */
+ * table entry, > 0 = persistent grid of that many workgroups.  A zchunk whose
+ * sweep (zchunk + 2R planes of a window) spans 4 GiB or more makes the next
+ * pass fail with ERM_E_PARAM (32-bit kernel offsets). */
 int   erm_batch_set_launch(erm_batch* b, int tile_y, int zchunk, int grid_blocks);
 /* Measure candidate launch shapes on `steps` source-pass steps of the current
  * shots and keep the fastest; returns it (any pointer may be NULL). */
