@@ -138,6 +138,7 @@ int build_split(erm_ctx* c) {
         if (m != 0) m = 7;
         grp[m].push_back(l);
     }
+    c->split_built = false;  // until every list below is in place
     for (int m = 0; m < 8; ++m) {
         if (c->tiles[m]) (void)hipFree(c->tiles[m]);
         c->tiles[m] = nullptr;

@@ -398,12 +398,17 @@ void shapes(const erm_unet* u, Shape* s) {
 }
 
 int ensure(erm_unet* u, int n, int h, int w) {
-    if (h % 8 || w % 8 || n <= 0) { g_uerr = "h and w must be multiples of 8 and n > 0"; return ERM_E_PARAM; }
+    if (h <= 0 || w <= 0 || h % 8 || w % 8 || n <= 0) {
+        g_uerr = "h and w must be positive multiples of 8 and n > 0";
+        return ERM_E_PARAM;
+    }
     if (u->bn == n && u->bh == h && u->bw == w) return 0;
     for (auto* a : u->act) if (a) (void)hipFree(a);
     for (auto* a : u->grad) if (a) (void)hipFree(a);
     if (u->xin) (void)hipFree(u->xin);
     if (u->tgt) (void)hipFree(u->tgt);
+    u->xin = u->tgt = nullptr;
+    u->bn = u->bh = u->bw = 0;  // no valid buffers until every allocation below succeeds
     u->act.assign(NBUF, nullptr);
     u->grad.assign(NBUF, nullptr);
     u->act_size.assign(NBUF, 0);
