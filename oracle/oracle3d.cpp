@@ -55,7 +55,7 @@ namespace {
 /* Reference radius-6 optimized first-derivative coefficients, zzzzz:205-210. */
 const double kCoeR6[7] = {0.0, 1.2213363647, -0.096931457519, 0.017447662353,
                           -0.0029672895159, 0.00035900539822, -0.00002184781161};
-/* Radius-4 staggered Taylor coefficients (exact for polynomials of degree <= 7). */
+/* Radius-4 staggered Taylor coefficients (exact for polynomials of degree <= 8). */
 const double kCoeR4[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0};
 
 /* make_ricker_new, zzzzz:136-157: dt in seconds, PI of cwp.h. */
