@@ -364,8 +364,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="learned workload: the reduced test case")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (the whole grid fits one MI355X); --gpus {args.gpus} is not "
-              f"supported in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (the whole grid fits one MI355X; shots would shard across GPUs, "
+              f"one process each); --gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
 
     erm = importlib.import_module(PKG)
