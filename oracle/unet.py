@@ -15,7 +15,11 @@ layers in forward order (15 layers).
 
 Pinning: the 3x3 convolution is checked against direct loops, the whole
 backward against central finite differences in FP64 and against
-torch.autograd in float64 (tests/test_oracle_unet.py).
+torch.autograd in float64 (tests/test_oracle_unet.py).  The optimizer
+(`adam`, Kingma & Ba 2015 Algorithm 1 with the constants of
+include/erm_unet.h) is pinned by two closed forms: a constant gradient moves
+every parameter by lr*sign(g) per step, and g followed by -g moves it by
+-lr*sign(g) and then +lr*sign(g)/19.
 
 `bf16=True` rounds every convolution operand (weights, inputs, output
 gradients) to bfloat16 with round-to-nearest-even, the rounding the MFMA
@@ -181,3 +185,18 @@ def init_params(channels, seed):
         sd = np.sqrt(2.0 / (ci * 9.0)) * (0.01 if li == len(specs) - 1 else 1.0)
         parts += [rng.normal(0, sd, ci * co * 9), np.zeros(co)]
     return np.concatenate(parts).astype(np.float32)
+
+
+def adam(p0, grads, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """Parameters after one Adam step per gradient in `grads`, starting from
+    p0 with zero moments (Kingma & Ba 2015, Algorithm 1, bias-corrected
+    moments; the constants of erm_unet_train_step), in FP64."""
+    p = np.asarray(p0, np.float64).copy()
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    for t, g in enumerate(grads, 1):
+        g = np.asarray(g, np.float64)
+        m = beta1 * m + (1.0 - beta1) * g
+        v = beta2 * v + (1.0 - beta2) * g * g
+        p -= lr * (m / (1.0 - beta1 ** t)) / (np.sqrt(v / (1.0 - beta2 ** t)) + eps)
+    return p

@@ -139,3 +139,24 @@ def test_param_count_matches_layout():
     n = U.num_params(c)
     assert n == len(U.init_params(c, 0))
     assert len(U.pack([(w, b) for w, b in U.unpack(np.zeros(n), c)])) == n
+
+
+def test_adam_closed_forms():
+    """Bias-corrected Adam: with a constant gradient the corrected moments are
+    g and g^2 at every step, so each step moves a parameter by
+    lr*g/(|g| + eps) (an uncorrected first step would move it by
+    0.1/sqrt(0.001) = 3.16 times that); with g then -g the second corrected
+    first moment is (0.09 - 0.1) g / 0.19 = -g/19 and the second moment g^2,
+    so the second step moves it back by lr*g/(19 (|g| + eps)) (a wrong beta1
+    changes the 19, a wrong beta2 the second moment)."""
+    rng = np.random.default_rng(0)
+    p0 = rng.standard_normal(64)
+    g = rng.standard_normal(64)
+    lr, eps = 1e-3, 1e-8
+    step = lr * g / (np.abs(g) + eps)
+    for steps in (1, 2, 5):
+        p = U.adam(p0, [g] * steps, lr)
+        np.testing.assert_allclose(p, p0 - steps * step, rtol=0, atol=1e-14)
+    p = U.adam(p0, [g, -g], lr)
+    np.testing.assert_allclose(p, p0 - step + step / 19.0, rtol=0, atol=1e-14)
+    assert np.array_equal(U.adam(p0, [np.zeros(64)], lr), p0)
