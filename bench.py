@@ -451,8 +451,11 @@ def main():
         dom = "k_velocity" if acc["vel_ms"] >= acc["stress_ms"] else "k_stress"
         ach, per_launch, avg = (ve_ach, ve_bytes, ve_avg) if dom == "k_velocity" else (st_ach, sb * cells, st_avg)
         traffic = None
+        # the committed PMC traffic figures cover the default workload only
+        # (296x296x168 grid, C-PML, excitation imaging)
         tf = os.path.join(ROOT, "profiles", "traffic_k_stress_k_velocity_296x296x168_cpml.json")
-        if os.path.exists(tf):
+        profiled = args.boundary == "cpml" and tuple(g.shape) == (168, 296, 296) and not xc
+        if profiled and os.path.exists(tf):
             with open(tf) as f:
                 traffic = json.load(f).get(dom, {}).get("hbm_bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
