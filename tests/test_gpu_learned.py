@@ -12,7 +12,6 @@ reported, so the share of the net in the gain is visible.
 """
 import importlib
 
-import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu

@@ -3,7 +3,6 @@ diagonal preconditioner erm2d_precond_diag through the C ABI."""
 import ctypes
 
 import numpy as np
-import pytest
 
 
 def test_precond_diag_scales_every_plane(erm):
