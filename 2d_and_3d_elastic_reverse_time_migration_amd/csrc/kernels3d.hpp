@@ -549,12 +549,12 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
         if (xh) {
             hxx = bld(rxx, o_x, so); hxy_ = bld(rxy, o_x, so); hxz = bld(rxz, o_x, so); htp = bld(rtp, o_x, so);
         }
-        float gyy[NYH], gxy[NYH], gyz[NYH], gtp[NYH];
+        float yh_yy[NYH], yh_xy[NYH], yh_yz[NYH], yh_tp[NYH];  // y-halo rows of this plane
 #pragma unroll
         for (int t = 0; t < NYH; ++t) {
             if (ty + t * TY < 2 * R) {
-                gyy[t] = bld(ryy, o_y[t], so); gxy[t] = bld(rxy, o_y[t], so);
-                gyz[t] = bld(ryz, o_y[t], so); gtp[t] = bld(rtp, o_y[t], so);
+                yh_yy[t] = bld(ryy, o_y[t], so); yh_xy[t] = bld(rxy, o_y[t], so);
+                yh_yz[t] = bld(ryz, o_y[t], so); yh_tp[t] = bld(rtp, o_y[t], so);
             }
         }
         float vx = 0.f, vy = 0.f, vz = 0.f, vxp = 0.f, vyp = 0.f, vzp = 0.f, bo = 0.f;
@@ -595,7 +595,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
         for (int t = 0; t < NYH; ++t) {
             if (ty + t * TY < 2 * R) {
                 const int lr = l_y[t];
-                s_yy[lr][tx] = gyy[t]; s_xy[lr][tx + R] = gxy[t]; s_yz[lr][tx] = gyz[t]; s_tp[lr][tx + R] = gtp[t];
+                s_yy[lr][tx] = yh_yy[t]; s_xy[lr][tx + R] = yh_xy[t]; s_yz[lr][tx] = yh_yz[t]; s_tp[lr][tx + R] = yh_tp[t];
             }
         }
         __syncthreads();
@@ -677,9 +677,9 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
                     const float ex_ = ex_array(ex, EX_VXP)[n], ey_ = ex_array(ex, EX_VYP)[n], ez_ = ex_array(ex, EX_VZP)[n];
                     const float den = ex_ * ex_ + ey_ * ey_ + ez_ * ez_;
                     const float pp = ex_ * nxp + ey_ * nyp + ez_ * nzp;
-                    const float ps = ex_ * (nvx - nxp) + ey_ * (nvy - nyp) + ez_ * (nvz - nzp);
+                    const float ips = ex_ * (nvx - nxp) + ey_ * (nvy - nyp) + ez_ * (nvz - nzp);
                     ex_array(ex, EX_IMG_PP)[n] = den > 0.f ? pp / den : 0.f;
-                    ex_array(ex, EX_IMG_PS)[n] = den > 0.f ? ps / den : 0.f;
+                    ex_array(ex, EX_IMG_PS)[n] = den > 0.f ? ips / den : 0.f;
                 }
             }
         }
