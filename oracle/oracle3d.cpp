@@ -32,7 +32,8 @@
  * exactness, exact decay under uniform damping, x/y mirror symmetry of an
  * explosive source, y<->x transposition symmetry, P arrival times against
  * r/vp, S-energy leakage of the P/S split, excitation time against the
- * travel time, and the FP32 instantiation against the FP64 one; the
+ * travel time, conservation of the discrete leapfrog energy without damping
+ * or source, and the FP32 instantiation against the FP64 one; the
  * cross-correlation shot (oracle3d_xcorr_shot, full source history in memory)
  * by a flat reflector imaged at its depth.  The direct-wave mute is restated
  * in oracle3d.py (mute_direct) and pinned there.

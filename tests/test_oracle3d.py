@@ -6,7 +6,8 @@ coefficient identities, single-step closed forms on polynomial fields (which
 pin each of the 21 staggered derivatives' direction, the dt/dx scaling and
 the roles of M, lambda, mu and buoyancy), exact damping decay, mirror and
 transposition symmetries, plane-wave speeds, exact P/S separation of a
-transverse wave, excitation time against travel time, and FP32 vs FP64.
+transverse wave, excitation time against travel time, conservation of the
+discrete leapfrog energy, and FP32 vs FP64.
 A plausible mistake (a D+ where the reference has D-, a swapped lambda/M,
 a dropped buoyancy, a transposed y stencil) fails at least one of them.
 """
@@ -445,3 +446,63 @@ def test_direct_wave_mute_window(oracle):
         assert not np.any(m[:, :cut[r], 0, r])
     q = cases.params(nx=24, ny=20, nz=18, b=8, R=4, lt=160, dt=1.0, freq=25.0, src=(12, 10, 2))
     np.testing.assert_array_equal(oracle.mute_direct(q, rec[:, :, :1, :1], vp), rec[:, :, :1, :1])  # off by default
+
+
+def _leapfrog_energy(v, s_new, s_old, rho, lam, mu):
+    """E^n = 1/2 sum rho |v^n|^2 + 1/2 sum sigma^{n+1/2} . S sigma^{n-1/2},
+    S = C^-1 of the isotropic stiffness acting on (exx, eyy, ezz) and on
+    engineering shear strains: normal block (1/2mu)(I - lam/(2mu+3lam) 11^T),
+    shear 1/mu."""
+    tr_new = s_new[0] + s_new[1] + s_new[2]
+    tr_old = s_old[0] + s_old[1] + s_old[2]
+    normal = (s_new[0] * s_old[0] + s_new[1] * s_old[1] + s_new[2] * s_old[2]
+              - lam / (2.0 * mu + 3.0 * lam) * tr_new * tr_old) / (2.0 * mu)
+    shear = (s_new[3] * s_old[3] + s_new[4] * s_old[4] + s_new[5] * s_old[5]) / mu
+    kinetic = rho * (v[0] ** 2 + v[1] ** 2 + v[2] ** 2)
+    return 0.5 * float(np.sum(kinetic)) + 0.5 * float(np.sum(normal + shear))
+
+
+@pytest.mark.parametrize("R", [4, 6])
+def test_discrete_energy_is_conserved(oracle, R):
+    """Without damping or source the staggered leapfrog conserves the discrete
+    energy above to roundoff: on the updated cells (the outer R cells stay
+    zero) every velocity-side difference is minus the transpose of the
+    matching stress-side one, and the stress update is C times the strain
+    increment with the same per-cell M, lambda, mu, while the velocity update
+    divides by the same per-cell density.  A D+ where a D- belongs, a swapped
+    lambda/M, a buoyancy or modulus from the wrong cell, or a different dt/dx
+    on the two sides of a derivative pair breaks it.  Heterogeneous model,
+    random initial velocity, 30 steps.  The same functional with lambda and M
+    swapped must drift, so the check is not satisfied trivially."""
+    p = cases.params(nx=24, ny=22, nz=20, b=0, R=R, dt=1.0, d=10.0, lt=10, src=(12, 11, 8))
+    vp, vs, rho = cases.layered(p, seed=3)
+    o = oracle.Oracle3D(p)
+    o.load_model(1, vp, vs, rho)
+    shape = tuple(p.padded_shape)
+    rng = np.random.default_rng(1)
+    v0 = []
+    for name in ("vx", "vy", "vz"):
+        a = np.zeros(shape)
+        a[R:-R, R:-R, R:-R] = rng.standard_normal((shape[0] - 2 * R, shape[1] - 2 * R, shape[2] - 2 * R))
+        o.set_field(name, a)
+        v0.append(a)
+    # the model as the oracle holds it: float32 inputs, field layout (z, y, x)
+    f64 = lambda a: a.astype(np.float64).transpose(2, 1, 0)
+    r, M, mu = f64(rho), f64(rho) * f64(vp) ** 2, f64(rho) * f64(vs) ** 2
+    sig_names = ("txx", "tyy", "tzz", "txy", "txz", "tyz")
+    zero = [np.zeros(shape)] * 6
+    v_prev, s_prev = v0, zero
+    E, E_swapped = [], []
+    for _ in range(30):
+        o.step(PASS_SOURCE, NO_SOURCE_IT, 1)  # state: sigma^{n+1/2}, v^{n+1}
+        s_new = [o.get_field(k) for k in sig_names]
+        E.append(_leapfrog_energy(v_prev, s_new, s_prev, r, M - 2.0 * mu, mu))
+        E_swapped.append(_leapfrog_energy(v_prev, s_new, s_prev, r, M, mu))  # lambda := M
+        v_prev = [o.get_field(k) for k in ("vx", "vy", "vz")]
+        s_prev = s_new
+    E = np.array(E)
+    drift = np.abs(E - E[0]).max() / E[0]
+    print(f"R={R}: energy {E[0]:.6e}, max relative drift over 30 steps {drift:.2e}")
+    assert drift < 1e-10
+    Es = np.array(E_swapped)
+    assert np.abs(Es - Es[0]).max() / abs(Es[0]) > 1e-3
