@@ -26,7 +26,8 @@
  * oracle2d.py from this file's steps (pinned by additivity over shots and its
  * dependence on the background model only).  How each piece is pinned:
  * closed-form derivatives, SLS Q at the reference frequency and over the band,
- * the Q -> infinity limit, attenuation, Born vs finite differences, the
+ * the Q -> infinity limit, conservation of the elastic leapfrog's discrete
+ * energy without damping or source, attenuation, Born vs finite differences, the
  * adjoint dot product, a flat reflector imaged at its depth, Q-compensation
  * amplitude restoration and 1/Q scaling (tests/test_oracle2d.py).
  */

@@ -341,3 +341,49 @@ def test_angle_gathers_flat_reflector(oracle2d_mod):
     den = sum(float(np.sqrt((ps[60 + d, band] ** 2).sum() * (ps[60 - d, band] ** 2).sum())) for d in range(4, 30))
     assert num / den > 0.9, num / den
     assert np.isfinite(ppc).all() and np.isfinite(psc).all()
+
+
+@pytest.mark.parametrize("R", [4, 6])
+def test_discrete_energy_is_conserved_elastic(oracle2d_mod, R):
+    """Elastic 2D (L = 0), no damping, no source: the staggered leapfrog
+    conserves E^n = 1/2 sum rho |v^n|^2 + 1/2 sum sigma^{n+1/2} . S
+    sigma^{n-1/2} to roundoff, with S the plane-strain compliance (normal
+    block [[M, -lam], [-lam, M]] / (M^2 - lam^2), shear 1/mu).  It holds only
+    if each velocity-side difference is minus the transpose of the matching
+    stress-side one on the updated cells and the same per-cell moduli and
+    density enter both updates; the same functional with a wrong lambda
+    (M / 2) must drift, so the check is not satisfied trivially."""
+    p = cases2d.params(nx=40, nz=36, b=0, R=R, L=0, lt=10, src=(20, 18), nrec=2, rec_z=18)
+    vp, vs, rho, _, _ = cases2d.model(p, seed=2, q=False)
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, vp, vs, rho)
+    nnx, nnz = p.padded_shape
+    rng = np.random.default_rng(4)
+    v_prev = []
+    for name in ("vx", "vz"):
+        a = np.zeros((nnx, nnz))
+        a[R:-R, R:-R] = rng.standard_normal((nnx - 2 * R, nnz - 2 * R))
+        o.set_field(name, a)
+        v_prev.append(a)
+    r = rho.astype(np.float64)
+    M = r * vp.astype(np.float64) ** 2
+    mu = r * vs.astype(np.float64) ** 2
+
+    def energy(v, sn, so, lam):
+        normal = (M * (sn[0] * so[0] + sn[1] * so[1]) - lam * (sn[0] * so[1] + sn[1] * so[0])) / (M * M - lam * lam)
+        return 0.5 * float(np.sum(r * (v[0] ** 2 + v[1] ** 2))) + 0.5 * float(np.sum(normal + sn[2] * so[2] / mu))
+
+    s_prev = [np.zeros((nnx, nnz))] * 3
+    E, Es = [], []
+    for _ in range(40):
+        o.step(1, 100000, 1)  # state: sigma^{n+1/2}, v^{n+1}
+        s_new = [o.get_field(k) for k in ("txx", "tzz", "txz")]
+        E.append(energy(v_prev, s_new, s_prev, M - 2.0 * mu))
+        Es.append(energy(v_prev, s_new, s_prev, 0.5 * M))  # a wrong lambda
+        v_prev = [o.get_field(k) for k in ("vx", "vz")]
+        s_prev = s_new
+    E, Es = np.array(E), np.array(Es)
+    drift = np.abs(E - E[0]).max() / E[0]
+    print(f"R={R}: energy {E[0]:.6e}, max relative drift over 40 steps {drift:.2e}")
+    assert drift < 1e-10
+    assert np.abs(Es - Es[0]).max() / abs(Es[0]) > 1e-3
