@@ -118,6 +118,62 @@ def test_gpu_full_config_grid_parity(erm, oracle):
     o.close()
 
 
+def test_gpu_discrete_energy_conserved_full_grid(erm):
+    """A size-independent property at the full 296x296x168 padded grid: with
+    no damping (zero boundary widths) and no source (time index past the
+    wavelet), the GPU's staggered leapfrog conserves the discrete energy
+    1/2 sum rho |v^n|^2 + 1/2 sum sigma^{n+1/2} . C^-1 sigma^{n-1/2} of every
+    cell, whatever the launch shape.  Tolerance 1e-5 relative over 6 steps
+    (FP32 state; the moduli in the functional are the FP64 values of the
+    FP32 inputs; the FP32 CPU oracle drifts by 2e-9 here).  The functional
+    with a wrong lambda (M / 2) must drift by more than 1e-3, so the check is
+    not satisfied trivially."""
+    R = 4
+    p = cases.params(nx=296, ny=296, nz=168, b=0, R=R, lt=4, dt=0.8, freq=15.0, src=(148, 148, 8),
+                     rec=(4, 4, 100, 100, 20, 20, 8))
+    vp, vs, rho = erm.make_model3d(1, 296, 296, 168, 10.0, 10.0, 10.0, 2)
+    _say(f"full-grid energy: padded {tuple(p.padded_shape)}")
+    g = erm.RTM3D(p)
+    g.load_model(1, vp, vs, rho)
+    shape = g.shape
+    rng = np.random.default_rng(8)
+    v_prev = []
+    for k in ("vx", "vy", "vz"):
+        a = np.zeros(shape, np.float32)
+        a[R:-R, R:-R, R:-R] = rng.standard_normal((shape[0] - 2 * R, shape[1] - 2 * R, shape[2] - 2 * R))
+        g.set_field(k, a)
+        v_prev.append(a.ravel())
+    f64 = lambda a: np.ascontiguousarray(a.astype(np.float64).transpose(2, 1, 0)).ravel()
+    r = f64(rho)
+    M = r * f64(vp) ** 2
+    mu = r * f64(vs) ** 2
+    w_n, w_s = 0.5 / mu, 1.0 / mu
+    w_t = {"right": (M - 2 * mu) / (2 * mu + 3 * (M - 2 * mu)) * w_n, "wrong": 0.5 * M / (2 * mu + 1.5 * M) * w_n}
+    dot = lambda a, b, w: float(np.einsum("i,i,i->", a, b, w))
+
+    def energies(v, sn, so):
+        kin = sum(dot(x, x, r) for x in v)
+        base = sum(dot(sn[c], so[c], w_n) for c in range(3)) + sum(dot(sn[c], so[c], w_s) for c in range(3, 6))
+        tn, to = sn[0] + sn[1] + sn[2], so[0] + so[1] + so[2]
+        return {k: 0.5 * kin + 0.5 * (base - dot(tn, to, wt)) for k, wt in w_t.items()}
+
+    s_prev = [np.zeros(M.size, np.float32)] * 6
+    E = {"right": [], "wrong": []}
+    for n in range(6):
+        g.step(PASS_SOURCE, 100000 + n, 1)  # state: sigma^{n+1/2}, v^{n+1}
+        s_new = [g.get_field(k).ravel() for k in ("txx", "tyy", "tzz", "txy", "txz", "tyz")]
+        for k, e in energies(v_prev, s_new, s_prev).items():
+            E[k].append(e)
+        v_prev = [g.get_field(k).ravel() for k in ("vx", "vy", "vz")]
+        s_prev = s_new
+    g.close()
+    e, ew = np.array(E["right"]), np.array(E["wrong"])
+    drift = np.abs(e - e[0]).max() / e[0]
+    _say(f"full-grid energy: {e[0]:.6e}, max relative drift over 6 steps {drift:.2e}")
+    assert drift < 1e-5
+    assert np.abs(ew - ew[0]).max() / abs(ew[0]) > 1e-3
+
+
 # --------------------------------------------------------------------------- full RTM shot
 def _reflector_case():
     nz, zi = 48, 30
