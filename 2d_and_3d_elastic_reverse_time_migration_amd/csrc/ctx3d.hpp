@@ -119,7 +119,7 @@ inline Geo make_geo(const erm_ctx* c) {
     g.nty = (c->nny + c->ty - 1) / c->ty;
     const int nch = (c->nnz - 2 * c->R + c->kchunk - 1) / c->kchunk;
     g.ntiles = g.ntx * g.nty * nch;
-    g.xcd_map = c->xcd_map;
+    g.xcd_map = c->xcd_map == 1 ? 1 : 0;  // mode 2 reorders only the region-split tile lists
     g.per_xcd = (g.ntiles + 7) / 8;
     return g;
 }

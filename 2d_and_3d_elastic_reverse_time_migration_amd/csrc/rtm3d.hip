@@ -118,6 +118,31 @@ std::vector<int> xcd_order(const std::vector<int>& t) {
     return out;
 }
 
+// XCD y-band order of a tile list (xcd_map 2): the list, sorted y-major
+// (by, bz, bx), is cut into 8 chunks of equal length, so XCD x owns a
+// contiguous band of rows of the grid; inside its chunk the tiles run
+// (bz, by, bx), so the workgroups an XCD holds at once form a compact patch
+// of one z layer whose x and y halos its own L2 serves.  Chunk x fills
+// positions x, x + 8, ...; positions past a shorter chunk hold the sentinel
+// `none` (a tile id that the kernels skip).
+std::vector<int> xcd_band_order(const std::vector<int>& t, const Geo& g, int none) {
+    const int n = (int)t.size();
+    auto key = [&](int l, bool ymajor) {
+        const long bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / (g.ntx * g.nty);
+        return ymajor ? (by * g.ntiles + bz) * g.ntx + bx : (bz * g.nty + by) * g.ntx + bx;
+    };
+    std::vector<int> s = t;
+    std::sort(s.begin(), s.end(), [&](int a, int b) { return key(a, true) < key(b, true); });
+    const int per = (n + 7) / 8;
+    std::vector<int> out((size_t)8 * per, none);
+    for (int x = 0; x < 8; ++x) {
+        const int b0 = std::min(n, x * per), b1 = std::min(n, b0 + per);
+        std::sort(s.begin() + b0, s.begin() + b1, [&](int a, int b) { return key(a, false) < key(b, false); });
+        for (int j = 0; j < b1 - b0; ++j) out[(size_t)8 * j + x] = s[b0 + j];
+    }
+    return out;
+}
+
 int build_split(erm_ctx* c) {
     if (c->split_built && c->split_ty == c->ty && c->split_kchunk == c->kchunk) return 0;
     Geo g = make_geo(c);
@@ -142,9 +167,10 @@ int build_split(erm_ctx* c) {
     for (int m = 0; m < 8; ++m) {
         if (c->tiles[m]) (void)hipFree(c->tiles[m]);
         c->tiles[m] = nullptr;
+        if (!grp[m].empty() && c->xcd_map == 1) grp[m] = xcd_order(grp[m]);
+        if (!grp[m].empty() && c->xcd_map == 2) grp[m] = xcd_band_order(grp[m], g, g.ntiles);
         c->ntl[m] = (int)grp[m].size();
         if (grp[m].empty()) continue;
-        if (c->xcd_map) grp[m] = xcd_order(grp[m]);
         if (dmalloc(c, &c->tiles[m], grp[m].size())) return ERM_E_NOMEM;
         HIPC(c, hipMemcpy(c->tiles[m], grp[m].data(), grp[m].size() * sizeof(int), hipMemcpyHostToDevice));
     }
@@ -620,7 +646,7 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
         const int align = pin->pitch_align == 1 ? 1 : (pin->pitch_align > 1 ? pin->pitch_align : 32);
         c->px = (c->nnx + align - 1) / align * align;  // 128-byte rows: a 64-wide tile row = 2 cache lines
     }
-    c->xcd_map = pin->xcd_map == 1 ? 1 : 0;
+    c->xcd_map = pin->xcd_map == 1 || pin->xcd_map == 2 ? pin->xcd_map : 0;
     c->region_split = pin->cpml_split != 1;
     c->pxy = (long)c->px * c->nny;
     c->ncell = (size_t)c->pxy * c->nnz;

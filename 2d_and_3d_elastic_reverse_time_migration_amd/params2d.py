@@ -2,7 +2,8 @@
 import ctypes
 
 FIELDS2D = ["vx", "vz", "txx", "tzz", "txz", "tp", "vxp", "vzp", "div", "curl",
-            "ex_time", "ex_amp", "img_tp", "img_pp", "img_ps", "ex_tp", "ex_vxp", "ex_vzp", "r_vxp", "r_vzp"]
+            "ex_time", "ex_amp", "img_tp", "img_pp", "img_ps", "ex_tp", "ex_vxp", "ex_vzp", "r_vxp", "r_vzp",
+            "ex_tp_time", "ex_tp_amp", "pdiv", "scurl"]
 FIELD2D_ID = {n: i for i, n in enumerate(FIELDS2D)}
 
 
@@ -15,7 +16,8 @@ class Params2D(ctypes.Structure):
                 + [("coe_attenuation", ctypes.c_float)]
                 + [(k, ctypes.c_int) for k in ["fd_order", "nsls", "shot_x", "shot_z", "nrec", "rec_x0", "rec_dx",
                                                 "rec_z"]]
-                + [("qcomp", ctypes.c_int), ("reserved", ctypes.c_int * 7)])
+                + [("qcomp", ctypes.c_int), ("ex_tp_shift", ctypes.c_int), ("ex_vzp_avg", ctypes.c_int),
+                   ("reserved", ctypes.c_int * 5)])
 
     def update(self, **kw):
         for k, v in kw.items():

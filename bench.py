@@ -349,7 +349,7 @@ def main():
                          "with the source wavefield reconstructed backward from boundary strips")
     ap.add_argument("--region-streams", type=int, choices=[0, 1], default=0,
                     help="run the plain-tile and C-PML-tile launches of a step concurrently on two streams")
-    ap.add_argument("--xcd-map", type=int, choices=[0, 1], default=0,
+    ap.add_argument("--xcd-map", type=int, choices=[0, 1, 2], default=0,
                     help="XCD-aware tile order (each XCD a contiguous chunk of tiles)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the plain copy kernel over GIB GiB (HBM counter calibration) and exit")

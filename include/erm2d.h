@@ -36,7 +36,14 @@ typedef struct erm2d_params {
     int   nrec, rec_x0, rec_dx, rec_z; /* receiver line, model coordinates */
     int   qcomp;       /* 1: Q-compensated receiver pass of erm2d_rtm (memory-term sign flipped,
                           rfwd_/receiver_fwd_ kernels, viscoelastic_equation.cu:1438-1822) */
-    int   reserved[7];
+    int   ex_tp_shift; /* the tp image's excitation time is recorded as it - ex_tp_shift
+                          (caculate_ex_tp_time_new; its commented launch passes it - wavelet_half,
+                          elastic_2D_kernel_3.cu:51); default 0 */
+    int   ex_vzp_avg;  /* 1: the _new excitation/imaging variants (caculate_ex_x_z_new,
+                          imaging_inner_product_ex_2D_new, elastic_2D_kernel_3.cu:91-109,582-605):
+                          vzp averaged over its four neighbours (i, k +- 1), (i +- 1, k) in the
+                          excitation components and in the receiver P and S motion; 0: own cell */
+    int   reserved[5];
 } erm2d_params;
 
 /* fields for erm2d_get_field / erm2d_set_field (padded grid, z fastest) */
@@ -46,6 +53,11 @@ enum erm2d_field {
     ERM2D_EX_TIME, ERM2D_EX_AMP, ERM2D_IMG_TP, ERM2D_IMG_PP, ERM2D_IMG_PS,
     ERM2D_EX_TP, ERM2D_EX_VXP, ERM2D_EX_VZP,  /* excitation state, read only */
     ERM2D_R_VXP, ERM2D_R_VZP,                /* receiver vector P at each cell's imaging step, read only */
+    ERM2D_EX_TP_TIME, ERM2D_EX_TP_AMP,       /* tp excitation: step of max |tp| and the signed tp there
+                                                (caculate_ex_tp_time_new, elastic_2D_kernel_3.cu:50-69) */
+    ERM2D_PDIV, ERM2D_SCURL,                 /* decom_new (elastic_2D_kernel_2.cu:671-791): vp * div at the
+                                                stress node, vs4 * curl at the sxz node, vs4 the mean of vs
+                                                at (i,k), (i,k+1), (i+1,k), (i+1,k+1); read only */
     ERM2D_FIELD_COUNT
 };
 /* passes for erm2d_step */
@@ -79,8 +91,11 @@ int   erm2d_get_dims(const erm2d_ctx* ctx, int* nnx, int* nnz);
  * viscoelastic_equation.cu:184-325, + the velocity update): rec [2][lt][nrec]. */
 int   erm2d_forward(erm2d_ctx* ctx, float* rec);
 /* Excitation-amplitude RTM in the background model: source pass
- * (caculate_ex_amp_time_new, elastic_2D_kernel_3.cu:30-48) then receiver pass with
- * imaging_correlation_ex_2D / imaging_inner_product_ex_2D (:542-580).
+ * (caculate_ex_amp_time_new, elastic_2D_kernel_3.cu:30-48, and caculate_ex_tp_time_new,
+ * :50-69) then receiver pass with imaging_correlation_ex_2D / imaging_inner_product_ex_2D
+ * (:542-580): I_PP = ex_vP.vP_r/|ex_vP|^2 and I_PS = ex_vP.vS_r/|ex_vP|^2 at the step of
+ * max |vP|; I_tp = tp_r / ex_tp_amp at the step of max |tp| (its own excitation time and
+ * the signed tp there).  A zero denominator images 0.
  * Images over the model region [nx][nz]; any pointer may be NULL. */
 int   erm2d_rtm(erm2d_ctx* ctx, const float* rec, float* img_tp, float* img_pp, float* img_ps);
 

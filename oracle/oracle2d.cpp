@@ -123,6 +123,10 @@ struct Oracle2D {
     std::vector<int> ex_time;
     std::vector<T> ex_amp, ex_tp, ex_vxp, ex_vzp, img_tp, img_pp, img_ps;
     std::vector<T> r_vxp, r_vzp;  // receiver vector P at each cell's imaging step
+    std::vector<int> ex_tp_time;   // tp excitation (caculate_ex_tp_time_new)
+    std::vector<T> ex_tp_amp;
+    std::vector<T> vp_m[2], vs_m[2];  // loaded vp, vs (decom_new)
+    int last_model = 1;                // model of the last pass (0 true, 1 background)
     std::vector<T> rec;  /* [2][lt][nrec] */
 
     size_t id(int i, int k) const { return (size_t)i * nnz + k; }
@@ -168,7 +172,9 @@ struct Oracle2D {
         }
         rec.assign((size_t)2 * p.lt * p.nrec, 0);
         ex_time.assign(N, -1);
-        for (auto* v : {&ex_amp, &ex_tp, &ex_vxp, &ex_vzp, &img_tp, &img_pp, &img_ps, &r_vxp, &r_vzp}) v->assign(N, 0);
+        ex_tp_time.assign(N, -1);
+        for (auto* v : {&ex_amp, &ex_tp, &ex_vxp, &ex_vzp, &img_tp, &img_pp, &img_ps, &r_vxp, &r_vzp, &ex_tp_amp})
+            v->assign(N, 0);
         return set_shot(p.shot_x, p.shot_z);
     }
 
@@ -182,6 +188,7 @@ struct Oracle2D {
     int load_model(int which, const float* vp, const float* vs, const float* rho, const float* qp, const float* qs) {
         Model& m = mod[which];
         m.M.assign(N, 0); m.U.assign(N, 0); m.B.assign(N, 0);
+        vp_m[which].assign(N, 0); vs_m[which].assign(N, 0);
         for (int l = 0; l < L; ++l) { m.kP[l].assign(N, 0); m.kS[l].assign(N, 0); m.decay[l].assign(N, 0); }
         for (int i = 0; i < nnx; ++i)
             for (int k = 0; k < nnz; ++k) {
@@ -190,6 +197,7 @@ struct Oracle2D {
                 size_t s = (size_t)ix * p.nz + iz, n = id(i, k);
                 double a = vp[s], b = vs[s], rr = rho[s];
                 if (!(a > 0) || b < 0 || !(rr > 0)) return -1;
+                vp_m[which][n] = (T)a; vs_m[which][n] = (T)b;
                 Sls q = sls_params(L, p.freq, L ? qp[s] : 1.0, L ? qs[s] : 1.0);
                 m.M[n] = (T)(rr * a * a * q.fM);
                 m.U[n] = (T)(rr * b * b * q.fU);
@@ -276,20 +284,61 @@ struct Oracle2D {
                 T nxp = a * (b * fv[VXP][n] + bo * px);
                 T nzp = a * (b * fv[VZP][n] + bo * pz);
                 fv[VX][n] = nvx; fv[VZ][n] = nvz; fv[VXP][n] = nxp; fv[VZP][n] = nzp;
-                if (mode == 1) {  /* caculate_ex_amp_time_new: |vP| = sqrt(vxp^2 + vzp^2) */
-                    T amp = std::sqrt(nxp * nxp + nzp * nzp);
-                    if (ex_amp[n] < amp) {
-                        ex_time[n] = it; ex_amp[n] = amp; ex_tp[n] = fv[TP][n];
-                        ex_vxp[n] = nxp; ex_vzp[n] = nzp;
-                    }
-                } else if (mode == 2 && ex_time[n] == it) {
-                    T etp = ex_tp[n];
-                    img_tp[n] = etp != (T)0 ? fv[TP][n] / etp : (T)0;
-                    T ex = ex_vxp[n], ez = ex_vzp[n], den = ex * ex + ez * ez;
-                    img_pp[n] = den > 0 ? (ex * nxp + ez * nzp) / den : (T)0;
-                    img_ps[n] = den > 0 ? (ex * (nvx - nxp) + ez * (nvz - nzp)) / den : (T)0;
-                    r_vxp[n] = nxp; r_vzp[n] = nzp;
+            }
+        if (mode == 1) excite(fv, it);
+        else if (mode == 2) image(fv, it);
+    }
+
+    /* vzp at cell n: own value, or the mean over (i, k+-1), (i+-1, k) of the
+     * _new variants (caculate_ex_x_z_new, elastic_2D_kernel_3.cu:91-109) */
+    T vzp_at(const T* a, size_t n) const {
+        if (!p.ex_vzp_avg) return a[n];
+        return (a[n + 1] + a[n - 1] + a[n + nnz] + a[n - nnz]) / (T)4;
+    }
+    /* excitation after the velocity update of step it (interior cells):
+     * caculate_ex_amp_time_new (elastic_2D_kernel_3.cu:30-48): ex_time = it where
+     * |vP| = sqrt(vxp^2 + vzp^2) exceeds ex_amp (strict '<': first maximum), with
+     * tp and the vP components (caculate_ex_x_z[_new], :71-109) of that step;
+     * caculate_ex_tp_time_new (:50-69): ex_tp_time = it - ex_tp_shift where |tp|
+     * exceeds |ex_tp_amp|, ex_tp_amp = tp (signed). */
+    void excite(std::vector<T>* fv, int it) {
+        std::vector<T> zq(N);
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) zq[id(i, k)] = vzp_at(fv[VZP].data(), id(i, k));
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) {
+                size_t n = id(i, k);
+                T nxp = fv[VXP][n], nzp = fv[VZP][n];
+                T amp = std::sqrt(nxp * nxp + nzp * nzp);
+                if (ex_amp[n] < amp) {
+                    ex_time[n] = it; ex_amp[n] = amp; ex_tp[n] = fv[TP][n];
+                    ex_vxp[n] = nxp; ex_vzp[n] = zq[n];
                 }
+                T tpv = fv[TP][n];
+                if (std::fabs(ex_tp_amp[n]) < std::fabs(tpv)) { ex_tp_time[n] = it - p.ex_tp_shift; ex_tp_amp[n] = tpv; }
+            }
+    }
+    /* imaging of step it (interior cells; imaging_correlation_ex_2D and
+     * imaging_inner_product_ex_2D[_new], elastic_2D_kernel_3.cu:542-605):
+     * where ex_tp_time == it, I_tp = tp / ex_tp_amp; where ex_time == it,
+     * I_PP = (ex_vxp vxp + ex_vzp vzp') / |ex|^2 and I_PS likewise with
+     * vS = v - vP, vzp' and vzs' averaged as in the excitation; a zero
+     * denominator images 0.  Also keeps the receiver vP of the imaging step. */
+    void image(std::vector<T>* fv, int it) {
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) {
+                size_t n = id(i, k);
+                if (ex_tp_time[n] == it) {
+                    T a = ex_tp_amp[n];
+                    img_tp[n] = a != (T)0 ? fv[TP][n] / a : (T)0;
+                }
+                if (ex_time[n] != it) continue;
+                T nxp = fv[VXP][n], nzp = vzp_at(fv[VZP].data(), n);
+                T nxs = fv[VX][n] - fv[VXP][n], nzs = vzp_at(fv[VZ].data(), n) - nzp;
+                T ex = ex_vxp[n], ez = ex_vzp[n], den = ex * ex + ez * ez;
+                img_pp[n] = den > 0 ? (ex * nxp + ez * nzp) / den : (T)0;
+                img_ps[n] = den > 0 ? (ex * nxs + ez * nzs) / den : (T)0;
+                r_vxp[n] = fv[VXP][n]; r_vzp[n] = fv[VZP][n];
             }
     }
 
@@ -300,6 +349,7 @@ struct Oracle2D {
     int step(int pass, int it_begin, int nsteps) {
         const Model& m = mod[pass == 0 ? 0 : 1];
         if (!m.have) return -4;
+        last_model = pass == 0 ? 0 : 1;
         for (int s = 0; s < nsteps; ++s) {
             if (pass == 2) {
                 int it = it_begin - s;
@@ -342,6 +392,21 @@ struct Oracle2D {
                 size_t n = id(i, k);
                 dv[n] = (T)(1.0 / p.dx) * dp(f[VX].data() + n, sX) + (T)(1.0 / p.dz) * dm(f[VZ].data() + n, 1);
                 cu[n] = (T)(1.0 / p.dz) * dp(f[VX].data() + n, 1) - (T)(1.0 / p.dx) * dm(f[VZ].data() + n, sX);
+            }
+    }
+
+    /* decom_new (elastic_2D_kernel_2.cu:671-791): the decom stencils scaled by
+     * vp at the stress node and by the mean of vs over (i,k), (i,k+1), (i+1,k),
+     * (i+1,k+1) (:726, :758, :789); model `which`. */
+    void decom_new(int which, std::vector<T>& dv, std::vector<T>& cu) const {
+        decom(dv, cu);
+        const std::vector<T>& a = vp_m[which];
+        const std::vector<T>& b = vs_m[which];
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) {
+                size_t n = id(i, k);
+                dv[n] *= a[n];
+                cu[n] *= (b[n] + b[n + 1] + b[n + nnz] + b[n + 1 + nnz]) / (T)4;
             }
     }
 
@@ -568,7 +633,9 @@ int oracle2d_zero(void* h, int what) {
         if (what & 1) o.zero();
         if (what & 2) {
             std::fill(o.ex_time.begin(), o.ex_time.end(), -1);
-            for (auto* v : {&o.ex_amp, &o.ex_tp, &o.ex_vxp, &o.ex_vzp, &o.r_vxp, &o.r_vzp}) std::fill(v->begin(), v->end(), 0);
+            std::fill(o.ex_tp_time.begin(), o.ex_tp_time.end(), -1);
+            for (auto* v : {&o.ex_amp, &o.ex_tp, &o.ex_vxp, &o.ex_vzp, &o.r_vxp, &o.r_vzp, &o.ex_tp_amp})
+                std::fill(v->begin(), v->end(), 0);
         }
         if (what & 4) for (auto* v : {&o.img_tp, &o.img_pp, &o.img_ps}) std::fill(v->begin(), v->end(), 0);
         return 0;
@@ -588,8 +655,16 @@ int oracle2d_get_field(void* h, int fid, double* dst) {
             for (size_t n = 0; n < o.N; ++n) dst[n] = s[n];
             return 0;
         }
+        if (fid == ERM2D_PDIV || fid == ERM2D_SCURL) {
+            std::vector<T> dv, cu;
+            o.decom_new(o.last_model, dv, cu);
+            const auto& s = fid == ERM2D_PDIV ? dv : cu;
+            for (size_t n = 0; n < o.N; ++n) dst[n] = s[n];
+            return 0;
+        }
         if (fid == ERM2D_EX_TIME) { for (size_t n = 0; n < o.N; ++n) dst[n] = o.ex_time[n]; return 0; }
-        const std::vector<T>* s = fid == ERM2D_EX_AMP ? &o.ex_amp : fid == ERM2D_IMG_TP ? &o.img_tp
+        if (fid == ERM2D_EX_TP_TIME) { for (size_t n = 0; n < o.N; ++n) dst[n] = o.ex_tp_time[n]; return 0; }
+        const std::vector<T>* s = fid == ERM2D_EX_TP_AMP ? &o.ex_tp_amp : fid == ERM2D_EX_AMP ? &o.ex_amp : fid == ERM2D_IMG_TP ? &o.img_tp
                                 : fid == ERM2D_IMG_PP ? &o.img_pp : fid == ERM2D_IMG_PS ? &o.img_ps
                                 : fid == ERM2D_EX_TP ? &o.ex_tp : fid == ERM2D_EX_VXP ? &o.ex_vxp
                                 : fid == ERM2D_EX_VZP ? &o.ex_vzp : fid == ERM2D_R_VXP ? &o.r_vxp
@@ -612,6 +687,10 @@ int oracle2d_record(void* h, double* dst, const double* src) {
         if (dst) for (size_t n = 0; n < o.rec.size(); ++n) dst[n] = o.rec[n];
         return 0;
     });
+}
+/* the imaging condition of step it on the current fields, no propagation */
+int oracle2d_image(void* h, int it) {
+    return disp(h, [&](auto& o) { o.image(o.f, it); return 0; });
 }
 int oracle2d_born(void* h, const float* dm, float* d) { return disp(h, [&](auto& o) { return o.born(dm, d); }); }
 int oracle2d_adjoint(void* h, const float* r, float* g) { return disp(h, [&](auto& o) { return o.adjoint(r, g); }); }

@@ -32,6 +32,7 @@ for name, res, args in [
     ("oracle2d_get_field", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("oracle2d_set_field", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("oracle2d_record", ctypes.c_int, [_vp, _dp, _dp]),
+    ("oracle2d_image", ctypes.c_int, [_vp, ctypes.c_int]),
     ("oracle2d_born", ctypes.c_int, [_vp, _fp, _fp]),
     ("oracle2d_adjoint", ctypes.c_int, [_vp, _fp, _fp]),
     ("oracle2d_sls", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
@@ -100,6 +101,11 @@ class Oracle2D:
     def set_field(self, name, a):
         a = np.ascontiguousarray(a, np.float64)
         self._c(lib.oracle2d_set_field(self._h, FIELD2D_ID[name], _ptr(a, ctypes.c_double)))
+
+    def image(self, it):
+        """The excitation imaging condition of step `it` on the current fields
+        (no propagation)."""
+        self._c(lib.oracle2d_image(self._h, it))
 
     def record(self, new=None):
         p = self.p

@@ -350,28 +350,47 @@ struct Oracle3D {
                     T nzp = a * (b * vzp[n] + bo * pz);
                     vx[n] = nvx; vy[n] = nvy; vz[n] = nvz;
                     vxp[n] = nxp; vyp[n] = nyp; vzp[n] = nzp;
-                    if (mode == 1) {
-                        /* caculate_ex_amp_time_new (elastic_2D_kernel_3.cu:30-48) in 3D:
-                         * amplitude |vP|, strict '<' so the first maximum wins. */
-                        T amp = std::sqrt(nxp * nxp + nyp * nyp + nzp * nzp);
-                        if (ex_amp[n] < amp) {
-                            ex_time[n] = it; ex_amp[n] = amp; ex_tp[n] = tp[n];
-                            ex_vxp[n] = nxp; ex_vyp[n] = nyp; ex_vzp[n] = nzp;
-                        }
-                    } else if (mode == 2) {
-                        if (ex_time[n] == it) {
-                            /* imaging_correlation_ex_2D (:542-558), imaging_inner_product_ex_2D
-                             * (:559-580); a zero denominator images 0. */
-                            T etp = ex_tp[n];
-                            img_tp[n] = etp != (T)0 ? tp[n] / etp : (T)0;
-                            T ex = ex_vxp[n], ey = ex_vyp[n], ez = ex_vzp[n];
-                            T den = ex * ex + ey * ey + ez * ez;
-                            T pp = ex * nxp + ey * nyp + ez * nzp;
-                            T ps = ex * (nvx - nxp) + ey * (nvy - nyp) + ez * (nvz - nzp);
-                            img_pp[n] = den > (T)0 ? pp / den : (T)0;
-                            img_ps[n] = den > (T)0 ? ps / den : (T)0;
-                        }
-                    }
+                    if (mode == 1) excite_cell(n, it, tp[n], nxp, nyp, nzp);
+                    else if (mode == 2) image_cell(n, it, tp[n], nvx, nvy, nvz, nxp, nyp, nzp);
+                }
+    }
+
+    /* caculate_ex_amp_time_new (elastic_2D_kernel_3.cu:30-48) in 3D, the
+     * cuda_cal_excitation_amp_time_new launch (:282): amplitude |vP|, strict
+     * '<' so the first maximum wins; tp and vP of that step are kept. */
+    void excite_cell(size_t n, int it, T tpv, T nxp, T nyp, T nzp) {
+        T amp = std::sqrt(nxp * nxp + nyp * nyp + nzp * nzp);
+        if (ex_amp[n] < amp) {
+            ex_time[n] = it; ex_amp[n] = amp; ex_tp[n] = tpv;
+            ex_vxp[n] = nxp; ex_vyp[n] = nyp; ex_vzp[n] = nzp;
+        }
+    }
+    /* imaging_correlation_ex + imaging_inner_product_ex (:442-446; 2D bodies
+     * elastic_2D_kernel_3.cu:542-580) at the cell's excitation step:
+     *   I_tp = tp_r / ex_tp,  I_PP = ex_vP . vP_r / |ex_vP|^2,
+     *   I_PS = ex_vP . (v_r - vP_r) / |ex_vP|^2;
+     * a zero denominator images 0.  The reference passes the stabilisers
+     * amp_max = tp_max = 0 (globals, 3D_elastic_modeling_parameter:139, never
+     * set in the shipped files), so the plain quotients are its result. */
+    void image_cell(size_t n, int it, T tpv, T nvx, T nvy, T nvz, T nxp, T nyp, T nzp) {
+        if (ex_time[n] != it) return;
+        T etp = ex_tp[n];
+        img_tp[n] = etp != (T)0 ? tpv / etp : (T)0;
+        T ex = ex_vxp[n], ey = ex_vyp[n], ez = ex_vzp[n];
+        T den = ex * ex + ey * ey + ez * ez;
+        T pp = ex * nxp + ey * nyp + ez * nzp;
+        T ps = ex * (nvx - nxp) + ey * (nvy - nyp) + ez * (nvz - nzp);
+        img_pp[n] = den > (T)0 ? pp / den : (T)0;
+        img_ps[n] = den > (T)0 ? ps / den : (T)0;
+    }
+    /* The imaging condition alone, applied to the current fields at step it
+     * (no propagation): pins the condition on prescribed receiver fields. */
+    void image_fields(int it) {
+        for (int k = R; k < nnz - R; ++k)
+            for (int j = R; j < nny - R; ++j)
+                for (int i = R; i < nnx - R; ++i) {
+                    size_t n = id(i, j, k);
+                    image_cell(n, it, f[9][n], f[0][n], f[1][n], f[2][n], f[10][n], f[11][n], f[12][n]);
                 }
     }
 
@@ -562,6 +581,9 @@ int oracle3d_get_field(void* h, int fid, double* dst) {
 }
 int oracle3d_set_field(void* h, int fid, const double* src) {
     return dispatch(h, [&](auto& o) { return o.set_field(fid, src); });
+}
+int oracle3d_image(void* h, int it) {
+    return dispatch(h, [&](auto& o) { o.image_fields(it); return 0; });
 }
 int oracle3d_get_record(void* h, double* dst) {
     return dispatch(h, [&](auto& o) {

@@ -58,6 +58,7 @@ def _load():
         "oracle3d_get_record": (ctypes.c_int, [vp, dp]),
         "oracle3d_set_record": (ctypes.c_int, [vp, dp]),
         "oracle3d_rtm_shot": (ctypes.c_int, [vp, ctypes.c_int]),
+        "oracle3d_image": (ctypes.c_int, [vp, ctypes.c_int]),
         "oracle3d_xcorr_shot": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
@@ -201,6 +202,12 @@ class Oracle3D:
     def set_record(self, rec):
         rec = np.ascontiguousarray(rec, np.float64)
         self._c(lib.oracle3d_set_record(self._h, _dp(rec)))
+
+    def image(self, it):
+        """Apply the excitation imaging condition of step `it` to the current
+        fields (v, vP, tp) without propagating: cells whose excitation time is
+        `it` get I_tp, I_PP, I_PS."""
+        self._c(lib.oracle3d_image(self._h, it))
 
     def rtm_shot(self, use_record=False):
         self._c(lib.oracle3d_rtm_shot(self._h, 1 if use_record else 0))

@@ -97,7 +97,7 @@ def test_gpu_cpml_matches_fp64_oracle(erm, oracle, R):
 def test_gpu_cpml_region_split_and_autotune_are_exact(erm):
     """The region-split launches (plain kernel on tiles without C-PML cells),
     run on one stream or concurrently on two (region_streams), in the
-    XCD-aware tile order (xcd_map), and every autotuned launch shape give
+    XCD-aware tile orders (xcd_map 1: contiguous chunks, 2: y bands), and every autotuned launch shape give
     bit-identical shots to one C-PML kernel over all tiles with the default
     shape."""
     p = cases.params(nx=150, ny=40, nz=44, b=12, R=4, lt=80, dt=1.0, freq=25.0, src=(75, 20, 3))
@@ -106,7 +106,7 @@ def test_gpu_cpml_region_split_and_autotune_are_exact(erm):
     sm = [cases.smooth(a, 3) for a in (vp, vs, rho)]
     out = []
     for split_off, tune, streams, xcd in ((1, False, 0, 0), (0, False, 0, 0), (0, True, 0, 0), (0, False, 1, 0),
-                                          (0, True, 1, 0), (0, False, 0, 1)):
+                                          (0, True, 1, 0), (0, False, 0, 1), (0, False, 0, 2), (0, True, 0, 2)):
         p.cpml_split = split_off
         p.region_streams = streams
         p.xcd_map = xcd

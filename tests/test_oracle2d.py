@@ -387,3 +387,163 @@ def test_discrete_energy_is_conserved_elastic(oracle2d_mod, R):
     print(f"R={R}: energy {E[0]:.6e}, max relative drift over 40 steps {drift:.2e}")
     assert drift < 1e-10
     assert np.abs(Es - Es[0]).max() / abs(Es[0]) > 1e-3
+
+
+def _source_snapshots(oracle2d_mod, p, seed=2):
+    vp, vs, rho, qp, qs = cases2d.model(p, seed=seed, q=p.nsls > 0)
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, vp, vs, rho, qp, qs)
+    snap = {k: [] for k in ("vx", "vz", "vxp", "vzp", "tp")}
+    for it in range(p.lt_forward):
+        o.step(1, it, 1)
+        for k in snap:
+            snap[k].append(o.get_field(k))
+    return o, {k: np.array(v) for k, v in snap.items()}
+
+
+@pytest.mark.parametrize("avg", [0, 1])
+def test_excitation_and_imaging_closed_forms_2d(oracle2d_mod, avg):
+    """2D excitation and imaging against source-pass snapshots.
+    Excitation: ex_time is the first step of maximum |vP| and ex_tp_time the
+    first step of maximum |tp| (numpy argmax over the snapshots), ex_tp_amp
+    the signed tp there, the stored vP components those of that step (vzp
+    averaged over (i, k+-1), (i+-1, k) for the _new variant).  Imaging: a
+    receiver field equal at every step to alpha x the source vector P plus an
+    S part gamma x the source vector P, and tp_r = alpha x tp_s, images
+    I_PP = alpha, I_PS = gamma, I_tp = alpha exactly at every excited cell; a
+    wrong time pairing, a |ex| normalisation, a PS sign flip or swapped PP/PS
+    images would not (shown on the same inputs)."""
+    p = cases2d.params(nx=30, nz=26, b=8, R=4, L=0, lt=60, freq=25.0, src=(15, 6), ex_vzp_avg=avg)
+    o, S = _source_snapshots(oracle2d_mod, p)
+    R = 4
+    inner = np.zeros(p.padded_shape, bool)
+    inner[R:-R, R:-R] = True
+    amp = np.sqrt(S["vxp"] ** 2 + S["vzp"] ** 2)
+    et = o.get_field("ex_time").astype(int)
+    want = np.where(amp.max(axis=0) > 0, np.argmax(amp, axis=0), -1)
+    np.testing.assert_array_equal(et[inner], want[inner])
+    ett = o.get_field("ex_tp_time").astype(int)
+    want_t = np.where(np.abs(S["tp"]).max(axis=0) > 0, np.argmax(np.abs(S["tp"]), axis=0), -1)
+    np.testing.assert_array_equal(ett[inner], want_t[inner])
+    hit_t = inner & (ett >= 0)
+    tp_at = np.take_along_axis(S["tp"], np.maximum(ett, 0)[None], axis=0)[0]
+    np.testing.assert_array_equal(o.get_field("ex_tp_amp")[hit_t], tp_at[hit_t])
+
+    def zmean(a):  # (i, k+-1), (i+-1, k) mean of the _new variant
+        q = np.zeros_like(a)
+        q[..., 1:-1, 1:-1] = (a[..., 1:-1, 2:] + a[..., 1:-1, :-2] + a[..., 2:, 1:-1] + a[..., :-2, 1:-1]) / 4
+        return q
+    vz_of = zmean if avg else (lambda a: a)
+    exz = np.take_along_axis(vz_of(S["vzp"]), np.maximum(et, 0)[None], axis=0)[0]
+    hit = inner & (et >= 0)
+    np.testing.assert_allclose(o.get_field("ex_vzp")[hit], exz[hit], rtol=1e-12, atol=1e-300)
+
+    alpha, gamma = 0.7, -1.3
+    o.zero(5)
+    for it in range(p.lt_forward):
+        o.set_field("vxp", alpha * S["vxp"][it])
+        o.set_field("vzp", alpha * S["vzp"][it])
+        o.set_field("vx", (alpha + gamma) * S["vxp"][it])
+        o.set_field("vz", (alpha + gamma) * S["vzp"][it])
+        o.set_field("tp", alpha * S["tp"][it])
+        o.image(it)
+    exv = np.stack([o.get_field("ex_vxp"), o.get_field("ex_vzp")])
+    den = (exv ** 2).sum(axis=0)
+    good = hit & (den > 0)
+    assert good.sum() > 0.5 * inner.sum()
+    ipp, ips, itp = o.get_field("img_pp"), o.get_field("img_ps"), o.get_field("img_tp")
+    np.testing.assert_allclose(ipp[good], alpha, rtol=1e-10)
+    np.testing.assert_allclose(ips[good], gamma, rtol=1e-10)
+    good_t = hit_t & (o.get_field("ex_tp_amp") != 0)
+    np.testing.assert_allclose(itp[good_t], alpha, rtol=1e-12)
+    # plausible mistakes on the same inputs
+    rpp = alpha * den
+    assert not np.allclose(rpp[good] / np.sqrt(den[good]), alpha, rtol=1e-3)
+    assert not np.allclose(-gamma * np.ones(good.sum()), gamma)
+    shifted = np.take_along_axis(S["tp"], np.clip(ett - 1, 0, None)[None], axis=0)[0]
+    sel = good_t & (ett > 0)
+    assert not np.allclose(alpha * shifted[sel] / tp_at[sel], alpha, rtol=1e-3)  # off-by-one step
+    # ex_tp_shift records the tp excitation time s steps earlier, amplitude unchanged
+    q = cases2d.params(nx=30, nz=26, b=8, R=4, L=0, lt=60, freq=25.0, src=(15, 6), ex_vzp_avg=avg, ex_tp_shift=5)
+    o2, _ = _source_snapshots(oracle2d_mod, q)
+    np.testing.assert_array_equal(o2.get_field("ex_tp_time")[hit_t], ett[hit_t] - 5)
+    np.testing.assert_array_equal(o2.get_field("ex_tp_amp")[hit_t], o.get_field("ex_tp_amp")[hit_t])
+
+
+def test_decom_discrete_identities(oracle2d_mod):
+    """div/curl (decom, elastic_2D_kernel_2.cu:621-666) on staggered fields
+    built from random potentials: v = (-D-z psi, D+x psi) from psi on the sxz
+    nodes has zero divergence, v = (D-x phi, D+z phi) from phi on the stress
+    nodes zero curl, to roundoff (the operators of different axes commute);
+    a D+/D- swap in either breaks both.  Linear fields: vx = a x, vz = b z give
+    div = a + b and curl = 0; vx = c z, vz = e x give curl = c - e (the
+    coefficients sum to 1 over their odd weights).  decom_new (:671-791)
+    scales div by vp and curl by the mean of vs over (i,k), (i,k+1), (i+1,k),
+    (i+1,k+1): for vs linear in x and z that mean is vs at (i+1/2, k+1/2)."""
+    R = 4
+    p = cases2d.params(nx=24, nz=22, b=8, R=R, L=0, lt=4)
+    nnx, nnz = p.padded_shape
+    o = oracle2d_mod.Oracle2D(p)
+    x = np.arange(nnx)[:, None] * p.dx * np.ones((1, nnz))
+    z = np.arange(nnz)[None, :] * p.dz * np.ones((nnx, 1))
+    vp0 = np.full((p.nx, p.nz), 2500.0, np.float32)
+    vs_lin = (1500.0 + 3.0 * np.arange(p.nx)[:, None] + 5.0 * np.arange(p.nz)[None, :]).astype(np.float32)
+    o.load_model(1, vp0, vs_lin, np.full_like(vp0, 2000.0))
+    c = oracle2d_mod.oracle3d.coefficients(R)
+    rng = np.random.default_rng(7)
+
+    def dpx(f):  # D+ along x (axis 0), same shape, interior only
+        out = np.zeros_like(f)
+        for m in range(1, R + 1):
+            out[R:-R, :] += c[m] * (np.roll(f, -m, 0) - np.roll(f, m - 1, 0))[R:-R, :]
+        return out
+
+    def dmx(f):
+        out = np.zeros_like(f)
+        for m in range(1, R + 1):
+            out[R:-R, :] += c[m] * (np.roll(f, -(m - 1), 0) - np.roll(f, m, 0))[R:-R, :]
+        return out
+
+    def dpz(f):
+        return dpx(f.T).T
+
+    def dmz(f):
+        return dmx(f.T).T
+
+    inner = np.zeros((nnx, nnz), bool)
+    inner[2 * R:-2 * R, 2 * R:-2 * R] = True
+    psi = rng.standard_normal((nnx, nnz))
+    o.set_field("vx", -dmz(psi) / p.dz)
+    o.set_field("vz", dpx(psi) / p.dx)
+    dv, cu = o.get_field("div"), o.get_field("curl")
+    assert np.max(np.abs(dv[inner])) < 1e-12 * np.max(np.abs(cu[inner]))
+    phi = rng.standard_normal((nnx, nnz))
+    o.set_field("vx", dmx(phi) / p.dx)
+    o.set_field("vz", dpz(phi) / p.dz)
+    dv, cu = o.get_field("div"), o.get_field("curl")
+    assert np.max(np.abs(cu[inner])) < 1e-12 * np.max(np.abs(dv[inner]))
+    # a swapped operator (D+ for D- along z in the S potential) leaves a divergence
+    o.set_field("vx", -dpz(psi) / p.dz)
+    o.set_field("vz", dpx(psi) / p.dx)
+    assert np.max(np.abs(o.get_field("div")[inner])) > 1e-3 * np.max(np.abs(o.get_field("curl")[inner]))
+    a, b, cc, e = 0.3, -0.7, 0.45, 0.2
+    rin = np.zeros((nnx, nnz), bool)
+    rin[R:-R, R:-R] = True
+    o.set_field("vx", a * (x - 0.5 * p.dx))   # vx lives at x - dx/2
+    o.set_field("vz", b * (z + 0.5 * p.dz))   # vz at z + dz/2
+    np.testing.assert_allclose(o.get_field("div")[rin], a + b, rtol=1e-9)
+    np.testing.assert_allclose(o.get_field("curl")[rin], 0.0, atol=1e-9)
+    o.set_field("vx", cc * z)
+    o.set_field("vz", e * x)
+    np.testing.assert_allclose(o.get_field("curl")[rin], cc - e, rtol=1e-9)
+    np.testing.assert_allclose(o.get_field("div")[rin], 0.0, atol=1e-9)
+    # decom_new: vp * div, vs(i+1/2, k+1/2) * curl
+    o.set_field("vx", a * (x - 0.5 * p.dx) + cc * z)
+    o.set_field("vz", b * (z + 0.5 * p.dz) + e * x)
+    np.testing.assert_allclose(o.get_field("pdiv")[rin], 2500.0 * (a + b), rtol=1e-9)
+    ii = np.clip(np.arange(nnx) - p.bl, 0, p.nx - 1)[:, None]
+    kk = np.clip(np.arange(nnz) - p.bu, 0, p.nz - 1)[None, :]
+    ii1 = np.clip(np.arange(nnx) + 1 - p.bl, 0, p.nx - 1)[:, None]
+    kk1 = np.clip(np.arange(nnz) + 1 - p.bu, 0, p.nz - 1)[None, :]
+    vs_mid = 1500.0 + 3.0 * (ii + ii1) / 2 + 5.0 * (kk + kk1) / 2
+    np.testing.assert_allclose(o.get_field("scurl")[rin], vs_mid[rin] * (cc - e), rtol=1e-6)

@@ -506,3 +506,68 @@ def test_discrete_energy_is_conserved(oracle, R):
     assert drift < 1e-10
     Es = np.array(E_swapped)
     assert np.abs(Es - Es[0]).max() / abs(Es[0]) > 1e-3
+
+
+def _excited_state(oracle, p, vp, vs, rho):
+    o = oracle.Oracle3D(p)
+    o.load_model(1, vp, vs, rho)
+    o.step(1, 0, p.lt_forward)
+    ex = {k: o.get_field(k) for k in ("ex_time", "ex_tp", "ex_vxp", "ex_vyp", "ex_vzp")}
+    return o, ex
+
+
+def test_imaging_condition_closed_forms(oracle):
+    """The excitation-amplitude imaging condition on prescribed receiver
+    fields.  With the receiver vector P equal to alpha times the source vector
+    P of each cell's excitation step and the receiver S part equal to gamma
+    times it, I_PP = alpha and I_PS = gamma exactly; with tp_r = alpha ex_tp,
+    I_tp = alpha; an S part orthogonal to ex_vP images 0; cells never excited
+    image 0.  A |ex_vP| (not |ex_vP|^2) normalisation, a PS sign flip (vP - v
+    for vS) or swapped PP/PS images fail these checks (shown on the same
+    inputs below)."""
+    p = cases.params(nx=20, ny=18, nz=16, b=6, R=4, lt=40, dt=1.0, freq=25.0, src=(10, 9, 5))
+    p.lt_forward = p.lt
+    vp, vs, rho = cases.layered(p, seed=3)
+    o, ex = _excited_state(oracle, p, vp, vs, rho)
+    et = ex["ex_time"]
+    exv = np.stack([ex["ex_vxp"], ex["ex_vyp"], ex["ex_vzp"]])
+    den = (exv ** 2).sum(axis=0)
+    R = 4
+    inner = np.zeros(et.shape, bool)
+    inner[R:-R, R:-R, R:-R] = True
+    hit = inner & (et >= 0) & (den > 0) & (ex["ex_tp"] != 0)
+    assert hit.sum() > 0.5 * inner.sum()
+    alpha, gamma = 0.7, -1.3
+    names = ("vx", "vy", "vz")
+
+    def image_with(vp_r, vs_r, tp_r):
+        o.zero(wavefield=True, images=True)
+        for c in range(3):
+            o.set_field(names[c], vp_r[c] + vs_r[c])
+            o.set_field("v" + "xyz"[c] + "p", vp_r[c])
+        o.set_field("tp", tp_r)
+        for it in range(p.lt_forward):
+            o.image(it)
+        return o.get_field("img_tp"), o.get_field("img_pp"), o.get_field("img_ps")
+
+    itp, ipp, ips = image_with(alpha * exv, gamma * exv, alpha * ex["ex_tp"])
+    np.testing.assert_allclose(ipp[hit], alpha, rtol=1e-12)
+    np.testing.assert_allclose(ips[hit], gamma, rtol=1e-12)
+    np.testing.assert_allclose(itp[hit], alpha, rtol=1e-12)
+    never = inner & (et < 0)
+    assert np.all(ipp[never] == 0) and np.all(ips[never] == 0) and np.all(itp[never] == 0)
+    # S part orthogonal to ex_vP: (ex_vP x e) for a fixed unit e
+    e = np.array([0.3, -0.5, 0.8])[:, None, None, None]
+    orth = np.cross(exv, np.broadcast_to(e, exv.shape), axis=0) * 1e3
+    _, ipp2, ips2 = image_with(alpha * exv, orth, alpha * ex["ex_tp"])
+    scale = np.sqrt((orth ** 2).sum(axis=0) / np.where(den > 0, den, 1))
+    np.testing.assert_allclose(ipp2[hit], alpha, rtol=1e-12)
+    assert np.max(np.abs(ips2[hit]) / (1 + scale[hit])) < 1e-12
+    # plausible mistakes give other values on the same inputs
+    rpp = (exv * (alpha * exv)).sum(axis=0)
+    rps = (exv * (gamma * exv)).sum(axis=0)
+    wrong_norm = rpp[hit] / np.sqrt(den[hit])          # |ex_vP| instead of |ex_vP|^2
+    assert not np.allclose(wrong_norm, alpha, rtol=1e-3)
+    assert not np.allclose(-rps[hit] / den[hit], gamma, rtol=1e-3)   # vP - v for vS
+    assert not np.allclose(rps[hit] / den[hit], alpha, rtol=1e-3)    # PP and PS swapped
+    o.close()
