@@ -69,6 +69,7 @@ struct erm_ctx {
     bool have_strips = false;
     float* wavelet_neg = nullptr;    // -wavelet, for the time-reversed source
     float* ximg[3] = {nullptr};      // cross-correlation images tp, pp, ps (padded grid)
+    float* decbuf = nullptr;         // one divergence / curl volume (allocated on first use)
     double vmax_loaded = 0.0;
     double pml_extra_stress = 0.0, pml_extra_velocity = 0.0;  // avg extra bytes per cell-update
     std::string err;

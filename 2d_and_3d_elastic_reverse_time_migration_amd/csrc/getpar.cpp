@@ -107,6 +107,7 @@ void params_default(erm_params* p) {
     p->add_receiver_bool = 1;
     p->receiver_num_z = 1;
     p->shot_num_x = p->shot_num_y = p->shot_num_z = 1;
+    p->xcd_map = 2;  // XCD y-band tile order: measured fastest at 296x296x168 with C-PML
 }
 
 int parse_par(int argc, const char* const* argv, erm_params* p, std::string* err) {
@@ -220,6 +221,7 @@ int parse_par(int argc, const char* const* argv, erm_params* p, std::string* err
     t.geti("tile_x", &p->tile_x);
     t.geti("tile_y", &p->tile_y);
     t.geti("zchunk", &p->zchunk);
+    t.geti("xcd_map", &p->xcd_map);
     if (p->fd_order != 8 && p->fd_order != 12) {
         if (err) *err = "fd_order must be 8 or 12";
         return ERM_E_PARAM;

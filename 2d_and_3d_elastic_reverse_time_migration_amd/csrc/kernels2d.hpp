@@ -43,6 +43,9 @@ struct Ex2 {
     int* time;
     float *amp, *tp, *vxp, *vzp, *img_tp, *img_pp, *img_ps;
     float *rvxp, *rvzp;  // receiver vector P at the imaging step (angle gathers)
+    int* tp_time;        // tp excitation: step of max |tp| (minus tp_shift) and the signed tp there
+    float* tp_amp;
+    int tp_shift;
 };
 
 __device__ __forceinline__ int clamp2(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -207,15 +210,44 @@ __global__ __launch_bounds__(TZ* TXR) void k2_velocity(F2 f, M2 m, G2 g, Coef2 c
             ex.time[n] = it; ex.amp[n] = amp; ex.tp[n] = ttp[a_][b_];
             ex.vxp[n] = nxp; ex.vzp[n] = nzp;
         }
+        // caculate_ex_tp_time_new (elastic_2D_kernel_3.cu:50-69): its own step, signed amplitude
+        const float tpc = ttp[a_][b_];
+        if (fabsf(ex.tp_amp[n]) < fabsf(tpc)) { ex.tp_time[n] = it - ex.tp_shift; ex.tp_amp[n] = tpc; }
     } else if constexpr (MODE == 2) {
+        if (ex.tp_time[n] == it) {  // imaging_correlation_ex_2D (:542-558)
+            const float ta = ex.tp_amp[n];
+            ex.img_tp[n] = ta != 0.f ? ttp[a_][b_] / ta : 0.f;
+        }
         if (ex.time[n] == it) {
-            const float etp = ex.tp[n];
-            ex.img_tp[n] = etp != 0.f ? ttp[a_][b_] / etp : 0.f;
             const float e1 = ex.vxp[n], e2 = ex.vzp[n], den = e1 * e1 + e2 * e2;
             ex.img_pp[n] = den > 0.f ? (e1 * nxp + e2 * nzp) / den : 0.f;
             ex.img_ps[n] = den > 0.f ? (e1 * (nvx - nxp) + e2 * (nvz - nzp)) / den : 0.f;
             ex.rvxp[n] = nxp; ex.rvzp[n] = nzp;
         }
+    }
+}
+
+// The _new variants caculate_ex_x_z_new / imaging_inner_product_ex_2D_new
+// (elastic_2D_kernel_3.cu:91-109,582-605), run after the velocity update of
+// step it (the neighbours' values of the step are complete only then): at
+// interior cells whose excitation step is it, vzp is the mean over
+// (i, k+-1), (i+-1, k) -- MODE 1 in the stored excitation component, MODE 2 in
+// the receiver P and S motion of the PP / PS images.
+template <int MODE>
+__global__ void k2_ex_avg(F2 f, Ex2 ex, G2 g, int it, int R) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (k < R || k >= g.nnz - R || i < R || i >= g.nnx - R) return;
+    const long n = (long)i * g.pz + k;
+    if (ex.time[n] != it) return;
+    auto avg = [&](const float* a) { return (a[n + 1] + a[n - 1] + a[n + g.pz] + a[n - g.pz]) * 0.25f; };
+    if constexpr (MODE == 1) {
+        ex.vzp[n] = avg(f.vzp);
+    } else {
+        const float nxp = f.vxp[n], nzp = avg(f.vzp);
+        const float nxs = f.vx[n] - nxp, nzs = avg(f.vz) - nzp;
+        const float e1 = ex.vxp[n], e2 = ex.vzp[n], den = e1 * e1 + e2 * e2;
+        ex.img_pp[n] = den > 0.f ? (e1 * nxp + e2 * nzp) / den : 0.f;
+        ex.img_ps[n] = den > 0.f ? (e1 * nxs + e2 * nzs) / den : 0.f;
     }
 }
 
@@ -383,11 +415,14 @@ __global__ __launch_bounds__(TZ* TXR) void k2a_s2(F2 l, G2 g, Coef2 cf, const fl
     l.vz[n] = a * b * l.vz[n] - g.cz * dpz<R>(w1, c, a_, b_) - g.cx * dpx<R>(w2, c, a_, b_);
 }
 
-// div / curl decomposition (decom, elastic_2D_kernel_2.cu:621-666)
+// div / curl decomposition (decom, elastic_2D_kernel_2.cu:621-666); with
+// svp / svs the decom_new scaling (:671-791): div times vp at the stress node,
+// curl times the mean of vs over (i,k), (i,k+1), (i+1,k), (i+1,k+1).
 template <int R, int TXR>
 __global__ __launch_bounds__(TZ* TXR) void k2_decom(const float* __restrict__ vx, const float* __restrict__ vz,
                                                     float* __restrict__ dv, float* __restrict__ cu, G2 g, Coef2 cf,
-                                                    float idx, float idz) {
+                                                    float idx, float idz, const float* __restrict__ svp,
+                                                    const float* __restrict__ svs) {
     __shared__ float tvx[TXR + 2 * R][TZ + 2 * R];
     __shared__ float tvz[TXR + 2 * R][TZ + 2 * R];
     const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
@@ -402,8 +437,12 @@ __global__ __launch_bounds__(TZ* TXR) void k2_decom(const float* __restrict__ vx
 #pragma unroll
     for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
     const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
-    dv[n] = idx * dpx<R>(tvx, c, a_, b_) + idz * dmz<R>(tvz, c, a_, b_);
-    cu[n] = idz * dpz<R>(tvx, c, a_, b_) - idx * dmx<R>(tvz, c, a_, b_);
+    float d = idx * dpx<R>(tvx, c, a_, b_) + idz * dmz<R>(tvz, c, a_, b_);
+    float r = idz * dpz<R>(tvx, c, a_, b_) - idx * dmx<R>(tvz, c, a_, b_);
+    if (svp) d *= svp[n];
+    if (svs) r *= (svs[n] + svs[n + 1] + svs[n + g.pz] + svs[n + 1 + g.pz]) * 0.25f;
+    dv[n] = d;
+    cu[n] = r;
 }
 
 // receivers: rec [2][lt][nrec]

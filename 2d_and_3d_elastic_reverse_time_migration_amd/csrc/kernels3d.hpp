@@ -858,6 +858,43 @@ struct StripFields {
     atomicAdd(&vz[n], rec[2 * comp_stride + it_off + r]);
 }
 
+// P/S decomposition by divergence and curl of v, the 3D form of decom
+// (elastic_2D_kernel_2.cu:563-669) with the staggered positions above:
+//   q = 0: div    at (i, j, k)          = D+x vx/dx + D+y vy/dy + D-z vz/dz
+//   q = 1: curl_x at (i, j-1/2, k+1/2)  = D-y vz/dy - D+z vy/dz
+//   q = 2: curl_y at (i-1/2, j, k+1/2)  = D+z vx/dz - D-x vz/dx
+//   q = 3: curl_z at (i-1/2, j-1/2, k)  = D-x vy/dx - D-y vx/dy
+// The outer R cells get 0.  Read on demand (field access, snapshots), not in
+// the time loop, so a plain per-cell stencil through the caches serves.
+template <int R>
+__global__ void k_decom(const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
+                        float* __restrict__ out, int q, int nnx, int nny, int nnz, int px, long pxy, Coef cf,
+                        float ix, float iy, float iz) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= (long)nnx * nny * nnz) return;
+    const int i = (int)(t % nnx), j = (int)((t / nnx) % nny), k = (int)(t / ((long)nnx * nny));
+    const long n = (long)k * pxy + (long)j * px + i;
+    if (i < R || i >= nnx - R || j < R || j >= nny - R || k < R || k >= nnz - R) { out[n] = 0.f; return; }
+    auto dp = [&](const float* f, long s) {
+        float a = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) a += cf.c[m] * (f[n + m * s] - f[n - (m - 1) * s]);
+        return a;
+    };
+    auto dm = [&](const float* f, long s) {
+        float a = 0.f;
+#pragma unroll
+        for (int m = 1; m <= R; ++m) a += cf.c[m] * (f[n + (m - 1) * s] - f[n - m * s]);
+        return a;
+    };
+    float v;
+    if (q == 0) v = ix * dp(vx, 1) + iy * dp(vy, px) + iz * dm(vz, pxy);
+    else if (q == 1) v = iy * dm(vz, px) - iz * dp(vy, pxy);
+    else if (q == 2) v = iz * dp(vx, pxy) - ix * dm(vz, 1);
+    else v = ix * dm(vy, 1) - iy * dm(vx, px);
+    out[n] = v;
+}
+
 // Plain streaming copy (float4), the calibration kernel for HBM counters and
 // the practical bandwidth ceiling: reads and writes n4*16 bytes.
 [[maybe_unused]] static __global__ void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, long n4) {

@@ -81,7 +81,9 @@ struct erm2d_ctx {
     float* W[3] = {nullptr};
     float* dv = nullptr;
     float* cu = nullptr;
-    struct Mod { float *M = nullptr, *U = nullptr, *B = nullptr, *kP[3] = {0}, *kS[3] = {0}, *decay[3] = {0}; bool have = false; } mod[2];
+    struct Mod { float *M = nullptr, *U = nullptr, *B = nullptr, *kP[3] = {0}, *kS[3] = {0}, *decay[3] = {0};
+                 float *vp = nullptr, *vs = nullptr; bool have = false; } mod[2];
+    int last_model = 1;     // model of the last pass (decom_new scaling)
     float* att = nullptr;
     std::vector<float> wavelet;
     long src = -1;
@@ -90,6 +92,8 @@ struct erm2d_ctx {
     float* rec2 = nullptr;  // second record buffer (residual / Born data)
     int* ex_time = nullptr;
     float* ex[6] = {nullptr};  // amp tp vxp vzp, receiver vxp vzp at the imaging step
+    int* ex_tp_time = nullptr; // tp excitation (caculate_ex_tp_time_new)
+    float* ex_tp_amp = nullptr;
     float* img[3] = {nullptr};
     float* hist = nullptr;  // [lt][5][mn]: E (3) then F (2)
     float* dm = nullptr;    // [3][mn]
@@ -161,6 +165,7 @@ Ex2 excite(const erm2d_ctx* c) {
     e.time = c->ex_time; e.amp = c->ex[0]; e.tp = c->ex[1]; e.vxp = c->ex[2]; e.vzp = c->ex[3];
     e.rvxp = c->ex[4]; e.rvzp = c->ex[5];
     e.img_tp = c->img[0]; e.img_pp = c->img[1]; e.img_ps = c->img[2];
+    e.tp_time = c->ex_tp_time; e.tp_amp = c->ex_tp_amp; e.tp_shift = c->p.ex_tp_shift;
     return e;
 }
 dim3 grid2(const erm2d_ctx* c) { return dim3((c->nnz + TZ - 1) / TZ, (c->nnx + TXR - 1) / TXR); }
@@ -196,6 +201,10 @@ void vel_m(erm2d_ctx* c, bool B, int which, int mode, int it, float* Fh) {
 void do_velocity(erm2d_ctx* c, bool B, int which, int mode, int it, float* Fh) {
     if (c->R == 4) vel_m<4>(c, B, which, mode, it, Fh);
     else vel_m<6>(c, B, which, mode, it, Fh);
+    if (c->p.ex_vzp_avg && !Fh && (mode == 1 || mode == 2)) {
+        if (mode == 1) hipLaunchKernelGGL(k2_ex_avg<1>, grid_pt(c), dim3(256), 0, c->stream, state(c, B), excite(c), geo(c), it, c->R);
+        else hipLaunchKernelGGL(k2_ex_avg<2>, grid_pt(c), dim3(256), 0, c->stream, state(c, B), excite(c), geo(c), it, c->R);
+    }
 }
 void born_stress_src(erm2d_ctx* c, const float* Eh) {
     dim3 g = grid_pt(c);
@@ -227,6 +236,8 @@ int zero_state(erm2d_ctx* c, bool B) {
 }
 int zero_excitation(erm2d_ctx* c) {
     hipLaunchKernelGGL(k2_filli, dim3(512), dim3(256), 0, c->stream, c->ex_time, (long)c->ncell, -1);
+    hipLaunchKernelGGL(k2_filli, dim3(512), dim3(256), 0, c->stream, c->ex_tp_time, (long)c->ncell, -1);
+    if (fillf(c, c->ex_tp_amp, c->ncell, 0.f)) return ERM_E_DEVICE;
     for (auto* e : c->ex) if (fillf(c, e, c->ncell, 0.f)) return ERM_E_DEVICE;
     return 0;
 }
@@ -236,6 +247,7 @@ float wavelet_at(const erm2d_ctx* c, int it) { return (it >= 0 && it < (int)c->w
 int run(erm2d_ctx* c, int pass, int it_begin, int nsteps, float* recbuf) {
     const int which = pass == ERM2D_PASS_MODEL ? 0 : 1;
     if (!c->mod[which].have) return c->fail(ERM_E_STATE, "%s model not loaded", which ? "background" : "true");
+    c->last_model = which;
     const long comp = (long)c->p.lt * c->p.nrec;
     const int rg = (c->p.nrec + 255) / 256;
     c->qsign = pass == ERM2D_PASS_RECEIVER && c->p.qcomp ? -1.f : 1.f;
@@ -277,7 +289,8 @@ void free2(erm2d_ctx* c) {
     for (auto& w : c->W) F(w);
     F(c->dv); F(c->cu);
     for (auto& m : c->mod) { F(m.M); F(m.U); F(m.B); for (int l = 0; l < 3; ++l) { F(m.kP[l]); F(m.kS[l]); F(m.decay[l]); } }
-    F(c->att); F(c->rec_idx); F(c->rec); F(c->rec2); F(c->ex_time);
+    F(c->att); F(c->rec_idx); F(c->rec); F(c->rec2); F(c->ex_time); F(c->ex_tp_time); F(c->ex_tp_amp);
+    for (auto& m : c->mod) { F(m.vp); F(m.vs); }
     for (auto& e : c->ex) F(e);
     for (auto& e : c->img) F(e);
     F(c->hist); F(c->dm); F(c->grad); F(c->illum);
@@ -361,6 +374,8 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
     CK2(dmal(c, &c->cu, c->ncell));
     CK2(dmal(c, &c->att, c->ncell));
     CK2(dmal(c, &c->ex_time, c->ncell));
+    CK2(dmal(c, &c->ex_tp_time, c->ncell));
+    CK2(dmal(c, &c->ex_tp_amp, c->ncell));
     for (auto& e : c->ex) CK2(dmal(c, &e, c->ncell));
     for (auto& e : c->img) CK2(dmal(c, &e, c->ncell));
     // damping: make_attenuation_new (zzzzz:238-325), coe * s^3
@@ -444,7 +459,7 @@ int erm2d_load_model(erm2d_ctx* c, int which, const float* vp, const float* vs, 
     if (c->L > 0 && (!qp || !qs)) return c->fail(ERM_E_PARAM, "qp/qs required when nsls > 0");
     HC(c, hipSetDevice(c->device));
     const erm2d_params& p = c->p;
-    std::vector<float> hM(c->ncell, 0.f), hU(c->ncell, 0.f), hB(c->ncell, 0.f);
+    std::vector<float> hM(c->ncell, 0.f), hU(c->ncell, 0.f), hB(c->ncell, 0.f), hvp(c->ncell, 0.f), hvs(c->ncell, 0.f);
     std::vector<float> hkP[3], hkS[3], hd[3];
     for (int l = 0; l < c->L; ++l) { hkP[l].assign(c->ncell, 0.f); hkS[l].assign(c->ncell, 0.f); hd[l].assign(c->ncell, 0.f); }
     for (int i = 0; i < c->nnx; ++i)
@@ -458,6 +473,8 @@ int erm2d_load_model(erm2d_ctx* c, int which, const float* vp, const float* vs, 
             hM[n] = (float)(r * a * a * q.fM);
             hU[n] = (float)(r * b * b * q.fU);
             hB[n] = (float)(1.0 / r);
+            hvp[n] = vp[s];
+            hvs[n] = vs[s];
             for (int l = 0; l < c->L; ++l) {
                 hkP[l][n] = (float)(r * a * a * q.gP[l]);
                 hkS[l][n] = (float)(r * b * b * q.gS[l]);
@@ -466,11 +483,14 @@ int erm2d_load_model(erm2d_ctx* c, int which, const float* vp, const float* vs, 
         }
     auto& m = c->mod[which];
     if (!m.M) {
-        if (dmal(c, &m.M, c->ncell) || dmal(c, &m.U, c->ncell) || dmal(c, &m.B, c->ncell)) return ERM_E_NOMEM;
+        if (dmal(c, &m.M, c->ncell) || dmal(c, &m.U, c->ncell) || dmal(c, &m.B, c->ncell) ||
+            dmal(c, &m.vp, c->ncell) || dmal(c, &m.vs, c->ncell))
+            return ERM_E_NOMEM;
         for (int l = 0; l < c->L; ++l)
             if (dmal(c, &m.kP[l], c->ncell) || dmal(c, &m.kS[l], c->ncell) || dmal(c, &m.decay[l], c->ncell)) return ERM_E_NOMEM;
     }
-    if (upload(c, m.M, hM) || upload(c, m.U, hU) || upload(c, m.B, hB)) return ERM_E_DEVICE;
+    if (upload(c, m.M, hM) || upload(c, m.U, hU) || upload(c, m.B, hB) || upload(c, m.vp, hvp) || upload(c, m.vs, hvs))
+        return ERM_E_DEVICE;
     for (int l = 0; l < c->L; ++l)
         if (upload(c, m.kP[l], hkP[l]) || upload(c, m.kS[l], hkS[l]) || upload(c, m.decay[l], hd[l])) return ERM_E_DEVICE;
     m.have = true;
@@ -544,26 +564,33 @@ int erm2d_get_field(erm2d_ctx* c, int field, float* dst) {
     if (!c || !dst) return ERM_E_PARAM;
     HC(c, hipSetDevice(c->device));
     if (field >= 0 && field < 8) return down2(c, dst, c->fA[field]);
-    if (field == ERM2D_DIV || field == ERM2D_CURL) {
+    if (field == ERM2D_DIV || field == ERM2D_CURL || field == ERM2D_PDIV || field == ERM2D_SCURL) {
         G2 g = geo(c);
+        const bool scaled = field == ERM2D_PDIV || field == ERM2D_SCURL;
+        const auto& md = c->mod[c->last_model];
+        if (scaled && !md.have) return c->fail(ERM_E_STATE, "model of the last pass not loaded");
+        const float* svp = scaled ? md.vp : nullptr;
+        const float* svs = scaled ? md.vs : nullptr;
         if (c->R == 4)
             hipLaunchKernelGGL((k2_decom<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, c->fA[0], c->fA[1], c->dv,
-                               c->cu, g, c->coef, (float)(1.0 / c->p.dx), (float)(1.0 / c->p.dz));
+                               c->cu, g, c->coef, (float)(1.0 / c->p.dx), (float)(1.0 / c->p.dz), svp, svs);
         else
             hipLaunchKernelGGL((k2_decom<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, c->fA[0], c->fA[1], c->dv,
-                               c->cu, g, c->coef, (float)(1.0 / c->p.dx), (float)(1.0 / c->p.dz));
+                               c->cu, g, c->coef, (float)(1.0 / c->p.dx), (float)(1.0 / c->p.dz), svp, svs);
         HC(c, hipGetLastError());
-        return down2(c, dst, field == ERM2D_DIV ? c->dv : c->cu);
+        return down2(c, dst, field == ERM2D_DIV || field == ERM2D_PDIV ? c->dv : c->cu);
     }
-    if (field == ERM2D_EX_TIME) {
+    if (field == ERM2D_EX_TIME || field == ERM2D_EX_TP_TIME) {
         std::vector<int> t(c->ncell);
-        HC(c, hipMemcpyAsync(t.data(), c->ex_time, c->ncell * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HC(c, hipMemcpyAsync(t.data(), field == ERM2D_EX_TIME ? c->ex_time : c->ex_tp_time, c->ncell * sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
         HC(c, hipStreamSynchronize(c->stream));
         for (int i = 0; i < c->nnx; ++i)
             for (int k = 0; k < c->nnz; ++k) dst[(size_t)i * c->nnz + k] = (float)t[(size_t)i * c->pz + k];
         return 0;
     }
     if (field == ERM2D_EX_AMP) return down2(c, dst, c->ex[0]);
+    if (field == ERM2D_EX_TP_AMP) return down2(c, dst, c->ex_tp_amp);
     if (field >= ERM2D_EX_TP && field <= ERM2D_R_VZP) return down2(c, dst, c->ex[1 + field - ERM2D_EX_TP]);
     if (field >= ERM2D_IMG_TP && field <= ERM2D_IMG_PS) return down2(c, dst, c->img[field - ERM2D_IMG_TP]);
     return c->fail(ERM_E_PARAM, "unknown field %d", field);

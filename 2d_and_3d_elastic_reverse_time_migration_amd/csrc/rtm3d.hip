@@ -430,12 +430,29 @@ void free_all(erm_ctx* c) {
     for (auto& q : c->fr) F(q);
     F(c->strip_idx); F(c->strips); F(c->wavelet_neg);
     for (auto& q : c->ximg) F(q);
+    F(c->decbuf);
     for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+}
+
+// divergence (q = 0) or curl component q of the current velocity into decbuf
+int decompose(erm_ctx* c, int q) {
+    if (!c->decbuf && dmalloc(c, &c->decbuf, c->ncell)) return ERM_E_NOMEM;
+    const long tot = (long)c->nnx * c->nny * c->nnz;
+    const dim3 grd((unsigned)((tot + 255) / 256));
+    const float ix = (float)(1.0 / c->p.dx), iy = (float)(1.0 / c->p.dy), iz = (float)(1.0 / c->p.dz);
+    if (c->R == 4)
+        hipLaunchKernelGGL(k_decom<4>, grd, dim3(256), 0, c->stream, c->f[0], c->f[1], c->f[2], c->decbuf, q, c->nnx,
+                           c->nny, c->nnz, c->px, c->pxy, c->coef, ix, iy, iz);
+    else
+        hipLaunchKernelGGL(k_decom<6>, grd, dim3(256), 0, c->stream, c->f[0], c->f[1], c->f[2], c->decbuf, q, c->nnx,
+                           c->nny, c->nnz, c->px, c->pxy, c->coef, ix, iy, iz);
+    HIPC(c, hipGetLastError());
+    return 0;
 }
 
 // download a padded-grid volume into a dense nnx*nny*nnz host array
@@ -977,6 +994,10 @@ int erm_get_field(erm_ctx* c, int field, float* dst) {
     }
     if (field >= ERM_F_EX_AMP && field <= ERM_F_EX_VZP) return download(c, c->ex[field - ERM_F_EX_AMP], dst);
     if (field >= ERM_F_IMG_TP && field <= ERM_F_IMG_PS) return download(c, c->img[field - ERM_F_IMG_TP], dst);
+    if (field >= ERM_F_DIV && field <= ERM_F_CURLZ) {
+        int rc = decompose(c, field - ERM_F_DIV);
+        return rc ? rc : download(c, c->decbuf, dst);
+    }
     return c->fail(ERM_E_PARAM, "unknown field %d", field);
 }
 

@@ -11,7 +11,7 @@ PASS_MODEL, PASS_SOURCE, PASS_RECEIVER = 0, 1, 2
 FIELDS = [
     "vx", "vy", "vz", "txx", "tyy", "tzz", "txy", "txz", "tyz", "tp", "vxp", "vyp", "vzp",
     "vxs", "vys", "vzs", "ex_time", "ex_amp", "ex_tp", "ex_vxp", "ex_vyp", "ex_vzp",
-    "img_tp", "img_pp", "img_ps",
+    "img_tp", "img_pp", "img_ps", "div", "curlx", "curly", "curlz",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 

@@ -94,7 +94,11 @@ typedef struct erm_params {
                                        32-bit offsets: (zchunk + 2R) x-y planes must span < 4 GiB, else
                                        erm_create fails (ERM_E_PARAM; ERM_E_SIZE if one plane is too big) */
     int   pitch_align;  /* row pitch multiple in floats (0: 32 = 128 B; 1: dense rows) */
-    int   xcd_map;      /* 0: plain workgroup order (default, measured faster); 1: XCD-aware tile mapping */
+    int   xcd_map;      /* workgroup -> tile order (results do not depend on it): 0 plain (x, y, z chunk);
+                           1 each XCD a contiguous chunk of tiles; 2 (default) each XCD a band of grid
+                           rows of the region-split tile lists, its tiles in (z chunk, y, x) order --
+                           measured fastest with C-PML (19.8 vs 19.4 Gcell/s at 296x296x168, L2 misses
+                           -40%); without the region split 2 acts as 0 */
     int   cpml_split;   /* 0 (default): C-PML steps launch the plain kernel on tiles without C-PML cells and
                            the C-PML kernel on the rest; 1: the C-PML kernel on every tile (same results) */
     int   region_streams; /* 1: the two region launches of a step run concurrently on two streams (they
@@ -112,6 +116,12 @@ enum erm_field {
     ERM_F_EX_TIME, ERM_F_EX_AMP, ERM_F_EX_TP,    /* ex_time returned as float; -1 = never excited */
     ERM_F_EX_VXP, ERM_F_EX_VYP, ERM_F_EX_VZP,
     ERM_F_IMG_TP, ERM_F_IMG_PP, ERM_F_IMG_PS,
+    /* P/S decomposition by divergence and curl of v (read only; the 3D form of
+     * decom, 2D_elastic_RTM/elastic_2D_kernel_2.cu:563-669): div at the stress
+     * node (i,j,k) = D+x vx/dx + D+y vy/dy + D-z vz/dz; curl_x = D-y vz/dy - D+z vy/dz
+     * at the syz node, curl_y = D+z vx/dz - D-x vz/dx at the sxz node,
+     * curl_z = D-x vy/dx - D-y vx/dy at the sxy node; 0 on the outer R cells. */
+    ERM_F_DIV, ERM_F_CURLX, ERM_F_CURLY, ERM_F_CURLZ,
     ERM_F_COUNT
 };
 

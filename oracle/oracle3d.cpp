@@ -464,8 +464,31 @@ struct Oracle3D {
     }
     void zero_images() { for (auto* v : {&img_tp, &img_pp, &img_ps}) std::fill(v->begin(), v->end(), (T)0); }
 
+    /* P/S decomposition by divergence and curl (the 3D form of decom,
+     * elastic_2D_kernel_2.cu:621-666, with the y positions of the 3D grid):
+     * q = 0 div at (i,j,k); 1 curl_x at the syz node; 2 curl_y at the sxz node;
+     * 3 curl_z at the sxy node; interior cells, 0 elsewhere. */
+    void decom(int q, double* dst) const {
+        const T* vx = f[0].data(); const T* vy = f[1].data(); const T* vz = f[2].data();
+        const ptrdiff_t sX = 1, sY = nnx, sZ = (ptrdiff_t)nnx * nny;
+        const T ix = (T)(1.0 / p.dx), iy = (T)(1.0 / p.dy), iz = (T)(1.0 / p.dz);
+        for (size_t n = 0; n < N; ++n) dst[n] = 0.0;
+        for (int k = R; k < nnz - R; ++k)
+            for (int j = R; j < nny - R; ++j)
+                for (int i = R; i < nnx - R; ++i) {
+                    size_t n = id(i, j, k);
+                    T v;
+                    if (q == 0) v = ix * dp(vx + n, sX) + iy * dp(vy + n, sY) + iz * dm(vz + n, sZ);
+                    else if (q == 1) v = iy * dm(vz + n, sY) - iz * dp(vy + n, sZ);
+                    else if (q == 2) v = iz * dp(vx + n, sZ) - ix * dm(vz + n, sX);
+                    else v = ix * dm(vy + n, sX) - iy * dm(vx + n, sY);
+                    dst[n] = (double)v;
+                }
+    }
+
     int get_field(int fid, double* dst) const {
         if (fid >= 0 && fid < 13) { for (size_t n = 0; n < N; ++n) dst[n] = (double)f[fid][n]; return 0; }
+        if (fid >= ERM_F_DIV && fid <= ERM_F_CURLZ) { decom(fid - ERM_F_DIV, dst); return 0; }
         if (fid >= ERM_F_VXS && fid <= ERM_F_VZS) {
             int c3 = fid - ERM_F_VXS;
             for (size_t n = 0; n < N; ++n) dst[n] = (double)(f[c3][n] - f[10 + c3][n]);

@@ -1,0 +1,143 @@
+"""FP64 oracle fixture of the benchmark's 3D shot at its full grid, and the
+script that writes it (python tests/golden/config2.py; about 5 minutes on 8
+cores).
+
+The benchmark workload (bench.py: 256x256x128 model, 20-cell C-PML,
+296x296x168 padded grid, radius 4, dt 0.8 ms, 15 Hz, 128x128 receivers, the
+seeded dipping-layer model and its box-smoothed migration model) with the
+time axis cut to NT = 500 steps per pass, migrated by the FP64 oracle: model
+pass in the true model, source pass with excitation, receiver pass with
+imaging.  Kept (float32 / int16, ~1 MB compressed):
+  slice_*    the inline slice y = ny/2 of the model region [nx][nz]: ex_time,
+             ex_amp, and the tp / PP / PS images;
+  grid_*     the same quantities on a lattice every 8 cells in x and y and
+             every 4 in z of the model region;
+  norm_*     L2 norms of the three whole images over cells with
+             ex_amp > 1e-3 max (FP64);
+  vz_slice   vz of the padded grid's slice y = nny/2 after the receiver pass.
+The GPU test (tests/test_gpu_config2.py) and bench.py's error statement run
+the same shot through liberm.so and compare with these arrays.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+FIXTURE = os.path.join(HERE, "config2_nt500.npz")
+NT = 500
+STRIDE = (8, 8, 4)
+
+
+def shot_inputs(erm, nt=NT):
+    """(params, true model, migration model) of the benchmark shot with nt steps per pass."""
+    import bench
+    p = bench.make_params(erm, nt, erm.BOUNDARY_CPML)
+    vp, vs, rho = erm.make_model3d(1, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, 2)
+    vp_s = bench.smooth_box(vp)
+    vs_s = (vp_s / 1.732).astype(np.float32)
+    rho_s = (230.0 * vp_s.astype(np.float64) ** 0.25).astype(np.float32)
+    return p, (vp, vs, rho), (vp_s, vs_s, rho_s)
+
+
+def sample(a3, p):
+    """(slice, lattice) of a model-region array [nx][ny][nz]."""
+    sx, sy, sz = STRIDE
+    return (np.ascontiguousarray(a3[:, p.ny // 2, :]),
+            np.ascontiguousarray(a3[::sx, ::sy, ::sz]))
+
+
+def model_region(field_zyx, p):
+    """padded (z, y, x) volume -> model region [nx][ny][nz]"""
+    cut = field_zyx[p.bu:p.bu + p.nz, p.bf:p.bf + p.ny, p.bl:p.bl + p.nx]
+    return np.ascontiguousarray(cut.transpose(2, 1, 0))
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def gpu_shot(erm, nt=NT, **launch):
+    """The same shot through liberm.so: (ex_time, ex_amp, [tp, pp, ps] images
+    over the model region [nx][ny][nz], vz of the padded slice y = nny/2)."""
+    p, true, mig = shot_inputs(erm, nt)
+    for k, v in launch.items():
+        setattr(p, k, v)
+    g = erm.RTM3D(p, 0)
+    try:
+        g.load_model(0, *true)
+        g.load_model(1, *mig)
+        g.forward_real(want_record=False)
+        g.forward_smoothed()
+        g.backward_smoothed()
+        et = model_region(g.get_field("ex_time"), p)
+        ea = model_region(g.get_field("ex_amp"), p)
+        imgs = g.get_images()
+        vz = g.get_field("vz")
+    finally:
+        g.close()
+    return p, et, ea, imgs, vz[:, vz.shape[1] // 2, :]
+
+
+def compare(fx, p, et, ea, imgs, vz_slice):
+    """Errors of a shot against the fixture: the fraction of excited cells
+    (ex_amp > 1e-3 max) whose excitation step differs, and the rel L2 of each
+    image over excited cells with the same step (slice and lattice together),
+    of the receiver-pass vz slice."""
+    et_s, et_g = sample(et, p)
+    amp = np.concatenate([fx["slice_ex_amp"].ravel(), fx["grid_ex_amp"].ravel()]).astype(np.float64)
+    t_ref = np.concatenate([fx["slice_ex_time"].ravel(), fx["grid_ex_time"].ravel()]).astype(np.int64)
+    t_got = np.concatenate([et_s.ravel(), et_g.ravel()]).astype(np.int64)
+    excited = amp > 1e-3 * amp.max()
+    same = excited & (t_ref == t_got)
+    out = {"nt": int(fx["nt"]), "ex_time_mismatch_frac": float(1.0 - same.sum() / max(excited.sum(), 1)),
+           "cells_compared": int(same.sum())}
+    for name, a in zip(("tp", "pp", "ps"), imgs):
+        s, g = sample(a, p)
+        got = np.concatenate([s.ravel(), g.ravel()])
+        ref = np.concatenate([fx["slice_img_" + name].ravel(), fx["grid_img_" + name].ravel()])
+        out["img_%s_rel_l2" % name] = rel_l2(got[same], ref[same])
+    out["vz_rel_l2"] = rel_l2(vz_slice, fx["vz_slice"])
+    return out
+
+
+def main():
+    import importlib
+    erm = importlib.import_module("2d_and_3d_elastic_reverse_time_migration_amd")
+    import oracle3d
+    oracle3d.set_threads(os.cpu_count() or 1)
+    p, true, mig = shot_inputs(erm)
+    t0 = time.time()
+    o = oracle3d.Oracle3D(p, 64)
+    o.load_model(0, *true)
+    o.load_model(1, *mig)
+    o.rtm_shot()
+    print(f"oracle shot: {time.time() - t0:.0f} s", flush=True)
+    out = {}
+    et = model_region(o.get_field("ex_time"), p)
+    ea = model_region(o.get_field("ex_amp"), p)
+    imgs = o.images()
+    for name, a in (("ex_time", et), ("ex_amp", ea), ("img_tp", imgs[0]), ("img_pp", imgs[1]), ("img_ps", imgs[2])):
+        s, g = sample(a, p)
+        dt = np.int16 if name == "ex_time" else np.float32
+        out["slice_" + name] = s.astype(dt)
+        out["grid_" + name] = g.astype(dt)
+    mask = ea > 1e-3 * ea.max()
+    for name, a in zip(("tp", "pp", "ps"), imgs):
+        out["norm_" + name] = np.float64(np.linalg.norm(a[mask]))
+    vz = o.get_field("vz")
+    out["vz_slice"] = vz[:, vz.shape[1] // 2, :].astype(np.float32)
+    out["nt"] = np.int32(NT)
+    np.savez_compressed(FIXTURE, **out)
+    print(f"wrote {FIXTURE} ({os.path.getsize(FIXTURE) / 1e6:.2f} MB)", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,15 +50,21 @@ def test_gpu2d_wavefields_match_oracle(erm, oracle2d_mod, L, R):
         ref = o.get_field(k)
         e, e32 = cases.rel_l2(g.get_field(k), ref), cases.rel_l2(o32.get_field(k), ref)
         assert e < max(1e-4, 10 * e32), (k, e, e32)
-    for k in ["div", "curl"]:
+    for k in ["div", "curl", "pdiv", "scurl"]:   # decom, decom_new (true model: last pass)
         e = cases.rel_l2(g.get_field(k), o.get_field(k))
         assert e < 1e-5, (k, e)
     g.close()
 
 
-def test_gpu2d_rtm_images_match_oracle(erm, oracle2d_mod):
+@pytest.mark.parametrize("avg,shift", [(0, 0), (1, 7)])
+def test_gpu2d_rtm_images_match_oracle(erm, oracle2d_mod, avg, shift):
+    """Excitation RTM: PP/PS at the step of max |vP| (vzp of the _new variants
+    averaged over its four neighbours when avg), tp at its own step of max |tp|
+    (recorded `shift` steps early) over the signed tp there."""
     nz, zi = 50, 32
-    p = cases2d.params(nx=60, nz=nz, b=14, R=4, L=0, lt=700, freq=25.0, coe=250.0, src=(30, 3), rec_dx=1, rec_z=2)
+    _say(f"2D RTM images parity ex_vzp_avg={avg} ex_tp_shift={shift}")
+    p = cases2d.params(nx=60, nz=nz, b=14, R=4, L=0, lt=700, freq=25.0, coe=250.0, src=(30, 3), rec_dx=1, rec_z=2,
+                       ex_vzp_avg=avg, ex_tp_shift=shift)
     z = np.arange(nz)[None, :]
     vp = (np.where(z < zi, 2000.0, 3000.0) * np.ones((p.nx, 1))).astype(np.float32)
     rho = (np.where(z < zi, 2000.0, 2400.0) * np.ones((p.nx, 1))).astype(np.float32)
@@ -88,6 +94,13 @@ def test_gpu2d_rtm_images_match_oracle(erm, oracle2d_mod):
     otp, opp, ops = o.images()
     for a, b in ((pp, opp), (ps, ops)):
         assert cases.rel_l2(a[cut], b[cut]) < 1e-4
+    gtt, ott = g.get_field("ex_tp_time"), o.get_field("ex_tp_time")
+    ota = o.get_field("ex_tp_amp")
+    big = np.abs(ota) > 1e-3 * np.abs(ota).max()
+    assert 1.0 - (gtt == ott)[big].mean() < 0.02
+    mt = (big & (gtt == ott))[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz]
+    assert cases.rel_l2(g.get_field("ex_tp_amp")[big & (gtt == ott)], ota[big & (gtt == ott)]) < 1e-5
+    assert cases.rel_l2(tp[mt], otp[mt]) < 1e-4
     col = np.abs(pp[26:35, 10:]).sum(axis=0)
     assert abs(int(np.argmax(col)) + 10 - zi) <= 2
     g.close()

@@ -85,16 +85,21 @@ def test_gpu_wavefields_match_fp64_oracle(erm, oracle, name):
     rec = g.get_record()
     er = cases.rel_l2(rec[:, :steps], o64.get_record()[:, :steps])
     assert er < 1e-5, er
+    # P/S decomposition by divergence and curl of the final velocity
+    for k in ("div", "curlx", "curly", "curlz"):
+        e = cases.rel_l2(g.get_field(k), o64.get_field(k))
+        assert e < 1e-5, (k, e)
     g.close()
 
 
 def test_gpu_full_config_grid_parity(erm, oracle):
-    """The 256x256x128 model + 20-cell boundary grid (296x296x168 padded) with
-    random fields in every cell: three steps on the GPU against the FP64
-    oracle, so every tile, z chunk and edge of the production launch shape is
-    compared."""
+    """The 256x256x128 model + 20-cell C-PML grid (296x296x168 padded) of the
+    benchmark, with its launch shape (64x4 tiles, z sweep 6, XCD y-band order,
+    region-split plain / C-PML tile lists) and random fields in every cell:
+    three steps on the GPU against the FP64 oracle, so every tile, z chunk,
+    memory-variable slab and edge of the production launch is compared."""
     p = cases.params(nx=256, ny=256, nz=128, b=20, R=4, lt=3, dt=0.8, freq=15.0, src=(128, 128, 1),
-                     rec=(128, 128, 0, 0, 2, 2, 1))
+                     rec=(128, 128, 0, 0, 2, 2, 1), boundary=1, cpml_r=1e-3, tile_y=4, zchunk=6, xcd_map=2)
     vp, vs, rho = erm.make_model3d(1, 256, 256, 128, 10.0, 10.0, 10.0, 2)
     _say(f"full-grid parity: padded {tuple(p.padded_shape)}")
     rng = np.random.default_rng(5)
@@ -105,7 +110,8 @@ def test_gpu_full_config_grid_parity(erm, oracle):
     for k, a in init.items():
         g.set_field(k, a)
     g.step(PASS_MODEL, 0, 3)
-    got = {k: g.get_field(k) for k in ["vx", "vy", "vz", "txx", "tyz", "tp", "vzp"]}
+    got = {k: g.get_field(k) for k in ["vx", "vy", "vz", "txx", "tyy", "tzz", "txy", "txz", "tyz", "tp",
+                                       "vxp", "vyp", "vzp"]}
     g.close()
     o = oracle.Oracle3D(p, 64)
     o.load_model(0, vp, vs, rho)

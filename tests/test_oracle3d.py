@@ -571,3 +571,65 @@ def test_imaging_condition_closed_forms(oracle):
     assert not np.allclose(-rps[hit] / den[hit], gamma, rtol=1e-3)   # vP - v for vS
     assert not np.allclose(rps[hit] / den[hit], alpha, rtol=1e-3)    # PP and PS swapped
     o.close()
+
+
+def test_div_curl_discrete_identities_3d(oracle):
+    """3D divergence / curl decomposition (the 3D form of decom,
+    elastic_2D_kernel_2.cu:621-666).  Staggered fields from random
+    potentials: v = curl A with A on the edge nodes (Ax at syz, Ay at sxz, Az
+    at sxy) has zero divergence; v = grad phi with phi on the stress nodes has
+    zero curl, to roundoff (operators of different axes commute; a D+/D-
+    mix-up breaks it, shown below).  Linear fields give div = a + b + e and
+    curl = (f - g, h - l, m - r) exactly (coefficient sum 1 over the odd
+    weights)."""
+    R = 4
+    p = cases.params(nx=14, ny=13, nz=12, b=6, R=R, lt=2)
+    o = oracle.Oracle3D(p)
+    nnz, nny, nnx = p.padded_shape
+    c = oracle.coefficients(R)
+    rng = np.random.default_rng(11)
+
+    def D(f, axis, plus):
+        out = np.zeros_like(f)
+        for m in range(1, R + 1):
+            if plus:
+                out += c[m] * (np.roll(f, -m, axis) - np.roll(f, m - 1, axis))
+            else:
+                out += c[m] * (np.roll(f, -(m - 1), axis) - np.roll(f, m, axis))
+        return out
+    X, Y, Z = 2, 1, 0  # array axes of (z, y, x) volumes
+    inner = np.zeros((nnz, nny, nnx), bool)
+    inner[2 * R:-2 * R, 2 * R:-2 * R, 2 * R:-2 * R] = True
+    ax, ay, az = (rng.standard_normal((nnz, nny, nnx)) for _ in range(3))
+    vx = D(az, Y, True) / p.dy - D(ay, Z, False) / p.dz
+    vy = D(ax, Z, False) / p.dz - D(az, X, True) / p.dx
+    vz = D(ay, X, True) / p.dx - D(ax, Y, True) / p.dy
+    for name, a in (("vx", vx), ("vy", vy), ("vz", vz)):
+        o.set_field(name, a)
+    dv = o.get_field("div")
+    cmax = max(np.max(np.abs(o.get_field(k)[inner])) for k in ("curlx", "curly", "curlz"))
+    assert np.max(np.abs(dv[inner])) < 1e-12 * cmax
+    phi = rng.standard_normal((nnz, nny, nnx))
+    o.set_field("vx", D(phi, X, False) / p.dx)
+    o.set_field("vy", D(phi, Y, False) / p.dy)
+    o.set_field("vz", D(phi, Z, True) / p.dz)
+    dmax = np.max(np.abs(o.get_field("div")[inner]))
+    for k in ("curlx", "curly", "curlz"):
+        assert np.max(np.abs(o.get_field(k)[inner])) < 1e-12 * dmax, k
+    o.set_field("vz", D(phi, Z, False) / p.dz)   # a D+ / D- mix-up in z
+    assert np.max(np.abs(o.get_field("curlx")[inner])) > 1e-3 * dmax
+    # linear fields at their staggered positions
+    k3, j3, i3 = np.meshgrid(np.arange(nnz), np.arange(nny), np.arange(nnx), indexing="ij")
+    x, y, z = i3 * p.dx, j3 * p.dy, k3 * p.dz
+    a, b, e, f, g, h, l, m, r = 0.3, -0.2, 0.5, 0.7, -0.4, 0.25, 0.6, -0.35, 0.15
+    o.set_field("vx", a * (x - p.dx / 2) + h * (z) + r * (y))           # vx at (i-1/2, j, k)
+    o.set_field("vy", b * (y - p.dy / 2) + g * (z) + m * (x))           # vy at (i, j-1/2, k)
+    o.set_field("vz", e * (z + p.dz / 2) + f * (y) + l * (x))           # vz at (i, j, k+1/2)
+    rin = np.zeros((nnz, nny, nnx), bool)
+    rin[R:-R, R:-R, R:-R] = True
+    np.testing.assert_allclose(o.get_field("div")[rin], a + b + e, rtol=1e-9)
+    np.testing.assert_allclose(o.get_field("curlx")[rin], f - g, rtol=1e-9)
+    np.testing.assert_allclose(o.get_field("curly")[rin], h - l, rtol=1e-9)
+    np.testing.assert_allclose(o.get_field("curlz")[rin], m - r, rtol=1e-9)
+    assert np.all(o.get_field("div")[~rin] == 0)
+    o.close()
