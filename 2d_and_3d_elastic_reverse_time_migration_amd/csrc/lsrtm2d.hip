@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <string>
@@ -24,6 +25,28 @@ using namespace erm2;
 
 namespace {
 thread_local std::string g_err2;
+
+// Staging of erm2d_angle_gathers_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel or a copy (on the host it is a stand-in, not a
+// device address): erm2d_create resolves each device's own address once,
+// with that device current, and the entry point uses that address.
+constexpr size_t kScratchFloats = (size_t)1 << 23;  // 32 MiB
+static __device__ __align__(256) float scratch[kScratchFloats];
+constexpr int kMaxDevices = 64;
+float* g_scratch_ptr[kMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+// the calling thread's current device must be `device`
+int resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return ERM_E_PARAM;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return ERM_E_DEVICE;
+    g_scratch_ptr[device] = static_cast<float*>(ptr);
+    return 0;
+}
 const double kC6[7] = {0.0, 1.2213363647, -0.096931457519, 0.017447662353,
                        -0.0029672895159, 0.00035900539822, -0.00002184781161};
 const double kC4[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0};
@@ -366,6 +389,7 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
         g_err2 = "stream create failed"; delete c; return ERM_E_DEVICE;
     }
     for (auto& e : c->ev) if (hipEventCreate(&e) != hipSuccess) { g_err2 = "event create failed"; free2(c); delete c; return ERM_E_DEVICE; }
+    if (resolve_scratch(device)) { g_err2 = "scratch symbol address lookup failed"; free2(c); delete c; return ERM_E_DEVICE; }
     for (int q = 0; q < 8; ++q) { CK2(dmal(c, &c->fA[q], c->ncell)); CK2(dmal(c, &c->fB[q], c->ncell)); }
     for (int l = 0; l < c->L; ++l)
         for (int q = 0; q < 4; ++q) { CK2(dmal(c, &c->rA[l][q], c->ncell)); CK2(dmal(c, &c->rB[l][q], c->ncell)); }
@@ -728,32 +752,60 @@ int erm2d_adjoint(erm2d_ctx* c, const float* r, float* gout) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// Angle gathers into the device staging d of nd = 2 nang mn + 3 mn floats
+// ([nang][mn] pp, [nang][mn] ps, [mn] ppc, [mn] psc, [mn] gamma), which is
+// cleared first, then copied to the host outputs.
+int angle_gathers_into(erm2d_ctx* c, float* d, size_t nd, int nang, float dang, float* pp, float* ps, float* pp_comp,
+                       float* ps_comp, float* gamma) {
+    const size_t mn = c->mn;
+    int rc = fillf(c, d, nd, 0.f);
+    if (rc) return rc;
+    const G2 g = geo(c);
+    hipLaunchKernelGGL(k2_adcig, dim3((unsigned)std::min<size_t>((mn + 255) / 256, 2048)), dim3(256), 0, c->stream,
+                       excite(c), g, nang, dang, d, d + nang * mn, d + 2 * nang * mn, d + 2 * nang * mn + mn,
+                       d + 2 * nang * mn + 2 * mn);
+    hipError_t e = hipGetLastError();
+    float* outs[5] = {pp, ps, pp_comp, ps_comp, gamma};
+    const size_t off[5] = {0, nang * mn, 2 * nang * mn, 2 * nang * mn + mn, 2 * nang * mn + 2 * mn};
+    const size_t len[5] = {nang * mn, nang * mn, mn, mn, mn};
+    for (int a = 0; a < 5 && e == hipSuccess; ++a)
+        if (outs[a]) e = hipMemcpyAsync(outs[a], d + off[a], len[a] * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return c->fail(ERM_E_DEVICE, "angle gathers: %s", hipGetErrorString(e));
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
 int erm2d_angle_gathers(erm2d_ctx* c, int nang, float dang, float* pp, float* ps, float* pp_comp,
                         float* ps_comp, float* gamma) {
     if (!c) return ERM_E_PARAM;
     if (nang <= 0 || !(dang > 0.f) || nang * dang > 180.f) return c->fail(ERM_E_PARAM, "bad angle bins");
     HC(c, hipSetDevice(c->device));
-    const size_t mn = c->mn;
-    float* d = nullptr;  // [nang][mn] pp, [nang][mn] ps, [mn] ppc, [mn] psc, [mn] gamma
-    const size_t nd = 2 * (size_t)nang * mn + 3 * mn;
+    const size_t nd = 2 * (size_t)nang * c->mn + 3 * c->mn;
+    float* d = nullptr;
     if (dmal(c, &d, nd)) return ERM_E_NOMEM;
-    int rc = fillf(c, d, nd, 0.f);
-    if (!rc) {
-        const G2 g = geo(c);
-        hipLaunchKernelGGL(k2_adcig, dim3((unsigned)std::min<size_t>((mn + 255) / 256, 2048)), dim3(256), 0,
-                           c->stream, excite(c), g, nang, dang, d, d + nang * mn, d + 2 * nang * mn,
-                           d + 2 * nang * mn + mn, d + 2 * nang * mn + 2 * mn);
-        hipError_t e = hipGetLastError();
-        float* outs[5] = {pp, ps, pp_comp, ps_comp, gamma};
-        const size_t off[5] = {0, nang * mn, 2 * nang * mn, 2 * nang * mn + mn, 2 * nang * mn + 2 * mn};
-        const size_t len[5] = {nang * mn, nang * mn, mn, mn, mn};
-        for (int a = 0; a < 5 && e == hipSuccess; ++a)
-            if (outs[a]) e = hipMemcpyAsync(outs[a], d + off[a], len[a] * 4, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) rc = c->fail(ERM_E_DEVICE, "angle gathers: %s", hipGetErrorString(e));
-    }
+    const int rc = angle_gathers_into(c, d, nd, nang, dang, pp, ps, pp_comp, ps_comp, gamma);
     (void)hipFree(d);
     return rc;
+}
+
+int erm2d_angle_gathers_noalloc(erm2d_ctx* c, int nang, float dang, float* pp, float* ps, float* pp_comp,
+                                float* ps_comp, float* gamma) {
+    if (!c) return ERM_E_PARAM;
+    if (nang <= 0 || !(dang > 0.f) || nang * dang > 180.f) return c->fail(ERM_E_PARAM, "bad angle bins");
+    const size_t nd = 2 * (size_t)nang * c->mn + 3 * c->mn;
+    if (nd > kScratchFloats)
+        return c->fail(ERM_E_SIZE, "angle gathers need %zu staging floats (2 nang nx nz + 3 nx nz); the allocation-free "
+                                   "entry point holds at most %zu", nd, kScratchFloats);
+    HC(c, hipSetDevice(c->device));
+    float* d = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+    if (!d) return c->fail(ERM_E_STATE, "scratch address of device %d not resolved", c->device);
+    return angle_gathers_into(c, d, nd, nang, dang, pp, ps, pp_comp, ps_comp, gamma);
 }
 
 int erm2d_illumination(erm2d_ctx* c, int nshots, const int* shots, float* out) {

@@ -148,6 +148,17 @@ int   erm2d_lsrtm_cg(erm2d_ctx* ctx, int nshots, const int* shots, const float* 
  * Any output may be NULL; nang * dang <= 180. */
 int   erm2d_angle_gathers(erm2d_ctx* ctx, int nang, float dang, float* pp, float* ps, float* pp_comp,
                           float* ps_comp, float* gamma);
+/* erm2d_angle_gathers without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as erm2d_angle_gathers; the device staging of the
+ * outputs is a fixed per-GPU array of 2^23 floats (32 MiB, held until the
+ * process ends), so 2 nang nx nz + 3 nx nz must not exceed 8388608
+ * (ERM_E_SIZE otherwise).  The array is shared by every call on that GPU and
+ * keeps the last call's contents (each call clears what it uses): calls that
+ * overlap on one GPU, from different contexts or threads, race. */
+int   erm2d_angle_gathers_noalloc(erm2d_ctx* ctx, int nang, float dang, float* pp, float* ps, float* pp_comp,
+                                  float* ps_comp, float* gamma);
 /* Source illumination of the background model: for each shot, the sum over
  * the time steps it = 0 .. lt-1 of vx^2 + vz^2 of the source wavefield
  * (after the step's velocity update, each component at its own staggered
