@@ -10,7 +10,9 @@
 // then the source forward pass with excitation and the receiver backward
 // pass with imaging, and writes ./result/result-{tp,pp,ps}-{sy}-{sx} and
 // ./someoutput/cut-ex-*-{sy}-{sx} (3D_output_file.cu:155-250,309-428).
-// A shot whose three result files already exist is skipped (resume).
+// A shot whose three result files already exist with their full size is
+// skipped (resume); files are written under a temporary name and renamed, so
+// a run killed while writing leaves no partial result under the final name.
 #include <sys/stat.h>
 
 #include <cstdio>
@@ -19,9 +21,9 @@
 
 #include "../../include/erm.h"
 
-static bool exists(const std::string& p) {
+static bool complete(const std::string& p, long long bytes) {
     struct stat st;
-    return stat(p.c_str(), &st) == 0;
+    return stat(p.c_str(), &st) == 0 && (long long)st.st_size == bytes;
 }
 
 int main(int argc, char** argv) {
@@ -55,8 +57,15 @@ int main(int argc, char** argv) {
                     return 1;
                 }
                 int sx = p.shot_start_x + isx * p.shot_interval_x, sy = p.shot_start_y + isy * p.shot_interval_y;
-                std::string probe = "result/result-ps-" + std::to_string(sy) + "-" + std::to_string(sx);
-                if (p.vsp == 0 && exists(probe)) {
+                int sz = p.shot_start_z + isz * p.shot_interval_z;
+                const long long bytes = 4LL * p.nx * p.ny * p.nz;
+                bool done = true;
+                for (const char* img : {"tp", "pp", "ps"}) {
+                    std::string f = std::string("result/result-") + img + "-" +
+                                    (p.vsp == 0 ? std::to_string(sy) + "-" + std::to_string(sx) : std::to_string(sz));
+                    done = done && complete(f, bytes);
+                }
+                if (done) {
                     fprintf(stderr, "shot isx=%d isy=%d isz=%d: results exist, skipped\n", isx + 1, isy + 1, isz + 1);
                     continue;
                 }

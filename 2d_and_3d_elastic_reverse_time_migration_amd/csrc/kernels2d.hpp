@@ -25,7 +25,28 @@ struct G2 {
     const float* att;    // damping d per cell
     int bl, bu, nx, nz;  // model region
     float qsign;         // sign of the memory-variable term in the stress update (-1: Q-compensated)
+    // C-PML (no reference counterpart; null: damping boundary).  psi: 10
+    // memory variables of ncell floats each -- stress side 0 Dx vx (x full),
+    // 1 Dx vz (x half), 2 Dz vz (z full), 3 Dz vx (z half); velocity side
+    // 4 Dx sxx (x half), 5 Dz sxz (z full), 6 Dx sxz (x full), 7 Dz szz
+    // (z half), 8 Dx tp (x half), 9 Dz tp (z half).  pcx [4][nnx] and pcz
+    // [4][nnz] hold (a_full, b_full, a_half, b_half); half = x - 1/2, z + 1/2.
+    float* psi;
+    const float* pcx;
+    const float* pcz;
+    long ncell;
 };
+
+// e += (psi = b psi + a e) for memory variable q at cell n, axis coefficients
+// pc of length nn at index idx; where a = 0 psi stays 0 and e passes.
+__device__ __forceinline__ float pml2(const G2& g, int q, long n, const float* pc, int nn, int half, int idx, float e) {
+    const float a = pc[(2 * half) * nn + idx], b = pc[(2 * half + 1) * nn + idx];
+    if (a == 0.f) return e;
+    float* ps = g.psi + (long)q * g.ncell + n;
+    const float v = b * *ps + a * e;
+    *ps = v;
+    return e + v;
+}
 
 struct F2 {  // one wavefield state (forward, Born, or adjoint)
     float *vx, *vz, *xx, *zz, *xz, *tp, *vxp, *vzp;
@@ -124,10 +145,18 @@ __global__ __launch_bounds__(TZ* TXR) void k2_stress(F2 f, M2 m, G2 g, Coef2 cf,
 #pragma unroll
     for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
     const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
-    const float E0 = g.cx * dpx<R>(tvx, c, a_, b_);
-    const float E1 = g.cz * dmz<R>(tvz, c, a_, b_);
-    const float E2 = g.cx * dmx<R>(tvz, c, a_, b_) + g.cz * dpz<R>(tvx, c, a_, b_);
     const long n = (long)i * g.pz + k;
+    float E0, E1, E2;
+    if (!g.psi) {
+        E0 = g.cx * dpx<R>(tvx, c, a_, b_);
+        E1 = g.cz * dmz<R>(tvz, c, a_, b_);
+        E2 = g.cx * dmx<R>(tvz, c, a_, b_) + g.cz * dpz<R>(tvx, c, a_, b_);
+    } else {
+        E0 = pml2(g, 0, n, g.pcx, g.nnx, 0, i, g.cx * dpx<R>(tvx, c, a_, b_));
+        E1 = pml2(g, 2, n, g.pcz, g.nnz, 0, k, g.cz * dmz<R>(tvz, c, a_, b_));
+        E2 = pml2(g, 1, n, g.pcx, g.nnx, 1, i, g.cx * dmx<R>(tvz, c, a_, b_)) +
+             pml2(g, 3, n, g.pcz, g.nnz, 1, k, g.cz * dpz<R>(tvx, c, a_, b_));
+    }
     if constexpr (SAVE) {
         long q;
         if (in_model(g, i, k, &q)) {
@@ -186,10 +215,21 @@ __global__ __launch_bounds__(TZ* TXR) void k2_velocity(F2 f, M2 m, G2 g, Coef2 c
 #pragma unroll
     for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
     const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
-    const float fx = g.cx * dmx<R>(txx, c, a_, b_) + g.cz * dmz<R>(txz, c, a_, b_);
-    const float fz = g.cx * dpx<R>(txz, c, a_, b_) + g.cz * dpz<R>(tzz, c, a_, b_);
-    const float px = g.cx * dmx<R>(ttp, c, a_, b_), pz = g.cz * dpz<R>(ttp, c, a_, b_);
     const long n = (long)i * g.pz + k;
+    float fx, fz, px, pz;
+    if (!g.psi) {
+        fx = g.cx * dmx<R>(txx, c, a_, b_) + g.cz * dmz<R>(txz, c, a_, b_);
+        fz = g.cx * dpx<R>(txz, c, a_, b_) + g.cz * dpz<R>(tzz, c, a_, b_);
+        px = g.cx * dmx<R>(ttp, c, a_, b_);
+        pz = g.cz * dpz<R>(ttp, c, a_, b_);
+    } else {
+        fx = pml2(g, 4, n, g.pcx, g.nnx, 1, i, g.cx * dmx<R>(txx, c, a_, b_)) +
+             pml2(g, 5, n, g.pcz, g.nnz, 0, k, g.cz * dmz<R>(txz, c, a_, b_));
+        fz = pml2(g, 6, n, g.pcx, g.nnx, 0, i, g.cx * dpx<R>(txz, c, a_, b_)) +
+             pml2(g, 7, n, g.pcz, g.nnz, 1, k, g.cz * dpz<R>(tzz, c, a_, b_));
+        px = pml2(g, 8, n, g.pcx, g.nnx, 1, i, g.cx * dmx<R>(ttp, c, a_, b_));
+        pz = pml2(g, 9, n, g.pcz, g.nnz, 1, k, g.cz * dpz<R>(ttp, c, a_, b_));
+    }
     if constexpr (SAVE) {
         long q;
         if (in_model(g, i, k, &q)) {

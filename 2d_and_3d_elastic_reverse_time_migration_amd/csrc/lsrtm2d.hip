@@ -107,6 +107,11 @@ struct erm2d_ctx {
     struct Mod { float *M = nullptr, *U = nullptr, *B = nullptr, *kP[3] = {0}, *kS[3] = {0}, *decay[3] = {0};
                  float *vp = nullptr, *vs = nullptr; bool have = false; } mod[2];
     int last_model = 1;     // model of the last pass (decom_new scaling)
+    bool cpml = false;      // C-PML boundary (modeling and RTM)
+    float* psi = nullptr;   // 10 memory variables x ncell
+    float* pcx = nullptr;   // [4][nnx] (a_full, b_full, a_half, b_half)
+    float* pcz = nullptr;   // [4][nnz]
+    double vmax_loaded = 0;
     float* att = nullptr;
     std::vector<float> wavelet;
     long src = -1;
@@ -167,6 +172,10 @@ G2 geo(const erm2d_ctx* c) {
     g.att = c->att;
     g.bl = c->p.bl; g.bu = c->p.bu; g.nx = c->p.nx; g.nz = c->p.nz;
     g.qsign = c->qsign;
+    g.psi = c->cpml ? c->psi : nullptr;
+    g.pcx = c->pcx;
+    g.pcz = c->pcz;
+    g.ncell = (long)c->ncell;
     return g;
 }
 F2 state(erm2d_ctx* c, bool B) {
@@ -194,10 +203,16 @@ Ex2 excite(const erm2d_ctx* c) {
 dim3 grid2(const erm2d_ctx* c) { return dim3((c->nnz + TZ - 1) / TZ, (c->nnx + TXR - 1) / TXR); }
 dim3 grid_pt(const erm2d_ctx* c) { return dim3((c->nnz + 255) / 256, c->nnx); }
 
+// geometry of a launch on state A (the C-PML memory variables belong to it) or B
+G2 geo_of(const erm2d_ctx* c, bool B) {
+    G2 g = geo(c);
+    if (B) g.psi = nullptr;
+    return g;
+}
 template <int R, int L, bool SAVE>
 void stress_r(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
     hipLaunchKernelGGL((k2_stress<R, L, TXR, SAVE>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, B), model(c, which),
-                       geo(c), c->coef, src, sv, Eh);
+                       geo_of(c, B), c->coef, src, sv, Eh);
 }
 template <int R, bool SAVE>
 void stress_l(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
@@ -212,7 +227,7 @@ void do_stress(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
 template <int R, int MODE, bool SAVE>
 void vel_r(erm2d_ctx* c, bool B, int which, int it, float* Fh) {
     hipLaunchKernelGGL((k2_velocity<R, TXR, MODE, SAVE>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, B),
-                       model(c, which), geo(c), c->coef, excite(c), it, Fh);
+                       model(c, which), geo_of(c, B), c->coef, excite(c), it, Fh);
 }
 template <int R>
 void vel_m(erm2d_ctx* c, bool B, int which, int mode, int it, float* Fh) {
@@ -253,6 +268,7 @@ void adj_s2(erm2d_ctx* c) {
 int zero_state(erm2d_ctx* c, bool B) {
     float** f = B ? c->fB : c->fA;
     for (int q = 0; q < 8; ++q) if (fillf(c, f[q], c->ncell, 0.f)) return ERM_E_DEVICE;
+    if (!B && c->psi && fillf(c, c->psi, 10 * c->ncell, 0.f)) return ERM_E_DEVICE;
     for (int l = 0; l < c->L; ++l)
         for (int q = 0; q < 4; ++q) if (fillf(c, B ? c->rB[l][q] : c->rA[l][q], c->ncell, 0.f)) return ERM_E_DEVICE;
     return 0;
@@ -313,6 +329,7 @@ void free2(erm2d_ctx* c) {
     F(c->dv); F(c->cu);
     for (auto& m : c->mod) { F(m.M); F(m.U); F(m.B); for (int l = 0; l < 3; ++l) { F(m.kP[l]); F(m.kS[l]); F(m.decay[l]); } }
     F(c->att); F(c->rec_idx); F(c->rec); F(c->rec2); F(c->ex_time); F(c->ex_tp_time); F(c->ex_tp_amp);
+    F(c->psi); F(c->pcx); F(c->pcz);
     for (auto& m : c->mod) { F(m.vp); F(m.vs); }
     for (auto& e : c->ex) F(e);
     for (auto& e : c->img) F(e);
@@ -365,6 +382,7 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
     if (p.nsls != 0 && p.nsls != 1 && p.nsls != 3) { g_err2 = "nsls must be 0, 1 or 3"; return ERM_E_PARAM; }
     if (p.nx <= 0 || p.nz <= 0 || p.lt <= 0 || p.dt <= 0 || p.dx <= 0 || p.dz <= 0) { g_err2 = "bad grid"; return ERM_E_PARAM; }
     if (p.bu < 0 || p.bd < 0 || p.bl < 0 || p.br < 0) { g_err2 = "negative boundary width"; return ERM_E_PARAM; }
+    if (p.boundary != ERM_BOUNDARY_DAMP && p.boundary != ERM_BOUNDARY_CPML) { g_err2 = "boundary must be 0 (damping) or 1 (CPML)"; return ERM_E_PARAM; }
     int nd = 0;
     if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) { g_err2 = "no HIP device"; return ERM_E_DEVICE; }
     if (device < 0 || device >= nd) { g_err2 = "device out of range"; return ERM_E_PARAM; }
@@ -397,6 +415,14 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
     CK2(dmal(c, &c->dv, c->ncell));
     CK2(dmal(c, &c->cu, c->ncell));
     CK2(dmal(c, &c->att, c->ncell));
+    c->cpml = p.boundary == ERM_BOUNDARY_CPML;
+    if (c->cpml) {
+        CK2(dmal(c, &c->psi, 10 * c->ncell));
+        CK2(dmal(c, &c->pcx, 4 * (size_t)c->nnx));
+        CK2(dmal(c, &c->pcz, 4 * (size_t)c->nnz));
+        CK2(fillf(c, c->pcx, 4 * (size_t)c->nnx, 0.f));
+        CK2(fillf(c, c->pcz, 4 * (size_t)c->nnz, 0.f));
+    }
     CK2(dmal(c, &c->ex_time, c->ncell));
     CK2(dmal(c, &c->ex_tp_time, c->ncell));
     CK2(dmal(c, &c->ex_tp_amp, c->ncell));
@@ -411,10 +437,12 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
                 axis[start + t] += p.coe_attenuation * std::pow(s, 3.0);
             }
         };
-        if (p.bu > 1) face(p.bu, false, az, 0);
-        if (p.bd > 1) face(p.bd, true, az, p.bu + p.nz);
-        if (p.bl > 1) face(p.bl, false, ax, 0);
-        if (p.br > 1) face(p.br, true, ax, p.bl + p.nx);
+        if (!c->cpml) {  // the C-PML replaces the damping
+            if (p.bu > 1) face(p.bu, false, az, 0);
+            if (p.bd > 1) face(p.bd, true, az, p.bu + p.nz);
+            if (p.bl > 1) face(p.bl, false, ax, 0);
+            if (p.br > 1) face(p.br, true, ax, p.bl + p.nx);
+        }
         std::vector<float> h(c->ncell, 0.f);
         for (int i = 0; i < c->nnx; ++i)
             for (int k = 0; k < c->nnz; ++k) h[(size_t)i * c->pz + k] = (float)(ax[i] + az[k]);
@@ -518,6 +546,34 @@ int erm2d_load_model(erm2d_ctx* c, int which, const float* vp, const float* vs, 
     for (int l = 0; l < c->L; ++l)
         if (upload(c, m.kP[l], hkP[l]) || upload(c, m.kS[l], hkS[l]) || upload(c, m.decay[l], hd[l])) return ERM_E_DEVICE;
     m.have = true;
+    for (size_t n = 0; n < c->mn; ++n) c->vmax_loaded = std::max(c->vmax_loaded, (double)vp[n]);
+    if (c->cpml) {
+        // C-PML coefficients (Roden & Gedney 2000; Komatitsch & Martin 2007), kappa = 1, as in 3D:
+        // d = d0 r^2, d0 = -3 vmax ln(Rc) / (2 L h), alpha = pi f0 (1 - r), b = exp(-(d+alpha) dt),
+        // a = d (b-1)/(d+alpha); r = depth into the layer / L, L = width - R; full and half
+        // (x - 1/2, z + 1/2) positions
+        const double vmax = p.cpml_vmax > 0 ? p.cpml_vmax : c->vmax_loaded;
+        const double Rc = p.cpml_r > 0 ? p.cpml_r : 1e-3, dts = p.dt / 1000.0;
+        const int n2[2] = {c->nnx, c->nnz}, lo[2] = {p.bl, p.bu}, hi[2] = {p.br, p.bd};
+        const double h[2] = {p.dx, p.dz}, half_shift[2] = {-0.5, +0.5};
+        for (int a = 0; a < 2; ++a) {
+            std::vector<float> pc(4 * (size_t)n2[a], 0.f);
+            for (int f = 0; f < 2; ++f)
+                for (int i = 0; i < n2[a]; ++i) {
+                    double x = i + (f ? half_shift[a] : 0.0), sd = 0, Lw = 0;
+                    pc[(2 * f + 1) * (size_t)n2[a] + i] = 1.f;
+                    if (lo[a] > 0 && x < lo[a]) { sd = lo[a] - x; Lw = std::max(lo[a] - c->R, 1); }
+                    else if (hi[a] > 0 && x > n2[a] - hi[a] - 1) { sd = x - (n2[a] - hi[a] - 1); Lw = std::max(hi[a] - c->R, 1); }
+                    if (Lw <= 0) continue;
+                    const double r = sd / Lw, d0 = -3.0 * vmax * std::log(Rc) / (2.0 * Lw * h[a]);
+                    const double d = d0 * r * r, al = M_PI * p.freq * (1.0 - std::min(r, 1.0));
+                    const double bb = std::exp(-(d + al) * dts);
+                    pc[(2 * f + 1) * (size_t)n2[a] + i] = (float)bb;
+                    pc[(2 * f) * (size_t)n2[a] + i] = (float)(d > 0 ? d * (bb - 1.0) / (d + al) : 0.0);
+                }
+            if (upload(c, a == 0 ? c->pcx : c->pcz, pc)) return ERM_E_DEVICE;
+        }
+    }
     if (which == 1) {
         const float* src[5] = {vp, vs, rho, c->L ? qp : nullptr, c->L ? qs : nullptr};
         for (int a = 0; a < 5; ++a) {
@@ -674,6 +730,7 @@ int erm2d_rtm(erm2d_ctx* c, const float* rec, float* tp, float* pp, float* ps) {
 
 int erm2d_born(erm2d_ctx* c, const float* dm, float* d) {
     if (!c || !dm) return ERM_E_PARAM;
+    if (c->cpml) return c->fail(ERM_E_PARAM, "the LSRTM operators need the damping boundary (boundary = 0)");
     HC(c, hipSetDevice(c->device));
     if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
     int rc;
@@ -712,6 +769,7 @@ int erm2d_born(erm2d_ctx* c, const float* dm, float* d) {
 
 int erm2d_adjoint(erm2d_ctx* c, const float* r, float* gout) {
     if (!c || !r) return ERM_E_PARAM;
+    if (c->cpml) return c->fail(ERM_E_PARAM, "the LSRTM operators need the damping boundary (boundary = 0)");
     HC(c, hipSetDevice(c->device));
     if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
     int rc;
@@ -853,6 +911,7 @@ extern "C" int erm2d_lsrtm_cg(erm2d_ctx* c, int nshots, const int* shots, const 
                               float* m, float* misfit, double* seconds, erm2d_precond_fn precond, void* user,
                               float tol) {
     if (!c || nshots <= 0 || !shots || !dobs || iters < 0 || !m || !misfit) return c ? c->fail(ERM_E_PARAM, "bad arguments") : ERM_E_PARAM;
+    if (c->cpml) return c->fail(ERM_E_PARAM, "the LSRTM operators need the damping boundary (boundary = 0)");
     const size_t nm = 3 * c->mn, nd = (size_t)2 * c->p.lt * c->p.nrec;
     const int sx0 = c->p.shot_x, sz0 = c->p.shot_z;
     auto t0 = std::chrono::steady_clock::now();

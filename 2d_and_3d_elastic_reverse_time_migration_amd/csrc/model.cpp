@@ -16,6 +16,7 @@
 // vp_set_density constant, 3D_elastic_RTM/zzzzz:1272-1287).
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <random>
 #include <vector>
 
@@ -74,12 +75,19 @@ int make_model3d(int kind, int nx, int ny, int nz, float dx, float dy, float dz,
     return ERM_OK;
 }
 
+// Written to `path`.tmp and renamed into place, so a file under `path` is
+// always complete (a run killed while writing leaves only the .tmp file).
 int write_raw(const char* path, const float* data, size_t n) {
-    FILE* f = fopen(path, "wb");
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = fopen(tmp.c_str(), "wb");
     if (!f) return ERM_E_IO;
     size_t w = fwrite(data, sizeof(float), n, f);
     int rc = fclose(f);
-    return (w == n && rc == 0) ? ERM_OK : ERM_E_IO;
+    if (w != n || rc != 0 || std::rename(tmp.c_str(), path) != 0) {
+        std::remove(tmp.c_str());
+        return ERM_E_IO;
+    }
+    return ERM_OK;
 }
 
 int read_raw(const char* path, float* data, size_t n) {

@@ -12,10 +12,13 @@
 //  - two fused kernels per step, no device-wide synchronisation inside a pass.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <filesystem>
 #include <string>
 #include <vector>
 
@@ -414,6 +417,19 @@ int check_grid(const erm_params* p, int R, std::string* err) {
     if (nnx * nny * nnz > (1L << 34)) { *err = "grid too large"; return ERM_E_SIZE; }
     if (p->receiver_num_x < 0 || p->receiver_num_y < 0) { *err = "receiver counts must be >= 0"; return ERM_E_PARAM; }
     if (p->boundary != ERM_BOUNDARY_DAMP && p->boundary != ERM_BOUNDARY_CPML) { *err = "boundary must be 0 (damping) or 1 (CPML)"; return ERM_E_PARAM; }
+    if (p->join_wavefield != 0 && p->wavefield_interval <= 0) { *err = "join_wavefield needs wavefield_interval > 0"; return ERM_E_PARAM; }
+    // reference keys whose code is not part of this library: refused when set
+    // to anything but their neutral value instead of being ignored
+    struct { const char* key; bool set; const char* what; } off[] = {
+        {"laplace", p->laplace != 0, "Laplacian image filter (cpu_laplace, zzzzz:1110-1147, host post-processing)"},
+        {"laplace_compensate", p->laplace_compensate != 0, "Laplacian compensation (no code in the reference)"},
+        {"precon", p->precon != 0, "image preconditioning / precon_z1-z2 depth window (kernel bodies absent in the reference)"},
+        {"vsp_precon", p->vsp_precon != 0, "VSP preconditioning (no code in the reference)"},
+        {"velocity_multiple", p->velocity_multiple != 0.f && p->velocity_multiple != 1.f,
+         "velocity scaling for model smoothing (SU smooth3d, outside the library: supply smoothed models)"},
+    };
+    for (const auto& o : off)
+        if (o.set) { *err = std::string(o.key) + " is not supported: " + o.what; return ERM_E_PARAM; }
     return 0;
 }
 
@@ -487,6 +503,51 @@ void pad_model(const erm_params& p, int nnx, int nny, int nnz, const float* src,
     }
 }
 
+// Wavefield snapshots (join_wavefield != 0: output_3d_wavefiled_{tao,vx,vz,
+// vzp,vzs}, 3D_output_file.cu:2-151, gated at 3D_forward_or_back_together.cu:
+// 50,198,341).  After the update of time index it with (it + 1) a multiple of
+// wavefield_interval: ./wavefield/{name}-{it+1} (padded grid), the per-device
+// file {name}-0-{it+1} (one device: the same grid) and cut-{name}-{it+1}
+// (model region), raw float32 in disk order (z fastest), for name = tp, vx,
+// vz, vzp, vzs (vzs = vz - vzp).  When the pass reaches its last step,
+// ./wavefield is renamed ./wavefield1 (model pass), ./wavefield2 (source) or
+// ./wavefield3 (receiver), replacing an older one.
+int write_snapshots(erm_ctx* c, int label) {
+    ::mkdir("wavefield", 0755);
+    const erm_params& p = c->p;
+    const char* names[5] = {"tp", "vx", "vz", "vzp", "vzs"};
+    const int fid[5] = {ERM_F_TP, ERM_F_VX, ERM_F_VZ, ERM_F_VZP, ERM_F_VZS};
+    const size_t N = (size_t)c->nnx * c->nny * c->nnz;
+    std::vector<float> full(N), disk(N), cut((size_t)p.nx * p.ny * p.nz);
+    for (int a = 0; a < 5; ++a) {
+        int rc = erm_get_field(c, fid[a], full.data());
+        if (rc) return rc;
+        for (int ix = 0; ix < c->nnx; ++ix)
+            for (int iy = 0; iy < c->nny; ++iy)
+                for (int iz = 0; iz < c->nnz; ++iz)
+                    disk[((size_t)ix * c->nny + iy) * c->nnz + iz] = full[((size_t)iz * c->nny + iy) * c->nnx + ix];
+        for (int ix = 0; ix < p.nx; ++ix)
+            for (int iy = 0; iy < p.ny; ++iy)
+                for (int iz = 0; iz < p.nz; ++iz)
+                    cut[((size_t)ix * p.ny + iy) * p.nz + iz] =
+                        full[((size_t)(iz + p.bu) * c->nny + iy + p.bf) * c->nnx + ix + p.bl];
+        const std::string f1 = format("wavefield/%s-%d", names[a], label), f2 = format("wavefield/%s-0-%d", names[a], label),
+                          f3 = format("wavefield/cut-%s-%d", names[a], label);
+        if (write_raw(f1.c_str(), disk.data(), N) || write_raw(f2.c_str(), disk.data(), N) ||
+            write_raw(f3.c_str(), cut.data(), cut.size()))
+            return c->fail(ERM_E_IO, "can not write wavefield snapshot %s", f1.c_str());
+    }
+    return 0;
+}
+int finish_snapshots(erm_ctx* c, int pass) {
+    const std::string dst = format("wavefield%d", pass + 1);
+    std::error_code ec;
+    std::filesystem::remove_all(dst, ec);
+    std::filesystem::rename("wavefield", dst, ec);
+    if (ec) return c->fail(ERM_E_IO, "can not rename wavefield to %s: %s", dst.c_str(), ec.message().c_str());
+    return 0;
+}
+
 int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false) {
     if (pass < ERM_PASS_MODEL || pass > ERM_PASS_RECEIVER) return c->fail(ERM_E_PARAM, "unknown pass %d", pass);
     const int which = pass == ERM_PASS_MODEL ? 0 : 1;
@@ -509,6 +570,8 @@ int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false
             c->kev.push_back(e);
         }
     }
+    const bool snap = c->p.join_wavefield != 0 && c->p.wavefield_interval > 0 && !strips;
+    bool wrote = false;
     HIPC(c, hipEventRecord(c->pass_ev[0], c->stream));
     for (int s = 0; s < nsteps; ++s) {
         const int it = pass == ERM_PASS_RECEIVER ? it_begin - s : it_begin + s;
@@ -535,6 +598,19 @@ int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false
         if (pass == ERM_PASS_MODEL && c->nr > 0)
             hipLaunchKernelGGL(k_record, dim3(rg), dim3(rb), 0, c->stream, c->f[0], c->f[1], c->f[2], c->rec_idx, c->nr,
                                c->rec, comp, (long)it * c->nr);
+        if (snap && (it + 1) % c->p.wavefield_interval == 0) {
+            HIPC(c, hipGetLastError());
+            int rc = write_snapshots(c, it + 1);
+            if (rc) return rc;
+            wrote = true;
+        }
+        const bool last = pass == ERM_PASS_RECEIVER ? it == 0
+                                                    : it + 1 == (pass == ERM_PASS_MODEL ? c->p.lt : c->p.lt_forward);
+        if (wrote && last) {
+            int rc = finish_snapshots(c, pass);
+            if (rc) return rc;
+            wrote = false;
+        }
     }
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(c->pass_ev[1], c->stream));
@@ -1038,7 +1114,7 @@ static int write_cut(erm_ctx* c, int field, const std::string& path) {
             for (int iz = 0; iz < p.nz; ++iz)
                 cut[((size_t)ix * p.ny + iy) * p.nz + iz] =
                     full[((size_t)(iz + p.bu) * c->nny + iy + p.bf) * c->nnx + ix + p.bl];
-    if (write_raw(path.c_str(), cut.data(), cut.size())) return c->fail(ERM_E_IO, "Can not oen this file : %s", path.c_str());
+    if (write_raw(path.c_str(), cut.data(), cut.size())) return c->fail(ERM_E_IO, "can not open file : %s", path.c_str());
     return 0;
 }
 
@@ -1327,13 +1403,13 @@ int erm_make_model3d(int kind, int nx, int ny, int nz, float dx, float dy, float
 
 int erm_write_raw(const char* path, const float* data, size_t n) {
     int rc = write_raw(path, data, n);
-    if (rc) set_global_error("Can not oen this file : %s", path ? path : "(null)");
+    if (rc) set_global_error("can not open file : %s", path ? path : "(null)");
     return rc;
 }
 
 int erm_read_raw(const char* path, float* data, size_t n) {
     int rc = read_raw(path, data, n);
-    if (rc) set_global_error("Can not oen this file : %s", path ? path : "(null)");
+    if (rc) set_global_error("can not open file : %s", path ? path : "(null)");
     return rc;
 }
 

@@ -17,7 +17,8 @@ class Params2D(ctypes.Structure):
                 + [(k, ctypes.c_int) for k in ["fd_order", "nsls", "shot_x", "shot_z", "nrec", "rec_x0", "rec_dx",
                                                 "rec_z"]]
                 + [("qcomp", ctypes.c_int), ("ex_tp_shift", ctypes.c_int), ("ex_vzp_avg", ctypes.c_int),
-                   ("reserved", ctypes.c_int * 5)])
+                   ("boundary", ctypes.c_int), ("cpml_r", ctypes.c_float), ("cpml_vmax", ctypes.c_float),
+                   ("reserved", ctypes.c_int * 2)])
 
     def update(self, **kw):
         for k, v in kw.items():

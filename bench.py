@@ -269,6 +269,118 @@ def run_learned(erm, args):
     print(json.dumps(line), flush=True)
 
 
+def rtm2d_case(erm):
+    """The 2D elastic RTM shot: 400x200 model, d = 10 m, three flat layers (vp
+    2000 / 2800 / 3500 m/s at depth thirds) plus a Gaussian vp anomaly (+300 m/s,
+    sigma 200 m, at x = 2000 m, z = 1000 m), vs = vp/1.732, density
+    230 vp^0.25; 8th-order FD, dt = 1 ms, nt = 2000, 40-cell C-PML, 20 Hz
+    Ricker at (2000 m, 20 m), 400 receivers at the surface; migration model =
+    the true model Gaussian-smoothed (sigma 6 cells)."""
+    learned = importlib.import_module(PKG + ".learned")
+    nx, nz = 400, 200
+    p = learned.survey_params(nx, nz, 2000, freq=20.0, d=10.0, dt=1.0, b=40)
+    p.boundary, p.cpml_r = 1, 1e-3
+    p.shot_x, p.shot_z = 200, 2
+    p.nrec, p.rec_x0, p.rec_dx, p.rec_z = 400, 0, 1, 0
+    x = np.arange(nx)[:, None] * 10.0
+    z = np.arange(nz)[None, :] * 10.0
+    vp = np.where(z < 670.0, 2000.0, np.where(z < 1330.0, 2800.0, 3500.0)) * np.ones((nx, 1))
+    vp = vp + 300.0 * np.exp(-((x - 2000.0) ** 2 + (z - 1000.0) ** 2) / (2 * 200.0 ** 2))
+    true = [vp.astype(np.float32), (vp / 1.732).astype(np.float32), (230.0 * vp ** 0.25).astype(np.float32)]
+    vps = learned._smooth(vp, 6.0)
+    mig = [vps.astype(np.float32), (vps / 1.732).astype(np.float32), (230.0 * vps ** 0.25).astype(np.float32)]
+    return p, true, mig
+
+
+def rtm2d_errors(p, ref, et_ref, ea_ref, imgs, et):
+    """rel L2 of tp / PP / PS images against FP64 over cells with ex_amp > 1e-3 max
+    whose excitation step agrees (model region), and the mismatch fraction."""
+    cut = lambda a: a[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz]
+    exc = cut(ea_ref > 1e-3 * ea_ref.max())
+    same = exc & (cut(et) == cut(et_ref))
+    out = {"ex_time_mismatch_frac": round(float(1 - same.sum() / max(exc.sum(), 1)), 6)}
+    for name, a, b in zip(("tp", "pp", "ps"), imgs, ref):
+        out["img_%s_rel_l2" % name] = float(np.linalg.norm((a - b)[same]) / max(np.linalg.norm(b[same]), 1e-300))
+    return out
+
+
+def run_rtm2d(erm, args):
+    """2D elastic RTM of one shot (the CPU-runnable configuration): GPU shot
+    time (model pass in the true model + source pass + receiver pass with
+    excitation imaging), the FP32 oracle's time for the same shot on the
+    host cores (median of 3), and both images' error against the FP64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle2d
+    p, true, mig = rtm2d_case(erm)
+    nt = p.lt
+    nnx, nnz = p.padded_shape
+    cells = (nnx - 8) * (nnz - 8)
+    g = erm.Elastic2D(p)
+    g.load_model(0, *true)
+    g.load_model(1, *mig)
+    for w in range(max(args.warmup, 1)):
+        rec = g.forward()
+        g.rtm(rec)
+    times = []
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        rec = g.forward()
+        imgs = g.rtm(rec)
+        times.append(time.perf_counter() - t0)
+        log(f"bench rtm2d: GPU shot {s + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
+    et_g = g.get_field("ex_time")
+    g.close()
+    gpu_s = float(np.median(times))
+
+    def oracle_shot(prec):
+        o = oracle2d.Oracle2D(p, prec)
+        o.load_model(0, *true)
+        o.load_model(1, *mig)
+        t0 = time.perf_counter()
+        o.step(0, 0, nt)
+        o.zero(7)
+        o.step(1, 0, nt)
+        o.zero(1)
+        o.step(2, nt - 1, nt)
+        el = time.perf_counter() - t0
+        return o, el
+
+    aff, nproc, model = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or aff)
+    oracle2d.oracle3d.set_threads(threads)
+    o64, _ = oracle_shot(64)
+    ref = o64.images()
+    et_ref, ea_ref = o64.get_field("ex_time"), o64.get_field("ex_amp")
+    cpu_times, o32 = [], None
+    for r in range(3):
+        o32, el = oracle_shot(32)
+        cpu_times.append(el)
+        log(f"bench rtm2d: FP32 oracle shot {r + 1}/3: {el:.2f} s on {threads} threads")
+    cpu_s = float(np.median(cpu_times))
+    err_gpu = rtm2d_errors(p, ref, et_ref, ea_ref, imgs, et_g)
+    err_cpu = rtm2d_errors(p, ref, et_ref, ea_ref, o32.images(), o32.get_field("ex_time"))
+    updates = cells * 3.0 * nt
+    line = {
+        "metric": "2D elastic RTM, one shot (s per shot)",
+        "value": round(gpu_s, 4), "unit": "s/shot", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(gpu_s * 1e3, 2), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (three flat layers + Gaussian anomaly)",
+        "config": {"workload": "2D elastic RTM, one shot: 400x200 model + 40-cell C-PML (480x280), d=10 m, 8th-order "
+                               "staggered FD, dt=1 ms, nt=2000, 20 Hz Ricker at (2000 m, 20 m), 400 surface "
+                               "receivers; step = model pass + source pass + receiver pass with excitation imaging",
+                   "cells_per_time_step": cells, "time_steps_per_shot": 3 * nt,
+                   "gpu_Gcell_updates_per_s": round(updates / gpu_s / 1e9, 3), "parallelism": "single GPU"},
+        "error_vs_fp64": {"gpu": err_gpu, "cpu_fp32": err_cpu},
+        "roofline": None,
+        "cpu_baseline": {"value": round(cpu_s, 3), "unit": "s/shot", "cores": threads, "kind": "port",
+                         "nproc": nproc, "affinity_cores": aff, "cpu_model": model,
+                         "runs_s": [round(t, 3) for t in cpu_times],
+                         "Gcell_updates_per_s": round(updates / cpu_s / 1e9, 4),
+                         "sample": "the full shot (3 x 2000 steps), FP32 oracle, OpenMP over x, median of 3"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_visco(erm, args):
     """2D viscoelastic LSRTM with plain CG: 600x300 model, d = 8 m, 3 SLS
     mechanisms, 60 shots, 600 receivers, 20 Hz, nt = 3000 (dt 0.8 ms), 40-cell
@@ -389,7 +501,7 @@ def main():
     ap.add_argument("--iters", type=int, default=20, help="visco workload: CG iterations")
     ap.add_argument("--workers", type=int, default=4, help="visco workload: concurrent shot contexts")
     ap.add_argument("--oracle-check", action="store_true", help="visco workload: FP64 oracle CG check (1 shot)")
-    ap.add_argument("--workload", choices=["shot", "batch", "visco", "learned"], default="shot",
+    ap.add_argument("--workload", choices=["shot", "batch", "visco", "learned", "rtm2d"], default="shot",
                     help="shot: the one-shot 3D configuration (default); batch: 24 windowed shots over a "
                          "600x600x250 model in one batched, graph-captured launch per kernel per step; "
                          "learned: LSRTM time to misfit, learned preconditioner vs plain CG")
@@ -412,6 +524,8 @@ def main():
         return run_visco(erm, args)
     if args.workload == "learned":
         return run_learned(erm, args)
+    if args.workload == "rtm2d":
+        return run_rtm2d(erm, args)
     nt = args.nt
     p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
     p.tile_y, p.zchunk = args.tile_y, args.zchunk

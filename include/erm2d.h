@@ -43,7 +43,12 @@ typedef struct erm2d_params {
                           imaging_inner_product_ex_2D_new, elastic_2D_kernel_3.cu:91-109,582-605):
                           vzp averaged over its four neighbours (i, k +- 1), (i +- 1, k) in the
                           excitation components and in the receiver P and S motion; 0: own cell */
-    int   reserved[5];
+    int   boundary;    /* ERM_BOUNDARY_DAMP (default: the reference's cubic damping) or ERM_BOUNDARY_CPML
+                          (convolutional PML, no reference counterpart; modeling and RTM only -- the
+                          LSRTM operators need the damping boundary) */
+    float cpml_r;      /* CPML target reflection coefficient (0: 1e-3) */
+    float cpml_vmax;   /* CPML reference velocity in m/s (0: max vp of the loaded models) */
+    int   reserved[2];
 } erm2d_params;
 
 /* fields for erm2d_get_field / erm2d_set_field (padded grid, z fastest) */

@@ -126,6 +126,16 @@ struct Oracle2D {
     std::vector<int> ex_tp_time;   // tp excitation (caculate_ex_tp_time_new)
     std::vector<T> ex_tp_amp;
     std::vector<T> vp_m[2], vs_m[2];  // loaded vp, vs (decom_new)
+    /* C-PML (no reference counterpart): coefficients a, b per axis at integer
+     * ("full") and staggered ("half": x - 1/2, z + 1/2) positions and 10
+     * memory variables on the whole grid (zero outside the layers):
+     * stress side 0 Dx vx (x full), 1 Dx vz (x half), 2 Dz vz (z full),
+     * 3 Dz vx (z half); velocity side 4 Dx sxx (x half), 5 Dz sxz (z full),
+     * 6 Dx sxz (x full), 7 Dz szz (z half), 8 Dx tp (x half), 9 Dz tp (z half). */
+    bool cpml = false;
+    double vmax_loaded = 0;
+    std::vector<T> ca[2][2], cb[2][2];  /* [axis 0 x, 1 z][0 full, 1 half] */
+    std::vector<T> psi[10];
     int last_model = 1;                // model of the last pass (0 true, 1 background)
     std::vector<T> rec;  /* [2][lt][nrec] */
 
@@ -156,10 +166,14 @@ struct Oracle2D {
                 axis[start + t] += p.coe_attenuation * std::pow(s, 3.0);
             }
         };
-        if (p.bu > 1) face(p.bu, false, az, 0);
-        if (p.bd > 1) face(p.bd, true, az, p.bu + p.nz);
-        if (p.bl > 1) face(p.bl, false, ax, 0);
-        if (p.br > 1) face(p.br, true, ax, p.bl + p.nx);
+        cpml = p.boundary == ERM_BOUNDARY_CPML;
+        if (p.boundary != ERM_BOUNDARY_DAMP && !cpml) return -1;
+        if (!cpml) {  /* C-PML replaces the damping */
+            if (p.bu > 1) face(p.bu, false, az, 0);
+            if (p.bd > 1) face(p.bd, true, az, p.bu + p.nz);
+            if (p.bl > 1) face(p.bl, false, ax, 0);
+            if (p.br > 1) face(p.br, true, ax, p.bl + p.nx);
+        }
         att.assign(N, 0);
         for (int i = 0; i < nnx; ++i)
             for (int k = 0; k < nnz; ++k) att[id(i, k)] = (T)(ax[i] + az[k]);
@@ -209,7 +223,47 @@ struct Oracle2D {
                 }
             }
         m.have = true;
+        for (size_t n = 0; n < N; ++n) vmax_loaded = std::max(vmax_loaded, (double)vp_m[which][n]);
+        if (cpml) build_cpml();
         return 0;
+    }
+
+    /* C-PML as in 3D (Roden & Gedney 2000; Komatitsch & Martin 2007), kappa = 1:
+     * psi' = b psi + a e, e' = e + psi'; b = exp(-(d+alpha) dt), a = d (b-1)/(d+alpha),
+     * d = d0 r^2, d0 = -3 vmax ln(Rc) / (2 L h), alpha = pi f0 (1 - r), r = depth into
+     * the layer / L, L = width - R (the outer R cells are never updated). */
+    void build_cpml() {
+        const double vmax = p.cpml_vmax > 0 ? p.cpml_vmax : vmax_loaded;
+        const double Rc = p.cpml_r > 0 ? p.cpml_r : 1e-3;
+        const double dts = p.dt / 1000.0;
+        const int n2[2] = {nnx, nnz}, lo[2] = {p.bl, p.bu}, hi[2] = {p.br, p.bd};
+        const double h[2] = {p.dx, p.dz}, half_shift[2] = {-0.5, +0.5};
+        for (int a = 0; a < 2; ++a)
+            for (int f = 0; f < 2; ++f) {
+                ca[a][f].assign(n2[a], 0);
+                cb[a][f].assign(n2[a], 1);
+                for (int i = 0; i < n2[a]; ++i) {
+                    double x = i + (f ? half_shift[a] : 0.0), s = 0, Lw = 0;
+                    if (lo[a] > 0 && x < lo[a]) { s = lo[a] - x; Lw = std::max(lo[a] - R, 1); }
+                    else if (hi[a] > 0 && x > n2[a] - hi[a] - 1) { s = x - (n2[a] - hi[a] - 1); Lw = std::max(hi[a] - R, 1); }
+                    if (Lw <= 0) continue;
+                    double r = s / Lw;
+                    double d0 = -3.0 * vmax * std::log(Rc) / (2.0 * Lw * h[a]);
+                    double d = d0 * r * r, al = M_PI * p.freq * (1.0 - std::min(r, 1.0));
+                    double bb = std::exp(-(d + al) * dts);
+                    cb[a][f][i] = (T)bb;
+                    ca[a][f][i] = (T)(d > 0 ? d * (bb - 1.0) / (d + al) : 0.0);
+                }
+            }
+        for (auto& v : psi) v.assign(N, (T)0);
+    }
+    inline T pml(int q, size_t n, int axis, int half, int idx, T e) {
+        if (!cpml) return e;
+        const T bq = cb[axis][half][idx], aq = ca[axis][half][idx];
+        if (aq == (T)0) return e;
+        T v = bq * psi[q][n] + aq * e;
+        psi[q][n] = v;
+        return e + v;
     }
 
     /* staggered operators on a z-fastest grid: D+ centred at +1/2, D- at -1/2 */
@@ -238,7 +292,14 @@ struct Oracle2D {
             for (int k = R; k < nnz - R; ++k) {
                 size_t n = id(i, k);
                 T E[3];
-                strain(fv[VX].data(), fv[VZ].data(), n, E);
+                {  /* the same expression tree with or without C-PML (pml() passes e through when off) */
+                    const ptrdiff_t sX = nnz;
+                    T e0 = cx * dp(fv[VX].data() + n, sX), e1 = cz * dm(fv[VZ].data() + n, 1);
+                    T e2x = cx * dm(fv[VZ].data() + n, sX), e2z = cz * dp(fv[VX].data() + n, 1);
+                    E[0] = pml(0, n, 0, 0, i, e0);
+                    E[1] = pml(2, n, 1, 0, k, e1);
+                    E[2] = pml(1, n, 0, 1, i, e2x) + pml(3, n, 1, 1, k, e2z);
+                }
                 if (Ehist && i >= p.bl && i < p.bl + p.nx && k >= p.bu && k < p.bu + p.nz) {
                     size_t q = (size_t)(i - p.bl) * p.nz + (k - p.bu), mn = (size_t)p.nx * p.nz;
                     Ehist[q] = E[0]; Ehist[mn + q] = E[1]; Ehist[2 * mn + q] = E[2];
@@ -272,13 +333,20 @@ struct Oracle2D {
             for (int k = R; k < nnz - R; ++k) {
                 size_t n = id(i, k);
                 T F[2];
-                force(fv[XX].data(), fv[ZZ].data(), fv[XZ].data(), n, F);
+                T px = cx * dm(fv[TP].data() + n, sX), pz = cz * dp(fv[TP].data() + n, 1);
+                {
+                    T f0x = cx * dm(fv[XX].data() + n, sX), f0z = cz * dm(fv[XZ].data() + n, 1);
+                    T f1x = cx * dp(fv[XZ].data() + n, sX), f1z = cz * dp(fv[ZZ].data() + n, 1);
+                    F[0] = pml(4, n, 0, 1, i, f0x) + pml(5, n, 1, 0, k, f0z);
+                    F[1] = pml(6, n, 0, 0, i, f1x) + pml(7, n, 1, 1, k, f1z);
+                    px = pml(8, n, 0, 1, i, px);
+                    pz = pml(9, n, 1, 1, k, pz);
+                }
                 if (Fhist && i >= p.bl && i < p.bl + p.nx && k >= p.bu && k < p.bu + p.nz) {
                     size_t q = (size_t)(i - p.bl) * p.nz + (k - p.bu), mn = (size_t)p.nx * p.nz;
                     Fhist[q] = F[0]; Fhist[mn + q] = F[1];
                 }
                 T b, a = ab(n, &b), bo = m.B[n];
-                T px = cx * dm(fv[TP].data() + n, sX), pz = cz * dp(fv[TP].data() + n, 1);
                 T nvx = a * (b * fv[VX][n] + bo * F[0]);
                 T nvz = a * (b * fv[VZ][n] + bo * F[1]);
                 T nxp = a * (b * fv[VXP][n] + bo * px);
@@ -379,6 +447,7 @@ struct Oracle2D {
 
     void zero() {
         for (auto& v : f) std::fill(v.begin(), v.end(), (T)0);
+        for (auto& v : psi) std::fill(v.begin(), v.end(), (T)0);
         for (int l = 0; l < L; ++l) for (auto& v : r[l]) std::fill(v.begin(), v.end(), (T)0);
     }
 
@@ -426,6 +495,7 @@ struct Oracle2D {
 
     /* d = L dm: background and Born wavefields stepped together. */
     int born(const float* dmv, float* d) {
+        if (cpml) return -1;  /* the LSRTM operators use the damping boundary */
         const Model& m = mod[1];
         if (!m.have) return -4;
         Pert P = unpack(dmv);
@@ -485,6 +555,7 @@ struct Oracle2D {
 
     /* g = L^T r: exact transpose of born(), stepping back from lt-1. */
     int adjoint(const float* rr, float* gout) {
+        if (cpml) return -1;
         const Model& m = mod[1];
         if (!m.have) return -4;
         size_t mn = (size_t)p.nx * p.nz;

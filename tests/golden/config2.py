@@ -88,14 +88,16 @@ def gpu_shot(erm, nt=NT, **launch):
 
 def compare(fx, p, et, ea, imgs, vz_slice):
     """Errors of a shot against the fixture: the fraction of excited cells
-    (ex_amp > 1e-3 max) whose excitation step differs, and the rel L2 of each
-    image over excited cells with the same step (slice and lattice together),
-    of the receiver-pass vz slice."""
+    (ex_amp > 1e-5 max: the wavefield reached them in the 500 steps; the
+    maximum sits at the source, 3-4 orders above the reflected field) whose
+    excitation step differs, and the rel L2 of each image over excited cells
+    with the same step (slice and lattice together), of the receiver-pass vz
+    slice."""
     et_s, et_g = sample(et, p)
     amp = np.concatenate([fx["slice_ex_amp"].ravel(), fx["grid_ex_amp"].ravel()]).astype(np.float64)
     t_ref = np.concatenate([fx["slice_ex_time"].ravel(), fx["grid_ex_time"].ravel()]).astype(np.int64)
     t_got = np.concatenate([et_s.ravel(), et_g.ravel()]).astype(np.int64)
-    excited = amp > 1e-3 * amp.max()
+    excited = amp > 1e-5 * amp.max()
     same = excited & (t_ref == t_got)
     out = {"nt": int(fx["nt"]), "ex_time_mismatch_frac": float(1.0 - same.sum() / max(excited.sum(), 1)),
            "cells_compared": int(same.sum())}

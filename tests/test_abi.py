@@ -157,3 +157,16 @@ def test_headers_are_c99_and_link(erm, tmp_path):
                         str(src), "-o", str(exe), "-L", libdir, "-lerm", "-Wl,-rpath," + libdir],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-2000:]
+
+
+@pytest.mark.parametrize("key,val", [("laplace", 1), ("laplace_compensate", 1), ("precon", 1), ("vsp_precon", 1),
+                                     ("velocity_multiple", 1.5)])
+def test_create_refuses_unsupported_reference_keys(erm, key, val):
+    """Reference keys whose code is not in the library are refused when set to
+    a non-neutral value (checked before any device work), not ignored."""
+    p = erm.default_params(nx=20, ny=20, nz=20, dx=10.0, dy=10.0, dz=10.0, dt=1.0, lt=10, freq=20.0,
+                           bu=4, bd=4, bl=4, br=4, bf=4, bb=4, receiver_num_x=1, receiver_num_y=1)
+    setattr(p, key, val)
+    with pytest.raises(erm.ErmError) as e:
+        erm.RTM3D(p)
+    assert e.value.code == -1 and key in str(e.value), str(e.value)

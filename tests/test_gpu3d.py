@@ -364,3 +364,35 @@ def test_gpu_cli_matches_api(erm, tmp_path):
     # a second run resumes: the shot's results exist, so it is skipped
     r2 = subprocess.run([exe, f"par={par}"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r2.returncode == 0 and "skipped" in r2.stderr
+
+
+def test_gpu_wavefield_snapshots(erm, tmp_path, monkeypatch):
+    """join_wavefield / wavefield_interval (output_3d_wavefiled_*,
+    3D_output_file.cu:2-151, gated at 3D_forward_or_back_together.cu:50): after
+    the step of time index it with (it + 1) % interval == 0 the padded-grid
+    and model-region snapshots of tp, vx, vz, vzp, vzs hold the fields of that
+    step in disk order; the directory becomes wavefield1 after the model
+    pass."""
+    monkeypatch.chdir(tmp_path)
+    p = cases.params(nx=20, ny=18, nz=16, b=6, R=4, lt=30)
+    p.join_wavefield, p.wavefield_interval = 1, 10
+    vp, vs, rho = cases.layered(p, seed=2)
+    _say("wavefield snapshots: model pass, interval 10")
+    with erm.RTM3D(p) as g:
+        g.load_model(0, vp, vs, rho)
+        g.zero_wavefield()
+        want = {}
+        for it in range(p.lt):
+            g.step(PASS_MODEL, it, 1)
+            if (it + 1) % 10 == 0:
+                want[it + 1] = {k: g.get_field(k) for k in ("tp", "vx", "vz", "vzp", "vzs")}
+    d = tmp_path / "wavefield1"
+    assert d.is_dir() and not (tmp_path / "wavefield").exists()
+    nnz, nny, nnx = p.padded_shape
+    for label, fields in want.items():
+        for name, a in fields.items():
+            full = np.fromfile(d / f"{name}-{label}", np.float32).reshape(nnx, nny, nnz)
+            np.testing.assert_array_equal(full, a.transpose(2, 1, 0))
+            np.testing.assert_array_equal(np.fromfile(d / f"{name}-0-{label}", np.float32), full.ravel())
+            cut = np.fromfile(d / f"cut-{name}-{label}", np.float32).reshape(p.nx, p.ny, p.nz)
+            np.testing.assert_array_equal(cut, full[p.bl:p.bl + p.nx, p.bf:p.bf + p.ny, p.bu:p.bu + p.nz])

@@ -5,7 +5,7 @@ benchmark's launch shape (64x4 tiles, z sweep 6, XCD y-band tile order,
 region-split C-PML tile lists) and compared with the FP64 oracle's fixture
 (tests/golden/config2.py).
 
-Tolerances: images rel L2 <= 1e-4 over excited cells (ex_amp > 1e-3 max) whose
+Tolerances: images rel L2 <= 1e-4 over excited cells (ex_amp > 1e-5 max) whose
 excitation step agrees, at most 2% of excited cells on another step (an argmax
 of FP32 samples can pick a neighbouring step where two are within rounding);
 the receiver-pass vz slice <= 1e-5.
@@ -30,7 +30,7 @@ def test_gpu_config2_shot_matches_fp64_fixture(erm):
     err = config2.compare(fx, p, et, ea, imgs, vz)
     print(err, flush=True)
     assert err["ex_time_mismatch_frac"] < 0.02, err
-    assert err["cells_compared"] > 10000, err
+    assert err["cells_compared"] > 5000, err
     for k in ("img_tp_rel_l2", "img_pp_rel_l2", "img_ps_rel_l2"):
         assert err[k] < 1e-4, (k, err)
     assert err["vz_rel_l2"] < 1e-5, err
