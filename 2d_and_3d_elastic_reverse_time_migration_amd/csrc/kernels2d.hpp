@@ -53,8 +53,9 @@ struct F2 {  // one wavefield state (forward, Born, or adjoint)
     float* r[3][4];  // memory variables per mechanism: p, xx, zz, xz
 };
 
-struct M2 {  // model: unrelaxed moduli, buoyancy, SLS coefficients
+struct M2 {  // model: unrelaxed moduli, buoyancy, SLS coefficients, loaded vp / vs
     const float *M, *U, *B;
+    const float *vp, *vs;
     const float* kP[3];
     const float* kS[3];
     const float* decay[3];  // dt / tau_sigma_l
@@ -298,8 +299,30 @@ __global__ void k2_ex_avg(F2 f, Ex2 ex, G2 g, int it, int R) {
 // source inside the damping bracket equals a*source added afterwards).
 // dm = [3][nx*nz] relative perturbations (dvp/vp, dvs/vs, drho/rho).
 // ---------------------------------------------------------------------------
+// Relative perturbations (sP, sS) of the P and S moduli rho vp^2, rho vs^2
+// from the inverted parameters (m1, m2, m3) of inversion_para (the mapping
+// of cuda_cal_dem_parameter_viscoelastic_media_new, viscoelastic_equation.cu:
+// 2431-2486, for relative perturbations): 1 (lambda, mu, rho):
+// sP = (lambda m1 + 2 mu m2) / (lambda + 2 mu), sS = m2; 2 (vp, vs, rho):
+// sP = 2 m1 + m3, sS = 2 m2 + m3; 3 (Ip, Is, rho): sP = 2 m1 - m3,
+// sS = 2 m2 - m3.
+__device__ __forceinline__ void rel_moduli(int para, float vp, float vs, float m1, float m2, float m3, float* sP,
+                                           float* sS) {
+    if (para == 1) {
+        *sP = ((vp * vp - 2.0f * vs * vs) * m1 + 2.0f * vs * vs * m2) / (vp * vp);
+        *sS = m2;
+    } else if (para == 3) {
+        *sP = 2.0f * m1 - m3;
+        *sS = 2.0f * m2 - m3;
+    } else {
+        *sP = 2.0f * m1 + m3;
+        *sS = 2.0f * m2 + m3;
+    }
+}
+
 template <int L>
-__global__ void k2_born_stress_src(F2 d, M2 m, G2 g, const float* __restrict__ dm, const float* __restrict__ Eh) {
+__global__ void k2_born_stress_src(F2 d, M2 m, G2 g, const float* __restrict__ dm, const float* __restrict__ Eh,
+                                   int para) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if (k >= g.nnz) return;
     long q;
@@ -307,7 +330,8 @@ __global__ void k2_born_stress_src(F2 d, M2 m, G2 g, const float* __restrict__ d
     const long n = (long)i * g.pz + k, mn = (long)g.nx * g.nz;
     const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h);
     const float E0 = Eh[q], E1 = Eh[mn + q], E2 = Eh[2 * mn + q];
-    const float sP = 2.0f * dm[q] + dm[2 * mn + q], sS = 2.0f * dm[mn + q] + dm[2 * mn + q];
+    float sP, sS;
+    rel_moduli(para, m.vp[n], m.vs[n], dm[q], dm[mn + q], dm[2 * mn + q], &sP, &sS);
     const float dM = m.M[n] * sP, dU = m.U[n] * sS, dL = dM - 2.0f * dU;
     float sp = 0.f, sxx = 0.f, szz = 0.f, sxz = 0.f;
 #pragma unroll
@@ -453,6 +477,26 @@ __global__ __launch_bounds__(TZ* TXR) void k2a_s2(F2 l, G2 g, Coef2 cf, const fl
     const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h), b = 1.0f - h;
     l.vx[n] = a * b * l.vx[n] - g.cx * dmx<R>(w0, c, a_, b_) - g.cz * dmz<R>(w2, c, a_, b_);
     l.vz[n] = a * b * l.vz[n] - g.cz * dpz<R>(w1, c, a_, b_) - g.cx * dpx<R>(w2, c, a_, b_);
+}
+
+// Gradient in the inverted parameters: the adjoint accumulates (2 GP, 2 GS,
+// GP + GS + GD) in the (vp, vs, rho) parameters, GP / GS the gradients of the
+// relative P / S modulus perturbations and GD the direct density part; the
+// chain rule of rel_moduli gives parameterisations 1 and 3.
+__global__ void k2_grad_para(float* __restrict__ grad, M2 m, G2 g, int para) {
+    const long mn = (long)g.nx * g.nz;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < mn; q += (long)gridDim.x * blockDim.x) {
+        const float GP = 0.5f * grad[q], GS = 0.5f * grad[mn + q], GD = grad[2 * mn + q] - GP - GS;
+        if (para == 1) {
+            const long n = (long)((int)(q / g.nz) + g.bl) * g.pz + (int)(q % g.nz) + g.bu;
+            const float a = m.vp[n], b = m.vs[n];
+            grad[q] = GP * (a * a - 2.0f * b * b) / (a * a);
+            grad[mn + q] = GP * 2.0f * b * b / (a * a) + GS;
+            grad[2 * mn + q] = GD;
+        } else if (para == 3) {
+            grad[2 * mn + q] = GD - GP - GS;
+        }
+    }
 }
 
 // div / curl decomposition (decom, elastic_2D_kernel_2.cu:621-666); with

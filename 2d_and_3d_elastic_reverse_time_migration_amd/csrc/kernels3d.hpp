@@ -299,6 +299,15 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
 #pragma unroll
     for (int m = 1; m <= R; ++m) c[m] = cf.c[m];
     const float hxy = active ? g.hx[ic] + g.hy[jc] : 0.f;
+    // C-PML coefficients of the tile in LDS (x by column, y by row, z by plane,
+    // the plane's entry written by one lane each step), so the update reads
+    // them without a dependent global load
+    __shared__ float4 pc_x[AX ? TX : 1], pc_y[AX ? TY : 1], pc_z[1];
+    if constexpr (AX != 0) {
+        const float4* cf4 = reinterpret_cast<const float4*>(P.psi + psi_total(P));
+        if (ty == 0) pc_x[tx] = cf4[ic];
+        if (ty == 1 && tx < TY) pc_y[tx] = cf4[g.nnx + clampi(j0 + tx, 0, g.nny - 1)];
+    }
 
     // array bases at plane kb-R
     const long base = (long)(kb - R) * g.pxy;
@@ -320,6 +329,10 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
         qy[p + 1] = bld(rvy, o_c, so);
         qz[p + 1] = bld(rvz, o_c, so);
     }
+    // Complete the queue before the sweep: otherwise the compiler keeps these
+    // loads pending into the loop and every step's queue shift waits on the
+    // previous step's stores.
+    __builtin_amdgcn_s_waitcnt(0);
 
     for (int k = kb; k < ke; ++k) {
 #pragma unroll
@@ -329,6 +342,7 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
         qx[2 * R] = bld(rvx, o_c, sf);
         qy[2 * R] = bld(rvy, o_c, sf);
         qz[2 * R] = bld(rvz, o_c, sf);
+        const float hzk = g.hz[k];  // issued with the plane's loads (a late load would stall the update)
         float hx0 = 0.f, hx1 = 0.f, hx2 = 0.f;
         if (xh) { hx0 = bld(rvx, o_x, so); hx1 = bld(rvy, o_x, so); hx2 = bld(rvz, o_x, so); }
         float hy0[NYH], hy1[NYH], hy2[NYH];
@@ -350,22 +364,32 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
         float ps[9];
         unsigned pqx = 0, pqy = 0, pqz = 0;  // slab offsets (< 2^32 floats)
         int szi = -1;
+        // (every lane loads, at a clamped slab cell when outside the slab, so
+        // no branch splits the loads; such lanes use zero memory variables,
+        // which the zero coefficients there keep at zero, and store nothing)
         if constexpr (AX != 0) {
             szi = (AX & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
-            if ((AX & 1) && sxi >= 0) {
-                pqx = ((unsigned)k * g.nny + j) * P.w[0] + sxi;
+            if constexpr ((AX & 1) != 0) {
+                pqx = ((unsigned)k * g.nny + jc) * P.w[0] + (unsigned)max(sxi, 0);
                 bldn<3>(rpsi, 12u * pqx, 4u * psi_stress(P, 0), ps);
             }
-            if ((AX & 2) && syi >= 0) {
-                pqy = ((unsigned)k * P.w[1] + syi) * g.nnx + i;
+            if constexpr ((AX & 2) != 0) {
+                pqy = ((unsigned)k * P.w[1] + (unsigned)max(syi, 0)) * g.nnx + ic;
                 bldn<3>(rpsi, 12u * pqy, 4u * psi_stress(P, 1), ps + 3);
             }
-            if ((AX & 4) && szi >= 0) {
-                pqz = ((unsigned)szi * g.nny + j) * g.nnx + i;
+            if constexpr ((AX & 4) != 0) {
+                pqz = ((unsigned)max(szi, 0) * g.nny + jc) * g.nnx + ic;
                 bldn<3>(rpsi, 12u * pqz, 4u * psi_stress(P, 2), ps + 6);
             }
         }
+        [[maybe_unused]] float4 zq{};
+        if constexpr ((AX & 4) != 0) {
+            if (tx == 0 && ty == 0) zq = reinterpret_cast<const float4*>(P.psi + psi_total(P) + 4u * (g.nnx + g.nny))[k];
+        }
         __syncthreads();  // previous plane's LDS reads are done
+        if constexpr ((AX & 4) != 0) {
+            if (tx == 0 && ty == 0) pc_z[0] = zq;
+        }
         sx[ty + R][tx + R] = qx[R];
         sy[ty + R][tx + R] = qy[R];
         sz[ty + R][tx + R] = qz[R];
@@ -377,7 +401,11 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
             }
         }
         __syncthreads();
-        if (active) {
+        // Every lane computes (inactive lanes on clamped, in-range data) and
+        // only the stores are guarded: a load consumed on some control paths
+        // only would make the compiler wait for this plane's stores before
+        // the next plane's queue shift, serialising two memory round trips.
+        {
             const int ci = tx + R, cj = ty + R;
             float dxvx = 0.f, dyvy = 0.f, dzvz = 0.f, dyvx = 0.f, dxvy = 0.f, dzvx = 0.f, dxvz = 0.f,
                   dzvy = 0.f, dyvz = 0.f;
@@ -401,40 +429,45 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
             float e_yx = g.cy * dyvx, e_xy = g.cx * dxvy, e_zx = g.cz * dzvx, e_xz = g.cx * dxvz;
             float e_zy = g.cz * dzvy, e_yz = g.cy * dyvz;
             if constexpr (AX != 0) {
-                if ((AX & 1) && sxi >= 0) {
-                    float cq[4];
-                    bldn<4>(rpsi, 16u * i, 4u * pml_coef(P, g.nnx, g.nny, 0), cq);  // (a_full, b_full, a_half, b_half) of x
-                    exx = pml_upd(cq[0], cq[1], exx, ps[0]);
-                    e_xy = pml_upd(cq[2], cq[3], e_xy, ps[1]);
-                    e_xz = pml_upd(cq[2], cq[3], e_xz, ps[2]);
-                    bstn<3>(rpsi, 12u * pqx, 4u * psi_stress(P, 0), ps);
+                if constexpr ((AX & 1) != 0) {
+                    const float4 cq = pc_x[tx];  // (a_full, b_full, a_half, b_half) of x
+                    for (int q = 0; q < 3; ++q) ps[q] = sxi >= 0 ? ps[q] : 0.f;
+                    exx = pml_upd(cq.x, cq.y, exx, ps[0]);
+                    e_xy = pml_upd(cq.z, cq.w, e_xy, ps[1]);
+                    e_xz = pml_upd(cq.z, cq.w, e_xz, ps[2]);
                 }
-                if ((AX & 2) && syi >= 0) {
-                    float cq[4];
-                    bldn<4>(rpsi, 16u * j, 4u * pml_coef(P, g.nnx, g.nny, 1), cq);
-                    eyy = pml_upd(cq[0], cq[1], eyy, ps[3]);
-                    e_yx = pml_upd(cq[2], cq[3], e_yx, ps[4]);
-                    e_yz = pml_upd(cq[2], cq[3], e_yz, ps[5]);
-                    bstn<3>(rpsi, 12u * pqy, 4u * psi_stress(P, 1), ps + 3);
+                if constexpr ((AX & 2) != 0) {
+                    const float4 cq = pc_y[ty];
+                    for (int q = 3; q < 6; ++q) ps[q] = syi >= 0 ? ps[q] : 0.f;
+                    eyy = pml_upd(cq.x, cq.y, eyy, ps[3]);
+                    e_yx = pml_upd(cq.z, cq.w, e_yx, ps[4]);
+                    e_yz = pml_upd(cq.z, cq.w, e_yz, ps[5]);
                 }
-                if ((AX & 4) && szi >= 0) {
-                    const float4 cq = reinterpret_cast<const float4*>(P.psi + pml_coef(P, g.nnx, g.nny, 2))[k];
+                if constexpr ((AX & 4) != 0) {
+                    const float4 cq = pc_z[0];
+                    for (int q = 6; q < 9; ++q) ps[q] = szi >= 0 ? ps[q] : 0.f;
                     ezz = pml_upd(cq.x, cq.y, ezz, ps[6]);
                     e_zx = pml_upd(cq.z, cq.w, e_zx, ps[7]);
                     e_zy = pml_upd(cq.z, cq.w, e_zy, ps[8]);
-                    bstn<3>(rpsi, 12u * pqz, 4u * psi_stress(P, 2), ps + 6);
                 }
             }
-            const float h = hxy + g.hz[k];
+            const float h = hxy + hzk;
             const float a = 1.0f / (1.0f + h), b = 1.0f - h;
             const float lam = M - 2.0f * mu;
-            bst(rxx, o_c, so, a * (b * sxx + M * exx + lam * (eyy + ezz)));
-            bst(ryy, o_c, so, a * (b * syy + M * eyy + lam * (exx + ezz)));
-            bst(rzz, o_c, so, a * (b * szz + M * ezz + lam * (exx + eyy)));
-            bst(rtp, o_c, so, a * (b * stp + M * (exx + eyy + ezz)));
-            bst(rxy, o_c, so, a * (b * sxy + mu * (e_yx + e_xy)));
-            bst(rxz, o_c, so, a * (b * sxz + mu * (e_zx + e_xz)));
-            bst(ryz, o_c, so, a * (b * syz + mu * (e_zy + e_yz)));
+            const float nxx = a * (b * sxx + M * exx + lam * (eyy + ezz));
+            const float nyy = a * (b * syy + M * eyy + lam * (exx + ezz));
+            const float nzz = a * (b * szz + M * ezz + lam * (exx + eyy));
+            const float ntp = a * (b * stp + M * (exx + eyy + ezz));
+            const float nxy = a * (b * sxy + mu * (e_yx + e_xy));
+            const float nxz = a * (b * sxz + mu * (e_zx + e_xz));
+            const float nyz = a * (b * syz + mu * (e_zy + e_yz));
+            if (active) {
+                bst(rxx, o_c, so, nxx); bst(ryy, o_c, so, nyy); bst(rzz, o_c, so, nzz); bst(rtp, o_c, so, ntp);
+                bst(rxy, o_c, so, nxy); bst(rxz, o_c, so, nxz); bst(ryz, o_c, so, nyz);
+            }
+            if constexpr ((AX & 1) != 0) if (sxi >= 0) bstn<3>(rpsi, 12u * pqx, 4u * psi_stress(P, 0), ps);
+            if constexpr ((AX & 2) != 0) if (syi >= 0) bstn<3>(rpsi, 12u * pqy, 4u * psi_stress(P, 1), ps + 3);
+            if constexpr ((AX & 4) != 0) if (szi >= 0) bstn<3>(rpsi, 12u * pqz, 4u * psi_stress(P, 2), ps + 6);
         }
     }
 }
@@ -509,6 +542,13 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
 #pragma unroll
     for (int m = 1; m <= R; ++m) c[m] = cf.c[m];
     const float hxy = active ? g.hx[ic] + g.hy[jc] : 0.f;
+    // C-PML coefficients in LDS, as in the stress tile
+    __shared__ float4 pc_x[AX ? TX : 1], pc_y[AX ? TY : 1], pc_z[1];
+    if constexpr (AX != 0) {
+        const float4* cf4 = reinterpret_cast<const float4*>(P.psi + psi_total(P));
+        if (ty == 0) pc_x[tx] = cf4[ic];
+        if (ty == 1 && tx < TY) pc_y[tx] = cf4[g.nnx + clampi(j0 + tx, 0, g.nny - 1)];
+    }
     const int sxi = (AX & 1) && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
     const int syi = (AX & 2) && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
 
@@ -532,6 +572,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
         qzz[p + 1] = bld(rzz, o_c, so);
         qtp[p + 1] = bld(rtp, o_c, so);
     }
+    __builtin_amdgcn_s_waitcnt(0);  // see the stress tile
 
     for (int k = kb; k < ke; ++k) {
 #pragma unroll
@@ -544,6 +585,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
         qyz[2 * R] = bld(ryz, o_c, sf);
         qzz[2 * R] = bld(rzz, o_c, sf);
         qtp[2 * R] = bld(rtp, o_c, sf);
+        const float hzk = g.hz[k];
         const float cxx = bld(rxx, o_c, so), cyy = bld(ryy, o_c, so), cxy = bld(rxy, o_c, so);
         float hxx = 0.f, hxy_ = 0.f, hxz = 0.f, htp = 0.f;
         if (xh) {
@@ -563,25 +605,38 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             vxp = bldo(rpx, o_c, so); vyp = bldo(rpy, o_c, so); vzp = bldo(rpz, o_c, so);
             bo = bldo(rB, o_c, so);
         }
+        // the cell's excitation amplitude (source pass) or time (receiver pass),
+        // prefetched with the plane's loads: read after the stores it would wait for them
+        [[maybe_unused]] float ex_amp_c = 0.f;
+        [[maybe_unused]] int ex_time_c = -1;
+        if constexpr (MODE == MODE_EXCITE) ex_amp_c = bldo(mkacc<Acc>(ex_array(ex, EX_AMP) + base, span), o_c, so);
+        if constexpr (MODE == MODE_IMAGE) ex_time_c = bldi(mkacc<Acc>(ex_array(ex, EX_TIME) + base, span), o_c, so);
         float ps[12];
         unsigned pqx = 0, pqy = 0, pqz = 0;  // slab offsets (< 2^32 floats)
         int szi = -1;
-        if constexpr (AX != 0) {
+        if constexpr (AX != 0) {  // unconditional loads at clamped slab cells, as in the stress tile
             szi = (AX & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
-            if ((AX & 1) && sxi >= 0) {
-                pqx = ((unsigned)k * g.nny + j) * P.w[0] + sxi;
+            if constexpr ((AX & 1) != 0) {
+                pqx = ((unsigned)k * g.nny + jc) * P.w[0] + (unsigned)max(sxi, 0);
                 bldn<4>(rpsi, 16u * pqx, 4u * psi_velocity(P, 0), ps);
             }
-            if ((AX & 2) && syi >= 0) {
-                pqy = ((unsigned)k * P.w[1] + syi) * g.nnx + i;
+            if constexpr ((AX & 2) != 0) {
+                pqy = ((unsigned)k * P.w[1] + (unsigned)max(syi, 0)) * g.nnx + ic;
                 bldn<4>(rpsi, 16u * pqy, 4u * psi_velocity(P, 1), ps + 4);
             }
-            if ((AX & 4) && szi >= 0) {
-                pqz = ((unsigned)szi * g.nny + j) * g.nnx + i;
+            if constexpr ((AX & 4) != 0) {
+                pqz = ((unsigned)max(szi, 0) * g.nny + jc) * g.nnx + ic;
                 bldn<4>(rpsi, 16u * pqz, 4u * psi_velocity(P, 2), ps + 8);
             }
         }
+        [[maybe_unused]] float4 zq{};
+        if constexpr ((AX & 4) != 0) {
+            if (tx == 0 && ty == 0) zq = reinterpret_cast<const float4*>(P.psi + psi_total(P) + 4u * (g.nnx + g.nny))[k];
+        }
         __syncthreads();
+        if constexpr ((AX & 4) != 0) {
+            if (tx == 0 && ty == 0) pc_z[0] = zq;
+        }
         s_xx[ty][tx + R] = cxx;
         s_yy[ty + R][tx] = cyy;
         s_xy[ty + R][tx + R] = cxy;
@@ -599,7 +654,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             }
         }
         __syncthreads();
-        if (active) {
+        {  // all lanes compute, only the stores are guarded (see the stress tile)
             const int ci = tx + R, cj = ty + R;
             float a1 = 0.f, a2 = 0.f, a3 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f, d1 = 0.f, d2 = 0.f, d3 = 0.f,
                   p1 = 0.f, p2 = 0.f, p3 = 0.f;
@@ -623,34 +678,32 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             float gxz = g.cx * d1, gyz = g.cy * d2, gzz = g.cz * d3;
             float px = g.cx * p1, py = g.cy * p2, pz = g.cz * p3;
             if constexpr (AX != 0) {
-                if ((AX & 1) && sxi >= 0) {
-                    float cq[4];
-                    bldn<4>(rpsi, 16u * i, 4u * pml_coef(P, g.nnx, g.nny, 0), cq);  // (a_full, b_full, a_half, b_half) of x
-                    gxx = pml_upd(cq[2], cq[3], gxx, ps[0]);
-                    gxy = pml_upd(cq[0], cq[1], gxy, ps[1]);
-                    gxz = pml_upd(cq[0], cq[1], gxz, ps[2]);
-                    px = pml_upd(cq[2], cq[3], px, ps[3]);
-                    bstn<4>(rpsi, 16u * pqx, 4u * psi_velocity(P, 0), ps);
+                if constexpr ((AX & 1) != 0) {
+                    const float4 cq = pc_x[tx];  // (a_full, b_full, a_half, b_half) of x
+                    for (int q = 0; q < 4; ++q) ps[q] = sxi >= 0 ? ps[q] : 0.f;
+                    gxx = pml_upd(cq.z, cq.w, gxx, ps[0]);
+                    gxy = pml_upd(cq.x, cq.y, gxy, ps[1]);
+                    gxz = pml_upd(cq.x, cq.y, gxz, ps[2]);
+                    px = pml_upd(cq.z, cq.w, px, ps[3]);
                 }
-                if ((AX & 2) && syi >= 0) {
-                    float cq[4];
-                    bldn<4>(rpsi, 16u * j, 4u * pml_coef(P, g.nnx, g.nny, 1), cq);
-                    gyx = pml_upd(cq[0], cq[1], gyx, ps[4]);
-                    gyy = pml_upd(cq[2], cq[3], gyy, ps[5]);
-                    gyz = pml_upd(cq[0], cq[1], gyz, ps[6]);
-                    py = pml_upd(cq[2], cq[3], py, ps[7]);
-                    bstn<4>(rpsi, 16u * pqy, 4u * psi_velocity(P, 1), ps + 4);
+                if constexpr ((AX & 2) != 0) {
+                    const float4 cq = pc_y[ty];
+                    for (int q = 4; q < 8; ++q) ps[q] = syi >= 0 ? ps[q] : 0.f;
+                    gyx = pml_upd(cq.x, cq.y, gyx, ps[4]);
+                    gyy = pml_upd(cq.z, cq.w, gyy, ps[5]);
+                    gyz = pml_upd(cq.x, cq.y, gyz, ps[6]);
+                    py = pml_upd(cq.z, cq.w, py, ps[7]);
                 }
-                if ((AX & 4) && szi >= 0) {
-                    const float4 cq = reinterpret_cast<const float4*>(P.psi + pml_coef(P, g.nnx, g.nny, 2))[k];
+                if constexpr ((AX & 4) != 0) {
+                    const float4 cq = pc_z[0];
+                    for (int q = 8; q < 12; ++q) ps[q] = szi >= 0 ? ps[q] : 0.f;
                     gzx = pml_upd(cq.x, cq.y, gzx, ps[8]);
                     gzy = pml_upd(cq.x, cq.y, gzy, ps[9]);
                     gzz = pml_upd(cq.z, cq.w, gzz, ps[10]);
                     pz = pml_upd(cq.z, cq.w, pz, ps[11]);
-                    bstn<4>(rpsi, 16u * pqz, 4u * psi_velocity(P, 2), ps + 8);
                 }
             }
-            const float h = hxy + g.hz[k];
+            const float h = hxy + hzk;
             const float a = 1.0f / (1.0f + h), b = 1.0f - h;
             const float nvx = a * (b * vx + bo * (gxx + gyx + gzx));
             const float nvy = a * (b * vy + bo * (gxy + gyy + gzy));
@@ -658,6 +711,10 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             const float nxp = a * (b * vxp + bo * px);
             const float nyp = a * (b * vyp + bo * py);
             const float nzp = a * (b * vzp + bo * pz);
+            if constexpr ((AX & 1) != 0) if (sxi >= 0) bstn<4>(rpsi, 16u * pqx, 4u * psi_velocity(P, 0), ps);
+            if constexpr ((AX & 2) != 0) if (syi >= 0) bstn<4>(rpsi, 16u * pqy, 4u * psi_velocity(P, 1), ps + 4);
+            if constexpr ((AX & 4) != 0) if (szi >= 0) bstn<4>(rpsi, 16u * pqz, 4u * psi_velocity(P, 2), ps + 8);
+            if (active) {
             bst(rvx, o_c, so, nvx); bst(rvy, o_c, so, nvy); bst(rvz, o_c, so, nvz);
             bst(rpx, o_c, so, nxp); bst(rpy, o_c, so, nyp); bst(rpz, o_c, so, nzp);
             // excitation / imaging state: the per-cell test reads one word
@@ -665,13 +722,13 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             const long n = base + (long)(so / 4u) + (long)(o_c / 4u);
             if constexpr (MODE == MODE_EXCITE) {
                 const float amp = sqrtf(nxp * nxp + nyp * nyp + nzp * nzp);
-                if (bld(mkacc<Acc>(ex_array(ex, EX_AMP) + base, span), o_c, so) < amp) {
+                if (ex_amp_c < amp) {
                     reinterpret_cast<int*>(ex_array(ex, EX_TIME))[n] = it;
                     ex_array(ex, EX_AMP)[n] = amp; ex_array(ex, EX_TP)[n] = qtp[R];
                     ex_array(ex, EX_VXP)[n] = nxp; ex_array(ex, EX_VYP)[n] = nyp; ex_array(ex, EX_VZP)[n] = nzp;
                 }
             } else if constexpr (MODE == MODE_IMAGE) {
-                if (bldi(mkacc<Acc>(ex_array(ex, EX_TIME) + base, span), o_c, so) == it) {
+                if (ex_time_c == it) {
                     const float etp = ex_array(ex, EX_TP)[n];
                     ex_array(ex, EX_IMG_TP)[n] = etp != 0.f ? qtp[R] / etp : 0.f;
                     const float ex_ = ex_array(ex, EX_VXP)[n], ey_ = ex_array(ex, EX_VYP)[n], ez_ = ex_array(ex, EX_VZP)[n];
@@ -681,6 +738,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
                     ex_array(ex, EX_IMG_PP)[n] = den > 0.f ? pp / den : 0.f;
                     ex_array(ex, EX_IMG_PS)[n] = den > 0.f ? ips / den : 0.f;
                 }
+            }
             }
         }
     }

@@ -189,6 +189,7 @@ M2 model(const erm2d_ctx* c, int which) {
     M2 m;
     const auto& d = c->mod[which];
     m.M = d.M; m.U = d.U; m.B = d.B;
+    m.vp = d.vp; m.vs = d.vs;
     for (int l = 0; l < 3; ++l) { m.kP[l] = d.kP[l]; m.kS[l] = d.kS[l]; m.decay[l] = d.decay[l]; }
     return m;
 }
@@ -244,11 +245,12 @@ void do_velocity(erm2d_ctx* c, bool B, int which, int mode, int it, float* Fh) {
         else hipLaunchKernelGGL(k2_ex_avg<2>, grid_pt(c), dim3(256), 0, c->stream, state(c, B), excite(c), geo(c), it, c->R);
     }
 }
+int para(const erm2d_ctx* c) { return c->p.inversion_para == 1 || c->p.inversion_para == 3 ? c->p.inversion_para : 2; }
 void born_stress_src(erm2d_ctx* c, const float* Eh) {
     dim3 g = grid_pt(c);
-    if (c->L == 0) hipLaunchKernelGGL((k2_born_stress_src<0>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh);
-    else if (c->L == 1) hipLaunchKernelGGL((k2_born_stress_src<1>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh);
-    else hipLaunchKernelGGL((k2_born_stress_src<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh);
+    if (c->L == 0) hipLaunchKernelGGL((k2_born_stress_src<0>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh, para(c));
+    else if (c->L == 1) hipLaunchKernelGGL((k2_born_stress_src<1>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh, para(c));
+    else hipLaunchKernelGGL((k2_born_stress_src<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh, para(c));
 }
 void adj_v(erm2d_ctx* c, const float* Fh) {
     if (c->R == 4) hipLaunchKernelGGL((k2a_v<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, c->grad);
@@ -383,6 +385,7 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
     if (p.nx <= 0 || p.nz <= 0 || p.lt <= 0 || p.dt <= 0 || p.dx <= 0 || p.dz <= 0) { g_err2 = "bad grid"; return ERM_E_PARAM; }
     if (p.bu < 0 || p.bd < 0 || p.bl < 0 || p.br < 0) { g_err2 = "negative boundary width"; return ERM_E_PARAM; }
     if (p.boundary != ERM_BOUNDARY_DAMP && p.boundary != ERM_BOUNDARY_CPML) { g_err2 = "boundary must be 0 (damping) or 1 (CPML)"; return ERM_E_PARAM; }
+    if (p.inversion_para < 0 || p.inversion_para > 3) { g_err2 = "inversion_para must be 1, 2 or 3 (0: 2)"; return ERM_E_PARAM; }
     int nd = 0;
     if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) { g_err2 = "no HIP device"; return ERM_E_DEVICE; }
     if (device < 0 || device >= nd) { g_err2 = "device out of range"; return ERM_E_PARAM; }
@@ -797,6 +800,9 @@ int erm2d_adjoint(erm2d_ctx* c, const float* r, float* gout) {
         adj_s1(c, h);
         adj_s2(c);
     }
+    if (para(c) != 2)
+        hipLaunchKernelGGL(k2_grad_para, dim3((unsigned)std::min<size_t>((c->mn + 255) / 256, 2048)), dim3(256), 0,
+                           c->stream, c->grad, model(c, 1), geo(c), para(c));
     HC(c, hipGetLastError());
     HC(c, hipEventRecord(c->ev[1], c->stream));
     HC(c, hipEventSynchronize(c->ev[1]));

@@ -48,7 +48,11 @@ typedef struct erm2d_params {
                           LSRTM operators need the damping boundary) */
     float cpml_r;      /* CPML target reflection coefficient (0: 1e-3) */
     float cpml_vmax;   /* CPML reference velocity in m/s (0: max vp of the loaded models) */
-    int   reserved[2];
+    int   inversion_para; /* LSRTM model parameters, relative perturbations (inversion_para of
+                          cuda_cal_dem_parameter_viscoelastic_media_new, viscoelastic_equation.cu:2431-2486):
+                          1 = (dlambda/lambda, dmu/mu, drho/rho); 2 (or 0, default) = (dvp/vp, dvs/vs,
+                          drho/rho); 3 = impedances (dIp/Ip, dIs/Is, drho/rho), Ip = rho vp, Is = rho vs */
+    int   reserved[1];
 } erm2d_params;
 
 /* fields for erm2d_get_field / erm2d_set_field (padded grid, z fastest) */
@@ -105,7 +109,7 @@ int   erm2d_forward(erm2d_ctx* ctx, float* rec);
 int   erm2d_rtm(erm2d_ctx* ctx, const float* rec, float* img_tp, float* img_pp, float* img_ps);
 
 /* LSRTM operators in the background model, model perturbation dm = [3][nx][nz]
- * (dvp/vp, dvs/vs, drho/rho; the reference's inversion_para = 2,
+ * of the inversion_para parameters (default 2: dvp/vp, dvs/vs, drho/rho;
  * viscoelastic_equation.cu:2431-2489):
  *   erm2d_born    d = L dm   (demig_fwd_... :328-477 with cuda_cal_dem_parameter_..._new :2398-2505)
  *   erm2d_adjoint g = L^T r  (adjoint_fwd_vx/vz/memory :1827-2198, cal_gradient_..._new :2248-2295),

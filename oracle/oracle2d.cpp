@@ -493,6 +493,28 @@ struct Oracle2D {
         return true;
     }
 
+    /* Relative perturbations of the P and S moduli (rho vp^2, rho vs^2) from
+     * the three inverted parameters (inversion_para, the mapping of
+     * cuda_cal_dem_parameter_viscoelastic_media_new, viscoelastic_equation.cu:
+     * 2431-2486, written for relative perturbations): 1 (lambda, mu, rho):
+     * sP = (lambda m1 + 2 mu m2) / (lambda + 2 mu), sS = m2; 2 (vp, vs, rho):
+     * sP = 2 m1 + m3, sS = 2 m2 + m3; 3 (Ip, Is, rho): sP = 2 m1 - m3,
+     * sS = 2 m2 - m3.  The density perturbation is m3 in all three. */
+    int para() const { return p.inversion_para == 1 || p.inversion_para == 3 ? p.inversion_para : 2; }
+    void rel_moduli(size_t n, T m1, T m2, T m3, T* sP, T* sS) const {
+        if (para() == 1) {
+            const T a = vp_m[1][n], b = vs_m[1][n];
+            *sP = ((a * a - 2 * b * b) * m1 + 2 * b * b * m2) / (a * a);
+            *sS = m2;
+        } else if (para() == 3) {
+            *sP = 2 * m1 - m3;
+            *sS = 2 * m2 - m3;
+        } else {
+            *sP = 2 * m1 + m3;
+            *sS = 2 * m2 + m3;
+        }
+    }
+
     /* d = L dm: background and Born wavefields stepped together. */
     int born(const float* dmv, float* d) {
         if (cpml) return -1;  /* the LSRTM operators use the damping boundary */
@@ -519,7 +541,8 @@ struct Oracle2D {
                     size_t n = id(i, k);
                     T b, a = ab(n, &b);
                     T E0 = Eh[q], E1 = Eh[mn + q], E2 = Eh[2 * mn + q];
-                    T sP = 2 * P.mp[q] + P.mr[q], sS = 2 * P.ms[q] + P.mr[q];
+                    T sP, sS;
+                    rel_moduli(n, P.mp[q], P.ms[q], P.mr[q], &sP, &sS);
                     T dM = m.M[n] * sP, dU = m.U[n] * sS, dL = dM - 2 * dU;
                     T sp = 0, sxx = 0, szz = 0, sxz = 0;
                     for (int l = 0; l < L; ++l) {
@@ -657,8 +680,23 @@ struct Oracle2D {
                     lv[1][n] += -cz * dp(W[1].data() + n, 1) - cx * dp(W[2].data() + n, sX);
                 }
         }
+        /* gp = 2 GP, gs = 2 GS, gr_ = GP + GS + GD in the (vp, vs, rho) parameters, GP / GS the
+         * gradients of the relative P / S modulus perturbation and GD the direct density part;
+         * the chain rule of rel_moduli gives the other parameterisations */
         for (size_t q = 0; q < mn; ++q) {
-            gout[q] = (float)gp[q]; gout[mn + q] = (float)gs[q]; gout[2 * mn + q] = (float)gr_[q];
+            const int i = (int)(q / p.nz) + p.bl, k = (int)(q % p.nz) + p.bu;
+            const T GP = gp[q] / 2, GS = gs[q] / 2, GD = gr_[q] - GP - GS;
+            T g1 = gp[q], g2 = gs[q], g3 = gr_[q];
+            if (para() == 1) {
+                const size_t n = id(i, k);
+                const T a = vp_m[1][n], b = vs_m[1][n];
+                g1 = GP * (a * a - 2 * b * b) / (a * a);
+                g2 = GP * 2 * b * b / (a * a) + GS;
+                g3 = GD;
+            } else if (para() == 3) {
+                g3 = GD - GP - GS;
+            }
+            gout[q] = (float)g1; gout[mn + q] = (float)g2; gout[2 * mn + q] = (float)g3;
         }
         return 0;
     }

@@ -10,7 +10,8 @@
   and 6 with 12 updated absorbing cells) while the run with no absorption
   differs by > 50x more (0.70); the reference's cubic damping sits in between
   (0.075 / 0.15).
-* GPU: wavefields and record <= 1e-5 vs the FP64 oracle (vector P <= 1e-4),
+* GPU: wavefields and record <= 1e-5 vs the FP64 oracle (vector P <= 1e-4 or
+  10x the FP32 oracle's own error, as in tests/test_gpu2d.py),
   elastic and viscoelastic; the LSRTM operators refuse the C-PML boundary.
 """
 import numpy as np
@@ -76,9 +77,13 @@ def test_gpu_cpml2d_matches_fp64_oracle(erm, oracle2d_mod, L, R):
     for k in ["vx", "vz", "txx", "tzz", "txz", "tp"]:
         e = cases.rel_l2(g.get_field(k), o.get_field(k))
         assert e < 1e-5, (k, e)
-    for k in ["vxp", "vzp"]:
-        e = cases.rel_l2(g.get_field(k), o.get_field(k))
-        assert e < 1e-4, (k, e)
+    o32 = oracle2d_mod.Oracle2D(p, 32)
+    o32.load_model(0, vp, vs, rho, qp, qs)
+    o32.step(0, 0, p.lt)
+    for k in ["vxp", "vzp"]:   # vector P integrates grad tp: as the FP32 oracle itself (see test_gpu2d)
+        ref = o.get_field(k)
+        e, e32 = cases.rel_l2(g.get_field(k), ref), cases.rel_l2(o32.get_field(k), ref)
+        assert e < max(1e-4, 10 * e32), (k, e, e32)
     g.load_model(1, *[cases2d.smooth2(a) for a in (vp, vs, rho)], qp, qs)
     with pytest.raises(erm.ErmError):
         g.born(np.zeros((3, p.nx, p.nz), np.float32))

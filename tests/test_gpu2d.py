@@ -106,14 +106,17 @@ def test_gpu2d_rtm_images_match_oracle(erm, oracle2d_mod, avg, shift):
     g.close()
 
 
-@pytest.mark.parametrize("L", [0, 1, 3])
-def test_gpu2d_born_and_adjoint_match_oracle(erm, oracle2d_mod, L):
+@pytest.mark.parametrize("L,para", [(0, 2), (1, 2), (3, 2), (3, 1), (0, 3)])
+def test_gpu2d_born_and_adjoint_match_oracle(erm, oracle2d_mod, L, para):
+    """Born data and adjoint gradient of each LSRTM parameterisation
+    (inversion_para 1: lambda, mu, rho; 2: vp, vs, rho; 3: impedances)."""
     import test_oracle2d as T
     p, bg = T._lsrtm_case(L, lt=200)
+    p.inversion_para = para
     rng = np.random.default_rng(5)
     dm = (rng.standard_normal((3, p.nx, p.nz)) * cases2d.taper_mask(p)).astype(np.float32)
     r = rng.standard_normal((2, p.lt, p.nrec)).astype(np.float32)
-    _say(f"2D Born/adjoint parity L={L}")
+    _say(f"2D Born/adjoint parity L={L} inversion_para={para}")
     g = erm.Elastic2D(p)
     g.load_model(1, *bg)
     d = g.born(dm)

@@ -156,9 +156,27 @@ def _lsrtm_case(L, R=4, lt=160):
     return p, (cases2d.smooth2(vp), cases2d.smooth2(vs), cases2d.smooth2(rho), qp, qs)
 
 
-@pytest.mark.parametrize("L", [0, 1, 3])
-def test_born_matches_finite_difference(oracle2d_mod, L):
+def perturbed_model(bg, dm, eps, para):
+    """(vp, vs, rho) of the background with the relative perturbation eps*dm of
+    the inversion_para parameters applied to the parameters themselves:
+    1 lambda, mu, rho; 2 vp, vs, rho; 3 Ip = rho vp, Is = rho vs, rho."""
+    vp, vs, rho = (np.asarray(a, np.float64) for a in bg[:3])
+    r2 = rho * (1 + eps * dm[2])
+    if para == 1:
+        lam, mu = rho * (vp ** 2 - 2 * vs ** 2) * (1 + eps * dm[0]), rho * vs ** 2 * (1 + eps * dm[1])
+        return np.sqrt((lam + 2 * mu) / r2), np.sqrt(mu / r2), r2
+    if para == 3:
+        return rho * vp * (1 + eps * dm[0]) / r2, rho * vs * (1 + eps * dm[1]) / r2, r2
+    return vp * (1 + eps * dm[0]), vs * (1 + eps * dm[1]), r2
+
+
+@pytest.mark.parametrize("L,para", [(0, 2), (1, 2), (3, 2), (0, 1), (3, 1), (0, 3), (3, 3)])
+def test_born_matches_finite_difference(oracle2d_mod, L, para):
+    """Born data of each parameterisation against a centred finite difference
+    of modelling with the perturbation applied to that parameterisation's own
+    parameters (a wrong chain rule between the parameter sets fails)."""
     p, bg = _lsrtm_case(L)
+    p.inversion_para = para
     rng = np.random.default_rng(7)
     w = cases2d.taper_mask(p)
     dm = (rng.standard_normal((3, p.nx, p.nz)) * w).astype(np.float32)
@@ -169,9 +187,7 @@ def test_born_matches_finite_difference(oracle2d_mod, L):
     h = 1e-4
     for sgn in (+1, -1):
         of = oracle2d_mod.Oracle2D(p)
-        vp = bg[0] * (1 + sgn * h * dm[0])
-        vs = bg[1] * (1 + sgn * h * dm[1])
-        rho = bg[2] * (1 + sgn * h * dm[2])
+        vp, vs, rho = perturbed_model(bg, dm, sgn * h, para)
         of.load_model(0, vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32), bg[3], bg[4])
         of.step(0, 0, p.lt)
         recs[sgn] = of.record()
@@ -181,9 +197,10 @@ def test_born_matches_finite_difference(oracle2d_mod, L):
     assert err < 5e-3, err
 
 
-@pytest.mark.parametrize("L,R", [(0, 4), (1, 4), (3, 4), (0, 6)])
-def test_adjoint_is_exact_transpose(oracle2d_mod, L, R):
+@pytest.mark.parametrize("L,R,para", [(0, 4, 2), (1, 4, 2), (3, 4, 2), (0, 6, 2), (3, 4, 1), (3, 4, 3)])
+def test_adjoint_is_exact_transpose(oracle2d_mod, L, R, para):
     p, bg = _lsrtm_case(L, R=R, lt=120)
+    p.inversion_para = para
     rng = np.random.default_rng(11)
     dm = rng.standard_normal((3, p.nx, p.nz)).astype(np.float32)
     r = rng.standard_normal((2, p.lt, p.nrec)).astype(np.float32)
