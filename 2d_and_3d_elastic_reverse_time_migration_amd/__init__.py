@@ -104,6 +104,9 @@ def _load():
         "erm2d_get_dims": (ctypes.c_int, [vp, ip, ip]),
         "erm2d_forward": (ctypes.c_int, [vp, fp]),
         "erm2d_rtm": (ctypes.c_int, [vp, fp, fp, fp, fp]),
+        "erm2d_angle_gathers_noalloc": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_float, fp, fp, fp, fp, fp]),
+        "erm2d_rtm_windows": (ctypes.c_int, [vp, ctypes.c_int] + [fp] * 10 + [ctypes.c_int, vp, fp, ctypes.c_int,
+                                                                          fp, fp, fp, fp]),
         "erm2d_born": (ctypes.c_int, [vp, fp, fp]),
         "erm2d_adjoint": (ctypes.c_int, [vp, fp, fp]),
         "erm2d_last_ms": (ctypes.c_int, [vp, P(ctypes.c_double)]),
@@ -574,6 +577,30 @@ class Elastic2D:
         v = ctypes.c_double()
         self._c(lib.erm2d_last_ms(self._h, ctypes.byref(v)))
         return v.value
+
+
+class Window2D(ctypes.Structure):
+    """Mirror of `erm2d_window`."""
+    _fields_ = [("shot_x", ctypes.c_int), ("shot_z", ctypes.c_int), ("x0", ctypes.c_int), ("width", ctypes.c_int)]
+
+
+def rtm_windows(p, true, background, windows, records=None, z2=-1, device=0):
+    """Windowed multi-shot 2D RTM over a full model (erm2d_rtm_windows):
+    true / background = (vp, vs, rho[, qp, qs]) [nx][nz]; windows =
+    [(shot_x, shot_z, x0, width), ...]; records: None (modelled) or the shots'
+    records concatenated.  Returns stacked (tp, pp, ps, illumination) [nx][nz]."""
+    if ctypes.sizeof(p) != ctypes.sizeof(Params2D):
+        raise TypeError("not an erm2d_params struct")
+    arrs = []
+    for m in (true, background):
+        m = list(m) + [None] * (5 - len(m))
+        arrs += [None if a is None else np.ascontiguousarray(a, np.float32) for a in m]
+    w = (Window2D * len(windows))(*[Window2D(*map(int, t)) for t in windows])
+    rec = None if records is None else np.ascontiguousarray(np.concatenate([np.ravel(r) for r in records]), np.float32)
+    outs = [np.zeros((p.nx, p.nz), np.float32) for _ in range(4)]
+    _check2(lib.erm2d_rtm_windows(ctypes.addressof(p), device, *[_fptr(a) for a in arrs], len(windows),
+                                  ctypes.cast(w, ctypes.c_void_p), _fptr(rec), int(z2), *[_fptr(o) for o in outs]))
+    return outs
 
 
 def _checku(rc):

@@ -905,6 +905,99 @@ int erm2d_precond_diag(const float* g, float* z, void* user) {
     return 0;
 }
 
+int erm2d_rtm_windows(const erm2d_params* p, int device, const float* vp, const float* vs, const float* rho,
+                      const float* qp, const float* qs, const float* bvp, const float* bvs, const float* brho,
+                      const float* bqp, const float* bqs, int nshots, const erm2d_window* win, const float* records,
+                      int z2, float* img_tp, float* img_pp, float* img_ps, float* illum) {
+    if (!p || !vp || !vs || !rho || !bvp || !bvs || !brho || nshots <= 0 || !win) { g_err2 = "bad arguments"; return ERM_E_PARAM; }
+    if (p->nsls > 0 && (!qp || !qs || !bqp || !bqs)) { g_err2 = "qp/qs required when nsls > 0"; return ERM_E_PARAM; }
+    if (p->rec_dx <= 0 || p->nx <= 0 || p->nz <= 0) { g_err2 = "bad grid or receiver spacing"; return ERM_E_PARAM; }
+    const int NX = p->nx, NZ = p->nz;
+    for (int s = 0; s < nshots; ++s) {
+        const erm2d_window& w = win[s];
+        if (w.x0 < 0 || w.width < p->rec_dx || w.x0 + w.width > NX || w.shot_x < w.x0 || w.shot_x >= w.x0 + w.width ||
+            w.shot_z < 0 || w.shot_z >= NZ) {
+            g_err2 = erm::format("window %d outside the model or not containing its shot", s);
+            return ERM_E_PARAM;
+        }
+    }
+    const size_t fm = (size_t)NX * NZ;
+    std::vector<double> acc[4];
+    for (auto& a : acc) a.assign(fm, 0.0);
+    std::vector<erm2d_ctx*> ctxs;  // one context per window width
+    std::vector<int> widths;
+    auto release = [&]() { for (auto* c : ctxs) erm2d_destroy(c); };
+    size_t rec_off = 0;
+    int rc = 0;
+    erm2d_ctx* cur = nullptr;
+    for (int s = 0; s < nshots && !rc; ++s) {
+        const erm2d_window& w = win[s];
+        erm2d_ctx* c = nullptr;
+        for (size_t q = 0; q < widths.size(); ++q)
+            if (widths[q] == w.width) c = ctxs[q];
+        const int nrec = w.width / p->rec_dx;
+        if (!c) {
+            erm2d_params q = *p;
+            q.nx = w.width;
+            q.nrec = nrec;
+            q.rec_x0 = 0;
+            q.shot_x = w.shot_x - w.x0;
+            q.shot_z = w.shot_z;
+            if ((rc = erm2d_create(&q, device, &c))) break;
+            ctxs.push_back(c);
+            widths.push_back(w.width);
+        }
+        cur = c;
+        // window models: cuda_get_partly_mode_boundary (+ _z1_z2 on the background)
+        const size_t wm = (size_t)w.width * NZ;
+        std::vector<float> cut[10];
+        const float* src[10] = {vp, vs, rho, qp, qs, bvp, bvs, brho, bqp, bqs};
+        for (int a = 0; a < 10; ++a) {
+            if (!src[a] || (p->nsls == 0 && (a == 3 || a == 4 || a == 8 || a == 9))) continue;
+            cut[a].resize(wm);
+            for (int ix = 0; ix < w.width; ++ix)
+                for (int iz = 0; iz < NZ; ++iz) {
+                    const int zz = (a >= 5 && z2 >= 0 && iz > z2) ? std::min(z2, NZ - 1) : iz;
+                    cut[a][(size_t)ix * NZ + iz] = src[a][(size_t)(ix + w.x0) * NZ + zz];
+                }
+        }
+        auto ptr = [&](int a) { return cut[a].empty() ? nullptr : cut[a].data(); };
+        if ((rc = erm2d_set_shot(c, w.shot_x - w.x0, w.shot_z))) break;
+        const float* rec = nullptr;
+        std::vector<float> modelled;
+        if (records) {
+            rec = records + rec_off;
+            rec_off += (size_t)2 * p->lt * nrec;
+        } else {
+            if ((rc = erm2d_load_model(c, 0, ptr(0), ptr(1), ptr(2), ptr(3), ptr(4)))) break;
+            modelled.resize((size_t)2 * p->lt * nrec);
+            if ((rc = erm2d_forward(c, modelled.data()))) break;
+            rec = modelled.data();
+        }
+        if ((rc = erm2d_load_model(c, 1, ptr(5), ptr(6), ptr(7), ptr(8), ptr(9)))) break;
+        std::vector<float> im[4];
+        for (auto& v : im) v.assign(wm, 0.f);
+        if ((rc = erm2d_rtm(c, rec, im[0].data(), im[1].data(), im[2].data()))) break;
+        if (illum) {
+            const int sh[2] = {w.shot_x - w.x0, w.shot_z};
+            if ((rc = erm2d_illumination(c, 1, sh, im[3].data()))) break;
+        }
+        // cuda_sum_new_acqusition[_illum]: add the window into the full model at x0
+        for (int a = 0; a < 4; ++a)
+            for (size_t t = 0; t < wm; ++t) acc[a][(size_t)w.x0 * NZ + t] += im[a][t];
+    }
+    if (rc) {
+        if (cur && !cur->err.empty()) g_err2 = cur->err;
+        release();
+        return rc;
+    }
+    release();
+    float* outs[4] = {img_tp, img_pp, img_ps, illum};
+    for (int a = 0; a < 4; ++a)
+        if (outs[a]) for (size_t t = 0; t < fm; ++t) outs[a][t] = (float)acc[a][t];
+    return 0;
+}
+
 int erm2d_last_ms(const erm2d_ctx* c, double* ms) {
     if (!c || !ms) return ERM_E_PARAM;
     *ms = c->last_ms;

@@ -201,6 +201,34 @@ int   erm2d_set_workers(erm2d_ctx* ctx, int n);
 typedef int (*erm2d_cg_monitor_fn)(int it, const float* m, const float* g, float misfit, void* user);
 int   erm2d_set_cg_monitor(erm2d_ctx* ctx, erm2d_cg_monitor_fn fn, void* user);
 
+/* Windowed multi-shot RTM over a full model, the reference's per-shot
+ * aperture (cuda_get_partly_mode[_boundary][_z1_z2] and
+ * cuda_sum_new_acqusition[_illum], viscoelastic_equation.cu:2606-2754): for
+ * each shot the window [x0, x0 + width) x full depth is cut from the full
+ * models (padded by edge replication like erm2d_load_model), the shot is
+ * modelled in the true window model (or its record taken from `records`),
+ * migrated in the background window model (erm2d_rtm: excitation imaging),
+ * and the window's tp / PP / PS images and source illumination (as
+ * erm2d_illumination) are added into the full-model outputs at column x0.
+ * Receivers of a window: every p->rec_dx cells from its left edge at depth
+ * p->rec_z, nrec = width / rec_dx (p->nx, p->nrec, p->rec_x0 and p->shot_* are
+ * replaced per window; p->nx is the full model's width).  z2 >= 0 replaces the
+ * background window model below depth z2 by its value at z2 (the
+ * _boundary_z1_z2 cut); z2 < 0 keeps it.
+ *   vp, vs, rho, qp, qs     true model [nx][nz] (qp, qs only when nsls > 0)
+ *   bvp, bvs, brho, bqp, bqs background model [nx][nz]
+ *   records  NULL (model every shot) or the shots' records one after another,
+ *            shot s: [2][lt][width_s / rec_dx]
+ *   img_tp, img_pp, img_ps, illum  [nx][nz] outputs (any may be NULL), overwritten */
+typedef struct erm2d_window {
+    int shot_x, shot_z;  /* source, full-model cell coordinates */
+    int x0, width;       /* window origin (receiver_x_cord) and width (receiver_interval x receiver_num) */
+} erm2d_window;
+int   erm2d_rtm_windows(const erm2d_params* p, int device, const float* vp, const float* vs, const float* rho,
+                        const float* qp, const float* qs, const float* bvp, const float* bvs, const float* brho,
+                        const float* bqp, const float* bqs, int nshots, const erm2d_window* windows,
+                        const float* records, int z2, float* img_tp, float* img_pp, float* img_ps, float* illum);
+
 /* Timing of the last operator call (HIP events): milliseconds. */
 int   erm2d_last_ms(const erm2d_ctx* ctx, double* ms);
 

@@ -208,3 +208,59 @@ class Oracle2D:
             f = self.get_field(k)
             out.append(np.ascontiguousarray(f[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz]))
         return out
+
+
+def rtm_windows(p, true, background, windows, records=None, z2=-1, prec=64, ex_times=None):
+    """Windowed multi-shot RTM over a full model, restating the per-shot
+    aperture of the reference (cuda_get_partly_mode_boundary[_z1_z2] and
+    cuda_sum_new_acqusition[_illum], viscoelastic_equation.cu:2606-2754) as
+    the library defines it (include/erm2d.h, erm2d_rtm_windows): window
+    [x0, x0 + width) x full depth of the full models, receivers every rec_dx
+    cells from the window's left edge, model pass in the true window (unless
+    records are given), excitation RTM in the background window (below depth
+    z2 replaced by its value at z2 when z2 >= 0), images and source
+    illumination added into the full model at column x0.  Returns (tp, pp,
+    ps, illumination) [nx][nz] in FP64; ex_times (a list) receives each
+    window's excitation-time field over its model region."""
+    import ctypes
+    nx, nz = p.nx, p.nz
+    acc = [np.zeros((nx, nz)) for _ in range(4)]
+    off = 0
+    for (sx, sz, x0, width) in windows:
+        q = type(p)()
+        ctypes.memmove(ctypes.addressof(q), ctypes.addressof(p), ctypes.sizeof(q))
+        q.nx, q.nrec, q.rec_x0 = width, width // p.rec_dx, 0
+        q.shot_x, q.shot_z = sx - x0, sz
+        cut_t = [None if a is None else np.asarray(a)[x0:x0 + width] for a in list(true) + [None] * (5 - len(true))]
+        cut_b = []
+        for a in list(background) + [None] * (5 - len(background)):
+            if a is None:
+                cut_b.append(None)
+                continue
+            b = np.array(np.asarray(a)[x0:x0 + width], copy=True)
+            if z2 >= 0:
+                b[:, z2 + 1:] = b[:, min(z2, nz - 1)][:, None]
+            cut_b.append(b)
+        o = Oracle2D(q, prec)
+        o.load_model(1, *cut_b)
+        if records is None:
+            o.load_model(0, *cut_t)
+            o.step(0, 0, q.lt)
+            rec = o.record()
+        else:
+            n = 2 * q.lt * q.nrec
+            rec = np.asarray(records).ravel()[off:off + n].reshape(2, q.lt, q.nrec)
+            off += n
+            o.record(rec)
+        o.zero(7)
+        o.step(1, 0, q.lt_forward)
+        o.zero(1)
+        o.step(2, q.lt - 1, q.lt)
+        imgs = o.images()
+        if ex_times is not None:
+            ex_times.append(o.get_field("ex_time")[q.bl:q.bl + q.nx, q.bu:q.bu + q.nz])
+        ill = o.illumination([(sx - x0, sz)])
+        for a, v in zip(acc, list(imgs) + [ill]):
+            a[x0:x0 + width] += v
+        o.close()
+    return acc

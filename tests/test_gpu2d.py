@@ -224,3 +224,45 @@ def test_gpu2d_new_entry_points_reject_bad_input(erm):
             g.angle_gathers(90, 3.0)                # 270 degrees > 180
         with pytest.raises(erm.ErmError):
             g.angle_gathers(10, -1.0)
+
+
+def test_gpu2d_rtm_windows_match_oracle(erm, oracle2d_mod):
+    """erm2d_rtm_windows (per-shot windows cut from the full model, images and
+    illumination stacked at the window origin) on three shots, two sharing a
+    window width: equal to the sum of the single-window GPU migrations placed
+    at x0, and to the oracle's restatement (images <= 1e-4 on cells whose
+    excitation step agrees in every covering window; illumination <= 1e-5)."""
+    import test_oracle2d as T
+    p, tr, bg = T._window_case()
+    wins = [(20, 3, 6, 30), (50, 3, 30, 36), (40, 4, 24, 30)]
+    z2 = 28
+    _say(f"2D windowed RTM: {len(wins)} shots")
+    got = erm.rtm_windows(p, tr, bg, wins, z2=z2)
+    ex_o = []
+    ref = oracle2d_mod.rtm_windows(p, tr, bg, wins, z2=z2, ex_times=ex_o)
+    same = np.ones((p.nx, p.nz), bool)
+    manual = [np.zeros((p.nx, p.nz)) for _ in range(3)]
+    for (sx, sz, x0, wd), eo in zip(wins, ex_o):
+        q = T.cases2d.params(nx=wd, nz=p.nz, b=10, R=4, L=0, lt=260, freq=25.0, rec_dx=2, rec_z=2, src=(sx - x0, sz))
+        q.nrec, q.rec_x0 = wd // 2, 0
+        cut_b = []
+        for a in bg:
+            b = np.array(a[x0:x0 + wd], copy=True)
+            b[:, z2 + 1:] = b[:, z2][:, None]
+            cut_b.append(b)
+        with erm.Elastic2D(q) as g:
+            g.load_model(0, *[a[x0:x0 + wd] for a in tr])
+            g.load_model(1, *cut_b)
+            rec = g.forward()
+            imgs = g.rtm(rec)
+            eg = g.get_field("ex_time")[q.bl:q.bl + q.nx, q.bu:q.bu + q.nz]
+        for k in range(3):
+            manual[k][x0:x0 + wd] += imgs[k]
+        same[x0:x0 + wd] &= eg == eo
+    for k in range(3):
+        np.testing.assert_array_equal(got[k], manual[k].astype(np.float32))
+    assert same.mean() > 0.95, same.mean()
+    for k in range(3):
+        e = cases.rel_l2(got[k][same], ref[k][same])
+        assert e < 1e-4, (k, e)
+    assert cases.rel_l2(got[3], ref[3]) < 1e-5

@@ -564,3 +564,54 @@ def test_decom_discrete_identities(oracle2d_mod):
     kk1 = np.clip(np.arange(nnz) + 1 - p.bu, 0, p.nz - 1)[None, :]
     vs_mid = 1500.0 + 3.0 * (ii + ii1) / 2 + 5.0 * (kk + kk1) / 2
     np.testing.assert_allclose(o.get_field("scurl")[rin], vs_mid[rin] * (cc - e), rtol=1e-6)
+
+
+def _window_case(L=0):
+    p = cases2d.params(nx=70, nz=36, b=10, R=4, L=L, lt=260, freq=25.0, rec_dx=2, rec_z=2)
+    vp, vs, rho, qp, qs = cases2d.model(p, seed=8, q=L > 0)
+    bg = [cases2d.smooth2(a) for a in (vp, vs, rho)] + ([qp, qs] if L else [])
+    tr = [vp, vs, rho] + ([qp, qs] if L else [])
+    return p, tr, bg
+
+
+def test_rtm_windows_composition(oracle2d_mod):
+    """Windowed multi-shot RTM (cuda_get_partly_mode_boundary[_z1_z2] +
+    cuda_sum_new_acqusition[_illum], viscoelastic_equation.cu:2606-2754), as
+    restated: a window spanning the whole model equals the full-aperture RTM
+    of the same shot; a shot's contribution lies inside its window (zero
+    elsewhere) and equals the RTM of that window cut alone; stacking is the
+    sum of the shots; the z2 cut equals migrating in the model with every
+    row below z2 replaced by row z2."""
+    p, tr, bg = _window_case()
+    full = oracle2d_mod.rtm_windows(p, tr, bg, [(35, 3, 0, 70)])
+    q = cases2d.params(nx=70, nz=36, b=10, R=4, L=0, lt=260, freq=25.0, rec_dx=2, rec_z=2, src=(35, 3))
+    q.nrec, q.rec_x0 = 35, 0
+    o = oracle2d_mod.Oracle2D(q)
+    o.load_model(0, *tr)
+    o.load_model(1, *bg)
+    o.step(0, 0, q.lt)
+    o.zero(7)
+    o.step(1, 0, q.lt)
+    o.zero(1)
+    o.step(2, q.lt - 1, q.lt)
+    for a, b in zip(full[:3], o.images()):
+        np.testing.assert_array_equal(a, b)
+    wins = [(20, 3, 6, 30), (50, 3, 30, 36)]
+    each = [oracle2d_mod.rtm_windows(p, tr, bg, [w]) for w in wins]
+    both = oracle2d_mod.rtm_windows(p, tr, bg, wins)
+    for k in range(4):
+        np.testing.assert_allclose(both[k], each[0][k] + each[1][k], rtol=1e-12, atol=1e-300)
+    for (sx, sz, x0, wd), e in zip(wins, each):
+        outside = np.ones(p.nx, bool)
+        outside[x0:x0 + wd] = False
+        assert all(np.all(e[k][outside] == 0) for k in range(4))
+        assert np.count_nonzero(e[1][x0:x0 + wd]) > 0 and np.all(e[3][x0:x0 + wd] > 0)
+    z2 = 20
+    cut = oracle2d_mod.rtm_windows(p, tr, bg, [wins[0]], z2=z2)
+    bgz = [np.array(a, copy=True) for a in bg]
+    for a in bgz:
+        a[:, z2 + 1:] = a[:, z2][:, None]
+    ref = oracle2d_mod.rtm_windows(p, tr, bgz, [wins[0]])
+    for k in range(4):
+        np.testing.assert_array_equal(cut[k], ref[k])
+    assert not np.allclose(cut[1], each[0][1])
