@@ -136,6 +136,9 @@ def _load():
         "erm2d_angle_gathers": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_float, fp, fp, fp, fp, fp]),
         "erm2d_precond_diag": (ctypes.c_int, [fp, fp, vp]),
         "erm2d_set_workers": (ctypes.c_int, [vp, ctypes.c_int]),
+        "erm2d_set_batch": (ctypes.c_int, [vp, ctypes.c_int]),
+        "erm2d_born_shots": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), fp, fp]),
+        "erm2d_adjoint_shots": (ctypes.c_int, [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), fp, fp]),
         "erm_batch_last_error": (ctypes.c_char_p, [vp]),
         "erm_batch_create": (ctypes.c_int, [vp, ctypes.c_int, P(vp)]),
         "erm_batch_destroy": (ctypes.c_int, [vp]),
@@ -488,6 +491,31 @@ class Elastic2D:
     def set_workers(self, n):
         """Run the shots of lsrtm_cg on n concurrent contexts/streams."""
         self._c(lib.erm2d_set_workers(self._h, n))
+
+    def set_batch(self, nb):
+        """Batched shots (erm2d_set_batch): born_shots / adjoint_shots and
+        lsrtm_cg run nb shots per launch (1: one shot per launch)."""
+        self._c(lib.erm2d_set_batch(self._h, nb))
+
+    @staticmethod
+    def _shots(shots):
+        a = np.ascontiguousarray(np.asarray(shots, np.int32).reshape(-1, 2))
+        return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+    def born_shots(self, shots, dm):
+        """Born records of every shot [(x, z), ...]: [nshots][2][lt][nrec]."""
+        a, ptr = self._shots(shots)
+        d = np.empty((len(a),) + tuple(self.record_shape()), np.float32)
+        self._c(lib.erm2d_born_shots(self._h, len(a), ptr, _fptr(np.ascontiguousarray(dm, np.float32)), _fptr(d)))
+        return d
+
+    def adjoint_shots(self, shots, r):
+        """Sum over shots of the adjoint of residual records r [nshots][2][lt][nrec]."""
+        a, ptr = self._shots(shots)
+        p = self.p
+        g = np.empty((3, p.nx, p.nz), np.float32)
+        self._c(lib.erm2d_adjoint_shots(self._h, len(a), ptr, _fptr(np.ascontiguousarray(r, np.float32)), _fptr(g)))
+        return g
 
     def angle_gathers(self, nang=30, dang=3.0):
         """Angle-domain gathers of the last rtm() (erm2d_angle_gathers):

@@ -35,7 +35,14 @@ struct G2 {
     const float* pcx;
     const float* pcz;
     long ncell;
+    // Batched shots (LSRTM operators): shot blockIdx.z of a launch works on
+    // its own copy of every wavefield array at + z*sst floats, of the E/F
+    // history at + z*hst and of the gradient at + z*gst, with its source cell
+    // srcs[z]; the model is shared.  Zero strides / null srcs: one shot.
+    long sst, hst, gst;
+    const long* srcs;
 };
+
 
 // e += (psi = b psi + a e) for memory variable q at cell n, axis coefficients
 // pc of length nn at index idx; where a = 0 psi stays 0 and e passes.
@@ -52,6 +59,16 @@ struct F2 {  // one wavefield state (forward, Born, or adjoint)
     float *vx, *vz, *xx, *zz, *xz, *tp, *vxp, *vzp;
     float* r[3][4];  // memory variables per mechanism: p, xx, zz, xz
 };
+
+// the state arrays of shot z of a batched launch
+__device__ __forceinline__ F2 shot_state(F2 f, long off) {
+    if (off == 0) return f;
+    f.vx += off; f.vz += off; f.xx += off; f.zz += off; f.xz += off; f.tp += off; f.vxp += off; f.vzp += off;
+    for (int l = 0; l < 3; ++l)
+        for (int q = 0; q < 4; ++q)
+            if (f.r[l][q]) f.r[l][q] += off;
+    return f;
+}
 
 struct M2 {  // model: unrelaxed moduli, buoyancy, SLS coefficients, loaded vp / vs
     const float *M, *U, *B;
@@ -136,6 +153,11 @@ __global__ __launch_bounds__(TZ* TXR) void k2_stress(F2 f, M2 m, G2 g, Coef2 cf,
                                                      float* __restrict__ Eh) {
     __shared__ float tvx[TXR + 2 * R][TZ + 2 * R];
     __shared__ float tvz[TXR + 2 * R][TZ + 2 * R];
+    if (const int z = blockIdx.z) {
+        f = shot_state(f, z * g.sst);
+        if constexpr (SAVE) Eh += z * g.hst;
+    }
+    if (src >= 0 && g.srcs) src = g.srcs[blockIdx.z];
     const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
     load_tile<R, TXR>(tvx, f.vx, g, i0, k0);
     load_tile<R, TXR>(tvz, f.vz, g, i0, k0);
@@ -204,6 +226,10 @@ __global__ __launch_bounds__(TZ* TXR) void k2_velocity(F2 f, M2 m, G2 g, Coef2 c
     __shared__ float tzz[TXR + 2 * R][TZ + 2 * R];
     __shared__ float txz[TXR + 2 * R][TZ + 2 * R];
     __shared__ float ttp[TXR + 2 * R][TZ + 2 * R];
+    if (const int z = blockIdx.z) {  // batched shots: MODE 0 only
+        f = shot_state(f, z * g.sst);
+        if constexpr (SAVE) Fh += z * g.hst;
+    }
     const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
     load_tile<R, TXR>(txx, f.xx, g, i0, k0);
     load_tile<R, TXR>(tzz, f.zz, g, i0, k0);
@@ -323,6 +349,7 @@ __device__ __forceinline__ void rel_moduli(int para, float vp, float vs, float m
 template <int L>
 __global__ void k2_born_stress_src(F2 d, M2 m, G2 g, const float* __restrict__ dm, const float* __restrict__ Eh,
                                    int para) {
+    if (const int z = blockIdx.z) { d = shot_state(d, z * g.sst); Eh += z * g.hst; }
     const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if (k >= g.nnz) return;
     long q;
@@ -348,6 +375,7 @@ __global__ void k2_born_stress_src(F2 d, M2 m, G2 g, const float* __restrict__ d
 }
 
 __global__ void k2_born_velocity_src(F2 d, M2 m, G2 g, const float* __restrict__ dm, const float* __restrict__ Fh) {
+    if (const int z = blockIdx.z) { d = shot_state(d, z * g.sst); Fh += z * g.hst; }
     const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if (k >= g.nnz) return;
     long q;
@@ -371,6 +399,7 @@ __global__ __launch_bounds__(TZ* TXR) void k2a_v(F2 l, M2 m, G2 g, Coef2 cf, con
                                                  float* __restrict__ grad) {
     __shared__ float zx[TXR + 2 * R][TZ + 2 * R];
     __shared__ float zz[TXR + 2 * R][TZ + 2 * R];
+    if (const int z = blockIdx.z) { l = shot_state(l, z * g.sst); Fh += z * g.hst; grad += z * g.gst; }
     const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
     const int tz = threadIdx.x, tx = threadIdx.y;
     auto Zat = [&](int ii, int kk, float* ox, float* oz) {
@@ -417,6 +446,11 @@ __global__ __launch_bounds__(TZ* TXR) void k2a_v(F2 l, M2 m, G2 g, Coef2 cf, con
 template <int L>
 __global__ void k2a_s1(F2 l, M2 m, G2 g, const float* __restrict__ Eh, float* __restrict__ grad,
                        float* __restrict__ W0, float* __restrict__ W1, float* __restrict__ W2, int R) {
+    if (const int z = blockIdx.z) {
+        l = shot_state(l, z * g.sst);
+        Eh += z * g.hst; grad += z * g.gst;
+        W0 += z * g.sst; W1 += z * g.sst; W2 += z * g.sst;
+    }
     const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
     if (k < R || k >= g.nnz - R || i < R || i >= g.nnx - R) return;
     const long n = (long)i * g.pz + k;
@@ -462,6 +496,7 @@ __global__ __launch_bounds__(TZ* TXR) void k2a_s2(F2 l, G2 g, Coef2 cf, const fl
     __shared__ float w0[TXR + 2 * R][TZ + 2 * R];
     __shared__ float w1[TXR + 2 * R][TZ + 2 * R];
     __shared__ float w2[TXR + 2 * R][TZ + 2 * R];
+    if (const int z = blockIdx.z) { l = shot_state(l, z * g.sst); W0 += z * g.sst; W1 += z * g.sst; W2 += z * g.sst; }
     const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
     load_tile<R, TXR>(w0, W0, g, i0, k0);
     load_tile<R, TXR>(w1, W1, g, i0, k0);
@@ -499,6 +534,15 @@ __global__ void k2_grad_para(float* __restrict__ grad, M2 m, G2 g, int para) {
     }
 }
 
+// out = sum over the ns shots of a batched launch of their gradients (stride n)
+__global__ void k2_sum_shots(const float* __restrict__ g, int ns, long n, float* __restrict__ out) {
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        float a = 0.f;
+        for (int z = 0; z < ns; ++z) a += g[z * n + q];
+        out[q] = a;
+    }
+}
+
 // div / curl decomposition (decom, elastic_2D_kernel_2.cu:621-666); with
 // svp / svs the decom_new scaling (:671-791): div times vp at the stress node,
 // curl times the mean of vs over (i,k), (i,k+1), (i+1,k), (i+1,k+1).
@@ -529,18 +573,21 @@ __global__ __launch_bounds__(TZ* TXR) void k2_decom(const float* __restrict__ vx
     cu[n] = r;
 }
 
-// receivers: rec [2][lt][nrec]
+// receivers: rec [2][lt][nrec]; batched shots along blockIdx.y (state
+// stride sst, record stride rst)
 __global__ void k2_record(const float* __restrict__ vx, const float* __restrict__ vz, const long* __restrict__ idx,
-                          int nr, float* __restrict__ rec, long comp, long off) {
+                          int nr, float* __restrict__ rec, long comp, long off, long sst, long rst) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
+    vx += blockIdx.y * sst; vz += blockIdx.y * sst; rec += blockIdx.y * rst;
     rec[off + r] = vx[idx[r]];
     rec[comp + off + r] = vz[idx[r]];
 }
 __global__ void k2_inject(float* __restrict__ vx, float* __restrict__ vz, const long* __restrict__ idx, int nr,
-                          const float* __restrict__ rec, long comp, long off) {
+                          const float* __restrict__ rec, long comp, long off, long sst, long rst) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nr) return;
+    vx += blockIdx.y * sst; vz += blockIdx.y * sst; rec += blockIdx.y * rst;
     atomicAdd(&vx[idx[r]], rec[off + r]);
     atomicAdd(&vz[idx[r]], rec[comp + off + r]);
 }

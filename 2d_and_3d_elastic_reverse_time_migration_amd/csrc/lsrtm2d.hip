@@ -132,6 +132,17 @@ struct erm2d_ctx {
     float qsign = 1.f;                      // -1 during a Q-compensated receiver pass
     std::vector<float> bg_host[5];          // background model as loaded (vp vs rho qp qs), for workers
     std::vector<erm2d_ctx*> workers;        // extra contexts (own streams) for concurrent shots
+    // Batched LSRTM operators (erm2d_set_batch): per-shot copies of the two
+    // states, the adjoint scratch, the E/F history, the gradients and the
+    // records for up to cap shots, each array [cap][...]; nb > 0 while a
+    // batched operator runs (the launches then cover nb shots along z).
+    struct Batch {
+        int cap = 0;
+        float *A[8] = {nullptr}, *B[8] = {nullptr}, *rA[3][4] = {{nullptr}}, *rB[3][4] = {{nullptr}}, *W[3] = {nullptr};
+        float *hist = nullptr, *grad = nullptr, *rec = nullptr;
+        long* srcs = nullptr;
+    } bt;
+    int nb = 0;
     erm2d_cg_monitor_fn monitor = nullptr;
     void* monitor_user = nullptr;
     std::string err;
@@ -176,15 +187,28 @@ G2 geo(const erm2d_ctx* c) {
     g.pcx = c->pcx;
     g.pcz = c->pcz;
     g.ncell = (long)c->ncell;
+    g.sst = c->nb ? (long)c->ncell : 0;
+    g.hst = c->nb ? (long)c->p.lt * 5 * (long)c->mn : 0;
+    g.gst = c->nb ? 3 * (long)c->mn : 0;
+    g.srcs = c->nb ? c->bt.srcs : nullptr;
     return g;
 }
 F2 state(erm2d_ctx* c, bool B) {
     F2 f;
-    float** a = B ? c->fB : c->fA;
+    float** a = c->nb ? (B ? c->bt.B : c->bt.A) : (B ? c->fB : c->fA);
     f.vx = a[0]; f.vz = a[1]; f.xx = a[2]; f.zz = a[3]; f.xz = a[4]; f.tp = a[5]; f.vxp = a[6]; f.vzp = a[7];
-    for (int l = 0; l < 3; ++l) for (int q = 0; q < 4; ++q) f.r[l][q] = B ? c->rB[l][q] : c->rA[l][q];
+    for (int l = 0; l < 3; ++l)
+        for (int q = 0; q < 4; ++q)
+            f.r[l][q] = c->nb ? (B ? c->bt.rB[l][q] : c->bt.rA[l][q]) : (B ? c->rB[l][q] : c->rA[l][q]);
     return f;
 }
+// the history / scratch, gradient, record and adjoint-scratch buffers of the
+// running operator (batched: shot 0's; the kernels offset the others)
+float* hist_of(erm2d_ctx* c) { return c->nb ? c->bt.hist : c->hist; }
+float* grad_of(erm2d_ctx* c) { return c->nb ? c->bt.grad : c->grad; }
+float* rec_of(erm2d_ctx* c) { return c->nb ? c->bt.rec : c->rec2; }
+float* const* w_of(erm2d_ctx* c) { return c->nb ? c->bt.W : c->W; }
+int nshot_of(const erm2d_ctx* c) { return c->nb ? c->nb : 1; }
 M2 model(const erm2d_ctx* c, int which) {
     M2 m;
     const auto& d = c->mod[which];
@@ -201,8 +225,8 @@ Ex2 excite(const erm2d_ctx* c) {
     e.tp_time = c->ex_tp_time; e.tp_amp = c->ex_tp_amp; e.tp_shift = c->p.ex_tp_shift;
     return e;
 }
-dim3 grid2(const erm2d_ctx* c) { return dim3((c->nnz + TZ - 1) / TZ, (c->nnx + TXR - 1) / TXR); }
-dim3 grid_pt(const erm2d_ctx* c) { return dim3((c->nnz + 255) / 256, c->nnx); }
+dim3 grid2(const erm2d_ctx* c) { return dim3((c->nnz + TZ - 1) / TZ, (c->nnx + TXR - 1) / TXR, nshot_of(c)); }
+dim3 grid_pt(const erm2d_ctx* c) { return dim3((c->nnz + 255) / 256, c->nnx, nshot_of(c)); }
 
 // geometry of a launch on state A (the C-PML memory variables belong to it) or B
 G2 geo_of(const erm2d_ctx* c, bool B) {
@@ -253,26 +277,31 @@ void born_stress_src(erm2d_ctx* c, const float* Eh) {
     else hipLaunchKernelGGL((k2_born_stress_src<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh, para(c));
 }
 void adj_v(erm2d_ctx* c, const float* Fh) {
-    if (c->R == 4) hipLaunchKernelGGL((k2a_v<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, c->grad);
-    else hipLaunchKernelGGL((k2a_v<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, c->grad);
+    float* gr = grad_of(c);
+    if (c->R == 4) hipLaunchKernelGGL((k2a_v<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, gr);
+    else hipLaunchKernelGGL((k2a_v<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, gr);
 }
 void adj_s1(erm2d_ctx* c, const float* Eh) {
     dim3 g = grid_pt(c);
-    if (c->L == 0) hipLaunchKernelGGL((k2a_s1<0>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, c->grad, c->W[0], c->W[1], c->W[2], c->R);
-    else if (c->L == 1) hipLaunchKernelGGL((k2a_s1<1>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, c->grad, c->W[0], c->W[1], c->W[2], c->R);
-    else hipLaunchKernelGGL((k2a_s1<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, c->grad, c->W[0], c->W[1], c->W[2], c->R);
+    float* gr = grad_of(c);
+    float* const* W = w_of(c);
+    if (c->L == 0) hipLaunchKernelGGL((k2a_s1<0>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, gr, W[0], W[1], W[2], c->R);
+    else if (c->L == 1) hipLaunchKernelGGL((k2a_s1<1>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, gr, W[0], W[1], W[2], c->R);
+    else hipLaunchKernelGGL((k2a_s1<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, gr, W[0], W[1], W[2], c->R);
 }
 void adj_s2(erm2d_ctx* c) {
-    if (c->R == 4) hipLaunchKernelGGL((k2a_s2<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, c->W[0], c->W[1], c->W[2]);
-    else hipLaunchKernelGGL((k2a_s2<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, c->W[0], c->W[1], c->W[2]);
+    float* const* W = w_of(c);
+    if (c->R == 4) hipLaunchKernelGGL((k2a_s2<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, W[0], W[1], W[2]);
+    else hipLaunchKernelGGL((k2a_s2<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, W[0], W[1], W[2]);
 }
 
 int zero_state(erm2d_ctx* c, bool B) {
-    float** f = B ? c->fB : c->fA;
-    for (int q = 0; q < 8; ++q) if (fillf(c, f[q], c->ncell, 0.f)) return ERM_E_DEVICE;
-    if (!B && c->psi && fillf(c, c->psi, 10 * c->ncell, 0.f)) return ERM_E_DEVICE;
+    const F2 f = state(c, B);
+    const size_t n = c->ncell * nshot_of(c);
+    for (float* a : {f.vx, f.vz, f.xx, f.zz, f.xz, f.tp, f.vxp, f.vzp}) if (fillf(c, a, n, 0.f)) return ERM_E_DEVICE;
+    if (!B && !c->nb && c->psi && fillf(c, c->psi, 10 * c->ncell, 0.f)) return ERM_E_DEVICE;
     for (int l = 0; l < c->L; ++l)
-        for (int q = 0; q < 4; ++q) if (fillf(c, B ? c->rB[l][q] : c->rA[l][q], c->ncell, 0.f)) return ERM_E_DEVICE;
+        for (int q = 0; q < 4; ++q) if (fillf(c, f.r[l][q], n, 0.f)) return ERM_E_DEVICE;
     return 0;
 }
 int zero_excitation(erm2d_ctx* c) {
@@ -298,7 +327,7 @@ int run(erm2d_ctx* c, int pass, int it_begin, int nsteps, float* recbuf) {
             if (it < 0 || it >= c->p.lt) return c->fail(ERM_E_PARAM, "time index out of range");
             if (c->p.nrec > 0)
                 hipLaunchKernelGGL(k2_inject, dim3(rg), dim3(256), 0, c->stream, c->fA[0], c->fA[1], c->rec_idx,
-                                   c->p.nrec, recbuf, comp, (long)it * c->p.nrec);
+                                   c->p.nrec, recbuf, comp, (long)it * c->p.nrec, 0L, 0L);
             do_stress(c, false, which, -1, 0.f, nullptr);
             do_velocity(c, false, which, 2, it, nullptr);
         } else {
@@ -307,7 +336,7 @@ int run(erm2d_ctx* c, int pass, int it_begin, int nsteps, float* recbuf) {
             do_velocity(c, false, which, pass, it, nullptr);
             if (pass == ERM2D_PASS_MODEL && it < c->p.lt && c->p.nrec > 0)
                 hipLaunchKernelGGL(k2_record, dim3(rg), dim3(256), 0, c->stream, c->fA[0], c->fA[1], c->rec_idx,
-                                   c->p.nrec, recbuf, comp, (long)it * c->p.nrec);
+                                   c->p.nrec, recbuf, comp, (long)it * c->p.nrec, 0L, 0L);
         }
     }
     c->qsign = 1.f;
@@ -336,6 +365,10 @@ void free2(erm2d_ctx* c) {
     for (auto& e : c->ex) F(e);
     for (auto& e : c->img) F(e);
     F(c->hist); F(c->dm); F(c->grad); F(c->illum);
+    for (int q = 0; q < 8; ++q) { F(c->bt.A[q]); F(c->bt.B[q]); }
+    for (int l = 0; l < 3; ++l) for (int q = 0; q < 4; ++q) { F(c->bt.rA[l][q]); F(c->bt.rB[l][q]); }
+    for (auto& w : c->bt.W) F(w);
+    F(c->bt.hist); F(c->bt.grad); F(c->bt.rec); F(c->bt.srcs);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -731,38 +764,128 @@ int erm2d_rtm(erm2d_ctx* c, const float* rec, float* tp, float* pp, float* ps) {
     return 0;
 }
 
-int erm2d_born(erm2d_ctx* c, const float* dm, float* d) {
-    if (!c || !dm) return ERM_E_PARAM;
-    if (c->cpml) return c->fail(ERM_E_PARAM, "the LSRTM operators need the damping boundary (boundary = 0)");
-    HC(c, hipSetDevice(c->device));
-    if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
+}  // extern "C"
+
+namespace {
+// One Born application over the shots of the running operator (one, or nb
+// batched along z): background and Born states stepped together, the Born
+// sources from the background's strain increments and forces of the step
+// (one step of E/F per shot in the history buffer); records to rec_of(c).
+// c->dm holds the perturbation.
+int born_core(erm2d_ctx* c) {
     int rc;
-    if ((rc = ensure_lsrtm(c))) return rc;
-    HC(c, hipMemcpyAsync(c->dm, dm, 3 * c->mn * 4, hipMemcpyHostToDevice, c->stream));
-    HC(c, hipEventRecord(c->ev[0], c->stream));
     if ((rc = zero_state(c, false)) || (rc = zero_state(c, true))) return rc;
     const long comp = (long)c->p.lt * c->p.nrec;
     const int rg = (c->p.nrec + 255) / 256;
-    float* Eh = c->hist;          // one step of E and F suffices here
-    float* Fh = c->hist + 3 * c->mn;
+    float* Eh = hist_of(c);
+    float* Fh = Eh + 3 * c->mn;
+    float* rec = rec_of(c);
+    const long sst = c->nb ? (long)c->ncell : 0, rst = c->nb ? 2 * comp : 0;
+    const F2 fb = state(c, true);
     for (int it = 0; it < c->p.lt; ++it) {
         do_stress(c, false, 1, c->src, wavelet_at(c, it), Eh);
         do_stress(c, true, 1, -1, 0.f, nullptr);
         born_stress_src(c, Eh);
         do_velocity(c, false, 1, 0, it, Fh);
         do_velocity(c, true, 1, 0, it, nullptr);
-        hipLaunchKernelGGL(k2_born_velocity_src, grid_pt(c), dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c),
-                           c->dm, Fh);
+        hipLaunchKernelGGL(k2_born_velocity_src, grid_pt(c), dim3(256), 0, c->stream, fb, model(c, 1), geo(c), c->dm, Fh);
         if (c->p.nrec > 0)
-            hipLaunchKernelGGL(k2_record, dim3(rg), dim3(256), 0, c->stream, c->fB[0], c->fB[1], c->rec_idx, c->p.nrec,
-                               c->rec2, comp, (long)it * c->p.nrec);
+            hipLaunchKernelGGL(k2_record, dim3(rg, nshot_of(c)), dim3(256), 0, c->stream, fb.vx, fb.vz, c->rec_idx,
+                               c->p.nrec, rec, comp, (long)it * c->p.nrec, sst, rst);
     }
     HC(c, hipGetLastError());
+    return 0;
+}
+
+// One adjoint application over the shots of the running operator: the
+// background pass stores E and F of every step, then the transposed Born
+// step runs backward from the residual records in rec_of(c); per-shot
+// gradients in grad_of(c), summed into c->grad and mapped to the inverted
+// parameters.
+int adjoint_core(erm2d_ctx* c) {
+    int rc;
+    const long comp = (long)c->p.lt * c->p.nrec;
+    const int rg = (c->p.nrec + 255) / 256;
+    const int ns = nshot_of(c);
+    if ((rc = zero_state(c, false)) || (rc = zero_state(c, true))) return rc;
+    if ((rc = fillf(c, grad_of(c), 3 * c->mn * ns, 0.f))) return rc;
+    for (int q = 0; q < 3; ++q) if ((rc = fillf(c, w_of(c)[q], c->ncell * ns, 0.f))) return rc;
+    float* hist = hist_of(c);
+    float* rec = rec_of(c);
+    const long sst = c->nb ? (long)c->ncell : 0, rst = c->nb ? 2 * comp : 0;
+    const F2 fb = state(c, true);
+    const size_t step = 5 * c->mn;
+    for (int it = 0; it < c->p.lt; ++it) {  // background, storing E and F per step
+        float* h = hist + (size_t)it * step;
+        do_stress(c, false, 1, c->src, wavelet_at(c, it), h);
+        do_velocity(c, false, 1, 0, it, h + 3 * c->mn);
+    }
+    for (int it = c->p.lt - 1; it >= 0; --it) {
+        const float* h = hist + (size_t)it * step;
+        if (c->p.nrec > 0)
+            hipLaunchKernelGGL(k2_inject, dim3(rg, ns), dim3(256), 0, c->stream, fb.vx, fb.vz, c->rec_idx, c->p.nrec,
+                               rec, comp, (long)it * c->p.nrec, sst, rst);
+        adj_v(c, h + 3 * c->mn);
+        adj_s1(c, h);
+        adj_s2(c);
+    }
+    const unsigned gb = (unsigned)std::min<size_t>((c->mn + 255) / 256, 2048);
+    if (c->nb)
+        hipLaunchKernelGGL(k2_sum_shots, dim3((unsigned)std::min<size_t>((3 * c->mn + 255) / 256, 4096)), dim3(256), 0,
+                           c->stream, c->bt.grad, ns, (long)(3 * c->mn), c->grad);
+    if (para(c) != 2)
+        hipLaunchKernelGGL(k2_grad_para, dim3(gb), dim3(256), 0, c->stream, c->grad, model(c, 1), geo(c), para(c));
+    HC(c, hipGetLastError());
+    return 0;
+}
+
+int lsrtm_ready(erm2d_ctx* c) {
+    if (c->cpml) return c->fail(ERM_E_PARAM, "the LSRTM operators need the damping boundary (boundary = 0)");
+    HC(c, hipSetDevice(c->device));
+    if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
+    return ensure_lsrtm(c);
+}
+int timed_ms(erm2d_ctx* c) {
     HC(c, hipEventRecord(c->ev[1], c->stream));
     HC(c, hipEventSynchronize(c->ev[1]));
     float ms = 0;
     HC(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     c->last_ms = ms;
+    return 0;
+}
+// source cells of shots [k0, k0 + ns) into the batch table (validated)
+int batch_sources(erm2d_ctx* c, const int* shots, int ns) {
+    std::vector<long> src(ns);
+    for (int k = 0; k < ns; ++k) {
+        const int i = c->p.bl + shots[2 * k], kk = c->p.bu + shots[2 * k + 1];
+        if (i < c->R || i >= c->nnx - c->R || kk < c->R || kk >= c->nnz - c->R)
+            return c->fail(ERM_E_PARAM, "source (%d,%d) is not an updated cell", i, kk);
+        src[k] = (long)i * c->pz + kk;
+    }
+    HC(c, hipMemcpyAsync(c->bt.srcs, src.data(), ns * sizeof(long), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+void free_batch(erm2d_ctx* c) {
+    auto F = [](void* p) { if (p) (void)hipFree(p); };
+    auto& b = c->bt;
+    for (int q = 0; q < 8; ++q) { F(b.A[q]); F(b.B[q]); }
+    for (int l = 0; l < 3; ++l) for (int q = 0; q < 4; ++q) { F(b.rA[l][q]); F(b.rB[l][q]); }
+    for (auto& w : b.W) F(w);
+    F(b.hist); F(b.grad); F(b.rec); F(b.srcs);
+    b = erm2d_ctx::Batch{};
+}
+}  // namespace
+
+extern "C" {
+
+int erm2d_born(erm2d_ctx* c, const float* dm, float* d) {
+    if (!c || !dm) return ERM_E_PARAM;
+    int rc;
+    if ((rc = lsrtm_ready(c))) return rc;
+    HC(c, hipMemcpyAsync(c->dm, dm, 3 * c->mn * 4, hipMemcpyHostToDevice, c->stream));
+    HC(c, hipEventRecord(c->ev[0], c->stream));
+    c->nb = 0;
+    if ((rc = born_core(c)) || (rc = timed_ms(c))) return rc;
     if (d && c->p.nrec > 0) {
         HC(c, hipMemcpyAsync(d, c->rec2, (size_t)2 * c->p.lt * c->p.nrec * 4, hipMemcpyDeviceToHost, c->stream));
         HC(c, hipStreamSynchronize(c->stream));
@@ -772,48 +895,106 @@ int erm2d_born(erm2d_ctx* c, const float* dm, float* d) {
 
 int erm2d_adjoint(erm2d_ctx* c, const float* r, float* gout) {
     if (!c || !r) return ERM_E_PARAM;
-    if (c->cpml) return c->fail(ERM_E_PARAM, "the LSRTM operators need the damping boundary (boundary = 0)");
-    HC(c, hipSetDevice(c->device));
-    if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
     int rc;
-    if ((rc = ensure_lsrtm(c))) return rc;
-    const long comp = (long)c->p.lt * c->p.nrec;
-    const int rg = (c->p.nrec + 255) / 256;
+    if ((rc = lsrtm_ready(c))) return rc;
     if (c->p.nrec > 0)
         HC(c, hipMemcpyAsync(c->rec2, r, (size_t)2 * c->p.lt * c->p.nrec * 4, hipMemcpyHostToDevice, c->stream));
     HC(c, hipEventRecord(c->ev[0], c->stream));
-    if ((rc = zero_state(c, false)) || (rc = zero_state(c, true))) return rc;
-    if ((rc = fillf(c, c->grad, 3 * c->mn, 0.f))) return rc;
-    for (auto& w : c->W) if ((rc = fillf(c, w, c->ncell, 0.f))) return rc;
-    const size_t step = 5 * c->mn;
-    for (int it = 0; it < c->p.lt; ++it) {  // background, storing E and F per step
-        float* h = c->hist + (size_t)it * step;
-        do_stress(c, false, 1, c->src, wavelet_at(c, it), h);
-        do_velocity(c, false, 1, 0, it, h + 3 * c->mn);
-    }
-    for (int it = c->p.lt - 1; it >= 0; --it) {
-        const float* h = c->hist + (size_t)it * step;
-        if (c->p.nrec > 0)
-            hipLaunchKernelGGL(k2_inject, dim3(rg), dim3(256), 0, c->stream, c->fB[0], c->fB[1], c->rec_idx, c->p.nrec,
-                               c->rec2, comp, (long)it * c->p.nrec);
-        adj_v(c, h + 3 * c->mn);
-        adj_s1(c, h);
-        adj_s2(c);
-    }
-    if (para(c) != 2)
-        hipLaunchKernelGGL(k2_grad_para, dim3((unsigned)std::min<size_t>((c->mn + 255) / 256, 2048)), dim3(256), 0,
-                           c->stream, c->grad, model(c, 1), geo(c), para(c));
-    HC(c, hipGetLastError());
-    HC(c, hipEventRecord(c->ev[1], c->stream));
-    HC(c, hipEventSynchronize(c->ev[1]));
-    float ms = 0;
-    HC(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    c->last_ms = ms;
+    c->nb = 0;
+    if ((rc = adjoint_core(c)) || (rc = timed_ms(c))) return rc;
     if (gout) {
         HC(c, hipMemcpyAsync(gout, c->grad, 3 * c->mn * 4, hipMemcpyDeviceToHost, c->stream));
         HC(c, hipStreamSynchronize(c->stream));
     }
     return 0;
+}
+
+int erm2d_set_batch(erm2d_ctx* c, int nb) {
+    if (!c) return ERM_E_PARAM;
+    if (nb < 1 || nb > 1024) return c->fail(ERM_E_PARAM, "batch must be 1..1024 shots");
+    HC(c, hipSetDevice(c->device));
+    HC(c, hipStreamSynchronize(c->stream));
+    free_batch(c);
+    if (nb == 1) return 0;
+    auto& b = c->bt;
+    const size_t n = c->ncell * nb;
+    int rc = 0;
+    for (int q = 0; q < 8 && !rc; ++q) rc = dmal(c, &b.A[q], n) || dmal(c, &b.B[q], n);
+    for (int l = 0; l < c->L && !rc; ++l)
+        for (int q = 0; q < 4 && !rc; ++q) rc = dmal(c, &b.rA[l][q], n) || dmal(c, &b.rB[l][q], n);
+    for (int q = 0; q < 3 && !rc; ++q) rc = dmal(c, &b.W[q], n);
+    if (!rc) rc = dmal(c, &b.hist, (size_t)c->p.lt * 5 * c->mn * nb);
+    if (!rc) rc = dmal(c, &b.grad, 3 * c->mn * nb);
+    if (!rc) rc = dmal(c, &b.rec, std::max<size_t>((size_t)2 * c->p.lt * c->p.nrec * nb, 1));
+    if (!rc) rc = dmal(c, &b.srcs, nb);
+    if (rc) {
+        free_batch(c);
+        return c->fail(ERM_E_NOMEM, "batch of %d shots does not fit in device memory", nb);
+    }
+    b.cap = nb;
+    return 0;
+}
+
+int erm2d_born_shots(erm2d_ctx* c, int nshots, const int* shots, const float* dm, float* d) {
+    if (!c || nshots < 1 || !shots || !dm) return c ? c->fail(ERM_E_PARAM, "bad arguments") : ERM_E_PARAM;
+    int rc;
+    if ((rc = lsrtm_ready(c))) return rc;
+    const size_t nd = (size_t)2 * c->p.lt * c->p.nrec;
+    if (c->bt.cap < 2) {  // one shot at a time on this context
+        const int sx0 = c->p.shot_x, sz0 = c->p.shot_z;
+        for (int k = 0; k < nshots && !rc; ++k)
+            if (!(rc = erm2d_set_shot(c, shots[2 * k], shots[2 * k + 1]))) rc = erm2d_born(c, dm, d ? d + k * nd : nullptr);
+        erm2d_set_shot(c, sx0, sz0);
+        return rc;
+    }
+    HC(c, hipMemcpyAsync(c->dm, dm, 3 * c->mn * 4, hipMemcpyHostToDevice, c->stream));
+    HC(c, hipEventRecord(c->ev[0], c->stream));
+    for (int k0 = 0; k0 < nshots; k0 += c->bt.cap) {
+        const int ns = std::min(c->bt.cap, nshots - k0);
+        if ((rc = batch_sources(c, shots + 2 * k0, ns))) return rc;
+        c->nb = ns;
+        rc = born_core(c);
+        c->nb = 0;
+        if (rc) return rc;
+        if (d && nd) HC(c, hipMemcpyAsync(d + k0 * nd, c->bt.rec, nd * ns * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    if ((rc = timed_ms(c))) return rc;
+    HC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int erm2d_adjoint_shots(erm2d_ctx* c, int nshots, const int* shots, const float* r, float* gout) {
+    if (!c || nshots < 1 || !shots || !r) return c ? c->fail(ERM_E_PARAM, "bad arguments") : ERM_E_PARAM;
+    int rc;
+    if ((rc = lsrtm_ready(c))) return rc;
+    const size_t nd = (size_t)2 * c->p.lt * c->p.nrec, nm = 3 * c->mn;
+    std::vector<double> acc(nm, 0.0);
+    std::vector<float> part(nm);
+    auto add = [&]() { for (size_t i = 0; i < nm; ++i) acc[i] += part[i]; };
+    if (c->bt.cap < 2) {
+        const int sx0 = c->p.shot_x, sz0 = c->p.shot_z;
+        for (int k = 0; k < nshots && !rc; ++k)
+            if (!(rc = erm2d_set_shot(c, shots[2 * k], shots[2 * k + 1])) && !(rc = erm2d_adjoint(c, r + k * nd, part.data())))
+                add();
+        erm2d_set_shot(c, sx0, sz0);
+    } else {
+        HC(c, hipEventRecord(c->ev[0], c->stream));
+        for (int k0 = 0; k0 < nshots && !rc; k0 += c->bt.cap) {
+            const int ns = std::min(c->bt.cap, nshots - k0);
+            if ((rc = batch_sources(c, shots + 2 * k0, ns))) return rc;
+            if (nd) HC(c, hipMemcpyAsync(c->bt.rec, r + k0 * nd, nd * ns * 4, hipMemcpyHostToDevice, c->stream));
+            c->nb = ns;
+            rc = adjoint_core(c);
+            c->nb = 0;
+            if (rc) return rc;
+            HC(c, hipMemcpyAsync(part.data(), c->grad, nm * 4, hipMemcpyDeviceToHost, c->stream));
+            HC(c, hipStreamSynchronize(c->stream));
+            add();
+        }
+        if (!rc) rc = timed_ms(c);
+    }
+    if (!rc && gout) for (size_t i = 0; i < nm; ++i) gout[i] = (float)acc[i];
+    return rc;
 }
 
 }  // extern "C"
@@ -1042,8 +1223,11 @@ extern "C" int erm2d_lsrtm_cg(erm2d_ctx* c, int nshots, const int* shots, const 
             if (rcs[w]) return w ? c->fail(rcs[w], "worker %d: %s", w, pool[w]->err.c_str()) : rcs[w];
         return 0;
     };
-    // L over all shots
+    // L over all shots: batched launches on this context (erm2d_set_batch),
+    // or one shot per launch spread over the workers
+    const bool batched = c->bt.cap > 1;
     auto applyL = [&](const std::vector<float>& x, std::vector<float>& out) -> int {
+        if (batched) return erm2d_born_shots(c, nshots, shots, x.data(), out.data());
         return parallel([&](erm2d_ctx* cw, int, int k) {
             int rc = erm2d_set_shot(cw, shots[2 * k], shots[2 * k + 1]);
             if (!rc) rc = erm2d_born(cw, x.data(), out.data() + (size_t)k * nd);
@@ -1051,6 +1235,11 @@ extern "C" int erm2d_lsrtm_cg(erm2d_ctx* c, int nshots, const int* shots, const 
         });
     };
     auto applyLT = [&](const std::vector<float>& y, std::vector<float>& out) -> int {
+        if (batched) {
+            int rc2 = erm2d_adjoint_shots(c, nshots, shots, y.data(), out.data());
+            if (!rc2) mask(out);
+            return rc2;
+        }
         for (auto& pp : part) std::fill(pp.begin(), pp.end(), 0.f);
         int rc = parallel([&](erm2d_ctx* cw, int w, int k) {
             int rc2 = erm2d_set_shot(cw, shots[2 * k], shots[2 * k + 1]);

@@ -116,6 +116,22 @@ int   erm2d_rtm(erm2d_ctx* ctx, const float* rec, float* img_tp, float* img_pp, 
  * where L^T is the exact transpose of the discrete L (dot-product exact). */
 int   erm2d_born(erm2d_ctx* ctx, const float* dm, float* d);
 int   erm2d_adjoint(erm2d_ctx* ctx, const float* r, float* g);
+
+/* Batched shots (no reference counterpart; the reference runs one shot per
+ * launch): erm2d_set_batch(ctx, nb) gives the context per-shot states and
+ * E/F histories for nb shots (nb * (lt*5*nx*nz + ~30*padded cells) floats of
+ * device memory; 1 frees them), after which erm2d_born_shots /
+ * erm2d_adjoint_shots run the shots in groups of nb, each time step of a
+ * group one launch per kernel with the shots along the grid's z dimension,
+ * and erm2d_lsrtm_cg uses them in place of the workers.  Results equal
+ * erm2d_born / erm2d_adjoint shot by shot (records bit-identical; the
+ * gradient is the sum over shots, in double on the host across groups).
+ *   shots  [nshots][2] source (x, z) cells of the model region
+ *   d      [nshots][2][lt][nrec] Born records (output)
+ *   r      [nshots][2][lt][nrec] residual records; g [3][nx][nz] output */
+int   erm2d_set_batch(erm2d_ctx* ctx, int nb);
+int   erm2d_born_shots(erm2d_ctx* ctx, int nshots, const int* shots, const float* dm, float* d);
+int   erm2d_adjoint_shots(erm2d_ctx* ctx, int nshots, const int* shots, const float* r, float* g);
 /* Least-squares RTM by CGLS (conjugate gradients on the normal equations,
  * Paige & Saunders 1982) over nshots shots: minimise sum_s |L_s m - d_s|^2.
  * With a preconditioner the flexible form is used: each preconditioned
