@@ -1,18 +1,18 @@
 // unet.hip - U-Net image-domain preconditioner for LSRTM (include/erm_unet.h).
 //
-// Convolutions are implicit GEMMs on BF16 MFMA with FP32 accumulation
-// (v_mfma_f32_16x16x16_bf16).  Operand layout of the 16x16x16 instruction for
-// a 64-lane wavefront (lane l, element j = 0..3 of the 4-wide operands):
-//   A[m][k]: m = l % 16, k = 4*(l/16) + j
-//   B[k][n]: n = l % 16, k = 4*(l/16) + j
-//   D[m][n]: n = l % 16, m = 4*(l/16) + j
+// Convolutions are implicit GEMMs on gfx950's BF16 MFMA with FP32
+// accumulation (v_mfma_f32_16x16x32_bf16).  For a 64-lane wavefront, lane l
+// holds row m = l % 16 of A and column n = l % 16 of B, each as 8 k-values of
+// lane group g = l / 16; the instruction sums A and B elements over the same
+// (g, j), so the kernels are free to choose which logical k each (g, j)
+// carries, as long as A and B agree.  D[m][n]: n = l % 16, m = 4*(l/16) + j.
 //   forward        Y[co][p]  = sum_{ci,t} W[co][ci][t] X[ci][p+o(t)]   (M=Cout, N=pixels, K=Cin*9)
 //   backward data  dX[ci][p] = sum_{co,t} W[co][ci][8-t] dY[co][p+o(t)] (M=Cin,  N=pixels, K=Cout*9)
 //   backward wts   dW[co][ci][t] = sum_p dY[co][p] X[ci][p+o(t)]        (M=Cout, N=Cin*9,  K=pixels, split-K)
 // with o(t) = (t/3-1, t%3-1) and zero padding; dY is masked by the ReLU of
 // the layer output where the layer has one.  Operands are staged in LDS as
-// bf16 (round to nearest even) so each lane reads its 4 k-values as one
-// 8-byte word.
+// bf16 (round to nearest even) so each lane reads its 8 k-values as one
+// 16-byte word.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,8 +29,15 @@ namespace {
 
 thread_local std::string g_uerr;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// one 16x16x32 MFMA step from two 16-byte LDS operands
+__device__ __forceinline__ f32x4 mfma32(s16x8 a, s16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c, 0,
+                                                   0, 0);
+}
 
 // round to nearest even
 __device__ __forceinline__ short to_bf16(float x) {
@@ -56,11 +63,13 @@ struct ConvArgs {
 // LDS-tiled implicit GEMM for the forward and backward-data convolutions.
 // A workgroup owns an 8x8 pixel tile of one image and up to 64 output
 // channels; the K loop runs over 16-channel chunks, and inside a chunk over
-// the 9 taps, so one MFMA k-step is (tap t, 16 channels).  Per chunk the
-// input patch (10x10 pixels with the 1-pixel halo, channel fastest) and the
-// weights ([64 rows][9 taps][16 channels]) are converted to bf16 into LDS
-// once; every lane then reads its 4 k-values of A and of B as one 8-byte LDS
-// word.  Waves take the (16-row, 16-pixel) output subtiles round-robin.
+// pairs of taps, so one MFMA k-step is (taps 2p and 2p+1, 16 channels): lane
+// group g takes tap 2p + g/2 and channels 8*(g%2) .. +7 (the tenth tap of the
+// last pair is zero).  Per chunk the input patch (10x10 pixels with the
+// 1-pixel halo, channel fastest) and the weights ([64 rows][9 taps][16
+// channels]) are converted to bf16 into LDS once; every lane then reads its 8
+// k-values of A and of B as one 16-byte LDS word.  Waves take the (16-row,
+// 16-pixel) output subtiles round-robin.
 //   FWD      : in = x (cin channels), A[m][t][c] = W[m][c][t]
 //   BWD_DATA : in = dy masked by y > 0 (cout channels), A[m][t][c] = W[c][m][8-t]
 constexpr int TP = 8;             // pixel tile edge
@@ -82,7 +91,7 @@ __global__ __launch_bounds__(256) void k_conv_tiled(ConvArgs a) {
     const int Mb = min(4, (M - m0 + 15) / 16);       // 16-row blocks in this workgroup
     const long hw = (long)a.h * a.w;
     const float* in = MODE == FWD ? a.x : a.dy;
-    const int li = lane & 15, lk = 4 * (lane >> 4);
+    const int li = lane & 15, lk = 4 * (lane >> 4), lg = lane >> 4, lc = 8 * (lg & 1);
     f32x4 acc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -113,17 +122,20 @@ __global__ __launch_bounds__(256) void k_conv_tiled(ConvArgs a) {
         }
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
+        for (int tp = 0; tp < 5; ++tp) {
+            const int tr = 2 * tp + (lg >> 1), t = min(tr, 8);  // this lane group's tap (9: the zero tap)
             const int dy = t / 3, dx = t % 3;
+            const s16x8 zero = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int st = wave + 4 * q;  // subtile: 16-row block st/4, 16-pixel block st%4
                 if (st >= Mb * 4) break;
                 const int mb = st >> 2, nb = st & 3;
                 const int n = nb * 16 + li, py = n / TP, px = n % TP;
-                const bf16x4 av = *reinterpret_cast<const bf16x4*>(&wl[((mb * 16 + li) * 9 + t) * CK + lk]);
-                const bf16x4 bv = *reinterpret_cast<const bf16x4*>(&patch[((py + dy) * TPH + px + dx) * CK + lk]);
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+                s16x8 av = *reinterpret_cast<const s16x8*>(&wl[((mb * 16 + li) * 9 + t) * CK + lc]);
+                s16x8 bv = *reinterpret_cast<const s16x8*>(&patch[((py + dy) * TPH + px + dx) * CK + lc]);
+                if (tr > 8) { av = zero; bv = zero; }
+                acc[q] = mfma32(av, bv, acc[q]);
             }
         }
     }
@@ -155,13 +167,14 @@ __global__ __launch_bounds__(256) void k_conv_tiled(ConvArgs a) {
 // 8x8 pixel tiles (split-K over gridDim.z), accumulating in registers, with one
 // atomic add per output at the end.  Per pixel tile, dY (ReLU-masked) goes to
 // LDS as [co][64 pixels] and the input patch as three copies shifted by
-// dx = 0, 1, 2 ([c][10 rows][8]), so every lane reads its 4 consecutive
-// pixels of A and of B as one 8-byte LDS word.
+// dx = 0, 1, 2 ([c][10 rows][8]).  One MFMA k-step is 32 pixels (4 rows of
+// the tile): lane group g takes row 4s + g, so every lane reads its 8
+// consecutive pixels of A and of B as one 16-byte LDS word.
 __global__ __launch_bounds__(256) void k_conv_wgrad(ConvArgs a) {
     __shared__ __attribute__((aligned(16))) short dyl[MT * TP * TP];
     __shared__ __attribute__((aligned(16))) short xs[3 * CK * TPH * TP];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int li = lane & 15, lk = 4 * (lane >> 4);
+    const int li = lane & 15, lk = 4 * (lane >> 4), lg = lane >> 4;
     const int c0 = blockIdx.x * CK, m0 = blockIdx.y * MT;
     const int M = a.cout, Cin = a.cin;
     const int Mb = min(4, (M - m0 + 15) / 16);
@@ -198,17 +211,16 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(ConvArgs a) {
         }
         __syncthreads();
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks) {             // 16 pixels per k-step: rows 2ks, 2ks+1
-            const int pk = ks * 16 + lk;             // this lane's first pixel of the k-step
-            const int py = pk / TP, px = pk % TP;    // 4 consecutive pixels in one row
+        for (int ks = 0; ks < 2; ++ks) {             // 32 pixels per k-step: rows 4ks .. 4ks+3
+            const int py = ks * 4 + lg;              // this lane's row: 8 consecutive pixels
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
                 const int st = wave + 4 * q;         // subtile = (16-row block, tap)
                 if (st >= Mb * 9) break;
                 const int mb = st / 9, t = st % 9, dy = t / 3, dx = t % 3;
-                const bf16x4 av = *reinterpret_cast<const bf16x4*>(&dyl[(mb * 16 + li) * TP * TP + pk]);
-                const bf16x4 bv = *reinterpret_cast<const bf16x4*>(&xs[((dx * CK + li) * TPH + py + dy) * TP + px]);
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
+                const s16x8 av = *reinterpret_cast<const s16x8*>(&dyl[(mb * 16 + li) * TP * TP + py * TP]);
+                const s16x8 bv = *reinterpret_cast<const s16x8*>(&xs[((dx * CK + li) * TPH + py + dy) * TP]);
+                acc[q] = mfma32(av, bv, acc[q]);
             }
         }
     }

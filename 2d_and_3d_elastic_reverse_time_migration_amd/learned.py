@@ -9,24 +9,24 @@ w = 1 / (W + eps max W), mean 1) and the U-Net learns the rest.
 1. random layered-plus-anomaly models (`random_model`), a smooth background
    and the relative perturbations (dvp/vp, dvs/vs) the Born operator is
    linearised in (erm2d.h, inversion_para 2 of the reference);
-2. Born data of each model and CG-LSRTM on it (`erm2d_lsrtm_cg`, with the
-   diagonal w as preconditioner; or plain when no eps is given); a CG
-   monitor records, at selected iterations k, the gradient
-   g_k = L^T(d - L m_k) and the model m_k, and the training pair is
-   (w g_k, dm - m_k): the update that would take the iterate to the model
-   the noise-free data were made from (d = L dm, so a perfect net makes one
-   step of the flexible CG exact).  CG on the training models runs for a
-   fixed, unconverged number of iterations only to supply the gradients the
-   preconditioner meets along the way;
-3. 64x64 patches, normalised by the RMS of the input image (targets scaled
-   by one global constant so both sides are O(1)), train the U-Net on the GPU
-   (erm_unet_train: BF16-MFMA forward/backward, Adam);
-4. on a held-out model: plain CG, CG with the diagonal alone, flexible CG
-   with the learned preconditioner at every iteration, and learned warm
-   starts (k learned iterations, then CG with the diagonal): iterations and
-   seconds to a data misfit |L m - d| / |d| <= tol (the illumination's cost
-   included), and model errors against the plain converged image and the
-   true perturbation.
+2. Born data of each model and CG-LSRTM on it (`erm2d_lsrtm_cg` with the
+   diagonal w as preconditioner, all shots of a model batched in one launch
+   per kernel); a CG monitor records, at selected iterations k, the gradient
+   g_k = L^T(d - L m_k) and the model m_k.  The training pair is
+   (w g_k, m_K - m_k): the image CG converges to after K iterations minus
+   the current one, i.e. what the inverse Hessian would map the gradient to
+   (target "converged", default; "true" uses dm - m_k instead);
+3. 64x64 patches; the input is divided by its RMS and each target by its
+   own RMS (the net learns the direction of the update, CG's line search its
+   length, so early and late iterations, whose gradient-to-update ratios
+   differ by orders of magnitude, train together); the U-Net trains on the
+   GPU (erm_unet_train: BF16-MFMA forward/backward, Adam);
+4. on a held-out model: plain CG, CG with the diagonal alone, and flexible
+   CG whose iterations alternate z = net(w g) and z = w g (schedule "alt2",
+   default), plus learned warm starts (k net iterations, then the diagonal):
+   iterations and seconds to a data misfit |L m - d| / |d| <= tol (the
+   illumination's cost included), and model errors against the plain
+   converged image and the true perturbation.
 
 The test suite runs a reduced version of the same steps
 (tests/test_gpu_learned.py).
@@ -138,11 +138,14 @@ def illum_weights(g, shots, eps=1e-3):
     return (w / w.mean()).astype(np.float32), time.perf_counter() - t0
 
 
-def training_pairs(g, shots, dm, iters, keep, pmask=3, w=None):
+def training_pairs(g, shots, dm, iters, keep, pmask=3, w=None, target="true"):
     """CG on Born data of dm (plain, or preconditioned by the diagonal w);
-    returns the list of (x_k, dm - m_k) for k in keep, x_k = g_k (plain) or
+    returns the list of (x_k, t_k) for k in keep, x_k = g_k (plain) or
     w * g_k (the net's input in the weighted preconditioner), 2-channel
-    images, and the misfit history."""
+    images, and the misfit history.  target "true": t_k = dm - m_k, the
+    update to the model the data were made from; "converged": t_k =
+    m_K - m_k with m_K the LSRTM image after all `iters` iterations, the
+    update CG itself was converging to (an inverse-Hessian image pair)."""
     d = born_data(g, shots, dm)
     rec = {}
 
@@ -153,18 +156,25 @@ def training_pairs(g, shots, dm, iters, keep, pmask=3, w=None):
     pre = None if w is None else DiagPrecond(w)
     m, mis, sec, it = g.lsrtm_cg(shots, d, iters, pmask=pmask, monitor=mon, precond=pre)
     scale = 1.0 if w is None else w[None]
-    pairs = [((scale * rec[k][0]).astype(np.float32), dm[:2] - rec[k][1]) for k in sorted(rec)]
+    goal = dm[:2] if target == "true" else m[:2]
+    pairs = [((scale * rec[k][0]).astype(np.float32), (goal - rec[k][1]).astype(np.float32)) for k in sorted(rec)]
     return pairs, mis
 
 
-def normalise_pairs(pairs, scale=None):
-    """x = g / rms(g); t = scale * (dm - m_k) / rms(g) with one global scale
-    (the median over pairs of rms(g) / rms(target), unless given)."""
+def normalise_pairs(pairs, scale=None, per_pair=False):
+    """x = g / rms(g).  Global: t = scale * target / rms(g) with one scale
+    (the median over pairs of rms(g) / rms(target), unless given).  Per
+    pair: t = target / rms(target), so the net learns the direction of the
+    update only (CG's line search sets its length), which keeps pairs from
+    late iterations, whose gradient-to-update ratio differs by orders of
+    magnitude, on one footing."""
     rg = np.array([np.sqrt(np.mean(p[0].astype(np.float64) ** 2)) for p in pairs])
     rt = np.array([np.sqrt(np.mean(p[1].astype(np.float64) ** 2)) for p in pairs])
+    xs = [(p[0] / r).astype(np.float32) for p, r in zip(pairs, rg)]
+    if per_pair:
+        return xs, [(p[1] / max(r, 1e-30)).astype(np.float32) for p, r in zip(pairs, rt)], None
     if scale is None:
         scale = float(np.median(rg / np.maximum(rt, 1e-30)))
-    xs = [(p[0] / r).astype(np.float32) for p, r in zip(pairs, rg)]
     ts = [(scale * p[1] / r).astype(np.float32) for p, r in zip(pairs, rg)]
     return xs, ts, scale
 
@@ -181,6 +191,38 @@ def make_patch_set(xs, ts, size=64, stride=32, flip=True):
     return np.ascontiguousarray(np.concatenate(X), np.float32), np.ascontiguousarray(np.concatenate(T), np.float32)
 
 
+class Scheduled:
+    """Preconditioner that applies the net, z = net(w g), at the CG
+    iterations `use(k)` selects (k = 0, 1, ... counts applications) and the
+    diagonal, z = w g, at the others; both are positively homogeneous, so
+    the flexible CG may mix them.  Calls the C adaptor erm_unet_precond."""
+
+    def __init__(self, net, w, shape, use):
+        from . import lib, _fptr
+        self._lib, self._fptr = lib, _fptr
+        self.net, self.w, self.use, self.k = net, w, use, 0
+        nx, nz = shape
+        if lib.erm_unet_bind_model(net._h, nx, nz) or lib.erm_unet_bind_weights(net._h, _fptr(w)):
+            raise RuntimeError("binding the net to the model failed")
+
+    def __call__(self, g):
+        k, self.k = self.k, self.k + 1
+        if not self.use(k):
+            return (self.w[None] * g).astype(np.float32)
+        z = np.empty_like(g)
+        rc = self._lib.erm_unet_precond(self._fptr(np.ascontiguousarray(g, np.float32)), self._fptr(z), self.net._h)
+        if rc:
+            raise RuntimeError(f"erm_unet_precond returned {rc}")
+        return z
+
+
+SCHEDULES = {
+    "alt2": lambda k: k % 2 == 0,          # net, diagonal, net, ...
+    "alt3": lambda k: k % 3 == 0,          # net every third iteration
+    "first3_alt2": lambda k: k < 3 or k % 2 == 0,
+}
+
+
 def time_to(mis, sec, tol):
     """(iterations, seconds) at which the misfit first reaches tol, or (None, None)."""
     idx = np.nonzero(np.asarray(mis) <= tol)[0]
@@ -189,11 +231,16 @@ def time_to(mis, sec, tol):
     return int(idx[0]), float(sec[idx[0]])
 
 
-def generate(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7,
-             keep=(0, 1, 2, 4, 6), workers=4, seed0=1000, illum_eps=1e-3, log=print):
+KEEP = (0, 1, 2, 3, 5, 8, 12, 17, 23)  # CG iterations whose pairs train the net (of CG_ITERS)
+CG_ITERS = 30
+
+
+def generate(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=CG_ITERS,
+             keep=KEEP, seed0=1000, illum_eps=1e-3, target="converged", log=print):
     """Training pairs from CG on random models (seeds seed0, seed0+1, ...):
     plain CG, or with illum_eps set, CG preconditioned by the illumination
-    diagonal (pairs (w g_k, dm - m_k))."""
+    diagonal (pairs (w g_k, t_k), t_k as in training_pairs).  All shots of
+    a model run batched in one launch per kernel (erm2d_set_batch)."""
     nx, nz = train_shape
     g = Elastic2D(survey_params(nx, nz, train_lt))
     keep = set(keep)
@@ -203,9 +250,9 @@ def generate(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=11
         bvp, bvs, brho, dm = random_model(nx, nz, seed0 + i)
         g.load_model(1, bvp, bvs, brho)
         if i == 0:
-            g.set_workers(workers)
+            g.set_batch(len(shots))
         w = None if illum_eps is None else illum_weights(g, shots, illum_eps)[0]
-        pr, mis = training_pairs(g, shots, dm, cg_iters, keep, w=w)
+        pr, mis = training_pairs(g, shots, dm, cg_iters, keep, w=w, target=target)
         pairs += pr
         log(f"train model {i}: {'plain' if w is None else 'illumination-preconditioned'} CG misfit"
             f" {mis[0]:.3f} -> {mis[-1]:.4f} ({len(pr)} pairs)")
@@ -213,9 +260,9 @@ def generate(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=11
     return pairs
 
 
-def train(pairs, channels=(16, 32, 64, 64), epochs=60, batch=32, lr=1e-3, seed=0, log=print):
+def train(pairs, channels=(16, 32, 64, 64), epochs=60, batch=32, lr=1e-3, seed=0, per_pair=True, log=print):
     """Normalise, cut patches and train a fresh U-Net; returns (net, stats)."""
-    xs, ts, scale = normalise_pairs(pairs)
+    xs, ts, scale = normalise_pairs(pairs, per_pair=per_pair)
     X, T = make_patch_set(xs, ts)
     net = UNet(channels, seed=seed)
     t0 = time.perf_counter()
@@ -231,8 +278,8 @@ def train(pairs, channels=(16, 32, 64, 64), epochs=60, batch=32, lr=1e-3, seed=0
     return net, st
 
 
-def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, workers=4,
-             test_seed=5000, log=print, every=25, warm=5, flexible=True, illum_eps=1e-3):
+def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, batch_shots=0,
+             test_seed=5000, log=print, every=25, warm=5, flexible=False, illum_eps=1e-3, schedules=("alt2",)):
     """Plain CG against CG preconditioned by `net` on a held-out model.
 
     With illum_eps set, the approximate inverse Hessian is z = net(w g) with
@@ -245,8 +292,8 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
     g = Elastic2D(survey_params(nx, nz, test_lt))
     bvp, bvs, brho, dm = random_model(nx, nz, test_seed)
     g.load_model(1, bvp, bvs, brho)
-    g.set_workers(workers)
     shots = shot_positions(nx, test_shots)
+    g.set_batch(len(shots) if batch_shots <= 0 else min(len(shots), batch_shots))
     d = born_data(g, shots, dm)
 
     def progress(name):
@@ -289,6 +336,20 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
         if tp and tl:
             res["time_ratio"] = tl / tp
         log(f"held-out: plain {ip} it / {tp} s, learned {il} it / {tl} s to misfit {tol}")
+    for name in schedules:
+        if w is None:
+            break
+        pre = Scheduled(net, w, (nx, nz), SCHEDULES[name])
+        m_s, mis_s, sec_s, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, precond=pre,
+                                          monitor=progress(name))
+        sec_s = sec_s + t_w
+        i_s, t_s = time_to(mis_s, sec_s, tol)
+        key = "schedule_" + name
+        res[key] = dict(misfit=[float(v) for v in mis_s], seconds=[float(v) for v in sec_s], iters_to_tol=i_s,
+                        s_to_tol=t_s, model_err_vs_plain=float(np.linalg.norm(m_s[:2] - m_plain[:2])
+                                                               / np.linalg.norm(m_plain[:2])))
+        res["model_err_vs_true"][key] = nrm(m_s)
+        log(f"held-out: plain {ip} it / {tp} s, schedule {name} {i_s} it / {t_s} s to misfit {tol}")
     warms = [warm] if isinstance(warm, int) else list(warm or [])
     for wk in [v for v in warms if v]:
         m_w, mis_w1, sec_w1, it_w = g.lsrtm_cg(shots, d, wk, pmask=3, tol=tol, precond=net, weights=w)
@@ -310,26 +371,25 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
     return res
 
 
-def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=7, keep=(0, 1, 2, 4, 6),
+def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, cg_iters=CG_ITERS, keep=KEEP,
         test_shape=(256, 96),
         test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, epochs=20, batch=32, lr=1e-3,
-        channels=(16, 32, 64, 64), workers=4, seed0=1000, test_seed=5000, warm=5, flexible=True, illum_eps=1e-3,
-        log=print):
+        channels=(16, 32, 64, 64), batch_shots=0, seed0=1000, test_seed=5000, warm=3, flexible=False, illum_eps=1e-3,
+        target="converged", per_pair=True, schedules=("alt2",), log=print):
     """The whole workflow; returns a dict of results."""
     res = {"config": dict(train_models=train_models, train_shape=train_shape, train_shots=train_shots,
                           train_lt=train_lt, cg_iters=cg_iters, keep=list(keep), test_shape=test_shape,
                           test_shots=test_shots,
                           test_lt=test_lt, tol=tol, epochs=epochs, batch=batch, lr=lr, channels=channels,
-                          illum_eps=illum_eps)}
+                          illum_eps=illum_eps, target=target, per_pair=per_pair)}
     t0 = time.perf_counter()
-    pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, keep=keep, workers=workers,
-                     seed0=seed0,
-                     illum_eps=illum_eps, log=log)
+    pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, keep=keep, seed0=seed0,
+                     illum_eps=illum_eps, target=target, log=log)
     res["datagen_s"] = time.perf_counter() - t0
-    net, st = train(pairs, channels, epochs, batch, lr, seed=seed0, log=log)
+    net, st = train(pairs, channels, epochs, batch, lr, seed=seed0, per_pair=per_pair, log=log)
     res.update(st)
-    res.update(evaluate(net, test_shape, test_shots, test_lt, test_iters, tol, workers, test_seed, log=log,
-                        warm=warm, flexible=flexible, illum_eps=illum_eps))
+    res.update(evaluate(net, test_shape, test_shots, test_lt, test_iters, tol, batch_shots, test_seed, log=log,
+                        warm=warm, flexible=flexible, illum_eps=illum_eps, schedules=schedules))
     net.close()
     return res
 
@@ -341,13 +401,19 @@ def main(argv=None):
     ap.add_argument("--train-models", type=int, default=160)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--test-iters", type=int, default=400)
-    ap.add_argument("--workers", type=int, default=4)
-    ap.add_argument("--warm", type=int, nargs="*", default=[5],
+    ap.add_argument("--batch-shots", type=int, default=0, help="shots per batched launch (0: all shots of a model)")
+    ap.add_argument("--warm", type=int, nargs="*", default=[3],
                     help="learned warm-start lengths to compare (net iterations before the net-free CG)")
-    ap.add_argument("--no-flexible", action="store_true", help="skip the every-iteration learned CG run")
-    ap.add_argument("--cg-iters", type=int, default=7, help="CG iterations per training model")
-    ap.add_argument("--keep", type=int, nargs="*", default=[0, 1, 2, 4, 6],
+    ap.add_argument("--flexible", action="store_true", help="also run CG with the net at every iteration")
+    ap.add_argument("--schedules", nargs="*", default=["alt2"], choices=sorted(SCHEDULES),
+                    help="net/diagonal schedules of the flexible CG to compare")
+    ap.add_argument("--cg-iters", type=int, default=CG_ITERS, help="CG iterations per training model")
+    ap.add_argument("--keep", type=int, nargs="*", default=list(KEEP),
                     help="CG iterations whose (gradient, remaining update) pairs are kept for training")
+    ap.add_argument("--target", choices=["true", "converged"], default="converged",
+                    help="training target: dm - m_k (true) or m_K - m_k with m_K the image after --cg-iters")
+    ap.add_argument("--global-scale", action="store_true",
+                    help="one global target scale instead of normalising each target by its own RMS")
     ap.add_argument("--illum-eps", type=float, default=1e-3,
                     help="illumination diagonal w = 1/(W + eps max W) composed with the net (train and test)")
     ap.add_argument("--no-illum", action="store_true", help="no diagonal: the net alone on plain-CG gradients")
@@ -355,8 +421,9 @@ def main(argv=None):
     if any(k >= a.cg_iters or k < 0 for k in a.keep):
         ap.error("--keep iterations must lie in [0, --cg-iters)")
     res = run(train_models=a.train_models, cg_iters=a.cg_iters, keep=a.keep, epochs=a.epochs,
-              test_iters=a.test_iters, workers=a.workers,
-              warm=a.warm, flexible=not a.no_flexible, illum_eps=None if a.no_illum else a.illum_eps,
+              test_iters=a.test_iters, batch_shots=a.batch_shots,
+              warm=a.warm, flexible=a.flexible, illum_eps=None if a.no_illum else a.illum_eps,
+              target=a.target, per_pair=not a.global_scale, schedules=tuple(a.schedules),
               log=lambda s: print(s, flush=True))
     txt = json.dumps(res)
     if a.out:
