@@ -388,6 +388,7 @@ int erm_batch_set_shots(erm_batch* b, const erm_shot_window* shots, int nshots) 
         const int rix = std::max(1, P.receiver_interval_x), riy = std::max(1, P.receiver_interval_y);
         q.receiver_start_x = 0; q.receiver_interval_x = rix; q.receiver_num_x = (w.wx + rix - 1) / rix;
         q.receiver_start_y = 0; q.receiver_interval_y = riy; q.receiver_num_y = (w.wy + riy - 1) / riy;
+        q.tile_x = 64;  // the batched kernels run 64-wide tiles
         q.tile_y = b->ty;
         q.zchunk = b->kchunk;
         q.xcd_map = 0;

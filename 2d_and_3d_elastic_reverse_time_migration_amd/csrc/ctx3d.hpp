@@ -57,7 +57,7 @@ struct erm_ctx {
     // (group 0: the plain kernel).  Built for the current ty/kchunk.
     int* tiles[8] = {nullptr};
     int ntl[8] = {0};
-    int split_ty = 0, split_kchunk = 0;
+    int split_tx = 0, split_ty = 0, split_kchunk = 0;
     bool split_built = false;
     bool region_split = true;
     // source-wavefield reconstruction from boundary strips
@@ -116,7 +116,7 @@ inline Geo make_geo(const erm_ctx* c) {
     g.cx = c->cx; g.cy = c->cy; g.cz = c->cz;
     g.hx = c->hxyz[0]; g.hy = c->hxyz[1]; g.hz = c->hxyz[2];
     g.kchunk = c->kchunk;
-    g.ntx = (c->nnx + 63) / 64;
+    g.ntx = (c->nnx + c->tx - 1) / c->tx;
     g.nty = (c->nny + c->ty - 1) / c->ty;
     const int nch = (c->nnz - 2 * c->R + c->kchunk - 1) / c->kchunk;
     g.ntiles = g.ntx * g.nty * nch;

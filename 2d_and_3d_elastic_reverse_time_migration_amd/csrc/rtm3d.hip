@@ -147,7 +147,7 @@ std::vector<int> xcd_band_order(const std::vector<int>& t, const Geo& g, int non
 }
 
 int build_split(erm_ctx* c) {
-    if (c->split_built && c->split_ty == c->ty && c->split_kchunk == c->kchunk) return 0;
+    if (c->split_built && c->split_tx == c->tx && c->split_ty == c->ty && c->split_kchunk == c->kchunk) return 0;
     Geo g = make_geo(c);
     const int R = c->R;
     std::vector<int> grp[8];
@@ -156,7 +156,7 @@ int build_split(erm_ctx* c) {
     };
     for (int l = 0; l < g.ntiles; ++l) {
         const int bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / (g.ntx * g.nty);
-        const int i0 = std::max(bx * 64, R), i1 = std::min(bx * 64 + 64, c->nnx - R) - 1;
+        const int i0 = std::max(bx * c->tx, R), i1 = std::min(bx * c->tx + c->tx, c->nnx - R) - 1;
         const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
         const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
         if (i0 > i1 || j0 > j1 || k0 > k1) continue;
@@ -177,6 +177,7 @@ int build_split(erm_ctx* c) {
         if (dmalloc(c, &c->tiles[m], grp[m].size())) return ERM_E_NOMEM;
         HIPC(c, hipMemcpy(c->tiles[m], grp[m].data(), grp[m].size() * sizeof(int), hipMemcpyHostToDevice));
     }
+    c->split_tx = c->tx;
     c->split_ty = c->ty;
     c->split_kchunk = c->kchunk;
     c->split_built = true;
@@ -287,20 +288,25 @@ void strip_velocity(erm_ctx* c, float* const* f, int lev, int restore) {
 }
 // reverse step it after the stress-side slabs of level it+1/2 are in place:
 // state {sigma^{it+1/2}, v^{it+1}} -> {sigma^{it-1/2}, v^{it}}
-template <int R, int TY>
+template <int R, int TX, int TY>
 void reverse_rest_rt(erm_ctx* c, int it) {
     const Geo g = reversed_geo(c);
     const Wave w = wave_of(c->fr);
-    hipLaunchKernelGGL((k_velocity<R, 64, TY, MODE_PLAIN, 0>), grid_of(g), dim3(64, TY), 0, c->stream, w, c->BU[1], g,
+    hipLaunchKernelGGL((k_velocity<R, TX, TY, MODE_PLAIN, 0>), grid_of(g), dim3(TX, TY), 0, c->stream, w, c->BU[1], g,
                        c->coef, make_excite(c), it, c->pml, nullptr);
     strip_velocity(c, c->fr, it, 1);
     const int src_k = c->src < 0 ? -1 : (int)(c->src / c->pxy), src_col = c->src < 0 ? -1 : (int)(c->src % c->pxy);
-    hipLaunchKernelGGL((k_stress<R, 64, TY, 0>), grid_of(g), dim3(64, TY), 0, c->stream, w, c->M[1], c->MU[1], g,
+    hipLaunchKernelGGL((k_stress<R, TX, TY, 0>), grid_of(g), dim3(TX, TY), 0, c->stream, w, c->M[1], c->MU[1], g,
                        c->coef, src_k, src_col, (const float*)c->wavelet_neg, c->wl, it, c->pml, nullptr);
 }
 void reverse_rest(erm_ctx* c, int it) {
-    if (c->R == 4) { if (c->ty == 4) reverse_rest_rt<4, 4>(c, it); else reverse_rest_rt<4, 8>(c, it); }
-    else { if (c->ty == 4) reverse_rest_rt<6, 4>(c, it); else reverse_rest_rt<6, 8>(c, it); }
+    if (c->R == 4) {
+        if (c->tx == 32) { if (c->ty == 8) reverse_rest_rt<4, 32, 8>(c, it); else reverse_rest_rt<4, 32, 16>(c, it); }
+        else if (c->ty == 4) reverse_rest_rt<4, 64, 4>(c, it);
+        else reverse_rest_rt<4, 64, 8>(c, it);
+    } else {
+        if (c->ty == 4) reverse_rest_rt<6, 64, 4>(c, it); else reverse_rest_rt<6, 64, 8>(c, it);
+    }
 }
 void reverse_step(erm_ctx* c, int it) {
     strip_stress(c, c->fr, it, 1);
@@ -365,9 +371,13 @@ int build_cpml(erm_ctx* c) {
     return 0;
 }
 
+// launch shapes: 64 x {4, 8} at both radii, 32 x {8, 16} at radius 4
 void do_stress(erm_ctx* c, int which, long src, int it) {
     if (c->R == 4) {
-        if (c->ty == 4) stress_launch<4, 64, 4>(c, which, src, it);
+        if (c->tx == 32) {
+            if (c->ty == 8) stress_launch<4, 32, 8>(c, which, src, it);
+            else stress_launch<4, 32, 16>(c, which, src, it);
+        } else if (c->ty == 4) stress_launch<4, 64, 4>(c, which, src, it);
         else stress_launch<4, 64, 8>(c, which, src, it);
     } else {
         if (c->ty == 4) stress_launch<6, 64, 4>(c, which, src, it);
@@ -378,7 +388,10 @@ void do_stress(erm_ctx* c, int which, long src, int it) {
 template <int MODE>
 void do_velocity_m(erm_ctx* c, int which, int it) {
     if (c->R == 4) {
-        if (c->ty == 4) velocity_launch<4, 64, 4, MODE>(c, which, it);
+        if (c->tx == 32) {
+            if (c->ty == 8) velocity_launch<4, 32, 8, MODE>(c, which, it);
+            else velocity_launch<4, 32, 16, MODE>(c, which, it);
+        } else if (c->ty == 4) velocity_launch<4, 64, 4, MODE>(c, which, it);
         else velocity_launch<4, 64, 8, MODE>(c, which, it);
     } else {
         if (c->ty == 4) velocity_launch<6, 64, 4, MODE>(c, which, it);
@@ -749,9 +762,11 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     c->cx = (float)(dts / pin->dx);
     c->cy = (float)(dts / pin->dy);
     c->cz = (float)(dts / pin->dz);
-    c->ty = pin->tile_y == 4 ? 4 : 8;
+    // tile shapes 64 x {4, 8}; at radius 4 also 32 x {8, 16} (tile_x = 32)
+    c->tx = pin->tile_x == 32 && R == 4 ? 32 : 64;
+    c->ty = c->tx == 64 ? (pin->tile_y == 4 ? 4 : 8) : (pin->tile_y == 16 ? 16 : 8);
     {
-        int tiles = ((c->nnx + 63) / 64) * ((c->nny + c->ty - 1) / c->ty);
+        int tiles = ((c->nnx + c->tx - 1) / c->tx) * ((c->nny + c->ty - 1) / c->ty);
         int nzi = c->nnz - 2 * R;
         int chunks = std::max(1, (4096 + tiles - 1) / tiles);
         int kc = (nzi + chunks - 1) / chunks;
@@ -1324,7 +1339,7 @@ int erm_autotune(erm_ctx* c, int steps, int* tile_y, int* zchunk, double* ms_per
     const int nzi = c->nnz - 2 * c->R;
     double best = 1e300;
     int bt = c->ty, bk = c->kchunk;
-    const int tys[2] = {4, 8};
+    const int tys[2] = {c->tx == 64 ? 4 : 8, c->tx == 64 ? 8 : 16};
     const int kcs[8] = {4, 5, 6, 7, 8, 12, 16, 32};  // z sweep: parallelism vs re-read z halos
     int rc = 0;
     for (int ty : tys)

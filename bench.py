@@ -517,6 +517,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true",
                     help="skip the CPU baseline and the LSRTM comparison (profiling runs)")
     ap.add_argument("--no-kernel-events", action="store_true")
+    ap.add_argument("--tile-x", type=int, default=0, choices=[0, 32, 64],
+                    help="tile width (0: 64); 32-wide tiles take tile_y 8 or 16")
     ap.add_argument("--tile-y", type=int, default=0, help="launch shape override (0: library default)")
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0: library default)")
     ap.add_argument("--boundary", choices=["cpml", "damp"], default="cpml")
@@ -562,7 +564,7 @@ def main():
         return run_rtm2d(erm, args)
     nt = args.nt
     p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
-    p.tile_y, p.zchunk = args.tile_y, args.zchunk
+    p.tile_x, p.tile_y, p.zchunk = args.tile_x, args.tile_y, args.zchunk
     p.xcd_map = args.xcd_map
     p.region_streams = args.region_streams
     vp, vs, rho = erm.make_model3d(1, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, 2)
