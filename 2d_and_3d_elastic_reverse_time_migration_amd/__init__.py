@@ -75,6 +75,8 @@ def _load():
         "erm_algorithmic_bytes": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double), P(ctypes.c_double)]),
         "erm_synchronize": (ctypes.c_int, [vp]),
         "erm_autotune": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_double)]),
+        "erm_autotune_shape": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                              P(ctypes.c_double)]),
         "erm_strip_bytes": (ctypes.c_int, [vp, P(ctypes.c_double)]),
         "erm_forward_smoothed_strips": (ctypes.c_int, [vp]),
         "erm_backward_smoothed_xcorr": (ctypes.c_int, [vp, fp]),
@@ -383,10 +385,11 @@ class RTM3D:
         return out
 
     def autotune(self, steps=4):
-        """Pick the fastest tile height / z sweep depth (clobbers the wavefield)."""
-        ty, kc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        self._c(lib.erm_autotune(self._h, steps, ctypes.byref(ty), ctypes.byref(kc), ctypes.byref(ms)))
-        return {"tile_y": ty.value, "zchunk": kc.value, "ms_per_step": ms.value}
+        """Pick the fastest tile shape / z sweep depth (clobbers the wavefield)."""
+        tx, ty, kc, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._c(lib.erm_autotune_shape(self._h, steps, ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(kc),
+                                       ctypes.byref(ms)))
+        return {"tile_x": tx.value, "tile_y": ty.value, "zchunk": kc.value, "ms_per_step": ms.value}
 
 
 def _check2(rc, ctx=None):

@@ -46,6 +46,11 @@ struct erm_ctx {
     int sxr = 0, syr = 0, szr = 0;  // shot position in model coordinates (sx_real, ...)
     long src = -1;                  // padded index of the source
     int tx = 64, ty = 8, kchunk = 32, xcd_map = 0;
+    // 32-wide tiles start at the first updated column x = R, and every
+    // volume's base is shifted by xoff floats inside its allocation so that
+    // this column starts a 128-byte line in every row (64-wide tiles: 0, 0)
+    int x0 = 0;
+    size_t xoff = 0;
     // CPML (boundary == ERM_BOUNDARY_CPML)
     bool cpml = false;
     float* psi_buf = nullptr;           // C-PML state: 21 memory variables, then coefficients (layout: Pml)
@@ -116,7 +121,8 @@ inline Geo make_geo(const erm_ctx* c) {
     g.cx = c->cx; g.cy = c->cy; g.cz = c->cz;
     g.hx = c->hxyz[0]; g.hy = c->hxyz[1]; g.hz = c->hxyz[2];
     g.kchunk = c->kchunk;
-    g.ntx = (c->nnx + c->tx - 1) / c->tx;
+    g.x0 = c->x0;
+    g.ntx = ((c->x0 ? c->nnx - 2 * c->R : c->nnx) + c->tx - 1) / c->tx;
     g.nty = (c->nny + c->ty - 1) / c->ty;
     const int nch = (c->nnz - 2 * c->R + c->kchunk - 1) / c->kchunk;
     g.ntiles = g.ntx * g.nty * nch;

@@ -53,6 +53,7 @@ struct Geo {
     const float* hy;
     const float* hz;
     int kchunk;        // z planes per workgroup
+    int x0;            // x of the first tile column (R with 32-wide tiles, which start at the first updated cell)
     int ntx, nty;      // column tiles in x and y
     int ntiles;        // ntx * nty * z chunks
     int per_xcd;       // tiles per XCD when xcd_map != 0
@@ -63,10 +64,9 @@ struct Geo {
 // fastest.  Workgroups are dispatched round-robin over the 8 XCDs (workgroup
 // b runs on XCD b % 8), each with its own L2; with xcd_map the tiles are
 // dealt so that XCD x walks a contiguous run [x*per, (x+1)*per) of tile ids
-// and its tiles in flight are spatial neighbours.  Measured on the
-// 296x296x168 grid with 128-byte rows, the plain order (all in-flight tiles
-// one contiguous band, halo lines shared through the MALL) is ~10% faster,
-// so it is the default.
+// and its tiles in flight are spatial neighbours (xcd_map 1; the default
+// y-band order of the region-split lists is applied to the tile lists on the
+// host instead).
 struct TileId {
     int bx, by, bz;
     bool valid;
@@ -266,7 +266,7 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
     const TileId tid = tile_of_block(g, blk);
     if (!tid.valid) return;  // uniform per block
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i0 = tid.bx * TX, j0 = tid.by * TY;
+    const int i0 = g.x0 + tid.bx * TX, j0 = tid.by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int kb = R + tid.bz * g.kchunk;
     const int ke = min(kb + g.kchunk, g.nnz - R);
@@ -516,7 +516,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
     const TileId tid = tile_of_block(g, blk);
     if (!tid.valid) return;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i0 = tid.bx * TX, j0 = tid.by * TY;
+    const int i0 = g.x0 + tid.bx * TX, j0 = tid.by * TY;
     const int i = i0 + tx, j = j0 + ty;
     const int kb = R + tid.bz * g.kchunk;
     const int ke = min(kb + g.kchunk, g.nnz - R);

@@ -98,6 +98,18 @@ int dmalloc(erm_ctx* c, T** ptr, size_t n) {
     return 0;
 }
 
+// a volume (ncell floats, the x = R column of every row 128-byte aligned
+// when xoff > 0): the allocation holds xoff floats in front of the base
+int vmalloc(erm_ctx* c, float** ptr, size_t n) {
+    float* p = nullptr;
+    if (int rc = dmalloc(c, &p, n + c->xoff)) return rc;
+    *ptr = p + c->xoff;
+    return 0;
+}
+void vfree(const erm_ctx* c, float* p) {
+    if (p) (void)hipFree(p - c->xoff);
+}
+
 inline dim3 grid_of(const Geo& g) { return dim3(g.xcd_map ? 8 * g.per_xcd : g.ntiles); }
 
 // Tile groups of the CPML region split for the current launch shape: group 0
@@ -156,7 +168,7 @@ int build_split(erm_ctx* c) {
     };
     for (int l = 0; l < g.ntiles; ++l) {
         const int bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / (g.ntx * g.nty);
-        const int i0 = std::max(bx * c->tx, R), i1 = std::min(bx * c->tx + c->tx, c->nnx - R) - 1;
+        const int i0 = std::max(c->x0 + bx * c->tx, R), i1 = std::min(c->x0 + bx * c->tx + c->tx, c->nnx - R) - 1;
         const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
         const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
         if (i0 > i1 || j0 > j1 || k0 > k1) continue;
@@ -449,17 +461,17 @@ int check_grid(const erm_params* p, int R, std::string* err) {
 
 void free_all(erm_ctx* c) {
     auto F = [](void* p) { if (p) (void)hipFree(p); };
-    for (auto& p : c->f) F(p);
-    for (int w = 0; w < 2; ++w) { F(c->M[w]); F(c->MU[w]); F(c->BU[w]); }
+    for (auto& p : c->f) vfree(c, p);
+    for (int w = 0; w < 2; ++w) { vfree(c, c->M[w]); vfree(c, c->MU[w]); vfree(c, c->BU[w]); }
     for (auto& p : c->hxyz) F(p);
-    F(c->wavelet); F(c->exbuf);
+    F(c->wavelet); vfree(c, c->exbuf);
     F(c->rec); F(c->rec_idx);
     F(c->psi_buf);
     for (auto& t : c->tiles) F(t);
-    for (auto& q : c->fr) F(q);
+    for (auto& q : c->fr) vfree(c, q);
     F(c->strip_idx); F(c->strips); F(c->wavelet_neg);
-    for (auto& q : c->ximg) F(q);
-    F(c->decbuf);
+    for (auto& q : c->ximg) vfree(c, q);
+    vfree(c, c->decbuf);
     for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -470,7 +482,7 @@ void free_all(erm_ctx* c) {
 
 // divergence (q = 0) or curl component q of the current velocity into decbuf
 int decompose(erm_ctx* c, int q) {
-    if (!c->decbuf && dmalloc(c, &c->decbuf, c->ncell)) return ERM_E_NOMEM;
+    if (!c->decbuf && vmalloc(c, &c->decbuf, c->ncell)) return ERM_E_NOMEM;
     const long tot = (long)c->nnx * c->nny * c->nnz;
     const dim3 grd((unsigned)((tot + 255) / 256));
     const float ix = (float)(1.0 / c->p.dx), iy = (float)(1.0 / c->p.dy), iz = (float)(1.0 / c->p.dz);
@@ -763,8 +775,13 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     c->cy = (float)(dts / pin->dy);
     c->cz = (float)(dts / pin->dz);
     // tile shapes 64 x {4, 8}; at radius 4 also 32 x {8, 16} (tile_x = 32)
-    c->tx = pin->tile_x == 32 && R == 4 ? 32 : 64;
+    // default 32-wide at radius 4 (32 x 8 measured fastest with C-PML), 64-wide at radius 6
+    c->tx = R == 4 && pin->tile_x != 64 ? 32 : 64;
     c->ty = c->tx == 64 ? (pin->tile_y == 4 ? 4 : 8) : (pin->tile_y == 16 ? 16 : 8);
+    if (c->px % 32 == 0) {  // tiles over the updated columns only, their first column line-aligned
+        c->x0 = R;
+        c->xoff = (32 - R % 32) % 32;
+    }
     {
         int tiles = ((c->nnx + c->tx - 1) / c->tx) * ((c->nny + c->ty - 1) / c->ty);
         int nzi = c->nnz - 2 * R;
@@ -809,8 +826,8 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     }
     for (auto& e : c->pass_ev)
         if (hipEventCreate(&e) != hipSuccess) { set_global_error("event create failed"); free_all(c); delete c; return ERM_E_DEVICE; }
-    for (auto& p : c->f) CK(dmalloc(c, &p, c->ncell));
-    CK(dmalloc(c, &c->exbuf, (size_t)EX_COUNT * c->ncell));
+    for (auto& p : c->f) CK(vmalloc(c, &p, c->ncell));
+    CK(vmalloc(c, &c->exbuf, (size_t)EX_COUNT * c->ncell));
     c->ex_time = reinterpret_cast<int*>(c->exbuf + (size_t)EX_TIME * c->ncell);
     for (int q = 0; q < 5; ++q) c->ex[q] = c->exbuf + (size_t)(EX_AMP + q) * c->ncell;
     for (int q = 0; q < 3; ++q) c->img[q] = c->exbuf + (size_t)(EX_IMG_TP + q) * c->ncell;
@@ -945,7 +962,7 @@ int erm_load_model(erm_ctx* c, int which, const float* vp, const float* vs, cons
         hB[n] = 1.0f / r;
     }
     if (!c->M[which]) {
-        if (dmalloc(c, &c->M[which], c->ncell) || dmalloc(c, &c->MU[which], c->ncell) || dmalloc(c, &c->BU[which], c->ncell))
+        if (vmalloc(c, &c->M[which], c->ncell) || vmalloc(c, &c->MU[which], c->ncell) || vmalloc(c, &c->BU[which], c->ncell))
             return ERM_E_NOMEM;
     }
     if (upload(c, c->M[which], hM.data()) || upload(c, c->MU[which], hU.data()) || upload(c, c->BU[which], hB.data()))
@@ -1256,7 +1273,7 @@ int erm_forward_smoothed_strips(erm_ctx* c) {
         if (dmalloc(c, &c->wavelet_neg, wn.size())) return ERM_E_NOMEM;
         HIPC(c, hipMemcpy(c->wavelet_neg, wn.data(), wn.size() * sizeof(float), hipMemcpyHostToDevice));
     }
-    for (auto& q : c->fr) if (!q && dmalloc(c, &q, c->ncell)) return ERM_E_NOMEM;
+    for (auto& q : c->fr) if (!q && vmalloc(c, &q, c->ncell)) return ERM_E_NOMEM;
     if ((rc = fill(c, c->strips, (size_t)kStripFields * c->n_strip, 0.f))) return rc;  // slot 0: the zero initial state
     if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) return rc;
     c->have_strips = false;
@@ -1295,7 +1312,7 @@ int erm_backward_smoothed_xcorr(erm_ctx* c, const float* rec) {
     if (!c->have_strips) return c->fail(ERM_E_STATE, "no boundary strips: run erm_forward_smoothed_strips first");
     int rc;
     if (rec && (rc = erm_set_record(c, rec))) return rc;
-    for (auto& q : c->ximg) if (!q && dmalloc(c, &q, c->ncell)) return ERM_E_NOMEM;
+    for (auto& q : c->ximg) if (!q && vmalloc(c, &q, c->ncell)) return ERM_E_NOMEM;
     if ((rc = erm_zero_wavefield(c))) return rc;
     for (auto& q : c->img) if ((rc = fill(c, q, c->ncell, 0.f))) return rc;
     for (auto& q : c->ximg) if ((rc = fill(c, q, c->ncell, 0.f))) return rc;
@@ -1330,7 +1347,7 @@ int erm_synchronize(erm_ctx* c) {
     return 0;
 }
 
-int erm_autotune(erm_ctx* c, int steps, int* tile_y, int* zchunk, double* ms_per_step) {
+int erm_autotune_shape(erm_ctx* c, int steps, int* tile_x, int* tile_y, int* zchunk, double* ms_per_step) {
     if (!c || steps <= 0) return c ? c->fail(ERM_E_PARAM, "steps must be positive") : ERM_E_PARAM;
     if (!c->have_model[1]) return c->fail(ERM_E_STATE, "smoothed model not loaded");
     HIPC(c, hipSetDevice(c->device));
@@ -1338,38 +1355,48 @@ int erm_autotune(erm_ctx* c, int steps, int* tile_y, int* zchunk, double* ms_per
     c->profiling = false;
     const int nzi = c->nnz - 2 * c->R;
     double best = 1e300;
-    int bt = c->ty, bk = c->kchunk;
-    const int tys[2] = {c->tx == 64 ? 4 : 8, c->tx == 64 ? 8 : 16};
+    int bx = c->tx, bt = c->ty, bk = c->kchunk;
+    // tile shapes: 64 x {4, 8}, and at radius 4 32 x {8, 16}
+    std::vector<std::pair<int, int>> shapes = {{64, 4}, {64, 8}};
+    if (c->R == 4) { shapes.push_back({32, 8}); shapes.push_back({32, 16}); }
     const int kcs[8] = {4, 5, 6, 7, 8, 12, 16, 32};  // z sweep: parallelism vs re-read z halos
     int rc = 0;
-    for (int ty : tys)
+    for (const auto& sh : shapes)
         for (int kc : kcs) {
             if (kc > nzi && kc != kcs[0]) continue;
             if (!sweep_fits(c, std::min(kc, nzi))) continue;
-            c->ty = ty;
+            c->tx = sh.first;
+            c->ty = sh.second;
             c->kchunk = std::min(kc, nzi);
             if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) break;
             if ((rc = run_pass(c, ERM_PASS_SOURCE, 0, 1))) break;  // warm-up
             if ((rc = run_pass(c, ERM_PASS_SOURCE, 1, steps))) break;
             const double per = c->last_pass_ms / steps;
-            if (per < best) { best = per; bt = c->ty; bk = c->kchunk; }
+            if (per < best) { best = per; bx = c->tx; bt = c->ty; bk = c->kchunk; }
         }
     if (!rc && best == 1e300) {  // no candidate fits the 32-bit span: time the current shape
+        c->tx = bx;
         c->ty = bt;
         c->kchunk = bk;
         if (!(rc = erm_zero_wavefield(c)) && !(rc = erm_zero_excitation(c)) && !(rc = run_pass(c, ERM_PASS_SOURCE, 0, 1)) &&
             !(rc = run_pass(c, ERM_PASS_SOURCE, 1, steps)))
             best = c->last_pass_ms / steps;
     }
+    c->tx = bx;
     c->ty = bt;
     c->kchunk = bk;
     c->profiling = prof;
     if (rc) return rc;
     if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) return rc;
+    if (tile_x) *tile_x = bx;
     if (tile_y) *tile_y = bt;
     if (zchunk) *zchunk = bk;
     if (ms_per_step) *ms_per_step = best;
     return 0;
+}
+
+int erm_autotune(erm_ctx* c, int steps, int* tile_y, int* zchunk, double* ms_per_step) {
+    return erm_autotune_shape(c, steps, nullptr, tile_y, zchunk, ms_per_step);
 }
 
 int erm_copy_bandwidth(int device, size_t bytes, int reps, double* ms_per_copy) {

@@ -518,7 +518,7 @@ def main():
                     help="skip the CPU baseline and the LSRTM comparison (profiling runs)")
     ap.add_argument("--no-kernel-events", action="store_true")
     ap.add_argument("--tile-x", type=int, default=0, choices=[0, 32, 64],
-                    help="tile width (0: 64); 32-wide tiles take tile_y 8 or 16")
+                    help="tile width (0: library default, 32 at radius 4); 32-wide tiles take tile_y 8 or 16")
     ap.add_argument("--tile-y", type=int, default=0, help="launch shape override (0: library default)")
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0: library default)")
     ap.add_argument("--boundary", choices=["cpml", "damp"], default="cpml")
@@ -579,7 +579,7 @@ def main():
     g.forward_real(want_record=False)  # the record stays on the device
     log(f"bench: modeled the receiver record in {time.perf_counter() - t0:.2f} s")
     tune = None
-    if not (args.tile_y or args.zchunk):
+    if not (args.tile_x or args.tile_y or args.zchunk):
         tune = g.autotune(steps=3)  # launch shape; results do not depend on it
         log(f"bench: autotuned {tune}")
 
@@ -657,7 +657,8 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
         import config2
         fx = np.load(fx_path)
-        shape = {"tile_y": tune["tile_y"] if tune else args.tile_y, "zchunk": tune["zchunk"] if tune else args.zchunk,
+        shape = {"tile_x": tune["tile_x"] if tune else args.tile_x,
+                 "tile_y": tune["tile_y"] if tune else args.tile_y, "zchunk": tune["zchunk"] if tune else args.zchunk,
                  "xcd_map": args.xcd_map}
         pe, et, ea, imgs, vz = config2.gpu_shot(erm, int(fx["nt"]), **shape)
         fp64 = config2.compare(fx, pe, et, ea, imgs, vz)

@@ -90,7 +90,9 @@ typedef struct erm_params {
                            (3D_forward_or_back_together.cu:173) */
     float cpml_r;       /* CPML target reflection coefficient (default 1e-3) */
     float cpml_vmax;    /* CPML reference velocity in m/s (0: max vp of the model) */
-    int   tile_x, tile_y, zchunk;   /* kernel launch shape (0: autotuned default).  The kernels use
+    int   tile_x, tile_y, zchunk;   /* kernel launch shape: tile_x 32 (default at radius 4; tile_y 8 or 16)
+                                       or 64 (tile_y 4 or 8; radius 6 always); the tiles cover the updated
+                                       columns from x = R, line-aligned.  The kernels use
                                        32-bit offsets: (zchunk + 2R) x-y planes must span < 4 GiB, else
                                        erm_create fails (ERM_E_PARAM; ERM_E_SIZE if one plane is too big) */
     int   pitch_align;  /* row pitch multiple in floats (0: 32 = 128 B; 1: dense rows) */
@@ -241,10 +243,12 @@ int   erm_reconstruct(erm_ctx* ctx, int it_end);
 int   erm_get_reconstructed_field(erm_ctx* ctx, int field, float* dst);
 /* Launch-shape autotuner (no reference counterpart; the reference launches
  * one block per z plane): times `steps` source-pass steps (smoothed model)
- * for tile heights {4, 8} x z sweep depths {4, 5, 6, 7, 8, 12, 16, 32} and keeps the
- * fastest for later passes (depths whose sweep spans 4 GiB or more are
- * skipped).  Results do not depend on the shape.  Zeroes the wavefield and
- * the excitation state afterwards. */
+ * for tiles 64 x {4, 8} (and 32 x {8, 16} at radius 4) x z sweep depths
+ * {4, 5, 6, 7, 8, 12, 16, 32} and keeps the fastest for later passes (depths
+ * whose sweep spans 4 GiB or more are skipped).  Results do not depend on
+ * the shape.  Zeroes the wavefield and the excitation state afterwards.
+ * erm_autotune reports the chosen tile height and depth only. */
+int   erm_autotune_shape(erm_ctx* ctx, int steps, int* tile_x, int* tile_y, int* zchunk, double* ms_per_step);
 int   erm_autotune(erm_ctx* ctx, int steps, int* tile_y, int* zchunk, double* ms_per_step);
 /* Measurement utility (no reference counterpart): time `reps` launches of a
  * plain float4 grid-stride copy of `bytes` bytes on `device` for each grid of

@@ -97,10 +97,10 @@ def test_gpu_cpml_matches_fp64_oracle(erm, oracle, R):
 def test_gpu_cpml_region_split_and_autotune_are_exact(erm):
     """The region-split launches (plain kernel on tiles without C-PML cells),
     run on one stream or concurrently on two (region_streams), in the
-    XCD-aware tile orders (xcd_map 1: contiguous chunks, 2: y bands), 32-wide
-    tiles (32 x 8, 32 x 16) and every autotuned launch shape give
-    bit-identical shots to one C-PML kernel over all tiles with the default
-    shape."""
+    XCD-aware tile orders (xcd_map 1: contiguous chunks, 2: y bands), tiles
+    32 x 8 (default), 32 x 16, 64 x 4, 64 x 8 and every autotuned launch shape
+    give bit-identical shots to one C-PML kernel over all tiles with the
+    default shape."""
     p = cases.params(nx=150, ny=40, nz=44, b=12, R=4, lt=80, dt=1.0, freq=25.0, src=(75, 20, 3))
     p.boundary = 1
     vp, vs, rho = cases.layered(p, seed=5)
@@ -108,8 +108,8 @@ def test_gpu_cpml_region_split_and_autotune_are_exact(erm):
     out = []
     for split_off, tune, streams, xcd, tx, ty in ((1, False, 0, 0, 0, 0), (0, False, 0, 0, 0, 0), (0, True, 0, 0, 0, 0),
                                                   (0, False, 1, 0, 0, 0), (0, True, 1, 0, 0, 0), (0, False, 0, 1, 0, 0),
-                                                  (0, False, 0, 2, 0, 0), (0, True, 0, 2, 0, 0), (0, False, 0, 2, 32, 8),
-                                                  (0, False, 0, 2, 32, 16), (0, True, 0, 2, 32, 0)):
+                                                  (0, False, 0, 2, 0, 0), (0, True, 0, 2, 0, 0), (0, False, 0, 2, 32, 16),
+                                                  (0, False, 0, 2, 64, 4), (0, False, 0, 0, 64, 8), (1, False, 0, 0, 64, 4)):
         p.cpml_split = split_off
         p.region_streams = streams
         p.xcd_map = xcd
@@ -122,7 +122,7 @@ def test_gpu_cpml_region_split_and_autotune_are_exact(erm):
             rec = g.forward_real()
             if tune:
                 t = g.autotune(steps=2)
-                assert t["tile_y"] in ((8, 16) if tx == 32 else (4, 8))
+                assert (t["tile_x"], t["tile_y"]) in ((64, 4), (64, 8), (32, 8), (32, 16))
                 assert t["zchunk"] in (4, 5, 6, 7, 8, 12, 16, 32)
             g.forward_smoothed()
             g.backward_smoothed()

@@ -46,7 +46,7 @@ PARITY_CASES = {
     # name: (params kwargs, steps)
     "r4_ragged": (dict(nx=70, ny=37, nz=29, b=8, R=4, lt=120), 120),
     "r6_reference_coeffs": (dict(nx=50, ny=44, nz=40, b=10, R=6, lt=100), 100),
-    "r4_many_chunks_ty4": (dict(nx=45, ny=30, nz=52, b=6, R=4, lt=90, zchunk=5, tile_y=4), 90),
+    "r4_many_chunks_ty4": (dict(nx=45, ny=30, nz=52, b=6, R=4, lt=90, zchunk=5, tile_x=64, tile_y=4), 90),
     "r6_src_near_corner": (dict(nx=33, ny=29, nz=27, b=4, R=6, lt=80, src=(3, 4, 3)), 80),
 }
 
@@ -94,12 +94,12 @@ def test_gpu_wavefields_match_fp64_oracle(erm, oracle, name):
 
 def test_gpu_full_config_grid_parity(erm, oracle):
     """The 256x256x128 model + 20-cell C-PML grid (296x296x168 padded) of the
-    benchmark, with its launch shape (64x4 tiles, z sweep 6, XCD y-band order,
+    benchmark, with its launch shape (32x8 tiles, z sweep 6, XCD y-band order,
     region-split plain / C-PML tile lists) and random fields in every cell:
     three steps on the GPU against the FP64 oracle, so every tile, z chunk,
     memory-variable slab and edge of the production launch is compared."""
     p = cases.params(nx=256, ny=256, nz=128, b=20, R=4, lt=3, dt=0.8, freq=15.0, src=(128, 128, 1),
-                     rec=(128, 128, 0, 0, 2, 2, 1), boundary=1, cpml_r=1e-3, tile_y=4, zchunk=6, xcd_map=2)
+                     rec=(128, 128, 0, 0, 2, 2, 1), boundary=1, cpml_r=1e-3, tile_x=32, tile_y=8, zchunk=6, xcd_map=2)
     vp, vs, rho = erm.make_model3d(1, 256, 256, 128, 10.0, 10.0, 10.0, 2)
     _say(f"full-grid parity: padded {tuple(p.padded_shape)}")
     rng = np.random.default_rng(5)

@@ -1,7 +1,7 @@
 """The benchmark's 3D shot (256x256x128 model + 20-cell C-PML, 296x296x168
 padded, radius 4, 128x128 receivers, the seeded dipping-layer model) with the
 time axis cut to 500 steps per pass, migrated through liberm.so with the
-benchmark's launch shape (64x4 tiles, z sweep 6, XCD y-band tile order,
+benchmark's launch shape (32x8 tiles, z sweep 6, XCD y-band tile order,
 region-split C-PML tile lists) and compared with the FP64 oracle's fixture
 (tests/golden/config2.py).
 
@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 def test_gpu_config2_shot_matches_fp64_fixture(erm):
     fx = np.load(config2.FIXTURE)
     print(f"config-2 shot at nt={int(fx['nt'])} vs the FP64 fixture", flush=True)
-    p, et, ea, imgs, vz = config2.gpu_shot(erm, int(fx["nt"]), tile_y=4, zchunk=6, xcd_map=2)
+    p, et, ea, imgs, vz = config2.gpu_shot(erm, int(fx["nt"]), tile_x=32, tile_y=8, zchunk=6, xcd_map=2)
     err = config2.compare(fx, p, et, ea, imgs, vz)
     print(err, flush=True)
     assert err["ex_time_mismatch_frac"] < 0.02, err
