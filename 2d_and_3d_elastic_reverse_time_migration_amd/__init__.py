@@ -147,6 +147,9 @@ def _load():
         "erm_batch_load_model": (ctypes.c_int, [vp, ctypes.c_int, fp, fp, fp]),
         "erm_batch_set_shots": (ctypes.c_int, [vp, vp, ctypes.c_int]),
         "erm_batch_set_launch": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "erm_batch_set_launch_shape": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+        "erm_batch_autotune_shape": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
+                                                    P(ctypes.c_int), P(ctypes.c_double)]),
         "erm_batch_autotune": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int),
                                               P(ctypes.c_double)]),
         "erm_batch_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
@@ -764,14 +767,15 @@ class Batch3D:
         self._c(lib.erm_batch_set_shots(self._h, ctypes.cast(arr, ctypes.c_void_p), len(windows)))
         self.windows = [tuple(map(int, w)) for w in windows]
 
-    def set_launch(self, tile_y, zchunk, grid_blocks=0):
-        self._c(lib.erm_batch_set_launch(self._h, tile_y, zchunk, grid_blocks))
+    def set_launch(self, tile_y, zchunk, grid_blocks=0, tile_x=64):
+        self._c(lib.erm_batch_set_launch_shape(self._h, tile_x, tile_y, zchunk, grid_blocks))
 
     def autotune(self, steps=4):
-        ty, kc, gr, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
-        self._c(lib.erm_batch_autotune(self._h, steps, ctypes.byref(ty), ctypes.byref(kc), ctypes.byref(gr),
-                                       ctypes.byref(ms)))
-        return {"tile_y": ty.value, "zchunk": kc.value, "grid_blocks": gr.value, "ms_per_step": ms.value}
+        tx, ty, kc, gr, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._c(lib.erm_batch_autotune_shape(self._h, steps, ctypes.byref(tx), ctypes.byref(ty), ctypes.byref(kc),
+                                             ctypes.byref(gr), ctypes.byref(ms)))
+        return {"tile_x": tx.value, "tile_y": ty.value, "zchunk": kc.value, "grid_blocks": gr.value,
+                "ms_per_step": ms.value}
 
     def use_graph(self, on=True):
         self._c(lib.erm_batch_use_graph(self._h, 1 if on else 0))

@@ -43,7 +43,7 @@ struct erm_batch {
     int* d_it = nullptr;
     float* d_stack = nullptr;
     // launch shape
-    int ty = 8, kchunk = 8, grid = 0;
+    int tx = 64, ty = 8, kchunk = 8, grid = 0;  // tiles 64 x {4, 8} or (radius 4) 32 x {8, 16}
     bool use_graph = true;
     bool tables_dirty = true;
     // timing
@@ -115,6 +115,7 @@ int build_tables(erm_batch* b) {
     double cells = 0;
     for (int s = 0; s < ns; ++s) {
         erm_ctx* c = b->shots[s];
+        c->tx = b->tx;
         c->ty = b->ty;
         c->kchunk = std::max(1, std::min(b->kchunk, c->nnz - 2 * c->R));
         if (!sweep_fits(c, c->kchunk))
@@ -141,7 +142,7 @@ int build_tables(erm_batch* b) {
         const int R = c->R;
         for (int l = 0; l < g.ntiles; ++l) {
             const int bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / (g.ntx * g.nty);
-            const int i0 = std::max(g.x0 + bx * 64, R), i1 = std::min(g.x0 + bx * 64 + 64, c->nnx - R) - 1;
+            const int i0 = std::max(g.x0 + bx * b->tx, R), i1 = std::min(g.x0 + bx * b->tx + b->tx, c->nnx - R) - 1;
             const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
             const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
             if (i0 > i1 || j0 > j1 || k0 > k1) continue;
@@ -197,45 +198,45 @@ inline int group_grid(const erm_batch* b, int m) {
     if (b->grid <= 0 || b->grid >= b->total_blocks) return b->total[m];
     return b->total[m] ? std::max(1, (int)((long)b->grid * b->total[m] / b->total_blocks)) : 0;
 }
-template <int R, int TY, int AX>
+template <int R, int TX, int TY, int AX>
 void stress_group(erm_batch* b, int which, int source_on, const Coef& cf, const float* wavelet, int wl) {
     const int gr = group_grid(b, AX);
     if (gr)
-        hipLaunchKernelGGL((k_stress_batch<R, 64, TY, AX>), dim3(gr), dim3(64, TY), 0, b->stream, b->d_steps[which],
+        hipLaunchKernelGGL((k_stress_batch<R, TX, TY, AX>), dim3(gr), dim3(TX, TY), 0, b->stream, b->d_steps[which],
                            b->d_table[AX], b->total[AX], cf, wavelet, wl, (const int*)b->d_it, source_on);
 }
-template <int R, int TY, int MODE, int AX>
+template <int R, int TX, int TY, int MODE, int AX>
 void velocity_group(erm_batch* b, int which, const Coef& cf) {
     const int gr = group_grid(b, AX);
     if (gr)
-        hipLaunchKernelGGL((k_velocity_batch<R, 64, TY, MODE, AX>), dim3(gr), dim3(64, TY), 0, b->stream,
+        hipLaunchKernelGGL((k_velocity_batch<R, TX, TY, MODE, AX>), dim3(gr), dim3(TX, TY), 0, b->stream,
                            b->d_steps[which], b->d_table[AX], b->total[AX], cf, (const int*)b->d_it);
 }
-template <int R, int TY, bool CPML>
+template <int R, int TX, int TY, bool CPML>
 void launch_stress(erm_batch* b, int which, int source_on, const Coef& cf, const float* wavelet, int wl) {
-    stress_group<R, TY, 0>(b, which, source_on, cf, wavelet, wl);
+    stress_group<R, TX, TY, 0>(b, which, source_on, cf, wavelet, wl);
     if constexpr (CPML) {
-        stress_group<R, TY, 7>(b, which, source_on, cf, wavelet, wl);
+        stress_group<R, TX, TY, 7>(b, which, source_on, cf, wavelet, wl);
     }
 }
-template <int R, int TY, int MODE, bool CPML>
+template <int R, int TX, int TY, int MODE, bool CPML>
 void launch_velocity(erm_batch* b, int which, const Coef& cf) {
-    velocity_group<R, TY, MODE, 0>(b, which, cf);
+    velocity_group<R, TX, TY, MODE, 0>(b, which, cf);
     if constexpr (CPML) {
-        velocity_group<R, TY, MODE, 7>(b, which, cf);
+        velocity_group<R, TX, TY, MODE, 7>(b, which, cf);
     }
 }
-template <int R, int TY, bool CPML>
+template <int R, int TX, int TY, bool CPML>
 void launch_step_rt(erm_batch* b, int pass) {
     erm_ctx* c0 = b->shots[0];
     const int which = pass == ERM_PASS_MODEL ? 0 : 1;
     if (pass == ERM_PASS_RECEIVER && b->total_rec > 0)
         hipLaunchKernelGGL(k_receivers_batch, dim3((b->total_rec + 255) / 256), dim3(256), 0, b->stream, b->d_rec,
                            b->d_shot_of, b->total_rec, (const int*)b->d_it, 1);
-    launch_stress<R, TY, CPML>(b, which, pass != ERM_PASS_RECEIVER, c0->coef, c0->wavelet, c0->wl);
-    if (pass == ERM_PASS_SOURCE) launch_velocity<R, TY, MODE_EXCITE, CPML>(b, which, c0->coef);
-    else if (pass == ERM_PASS_RECEIVER) launch_velocity<R, TY, MODE_IMAGE, CPML>(b, which, c0->coef);
-    else launch_velocity<R, TY, MODE_PLAIN, CPML>(b, which, c0->coef);
+    launch_stress<R, TX, TY, CPML>(b, which, pass != ERM_PASS_RECEIVER, c0->coef, c0->wavelet, c0->wl);
+    if (pass == ERM_PASS_SOURCE) launch_velocity<R, TX, TY, MODE_EXCITE, CPML>(b, which, c0->coef);
+    else if (pass == ERM_PASS_RECEIVER) launch_velocity<R, TX, TY, MODE_IMAGE, CPML>(b, which, c0->coef);
+    else launch_velocity<R, TX, TY, MODE_PLAIN, CPML>(b, which, c0->coef);
     if (pass == ERM_PASS_MODEL && b->total_rec > 0)
         hipLaunchKernelGGL(k_receivers_batch, dim3((b->total_rec + 255) / 256), dim3(256), 0, b->stream, b->d_rec,
                            b->d_shot_of, b->total_rec, (const int*)b->d_it, 0);
@@ -244,11 +245,14 @@ void launch_step_rt(erm_batch* b, int pass) {
 void launch_step(erm_batch* b, int pass) {
     const bool cpml = b->shots[0]->cpml;
     if (b->R == 4) {
-        if (b->ty == 4) { if (cpml) launch_step_rt<4, 4, true>(b, pass); else launch_step_rt<4, 4, false>(b, pass); }
-        else { if (cpml) launch_step_rt<4, 8, true>(b, pass); else launch_step_rt<4, 8, false>(b, pass); }
+        if (b->tx == 32) {
+            if (b->ty == 8) { if (cpml) launch_step_rt<4, 32, 8, true>(b, pass); else launch_step_rt<4, 32, 8, false>(b, pass); }
+            else { if (cpml) launch_step_rt<4, 32, 16, true>(b, pass); else launch_step_rt<4, 32, 16, false>(b, pass); }
+        } else if (b->ty == 4) { if (cpml) launch_step_rt<4, 64, 4, true>(b, pass); else launch_step_rt<4, 64, 4, false>(b, pass); }
+        else { if (cpml) launch_step_rt<4, 64, 8, true>(b, pass); else launch_step_rt<4, 64, 8, false>(b, pass); }
     } else {
-        if (b->ty == 4) { if (cpml) launch_step_rt<6, 4, true>(b, pass); else launch_step_rt<6, 4, false>(b, pass); }
-        else { if (cpml) launch_step_rt<6, 8, true>(b, pass); else launch_step_rt<6, 8, false>(b, pass); }
+        if (b->ty == 4) { if (cpml) launch_step_rt<6, 64, 4, true>(b, pass); else launch_step_rt<6, 64, 4, false>(b, pass); }
+        else { if (cpml) launch_step_rt<6, 64, 8, true>(b, pass); else launch_step_rt<6, 64, 8, false>(b, pass); }
     }
 }
 
@@ -328,7 +332,8 @@ int erm_batch_create(const erm_params* p, int device, erm_batch** out) {
         return ERM_E_DEVICE;
     }
     b->cus = prop.multiProcessorCount;
-    if (p->tile_y == 4) b->ty = 4;
+    b->tx = R == 4 && p->tile_x != 64 ? 32 : 64;  // as erm_create: 32 x 8 by default at radius 4
+    b->ty = b->tx == 64 ? (p->tile_y == 4 ? 4 : 8) : (p->tile_y == 16 ? 16 : 8);
     if (p->zchunk > 0) b->kchunk = p->zchunk;
     *out = b;
     return 0;
@@ -388,7 +393,7 @@ int erm_batch_set_shots(erm_batch* b, const erm_shot_window* shots, int nshots) 
         const int rix = std::max(1, P.receiver_interval_x), riy = std::max(1, P.receiver_interval_y);
         q.receiver_start_x = 0; q.receiver_interval_x = rix; q.receiver_num_x = (w.wx + rix - 1) / rix;
         q.receiver_start_y = 0; q.receiver_interval_y = riy; q.receiver_num_y = (w.wy + riy - 1) / riy;
-        q.tile_x = 64;  // the batched kernels run 64-wide tiles
+        q.tile_x = b->tx;
         q.tile_y = b->ty;
         q.zchunk = b->kchunk;
         q.xcd_map = 0;
@@ -405,14 +410,21 @@ int erm_batch_set_shots(erm_batch* b, const erm_shot_window* shots, int nshots) 
     return 0;
 }
 
-int erm_batch_set_launch(erm_batch* b, int tile_y, int zchunk, int grid_blocks) {
-    if (!b || (tile_y != 4 && tile_y != 8) || zchunk < 1 || grid_blocks < 0)
-        return b ? b->fail(ERM_E_PARAM, "tile_y must be 4 or 8, zchunk >= 1, grid_blocks >= 0") : ERM_E_PARAM;
+int erm_batch_set_launch_shape(erm_batch* b, int tile_x, int tile_y, int zchunk, int grid_blocks) {
+    if (!b) return ERM_E_PARAM;
+    const bool ok = (tile_x == 64 && (tile_y == 4 || tile_y == 8)) || (tile_x == 32 && b->R == 4 && (tile_y == 8 || tile_y == 16));
+    if (!ok || zchunk < 1 || grid_blocks < 0)
+        return b->fail(ERM_E_PARAM, "tile 64 x {4, 8} or (radius 4) 32 x {8, 16}, zchunk >= 1, grid_blocks >= 0");
+    b->tx = tile_x;
     b->ty = tile_y;
     b->kchunk = zchunk;
     b->grid = grid_blocks;
     b->tables_dirty = true;
     return 0;
+}
+
+int erm_batch_set_launch(erm_batch* b, int tile_y, int zchunk, int grid_blocks) {
+    return erm_batch_set_launch_shape(b, 64, tile_y, zchunk, grid_blocks);
 }
 
 int erm_batch_use_graph(erm_batch* b, int on) {
@@ -421,26 +433,29 @@ int erm_batch_use_graph(erm_batch* b, int on) {
     return 0;
 }
 
-int erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* grid_blocks, double* ms_per_step) {
+int erm_batch_autotune_shape(erm_batch* b, int steps, int* tile_x, int* tile_y, int* zchunk, int* grid_blocks,
+                             double* ms_per_step) {
     if (!b || steps <= 0) return b ? b->fail(ERM_E_PARAM, "steps must be positive") : ERM_E_PARAM;
     if (b->shots.empty()) return b->fail(ERM_E_STATE, "no shots");
     if (!b->have_model[1]) return b->fail(ERM_E_STATE, "smoothed model not loaded");
     BHC(b, hipSetDevice(b->device));
-    const int ty0 = b->ty, kc0 = b->kchunk, gr0 = b->grid;
+    const int tx0 = b->tx, ty0 = b->ty, kc0 = b->kchunk, gr0 = b->grid;
     double best = 1e300;
-    int bt = ty0, bk = kc0, bg = gr0;
-    const int tys[2] = {4, 8};
+    int bx = tx0, bt = ty0, bk = kc0, bg = gr0;
+    std::vector<std::pair<int, int>> shapes = {{64, 4}, {64, 8}};
+    if (b->R == 4) { shapes.push_back({32, 8}); shapes.push_back({32, 16}); }
     const int kcs[4] = {4, 6, 8, 16};
-    for (int ty : tys)
+    for (const auto& sh : shapes)
         for (int kc : kcs) {
+            const int tx = sh.first, ty = sh.second;
             // resident workgroups: 16 waves per CU at 4 waves/SIMD
-            const int per_cu = std::max(1, 16 / ty);
+            const int per_cu = std::max(1, 16 * 64 / (tx * ty));
             const int grids[3] = {0, b->cus * per_cu, 2 * b->cus * per_cu};
             bool fits = true;
             for (erm_ctx* c : b->shots) fits = fits && sweep_fits(c, std::max(1, std::min(kc, c->nnz - 2 * c->R)));
             if (!fits) continue;
             for (int gr : grids) {
-                b->ty = ty; b->kchunk = kc; b->grid = gr;
+                b->tx = tx; b->ty = ty; b->kchunk = kc; b->grid = gr;
                 b->tables_dirty = true;
                 int rc;
                 if ((rc = zero_shots(b, true, true, false))) return rc;
@@ -448,10 +463,10 @@ int erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* g
                 if ((rc = run_steps(b, ERM_PASS_SOURCE, 0, 1, false, &ms))) return rc;       // warm-up
                 if ((rc = run_steps(b, ERM_PASS_SOURCE, 1, steps, false, &ms))) return rc;
                 const double per = ms / steps;
-                if (per < best) { best = per; bt = ty; bk = kc; bg = gr; }
+                if (per < best) { best = per; bx = tx; bt = ty; bk = kc; bg = gr; }
             }
         }
-    b->ty = bt; b->kchunk = bk; b->grid = bg;
+    b->tx = bx; b->ty = bt; b->kchunk = bk; b->grid = bg;
     b->tables_dirty = true;
     int rc;
     if (best == 1e300) {  // no candidate fits the 32-bit span: time the current shape
@@ -463,11 +478,16 @@ int erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* g
     }
     rc = zero_shots(b, true, true, false);
     if (rc) return rc;
+    if (tile_x) *tile_x = bx;
     if (tile_y) *tile_y = bt;
     if (zchunk) *zchunk = bk;
     if (grid_blocks) *grid_blocks = bg;
     if (ms_per_step) *ms_per_step = best;
     return 0;
+}
+
+int erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* grid_blocks, double* ms_per_step) {
+    return erm_batch_autotune_shape(b, steps, nullptr, tile_y, zchunk, grid_blocks, ms_per_step);
 }
 
 int erm_batch_set_record(erm_batch* b, int shot, const float* rec) {

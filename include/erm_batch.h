@@ -48,13 +48,19 @@ int   erm_batch_destroy(erm_batch* b);
 int   erm_batch_load_model(erm_batch* b, int which, const float* vp, const float* vs, const float* rho);
 /* (Re)define the shots; windows must lie inside the model and contain their source. */
 int   erm_batch_set_shots(erm_batch* b, const erm_shot_window* shots, int nshots);
-/* Launch shape: tile_y 4 or 8, zchunk >= 1, grid_blocks 0 = one workgroup per
+/* Launch shape: tiles 64 x {4, 8} or, at radius 4, 32 x {8, 16} (the default
+ * 32 x 8, as erm_create), zchunk >= 1, grid_blocks 0 = one workgroup per
  * table entry, > 0 = persistent grid of that many workgroups.  A zchunk whose
  * sweep (zchunk + 2R planes of a window) spans 4 GiB or more makes the next
- * pass fail with ERM_E_PARAM (32-bit kernel offsets). */
+ * pass fail with ERM_E_PARAM (32-bit kernel offsets).  erm_batch_set_launch
+ * sets a 64-wide tile. */
+int   erm_batch_set_launch_shape(erm_batch* b, int tile_x, int tile_y, int zchunk, int grid_blocks);
 int   erm_batch_set_launch(erm_batch* b, int tile_y, int zchunk, int grid_blocks);
 /* Measure candidate launch shapes on `steps` source-pass steps of the current
- * shots and keep the fastest; returns it (any pointer may be NULL). */
+ * shots and keep the fastest; returns it (any pointer may be NULL;
+ * erm_batch_autotune does not report the tile width). */
+int   erm_batch_autotune_shape(erm_batch* b, int steps, int* tile_x, int* tile_y, int* zchunk, int* grid_blocks,
+                               double* ms_per_step);
 int   erm_batch_autotune(erm_batch* b, int steps, int* tile_y, int* zchunk, int* grid_blocks, double* ms_per_step);
 /* 1 (default): replay a captured hipGraph per step; 0: plain launches. */
 int   erm_batch_use_graph(erm_batch* b, int on);

@@ -105,12 +105,16 @@ def test_graph_persistent_and_autotune_identical(erm):
         b.use_graph(True)
         b.set_launch(8, 4, 64)  # persistent grid: 64 workgroups stride over the table
         g1 = b.run()
+        b.set_launch(16, 6, 0, tile_x=32)  # 32-wide tiles
+        g3 = b.run()
         best = b.autotune(steps=2)
         g2 = b.run()
-    assert best["tile_y"] in (4, 8) and best["zchunk"] in (4, 6, 8, 16) and best["ms_per_step"] > 0
+    assert (best["tile_x"], best["tile_y"]) in ((64, 4), (64, 8), (32, 8), (32, 16))
+    assert best["zchunk"] in (4, 6, 8, 16) and best["ms_per_step"] > 0
     for q in range(3):
         np.testing.assert_array_equal(g1[q], ref[q])
         np.testing.assert_array_equal(g2[q], ref[q])
+        np.testing.assert_array_equal(g3[q], ref[q])
 
 
 def test_batch_errors(erm):

@@ -3,10 +3,12 @@ layout over the 600x600x250 model, windows of 192..320 cells laterally and
 full depth, 20-cell C-PML, radius 4 (bench.py --workload batch), with the time
 axis cut to 200 steps per pass.
 
-* every window's records, excitation times and images from the batched run
-  are bit-identical to the same window migrated alone through the one-shot
-  path (each window is its own context; same per-cell arithmetic);
-* the stacked images equal the sum of the window images at their offsets;
+* the records, excitation times and images of six windows of the batched
+  run (the smallest, the largest and four others) are bit-identical to the
+  same window migrated alone through the one-shot path (each window is its
+  own context with the same per-cell arithmetic; tests/test_gpu_batch.py
+  shows it for every window and launch shape on smaller layouts);
+* the stacked images equal the sum of all 24 window images at their offsets;
 * the smallest window (shot 7) matches the FP64 oracle's fixture
   (tests/golden/config3.py) as the single-shot benchmark does
   (tests/test_gpu_config2.py): images rel L2 <= 1e-4 over excited cells
@@ -27,7 +29,7 @@ import config3  # noqa: E402
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.timeout(900)
+@pytest.mark.timeout(600)
 def test_gpu_config3_batch_layout(erm):
     p, true, mig, wins = config3.batch_inputs(erm)
     print(f"config-3 batch: {len(wins)} windows over {p.nx}x{p.ny}x{p.nz}, nt={p.lt}", flush=True)
@@ -37,11 +39,19 @@ def test_gpu_config3_batch_layout(erm):
         b.load_model(0, *true)
         b.load_model(1, *mig)
         b.set_shots(wins)
-        b.set_launch(4, 6, 0)
+        b.set_launch(8, 6, 0, tile_x=32)
         stacked = b.run()
+        largest = max(range(len(wins)), key=lambda s: wins[s][5] * wins[s][6])
+        alone = {config3.WINDOW, largest, 0, 8, 15, 23}
         for s, w in enumerate(wins):
             v = b.shot(s)
-            imgs_b, rec_b, et_b = v.get_images(), v.get_record(), v.get_field("ex_time")
+            imgs_b = v.get_images()
+            sx, sy, sz, x0, y0, wx, wy = w
+            for k in range(3):
+                expect[k][x0:x0 + wx, y0:y0 + wy, :] += imgs_b[k]
+            if s not in alone:
+                continue
+            rec_b, et_b = v.get_record(), v.get_field("ex_time")
             q = config3.window_params(erm, p, w)
             if s == config3.WINDOW:
                 err = config2.compare(fx, q, config2.model_region(et_b, q),
@@ -64,10 +74,8 @@ def test_gpu_config3_batch_layout(erm):
                 et = c.get_field("ex_time")
             np.testing.assert_array_equal(rec_b, rec)
             np.testing.assert_array_equal(et_b, et)
-            sx, sy, sz, x0, y0, wx, wy = w
             for k in range(3):
                 np.testing.assert_array_equal(imgs_b[k], imgs[k])
-                expect[k][x0:x0 + wx, y0:y0 + wy, :] += imgs[k]
             print(f"window {s} {w}: bit-identical to its one-shot migration", flush=True)
     for k in range(3):
         np.testing.assert_allclose(stacked[k], expect[k], rtol=1e-6, atol=1e-6 * np.abs(expect[k]).max())
