@@ -67,6 +67,9 @@ struct Geo {
 // and its tiles in flight are spatial neighbours (xcd_map 1; the default
 // y-band order of the region-split lists is applied to the tile lists on the
 // host instead).
+// tile-list entries: tile id in bits 0..27, the C-PML axis mask in 28..30
+constexpr int kTileIdMask = (1 << 28) - 1;
+
 struct TileId {
     int bx, by, bz;
     bool valid;
@@ -256,7 +259,7 @@ template <int R, int TX, int TY, int AX>
 __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restrict__ Mm,
                                             const float* __restrict__ MUm, const Geo& g, const Coef& cf,
                                             int src_k, int src_col, const float* __restrict__ wavelet,
-                                            int wl, int it, const Pml& P, int blk) {
+                                            int wl, int it, const Pml& P, int blk, int tm) {
     constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
     constexpr int NYH = (2 * R + TY - 1) / TY;
     __shared__ float sx[LY][LX];
@@ -292,8 +295,8 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
     }
     const bool is_src = active && col == src_col;
     // C-PML: x/y slab positions are fixed per column
-    const int sxi = (AX & 1) && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
-    const int syi = (AX & 2) && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
+    const int sxi = (AX & 1) && (tm & 1) && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
+    const int syi = (AX & 2) && (tm & 2) && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
 
     float c[R + 1];
 #pragma unroll
@@ -368,27 +371,33 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
         // no branch splits the loads; such lanes use zero memory variables,
         // which the zero coefficients there keep at zero, and store nothing)
         if constexpr (AX != 0) {
-            szi = (AX & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
+            szi = (AX & 4) && (tm & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
             if constexpr ((AX & 1) != 0) {
-                pqx = ((unsigned)k * g.nny + jc) * P.w[0] + (unsigned)max(sxi, 0);
-                bldn<3>(rpsi, 12u * pqx, 4u * psi_stress(P, 0), ps);
+                if (tm & 1) {
+                    pqx = ((unsigned)k * g.nny + jc) * P.w[0] + (unsigned)max(sxi, 0);
+                    bldn<3>(rpsi, 12u * pqx, 4u * psi_stress(P, 0), ps);
+                }
             }
             if constexpr ((AX & 2) != 0) {
-                pqy = ((unsigned)k * P.w[1] + (unsigned)max(syi, 0)) * g.nnx + ic;
-                bldn<3>(rpsi, 12u * pqy, 4u * psi_stress(P, 1), ps + 3);
+                if (tm & 2) {
+                    pqy = ((unsigned)k * P.w[1] + (unsigned)max(syi, 0)) * g.nnx + ic;
+                    bldn<3>(rpsi, 12u * pqy, 4u * psi_stress(P, 1), ps + 3);
+                }
             }
             if constexpr ((AX & 4) != 0) {
-                pqz = ((unsigned)max(szi, 0) * g.nny + jc) * g.nnx + ic;
-                bldn<3>(rpsi, 12u * pqz, 4u * psi_stress(P, 2), ps + 6);
+                if (tm & 4) {
+                    pqz = ((unsigned)max(szi, 0) * g.nny + jc) * g.nnx + ic;
+                    bldn<3>(rpsi, 12u * pqz, 4u * psi_stress(P, 2), ps + 6);
+                }
             }
         }
         [[maybe_unused]] float4 zq{};
         if constexpr ((AX & 4) != 0) {
-            if (tx == 0 && ty == 0) zq = reinterpret_cast<const float4*>(P.psi + psi_total(P) + 4u * (g.nnx + g.nny))[k];
+            if ((tm & 4) && tx == 0 && ty == 0) zq = reinterpret_cast<const float4*>(P.psi + psi_total(P) + 4u * (g.nnx + g.nny))[k];
         }
         __syncthreads();  // previous plane's LDS reads are done
         if constexpr ((AX & 4) != 0) {
-            if (tx == 0 && ty == 0) pc_z[0] = zq;
+            if ((tm & 4) && tx == 0 && ty == 0) pc_z[0] = zq;
         }
         sx[ty + R][tx + R] = qx[R];
         sy[ty + R][tx + R] = qy[R];
@@ -429,21 +438,21 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
             float e_yx = g.cy * dyvx, e_xy = g.cx * dxvy, e_zx = g.cz * dzvx, e_xz = g.cx * dxvz;
             float e_zy = g.cz * dzvy, e_yz = g.cy * dyvz;
             if constexpr (AX != 0) {
-                if constexpr ((AX & 1) != 0) {
+                if constexpr ((AX & 1) != 0) if (tm & 1) {
                     const float4 cq = pc_x[tx];  // (a_full, b_full, a_half, b_half) of x
                     for (int q = 0; q < 3; ++q) ps[q] = sxi >= 0 ? ps[q] : 0.f;
                     exx = pml_upd(cq.x, cq.y, exx, ps[0]);
                     e_xy = pml_upd(cq.z, cq.w, e_xy, ps[1]);
                     e_xz = pml_upd(cq.z, cq.w, e_xz, ps[2]);
                 }
-                if constexpr ((AX & 2) != 0) {
+                if constexpr ((AX & 2) != 0) if (tm & 2) {
                     const float4 cq = pc_y[ty];
                     for (int q = 3; q < 6; ++q) ps[q] = syi >= 0 ? ps[q] : 0.f;
                     eyy = pml_upd(cq.x, cq.y, eyy, ps[3]);
                     e_yx = pml_upd(cq.z, cq.w, e_yx, ps[4]);
                     e_yz = pml_upd(cq.z, cq.w, e_yz, ps[5]);
                 }
-                if constexpr ((AX & 4) != 0) {
+                if constexpr ((AX & 4) != 0) if (tm & 4) {
                     const float4 cq = pc_z[0];
                     for (int q = 6; q < 9; ++q) ps[q] = szi >= 0 ? ps[q] : 0.f;
                     ezz = pml_upd(cq.x, cq.y, ezz, ps[6]);
@@ -482,9 +491,11 @@ void k_stress(Wave w, const float* __restrict__ Mm,
                                                    const float* __restrict__ MUm, Geo g, Coef cf,
                                                    int src_k, int src_col, const float* __restrict__ wavelet,
                                                    int wl, int it, Pml P, const int* __restrict__ blist) {
-    // blist (optional): tile ids of this launch (region-split launches); else one tile per workgroup
-    const int blk = blist ? blist[blockIdx.x] : (int)blockIdx.x;
-    stress_tile<R, TX, TY, AX>(w, Mm, MUm, g, cf, src_k, src_col, wavelet, wl, it, P, blk);
+    // blist (optional): tile ids of this launch (region-split launches; bits
+    // 28..30 the C-PML axes whose slabs the tile touches); else one tile per workgroup
+    const int e = blist ? blist[blockIdx.x] : (int)blockIdx.x;
+    const int blk = blist ? (e & kTileIdMask) : e, tm = blist ? (e >> 28) & 7 : 7;
+    stress_tile<R, TX, TY, AX>(w, Mm, MUm, g, cf, src_k, src_col, wavelet, wl, it, P, blk, tm);
 }
 
 // ---------------------------------------------------------------------------
@@ -503,7 +514,7 @@ void k_stress(Wave w, const float* __restrict__ Mm,
 // ---------------------------------------------------------------------------
 template <int R, int TX, int TY, int MODE, int AX>
 __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __restrict__ Bm, const Geo& g,
-                                              const Coef& cf, const Excite& ex, int it, const Pml& P, int blk) {
+                                              const Coef& cf, const Excite& ex, int it, const Pml& P, int blk, int tm) {
     constexpr int LX = TX + 2 * R, LY = TY + 2 * R;
     constexpr int NYH = (2 * R + TY - 1) / TY;
     __shared__ float s_xx[TY][LX];   // x-halo only
@@ -549,8 +560,8 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
         if (ty == 0) pc_x[tx] = cf4[ic];
         if (ty == 1 && tx < TY) pc_y[tx] = cf4[g.nnx + clampi(j0 + tx, 0, g.nny - 1)];
     }
-    const int sxi = (AX & 1) && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
-    const int syi = (AX & 2) && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
+    const int sxi = (AX & 1) && (tm & 1) && active ? slab_of(i, g.nnx, P.lo[0], P.hi[0]) : -1;
+    const int syi = (AX & 2) && (tm & 2) && active ? slab_of(j, g.nny, P.lo[1], P.hi[1]) : -1;
 
     const long base = (long)(kb - R) * g.pxy;
     const long span = ((long)g.nnz * g.pxy - base) * 4;
@@ -615,27 +626,33 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
         unsigned pqx = 0, pqy = 0, pqz = 0;  // slab offsets (< 2^32 floats)
         int szi = -1;
         if constexpr (AX != 0) {  // unconditional loads at clamped slab cells, as in the stress tile
-            szi = (AX & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
+            szi = (AX & 4) && (tm & 4) && active ? slab_of(k, g.nnz, P.lo[2], P.hi[2]) : -1;
             if constexpr ((AX & 1) != 0) {
-                pqx = ((unsigned)k * g.nny + jc) * P.w[0] + (unsigned)max(sxi, 0);
-                bldn<4>(rpsi, 16u * pqx, 4u * psi_velocity(P, 0), ps);
+                if (tm & 1) {
+                    pqx = ((unsigned)k * g.nny + jc) * P.w[0] + (unsigned)max(sxi, 0);
+                    bldn<4>(rpsi, 16u * pqx, 4u * psi_velocity(P, 0), ps);
+                }
             }
             if constexpr ((AX & 2) != 0) {
-                pqy = ((unsigned)k * P.w[1] + (unsigned)max(syi, 0)) * g.nnx + ic;
-                bldn<4>(rpsi, 16u * pqy, 4u * psi_velocity(P, 1), ps + 4);
+                if (tm & 2) {
+                    pqy = ((unsigned)k * P.w[1] + (unsigned)max(syi, 0)) * g.nnx + ic;
+                    bldn<4>(rpsi, 16u * pqy, 4u * psi_velocity(P, 1), ps + 4);
+                }
             }
             if constexpr ((AX & 4) != 0) {
-                pqz = ((unsigned)max(szi, 0) * g.nny + jc) * g.nnx + ic;
-                bldn<4>(rpsi, 16u * pqz, 4u * psi_velocity(P, 2), ps + 8);
+                if (tm & 4) {
+                    pqz = ((unsigned)max(szi, 0) * g.nny + jc) * g.nnx + ic;
+                    bldn<4>(rpsi, 16u * pqz, 4u * psi_velocity(P, 2), ps + 8);
+                }
             }
         }
         [[maybe_unused]] float4 zq{};
         if constexpr ((AX & 4) != 0) {
-            if (tx == 0 && ty == 0) zq = reinterpret_cast<const float4*>(P.psi + psi_total(P) + 4u * (g.nnx + g.nny))[k];
+            if ((tm & 4) && tx == 0 && ty == 0) zq = reinterpret_cast<const float4*>(P.psi + psi_total(P) + 4u * (g.nnx + g.nny))[k];
         }
         __syncthreads();
         if constexpr ((AX & 4) != 0) {
-            if (tx == 0 && ty == 0) pc_z[0] = zq;
+            if ((tm & 4) && tx == 0 && ty == 0) pc_z[0] = zq;
         }
         s_xx[ty][tx + R] = cxx;
         s_yy[ty + R][tx] = cyy;
@@ -678,7 +695,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
             float gxz = g.cx * d1, gyz = g.cy * d2, gzz = g.cz * d3;
             float px = g.cx * p1, py = g.cy * p2, pz = g.cz * p3;
             if constexpr (AX != 0) {
-                if constexpr ((AX & 1) != 0) {
+                if constexpr ((AX & 1) != 0) if (tm & 1) {
                     const float4 cq = pc_x[tx];  // (a_full, b_full, a_half, b_half) of x
                     for (int q = 0; q < 4; ++q) ps[q] = sxi >= 0 ? ps[q] : 0.f;
                     gxx = pml_upd(cq.z, cq.w, gxx, ps[0]);
@@ -686,7 +703,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
                     gxz = pml_upd(cq.x, cq.y, gxz, ps[2]);
                     px = pml_upd(cq.z, cq.w, px, ps[3]);
                 }
-                if constexpr ((AX & 2) != 0) {
+                if constexpr ((AX & 2) != 0) if (tm & 2) {
                     const float4 cq = pc_y[ty];
                     for (int q = 4; q < 8; ++q) ps[q] = syi >= 0 ? ps[q] : 0.f;
                     gyx = pml_upd(cq.x, cq.y, gyx, ps[4]);
@@ -694,7 +711,7 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
                     gyz = pml_upd(cq.x, cq.y, gyz, ps[6]);
                     py = pml_upd(cq.z, cq.w, py, ps[7]);
                 }
-                if constexpr ((AX & 4) != 0) {
+                if constexpr ((AX & 4) != 0) if (tm & 4) {
                     const float4 cq = pc_z[0];
                     for (int q = 8; q < 12; ++q) ps[q] = szi >= 0 ? ps[q] : 0.f;
                     gzx = pml_upd(cq.x, cq.y, gzx, ps[8]);
@@ -753,8 +770,9 @@ __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(R != 4 |
 void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
                                                      Coef cf, Excite ex, int it, Pml P,
                                                      const int* __restrict__ blist) {
-    const int blk = blist ? blist[blockIdx.x] : (int)blockIdx.x;
-    velocity_tile<R, TX, TY, MODE, AX>(w, Bm, g, cf, ex, it, P, blk);
+    const int e = blist ? blist[blockIdx.x] : (int)blockIdx.x;
+    const int blk = blist ? (e & kTileIdMask) : e, tm = blist ? (e >> 28) & 7 : 7;
+    velocity_tile<R, TX, TY, MODE, AX>(w, Bm, g, cf, ex, it, P, blk, tm);
 }
 
 // ---------------------------------------------------------------------------
@@ -786,7 +804,7 @@ void k_stress_batch(const ShotStep* __restrict__ shots,
         const int2 t = table[b];
         const ShotStep& s = shots[t.x];
         stress_tile<R, TX, TY, AX>(s.w, s.M, s.MU, s.g, cf, source_on ? s.src_k : -1, source_on ? s.src_col : -1,
-                                     wavelet, wl, it, s.P, t.y);
+                                     wavelet, wl, it, s.P, t.y, 7);
     }
 }
 
@@ -799,7 +817,7 @@ void k_velocity_batch(const ShotStep* __restrict__ shots,
     for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
         const int2 t = table[b];
         const ShotStep& s = shots[t.x];
-        velocity_tile<R, TX, TY, MODE, AX>(s.w, s.B, s.g, cf, s.ex, it, s.P, t.y);
+        velocity_tile<R, TX, TY, MODE, AX>(s.w, s.B, s.g, cf, s.ex, it, s.P, t.y, 7);
     }
 }
 

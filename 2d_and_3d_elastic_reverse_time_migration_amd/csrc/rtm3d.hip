@@ -163,6 +163,8 @@ int build_split(erm_ctx* c) {
     Geo g = make_geo(c);
     const int R = c->R;
     std::vector<int> grp[8];
+    std::vector<int> axes(g.ntiles, 0);  // per tile: the axes whose slabs it touches
+    if (g.ntiles > kTileIdMask) return c->fail(ERM_E_SIZE, "too many tiles for the tile lists");
     auto inside = [](int lo_c, int hi_c, int n, int lo, int hi) {  // [lo_c, hi_c] avoids both slabs
         return lo_c > lo && hi_c < n - hi - 1;
     };
@@ -175,6 +177,7 @@ int build_split(erm_ctx* c) {
         int m = (inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) ? 0 : 1) |
                 (inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) ? 0 : 2) |
                 (inside(k0, k1, c->nnz, c->slab_lo[2], c->slab_hi[2]) ? 0 : 4);
+        axes[l] = m;
         if (m != 0) m = 7;
         grp[m].push_back(l);
     }
@@ -184,6 +187,9 @@ int build_split(erm_ctx* c) {
         c->tiles[m] = nullptr;
         if (!grp[m].empty() && c->xcd_map == 1) grp[m] = xcd_order(grp[m]);
         if (!grp[m].empty() && c->xcd_map == 2) grp[m] = xcd_band_order(grp[m], g, g.ntiles);
+        // C-PML tiles carry their axis mask (bits 28..30) so the kernel skips the
+        // memory variables of the axes whose slabs the tile does not touch
+        for (int& t : grp[m]) if (t < g.ntiles) t |= axes[t] << 28;
         c->ntl[m] = (int)grp[m].size();
         if (grp[m].empty()) continue;
         if (dmalloc(c, &c->tiles[m], grp[m].size())) return ERM_E_NOMEM;
