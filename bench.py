@@ -525,6 +525,8 @@ def main():
     ap.add_argument("--imaging", choices=["excitation", "xcorr"], default="excitation",
                     help="shot workload: excitation-amplitude imaging (default), or zero-lag cross-correlation "
                          "with the source wavefield reconstructed backward from boundary strips")
+    ap.add_argument("--cpml-split", type=int, choices=[0, 1], default=0,
+                    help="C-PML launches: 0 plain + C-PML tile lists (default), 1 one C-PML launch over all tiles")
     ap.add_argument("--region-streams", type=int, choices=[0, 1], default=0,
                     help="run the plain-tile and C-PML-tile launches of a step concurrently on two streams")
     ap.add_argument("--xcd-map", type=int, choices=[0, 1, 2], default=2,
@@ -567,6 +569,7 @@ def main():
     p.tile_x, p.tile_y, p.zchunk = args.tile_x, args.tile_y, args.zchunk
     p.xcd_map = args.xcd_map
     p.region_streams = args.region_streams
+    p.cpml_split = args.cpml_split
     vp, vs, rho = erm.make_model3d(1, p.nx, p.ny, p.nz, p.dx, p.dy, p.dz, 2)
     vp_s = smooth_box(vp)
     vs_s = (vp_s / 1.732).astype(np.float32)
