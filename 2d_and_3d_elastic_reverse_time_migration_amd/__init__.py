@@ -111,6 +111,8 @@ def _load():
                                                                           fp, fp, fp, fp]),
         "erm2d_born": (ctypes.c_int, [vp, fp, fp]),
         "erm2d_adjoint": (ctypes.c_int, [vp, fp, fp]),
+        "erm2d_adjoint_ref": (ctypes.c_int, [vp, fp, fp]),
+        "erm2d_rtm_xcorr": (ctypes.c_int, [vp, fp, ctypes.c_int, fp]),
         "erm2d_last_ms": (ctypes.c_int, [vp, P(ctypes.c_double)]),
         "erm2d_lsrtm_cg": (ctypes.c_int, [vp, ctypes.c_int, ip, fp, ctypes.c_int, ctypes.c_uint, fp, fp,
                                           P(ctypes.c_double), vp, vp, ctypes.c_float]),
@@ -492,6 +494,24 @@ class Elastic2D:
         p = self.p
         g = np.empty((3, p.nx, p.nz), np.float32)
         self._c(lib.erm2d_adjoint(self._h, _fptr(np.ascontiguousarray(r, np.float32)), _fptr(g)))
+        return g
+
+    def rtm_xcorr(self, rec, strategy=0):
+        """Cross-correlation RTM (erm2d_rtm_xcorr) with source strategy 0
+        (history), 1 (reversed propagation) or 2 (boundary strips); images
+        [5][nx][nz]: PP, PS, PPd, PSc, PSc polarity-corrected."""
+        p = self.p
+        out = np.empty((5, p.nx, p.nz), np.float32)
+        r = None if rec is None else np.ascontiguousarray(rec, np.float32)
+        self._c(lib.erm2d_rtm_xcorr(self._h, _fptr(r), strategy, _fptr(out)))
+        return out
+
+    def adjoint_ref(self, r):
+        """The reference's continuous-adjoint equations and gradient
+        (erm2d_adjoint_ref): (g_lambda, g_mu, g_rho) [3][nx][nz]."""
+        p = self.p
+        g = np.empty((3, p.nx, p.nz), np.float32)
+        self._c(lib.erm2d_adjoint_ref(self._h, _fptr(np.ascontiguousarray(r, np.float32)), _fptr(g)))
         return g
 
     def set_workers(self, n):

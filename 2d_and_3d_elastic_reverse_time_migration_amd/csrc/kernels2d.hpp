@@ -25,6 +25,7 @@ struct G2 {
     const float* att;    // damping d per cell
     int bl, bu, nx, nz;  // model region
     float qsign;         // sign of the memory-variable term in the stress update (-1: Q-compensated)
+    float dsgn;          // sign of the derivative terms (-1: reversed propagation, rfwd_..._old)
     // C-PML (no reference counterpart; null: damping boundary).  psi: 10
     // memory variables of ncell floats each -- stress side 0 Dx vx (x full),
     // 1 Dx vz (x half), 2 Dz vz (z full), 3 Dz vx (z half); velocity side
@@ -206,10 +207,15 @@ __global__ __launch_bounds__(TZ* TXR) void k2_stress(F2 f, M2 m, G2 g, Coef2 cf,
     // memory term dt * r; with qsign = -1 its sign is flipped (Q-compensated
     // propagation, rfwd_txxzzxzpp_viscoelastic_and_memory, viscoelastic_equation.cu:1541-1569)
     const float dq = g.dtr * g.qsign;
-    f.xx[n] = a * (b * xx + M * E0 + La * E1 + dq * (sp + sxx));
-    f.zz[n] = a * (b * zz + La * E0 + M * E1 + dq * (sp + szz));
-    f.tp[n] = a * (b * f.tp[n] + M * (E0 + E1) + dq * sp);
-    f.xz[n] = a * (b * f.xz[n] + U * E2 + dq * sxz);
+    // dsgn = -1 (with qsign = -1): the reversed step of
+    // rfwd_txxzzxzpp_viscoelastic_and_memory_old (:1407-1433); the memory
+    // variables above are driven by +E as in that kernel (an exact sign flip,
+    // so the forward step is unchanged bit for bit)
+    const float e0 = g.dsgn * E0, e1 = g.dsgn * E1, e2 = g.dsgn * E2;
+    f.xx[n] = a * (b * xx + M * e0 + La * e1 + dq * (sp + sxx));
+    f.zz[n] = a * (b * zz + La * e0 + M * e1 + dq * (sp + szz));
+    f.tp[n] = a * (b * f.tp[n] + M * (e0 + e1) + dq * sp);
+    f.xz[n] = a * (b * f.xz[n] + U * e2 + dq * sxz);
 }
 
 // ---------------------------------------------------------------------------
@@ -266,10 +272,10 @@ __global__ __launch_bounds__(TZ* TXR) void k2_velocity(F2 f, M2 m, G2 g, Coef2 c
     }
     const float h = g.att[n] * g.dtr * 0.5f;
     const float a = 1.0f / (1.0f + h), b = 1.0f - h, bo = m.B[n];
-    const float nvx = a * (b * f.vx[n] + bo * fx);
-    const float nvz = a * (b * f.vz[n] + bo * fz);
-    const float nxp = a * (b * f.vxp[n] + bo * px);
-    const float nzp = a * (b * f.vzp[n] + bo * pz);
+    const float nvx = a * (b * f.vx[n] + bo * (g.dsgn * fx));
+    const float nvz = a * (b * f.vz[n] + bo * (g.dsgn * fz));
+    const float nxp = a * (b * f.vxp[n] + bo * (g.dsgn * px));
+    const float nzp = a * (b * f.vzp[n] + bo * (g.dsgn * pz));
     f.vx[n] = nvx; f.vz[n] = nvz; f.vxp[n] = nxp; f.vzp[n] = nzp;
     if constexpr (MODE == 1) {
         const float amp = sqrtf(nxp * nxp + nzp * nzp);
@@ -512,6 +518,281 @@ __global__ __launch_bounds__(TZ* TXR) void k2a_s2(F2 l, G2 g, Coef2 cf, const fl
     const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h), b = 1.0f - h;
     l.vx[n] = a * b * l.vx[n] - g.cx * dmx<R>(w0, c, a_, b_) - g.cz * dmz<R>(w2, c, a_, b_);
     l.vz[n] = a * b * l.vz[n] - g.cz * dpz<R>(w1, c, a_, b_) - g.cx * dpx<R>(w2, c, a_, b_);
+}
+
+// ---------------------------------------------------------------------------
+// Reference-mode adjoint (adjoint_mode 1): the reference's continuous-adjoint
+// state equations, stepping backward in time with the forward kernels'
+// damping bracket a[b u + ...].  Adjoint state in an F2: vx, vz the adjoint
+// velocities, xx, zz, xz the strain-like adjoint stresses, r[l][1..3] the
+// memory variables (xx, zz, xz) of mechanism l (r[l][0] unused: the adjoint
+// r_p of the reference feeds neither the velocities nor the gradient).
+//   k2r_s: stresses += E(v) (the strain increments, no moduli: the kernel
+//          that updates the reference's adjoint stresses is not shipped; the
+//          positions of viscoelastic_equation.cu:137-176), then per mechanism
+//          r = a[(b - dt/tau) r + dt s] (adjoint_fwd_memory_new, :2190-2196).
+//   k2r_v: velocities (adjoint_fwd_vx/vz_viscoelastic, :1923-1971,
+//          :2071-2120) with the cell's moduli outside the derivatives,
+//          Mp = M/rho, Ms = U/rho, P_l = kP_l/rho, S_l = kS_l/rho:
+//            vx += Mp cx D-x sxx + (Mp-2Ms) cx D-x szz + Ms cz D-z sxz
+//                  + sum_l P_l cx D-x rxx + (P_l-2S_l) cx D-x rzz + S_l cz D-z rxz
+//            vz += Mp cz D+z szz + (Mp-2Ms) cz D+z sxx + Ms cx D+x sxz
+//                  + sum_l (P_l-2S_l) cz D+z rxx + P_l cz D+z rzz + S_l cx D+x rxz
+//          then, at model cells, the gradient of
+//          cal_gradient_in_viscoelastic_media_new (:2248-2295) from the
+//          background's strain increments E and forces F of the step:
+//            g_rho += vx vx_t + vz vz_t,  v_t = B F / dt
+//            g_lam -= [mp (sxx+szz) + sum_l np_l (rxx+rzz)] (vx_x + vz_z)
+//            g_mu  -= 2A + 2B + C + sum_l (2D_l + 2E_l + F_l)
+//          with vx_x = E0/dt, vz_z = E1/dt, vx_z + vz_x = E2/dt,
+//          A = (mp vx_x + (mp-ms) vz_z) sxx, B = (mp vz_z + (mp-ms) vx_x) szz,
+//          C = ms (vx_z+vz_x) sxz, D/E/F likewise with np_l, ns_l and r_l;
+//          mp = M/(rho vp^2), ms = U/(rho vs^2), np_l = kP_l/(rho vp^2),
+//          ns_l = kS_l/(rho vs^2) (ms = ns_l = 0 where vs = 0).
+// grad = [3][nx*nz] (g_lam, g_mu, g_rho), accumulated over the steps.
+// ---------------------------------------------------------------------------
+template <int R, int L, int TXR>
+__global__ __launch_bounds__(TZ* TXR) void k2r_s(F2 l, M2 m, G2 g, Coef2 cf) {
+    __shared__ float tvx[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float tvz[TXR + 2 * R][TZ + 2 * R];
+    if (const int z = blockIdx.z) l = shot_state(l, z * g.sst);
+    const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
+    load_tile<R, TXR>(tvx, l.vx, g, i0, k0);
+    load_tile<R, TXR>(tvz, l.vz, g, i0, k0);
+    __syncthreads();
+    const int k = k0 + threadIdx.x, i = i0 + threadIdx.y;
+    if (i < R || i >= g.nnx - R || k < R || k >= g.nnz - R) return;
+    float c[R + 1];
+#pragma unroll
+    for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
+    const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
+    const long n = (long)i * g.pz + k;
+    const float E0 = g.cx * dpx<R>(tvx, c, a_, b_);
+    const float E1 = g.cz * dmz<R>(tvz, c, a_, b_);
+    const float E2 = g.cx * dmx<R>(tvz, c, a_, b_) + g.cz * dpz<R>(tvx, c, a_, b_);
+    const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h), b = 1.0f - h;
+    const float sxx = a * (b * l.xx[n] + E0), szz = a * (b * l.zz[n] + E1), sxz = a * (b * l.xz[n] + E2);
+    l.xx[n] = sxx; l.zz[n] = szz; l.xz[n] = sxz;
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        const float bl = b - m.decay[s][n];
+        l.r[s][1][n] = a * (bl * l.r[s][1][n] + g.dtr * sxx);
+        l.r[s][2][n] = a * (bl * l.r[s][2][n] + g.dtr * szz);
+        l.r[s][3][n] = a * (bl * l.r[s][3][n] + g.dtr * sxz);
+    }
+}
+
+template <int R, int L, int TXR>
+__global__ __launch_bounds__(TZ* TXR) void k2r_v(F2 l, M2 m, G2 g, Coef2 cf, const float* __restrict__ Eh,
+                                                 float* __restrict__ grad) {
+    __shared__ float t0[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float t1[TXR + 2 * R][TZ + 2 * R];
+    __shared__ float t2[TXR + 2 * R][TZ + 2 * R];
+    if (const int z = blockIdx.z) { l = shot_state(l, z * g.sst); Eh += z * g.hst; grad += z * g.gst; }
+    const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * TXR;
+    const int k = k0 + threadIdx.x, i = i0 + threadIdx.y;
+    const bool upd = !(i < R || i >= g.nnx - R || k < R || k >= g.nnz - R);
+    const long n = (long)clamp2(i, 0, g.nnx - 1) * g.pz + clamp2(k, 0, g.nnz - 1);
+    float c[R + 1];
+#pragma unroll
+    for (int q = 1; q <= R; ++q) c[q] = cf.c[q];
+    const int a_ = threadIdx.y + R, b_ = threadIdx.x + R;
+    const float Bn = m.B[n], Mp = m.M[n] * Bn, Ms = m.U[n] * Bn;
+    load_tile<R, TXR>(t0, l.xx, g, i0, k0);
+    load_tile<R, TXR>(t1, l.zz, g, i0, k0);
+    load_tile<R, TXR>(t2, l.xz, g, i0, k0);
+    __syncthreads();
+    float fx = Mp * g.cx * dmx<R>(t0, c, a_, b_) + (Mp - 2.0f * Ms) * g.cx * dmx<R>(t1, c, a_, b_) +
+               Ms * g.cz * dmz<R>(t2, c, a_, b_);
+    float fz = Mp * g.cz * dpz<R>(t1, c, a_, b_) + (Mp - 2.0f * Ms) * g.cz * dpz<R>(t0, c, a_, b_) +
+               Ms * g.cx * dpx<R>(t2, c, a_, b_);
+    const float sxx = t0[a_][b_], szz = t1[a_][b_], sxz = t2[a_][b_];
+    float rsum_l = 0.f, rdef = 0.f;  // memory parts of the gradient (accumulated per mechanism)
+    long q = -1;
+    const bool inm = upd && in_model(g, i, k, &q);
+    const long mn = (long)g.nx * g.nz;
+    float vxx = 0.f, vzz = 0.f, vxz = 0.f;
+    const float idt = 1.0f / g.dtr;
+    if (inm) { vxx = Eh[q] * idt; vzz = Eh[mn + q] * idt; vxz = Eh[2 * mn + q] * idt; }
+    const float rho = 1.0f / Bn, vp = m.vp[n], vs = m.vs[n];
+    const float ip = 1.0f / (rho * vp * vp), is = vs > 0.f ? 1.0f / (rho * vs * vs) : 0.f;
+#pragma unroll
+    for (int s = 0; s < L; ++s) {
+        __syncthreads();  // the previous tiles are consumed
+        load_tile<R, TXR>(t0, l.r[s][1], g, i0, k0);
+        load_tile<R, TXR>(t1, l.r[s][2], g, i0, k0);
+        load_tile<R, TXR>(t2, l.r[s][3], g, i0, k0);
+        __syncthreads();
+        const float P = m.kP[s][n] * Bn, S = m.kS[s][n] * Bn;
+        fx += P * g.cx * dmx<R>(t0, c, a_, b_) + (P - 2.0f * S) * g.cx * dmx<R>(t1, c, a_, b_) +
+              S * g.cz * dmz<R>(t2, c, a_, b_);
+        fz += (P - 2.0f * S) * g.cz * dpz<R>(t0, c, a_, b_) + P * g.cz * dpz<R>(t1, c, a_, b_) +
+              S * g.cx * dpx<R>(t2, c, a_, b_);
+        const float np = m.kP[s][n] * ip, ns = m.kS[s][n] * is;
+        const float rxx = t0[a_][b_], rzz = t1[a_][b_], rxz = t2[a_][b_];
+        rsum_l += np * (rxx + rzz);
+        rdef += 2.0f * (np * vxx + (np - ns) * vzz) * rxx + 2.0f * (np * vzz + (np - ns) * vxx) * rzz + ns * vxz * rxz;
+    }
+    if (!upd) return;
+    const float h = g.att[n] * g.dtr * 0.5f, a = 1.0f / (1.0f + h), b = 1.0f - h;
+    const float nvx = a * (b * l.vx[n] + fx), nvz = a * (b * l.vz[n] + fz);
+    l.vx[n] = nvx; l.vz[n] = nvz;
+    if (inm) {
+        const float* Fh = Eh + 3 * mn;
+        const float vxt = Bn * Fh[q] * idt, vzt = Bn * Fh[mn + q] * idt;
+        const float mp = m.M[n] * ip, ms = m.U[n] * is;
+        const float A = (mp * vxx + (mp - ms) * vzz) * sxx, B = (mp * vzz + (mp - ms) * vxx) * szz;
+        const float C = ms * vxz * sxz;
+        grad[q] -= (mp * (sxx + szz) + rsum_l) * (vxx + vzz);
+        grad[mn + q] -= 2.0f * A + 2.0f * B + C + rdef;
+        grad[2 * mn + q] += nvx * vxt + nvz * vzt;
+    }
+}
+
+// Reference-mode gradient (g_lam, g_mu, g_rho) for absolute perturbations,
+// mapped to the inverted parameters (relative perturbations) by the
+// transpose of the reference's parameter map (cuda_cal_dem_parameter_
+// viscoelastic_media_new, viscoelastic_equation.cu:2431-2486), with the sign
+// of the density term reversed (with the reference's sign the density part
+// of <dm, g> is -<L dm, d>; the moduli parts are +):
+//   1: (lam g_lam, mu g_mu, -rho g_rho)
+//   2: (2 rho vp^2 g_lam, rho vs^2 (2 g_mu - 4 g_lam), rho (vp^2-2vs^2) g_lam + rho vs^2 g_mu - rho g_rho)
+//   3: as 2 with the third component -rho (vp^2-2vs^2) g_lam - rho vs^2 g_mu - rho g_rho.
+__global__ void k2r_grad_map(float* __restrict__ grad, M2 m, G2 g, int para) {
+    const long mn = (long)g.nx * g.nz;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < mn; q += (long)gridDim.x * blockDim.x) {
+        const long n = (long)((int)(q / g.nz) + g.bl) * g.pz + (int)(q % g.nz) + g.bu;
+        const float rho = 1.0f / m.B[n], vp = m.vp[n], vs = m.vs[n];
+        const float gl = grad[q], gm = grad[mn + q], gr = grad[2 * mn + q];
+        const float lam = rho * (vp * vp - 2.0f * vs * vs), mu = rho * vs * vs;
+        if (para == 1) {
+            grad[q] = lam * gl;
+            grad[mn + q] = mu * gm;
+            grad[2 * mn + q] = -rho * gr;
+        } else {
+            grad[q] = 2.0f * rho * vp * vp * gl;
+            grad[mn + q] = mu * (2.0f * gm - 4.0f * gl);
+            grad[2 * mn + q] = (para == 3 ? -1.0f : 1.0f) * (lam * gl + mu * gm) - rho * gr;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cross-correlation RTM (erm2d_rtm_xcorr).  Per receiver step, over interior
+// cells, with the source fields (sxp, szp, sdv: vector P and div v after the
+// matching source step) taken over the model region only (history layout
+// [mn] when hist, else the padded grid) and the receiver's v, vP, div, curl:
+//   img [5][mn]: PP += vP_s.vP_r, PS += vP_s.(v_r - vP_r), PPd += div_s div_r,
+//                PSc += div_s curl_r (model cells); PScp in k2x_polarity;
+//   vote: 1 / 2 / 3 for set > 0 / < 0 / = 0, set = -sgn(vzp_s) sgn(rvxs)
+//         (vP_s . rvs), the cases of set_sign_basedon_polarization_ps
+//         (elastic_2D_kernel_2.cu:941-982; the case vzp_s < 0, rvxs = 0
+//         left undefined there votes 3); prod = div_s curl_r.
+// ---------------------------------------------------------------------------
+__global__ void k2x_image(G2 g, const float* __restrict__ sxp, const float* __restrict__ szp,
+                          const float* __restrict__ sdv, int hist, F2 r, const float* __restrict__ rdv,
+                          const float* __restrict__ rcu, float* __restrict__ img, int* __restrict__ vote,
+                          float* __restrict__ prod, int R) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+    if (k < R || k >= g.nnz - R || i < R || i >= g.nnx - R) return;
+    const long n = (long)i * g.pz + k;
+    long q = 0;
+    const bool inm = in_model(g, i, k, &q);
+    float a = 0.f, b = 0.f, d = 0.f;
+    if (inm) {
+        const long s = hist ? q : n;
+        a = sxp[s]; b = szp[s]; d = sdv[s];
+    }
+    const float rxs = r.vx[n] - r.vxp[n], rzs = r.vz[n] - r.vzp[n];
+    const float dot = a * rxs + b * rzs;
+    float set = 0.f;
+    if (b >= 0.f && rxs >= 0.f) set = -dot;
+    else if (b >= 0.f && rxs < 0.f) set = dot;
+    else if (b < 0.f && rxs > 0.f) set = dot;
+    else if (b < 0.f && rxs < 0.f) set = -dot;
+    vote[n] = set > 0.f ? 1 : (set < 0.f ? 2 : 3);
+    const float pr = d * rcu[n];
+    prod[n] = pr;
+    if (inm) {
+        const long mn = (long)g.nx * g.nz;
+        img[q] += a * r.vxp[n] + b * r.vzp[n];
+        img[mn + q] += dot;
+        img[2 * mn + q] += d * rdv[n];
+        img[3 * mn + q] += pr;
+    }
+}
+// PS polarity: the votes of the 9x9 window (filter_sign_new_share,
+// elastic_2D_kernel_2.cu:875-919; interior cells only) decide each cell's sign
+// (compare_sign, :921-939: +1 if its count is not below either other, -1 or 0
+// if strictly above both others, otherwise the previous sign stays), then
+// PScp += sign * prod at model cells.  One workgroup: 64 cells along z x 4
+// rows, the votes of the tile and a 4-cell halo staged in LDS.
+constexpr int PW = 4;  // window half width
+__global__ __launch_bounds__(TZ * 4) void k2x_polarity(G2 g, const int* __restrict__ vote, int* __restrict__ sign,
+                                                       const float* __restrict__ prod, float* __restrict__ img, int R) {
+    __shared__ int tv[4 + 2 * PW][TZ + 2 * PW];
+    const int tz = threadIdx.x, tx = threadIdx.y;
+    const int k0 = blockIdx.x * TZ, i0 = blockIdx.y * 4;
+    for (int rr = tx; rr < 4 + 2 * PW; rr += 4)
+        for (int cc = tz; cc < TZ + 2 * PW; cc += TZ) {
+            const int ii = i0 - PW + rr, kk = k0 - PW + cc;
+            const bool in = ii >= R && ii < g.nnx - R && kk >= R && kk < g.nnz - R;
+            tv[rr][cc] = in ? vote[(long)ii * g.pz + kk] : 0;
+        }
+    __syncthreads();
+    const int k = k0 + tz, i = i0 + tx;
+    if (k < R || k >= g.nnz - R || i < R || i >= g.nnx - R) return;
+    int c1 = 0, c2 = 0, c3 = 0;
+    for (int di = 0; di <= 2 * PW; ++di)
+#pragma unroll
+        for (int dk = 0; dk <= 2 * PW; ++dk) {
+            const int v = tv[tx + di][tz + dk];
+            c1 += v == 1; c2 += v == 2; c3 += v == 3;
+        }
+    const long n = (long)i * g.pz + k;
+    int s = sign[n];
+    if (c1 >= c2 && c1 >= c3) s = 1;
+    else if (c2 > c1 && c2 > c3) s = -1;
+    else if (c3 > c1 && c3 > c2) s = 0;
+    sign[n] = s;
+    long q;
+    if (in_model(g, i, k, &q)) img[4 * (long)g.nx * g.nz + q] += (float)s * prod[n];
+}
+// source history of one step: vxp, vzp and div of the model region -> h [3][mn]
+__global__ void k2x_save_hist(G2 g, const float* __restrict__ vxp, const float* __restrict__ vzp,
+                              const float* __restrict__ dv, float* __restrict__ h) {
+    const long mn = (long)g.nx * g.nz;
+    for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < mn; q += (long)gridDim.x * blockDim.x) {
+        const long n = (long)((int)(q / g.nz) + g.bl) * g.pz + (int)(q % g.nz) + g.bu;
+        h[q] = vxp[n]; h[mn + q] = vzp[n]; h[2 * mn + q] = dv[n];
+    }
+}
+// boundary strips: the four fields f0..f3 at the ring cells idx[0..nr) to
+// buf [4][nr] (save) or back (restore); restore with a null buf writes zeros
+__global__ void k2x_ring(float* __restrict__ f0, float* __restrict__ f1, float* __restrict__ f2, float* __restrict__ f3,
+                         const long* __restrict__ idx, long nr, float* __restrict__ buf, int save) {
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nr; t += (long)gridDim.x * blockDim.x) {
+        const long n = idx[t];
+        if (save) {
+            buf[t] = f0[n]; buf[nr + t] = f1[n]; buf[2 * nr + t] = f2[n]; buf[3 * nr + t] = f3[n];
+        } else if (buf) {
+            f0[n] = buf[t]; f1[n] = buf[nr + t]; f2[n] = buf[2 * nr + t]; f3[n] = buf[3 * nr + t];
+        } else {
+            f0[n] = 0.f; f1[n] = 0.f; f2[n] = 0.f; f3[n] = 0.f;
+        }
+    }
+}
+
+// Residual records of the mapped reference-mode adjoint: each receiver's
+// trace times dt / rho at the receiver cell (the scale that makes the mapped
+// gradient an approximate transpose of the Born operator); rec [2][lt][nr]
+// per shot, shots along blockIdx.y (record stride rst).
+__global__ void k2r_scale_rec(float* __restrict__ rec, const long* __restrict__ idx, int nr, long lt,
+                              const float* __restrict__ B, float dt, long rst) {
+    rec += blockIdx.y * rst;
+    const long n = 2 * lt * nr;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x)
+        rec[t] *= dt * B[idx[t % nr]];
 }
 
 // Gradient in the inverted parameters: the adjoint accumulates (2 GP, 2 GS,

@@ -130,6 +130,18 @@ struct erm2d_ctx {
     hipEvent_t ev[2] = {nullptr, nullptr};
     double last_ms = 0;
     float qsign = 1.f;                      // -1 during a Q-compensated receiver pass
+    float dsgn = 1.f;                       // -1 during a reversed source step (erm2d_rtm_xcorr)
+    float* psiB = nullptr;                  // C-PML memory variables of state B (erm2d_rtm_xcorr)
+    // erm2d_rtm_xcorr buffers (allocated on first use)
+    float* xc_img = nullptr;                // [5][mn]
+    int* xc_vote = nullptr;                 // per cell, padded grid
+    int* xc_sign = nullptr;
+    float* xc_hist = nullptr;               // strategy 0: [lt_forward][3][mn]
+    size_t xc_hist_n = 0;
+    long* xc_ring = nullptr;                // strategy 2: ring cell indices
+    long xc_nr = 0;
+    float* xc_strip = nullptr;              // strategy 2: [lt_forward][2][4][nr] (stress side, velocity side)
+    size_t xc_strip_n = 0;
     std::vector<float> bg_host[5];          // background model as loaded (vp vs rho qp qs), for workers
     std::vector<erm2d_ctx*> workers;        // extra contexts (own streams) for concurrent shots
     // Batched LSRTM operators (erm2d_set_batch): per-shot copies of the two
@@ -183,6 +195,7 @@ G2 geo(const erm2d_ctx* c) {
     g.att = c->att;
     g.bl = c->p.bl; g.bu = c->p.bu; g.nx = c->p.nx; g.nz = c->p.nz;
     g.qsign = c->qsign;
+    g.dsgn = c->dsgn;
     g.psi = c->cpml ? c->psi : nullptr;
     g.pcx = c->pcx;
     g.pcz = c->pcz;
@@ -231,7 +244,7 @@ dim3 grid_pt(const erm2d_ctx* c) { return dim3((c->nnz + 255) / 256, c->nnx, nsh
 // geometry of a launch on state A (the C-PML memory variables belong to it) or B
 G2 geo_of(const erm2d_ctx* c, bool B) {
     G2 g = geo(c);
-    if (B) g.psi = nullptr;
+    if (B) g.psi = c->cpml && !c->nb ? c->psiB : nullptr;
     return g;
 }
 template <int R, int L, bool SAVE>
@@ -295,11 +308,30 @@ void adj_s2(erm2d_ctx* c) {
     else hipLaunchKernelGGL((k2a_s2<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, W[0], W[1], W[2]);
 }
 
+// reference-mode adjoint steps on state B (see k2r_s / k2r_v)
+template <int R, int L>
+void ref_step_rl(erm2d_ctx* c, const float* Eh) {
+    hipLaunchKernelGGL((k2r_s<R, L, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef);
+    hipLaunchKernelGGL((k2r_v<R, L, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef,
+                       Eh, grad_of(c));
+}
+template <int R>
+void ref_step_r(erm2d_ctx* c, const float* Eh) {
+    if (c->L == 0) ref_step_rl<R, 0>(c, Eh);
+    else if (c->L == 1) ref_step_rl<R, 1>(c, Eh);
+    else ref_step_rl<R, 3>(c, Eh);
+}
+void ref_step(erm2d_ctx* c, const float* Eh) {
+    if (c->R == 4) ref_step_r<4>(c, Eh);
+    else ref_step_r<6>(c, Eh);
+}
+
 int zero_state(erm2d_ctx* c, bool B) {
     const F2 f = state(c, B);
     const size_t n = c->ncell * nshot_of(c);
     for (float* a : {f.vx, f.vz, f.xx, f.zz, f.xz, f.tp, f.vxp, f.vzp}) if (fillf(c, a, n, 0.f)) return ERM_E_DEVICE;
     if (!B && !c->nb && c->psi && fillf(c, c->psi, 10 * c->ncell, 0.f)) return ERM_E_DEVICE;
+    if (B && !c->nb && c->psiB && fillf(c, c->psiB, 10 * c->ncell, 0.f)) return ERM_E_DEVICE;
     for (int l = 0; l < c->L; ++l)
         for (int q = 0; q < 4; ++q) if (fillf(c, f.r[l][q], n, 0.f)) return ERM_E_DEVICE;
     return 0;
@@ -360,7 +392,8 @@ void free2(erm2d_ctx* c) {
     F(c->dv); F(c->cu);
     for (auto& m : c->mod) { F(m.M); F(m.U); F(m.B); for (int l = 0; l < 3; ++l) { F(m.kP[l]); F(m.kS[l]); F(m.decay[l]); } }
     F(c->att); F(c->rec_idx); F(c->rec); F(c->rec2); F(c->ex_time); F(c->ex_tp_time); F(c->ex_tp_amp);
-    F(c->psi); F(c->pcx); F(c->pcz);
+    F(c->psi); F(c->pcx); F(c->pcz); F(c->psiB);
+    F(c->xc_img); F(c->xc_vote); F(c->xc_sign); F(c->xc_hist); F(c->xc_ring); F(c->xc_strip);
     for (auto& m : c->mod) { F(m.vp); F(m.vs); }
     for (auto& e : c->ex) F(e);
     for (auto& e : c->img) F(e);
@@ -419,6 +452,7 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
     if (p.bu < 0 || p.bd < 0 || p.bl < 0 || p.br < 0) { g_err2 = "negative boundary width"; return ERM_E_PARAM; }
     if (p.boundary != ERM_BOUNDARY_DAMP && p.boundary != ERM_BOUNDARY_CPML) { g_err2 = "boundary must be 0 (damping) or 1 (CPML)"; return ERM_E_PARAM; }
     if (p.inversion_para < 0 || p.inversion_para > 3) { g_err2 = "inversion_para must be 1, 2 or 3 (0: 2)"; return ERM_E_PARAM; }
+    if (p.adjoint_mode != 0 && p.adjoint_mode != 1) { g_err2 = "adjoint_mode must be 0 (exact transpose) or 1 (reference)"; return ERM_E_PARAM; }
     int nd = 0;
     if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) { g_err2 = "no HIP device"; return ERM_E_DEVICE; }
     if (device < 0 || device >= nd) { g_err2 = "device out of range"; return ERM_E_PARAM; }
@@ -741,6 +775,146 @@ static int cut_model(erm2d_ctx* c, const float* d, float* out) {
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+int timed_ms(erm2d_ctx* c);
+void decom_into(erm2d_ctx* c, bool B, float* dv, float* cu) {
+    const F2 f = state(c, B);
+    const G2 g = geo(c);
+    if (c->R == 4)
+        hipLaunchKernelGGL((k2_decom<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, f.vx, f.vz, dv, cu, g, c->coef,
+                           (float)(1.0 / c->p.dx), (float)(1.0 / c->p.dz), (const float*)nullptr, (const float*)nullptr);
+    else
+        hipLaunchKernelGGL((k2_decom<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, f.vx, f.vz, dv, cu, g, c->coef,
+                           (float)(1.0 / c->p.dx), (float)(1.0 / c->p.dz), (const float*)nullptr, (const float*)nullptr);
+}
+// ring of the strips strategy: cells outside the model region shrunk by one
+// cell within R + 1 of it (R for the reversed stencils, one more for the
+// div / curl of the model region's border cells)
+std::vector<long> xcorr_ring(const erm2d_ctx* c) {
+    const int i0 = c->p.bl + 1, i1 = c->p.bl + c->p.nx - 1, k0 = c->p.bu + 1, k1 = c->p.bu + c->p.nz - 1;
+    std::vector<long> out;
+    for (int i = std::max(i0 - c->R - 1, 0); i < std::min(i1 + c->R + 1, c->nnx); ++i)
+        for (int k = std::max(k0 - c->R - 1, 0); k < std::min(k1 + c->R + 1, c->nnz); ++k)
+            if (i < i0 || i >= i1 || k < k0 || k >= k1) out.push_back((long)i * c->pz + k);
+    return out;
+}
+void ring_io(erm2d_ctx* c, bool vside, float* buf, int save) {
+    const F2 f = state(c, false);
+    float* a[4] = {vside ? f.vx : f.xx, vside ? f.vz : f.zz, vside ? f.vxp : f.xz, vside ? f.vzp : f.tp};
+    hipLaunchKernelGGL(k2x_ring, dim3((unsigned)std::min<long>((c->xc_nr + 255) / 256, 1024)), dim3(256), 0, c->stream,
+                       a[0], a[1], a[2], a[3], c->xc_ring, c->xc_nr, buf, save);
+}
+}  // namespace
+
+extern "C" {
+
+int erm2d_rtm_xcorr(erm2d_ctx* c, const float* rec, int strategy, float* images) {
+    if (!c) return ERM_E_PARAM;
+    if (strategy < 0 || strategy > 2) return c->fail(ERM_E_PARAM, "strategy must be 0 (history), 1 (reverse) or 2 (strips)");
+    if (strategy == 2 && c->L != 0) return c->fail(ERM_E_PARAM, "the strips strategy needs an elastic medium (nsls = 0)");
+    HC(c, hipSetDevice(c->device));
+    if (!c->mod[1].have) return c->fail(ERM_E_STATE, "background model not loaded");
+    const int ltf = std::min(c->p.lt_forward < 0 ? c->p.lt : c->p.lt_forward, c->p.lt);
+    const size_t mn = c->mn;
+    if (!c->xc_img && dmal(c, &c->xc_img, 5 * mn)) return ERM_E_NOMEM;
+    if (!c->xc_vote && dmal(c, &c->xc_vote, c->ncell)) return ERM_E_NOMEM;
+    if (!c->xc_sign && dmal(c, &c->xc_sign, c->ncell)) return ERM_E_NOMEM;
+    if (c->cpml && !c->psiB && dmal(c, &c->psiB, 10 * c->ncell)) return ERM_E_NOMEM;
+    if (strategy == 0 && c->xc_hist_n < (size_t)ltf * 3 * mn) {
+        if (c->xc_hist) { (void)hipFree(c->xc_hist); c->xc_hist = nullptr; c->xc_hist_n = 0; }
+        if (dmal(c, &c->xc_hist, (size_t)ltf * 3 * mn)) return ERM_E_NOMEM;
+        c->xc_hist_n = (size_t)ltf * 3 * mn;
+    }
+    if (strategy == 2) {
+        if (!c->xc_ring) {
+            std::vector<long> rg = xcorr_ring(c);
+            c->xc_nr = (long)rg.size();
+            if (dmal(c, &c->xc_ring, rg.size())) return ERM_E_NOMEM;
+            HC(c, hipMemcpyAsync(c->xc_ring, rg.data(), rg.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
+        }
+        const size_t need = (size_t)ltf * 8 * c->xc_nr;
+        if (c->xc_strip_n < need) {
+            if (c->xc_strip) { (void)hipFree(c->xc_strip); c->xc_strip = nullptr; c->xc_strip_n = 0; }
+            if (dmal(c, &c->xc_strip, need)) return ERM_E_NOMEM;
+            c->xc_strip_n = need;
+        }
+    }
+    if (rec && c->p.nrec > 0)
+        HC(c, hipMemcpyAsync(c->rec, rec, (size_t)2 * c->p.lt * c->p.nrec * 4, hipMemcpyHostToDevice, c->stream));
+    HC(c, hipEventRecord(c->ev[0], c->stream));
+    int rc;
+    if ((rc = zero_state(c, false)) || (rc = zero_state(c, true))) return rc;
+    if ((rc = fillf(c, c->xc_img, 5 * mn, 0.f))) return rc;
+    hipLaunchKernelGGL(k2_filli, dim3(512), dim3(256), 0, c->stream, c->xc_sign, (long)c->ncell, 0);
+    c->last_model = 1;
+    const int which = 1;
+    const long nr8 = 8 * c->xc_nr;
+    float* sdv = c->W[0];   // source div (padded grid)
+    float* prod = c->W[1];  // div_s curl_r
+    // source pass (state A)
+    for (int it = 0; it < ltf; ++it) {
+        do_stress(c, false, which, c->src, wavelet_at(c, it), nullptr);
+        if (strategy == 2) ring_io(c, false, c->xc_strip + it * nr8, 1);
+        do_velocity(c, false, which, 0, it, nullptr);
+        if (strategy == 2) ring_io(c, true, c->xc_strip + it * nr8 + 4 * c->xc_nr, 1);
+        if (strategy == 0) {
+            decom_into(c, false, sdv, prod);
+            hipLaunchKernelGGL(k2x_save_hist, dim3((unsigned)std::min<size_t>((mn + 255) / 256, 2048)), dim3(256), 0,
+                               c->stream, geo(c), c->fA[6], c->fA[7], sdv, c->xc_hist + (size_t)it * 3 * mn);
+        }
+    }
+    // receiver pass (state B), imaging, source reversal
+    const long comp = (long)c->p.lt * c->p.nrec;
+    const int rg = (c->p.nrec + 255) / 256;
+    const F2 fb = state(c, true);
+    const dim3 gpt = grid_pt(c), gpol((c->nnz + TZ - 1) / TZ, (c->nnx + 3) / 4);
+    for (int it = c->p.lt - 1; it >= 0; --it) {
+        if (c->p.nrec > 0)
+            hipLaunchKernelGGL(k2_inject, dim3(rg), dim3(256), 0, c->stream, fb.vx, fb.vz, c->rec_idx, c->p.nrec, c->rec,
+                               comp, (long)it * c->p.nrec, 0L, 0L);
+        c->qsign = c->p.qcomp ? -1.f : 1.f;
+        do_stress(c, true, which, -1, 0.f, nullptr);
+        c->qsign = 1.f;
+        do_velocity(c, true, which, 0, it, nullptr);
+        if (it >= ltf) continue;
+        decom_into(c, true, c->dv, c->cu);
+        const float *sx, *sz, *sd;
+        int hist = 0;
+        if (strategy == 0) {
+            const float* h = c->xc_hist + (size_t)it * 3 * mn;
+            sx = h; sz = h + mn; sd = h + 2 * mn; hist = 1;
+        } else {
+            decom_into(c, false, sdv, prod);
+            sx = c->fA[6]; sz = c->fA[7]; sd = sdv;
+        }
+        hipLaunchKernelGGL(k2x_image, gpt, dim3(256), 0, c->stream, geo(c), sx, sz, sd, hist, fb, c->dv, c->cu,
+                           c->xc_img, c->xc_vote, prod, c->R);
+        hipLaunchKernelGGL(k2x_polarity, gpol, dim3(TZ, 4), 0, c->stream, geo(c), c->xc_vote, c->xc_sign, prod,
+                           c->xc_img, c->R);
+        if (strategy != 0) {  // undo source step it (rfwd_..._old)
+            if (strategy == 2) ring_io(c, false, c->xc_strip + it * nr8, 0);
+            c->dsgn = -1.f;
+            do_velocity(c, false, which, 0, it, nullptr);
+            c->dsgn = 1.f;
+            if (strategy == 2) ring_io(c, true, it > 0 ? c->xc_strip + (it - 1) * nr8 + 4 * c->xc_nr : nullptr, 0);
+            c->dsgn = -1.f;
+            c->qsign = -1.f;
+            do_stress(c, false, which, c->src, -wavelet_at(c, it), nullptr);
+            c->dsgn = 1.f;
+            c->qsign = 1.f;
+        }
+    }
+    HC(c, hipGetLastError());
+    if ((rc = timed_ms(c))) return rc;
+    if (images) {
+        HC(c, hipMemcpyAsync(images, c->xc_img, 5 * mn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        HC(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
 int erm2d_rtm(erm2d_ctx* c, const float* rec, float* tp, float* pp, float* ps) {
     if (!c) return ERM_E_PARAM;
     HC(c, hipSetDevice(c->device));
@@ -802,7 +976,9 @@ int born_core(erm2d_ctx* c) {
 // step runs backward from the residual records in rec_of(c); per-shot
 // gradients in grad_of(c), summed into c->grad and mapped to the inverted
 // parameters.
+int adjoint_ref_core(erm2d_ctx* c, bool mapped);
 int adjoint_core(erm2d_ctx* c) {
+    if (c->p.adjoint_mode == 1) return adjoint_ref_core(c, true);
     int rc;
     const long comp = (long)c->p.lt * c->p.nrec;
     const int rg = (c->p.nrec + 255) / 256;
@@ -835,6 +1011,48 @@ int adjoint_core(erm2d_ctx* c) {
                            c->stream, c->bt.grad, ns, (long)(3 * c->mn), c->grad);
     if (para(c) != 2)
         hipLaunchKernelGGL(k2_grad_para, dim3(gb), dim3(256), 0, c->stream, c->grad, model(c, 1), geo(c), para(c));
+    HC(c, hipGetLastError());
+    return 0;
+}
+
+// Reference-mode adjoint over the shots of the running operator (one, or nb
+// batched along z): background pass storing E and F per step, then the
+// reversed adjoint steps from the residual records in rec_of(c), gradient
+// (g_lam, g_mu, g_rho) of every step accumulated per shot and summed into
+// c->grad.  mapped: the records are first scaled per receiver by dt / rho
+// and the sum is mapped to the inverted parameters (k2r_grad_map).
+int adjoint_ref_core(erm2d_ctx* c, bool mapped) {
+    int rc;
+    const long comp = (long)c->p.lt * c->p.nrec;
+    const int rg = (c->p.nrec + 255) / 256;
+    const int ns = nshot_of(c);
+    if ((rc = zero_state(c, false)) || (rc = zero_state(c, true))) return rc;
+    if ((rc = fillf(c, grad_of(c), 3 * c->mn * ns, 0.f))) return rc;
+    float* hist = hist_of(c);
+    float* rec = rec_of(c);
+    const long sst = c->nb ? (long)c->ncell : 0, rst = c->nb ? 2 * comp : 0;
+    const F2 fb = state(c, true);
+    const size_t step = 5 * c->mn;
+    if (mapped && c->p.nrec > 0)
+        hipLaunchKernelGGL(k2r_scale_rec, dim3((unsigned)std::min<long>((2 * comp + 255) / 256, 1024), ns), dim3(256), 0,
+                           c->stream, rec, c->rec_idx, c->p.nrec, (long)c->p.lt, c->mod[1].B, c->dtr, rst);
+    for (int it = 0; it < c->p.lt; ++it) {
+        float* h = hist + (size_t)it * step;
+        do_stress(c, false, 1, c->src, wavelet_at(c, it), h);
+        do_velocity(c, false, 1, 0, it, h + 3 * c->mn);
+    }
+    for (int it = c->p.lt - 1; it >= 0; --it) {
+        if (c->p.nrec > 0)
+            hipLaunchKernelGGL(k2_inject, dim3(rg, ns), dim3(256), 0, c->stream, fb.vx, fb.vz, c->rec_idx, c->p.nrec,
+                               rec, comp, (long)it * c->p.nrec, sst, rst);
+        ref_step(c, hist + (size_t)it * step);
+    }
+    if (c->nb)
+        hipLaunchKernelGGL(k2_sum_shots, dim3((unsigned)std::min<size_t>((3 * c->mn + 255) / 256, 4096)), dim3(256), 0,
+                           c->stream, c->bt.grad, ns, (long)(3 * c->mn), c->grad);
+    if (mapped)
+        hipLaunchKernelGGL(k2r_grad_map, dim3((unsigned)std::min<size_t>((c->mn + 255) / 256, 2048)), dim3(256), 0,
+                           c->stream, c->grad, model(c, 1), geo(c), para(c));
     HC(c, hipGetLastError());
     return 0;
 }
@@ -902,6 +1120,22 @@ int erm2d_adjoint(erm2d_ctx* c, const float* r, float* gout) {
     HC(c, hipEventRecord(c->ev[0], c->stream));
     c->nb = 0;
     if ((rc = adjoint_core(c)) || (rc = timed_ms(c))) return rc;
+    if (gout) {
+        HC(c, hipMemcpyAsync(gout, c->grad, 3 * c->mn * 4, hipMemcpyDeviceToHost, c->stream));
+        HC(c, hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int erm2d_adjoint_ref(erm2d_ctx* c, const float* r, float* gout) {
+    if (!c || !r) return ERM_E_PARAM;
+    int rc;
+    if ((rc = lsrtm_ready(c))) return rc;
+    if (c->p.nrec > 0)
+        HC(c, hipMemcpyAsync(c->rec2, r, (size_t)2 * c->p.lt * c->p.nrec * 4, hipMemcpyHostToDevice, c->stream));
+    HC(c, hipEventRecord(c->ev[0], c->stream));
+    c->nb = 0;
+    if ((rc = adjoint_ref_core(c, false)) || (rc = timed_ms(c))) return rc;
     if (gout) {
         HC(c, hipMemcpyAsync(gout, c->grad, 3 * c->mn * 4, hipMemcpyDeviceToHost, c->stream));
         HC(c, hipStreamSynchronize(c->stream));

@@ -518,19 +518,30 @@ int upload(erm_ctx* c, float* d, const float* h) {
     return 0;
 }
 
-// edge-replicated padding of a model-region array in disk order [nx][ny][nz]
-// into the dense padded x-fastest layout (add_pml_layers_v_h, 3D_zzzzz:141-259).
-void pad_model(const erm_params& p, int nnx, int nny, int nnz, const float* src, std::vector<float>& dst) {
-    dst.resize((size_t)nnx * nny * nnz);
-    for (int k = 0; k < nnz; ++k) {
-        int kz = std::min(std::max(k - p.bu, 0), p.nz - 1);
-        for (int j = 0; j < nny; ++j) {
-            int jy = std::min(std::max(j - p.bf, 0), p.ny - 1);
-            for (int i = 0; i < nnx; ++i) {
-                int ix = std::min(std::max(i - p.bl, 0), p.nx - 1);
-                dst[((size_t)k * nny + j) * nnx + i] = src[((size_t)ix * p.ny + jy) * p.nz + kz];
-            }
-        }
+// Moduli of the padded grid from the model region in disk order [nx][ny][nz]
+// (z fastest), edge-replicated into the absorbing layers (add_pml_layers_v_h,
+// 3D_zzzzz:141-259): one thread per padded cell, x fastest.  A missing vs is
+// vp/sqrt(3) (vp_set_vs, zzzzz:1261-1270), a missing rho 230 vp^0.25
+// (vp_set_density, :1272-1287), both formed in double and rounded as the
+// host did; M = rho vp^2, MU = rho vs^2, BU = 1/rho in float.
+__global__ void k_model_moduli(const float* __restrict__ vp, const float* __restrict__ vs,
+                               const float* __restrict__ rho, int nx, int ny, int nz, int bl, int bf, int bu,
+                               int nnx, int nny, int nnz, int px, float* __restrict__ M, float* __restrict__ MU,
+                               float* __restrict__ BU) {
+    const long total = (long)nnx * nny * nnz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nnx);
+        const long r = t / nnx;
+        const int j = (int)(r % nny), k = (int)(r / nny);
+        const int ix = min(max(i - bl, 0), nx - 1), jy = min(max(j - bf, 0), ny - 1), kz = min(max(k - bu, 0), nz - 1);
+        const long q = ((long)ix * ny + jy) * nz + kz;
+        const float a = vp[q];
+        const float b = vs ? vs[q] : (float)((double)a / sqrt(3.0));
+        const float rr = rho ? rho[q] : (float)(1000 * 0.23 * sqrt(sqrt((double)a)));
+        const long n = ((long)k * nny + j) * px + i;
+        M[n] = rr * a * a;
+        MU[n] = rr * b * b;
+        BU[n] = 1.0f / rr;
     }
 }
 
@@ -950,32 +961,39 @@ int erm_load_model(erm_ctx* c, int which, const float* vp, const float* vs, cons
     if (!c) return ERM_E_PARAM;
     if (which < 0 || which > 1 || !vp) return c->fail(ERM_E_PARAM, "erm_load_model: bad arguments");
     HIPC(c, hipSetDevice(c->device));
-    std::vector<float> pvp, pvs, prho;
-    pad_model(c->p, c->nnx, c->nny, c->nnz, vp, pvp);
-    const size_t N = pvp.size();
-    if (vs) pad_model(c->p, c->nnx, c->nny, c->nnz, vs, pvs);
-    else { pvs.resize(N); for (size_t n = 0; n < N; ++n) pvs[n] = (float)(pvp[n] / std::sqrt(3.0)); }
-    if (rho) pad_model(c->p, c->nnx, c->nny, c->nnz, rho, prho);
-    else { prho.resize(N); for (size_t n = 0; n < N; ++n) prho[n] = (float)(1000 * 0.23 * std::sqrt(std::sqrt((double)pvp[n]))); }
-    for (size_t n = 0; n < N; ++n)
-        if (!(prho[n] > 0.f) || !(pvp[n] > 0.f) || pvs[n] < 0.f)
-            return c->fail(ERM_E_PARAM, "model has non-positive vp/rho or negative vs at padded cell %zu", n);
-    std::vector<float> hM(N), hU(N), hB(N);
-    for (size_t n = 0; n < N; ++n) {
-        float r = prho[n], a = pvp[n], b = pvs[n];
-        hM[n] = r * a * a;
-        hU[n] = r * b * b;
-        hB[n] = 1.0f / r;
+    const erm_params& p = c->p;
+    const size_t n = (size_t)p.nx * p.ny * p.nz;
+    // the padded cells replicate model cells, so the model region decides validity and vmax
+    double vm = 0;
+    for (size_t q = 0; q < n; ++q) {
+        const double a = vp[q];
+        const double b = vs ? (double)vs[q] : (double)(float)(a / std::sqrt(3.0));
+        const double r = rho ? (double)rho[q] : (double)(float)(1000 * 0.23 * std::sqrt(std::sqrt(a)));
+        if (!(r > 0) || !(a > 0) || b < 0)
+            return c->fail(ERM_E_PARAM, "model has non-positive vp/rho or negative vs at model cell %zu", q);
+        vm = std::max(vm, a);
     }
     if (!c->M[which]) {
         if (vmalloc(c, &c->M[which], c->ncell) || vmalloc(c, &c->MU[which], c->ncell) || vmalloc(c, &c->BU[which], c->ncell))
             return ERM_E_NOMEM;
     }
-    if (upload(c, c->M[which], hM.data()) || upload(c, c->MU[which], hU.data()) || upload(c, c->BU[which], hB.data()))
-        return ERM_E_DEVICE;
+    float* raw = nullptr;  // model-region arrays in disk order, for the padding kernel
+    if (dmalloc(c, &raw, 3 * n)) return ERM_E_NOMEM;
+    const float* srcs[3] = {vp, vs, rho};
+    hipError_t e = hipSuccess;
+    for (int a = 0; a < 3 && e == hipSuccess; ++a)
+        if (srcs[a]) e = hipMemcpyAsync(raw + a * n, srcs[a], n * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        const long total = (long)c->nnx * c->nny * c->nnz;
+        hipLaunchKernelGGL(k_model_moduli, dim3((unsigned)std::min<long>((total + 255) / 256, 65536)), dim3(256), 0, c->stream,
+                           raw, vs ? raw + n : nullptr, rho ? raw + 2 * n : nullptr, p.nx, p.ny, p.nz, p.bl, p.bf, p.bu,
+                           c->nnx, c->nny, c->nnz, c->px, c->M[which], c->MU[which], c->BU[which]);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(raw);
+    if (e != hipSuccess) return c->fail(ERM_E_DEVICE, "erm_load_model: %s", hipGetErrorString(e));
     c->have_model[which] = true;
-    double vm = 0;
-    for (size_t n = 0; n < N; ++n) vm = std::max(vm, (double)pvp[n]);
     c->vmax_loaded = std::max(c->vmax_loaded, vm);
     if (c->cpml) return build_cpml(c);
     return 0;

@@ -18,7 +18,7 @@ class Params2D(ctypes.Structure):
                                                 "rec_z"]]
                 + [("qcomp", ctypes.c_int), ("ex_tp_shift", ctypes.c_int), ("ex_vzp_avg", ctypes.c_int),
                    ("boundary", ctypes.c_int), ("cpml_r", ctypes.c_float), ("cpml_vmax", ctypes.c_float),
-                   ("inversion_para", ctypes.c_int), ("reserved", ctypes.c_int * 1)])
+                   ("inversion_para", ctypes.c_int), ("adjoint_mode", ctypes.c_int)])
 
     def update(self, **kw):
         for k, v in kw.items():

@@ -52,7 +52,10 @@ typedef struct erm2d_params {
                           cuda_cal_dem_parameter_viscoelastic_media_new, viscoelastic_equation.cu:2431-2486):
                           1 = (dlambda/lambda, dmu/mu, drho/rho); 2 (or 0, default) = (dvp/vp, dvs/vs,
                           drho/rho); 3 = impedances (dIp/Ip, dIs/Is, drho/rho), Ip = rho vp, Is = rho vs */
-    int   reserved[1];
+    int   adjoint_mode;   /* LSRTM adjoint of erm2d_adjoint / erm2d_adjoint_shots / erm2d_lsrtm_cg:
+                          0 (default) = exact transpose of the discrete Born operator;
+                          1 = the reference's continuous-adjoint equations and gradient
+                          (erm2d_adjoint_ref), mapped to the inversion_para parameters */
 } erm2d_params;
 
 /* fields for erm2d_get_field / erm2d_set_field (padded grid, z fastest) */
@@ -108,6 +111,35 @@ int   erm2d_forward(erm2d_ctx* ctx, float* rec);
  * Images over the model region [nx][nz]; any pointer may be NULL. */
 int   erm2d_rtm(erm2d_ctx* ctx, const float* rec, float* img_tp, float* img_pp, float* img_ps);
 
+/* Cross-correlation RTM in the background model with a choice of
+ * source-wavefield strategy (the reference's 2D options: full history,
+ * save_all_wavefield / save_and_set_wavefiled, elastic_2D_kernel_2.cu:793-827,
+ * viscoelastic_equation.cu:3-21; reversed propagation,
+ * rfwd_txxzzxzpp_viscoelastic_and_memory_old, :1298-1435) and the reference's
+ * PS polarity correction (set_sign_basedon_polarization_ps, filter_sign_new_share,
+ * compare_sign, elastic_2D_kernel_2.cu:875-982):
+ *   strategy ERM2D_SRC_HISTORY  the source pass stores vxp, vzp, div v of the model
+ *                               region every step (lt_forward * 3 * nx * nz floats)
+ *            ERM2D_SRC_REVERSE  the final source state is stepped backward (derivative
+ *                               and memory terms negated, memory updated forward-like);
+ *                               approximate: the absorbing boundary keeps absorbing
+ *            ERM2D_SRC_STRIPS   as REVERSE with the fields of a ring of width R + 1 around
+ *                               the model region (shrunk by one cell) stored every step and
+ *                               restored before each reversed half step: exact for
+ *                               elastic media (nsls = 0 only; no reference counterpart)
+ * The receiver pass images after its step it with the source wavefield after forward
+ * step it, for it < lt_forward; the source fields enter over the model region only.
+ * images [5][nx][nz] (may be NULL): PP = sum vP_s.vP_r, PS = sum vP_s.(v_r - vP_r),
+ * PPd = sum div_s div_r, PSc = sum div_s curl_r (decom stencils), and PSc with the
+ * polarity correction: each step every interior cell votes +1 / -1 / 0 by the sign of
+ * -sgn(vzp_s) sgn(rvxs) (vP_s . rvs), the 9x9 window majority sets the cell's sign
+ * (+1 wins ties with either class; a -1 / 0 tie keeps the previous sign), and
+ * PScp += sign div_s curl_r. */
+#define ERM2D_SRC_HISTORY 0
+#define ERM2D_SRC_REVERSE 1
+#define ERM2D_SRC_STRIPS  2
+int   erm2d_rtm_xcorr(erm2d_ctx* ctx, const float* rec, int strategy, float* images);
+
 /* LSRTM operators in the background model, model perturbation dm = [3][nx][nz]
  * of the inversion_para parameters (default 2: dvp/vp, dvs/vs, drho/rho;
  * viscoelastic_equation.cu:2431-2489):
@@ -116,6 +148,28 @@ int   erm2d_rtm(erm2d_ctx* ctx, const float* rec, float* img_tp, float* img_pp, 
  * where L^T is the exact transpose of the discrete L (dot-product exact). */
 int   erm2d_born(erm2d_ctx* ctx, const float* dm, float* d);
 int   erm2d_adjoint(erm2d_ctx* ctx, const float* r, float* g);
+/* The reference's own adjoint-state operator and gradient (the continuous
+ * adjoint; adjoint_fwd_vx_viscoelastic / adjoint_fwd_vz_viscoelastic,
+ * viscoelastic_equation.cu:1827-2122, adjoint_fwd_memory_new :2158-2198,
+ * cal_gradient_in_viscoelastic_media_new :2248-2295) in the background model.
+ * Adjoint state: velocities, strain-like stresses (updated by the strain
+ * increments of the adjoint velocities: the reference does not ship that
+ * kernel) and per-mechanism memory variables; the residual r [2][lt][nrec] is
+ * added to the adjoint velocities at the receivers before each reversed step,
+ * and the gradient of every step is accumulated after it from the stored
+ * background strain rates and accelerations.
+ *   g [3][nx][nz] = (g_lambda, g_mu, g_rho), the raw sums of the reference's
+ *   gradient kernel for absolute perturbations.
+ * Measured against the Born operator (tests/test_oracle2d.py): for
+ * perturbations of lambda or mu alone <L dm, L dm> = (rho_r / dt) <dm, g>
+ * to 0.3%, rho_r the density at the receivers; for rho alone the sign is
+ * reversed (the reference adds rv . v_t where its Born source -drho v_t needs
+ * the opposite sign).  adjoint_mode = 1 turns this into an approximate
+ * transpose for erm2d_adjoint / erm2d_adjoint_shots / erm2d_lsrtm_cg: the
+ * residual is scaled by dt / rho at each receiver, the density term negated,
+ * and (g_lambda, g_mu, g_rho) mapped to the inversion_para parameters by the
+ * transpose of the reference's parameter map (:2431-2486). */
+int   erm2d_adjoint_ref(erm2d_ctx* ctx, const float* r, float* g);
 
 /* Batched shots (no reference counterpart; the reference runs one shot per
  * launch): erm2d_set_batch(ctx, nb) gives the context per-shot states and

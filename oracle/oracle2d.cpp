@@ -34,6 +34,7 @@
 #include "../include/erm2d.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -702,6 +703,404 @@ struct Oracle2D {
     }
 };
 
+/* ---------------------------------------------------------------------------
+ * Reference-mode adjoint: the reference's continuous-adjoint state equations
+ * and gradient, restated (adjoint_mode = 1; the exact transpose above is the
+ * default).  Adjoint state: velocities av (x, z), strain-like stresses as
+ * (xx, zz, xz) and per mechanism memory variables ar (xx, zz, xz).  One
+ * reversed step, every update in the damping bracket a[b u + ...]:
+ *   as += E(av)                 the strain increments of av (the kernel that
+ *                               updates the adjoint stresses is absent from the
+ *                               reference: no moduli, the positions of
+ *                               viscoelastic_equation.cu:137-176)
+ *   ar_l = a[(b - dt/tau_l) ar_l + dt as]   adjoint_fwd_memory_new (:2190-2196)
+ *   avx  = a[b avx + Mp cx D-x as_xx + (Mp - 2Ms) cx D-x as_zz + Ms cz D-z as_xz
+ *          + sum_l P_l cx D-x ar_xx + (P_l - 2S_l) cx D-x ar_zz + S_l cz D-z ar_xz]
+ *                               adjoint_fwd_vx_viscoelastic (:1923-1971)
+ *   avz  = a[b avz + Mp cz D+z as_zz + (Mp - 2Ms) cz D+z as_xx + Ms cx D+x as_xz
+ *          + sum_l (P_l - 2S_l) cz D+z ar_xx + P_l cz D+z ar_zz + S_l cx D+x ar_xz]
+ *                               adjoint_fwd_vz_viscoelastic (:2071-2120)
+ * with the moduli of the cell (no averaging) Mp = vp^2 tau_eps/tau_sig,
+ * P_l = vp^2 (1 - tau_eps/tau_sig)/tau_sig (:1908-1914), i.e. M/rho, U/rho,
+ * kP_l/rho, kS_l/rho of the forward model.  The residual records are added
+ * to av at the receivers before each reversed step (it = lt-1 .. 0).  The
+ * gradient of cal_gradient_in_viscoelastic_media_new (:2248-2295) is
+ * accumulated after the step over the model region from the background's
+ * strain rates vx_x = E0/dt, vz_z = E1/dt, vx_z + vz_x = E2/dt and
+ * accelerations v_t = B F/dt of step it:
+ *   g_rho    += avx vx_t + avz vz_t
+ *   g_lambda -= [mp (as_xx + as_zz) + sum_l np_l (ar_xx + ar_zz)] (vx_x + vz_z)
+ *   g_mu     -= 2A + 2B + C + sum_l (2D_l + 2E_l + F_l)
+ * A = (mp vx_x + (mp - ms) vz_z) as_xx, B = (mp vz_z + (mp - ms) vx_x) as_zz,
+ * C = ms (vx_z + vz_x) as_xz, D, E, F the same with np_l, ns_l and ar_l;
+ * mp = tau_eps/tau_sig = M/(rho vp^2), np_l = (1 - mp)/tau_sig = kP_l/(rho vp^2)
+ * (ms, ns_l likewise with vs; 0 where vs = 0).  The adjoint tp and r_p of the
+ * reference feed neither the velocities nor the gradient and are not kept.
+ * ------------------------------------------------------------------------- */
+template <typename T>
+struct RefAdjoint {
+    Oracle2D<T>& o;
+    std::vector<T> av[2], as[3], ar[3][3];
+    explicit RefAdjoint(Oracle2D<T>& oo) : o(oo) {
+        for (auto& v : av) v.assign(o.N, 0);
+        for (auto& v : as) v.assign(o.N, 0);
+        for (int l = 0; l < o.L; ++l) for (auto& v : ar[l]) v.assign(o.N, 0);
+    }
+    void step(const typename Oracle2D<T>::Model& m) {
+        const int R = o.R, nnx = o.nnx, nnz = o.nnz;
+        const ptrdiff_t sX = nnz;
+#pragma omp parallel for schedule(static)
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) {
+                const size_t n = o.id(i, k);
+                T E[3];
+                o.strain(av[0].data(), av[1].data(), n, E);
+                T b, a = o.ab(n, &b);
+                as[0][n] = a * (b * as[0][n] + E[0]);
+                as[1][n] = a * (b * as[1][n] + E[1]);
+                as[2][n] = a * (b * as[2][n] + E[2]);
+                for (int l = 0; l < o.L; ++l) {
+                    const T bl = b - m.decay[l][n];
+                    for (int q = 0; q < 3; ++q) ar[l][q][n] = a * (bl * ar[l][q][n] + o.dtr * as[q][n]);
+                }
+            }
+#pragma omp parallel for schedule(static)
+        for (int i = R; i < nnx - R; ++i)
+            for (int k = R; k < nnz - R; ++k) {
+                const size_t n = o.id(i, k);
+                T b, a = o.ab(n, &b);
+                const T Bn = m.B[n], Mp = m.M[n] * Bn, Ms = m.U[n] * Bn;
+                T fx = Mp * o.cx * o.dm(as[0].data() + n, sX) + (Mp - 2 * Ms) * o.cx * o.dm(as[1].data() + n, sX) +
+                       Ms * o.cz * o.dm(as[2].data() + n, 1);
+                T fz = Mp * o.cz * o.dp(as[1].data() + n, 1) + (Mp - 2 * Ms) * o.cz * o.dp(as[0].data() + n, 1) +
+                       Ms * o.cx * o.dp(as[2].data() + n, sX);
+                for (int l = 0; l < o.L; ++l) {
+                    const T P = m.kP[l][n] * Bn, S = m.kS[l][n] * Bn;
+                    fx += P * o.cx * o.dm(ar[l][0].data() + n, sX) + (P - 2 * S) * o.cx * o.dm(ar[l][1].data() + n, sX) +
+                          S * o.cz * o.dm(ar[l][2].data() + n, 1);
+                    fz += (P - 2 * S) * o.cz * o.dp(ar[l][0].data() + n, 1) + P * o.cz * o.dp(ar[l][1].data() + n, 1) +
+                          S * o.cx * o.dp(ar[l][2].data() + n, sX);
+                }
+                av[0][n] = a * (b * av[0][n] + fx);
+                av[1][n] = a * (b * av[1][n] + fz);
+            }
+    }
+    /* gradient of one step (cal_gradient_in_viscoelastic_media_new), E/F of the step */
+    void gradient(const typename Oracle2D<T>::Model& m, const T* Eh, const T* Fh, T* gl, T* gm, T* gr) {
+        const size_t mn = (size_t)o.p.nx * o.p.nz;
+        const T idt = (T)1 / o.dtr;
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < (long)mn; ++q) {
+            const int i = (int)(q / o.p.nz) + o.p.bl, k = (int)(q % o.p.nz) + o.p.bu;
+            if (!o.interior(i, k)) continue;
+            const size_t n = o.id(i, k);
+            const T vxx = Eh[q] * idt, vzz = Eh[mn + q] * idt, vxz = Eh[2 * mn + q] * idt;
+            const T vxt = m.B[n] * Fh[q] * idt, vzt = m.B[n] * Fh[mn + q] * idt;
+            const T rho = (T)1 / m.B[n], vp = o.vp_m[1][n], vs = o.vs_m[1][n];
+            const T mp = m.M[n] / (rho * vp * vp), ms = vs > 0 ? m.U[n] / (rho * vs * vs) : (T)0;
+            gr[q] += av[0][n] * vxt + av[1][n] * vzt;
+            T lam = mp * (as[0][n] + as[1][n]);
+            T A = (mp * vxx + (mp - ms) * vzz) * as[0][n], B = (mp * vzz + (mp - ms) * vxx) * as[1][n];
+            T C = ms * vxz * as[2][n], DEF = 0;
+            for (int l = 0; l < o.L; ++l) {
+                const T np = m.kP[l][n] / (rho * vp * vp), ns = vs > 0 ? m.kS[l][n] / (rho * vs * vs) : (T)0;
+                lam += np * (ar[l][0][n] + ar[l][1][n]);
+                DEF += 2 * (np * vxx + (np - ns) * vzz) * ar[l][0][n] + 2 * (np * vzz + (np - ns) * vxx) * ar[l][1][n] +
+                       ns * vxz * ar[l][2][n];
+            }
+            gl[q] -= lam * (vxx + vzz);
+            gm[q] -= 2 * A + 2 * B + C + DEF;
+        }
+    }
+};
+
+/* reference-mode adjoint operator: background pass storing E/F, then the
+ * reversed adjoint steps from the residual records; g = (g_lambda, g_mu, g_rho)
+ * over the model region, the raw sums of the reference's gradient kernel */
+template <typename T>
+int adjoint_ref(Oracle2D<T>& o, const float* rr, float* gout) {
+    if (o.cpml) return -1;
+    const auto& m = o.mod[1];
+    if (!m.have) return -4;
+    const auto& p = o.p;
+    const size_t mn = (size_t)p.nx * p.nz;
+    std::vector<T> EH((size_t)p.lt * 3 * mn), FH((size_t)p.lt * 2 * mn);
+    o.zero();
+    for (int it = 0; it < p.lt; ++it) {
+        o.add_source(o.f, it);
+        o.stress(o.f, o.r, m, EH.data() + (size_t)it * 3 * mn);
+        o.velocity(o.f, m, 0, it, FH.data() + (size_t)it * 2 * mn);
+    }
+    RefAdjoint<T> A(o);
+    std::vector<T> gl(mn, 0), gm(mn, 0), gr(mn, 0);
+    for (int it = p.lt - 1; it >= 0; --it) {
+        for (int q = 0; q < p.nrec; ++q) {
+            A.av[0][o.recs[q]] += (T)rr[((size_t)0 * p.lt + it) * p.nrec + q];
+            A.av[1][o.recs[q]] += (T)rr[((size_t)1 * p.lt + it) * p.nrec + q];
+        }
+        A.step(m);
+        A.gradient(m, EH.data() + (size_t)it * 3 * mn, FH.data() + (size_t)it * 2 * mn, gl.data(), gm.data(), gr.data());
+    }
+    for (size_t q = 0; q < mn; ++q) { gout[q] = (float)gl[q]; gout[mn + q] = (float)gm[q]; gout[2 * mn + q] = (float)gr[q]; }
+    return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Cross-correlation RTM with a choice of source-wavefield strategy and the
+ * reference's PS polarity correction.
+ *
+ * Reversed step (rfwd_txxzzxzpp_viscoelastic_and_memory_old,
+ * viscoelastic_equation.cu:1298-1435: the derivative and memory terms of the
+ * stress update negated, the memory variables updated as in the forward
+ * step; the velocity counterpart, absent from the reference, negates the
+ * stress-divergence term): undoes forward step it from (v^{it+1}, s^{it}):
+ *   v  = a[b v  - B F(s)],  vP = a[b vP - B grad tp]            (velocity first)
+ *   r' = a[(b - dt/tau) r + K E(v)],  s = a[b (s - w_it) - C E(v) - dt r'] with
+ *   the wavelet of step it taken off sxx, szz where the forward step adds it.
+ * In the undamped model region of an elastic medium this is the exact inverse.
+ *
+ * Strategies (erm2d_rtm_xcorr):
+ *   0 history  - the source pass stores vxp, vzp and div v of the model region
+ *                after every step (save_all_wavefield / save_and_set_wavefiled,
+ *                elastic_2D_kernel_2.cu:793-827, viscoelastic_equation.cu:3-21);
+ *   1 reverse  - the source state at the end of the source pass is stepped
+ *                backward with the reversed step alone (the absorbing boundary
+ *                keeps absorbing, so the reconstruction is approximate);
+ *   2 strips   - as 1, but the fields of a ring of width R around the model
+ *                region shrunk by one cell are stored every source step and
+ *                restored before each reversed half step: exact for elastic
+ *                media (no reference counterpart; nsls = 0 only).
+ * Receiver pass at it = lt-1 .. 0: inject, stress, velocity, then image with
+ * the source wavefield after forward step it (history it, or the reversed
+ * state before undoing step it), for it < lt_forward.
+ * Images over the model region, accumulated over the steps:
+ *   0 PP  = sum vP_s . vP_r              1 PS  = sum vP_s . (v_r - vP_r)
+ *   2 PPd = sum div_s div_r              3 PSc = sum div_s curl_r
+ * The source fields enter only over the model region (zero outside it, as
+ * the history of save_and_set_wavefiled holds them), in every strategy.
+ *   4 PSc with the polarity correction: per step, every interior cell votes
+ *     +1 / -1 / 0 by the sign of set = -sgn(vzp_s) sgn(rvxs) (vxp_s rvxs +
+ *     vzp_s rvzs) (set_sign_basedon_polarization_ps, elastic_2D_kernel_2.cu:
+ *     941-982; rvs = v_r - vP_r; the case vzp_s < 0, rvxs = 0 that the
+ *     reference leaves undefined votes 0), the votes are counted over the 9x9
+ *     window (filter_sign_new_share, :875-919), the majority sets the cell's
+ *     sign (compare_sign, :921-939: +1 wins ties with either other class;
+ *     a tie of -1 and 0 keeps the previous step's sign), and
+ *     PScp += sign div_s curl_r.
+ * div / curl are the decom stencils (elastic_2D_kernel_2.cu:621-666).
+ * ------------------------------------------------------------------------- */
+template <typename T>
+struct XcorrRTM {
+    Oracle2D<T>& o;
+    explicit XcorrRTM(Oracle2D<T>& oo) : o(oo) {}
+    using Model = typename Oracle2D<T>::Model;
+    enum { VX, VZ, XX, ZZ, XZ, TP, VXP, VZP };
+
+    void rvelocity(std::vector<T>* fv, const Model& m) {
+        const ptrdiff_t sX = o.nnz;
+#pragma omp parallel for schedule(static)
+        for (int i = o.R; i < o.nnx - o.R; ++i)
+            for (int k = o.R; k < o.nnz - o.R; ++k) {
+                size_t n = o.id(i, k);
+                T px = o.cx * o.dm(fv[TP].data() + n, sX), pz = o.cz * o.dp(fv[TP].data() + n, 1);
+                T f0x = o.cx * o.dm(fv[XX].data() + n, sX), f0z = o.cz * o.dm(fv[XZ].data() + n, 1);
+                T f1x = o.cx * o.dp(fv[XZ].data() + n, sX), f1z = o.cz * o.dp(fv[ZZ].data() + n, 1);
+                T F0 = o.pml(4, n, 0, 1, i, f0x) + o.pml(5, n, 1, 0, k, f0z);
+                T F1 = o.pml(6, n, 0, 0, i, f1x) + o.pml(7, n, 1, 1, k, f1z);
+                px = o.pml(8, n, 0, 1, i, px);
+                pz = o.pml(9, n, 1, 1, k, pz);
+                T b, a = o.ab(n, &b), bo = m.B[n];
+                fv[VX][n] = a * (b * fv[VX][n] + bo * -F0);
+                fv[VZ][n] = a * (b * fv[VZ][n] + bo * -F1);
+                fv[VXP][n] = a * (b * fv[VXP][n] + bo * -px);
+                fv[VZP][n] = a * (b * fv[VZP][n] + bo * -pz);
+            }
+    }
+    void rstress(std::vector<T>* fv, std::vector<T> (*rv)[4], const Model& m, int it) {
+        const ptrdiff_t sX = o.nnz;
+        /* the wavelet enters where the forward step adds it, with the opposite sign */
+        if (it >= 0 && it < (int)o.wavelet.size()) { fv[XX][o.src] -= o.wavelet[it]; fv[ZZ][o.src] -= o.wavelet[it]; }
+#pragma omp parallel for schedule(static)
+        for (int i = o.R; i < o.nnx - o.R; ++i)
+            for (int k = o.R; k < o.nnz - o.R; ++k) {
+                size_t n = o.id(i, k);
+                T e0 = o.cx * o.dp(fv[VX].data() + n, sX), e1 = o.cz * o.dm(fv[VZ].data() + n, 1);
+                T e2x = o.cx * o.dm(fv[VZ].data() + n, sX), e2z = o.cz * o.dp(fv[VX].data() + n, 1);
+                T E[3] = {o.pml(0, n, 0, 0, i, e0), o.pml(2, n, 1, 0, k, e1),
+                          o.pml(1, n, 0, 1, i, e2x) + o.pml(3, n, 1, 1, k, e2z)};
+                T b, a = o.ab(n, &b);
+                T M = m.M[n], U = m.U[n], La = M - 2 * U;
+                T sp = 0, sxx = 0, szz = 0, sxz = 0;
+                for (int l = 0; l < o.L; ++l) {
+                    T bl = b - m.decay[l][n], kP = m.kP[l][n], kS = m.kS[l][n];
+                    T rp = a * (bl * rv[l][0][n] + kP * (E[0] + E[1]));
+                    T rxx = a * (bl * rv[l][1][n] - 2 * kS * E[1]);
+                    T rzz = a * (bl * rv[l][2][n] - 2 * kS * E[0]);
+                    T rxz = a * (bl * rv[l][3][n] + kS * E[2]);
+                    rv[l][0][n] = rp; rv[l][1][n] = rxx; rv[l][2][n] = rzz; rv[l][3][n] = rxz;
+                    sp += rp; sxx += rxx; szz += rzz; sxz += rxz;
+                }
+                const T ne0 = -E[0], ne1 = -E[1], ne2 = -E[2], dq = -o.dtr;
+                fv[XX][n] = a * (b * fv[XX][n] + M * ne0 + La * ne1 + dq * (sp + sxx));
+                fv[ZZ][n] = a * (b * fv[ZZ][n] + La * ne0 + M * ne1 + dq * (sp + szz));
+                fv[TP][n] = a * (b * fv[TP][n] + M * (ne0 + ne1) + dq * sp);
+                fv[XZ][n] = a * (b * fv[XZ][n] + U * ne2 + dq * sxz);
+            }
+    }
+    /* the ring: cells outside the model region shrunk by one cell, within R + 1
+     * of it (R for the reversed stencils, one more for the div / curl of the
+     * model region's border cells) */
+    std::vector<size_t> ring() const {
+        const int i0 = o.p.bl + 1, i1 = o.p.bl + o.p.nx - 1, k0 = o.p.bu + 1, k1 = o.p.bu + o.p.nz - 1;
+        std::vector<size_t> out;
+        for (int i = std::max(i0 - o.R - 1, 0); i < std::min(i1 + o.R + 1, o.nnx); ++i)
+            for (int k = std::max(k0 - o.R - 1, 0); k < std::min(k1 + o.R + 1, o.nnz); ++k)
+                if (i < i0 || i >= i1 || k < k0 || k >= k1) out.push_back(o.id(i, k));
+        return out;
+    }
+    void div_curl(const std::vector<T>* fv, std::vector<T>& dv, std::vector<T>& cu) const {
+        dv.assign(o.N, 0); cu.assign(o.N, 0);
+        const ptrdiff_t sX = o.nnz;
+        for (int i = o.R; i < o.nnx - o.R; ++i)
+            for (int k = o.R; k < o.nnz - o.R; ++k) {
+                size_t n = o.id(i, k);
+                dv[n] = (T)(1.0 / o.p.dx) * o.dp(fv[VX].data() + n, sX) + (T)(1.0 / o.p.dz) * o.dm(fv[VZ].data() + n, 1);
+                cu[n] = (T)(1.0 / o.p.dz) * o.dp(fv[VX].data() + n, 1) - (T)(1.0 / o.p.dx) * o.dm(fv[VZ].data() + n, sX);
+            }
+    }
+
+    int run(int strategy, float* out) {
+        auto& p = o.p;
+        const Model& m = o.mod[1];
+        if (!m.have) return -4;
+        if (strategy < 0 || strategy > 2) return -1;
+        if (strategy == 2 && o.L != 0) return -1;
+        /* each state keeps its own C-PML memory variables: the receiver's live
+         * in psiR and are swapped into the oracle around its steps */
+        std::vector<T> psiR[10];
+        for (auto& v : psiR) v.assign(o.cpml ? o.N : 0, (T)0);
+        auto swap_psi = [&]() { for (int q = 0; q < 10; ++q) std::swap(o.psi[q], psiR[q]); };
+        const size_t mn = (size_t)p.nx * p.nz;
+        const int ltf = p.lt_forward < 0 ? p.lt : std::min(p.lt_forward, p.lt);
+        std::vector<size_t> rg = ring();
+        const size_t nr = rg.size();
+        std::vector<T> hist, ringv, rings;
+        if (strategy == 0) hist.assign((size_t)ltf * 3 * mn, 0);
+        if (strategy == 2) { ringv.assign((size_t)ltf * nr * 4, 0); rings.assign((size_t)ltf * nr * 4, 0); }
+        const int vfield[4] = {VX, VZ, VXP, VZP}, sfield[4] = {XX, ZZ, XZ, TP};
+        std::vector<T> dv, cu;
+        o.zero();
+        for (int it = 0; it < ltf; ++it) {
+            o.add_source(o.f, it);
+            o.stress(o.f, o.r, m);
+            if (strategy == 2)
+                for (int q = 0; q < 4; ++q)
+                    for (size_t j = 0; j < nr; ++j) rings[((size_t)it * 4 + q) * nr + j] = o.f[sfield[q]][rg[j]];
+            o.velocity(o.f, m, 0, it);
+            if (strategy == 2)
+                for (int q = 0; q < 4; ++q)
+                    for (size_t j = 0; j < nr; ++j) ringv[((size_t)it * 4 + q) * nr + j] = o.f[vfield[q]][rg[j]];
+            if (strategy == 0) {
+                div_curl(o.f, dv, cu);
+                T* h = hist.data() + (size_t)it * 3 * mn;
+                for (size_t q = 0; q < mn; ++q) {
+                    size_t n = o.id((int)(q / p.nz) + p.bl, (int)(q % p.nz) + p.bu);
+                    h[q] = o.f[VXP][n]; h[mn + q] = o.f[VZP][n]; h[2 * mn + q] = dv[n];
+                }
+            }
+        }
+        /* receiver state */
+        std::vector<T> g[8];
+        std::vector<T> gr[3][4];
+        for (auto& v : g) v.assign(o.N, 0);
+        for (int l = 0; l < o.L; ++l) for (auto& v : gr[l]) v.assign(o.N, 0);
+        std::vector<double> img[5];
+        for (auto& v : img) v.assign(mn, 0.0);
+        std::vector<int> sign(o.N, 0), vote(o.N, 0);
+        std::vector<T> prod(o.N, 0), rdv, rcu;
+        const T qs = p.qcomp ? (T)-1 : (T)1;
+        for (int it = p.lt - 1; it >= 0; --it) {
+            for (int q = 0; q < p.nrec; ++q) {
+                g[VX][o.recs[q]] += o.rec[((size_t)0 * p.lt + it) * p.nrec + q];
+                g[VZ][o.recs[q]] += o.rec[((size_t)1 * p.lt + it) * p.nrec + q];
+            }
+            swap_psi();
+            o.qsign = qs;
+            o.stress(g, gr, m);
+            o.qsign = 1;
+            o.velocity(g, m, 0, it);
+            swap_psi();
+            if (it >= ltf) continue;
+            /* source fields after forward step it */
+            std::vector<T> sxp(o.N, 0), szp(o.N, 0), sdv(o.N, 0);
+            if (strategy == 0) {
+                const T* h = hist.data() + (size_t)it * 3 * mn;
+                for (size_t q = 0; q < mn; ++q) {
+                    size_t n = o.id((int)(q / p.nz) + p.bl, (int)(q % p.nz) + p.bu);
+                    sxp[n] = h[q]; szp[n] = h[mn + q]; sdv[n] = h[2 * mn + q];
+                }
+            } else {
+                div_curl(o.f, dv, cu);
+                for (size_t q = 0; q < mn; ++q) {
+                    size_t n = o.id((int)(q / p.nz) + p.bl, (int)(q % p.nz) + p.bu);
+                    sxp[n] = o.f[VXP][n]; szp[n] = o.f[VZP][n]; sdv[n] = dv[n];
+                }
+            }
+            div_curl(g, rdv, rcu);
+            for (int i = o.R; i < o.nnx - o.R; ++i)
+                for (int k = o.R; k < o.nnz - o.R; ++k) {
+                    size_t n = o.id(i, k);
+                    const T rxs = g[VX][n] - g[VXP][n], rzs = g[VZ][n] - g[VZP][n];
+                    const T dot = sxp[n] * rxs + szp[n] * rzs;
+                    T set = 0;
+                    if (szp[n] >= 0 && rxs >= 0) set = -dot;
+                    else if (szp[n] >= 0 && rxs < 0) set = dot;
+                    else if (szp[n] < 0 && rxs > 0) set = dot;
+                    else if (szp[n] < 0 && rxs < 0) set = -dot;
+                    vote[n] = set > 0 ? 1 : (set < 0 ? 2 : 3);
+                    prod[n] = sdv[n] * rcu[n];
+                    size_t q;
+                    if (o.model_cell(i, k, &q)) {
+                        img[0][q] += sxp[n] * g[VXP][n] + szp[n] * g[VZP][n];
+                        img[1][q] += dot;
+                        img[2][q] += sdv[n] * rdv[n];
+                        img[3][q] += prod[n];
+                    }
+                }
+            for (int i = o.R; i < o.nnx - o.R; ++i)
+                for (int k = o.R; k < o.nnz - o.R; ++k) {
+                    int cnt[4] = {0, 0, 0, 0};
+                    for (int di = -4; di <= 4; ++di)
+                        for (int dk = -4; dk <= 4; ++dk) {
+                            const int ii = i + di, kk = k + dk;
+                            if (o.interior(ii, kk)) ++cnt[vote[o.id(ii, kk)]];
+                        }
+                    size_t n = o.id(i, k);
+                    if (cnt[1] >= cnt[2] && cnt[1] >= cnt[3]) sign[n] = 1;
+                    else if (cnt[2] > cnt[1] && cnt[2] > cnt[3]) sign[n] = -1;
+                    else if (cnt[3] > cnt[1] && cnt[3] > cnt[2]) sign[n] = 0;
+                    size_t q;
+                    if (o.model_cell(i, k, &q)) img[4][q] += (T)sign[n] * prod[n];
+                }
+            /* undo source step it */
+            if (strategy != 0) {
+                if (strategy == 2)
+                    for (int q = 0; q < 4; ++q)
+                        for (size_t j = 0; j < nr; ++j) o.f[sfield[q]][rg[j]] = rings[((size_t)it * 4 + q) * nr + j];
+                rvelocity(o.f, m);
+                if (strategy == 2)
+                    for (int q = 0; q < 4; ++q)
+                        for (size_t j = 0; j < nr; ++j)
+                            o.f[vfield[q]][rg[j]] = it > 0 ? ringv[((size_t)(it - 1) * 4 + q) * nr + j] : (T)0;
+                rstress(o.f, o.r, m, it);
+            }
+        }
+        for (int a = 0; a < 5; ++a)
+            for (size_t q = 0; q < mn; ++q) out[a * mn + q] = (float)img[a][q];
+        return 0;
+    }
+};
+
 template <typename T>
 struct Box { Oracle2D<T> o; };
 
@@ -803,6 +1202,47 @@ int oracle2d_image(void* h, int it) {
 }
 int oracle2d_born(void* h, const float* dm, float* d) { return disp(h, [&](auto& o) { return o.born(dm, d); }); }
 int oracle2d_adjoint(void* h, const float* r, float* g) { return disp(h, [&](auto& o) { return o.adjoint(r, g); }); }
+/* cross-correlation RTM (records as set by oracle2d_record); out [5][nx][nz] */
+int oracle2d_rtm_xcorr(void* h, int strategy, float* out) {
+    return disp(h, [&](auto& o) {
+        using T = typename std::remove_reference<decltype(o.c[0])>::type;
+        XcorrRTM<T> X(o);
+        return X.run(strategy, out);
+    });
+}
+/* reference-mode adjoint: g = (g_lambda, g_mu, g_rho) [3][nx][nz] */
+int oracle2d_adjoint_ref(void* h, const float* r, float* g) {
+    return disp(h, [&](auto& o) { return adjoint_ref(o, r, g); });
+}
+/* reference-mode adjoint propagation alone: start from av = (vx, vz) with zero
+ * adjoint stresses and memory, run nsteps reversed steps without records, and
+ * return av (out [2][nnx*nnz]) and the moduli-weighted stresses of the
+ * adjoint state S = (Mp as_xx + (Mp-2Ms) as_zz + sum_l P_l ar_xx + (P_l-2S_l) ar_zz,
+ * the zz counterpart, Ms as_xz + sum_l S_l ar_xz) (out [3][nnx*nnz]) */
+int oracle2d_ref_adjoint_steps(void* h, const double* vx, const double* vz, int nsteps, double* vout, double* sout) {
+    return disp(h, [&](auto& o) {
+        const auto& m = o.mod[1];
+        if (!m.have) return -4;
+        using T = typename std::remove_reference<decltype(o.c[0])>::type;
+        RefAdjoint<T> A(o);
+        for (size_t n = 0; n < o.N; ++n) { A.av[0][n] = (T)vx[n]; A.av[1][n] = (T)vz[n]; }
+        for (int s = 0; s < nsteps; ++s) A.step(m);
+        for (size_t n = 0; n < o.N; ++n) {
+            vout[n] = A.av[0][n]; vout[o.N + n] = A.av[1][n];
+            const T Bn = m.B[n], Mp = m.M[n] * Bn, Ms = m.U[n] * Bn;
+            T sx = Mp * A.as[0][n] + (Mp - 2 * Ms) * A.as[1][n], sz = (Mp - 2 * Ms) * A.as[0][n] + Mp * A.as[1][n];
+            T sxz = Ms * A.as[2][n];
+            for (int l = 0; l < o.L; ++l) {
+                const T P = m.kP[l][n] * Bn, S = m.kS[l][n] * Bn;
+                sx += P * A.ar[l][0][n] + (P - 2 * S) * A.ar[l][1][n];
+                sz += (P - 2 * S) * A.ar[l][0][n] + P * A.ar[l][1][n];
+                sxz += S * A.ar[l][2][n];
+            }
+            sout[n] = sx; sout[o.N + n] = sz; sout[2 * o.N + n] = sxz;
+        }
+        return 0;
+    });
+}
 /* SLS parameter check: returns fM, fU, gP[0..2], gS[0..2], taus[0..2] for one cell */
 int oracle2d_sls(int L, double freq, double qp, double qs, double* out) {
     Sls s = sls_params(L, freq, qp, qs);

@@ -35,6 +35,9 @@ for name, res, args in [
     ("oracle2d_image", ctypes.c_int, [_vp, ctypes.c_int]),
     ("oracle2d_born", ctypes.c_int, [_vp, _fp, _fp]),
     ("oracle2d_adjoint", ctypes.c_int, [_vp, _fp, _fp]),
+    ("oracle2d_adjoint_ref", ctypes.c_int, [_vp, _fp, _fp]),
+    ("oracle2d_rtm_xcorr", ctypes.c_int, [_vp, ctypes.c_int, _fp]),
+    ("oracle2d_ref_adjoint_steps", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _dp, _dp]),
     ("oracle2d_sls", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]),
 ]:
     fn = getattr(lib, name)
@@ -128,6 +131,41 @@ class Oracle2D:
         self._c(lib.oracle2d_adjoint(self._h, _ptr(r, ctypes.c_float), _ptr(g, ctypes.c_float)))
         return g
 
+    def adjoint_ref(self, r):
+        """Reference-mode adjoint (the reference's continuous-adjoint
+        equations and gradient kernel): (g_lambda, g_mu, g_rho) [3][nx][nz]."""
+        p = self.p
+        r = _f(r)
+        g = np.zeros((3, p.nx, p.nz), np.float32)
+        self._c(lib.oracle2d_adjoint_ref(self._h, _ptr(r, ctypes.c_float), _ptr(g, ctypes.c_float)))
+        return g
+
+    def ref_adjoint_steps(self, vx, vz, nsteps):
+        """Reference-mode adjoint propagation from velocities (vx, vz) on the
+        padded grid with zero adjoint stresses and memory, nsteps reversed
+        steps without records; returns (av [2][nnx][nnz], S [3][nnx][nnz]),
+        S the moduli-weighted adjoint stresses (xx, zz, xz)."""
+        nnx, nnz = self.shape
+        vx = np.ascontiguousarray(vx, np.float64)
+        vz = np.ascontiguousarray(vz, np.float64)
+        v = np.zeros((2, nnx, nnz))
+        s = np.zeros((3, nnx, nnz))
+        self._c(lib.oracle2d_ref_adjoint_steps(self._h, _ptr(vx, ctypes.c_double), _ptr(vz, ctypes.c_double), nsteps,
+                                               _ptr(v, ctypes.c_double), _ptr(s, ctypes.c_double)))
+        return v, s
+
+    def rtm_xcorr(self, rec, strategy):
+        """Cross-correlation RTM of the records rec [2][lt][nrec] in the
+        background model with source-wavefield strategy 0 (history),
+        1 (reversed propagation) or 2 (boundary strips + reversal); images
+        [5][nx][nz]: PP, PS (vector P/S), PPd (div div), PSc (div curl),
+        PSc with the polarity correction."""
+        p = self.p
+        self.record(np.asarray(rec, np.float64))
+        out = np.zeros((5, p.nx, p.nz), np.float32)
+        self._c(lib.oracle2d_rtm_xcorr(self._h, strategy, _ptr(out, ctypes.c_float)))
+        return out
+
     def illumination(self, shots):
         """Source illumination of the background model (the quantity
         erm2d_illumination returns): for each shot, zero state, then after each
@@ -208,6 +246,30 @@ class Oracle2D:
             f = self.get_field(k)
             out.append(np.ascontiguousarray(f[p.bl:p.bl + p.nx, p.bu:p.bu + p.nz]))
         return out
+
+
+def ref_adjoint_mapped(o, bg, r):
+    """adjoint_mode 1 of erm2d_adjoint restated: the residual scaled by
+    dt / rho at each receiver cell, the reference-mode adjoint (g_lambda, g_mu,
+    g_rho), the density term negated, then the transpose of the reference's
+    parameter map for inversion_para (viscoelastic_equation.cu:2431-2486,
+    relative perturbations): 1 (lambda m1, mu m2, rho m3); 2 / 3 (vp, vs, rho) /
+    (Ip, Is, rho).  bg = (vp, vs, rho) of the background model region."""
+    p = o.p
+    vp, vs, rho = (np.asarray(a, np.float64) for a in bg[:3])
+    ix = np.array([p.rec_x0 + q * p.rec_dx for q in range(p.nrec)])
+    iz = np.full(p.nrec, p.rec_z)
+    # receivers on the padding use the edge-replicated model, as the padded grid does
+    rr = rho[np.clip(ix, 0, p.nx - 1), np.clip(iz, 0, p.nz - 1)]
+    rs = np.asarray(r, np.float64) * (p.dt * 1e-3 / rr)[None, None, :]
+    gl, gm, gr = o.adjoint_ref(rs.astype(np.float32)).astype(np.float64)
+    gr = -gr
+    lam, mu = rho * (vp ** 2 - 2 * vs ** 2), rho * vs ** 2
+    para = p.inversion_para if p.inversion_para in (1, 3) else 2
+    if para == 1:
+        return np.stack([lam * gl, mu * gm, rho * gr])
+    third = lam * gl + mu * gm if para == 2 else -(lam * gl + mu * gm)
+    return np.stack([2 * rho * vp ** 2 * gl, mu * (2 * gm - 4 * gl), third + rho * gr])
 
 
 def rtm_windows(p, true, background, windows, records=None, z2=-1, prec=64, ex_times=None):

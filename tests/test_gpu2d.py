@@ -135,6 +135,88 @@ def test_gpu2d_born_and_adjoint_match_oracle(erm, oracle2d_mod, L, para):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("L,R", [(0, 4), (1, 4), (3, 4), (1, 6)])
+def test_gpu2d_reference_adjoint_matches_oracle(erm, oracle2d_mod, L, R):
+    """The reference's continuous adjoint and gradient (erm2d_adjoint_ref:
+    adjoint_fwd_vx/vz_viscoelastic, adjoint_fwd_memory_new,
+    cal_gradient_in_viscoelastic_media_new) against the FP64 oracle, and
+    adjoint_mode = 1 (erm2d_adjoint with the receiver scaling, density sign
+    and parameter map) against its oracle restatement."""
+    import test_oracle2d as T
+    p, bg = T._lsrtm_case(L, R=R, lt=200)
+    p.inversion_para = 2
+    rng = np.random.default_rng(6)
+    r = rng.standard_normal((2, p.lt, p.nrec)).astype(np.float32)
+    _say(f"2D reference-mode adjoint parity L={L} R={R}")
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, *bg)
+    og = o.adjoint_ref(r)
+    with erm.Elastic2D(p) as g:
+        g.load_model(1, *bg)
+        gr = g.adjoint_ref(r)
+    for c in range(3):
+        e = cases.rel_l2(gr[c], og[c])
+        assert e < 1e-4, (c, e)
+    p.adjoint_mode = 1
+    om = oracle2d_mod.ref_adjoint_mapped(o, bg, r)
+    with erm.Elastic2D(p) as g:
+        g.load_model(1, *bg)
+        gm = g.adjoint(r)
+    for c in range(3):
+        e = cases.rel_l2(gm[c], om[c])
+        assert e < 1e-4, ("mapped", c, e)
+
+
+@pytest.mark.gpu
+def test_gpu2d_reference_adjoint_batched_and_cg(erm):
+    """adjoint_mode = 1 through the batched operator equals the one-shot
+    operator summed over shots, and CG-LSRTM runs on it with a non-increasing
+    misfit (the flexible update's exact line search)."""
+    import test_oracle2d as T
+    p, bg = T._lsrtm_case(1, lt=160)
+    p.adjoint_mode = 1
+    rng = np.random.default_rng(9)
+    shots = np.array([[8, 3], [20, 3], [31, 3]], np.int32)
+    r = rng.standard_normal((len(shots), 2, p.lt, p.nrec)).astype(np.float32)
+    _say("2D reference-mode adjoint: batched vs one-shot, CG")
+    with erm.Elastic2D(p) as g:
+        g.load_model(1, *bg)
+        one = g.adjoint_shots(shots, r)
+        g.set_batch(3)
+        bat = g.adjoint_shots(shots, r)
+        assert cases.rel_l2(bat, one) < 1e-5
+        dm = (rng.standard_normal((3, p.nx, p.nz)) * cases2d.taper_mask(p)).astype(np.float32)
+        d = g.born_shots(shots, dm)
+        m, misfit = g.lsrtm_cg(shots, d, 6)[:2]
+    assert np.all(np.diff(misfit) <= 1e-6) and misfit[-1] < 0.9 * misfit[0], misfit
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("strategy,L,R,boundary", [(0, 0, 4, 0), (1, 0, 4, 0), (2, 0, 4, 0), (0, 1, 4, 0),
+                                                   (1, 1, 4, 0), (2, 0, 6, 0), (0, 0, 4, 1), (2, 0, 4, 1)])
+def test_gpu2d_rtm_xcorr_matches_oracle(erm, oracle2d_mod, strategy, L, R, boundary):
+    """Cross-correlation RTM (erm2d_rtm_xcorr) with each source-wavefield
+    strategy (history, reversed propagation, boundary strips) against the FP64
+    oracle: PP, PS, PPd, PSc and the polarity-corrected PSc."""
+    import test_oracle2d as T
+    p, m = T._xcorr_case(L=L, R=R, boundary=boundary)
+    rec = T._xcorr_records(oracle2d_mod, p, m)
+    bg = T._smooth_bg(m)
+    _say(f"2D xcorr RTM parity strategy={strategy} L={L} R={R} boundary={boundary}")
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, *bg)
+    ref = o.rtm_xcorr(rec, strategy)
+    with erm.Elastic2D(p) as g:
+        g.load_model(1, *bg)
+        img = g.rtm_xcorr(rec, strategy)
+        again = g.rtm_xcorr(rec, strategy)
+    np.testing.assert_array_equal(img, again)
+    for k in range(5):
+        e = cases.rel_l2(img[k], ref[k])
+        assert e < (1e-4 if k < 4 else 1e-3), (k, e)
+
+
+@pytest.mark.gpu
 def test_gpu2d_q_compensated_rtm_matches_oracle(erm, oracle2d_mod):
     """Viscoelastic RTM with the Q-compensated receiver pass (qcomp = 1) on
     the GPU against the FP64 oracle: receiver wavefield and images."""

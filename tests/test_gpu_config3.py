@@ -17,6 +17,7 @@ axis cut to 200 steps per pass.
 """
 import os
 import sys
+import time
 
 import numpy as np
 import pytest
@@ -31,6 +32,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.timeout(600)
 def test_gpu_config3_batch_layout(erm):
+    t0 = time.time()
     p, true, mig, wins = config3.batch_inputs(erm)
     print(f"config-3 batch: {len(wins)} windows over {p.nx}x{p.ny}x{p.nz}, nt={p.lt}", flush=True)
     expect = [np.zeros((p.nx, p.ny, p.nz), np.float32) for _ in range(3)]
@@ -40,7 +42,9 @@ def test_gpu_config3_batch_layout(erm):
         b.load_model(1, *mig)
         b.set_shots(wins)
         b.set_launch(8, 6, 0, tile_x=32)
+        print(f"  [{time.time() - t0:.0f} s] models loaded, shots set", flush=True)
         stacked = b.run()
+        print(f"  [{time.time() - t0:.0f} s] batched run done", flush=True)
         largest = max(range(len(wins)), key=lambda s: wins[s][5] * wins[s][6])
         alone = {config3.WINDOW, largest, 0, 8, 15, 23}
         for s, w in enumerate(wins):
@@ -57,7 +61,7 @@ def test_gpu_config3_batch_layout(erm):
                 err = config2.compare(fx, q, config2.model_region(et_b, q),
                                       config2.model_region(v.get_field("ex_amp"), q), imgs_b,
                                       v.get_field("vz")[:, et_b.shape[1] // 2, :])
-                print(f"window {s} vs FP64: {err}", flush=True)
+                print(f"  [{time.time() - t0:.0f} s] window {s} vs FP64: {err}", flush=True)
                 assert err["ex_time_mismatch_frac"] < 0.02, err
                 assert err["cells_compared"] > 2000, err
                 for k in ("img_tp_rel_l2", "img_pp_rel_l2", "img_ps_rel_l2"):
@@ -76,6 +80,6 @@ def test_gpu_config3_batch_layout(erm):
             np.testing.assert_array_equal(et_b, et)
             for k in range(3):
                 np.testing.assert_array_equal(imgs_b[k], imgs[k])
-            print(f"window {s} {w}: bit-identical to its one-shot migration", flush=True)
+            print(f"  [{time.time() - t0:.0f} s] window {s} {w}: bit-identical to its one-shot migration", flush=True)
     for k in range(3):
         np.testing.assert_allclose(stacked[k], expect[k], rtol=1e-6, atol=1e-6 * np.abs(expect[k]).max())

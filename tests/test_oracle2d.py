@@ -615,3 +615,264 @@ def test_rtm_windows_composition(oracle2d_mod):
     for k in range(4):
         np.testing.assert_array_equal(cut[k], ref[k])
     assert not np.allclose(cut[1], each[0][1])
+
+
+# ------------------------------------------------------- reference-mode adjoint
+@pytest.mark.parametrize("L,R", [(0, 4), (0, 6), (1, 4), (1, 6), (3, 4)])
+def test_ref_adjoint_propagation_is_the_forward_system(oracle2d_mod, L, R):
+    """The reference's adjoint velocities (adjoint_fwd_vx/vz_viscoelastic,
+    viscoelastic_equation.cu:1923-2120) and memory (adjoint_fwd_memory_new,
+    :2190-2196) with the strain-increment stress update, in a homogeneous
+    medium without damping, are the forward viscoelastic system of a medium
+    with density 1 and the same vp, vs, Q: starting both from the same
+    velocities (zero stresses and memory), the adjoint velocities equal the
+    forward ones and the moduli-weighted adjoint stresses
+    Mp s_xx + (Mp-2Ms) s_zz + sum_l P_l r_xx + (P_l-2S_l) r_zz (and the zz, xz
+    counterparts) equal the forward stresses at every step, to roundoff
+    (derivation: S = C s + K r, Q_l = K_l (s - r_l/tau_l)/(1 - dt/tau_l) follow
+    the forward recurrences exactly).  This pins the adjoint stencils' D+/D-
+    directions and the placement of Mp, Ms, P_l, S_l against the forward
+    oracle pinned above."""
+    p = cases2d.params(nx=40, nz=36, b=10, R=R, L=L, lt=400, coe=0.0)
+    vp = np.full((p.nx, p.nz), 2400.0, np.float32)
+    vs = (vp / 1.732).astype(np.float32)
+    rho = np.full_like(vp, 2.3)
+    qp = np.full_like(vp, 15.0) if L else None
+    qs = (0.6 * qp).astype(np.float32) if L else None
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, vp, vs, rho, qp, qs)
+    nnx, nnz = o.shape
+    rng = np.random.default_rng(3)
+    vx, vz = np.zeros((nnx, nnz)), np.zeros((nnx, nnz))
+    vx[R:-R, R:-R] = rng.standard_normal((nnx - 2 * R, nnz - 2 * R))
+    vz[R:-R, R:-R] = rng.standard_normal((nnx - 2 * R, nnz - 2 * R))
+    f = oracle2d_mod.Oracle2D(p)
+    f.load_model(1, vp, vs, np.ones_like(rho), qp, qs)
+    f.set_field("vx", vx)
+    f.set_field("vz", vz)
+    f.step(1, 380, 25)  # past the wavelet: no source
+    av, S = o.ref_adjoint_steps(vx, vz, 25)
+    for a, name in ((av[0], "vx"), (av[1], "vz"), (S[0], "txx"), (S[1], "tzz"), (S[2], "txz")):
+        b = f.get_field(name)
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b), name
+    # the same check sees a 1% change of vp
+    g = oracle2d_mod.Oracle2D(p)
+    g.load_model(1, (1.01 * vp).astype(np.float32), vs, np.ones_like(rho), qp, qs)
+    g.set_field("vx", vx)
+    g.set_field("vz", vz)
+    g.step(1, 380, 25)
+    assert np.linalg.norm(av[0] - g.get_field("vx")) > 1e-3 * np.linalg.norm(av[0])
+
+
+@pytest.mark.parametrize("L,q", [(0, 0.0), (1, 5.0), (1, 40.0), (3, 5.0)])
+def test_ref_adjoint_gradient_against_born(oracle2d_mod, L, q):
+    """The reference gradient (cal_gradient_in_viscoelastic_media_new,
+    viscoelastic_equation.cu:2248-2295) of the adjoint driven by d = L dm
+    against the Born operator (cuda_cal_dem_parameter_viscoelastic_media_new,
+    :2398-2505, inversion_para 1), one parameter class at a time:
+    <L dm, L dm> = (rho_r / dt) <dm_abs, g> with dm_abs = (lambda m1, mu m2,
+    rho m3) and rho_r the density at the receivers.  The moduli classes agree
+    to 1% (measured 0.1-0.3%: the continuous adjoint is not the exact
+    transpose), so a dropped term, a wrong factor 2 in g_mu, a sign or a
+    lambda/mu swap fails; Q = 5 makes the memory terms (np, ns) a large part
+    of the gradient.  The density class comes out with the opposite sign: the
+    reference accumulates +rv.v_t while its Born source is -drho v_t (both
+    restated as shipped)."""
+    lt = 360
+    p = cases2d.params(nx=60, nz=48, b=16, R=4, L=L, lt=lt, coe=200.0, freq=12.0, src=(30, 4), rec_z=2)
+    p.inversion_para = 1
+    vp, vs, rho, qp, qs = cases2d.model(p, seed=2, contrast=False, q=L > 0)
+    vp = cases2d.smooth2(vp, 20)
+    vs = (vp / 1.732).astype(np.float32)
+    rho = np.full_like(vp, 1800.0)
+    if L:
+        qp = np.full_like(vp, q)
+        qs = (0.6 * qp).astype(np.float32)
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, vp, vs, rho, qp, qs)
+    rng = np.random.default_rng(5)
+    w = cases2d.taper_mask(p, 6)
+    base = cases2d.smooth2((rng.standard_normal((p.nx, p.nz)) * w).astype(np.float32), 3) * w
+    v64, s64, r64 = (a.astype(np.float64) for a in (vp, vs, rho))
+    scale = [r64 * (v64 ** 2 - 2 * s64 ** 2), r64 * s64 ** 2, r64]
+    ratios = []
+    for c in range(3):
+        dm = np.zeros((3, p.nx, p.nz), np.float32)
+        dm[c] = base
+        d = o.born(dm)
+        g = o.adjoint_ref(d).astype(np.float64)
+        lhs = float(np.sum(d.astype(np.float64) ** 2))
+        rhs = float(np.sum(dm[c] * scale[c] * g[c]))
+        ratios.append(lhs / rhs * 1800.0 / (p.dt * 1e-3))
+    assert abs(ratios[0] - 1) < 0.01 and abs(ratios[1] - 1) < 0.01, ratios
+    assert abs(ratios[2] + 1) < 0.01, ratios
+
+
+def test_ref_adjoint_mapped_is_an_approximate_transpose(oracle2d_mod):
+    """adjoint_mode 1 (the reference gradient with the receiver scaling, the
+    density sign and the parameter map; oracle2d.ref_adjoint_mapped) against
+    the Born operator for each parameterisation: <L dm, L dm> = <dm, g> to 1%
+    with dm in all three parameters at once."""
+    p = cases2d.params(nx=60, nz=48, b=16, R=4, L=1, lt=360, coe=200.0, freq=12.0, src=(30, 4), rec_z=2)
+    vp, vs, rho, qp, qs = cases2d.model(p, seed=2, contrast=False, q=True)
+    vp = cases2d.smooth2(vp, 20)
+    vs = (vp / 1.732).astype(np.float32)
+    rho = (230.0 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    rng = np.random.default_rng(8)
+    w = cases2d.taper_mask(p, 6)
+    dm = np.stack([cases2d.smooth2((rng.standard_normal((p.nx, p.nz)) * w).astype(np.float32), 3) * w
+                   for _ in range(3)]).astype(np.float32)
+    for para in (1, 2, 3):
+        p.inversion_para = para
+        o = oracle2d_mod.Oracle2D(p)
+        o.load_model(1, vp, vs, rho, qp, qs)
+        d = o.born(dm).astype(np.float64)
+        g = oracle2d_mod.ref_adjoint_mapped(o, (vp, vs, rho), d)
+        lhs, rhs = float(np.sum(d * d)), float(np.sum(dm.astype(np.float64) * g))
+        assert abs(rhs / lhs - 1) < 0.01, (para, lhs, rhs)
+
+
+# ------------------------------------------------ cross-correlation RTM strategies
+def _xcorr_case(L=0, R=4, coe=200.0, boundary=0, nx=48, nz=40, lt=300):
+    p = cases2d.params(nx=nx, nz=nz, b=14, R=R, L=L, lt=lt, coe=coe, freq=18.0, src=(nx // 2, 3), rec_z=2)
+    p.boundary = boundary
+    vp, vs, rho, qp, qs = cases2d.model(p, seed=4, contrast=True, q=L > 0)
+    return p, (vp, vs, rho, qp, qs)
+
+
+def _xcorr_records(oracle2d_mod, p, m):
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(0, *m)
+    o.step(0, 0, p.lt)
+    return o.record()
+
+
+def _smooth_bg(m):
+    return tuple(cases2d.smooth2(a, 8) if a is not None else None for a in m[:3]) + tuple(m[3:])
+
+
+@pytest.mark.parametrize("R,boundary", [(4, 0), (6, 0), (4, 1)])
+def test_xcorr_strips_reconstruct_the_history(oracle2d_mod, R, boundary):
+    """Boundary strips + reversed propagation rebuild the source wavefield of
+    the model region exactly (elastic): all five images equal those of the
+    stored history (damping and C-PML boundaries, both radii)."""
+    p, m = _xcorr_case(R=R, boundary=boundary)
+    rec = _xcorr_records(oracle2d_mod, p, m)
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, *_smooth_bg(m))
+    h = o.rtm_xcorr(rec, 0)
+    s = o.rtm_xcorr(rec, 2)
+    for k in range(5):
+        assert np.linalg.norm(s[k] - h[k]) <= 1e-5 * np.linalg.norm(h[k]), k
+
+
+def test_xcorr_reverse_is_exact_without_absorption(oracle2d_mod):
+    """The reversed step (rfwd_txxzzxzpp_viscoelastic_and_memory_old: derivative
+    and memory terms negated, the wavelet taken off where it was added) is the
+    exact inverse of the forward step where nothing is damped: with no
+    absorbing boundary at all the reversal-only strategy equals the history;
+    with the damping boundary it does not (the boundary keeps absorbing)."""
+    p, m = _xcorr_case(coe=0.0)
+    rec = _xcorr_records(oracle2d_mod, p, m)
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, *_smooth_bg(m))
+    h, r = o.rtm_xcorr(rec, 0), o.rtm_xcorr(rec, 1)
+    for k in range(4):
+        assert np.linalg.norm(r[k] - h[k]) <= 1e-5 * np.linalg.norm(h[k]), k
+    p2, m2 = _xcorr_case(coe=200.0)
+    rec2 = _xcorr_records(oracle2d_mod, p2, m2)
+    o2 = oracle2d_mod.Oracle2D(p2)
+    o2.load_model(1, *_smooth_bg(m2))
+    h2, r2 = o2.rtm_xcorr(rec2, 0), o2.rtm_xcorr(rec2, 1)
+    assert np.linalg.norm(r2[0] - h2[0]) > 1e-2 * np.linalg.norm(h2[0])
+
+
+def _window_counts(vote, interior):
+    """9x9 window counts of each vote class over interior cells (numpy)."""
+    out = []
+    for cls in (1, 2, 3):
+        a = np.pad(((vote == cls) & interior).astype(np.int64), 4)
+        c = np.cumsum(np.cumsum(a, 0), 1)
+        c = np.pad(c, ((1, 0), (1, 0)))
+        nx, nz = vote.shape
+        out.append(c[9:9 + nx, 9:9 + nz] - c[:nx, 9:9 + nz] - c[9:9 + nx, :nz] + c[:nx, :nz])
+    return out
+
+
+def test_xcorr_images_match_snapshot_correlation(oracle2d_mod):
+    """The five images against a numpy restatement built from per-step
+    snapshots of a source and a receiver propagation through the pinned
+    stepping API (pins the time alignment, the model-region masking of the
+    source fields, the vote classes, the 9x9 window and the tie rules)."""
+    p, m = _xcorr_case(nx=36, nz=30, lt=160)
+    rec = _xcorr_records(oracle2d_mod, p, m)
+    bg = _smooth_bg(m)
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, *bg)
+    got = o.rtm_xcorr(rec, 0)
+    R, sl = 4, (slice(p.bl, p.bl + p.nx), slice(p.bu, p.bu + p.nz))
+    src = oracle2d_mod.Oracle2D(p)
+    src.load_model(1, *bg)
+    hist = []
+    for it in range(p.lt):
+        src.step(1, it, 1)
+        hist.append({k: src.get_field(k) for k in ("vxp", "vzp", "div")})
+    rcv = oracle2d_mod.Oracle2D(p)
+    rcv.load_model(1, *bg)
+    rcv.record(rec)
+    nnx, nnz = rcv.shape
+    interior = np.zeros((nnx, nnz), bool)
+    interior[R:-R, R:-R] = True
+    model = np.zeros((nnx, nnz), bool)
+    model[sl] = True
+    img = np.zeros((5, nnx, nnz))
+    sign = np.zeros((nnx, nnz))
+    for it in range(p.lt - 1, -1, -1):
+        rcv.step(2, it, 1)
+        f = {k: rcv.get_field(k) for k in ("vx", "vz", "vxp", "vzp", "div", "curl")}
+        s = {k: np.where(model, v, 0.0) for k, v in hist[it].items()}
+        rxs, rzs = f["vx"] - f["vxp"], f["vz"] - f["vzp"]
+        dot = s["vxp"] * rxs + s["vzp"] * rzs
+        flip = np.where(s["vzp"] >= 0, np.where(rxs >= 0, -1.0, 1.0), np.where(rxs > 0, 1.0, np.where(rxs < 0, -1.0, 0.0)))
+        st = flip * dot
+        vote = np.where(st > 0, 1, np.where(st < 0, 2, 3))
+        c1, c2, c3 = _window_counts(vote, interior)
+        new = np.where((c1 >= c2) & (c1 >= c3), 1.0, np.where((c2 > c1) & (c2 > c3), -1.0,
+                                                              np.where((c3 > c1) & (c3 > c2), 0.0, sign)))
+        sign = np.where(interior, new, sign)
+        prod = s["div"] * f["curl"]
+        img[0] += s["vxp"] * f["vxp"] + s["vzp"] * f["vzp"]
+        img[1] += dot
+        img[2] += s["div"] * f["div"]
+        img[3] += prod
+        img[4] += sign * prod
+    for k in range(5):
+        ref = img[k][sl]
+        assert np.linalg.norm(got[k] - ref) <= 1e-5 * np.linalg.norm(ref), k
+
+
+def test_xcorr_polarity_correction_makes_ps_mirror_symmetric(oracle2d_mod):
+    """In a model symmetric about the source (layers and a centred anomaly,
+    receivers symmetric), the div-curl PS image reverses polarity across the
+    source (mirror correlation < -0.8) and the polarity-corrected image does
+    not (> +0.8), as the PP images (> +0.8); a sign rule that ignored the
+    side of the source would leave the corrected image antisymmetric."""
+    nx, nz = 81, 60
+    p = cases2d.params(nx=nx, nz=nz, b=16, R=4, L=0, lt=500, coe=200.0, freq=15.0, src=(40, 3), rec_z=2,
+                       rec_dx=2, nrec=40)
+    x, z = np.arange(nx)[:, None], np.arange(nz)[None, :]
+    vp = (np.where(z < 30, 2000.0, 2700.0) + 300 * np.exp(-((x - 40) ** 2 + (z - 45) ** 2) / 50.0)).astype(np.float32)
+    vs = (vp / 1.732).astype(np.float32)
+    rho = (230 * vp.astype(np.float64) ** 0.25).astype(np.float32)
+    rec = _xcorr_records(oracle2d_mod, p, (vp, vs, rho, None, None))
+    o = oracle2d_mod.Oracle2D(p)
+    o.load_model(1, *[cases2d.smooth2(a, 8) for a in (vp, vs, rho)])
+    img = o.rtm_xcorr(rec, 0)
+
+    def mirror_corr(a):
+        a = a[:, 5:].astype(np.float64)
+        return float(np.sum(a * a[::-1]) / np.sum(a * a))
+
+    assert mirror_corr(img[0]) > 0.8 and mirror_corr(img[2]) > 0.8
+    assert mirror_corr(img[3]) < -0.8, mirror_corr(img[3])
+    assert mirror_corr(img[4]) > 0.8, mirror_corr(img[4])
