@@ -80,6 +80,9 @@ def _load():
         "erm_strip_bytes": (ctypes.c_int, [vp, P(ctypes.c_double)]),
         "erm_forward_smoothed_strips": (ctypes.c_int, [vp]),
         "erm_backward_smoothed_xcorr": (ctypes.c_int, [vp, fp]),
+        "erm_history_bytes": (ctypes.c_int, [vp, ctypes.c_int, P(ctypes.c_double)]),
+        "erm_forward_smoothed_history": (ctypes.c_int, [vp, ctypes.c_int]),
+        "erm_backward_smoothed_xcorr_history": (ctypes.c_int, [vp, fp]),
         "erm_get_xcorr_images": (ctypes.c_int, [vp, fp, fp, fp]),
         "erm_reconstruct": (ctypes.c_int, [vp, ctypes.c_int]),
         "erm_get_reconstructed_field": (ctypes.c_int, [vp, ctypes.c_int, fp]),
@@ -374,6 +377,20 @@ class RTM3D:
     def backward_smoothed_xcorr(self, rec=None):
         """Receiver pass with source reconstruction and cross-correlation imaging."""
         self._c(lib.erm_backward_smoothed_xcorr(self._h, _fptr(None if rec is None else np.ascontiguousarray(rec, np.float32))))
+
+    def history_bytes(self, decim):
+        b = ctypes.c_double()
+        self._c(lib.erm_history_bytes(self._h, decim, ctypes.byref(b)))
+        return b.value
+
+    def forward_smoothed_history(self, decim=1):
+        """Source pass storing tp and vP of the model region every decim steps."""
+        self._c(lib.erm_forward_smoothed_history(self._h, decim))
+
+    def backward_smoothed_xcorr_history(self, rec=None):
+        """Receiver pass with cross-correlation imaging against the stored source steps."""
+        self._c(lib.erm_backward_smoothed_xcorr_history(
+            self._h, _fptr(None if rec is None else np.ascontiguousarray(rec, np.float32))))
 
     def get_xcorr_images(self):
         p = self.params

@@ -74,6 +74,10 @@ struct erm_ctx {
     bool have_strips = false;
     float* wavelet_neg = nullptr;    // -wavelet, for the time-reversed source
     float* ximg[3] = {nullptr};      // cross-correlation images tp, pp, ps (padded grid)
+    float* hist = nullptr;           // decimated source history: [slots][4][nx*ny*nz] tp, vxp, vyp, vzp of the model region
+    size_t hist_n = 0;               // floats allocated
+    int hist_decim = 0;              // steps per slot (0: none)
+    bool have_hist = false;
     float* decbuf = nullptr;         // one divergence / curl volume (allocated on first use)
     double vmax_loaded = 0.0;
     double pml_extra_stress = 0.0, pml_extra_velocity = 0.0;  // avg extra bytes per cell-update

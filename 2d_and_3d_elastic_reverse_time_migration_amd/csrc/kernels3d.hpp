@@ -883,6 +883,34 @@ struct StripFields {
     ips[n] += sx * (r.vx[n] - rx) + sy * (r.vy[n] - ry) + sz * (r.vz[n] - rz);
 }
 
+// Decimated source history (erm_forward_smoothed_history): tp, vxp, vyp, vzp
+// of the model region (i fastest) after a source step -> h [4][nx*ny*nz].
+[[maybe_unused]] static __global__ void k_hist_save(Wave s, float* __restrict__ h, int nx, int ny, int nz, int ox,
+                                                    int oy, int oz, int px, long pxy) {
+    const long nm = (long)nx * ny * nz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nm; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((long)nx * ny));
+        const long n = (long)(k + oz) * pxy + (long)(j + oy) * px + (i + ox);
+        h[t] = s.tp[n]; h[nm + t] = s.vxp[n]; h[2 * nm + t] = s.vyp[n]; h[3 * nm + t] = s.vzp[n];
+    }
+}
+// Zero-lag cross-correlation with a stored source step (k_xcorr with the
+// source read from h), each product weighted by w (the decimation factor).
+[[maybe_unused]] static __global__ void k_xcorr_hist(const float* __restrict__ h, Wave r, float* __restrict__ itp,
+                                                     float* __restrict__ ipp, float* __restrict__ ips, int nx, int ny,
+                                                     int nz, int ox, int oy, int oz, int px, long pxy, float w) {
+    const long nm = (long)nx * ny * nz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nm; t += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((long)nx * ny));
+        const long n = (long)(k + oz) * pxy + (long)(j + oy) * px + (i + ox);
+        const float st = h[t], sx = h[nm + t], sy = h[2 * nm + t], sz = h[3 * nm + t];
+        const float rx = r.vxp[n], ry = r.vyp[n], rz = r.vzp[n];
+        itp[n] += w * (st * r.tp[n]);
+        ipp[n] += w * (sx * rx + sy * ry + sz * rz);
+        ips[n] += w * (sx * (r.vx[n] - rx) + sy * (r.vy[n] - ry) + sz * (r.vz[n] - rz));
+    }
+}
+
 // it += delta (one lane; a plain vector store).
 [[maybe_unused]] static __global__ void k_step_advance(int* itp, int delta) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *itp += delta;

@@ -475,7 +475,7 @@ void free_all(erm_ctx* c) {
     F(c->psi_buf);
     for (auto& t : c->tiles) F(t);
     for (auto& q : c->fr) vfree(c, q);
-    F(c->strip_idx); F(c->strips); F(c->wavelet_neg);
+    F(c->strip_idx); F(c->strips); F(c->wavelet_neg); F(c->hist);
     for (auto& q : c->ximg) vfree(c, q);
     vfree(c, c->decbuf);
     for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
@@ -590,7 +590,7 @@ int finish_snapshots(erm_ctx* c, int pass) {
     return 0;
 }
 
-int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false) {
+int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false, bool hist = false) {
     if (pass < ERM_PASS_MODEL || pass > ERM_PASS_RECEIVER) return c->fail(ERM_E_PARAM, "unknown pass %d", pass);
     const int which = pass == ERM_PASS_MODEL ? 0 : 1;
     if (!c->have_model[which]) return c->fail(ERM_E_STATE, "%s model not loaded", which ? "smoothed" : "true");
@@ -627,6 +627,18 @@ int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false
         do_velocity(c, which, pass == ERM_PASS_SOURCE ? MODE_EXCITE : pass == ERM_PASS_RECEIVER ? MODE_IMAGE : MODE_PLAIN, it);
         if (c->profiling) HIPC(c, hipEventRecord(c->kev[3 * s + 2], c->stream));
         if (strips && pass == ERM_PASS_SOURCE) strip_velocity(c, c->f, it + 1, 0);
+        if (hist && (it + 1) % c->hist_decim == 0) {  // a stored step of the decimated history
+            const erm_params& P = c->p;
+            const long nm = (long)P.nx * P.ny * P.nz;
+            const float* slot = c->hist + (size_t)(it / c->hist_decim) * 4 * nm;
+            const dim3 hg((unsigned)std::min<long>((nm + 255) / 256, 16384));
+            if (pass == ERM_PASS_SOURCE)
+                hipLaunchKernelGGL(k_hist_save, hg, dim3(256), 0, c->stream, make_wave(c), (float*)slot, P.nx, P.ny, P.nz,
+                                   P.bl, P.bf, P.bu, c->px, c->pxy);
+            else if (pass == ERM_PASS_RECEIVER)
+                hipLaunchKernelGGL(k_xcorr_hist, hg, dim3(256), 0, c->stream, slot, make_wave(c), c->ximg[0], c->ximg[1],
+                                   c->ximg[2], P.nx, P.ny, P.nz, P.bl, P.bf, P.bu, c->px, c->pxy, (float)c->hist_decim);
+        }
         if (strips && pass == ERM_PASS_RECEIVER) {
             // source state {sigma^{it+1/2}, v^{it+1}}: complete the slab cells, image, step back
             strip_stress(c, c->fr, it, 1);
@@ -1328,6 +1340,53 @@ int erm_get_reconstructed_field(erm_ctx* c, int field, float* dst) {
     if (!c || !dst || field < 0 || field >= 13 || !c->fr[field]) return c ? c->fail(ERM_E_PARAM, "bad field") : ERM_E_PARAM;
     HIPC(c, hipSetDevice(c->device));
     return download(c, c->fr[field], dst);
+}
+
+int erm_history_bytes(const erm_ctx* c, int decim, double* bytes) {
+    if (!c || !bytes || decim < 1) return ERM_E_PARAM;
+    const erm_params& p = c->p;
+    *bytes = 4.0 * 4.0 * (double)p.nx * p.ny * p.nz * (double)(p.lt / decim);
+    return 0;
+}
+
+int erm_forward_smoothed_history(erm_ctx* c, int decim) {
+    if (!c) return ERM_E_PARAM;
+    if (decim < 1) return c->fail(ERM_E_PARAM, "decim must be >= 1");
+    HIPC(c, hipSetDevice(c->device));
+    if (c->p.lt_forward != c->p.lt) return c->fail(ERM_E_PARAM, "the source history needs lt_forward == lt");
+    const size_t nm = (size_t)c->p.nx * c->p.ny * c->p.nz, need = (size_t)(c->p.lt / decim) * 4 * nm;
+    if (need == 0) return c->fail(ERM_E_PARAM, "decim larger than lt");
+    if (c->hist_n < need) {
+        if (c->hist) (void)hipFree(c->hist);
+        c->hist = nullptr;
+        c->hist_n = 0;
+        size_t fr = 0, tot = 0;
+        HIPC(c, hipMemGetInfo(&fr, &tot));
+        if (need * 4 + (64UL << 20) > fr)
+            return c->fail(ERM_E_NOMEM, "the source history needs %.1f GB, %.1f GB free (raise decim)", need * 4e-9, fr * 1e-9);
+        if (dmalloc(c, &c->hist, need)) return ERM_E_NOMEM;
+        c->hist_n = need;
+    }
+    c->hist_decim = decim;
+    c->have_hist = false;
+    int rc;
+    if ((rc = erm_zero_wavefield(c)) || (rc = erm_zero_excitation(c))) return rc;
+    if ((rc = run_pass(c, ERM_PASS_SOURCE, 0, c->p.lt, false, true))) return rc;
+    c->have_hist = true;
+    return 0;
+}
+
+int erm_backward_smoothed_xcorr_history(erm_ctx* c, const float* rec) {
+    if (!c) return ERM_E_PARAM;
+    HIPC(c, hipSetDevice(c->device));
+    if (!c->have_hist) return c->fail(ERM_E_STATE, "no source history: run erm_forward_smoothed_history first");
+    int rc;
+    if (rec && (rc = erm_set_record(c, rec))) return rc;
+    for (auto& q : c->ximg) if (!q && vmalloc(c, &q, c->ncell)) return ERM_E_NOMEM;
+    if ((rc = erm_zero_wavefield(c))) return rc;
+    for (auto& q : c->img) if ((rc = fill(c, q, c->ncell, 0.f))) return rc;
+    for (auto& q : c->ximg) if ((rc = fill(c, q, c->ncell, 0.f))) return rc;
+    return run_pass(c, ERM_PASS_RECEIVER, c->p.lt - 1, c->p.lt, false, true);
 }
 
 int erm_backward_smoothed_xcorr(erm_ctx* c, const float* rec) {

@@ -412,6 +412,60 @@ def run_rtm2d(erm, args):
     print(json.dumps(line), flush=True)
 
 
+def run_rtm2d_xcorr(erm, args):
+    """2D elastic RTM of the same shot with cross-correlation imaging, one
+    line per source-wavefield strategy (erm2d_rtm_xcorr: stored history,
+    reversed propagation, boundary strips + reversal): GPU time per shot
+    (source pass + receiver pass with imaging; the model pass runs once
+    beforehand), the HBM the strategy holds, and each image's error against
+    the FP64 oracle's history images."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle2d
+    p, true, mig = rtm2d_case(erm)
+    nt = p.lt
+    g = erm.Elastic2D(p)
+    g.load_model(0, *true)
+    g.load_model(1, *mig)
+    rec = g.forward()
+    o = oracle2d.Oracle2D(p, 64)
+    o.load_model(1, *mig)
+    t0 = time.perf_counter()
+    ref = o.rtm_xcorr(rec, 0)
+    log(f"bench rtm2d xcorr: FP64 oracle history images in {time.perf_counter() - t0:.1f} s")
+    names = ("pp", "ps", "ppd", "psc", "psc_polarity")
+    mn = p.nx * p.nz
+    ring = sum(1 for i in range(max(p.bl - 4, 0), min(p.bl + p.nx + 4, p.padded_shape[0]))
+               for k in range(max(p.bu - 4, 0), min(p.bu + p.nz + 4, p.padded_shape[1]))
+               if not (p.bl + 1 <= i < p.bl + p.nx - 1 and p.bu + 1 <= k < p.bu + p.nz - 1))
+    hbm = {0: 4.0 * nt * 3 * mn, 1: 0.0, 2: 4.0 * nt * 8 * ring}
+    res = {}
+    for strat, label in ((0, "history"), (1, "reverse"), (2, "strips")):
+        g.rtm_xcorr(rec, strat)  # warm-up (allocations)
+        times = []
+        for _ in range(max(args.steps, 1)):
+            t0 = time.perf_counter()
+            img = g.rtm_xcorr(rec, strat)
+            times.append(time.perf_counter() - t0)
+        err = {n: round(float(np.linalg.norm(img[k] - ref[k]) / np.linalg.norm(ref[k])), 7) for k, n in enumerate(names)}
+        res[label] = {"s_per_shot": round(float(np.median(times)), 4), "hbm_GB": round(hbm[strat] / 1e9, 3),
+                      "rel_l2_vs_fp64_history": err}
+        log(f"bench rtm2d xcorr: {label}: {res[label]}")
+    g.close()
+    line = {
+        "metric": "2D elastic RTM with cross-correlation imaging, one shot (s per shot), by source-wavefield strategy",
+        "value": res["history"]["s_per_shot"], "unit": "s/shot", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": round(res["history"]["s_per_shot"] * 1e3, 2), "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (three flat layers + Gaussian anomaly)",
+        "config": {"workload": "2D elastic RTM, one shot: 400x200 model + 40-cell C-PML (480x280), d=10 m, 8th-order "
+                               "staggered FD, dt=1 ms, nt=2000, 20 Hz Ricker, 400 surface receivers; step = source "
+                               "pass + receiver pass with cross-correlation imaging (PP, PS, div-div, div-curl, "
+                               "div-curl with the PS polarity correction)",
+                   "strategies": res, "parallelism": "single GPU"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_visco(erm, args):
     """2D viscoelastic LSRTM with plain CG: 600x300 model, d = 8 m, 3 SLS
     mechanisms, 60 shots, 600 receivers, 20 Hz, nt = 3000 (dt 0.8 ms), 40-cell
@@ -522,7 +576,9 @@ def main():
     ap.add_argument("--tile-y", type=int, default=0, help="launch shape override (0: library default)")
     ap.add_argument("--zchunk", type=int, default=0, help="z planes per workgroup (0: library default)")
     ap.add_argument("--boundary", choices=["cpml", "damp"], default="cpml")
-    ap.add_argument("--imaging", choices=["excitation", "xcorr"], default="excitation",
+    ap.add_argument("--decim", type=int, default=4,
+                    help="--imaging history: source steps per stored history slot")
+    ap.add_argument("--imaging", choices=["excitation", "xcorr", "history"], default="excitation",
                     help="shot workload: excitation-amplitude imaging (default), or zero-lag cross-correlation "
                          "with the source wavefield reconstructed backward from boundary strips")
     ap.add_argument("--cpml-split", type=int, choices=[0, 1], default=0,
@@ -563,7 +619,7 @@ def main():
     if args.workload == "learned":
         return run_learned(erm, args)
     if args.workload == "rtm2d":
-        return run_rtm2d(erm, args)
+        return run_rtm2d_xcorr(erm, args) if args.imaging == "xcorr" else run_rtm2d(erm, args)
     nt = args.nt
     p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
     p.tile_x, p.tile_y, p.zchunk = args.tile_x, args.tile_y, args.zchunk
@@ -592,12 +648,18 @@ def main():
     sb, vb_src = g.algorithmic_bytes(1)
     _, vb_rcv = g.algorithmic_bytes(2)
 
-    xc = args.imaging == "xcorr"
-    fwd, bwd = (g.forward_smoothed_strips, g.backward_smoothed_xcorr) if xc else (g.forward_smoothed,
-                                                                                 g.backward_smoothed)
-    strip_gb = g.strip_bytes() / 1e9 if xc else None
-    if xc:
+    xc = args.imaging != "excitation"
+    if args.imaging == "xcorr":
+        fwd, bwd = g.forward_smoothed_strips, g.backward_smoothed_xcorr
+        strip_gb = g.strip_bytes() / 1e9
         log(f"bench: cross-correlation imaging, boundary strips {strip_gb:.1f} GB in HBM")
+    elif args.imaging == "history":
+        fwd, bwd = (lambda: g.forward_smoothed_history(args.decim)), g.backward_smoothed_xcorr_history
+        strip_gb = g.history_bytes(args.decim) / 1e9
+        log(f"bench: cross-correlation imaging, source history every {args.decim} steps, {strip_gb:.1f} GB in HBM")
+    else:
+        fwd, bwd = g.forward_smoothed, g.backward_smoothed
+        strip_gb = None
     for w in range(args.warmup):
         fwd()
         bwd()
@@ -684,9 +746,14 @@ def main():
             lsrtm = {"error": repr(e)}
         log(f"bench: LSRTM learned vs plain: {lsrtm}")
 
-    imaging = ("excitation-amplitude imaging" if not xc else
-               "zero-lag cross-correlation imaging, source wavefield reconstructed backward from %.0f GB of "
-               "boundary strips (the reconstruction's extra steps are not counted as updates)" % strip_gb)
+    if args.imaging == "xcorr":
+        imaging = ("zero-lag cross-correlation imaging, source wavefield reconstructed backward from %.0f GB of "
+                   "boundary strips (the reconstruction's extra steps are not counted as updates)" % strip_gb)
+    elif args.imaging == "history":
+        imaging = ("zero-lag cross-correlation imaging against a source history of tp and vP stored every %d "
+                   "steps (%.0f GB in HBM)" % (args.decim, strip_gb))
+    else:
+        imaging = "excitation-amplitude imaging"
     line = {
         "metric": "Gcell-updates/s 3D elastic RTM; LSRTM s to 1e-2 misfit, learned vs plain",
         "value": round(value, 3),

@@ -239,6 +239,20 @@ int   erm_strip_bytes(const erm_ctx* ctx, double* bytes);
 int   erm_forward_smoothed_strips(erm_ctx* ctx);
 int   erm_backward_smoothed_xcorr(erm_ctx* ctx, const float* rec);
 int   erm_get_xcorr_images(erm_ctx* ctx, float* tp, float* pp, float* ps);
+/* Cross-correlation imaging with a decimated source history (the reference's
+ * full-history option, save_all_wavefield / set_all_wavefield,
+ * 2D_elastic_RTM/elastic_2D_kernel_2.cu:793-827, in 3D and thinned in time):
+ * erm_forward_smoothed_history runs the source pass (excitation kept) and
+ * stores tp, vxp, vyp, vzp of the model region after every step it with
+ * (it + 1) % decim == 0, in HBM (erm_history_bytes: 16 B x nx*ny*nz x
+ * floor(lt / decim)); erm_backward_smoothed_xcorr_history runs the receiver
+ * pass (excitation imaging as erm_backward_smoothed) and adds decim times the
+ * zero-lag products with the stored steps to the images of erm_get_xcorr_images
+ * (decim = 1: every step, the images of erm_backward_smoothed_xcorr).  No
+ * reconstruction, so no boundary-width condition; lt_forward == lt. */
+int   erm_history_bytes(const erm_ctx* ctx, int decim, double* bytes);
+int   erm_forward_smoothed_history(erm_ctx* ctx, int decim);
+int   erm_backward_smoothed_xcorr_history(erm_ctx* ctx, const float* rec);
 int   erm_reconstruct(erm_ctx* ctx, int it_end);
 int   erm_get_reconstructed_field(erm_ctx* ctx, int field, float* dst);
 /* Launch-shape autotuner (no reference counterpart; the reference launches

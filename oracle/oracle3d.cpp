@@ -646,7 +646,10 @@ int oracle3d_rtm_shot(void* h, int use_record) {
  *   I_tp += tp_s(it) tp_r,  I_PP += vP_s(it).vP_r,  I_PS += vP_s(it).(v_r - vP_r).
  * Outputs over the model region in disk order [nx][ny][nz]; requires
  * lt_forward == lt and a record (modelled here when use_record == 0). */
-int oracle3d_xcorr_shot(void* h, int use_record, double* itp, double* ipp, double* ips) {
+/* decim > 1: the products of the receiver steps it with (it + 1) % decim == 0
+ * only, each weighted by decim (the decimated history of
+ * erm_forward_smoothed_history) */
+int oracle3d_xcorr_shot_decim(void* h, int use_record, int decim, double* itp, double* ipp, double* ips) {
     return dispatch(h, [&](auto& o) {
         using T = typename std::remove_reference<decltype(o.rec[0])>::type;
         const erm_params& p = o.p;
@@ -677,18 +680,23 @@ int oracle3d_xcorr_shot(void* h, int use_record, double* itp, double* ipp, doubl
         o.zero_images();
         for (int it = p.lt - 1; it >= 0; --it) {
             if ((rc = o.step(ERM_PASS_RECEIVER, it, 1))) return rc;
+            if ((it + 1) % decim != 0) continue;
             const T* h = &hist[(size_t)it * 4 * nm];
+            const double w = decim;
             for (size_t q = 0; q < nm; ++q) {
                 const size_t n = cell(q);
                 const double sx = h[nm + q], sy = h[2 * nm + q], sz = h[3 * nm + q];
                 const double rx = o.f[10][n], ry = o.f[11][n], rz = o.f[12][n];
-                itp[q] += (double)h[q] * (double)o.f[9][n];
-                ipp[q] += sx * rx + sy * ry + sz * rz;
-                ips[q] += sx * ((double)o.f[0][n] - rx) + sy * ((double)o.f[1][n] - ry) + sz * ((double)o.f[2][n] - rz);
+                itp[q] += w * ((double)h[q] * (double)o.f[9][n]);
+                ipp[q] += w * (sx * rx + sy * ry + sz * rz);
+                ips[q] += w * (sx * ((double)o.f[0][n] - rx) + sy * ((double)o.f[1][n] - ry) + sz * ((double)o.f[2][n] - rz));
             }
         }
         return 0;
     });
+}
+int oracle3d_xcorr_shot(void* h, int use_record, double* itp, double* ipp, double* ips) {
+    return oracle3d_xcorr_shot_decim(h, use_record, 1, itp, ipp, ips);
 }
 
 }  // extern "C"

@@ -60,6 +60,7 @@ def _load():
         "oracle3d_rtm_shot": (ctypes.c_int, [vp, ctypes.c_int]),
         "oracle3d_image": (ctypes.c_int, [vp, ctypes.c_int]),
         "oracle3d_xcorr_shot": (ctypes.c_int, [vp, ctypes.c_int, dp, dp, dp]),
+        "oracle3d_xcorr_shot_decim": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -212,11 +213,13 @@ class Oracle3D:
     def rtm_shot(self, use_record=False):
         self._c(lib.oracle3d_rtm_shot(self._h, 1 if use_record else 0))
 
-    def xcorr_shot(self, use_record=False):
-        """Cross-correlation images (tp, PP, PS) with the full source history, [nx][ny][nz]."""
+    def xcorr_shot(self, use_record=False, decim=1):
+        """Cross-correlation images (tp, PP, PS) with the full source history,
+        [nx][ny][nz]; decim > 1 keeps the receiver steps with (it+1) % decim == 0,
+        each weighted by decim."""
         p = self.p
         out = [np.empty((p.nx, p.ny, p.nz), np.float64) for _ in range(3)]
-        self._c(lib.oracle3d_xcorr_shot(self._h, 1 if use_record else 0, *[_dp(o) for o in out]))
+        self._c(lib.oracle3d_xcorr_shot_decim(self._h, 1 if use_record else 0, decim, *[_dp(o) for o in out]))
         return out
 
     def images(self):

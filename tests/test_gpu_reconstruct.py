@@ -81,12 +81,55 @@ def test_xcorr_images_match_oracle(erm, oracle, boundary):
         assert e < 1e-4, (name, e)
 
 
+@pytest.mark.parametrize("boundary,decim", [(0, 1), (1, 1), (1, 3)])
+def test_xcorr_history_matches_strips_and_oracle(erm, oracle, boundary, decim):
+    """The decimated-history strategy: with decim = 1 its images equal the
+    boundary-strip reconstruction's (<= 1e-5) and, for any decim, the FP64
+    oracle's decimated cross-correlation (<= 1e-4); the excitation images of
+    its receiver pass are those of erm_backward_smoothed (bit-identical)."""
+    p, true, sm = _case(boundary)
+    with erm.RTM3D(p) as g:
+        g.load_model(0, *true)
+        g.load_model(1, *sm)
+        rec = g.forward_real()
+        g.forward_smoothed()
+        g.backward_smoothed()
+        ex_ref = g.get_images()
+        if decim == 1:
+            g.forward_smoothed_strips()
+            g.backward_smoothed_xcorr()
+            xs = g.get_xcorr_images()
+        assert g.history_bytes(decim) == 16.0 * p.nx * p.ny * p.nz * (p.lt // decim)
+        g.forward_smoothed_history(decim)
+        g.backward_smoothed_xcorr_history()
+        ex = g.get_images()
+        xh = g.get_xcorr_images()
+    for a, b in zip(ex, ex_ref):
+        np.testing.assert_array_equal(a, b)
+    if decim == 1:
+        for a, b in zip(xh, xs):
+            assert np.linalg.norm(a - b) <= 1e-5 * np.linalg.norm(b)
+    o = oracle.Oracle3D(p, 64)
+    o.load_model(0, *true)
+    o.load_model(1, *sm)
+    o.set_record(rec)
+    xo = o.xcorr_shot(use_record=True, decim=decim)
+    for name, a, b in zip(("tp", "pp", "ps"), xh, xo):
+        e = np.linalg.norm(a - b) / np.linalg.norm(b)
+        print(f"history decim {decim} {name}: rel L2 vs FP64 oracle {e:.2e}")
+        assert e < 1e-4, (name, e)
+
+
 def test_strip_errors(erm):
     p, true, sm = _case(0)
     with erm.RTM3D(p) as g:
         g.load_model(1, *sm)
         with pytest.raises(erm.ErmError):
             g.backward_smoothed_xcorr()          # no strips yet
+        with pytest.raises(erm.ErmError):
+            g.backward_smoothed_xcorr_history()  # no history yet
+        with pytest.raises(erm.ErmError):
+            g.forward_smoothed_history(0)
     p.lt_forward = p.lt - 10
     with erm.RTM3D(p) as g:
         g.load_model(1, *sm)
