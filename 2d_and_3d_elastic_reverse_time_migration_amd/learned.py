@@ -375,19 +375,31 @@ def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, c
         test_shape=(256, 96),
         test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, epochs=20, batch=32, lr=1e-3,
         channels=(16, 32, 64, 64), batch_shots=0, seed0=1000, test_seed=5000, warm=3, flexible=False, illum_eps=1e-3,
-        target="converged", per_pair=True, schedules=("alt2",), log=print):
-    """The whole workflow; returns a dict of results."""
+        target="converged", per_pair=True, schedules=("alt2",), log=print, params_in=None, params_out=None):
+    """The whole workflow; returns a dict of results.  params_out: save the
+    trained net's parameters (numpy .npy); params_in: skip data generation
+    and training and evaluate a net with these saved parameters."""
     res = {"config": dict(train_models=train_models, train_shape=train_shape, train_shots=train_shots,
                           train_lt=train_lt, cg_iters=cg_iters, keep=list(keep), test_shape=test_shape,
                           test_shots=test_shots,
                           test_lt=test_lt, tol=tol, epochs=epochs, batch=batch, lr=lr, channels=channels,
                           illum_eps=illum_eps, target=target, per_pair=per_pair)}
-    t0 = time.perf_counter()
-    pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, keep=keep, seed0=seed0,
-                     illum_eps=illum_eps, target=target, log=log)
-    res["datagen_s"] = time.perf_counter() - t0
-    net, st = train(pairs, channels, epochs, batch, lr, seed=seed0, per_pair=per_pair, log=log)
-    res.update(st)
+    if params_in:
+        net = UNet(channels, seed=seed0)
+        net.set_params(np.load(params_in, allow_pickle=False))
+        res["params_in"] = params_in
+    else:
+        t0 = time.perf_counter()
+        pairs = generate(train_models, train_shape, train_shots, train_lt, cg_iters, keep=keep, seed0=seed0,
+                         illum_eps=illum_eps, target=target, log=log)
+        res["datagen_s"] = time.perf_counter() - t0
+        net, st = train(pairs, channels, epochs, batch, lr, seed=seed0, per_pair=per_pair, log=log)
+        res.update(st)
+        if params_out:
+            np.save(params_out, net.get_params())
+    if test_iters <= 0:
+        net.close()
+        return res
     res.update(evaluate(net, test_shape, test_shots, test_lt, test_iters, tol, batch_shots, test_seed, log=log,
                         warm=warm, flexible=flexible, illum_eps=illum_eps, schedules=schedules))
     net.close()

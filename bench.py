@@ -237,7 +237,7 @@ LEARNED_QUICK = dict(train_models=48, train_shape=(64, 48), train_shots=4, train
                      test_shots=8, test_lt=700, test_iters=600, tol=1e-2, epochs=30, batch=16, warm=0)
 
 
-def learned_case(erm, quick):
+def learned_case(erm, quick, overrides=None):
     """LSRTM seconds to data misfit 1e-2 on a held-out model: plain CG, CG with
     the illumination diagonal, and the learned preconditioner (flexible CG
     alternating z = net(w g) and z = w g), from the whole learned workflow of
@@ -248,8 +248,13 @@ def learned_case(erm, quick):
     (about 2 minutes)."""
     learned = importlib.import_module(PKG + ".learned")
     kw = dict(LEARNED_QUICK) if quick else dict(warm=3)
+    kw.update(overrides or {})
     t0 = time.perf_counter()
     res = learned.run(log=log, **kw)
+    if kw.get("test_iters", 1) <= 0:  # training only (parameters saved)
+        return {"tol": res["config"]["tol"], "trained_only": True, "params_out": kw.get("params_out"),
+                "train_s": res.get("train_s"), "datagen_s": res.get("datagen_s"), "scale": "training only",
+                "learned_vs_plain_time": None}
     el = time.perf_counter() - t0
     cfg = res["config"]
     p, d, l = res["plain"], res["illumination"], res["schedule_alt2"]
@@ -260,7 +265,8 @@ def learned_case(erm, quick):
                  % (cfg["train_models"], *cfg["train_shape"], cfg["train_shots"], *cfg["test_shape"],
                     cfg["test_shots"], cfg["test_lt"],
                     " (reduced case)" if quick else " (scaled case; the configured one is 200 models 400x200 and "
-                                                   "a held-out 800x300 with 80 shots)"),
+                                                   "a held-out 800x300 with 80 shots)")
+                 + (" (net loaded from %s)" % kw["params_in"] if kw.get("params_in") else ""),
         "plain": {"iters": p["iters_to_tol"], "s": p["s_to_tol"]},
         "illumination_diagonal": {"iters": d["iters_to_tol"], "s": d["s_to_tol"]},
         "learned": {"iters": l["iters_to_tol"], "s": l["s_to_tol"], "schedule": "net and diagonal alternating"},
@@ -282,7 +288,20 @@ def run_learned(erm, args):
     (learned_case); one step = one whole workflow; value = learned seconds /
     plain seconds."""
     t0 = time.perf_counter()
-    out = learned_case(erm, args.quick)
+    ov = {}
+    if args.learned_test_shape:
+        ov["test_shape"] = tuple(args.learned_test_shape)
+    for k, v in (("test_shots", args.learned_test_shots), ("test_lt", args.learned_test_lt),
+                 ("test_iters", args.learned_test_iters), ("params_in", args.learned_params_in),
+                 ("params_out", args.learned_params_out)):
+        if v is not None:
+            ov[k] = v
+    if args.learned_params_in:
+        ov["warm"] = 0
+    out = learned_case(erm, args.quick, ov)
+    if out.get("trained_only"):
+        print(json.dumps({"learned_training": out}), flush=True)
+        return
     el = time.perf_counter() - t0
     line = {
         "metric": "LSRTM seconds to data misfit %g, learned vs plain CG" % out["tol"],
@@ -479,7 +498,10 @@ def run_visco(erm, args):
     bvp, bvs, brho, qp, qs, dm = visco_model(nx, nz, 4)
     g = erm.Elastic2D(p)
     g.load_model(1, bvp, bvs, brho, qp, qs)
-    g.set_workers(args.workers)
+    if args.batch > 1:
+        g.set_batch(args.batch)  # shots per launch (each holds its own E/F history: ~10.8 GB here)
+    else:
+        g.set_workers(args.workers)
     shots = learned.shot_positions(nx, args.shots)
     t0 = time.perf_counter()
     d = learned.born_data(g, shots, dm)
@@ -506,7 +528,8 @@ def run_visco(erm, args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded layered model, smooth random Q, Born data)",
         "config": {"workload": f"2D visco LSRTM, {nx}x{nz}, d=8 m, L=3 SLS, {len(shots)} shots, 600 receivers, "
                                f"20 Hz, nt={p.lt}, dt=0.8 ms, 40-cell damping, {args.iters} CG iterations",
-                   "workers": args.workers, "parallelism": "single GPU"},
+                   "workers": args.workers if args.batch <= 1 else None,
+                   "shots_per_launch": args.batch if args.batch > 1 else 1, "parallelism": "single GPU"},
         "misfit": [round(float(v), 5) for v in mis], "seconds": [round(float(v), 2) for v in sec],
         "model_error_vs_true": round(err, 4),
         "operator_Gcell_updates_per_s": round(updates / el / 1e9, 3),
@@ -592,12 +615,22 @@ def main():
     ap.add_argument("--shots", type=int, default=60, help="visco workload: shots")
     ap.add_argument("--iters", type=int, default=20, help="visco workload: CG iterations")
     ap.add_argument("--workers", type=int, default=4, help="visco workload: concurrent shot contexts")
+    ap.add_argument("--batch", type=int, default=0, help="visco workload: shots per batched launch (0: workers)")
     ap.add_argument("--oracle-check", action="store_true", help="visco workload: FP64 oracle CG check (1 shot)")
     ap.add_argument("--workload", choices=["shot", "batch", "visco", "learned", "rtm2d"], default="shot",
                     help="shot: the one-shot 3D configuration (default); batch: 24 windowed shots over a "
                          "600x600x250 model in one batched, graph-captured launch per kernel per step; "
                          "learned: LSRTM time to misfit, learned preconditioner vs plain CG")
     ap.add_argument("--quick", action="store_true", help="learned workload: the reduced test case")
+    ap.add_argument("--learned-test-shape", type=int, nargs=2, default=None, metavar=("NX", "NZ"),
+                    help="learned workload: held-out model size")
+    ap.add_argument("--learned-test-shots", type=int, default=None)
+    ap.add_argument("--learned-test-lt", type=int, default=None)
+    ap.add_argument("--learned-test-iters", type=int, default=None,
+                    help="learned workload: CG iteration cap (0: train and save only)")
+    ap.add_argument("--learned-params-out", default=None, help="learned workload: save the trained net (.npy)")
+    ap.add_argument("--learned-params-in", default=None,
+                    help="learned workload: evaluate a saved net instead of generating data and training")
     ap.add_argument("--no-learned", action="store_true",
                     help="shot workload: skip the reduced learned-vs-plain LSRTM comparison")
     args = ap.parse_args()

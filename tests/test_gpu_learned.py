@@ -6,8 +6,8 @@ training models (4 shots each) run 30 CG iterations preconditioned by the
 illumination diagonal w; pairs (w * gradient_k, m_30 - m_k) at 9 iterations k,
 each target normalised by its own RMS, train the U-Net on the GPU.  On a
 held-out 128x48 model with 8 shots, flexible CG alternating the net
-(z = net(w g)) and the diagonal (z = w g) must reach the data-misfit target
-in at most half the iterations and time of plain CG, and in at most 0.8x the
+(z = net(w g)) and the diagonal (z = w g) must reach the metric's data-misfit
+target 1e-2 in at most half the iterations and time of plain CG, and in at most 0.8x the
 iterations and 0.85x the time of CG with the diagonal alone, so the net's
 own contribution is tested, not only the diagonal's.  All runs use the same
 operators and the same CG code.
@@ -27,7 +27,7 @@ def test_learned_preconditioner_beats_plain_cg_and_the_diagonal(erm):
                        illum_eps=1e-3, log=lambda s: None)
     net, st = L.train(pairs, epochs=30, batch=16, lr=1e-3, seed=1, log=print)
     assert st["loss_first_last"][1] < 0.7 * st["loss_first_last"][0]
-    res = L.evaluate(net, test_shape=(128, 48), test_shots=8, test_lt=700, test_iters=200, tol=0.03,
+    res = L.evaluate(net, test_shape=(128, 48), test_shots=8, test_lt=700, test_iters=500, tol=1e-2,
                      test_seed=5000, log=print, every=50, warm=0, illum_eps=1e-3, schedules=("alt2",))
     net.close()
     p, d, w = res["plain"], res["illumination"], res["schedule_alt2"]
