@@ -279,7 +279,8 @@ def train(pairs, channels=(16, 32, 64, 64), epochs=60, batch=32, lr=1e-3, seed=0
 
 
 def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, batch_shots=0,
-             test_seed=5000, log=print, every=25, warm=5, flexible=False, illum_eps=1e-3, schedules=("alt2",)):
+             test_seed=5000, log=print, every=25, warm=5, flexible=False, illum_eps=1e-3, schedules=("alt2",),
+             plain_file=None, plain_only=False):
     """Plain CG against CG preconditioned by `net` on a held-out model.
 
     With illum_eps set, the approximate inverse Hessian is z = net(w g) with
@@ -287,7 +288,10 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
     alone (z = w g) is run as well, so the net's own contribution is
     visible, and the illumination's cost is included in the times of both.
     Warm starts run `warm` learned iterations and then continue with the
-    net-free preconditioner (plain CG, or the diagonal)."""
+    net-free preconditioner (plain CG, or the diagonal).  For runs longer
+    than one sitting: plain_only runs plain CG alone and saves it to
+    plain_file (.npz); with plain_file given otherwise, plain CG is read from
+    it instead of being run (same model, shots and data: all seeded)."""
     nx, nz = test_shape
     g = Elastic2D(survey_params(nx, nz, test_lt))
     bvp, bvs, brho, dm = random_model(nx, nz, test_seed)
@@ -304,11 +308,21 @@ def evaluate(net, test_shape=(256, 96), test_shots=12, test_lt=1100, test_iters=
 
     res = {}
     nrm = lambda a: float(np.linalg.norm(a[:2] - dm[:2]) / np.linalg.norm(dm[:2]))
-    m_plain, mis_p, sec_p, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, monitor=progress("plain"))
+    if plain_file and not plain_only:
+        z = np.load(plain_file, allow_pickle=False)
+        m_plain, mis_p, sec_p = z["m"], z["misfit"], z["seconds"]
+        res["plain_from"] = plain_file
+    else:
+        m_plain, mis_p, sec_p, _ = g.lsrtm_cg(shots, d, test_iters, pmask=3, tol=tol, monitor=progress("plain"))
+        if plain_file:
+            np.savez(plain_file, m=m_plain, misfit=mis_p, seconds=sec_p)
     ip, tp = time_to(mis_p, sec_p, tol)
     res["plain"] = dict(misfit=[float(v) for v in mis_p], seconds=[float(v) for v in sec_p], iters_to_tol=ip,
                         s_to_tol=tp)
     res["model_err_vs_true"] = {"plain": nrm(m_plain)}
+    if plain_only:
+        g.close()
+        return res
     w, t_w, base = None, 0.0, None
     if illum_eps is not None:
         w, t_w = illum_weights(g, shots, illum_eps)
@@ -375,7 +389,8 @@ def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, c
         test_shape=(256, 96),
         test_shots=12, test_lt=1100, test_iters=400, tol=1e-2, epochs=20, batch=32, lr=1e-3,
         channels=(16, 32, 64, 64), batch_shots=0, seed0=1000, test_seed=5000, warm=3, flexible=False, illum_eps=1e-3,
-        target="converged", per_pair=True, schedules=("alt2",), log=print, params_in=None, params_out=None):
+        target="converged", per_pair=True, schedules=("alt2",), log=print, params_in=None, params_out=None,
+        plain_file=None, plain_only=False):
     """The whole workflow; returns a dict of results.  params_out: save the
     trained net's parameters (numpy .npy); params_in: skip data generation
     and training and evaluate a net with these saved parameters."""
@@ -384,7 +399,9 @@ def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, c
                           test_shots=test_shots,
                           test_lt=test_lt, tol=tol, epochs=epochs, batch=batch, lr=lr, channels=channels,
                           illum_eps=illum_eps, target=target, per_pair=per_pair)}
-    if params_in:
+    if plain_only:
+        net = None
+    elif params_in:
         net = UNet(channels, seed=seed0)
         net.set_params(np.load(params_in, allow_pickle=False))
         res["params_in"] = params_in
@@ -398,11 +415,14 @@ def run(train_models=160, train_shape=(128, 96), train_shots=6, train_lt=1100, c
         if params_out:
             np.save(params_out, net.get_params())
     if test_iters <= 0:
-        net.close()
+        if net:
+            net.close()
         return res
     res.update(evaluate(net, test_shape, test_shots, test_lt, test_iters, tol, batch_shots, test_seed, log=log,
-                        warm=warm, flexible=flexible, illum_eps=illum_eps, schedules=schedules))
-    net.close()
+                        warm=warm, flexible=flexible, illum_eps=illum_eps, schedules=schedules,
+                        plain_file=plain_file, plain_only=plain_only))
+    if net:
+        net.close()
     return res
 
 

@@ -251,6 +251,12 @@ def learned_case(erm, quick, overrides=None):
     kw.update(overrides or {})
     t0 = time.perf_counter()
     res = learned.run(log=log, **kw)
+    if kw.get("plain_only"):
+        cfg, pl = res["config"], res["plain"]
+        return {"tol": cfg["tol"], "plain_only": True, "plain": {"iters": pl["iters_to_tol"], "s": pl["s_to_tol"]},
+                "model_err_vs_true": res["model_err_vs_true"], "plain_file": kw.get("plain_file"),
+                "scale": "held-out %dx%d with %d shots, nt %d" % (*cfg["test_shape"], cfg["test_shots"], cfg["test_lt"]),
+                "learned_vs_plain_time": None}
     if kw.get("test_iters", 1) <= 0:  # training only (parameters saved)
         return {"tol": res["config"]["tol"], "trained_only": True, "params_out": kw.get("params_out"),
                 "train_s": res.get("train_s"), "datagen_s": res.get("datagen_s"), "scale": "training only",
@@ -291,6 +297,14 @@ def run_learned(erm, args):
     ov = {}
     if args.learned_test_shape:
         ov["test_shape"] = tuple(args.learned_test_shape)
+    if args.learned_train_shape:
+        ov["train_shape"] = tuple(args.learned_train_shape)
+    for k, v in (("train_models", args.learned_train_models), ("train_lt", args.learned_train_lt),
+                 ("train_shots", args.learned_train_shots), ("plain_file", args.learned_plain_file)):
+        if v is not None:
+            ov[k] = v
+    if args.learned_plain_only:
+        ov["plain_only"] = True
     for k, v in (("test_shots", args.learned_test_shots), ("test_lt", args.learned_test_lt),
                  ("test_iters", args.learned_test_iters), ("params_in", args.learned_params_in),
                  ("params_out", args.learned_params_out)):
@@ -299,8 +313,8 @@ def run_learned(erm, args):
     if args.learned_params_in:
         ov["warm"] = 0
     out = learned_case(erm, args.quick, ov)
-    if out.get("trained_only"):
-        print(json.dumps({"learned_training": out}), flush=True)
+    if out.get("trained_only") or out.get("plain_only"):
+        print(json.dumps({"learned_training" if out.get("trained_only") else "learned_plain_cg": out}), flush=True)
         return
     el = time.perf_counter() - t0
     line = {
@@ -629,6 +643,14 @@ def main():
     ap.add_argument("--learned-test-iters", type=int, default=None,
                     help="learned workload: CG iteration cap (0: train and save only)")
     ap.add_argument("--learned-params-out", default=None, help="learned workload: save the trained net (.npy)")
+    ap.add_argument("--learned-train-shape", type=int, nargs=2, default=None, metavar=("NX", "NZ"))
+    ap.add_argument("--learned-train-models", type=int, default=None)
+    ap.add_argument("--learned-train-lt", type=int, default=None)
+    ap.add_argument("--learned-train-shots", type=int, default=None)
+    ap.add_argument("--learned-plain-file", default=None,
+                    help="learned workload: plain-CG results file (.npz), written with --learned-plain-only, "
+                         "read otherwise (plain CG is then not rerun)")
+    ap.add_argument("--learned-plain-only", action="store_true", help="learned workload: run plain CG only")
     ap.add_argument("--learned-params-in", default=None,
                     help="learned workload: evaluate a saved net instead of generating data and training")
     ap.add_argument("--no-learned", action="store_true",
