@@ -160,6 +160,7 @@ def _load():
         "erm_batch_use_graph": (ctypes.c_int, [vp, ctypes.c_int]),
         "erm_batch_set_record": (ctypes.c_int, [vp, ctypes.c_int, fp]),
         "erm_batch_run": (ctypes.c_int, [vp, fp, fp, fp]),
+        "erm_batch_set_progress": (ctypes.c_int, [vp, ctypes.c_void_p, ctypes.c_void_p]),
         "erm_batch_shot_context": (ctypes.c_int, [vp, ctypes.c_int, P(vp)]),
         "erm_batch_timing": (ctypes.c_int, [vp, P(ctypes.c_double), P(ctypes.c_double), P(ctypes.c_double),
                                             P(ctypes.c_double), P(ctypes.c_int), P(ctypes.c_int), P(ctypes.c_int)]),
@@ -793,6 +794,14 @@ class Batch3D:
 
     def _c(self, rc):
         _checkb(rc, self._h)
+
+    _PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_double, ctypes.c_void_p)
+
+    def set_progress(self, fn):
+        """fn(pass, ms) after every completed pass of run() (None removes it)."""
+        self._progress_cb = None if fn is None else self._PROGRESS(lambda ps, ms, user: fn(ps, ms))
+        self._c(lib.erm_batch_set_progress(self._h, ctypes.cast(self._progress_cb, ctypes.c_void_p)
+                                           if self._progress_cb else None, None))
 
     def load_model(self, which, vp, vs, rho):
         arrs = [np.ascontiguousarray(a, np.float32) for a in (vp, vs, rho)]

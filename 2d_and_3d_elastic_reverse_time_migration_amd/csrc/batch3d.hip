@@ -30,6 +30,8 @@ struct erm_batch {
     std::vector<erm_shot_window> win;
     std::vector<erm_ctx*> shots;
     std::vector<char> have_rec;
+    erm_batch_progress_fn progress = nullptr;
+    void* progress_user = nullptr;
     // device tables
     ShotStep* d_steps[2] = {nullptr, nullptr};
     // block tables per C-PML axis group (see build_split in rtm3d.hip): group m
@@ -519,6 +521,7 @@ int erm_batch_run(erm_batch* b, float* stp, float* spp, float* sps) {
         b->ms[0] = ms;
         b->steps[0] = b->p.lt;
         if (hipStreamSynchronize(b->stream) != hipSuccess) return b->fail(ERM_E_DEVICE, "model pass failed");
+        if (b->progress) b->progress(ERM_PASS_MODEL, ms, b->progress_user);
         for (size_t s = 0; s < b->shots.size(); ++s) {
             if ((rc = mute_direct(b->shots[s])))  // as erm_forward_real: the window's own direct wave
                 return b->fail(rc, "shot %zu: %s", s, b->shots[s]->err.c_str());
@@ -530,10 +533,12 @@ int erm_batch_run(erm_batch* b, float* stp, float* spp, float* sps) {
     if ((rc = run_steps(b, ERM_PASS_SOURCE, 0, b->p.lt_forward, b->use_graph, &ms))) return rc;
     b->ms[1] = ms;
     b->steps[1] = b->p.lt_forward;
+    if (b->progress) b->progress(ERM_PASS_SOURCE, ms, b->progress_user);
     if ((rc = zero_shots(b, true, false, true))) return rc;
     if ((rc = run_steps(b, ERM_PASS_RECEIVER, b->p.lt - 1, b->p.lt, b->use_graph, &ms))) return rc;
     b->ms[2] = ms;
     b->steps[2] = b->p.lt;
+    if (b->progress) b->progress(ERM_PASS_RECEIVER, ms, b->progress_user);
     // stack the window images into the full model
     const erm_params& P = b->p;
     const size_t nfull = (size_t)P.nx * P.ny * P.nz;
@@ -560,6 +565,13 @@ int erm_batch_run(erm_batch* b, float* stp, float* spp, float* sps) {
                     outs[q][((size_t)i * P.ny + j) * P.nz + k] = tmp[((size_t)k * P.ny + j) * P.nx + i];
     }
     BHC(b, hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int erm_batch_set_progress(erm_batch* b, erm_batch_progress_fn fn, void* user) {
+    if (!b) return ERM_E_PARAM;
+    b->progress = fn;
+    b->progress_user = user;
     return 0;
 }
 

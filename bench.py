@@ -160,6 +160,8 @@ def run_batch(erm, args):
     b.load_model(1, vp_s, vs_s, rho_s)
     del vp, vs, rho
     b.set_shots(wins)
+    b.set_progress(lambda ps, ms: log(f"bench batch: {('model', 'source', 'receiver')[ps]} pass of all shots done "
+                                      f"({ms / 1e3:.1f} s)"))
     tune = b.autotune(steps=3)
     log(f"bench batch: autotuned {tune}")
     t0 = time.perf_counter()

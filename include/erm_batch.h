@@ -73,6 +73,11 @@ int   erm_batch_set_record(erm_batch* b, int shot, const float* rec);
  * source pass with excitation, receiver pass with imaging; then stack.
  * Stacked images over the model region in disk order [nx][ny][nz]; any may be NULL. */
 int   erm_batch_run(erm_batch* b, float* stacked_tp, float* stacked_pp, float* stacked_ps);
+/* Optional progress report (no reference counterpart): fn(pass, ms, user) is
+ * called from erm_batch_run after each completed pass (ERM_PASS_MODEL /
+ * _SOURCE / _RECEIVER) with that pass's device time; NULL removes it. */
+typedef void (*erm_batch_progress_fn)(int pass, double ms, void* user);
+int   erm_batch_set_progress(erm_batch* b, erm_batch_progress_fn fn, void* user);
 /* The context of shot s (owned by the batch): dims, fields, records, images. */
 int   erm_batch_shot_context(erm_batch* b, int shot, erm_ctx** ctx);
 /* Device time of the last run per pass (ms) and the cell-updates per step of
