@@ -146,7 +146,8 @@ int build_tables(erm_batch* b) {
             const int bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / (g.ntx * g.nty);
             const int i0 = std::max(g.x0 + bx * b->tx, R), i1 = std::min(g.x0 + bx * b->tx + b->tx, c->nnx - R) - 1;
             const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
-            const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
+            const ZSpan zs = chunk_span(g, R, bz);
+            const int k0 = zs.kb, k1 = zs.ke - 1;
             if (i0 > i1 || j0 > j1 || k0 > k1) continue;
             int m = 0;
             if (c->cpml) {

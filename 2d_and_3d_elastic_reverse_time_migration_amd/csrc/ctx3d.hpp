@@ -118,6 +118,29 @@ inline int max_sweep(const erm_ctx* c) {
     return k < 1 ? 0 : (int)std::min<long long>(k, 1 << 30);
 }
 
+// First chunk boundary z0 of the z sweeps (chunk_span).  With C-PML, a chunk
+// that holds slab planes runs the C-PML kernel over all its planes, so z0 is
+// the shift in (R - kchunk, R] that leaves the fewest non-slab planes in such
+// chunks (ties keep R); without C-PML it is R.
+inline int chunk_origin(const erm_ctx* c) {
+    const int R = c->R, kc = c->kchunk, top = c->nnz - R;
+    if (!c->cpml || kc <= 1) return R;
+    const int in0 = c->slab_lo[2] + 1, in1 = c->nnz - c->slab_hi[2] - 2;  // planes outside both z slabs
+    int best = R;
+    long best_cost = -1;
+    for (int z0 = R; z0 > R - kc; --z0) {
+        long cost = 0;
+        for (int a = z0; a < top; a += kc) {
+            const int kb = std::max(R, a), ke = std::min(a + kc, top);
+            const int lo = std::max(kb, in0), hi = std::min(ke - 1, in1);
+            const int n_in = hi >= lo ? hi - lo + 1 : 0;
+            if (n_in < ke - kb) cost += n_in;
+        }
+        if (best_cost < 0 || cost < best_cost) { best_cost = cost; best = z0; }
+    }
+    return best;
+}
+
 inline Geo make_geo(const erm_ctx* c) {
     Geo g;
     g.nnx = c->nnx; g.nny = c->nny; g.nnz = c->nnz;
@@ -125,10 +148,11 @@ inline Geo make_geo(const erm_ctx* c) {
     g.cx = c->cx; g.cy = c->cy; g.cz = c->cz;
     g.hx = c->hxyz[0]; g.hy = c->hxyz[1]; g.hz = c->hxyz[2];
     g.kchunk = c->kchunk;
+    g.z0 = chunk_origin(c);
     g.x0 = c->x0;
     g.ntx = ((c->x0 ? c->nnx - 2 * c->R : c->nnx) + c->tx - 1) / c->tx;
     g.nty = (c->nny + c->ty - 1) / c->ty;
-    const int nch = (c->nnz - 2 * c->R + c->kchunk - 1) / c->kchunk;
+    const int nch = (c->nnz - c->R - g.z0 + c->kchunk - 1) / c->kchunk;
     g.ntiles = g.ntx * g.nty * nch;
     g.xcd_map = c->xcd_map == 1 ? 1 : 0;  // mode 2 reorders only the region-split tile lists
     g.per_xcd = (g.ntiles + 7) / 8;

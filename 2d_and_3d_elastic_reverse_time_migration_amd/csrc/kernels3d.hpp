@@ -53,6 +53,7 @@ struct Geo {
     const float* hy;
     const float* hz;
     int kchunk;        // z planes per workgroup
+    int z0;            // first chunk boundary, R - kchunk < z0 <= R (chunk_span)
     int x0;            // x of the first tile column (R with 32-wide tiles, which start at the first updated cell)
     int ntx, nty;      // column tiles in x and y
     int ntiles;        // ntx * nty * z chunks
@@ -74,6 +75,17 @@ struct TileId {
     int bx, by, bz;
     bool valid;
 };
+
+// Updated planes [kb, ke) of z chunk bz: the chunk boundaries sit at
+// z0 + m*kchunk, so with z0 < R the first chunk is shorter (empty when kb >= ke).
+// The host places z0 so that boundaries fall on C-PML slab edges.
+struct ZSpan {
+    int kb, ke;
+};
+__host__ __device__ __forceinline__ ZSpan chunk_span(const Geo& g, int R, int bz) {
+    const int a = g.z0 + bz * g.kchunk, b = a + g.kchunk, top = g.nnz - R;
+    return ZSpan{a > R ? a : R, b < top ? b : top};
+}
 __device__ __forceinline__ TileId tile_of_block(const Geo& g, int blk) {
     int l = blk;
     if (g.xcd_map) l = (l & 7) * g.per_xcd + (l >> 3);
@@ -271,8 +283,8 @@ __device__ __forceinline__ void stress_tile(const Wave& w, const float* __restri
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = g.x0 + tid.bx * TX, j0 = tid.by * TY;
     const int i = i0 + tx, j = j0 + ty;
-    const int kb = R + tid.bz * g.kchunk;
-    const int ke = min(kb + g.kchunk, g.nnz - R);
+    const ZSpan zs = chunk_span(g, R, tid.bz);
+    const int kb = zs.kb, ke = zs.ke;
     if (kb >= ke) return;  // uniform per block
     const bool active = (i >= R) && (i < g.nnx - R) && (j >= R) && (j < g.nny - R);
     const int ic = clampi(i, 0, g.nnx - 1), jc = clampi(j, 0, g.nny - 1);
@@ -529,8 +541,8 @@ __device__ __forceinline__ void velocity_tile(const Wave& w, const float* __rest
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int i0 = g.x0 + tid.bx * TX, j0 = tid.by * TY;
     const int i = i0 + tx, j = j0 + ty;
-    const int kb = R + tid.bz * g.kchunk;
-    const int ke = min(kb + g.kchunk, g.nnz - R);
+    const ZSpan zs = chunk_span(g, R, tid.bz);
+    const int kb = zs.kb, ke = zs.ke;
     if (kb >= ke) return;
     const bool active = (i >= R) && (i < g.nnx - R) && (j >= R) && (j < g.nny - R);
     const int ic = clampi(i, 0, g.nnx - 1), jc = clampi(j, 0, g.nny - 1);

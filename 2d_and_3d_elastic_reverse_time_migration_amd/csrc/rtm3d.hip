@@ -172,7 +172,8 @@ int build_split(erm_ctx* c) {
         const int bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / (g.ntx * g.nty);
         const int i0 = std::max(c->x0 + bx * c->tx, R), i1 = std::min(c->x0 + bx * c->tx + c->tx, c->nnx - R) - 1;
         const int j0 = std::max(by * c->ty, R), j1 = std::min(by * c->ty + c->ty, c->nny - R) - 1;
-        const int k0 = R + bz * c->kchunk, k1 = std::min(k0 + c->kchunk, c->nnz - R) - 1;
+        const ZSpan zs = chunk_span(g, R, bz);
+        const int k0 = zs.kb, k1 = zs.ke - 1;
         if (i0 > i1 || j0 > j1 || k0 > k1) continue;
         int m = (inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) ? 0 : 1) |
                 (inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) ? 0 : 2) |
