@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -105,6 +106,36 @@ int load_window(erm_batch* b, int s, int which) {
     return 0;
 }
 
+// XCD y-band order of a block table (xcd_map 2; the one-shot tile lists use
+// the same order): workgroup b runs on XCD b % 8, each XCD with its own L2.
+// The table, sorted (shot, y, z chunk, x), is cut into 8 chunks of equal
+// length; XCD x takes chunk x at positions x, x + 8, ..., in (shot, z chunk,
+// y, x) order, so the workgroups an XCD holds at once form a compact patch of
+// one window's z layer whose x and y halos its own L2 serves.  Positions past
+// a shorter chunk hold a sentinel (shot 0, tile id past its tiles) that the
+// kernels skip.
+std::vector<int2> band_order(const std::vector<int2>& t, const std::vector<Geo>& geo) {
+    const int n = (int)t.size();
+    auto key = [&](const int2& e, bool ymajor) {
+        const Geo& g = geo[e.x];
+        const long l = e.y & kTileIdMask;
+        const long bx = l % g.ntx, by = (l / g.ntx) % g.nty, bz = l / ((long)g.ntx * g.nty);
+        const long in = ymajor ? (by * g.ntiles + bz) * g.ntx + bx : (bz * g.nty + by) * g.ntx + bx;
+        return std::make_pair(e.x, in);
+    };
+    std::vector<int2> s = t;
+    std::sort(s.begin(), s.end(), [&](const int2& a, const int2& b) { return key(a, true) < key(b, true); });
+    const int per = (n + 7) / 8;
+    std::vector<int2> out((size_t)8 * per, make_int2(0, geo[0].ntiles));
+    for (int x = 0; x < 8; ++x) {
+        const int b0 = std::min(n, x * per), b1 = std::min(n, b0 + per);
+        std::sort(s.begin() + b0, s.begin() + b1,
+                  [&](const int2& a, const int2& b) { return key(a, false) < key(b, false); });
+        for (int j = 0; j < b1 - b0; ++j) out[(size_t)8 * j + x] = s[b0 + j];
+    }
+    return out;
+}
+
 // Device tables for the current shots and launch shape.
 int build_tables(erm_batch* b) {
     if (!b->tables_dirty) return 0;
@@ -114,6 +145,7 @@ int build_tables(erm_batch* b) {
     std::vector<int2> table[8];
     std::vector<RecShot> rs;
     std::vector<int> shot_of;
+    std::vector<Geo> geos;
     double cells = 0;
     for (int s = 0; s < ns; ++s) {
         erm_ctx* c = b->shots[s];
@@ -123,8 +155,9 @@ int build_tables(erm_batch* b) {
         if (!sweep_fits(c, c->kchunk))
             return b->fail(ERM_E_PARAM, "zchunk %d too large for shot %d's %ld-cell planes (at most %d: 32-bit offsets)",
                            c->kchunk, s, c->pxy, max_sweep(c));
-        c->xcd_map = 0;
+        c->xcd_map = c->xcd_map == 2 ? 2 : 0;  // the table ids are plain tile ids (make_geo maps 2 to 0)
         const Geo g = make_geo(c);
+        if (g.ntiles > kTileIdMask) return b->fail(ERM_E_SIZE, "shot %d: too many tiles for the block table", s);
         for (int which = 0; which < 2; ++which) {
             ShotStep d{};
             d.w = make_wave(c);
@@ -149,15 +182,19 @@ int build_tables(erm_batch* b) {
             const ZSpan zs = chunk_span(g, R, bz);
             const int k0 = zs.kb, k1 = zs.ke - 1;
             if (i0 > i1 || j0 > j1 || k0 > k1) continue;
-            int m = 0;
+            int m = 0, axes = 0;
             if (c->cpml) {
                 m = (inside(i0, i1, c->nnx, c->slab_lo[0], c->slab_hi[0]) ? 0 : 1) |
                     (inside(j0, j1, c->nny, c->slab_lo[1], c->slab_hi[1]) ? 0 : 2) |
                     (inside(k0, k1, c->nnz, c->slab_lo[2], c->slab_hi[2]) ? 0 : 4);
+                axes = c->region_split ? m : 7;
                 if (!c->region_split || m != 0) m = 7;  // two groups: plain and all-axes C-PML
             }
-            table[m].push_back(make_int2(s, l));
+            // C-PML entries carry the axes whose slabs the tile touches (bits
+            // 28..30): the kernel skips the memory variables of the others
+            table[m].push_back(make_int2(s, l | (m == 7 ? axes << 28 : 0)));
         }
+        geos.push_back(g);
         RecShot r{};
         r.vx = c->f[0]; r.vy = c->f[1]; r.vz = c->f[2];
         r.idx = c->rec_idx;
@@ -169,6 +206,9 @@ int build_tables(erm_batch* b) {
         for (int q = 0; q < c->nr; ++q) shot_of.push_back(s);
         cells += (double)(c->nnx - 2 * c->R) * (c->nny - 2 * c->R) * (c->nnz - 2 * c->R);
     }
+    if (b->p.xcd_map == 2 && ns > 0)
+        for (auto& t : table)
+            if (!t.empty()) t = band_order(t, geos);
     free_tables(b);
     for (int which = 0; which < 2; ++which) {
         BHC(b, hipMalloc(&b->d_steps[which], std::max<size_t>(1, st[which].size()) * sizeof(ShotStep)));
@@ -204,15 +244,21 @@ inline int group_grid(const erm_batch* b, int m) {
 template <int R, int TX, int TY, int AX>
 void stress_group(erm_batch* b, int which, int source_on, const Coef& cf, const float* wavelet, int wl) {
     const int gr = group_grid(b, AX);
-    if (gr)
-        hipLaunchKernelGGL((k_stress_batch<R, TX, TY, AX>), dim3(gr), dim3(TX, TY), 0, b->stream, b->d_steps[which],
+    if (gr == b->total[AX] && gr)
+        hipLaunchKernelGGL((k_stress_batch<R, TX, TY, AX, false>), dim3(gr), dim3(TX, TY), 0, b->stream, b->d_steps[which],
+                           b->d_table[AX], b->total[AX], cf, wavelet, wl, (const int*)b->d_it, source_on);
+    else if (gr)
+        hipLaunchKernelGGL((k_stress_batch<R, TX, TY, AX, true>), dim3(gr), dim3(TX, TY), 0, b->stream, b->d_steps[which],
                            b->d_table[AX], b->total[AX], cf, wavelet, wl, (const int*)b->d_it, source_on);
 }
 template <int R, int TX, int TY, int MODE, int AX>
 void velocity_group(erm_batch* b, int which, const Coef& cf) {
     const int gr = group_grid(b, AX);
-    if (gr)
-        hipLaunchKernelGGL((k_velocity_batch<R, TX, TY, MODE, AX>), dim3(gr), dim3(TX, TY), 0, b->stream,
+    if (gr == b->total[AX] && gr)
+        hipLaunchKernelGGL((k_velocity_batch<R, TX, TY, MODE, AX, false>), dim3(gr), dim3(TX, TY), 0, b->stream,
+                           b->d_steps[which], b->d_table[AX], b->total[AX], cf, (const int*)b->d_it);
+    else if (gr)
+        hipLaunchKernelGGL((k_velocity_batch<R, TX, TY, MODE, AX, true>), dim3(gr), dim3(TX, TY), 0, b->stream,
                            b->d_steps[which], b->d_table[AX], b->total[AX], cf, (const int*)b->d_it);
 }
 template <int R, int TX, int TY, bool CPML>
@@ -399,7 +445,7 @@ int erm_batch_set_shots(erm_batch* b, const erm_shot_window* shots, int nshots) 
         q.tile_x = b->tx;
         q.tile_y = b->ty;
         q.zchunk = b->kchunk;
-        q.xcd_map = 0;
+        q.xcd_map = P.xcd_map == 2 ? 2 : 0;  // band order of the window's own one-shot tile lists
         erm_ctx* c = nullptr;
         int rc = erm_create(&q, b->device, &c);
         if (rc) { free_shots(b); return b->fail(rc, "shot %d: %s", s, erm_last_error(nullptr)); }

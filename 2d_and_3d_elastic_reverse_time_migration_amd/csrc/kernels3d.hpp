@@ -789,7 +789,8 @@ void k_velocity(Wave w, const float* __restrict__ Bm, Geo g,
 
 // ---------------------------------------------------------------------------
 // Batched launches: many shots (each its own window grid, of any size) in one
-// launch per kernel.  table[b] = (shot, block of that shot's tiling); the
+// launch per kernel.  table[b] = (shot, block of that shot's tiling, with the
+// C-PML axis mask in bits 28..30 as in the one-shot tile lists); the
 // grid either covers the table once or is persistent (fewer workgroups, each
 // striding over the table).  The time step is read from device memory so that
 // one captured step can be replayed (k_step_advance moves it).
@@ -805,7 +806,10 @@ struct ShotStep {
     int src_k, src_col;
 };
 
-template <int R, int TX, int TY, int AX>
+// LOOP: persistent grid (workgroups stride over the table); without it the
+// grid covers the table once, one entry per workgroup, which frees the loop's
+// scalar registers (the plain velocity kernels then keep their occupancy).
+template <int R, int TX, int TY, int AX, bool LOOP>
 __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(R != 4 || AX == 0 ? 1 : (AX == 1 || AX == 2 || AX == 4) ? 4 : 3)))
 void k_stress_batch(const ShotStep* __restrict__ shots,
                                                          const int2* __restrict__ table, int total, Coef cf,
@@ -816,11 +820,12 @@ void k_stress_batch(const ShotStep* __restrict__ shots,
         const int2 t = table[b];
         const ShotStep& s = shots[t.x];
         stress_tile<R, TX, TY, AX>(s.w, s.M, s.MU, s.g, cf, source_on ? s.src_k : -1, source_on ? s.src_col : -1,
-                                     wavelet, wl, it, s.P, t.y, 7);
+                                     wavelet, wl, it, s.P, t.y & kTileIdMask, (t.y >> 28) & 7);
+        if constexpr (!LOOP) break;
     }
 }
 
-template <int R, int TX, int TY, int MODE, int AX>
+template <int R, int TX, int TY, int MODE, int AX, bool LOOP>
 __global__ __launch_bounds__(TX* TY) __attribute__((amdgpu_waves_per_eu(R != 4 || AX == 0 ? 1 : 3)))
 void k_velocity_batch(const ShotStep* __restrict__ shots,
                                                            const int2* __restrict__ table, int total, Coef cf,
@@ -829,7 +834,8 @@ void k_velocity_batch(const ShotStep* __restrict__ shots,
     for (int b = (int)blockIdx.x; b < total; b += (int)gridDim.x) {
         const int2 t = table[b];
         const ShotStep& s = shots[t.x];
-        velocity_tile<R, TX, TY, MODE, AX>(s.w, s.B, s.g, cf, s.ex, it, s.P, t.y, 7);
+        velocity_tile<R, TX, TY, MODE, AX>(s.w, s.B, s.g, cf, s.ex, it, s.P, t.y & kTileIdMask, (t.y >> 28) & 7);
+        if constexpr (!LOOP) break;
     }
 }
 

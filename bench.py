@@ -181,21 +181,24 @@ def run_batch(erm, args):
     cells = t["cells_per_step"]
     updates = cells * (t["steps_source"] + t["steps_receiver"]) * args.steps
     value = updates / el / 1e9
-    # one shot per launch on the same windows: the one-shot path over a few source-pass steps
-    seq_steps = 10
-    seq_ms = 0.0
-    seq_cells = 0.0
+    batched_pass_value = updates / (passes_ms * 1e-3) / 1e9
+    # one shot per launch on the same windows: each window's full source and
+    # receiver passes through the one-shot path (its own launches, same launch
+    # shape), timed by the same pass events as the batched passes
+    seq_ms, seq_updates = 0.0, 0.0
     for s in range(len(wins)):
         v = b.shot(s)
         v.set_profiling(False)
-        v.zero_wavefield()
         v.zero_excitation()
-        v.step(1, 0, 1)
-        v.step(1, 1, seq_steps)
+        v.forward_smoothed()
+        seq_ms += v.timing()["pass_ms"]
+        v.backward_smoothed()
         seq_ms += v.timing()["pass_ms"]
         nnz, nny, nnx = v.shape
-        seq_cells += (nnx - 8) * (nny - 8) * (nnz - 8)
-    seq_value = seq_cells * seq_steps / (seq_ms * 1e-3) / 1e9
+        seq_updates += (nnx - 8) * (nny - 8) * (nnz - 8) * 2.0 * nt
+    seq_value = seq_updates / (seq_ms * 1e-3) / 1e9
+    log(f"bench batch: pass-event rates: batched {batched_pass_value:.3f}, one shot per launch {seq_value:.3f} "
+        f"Gcell-updates/s")
     # per-step roofline of the batched step (both kernels): algorithmic bytes / autotuned step time
     step_bytes = cells * (76.0 + 84.0)
     roof = {"bound": "hbm", "achieved": round(step_bytes / (tune["ms_per_step"] * 1e-3) / 1e9, 1),
@@ -215,6 +218,7 @@ def run_batch(erm, args):
                                "step = source + receiver pass of all shots in one batched graph-captured launch "
                                "per kernel per step" % ("C-PML" if args.boundary == "cpml" else "damping", nt),
                    "cells_per_time_step": cells, "autotuned": tune, "parallelism": "single GPU"},
+        "batched_pass_Gcells": round(batched_pass_value, 3),
         "one_shot_per_launch_Gcells": round(seq_value, 3),
         "roofline": roof,
         "cpu_baseline": None,
