@@ -27,7 +27,7 @@ LIB_PATH = os.path.join(_HERE, "liberm.so")
 
 from .params import *  # noqa: F401,F403
 from .params import PATH_MAX, Params, FIELD_ID, FIELDS  # noqa: F401
-from .params2d import Params2D, FIELD2D_ID, FIELDS2D  # noqa: F401
+from .params2d import Params2D, FIELD2D_ID, FIELDS2D, SRC_HISTORY, SRC_REVERSE, SRC_STRIPS  # noqa: F401
 
 
 class ErmError(RuntimeError):

@@ -768,9 +768,14 @@ __global__ void k2x_save_hist(G2 g, const float* __restrict__ vxp, const float* 
     }
 }
 // boundary strips: the four fields f0..f3 at the ring cells idx[0..nr) to
-// buf [4][nr] (save) or back (restore); restore with a null buf writes zeros
+// buf [4][nr] (save) or back (restore); restore with a null buf writes zeros.
+// Batched shots along blockIdx.y: shot z's fields at + z*sst, its buf at + z*bst.
 __global__ void k2x_ring(float* __restrict__ f0, float* __restrict__ f1, float* __restrict__ f2, float* __restrict__ f3,
-                         const long* __restrict__ idx, long nr, float* __restrict__ buf, int save) {
+                         const long* __restrict__ idx, long nr, float* __restrict__ buf, int save, long sst, long bst) {
+    if (const long z = blockIdx.y) {
+        f0 += z * sst; f1 += z * sst; f2 += z * sst; f3 += z * sst;
+        if (buf) buf += z * bst;
+    }
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nr; t += (long)gridDim.x * blockDim.x) {
         const long n = idx[t];
         if (save) {

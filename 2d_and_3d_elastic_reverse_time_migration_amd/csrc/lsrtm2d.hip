@@ -123,7 +123,8 @@ struct erm2d_ctx {
     int* ex_tp_time = nullptr; // tp excitation (caculate_ex_tp_time_new)
     float* ex_tp_amp = nullptr;
     float* img[3] = {nullptr};
-    float* hist = nullptr;  // [lt][5][mn]: E (3) then F (2)
+    float* hist = nullptr;  // [lt][5][mn]: E (3) then F (2); [5][mn] (one step) with adjoint_source strips
+    float* ls_strip = nullptr;  // adjoint_source strips: [lt][2][4][nr] (stress side, velocity side)
     float* dm = nullptr;    // [3][mn]
     float* grad = nullptr;  // [3][mn]
     float* illum = nullptr; // source illumination accumulator (padded grid)
@@ -152,6 +153,7 @@ struct erm2d_ctx {
         int cap = 0;
         float *A[8] = {nullptr}, *B[8] = {nullptr}, *rA[3][4] = {{nullptr}}, *rB[3][4] = {{nullptr}}, *W[3] = {nullptr};
         float *hist = nullptr, *grad = nullptr, *rec = nullptr;
+        float* strip = nullptr;  // adjoint_source strips: [cap][lt][2][4][nr]
         long* srcs = nullptr;
     } bt;
     int nb = 0;
@@ -188,6 +190,14 @@ int fillf(erm2d_ctx* c, float* p, size_t n, float v) {
     return 0;
 }
 
+// exact-transpose adjoint from boundary strips (adjoint_source 2): the E/F
+// history holds one step per shot, and the ring strips every step
+bool ls_strips(const erm2d_ctx* c) { return c->p.adjoint_source == ERM2D_SRC_STRIPS; }
+int hist_steps(const erm2d_ctx* c) { return ls_strips(c) ? 1 : c->p.lt; }
+std::vector<long> xcorr_ring(const erm2d_ctx* c);
+// the ring cell indices of the strips strategies (built on first use)
+int ensure_ring(erm2d_ctx* c);
+
 G2 geo(const erm2d_ctx* c) {
     G2 g;
     g.nnx = c->nnx; g.nnz = c->nnz; g.pz = c->pz;
@@ -201,7 +211,7 @@ G2 geo(const erm2d_ctx* c) {
     g.pcz = c->pcz;
     g.ncell = (long)c->ncell;
     g.sst = c->nb ? (long)c->ncell : 0;
-    g.hst = c->nb ? (long)c->p.lt * 5 * (long)c->mn : 0;
+    g.hst = c->nb ? (long)hist_steps(c) * 5 * (long)c->mn : 0;
     g.gst = c->nb ? 3 * (long)c->mn : 0;
     g.srcs = c->nb ? c->bt.srcs : nullptr;
     return g;
@@ -378,7 +388,12 @@ int run(erm2d_ctx* c, int pass, int it_begin, int nsteps, float* recbuf) {
 }
 
 int ensure_lsrtm(erm2d_ctx* c) {
-    if (!c->hist && dmal(c, &c->hist, (size_t)c->p.lt * 5 * c->mn)) return ERM_E_NOMEM;
+    if (!c->hist && dmal(c, &c->hist, (size_t)hist_steps(c) * 5 * c->mn)) return ERM_E_NOMEM;
+    if (ls_strips(c)) {
+        int rc;
+        if ((rc = ensure_ring(c))) return rc;
+        if (!c->ls_strip && dmal(c, &c->ls_strip, (size_t)c->p.lt * 8 * c->xc_nr)) return ERM_E_NOMEM;
+    }
     if (!c->dm && dmal(c, &c->dm, 3 * c->mn)) return ERM_E_NOMEM;
     if (!c->grad && dmal(c, &c->grad, 3 * c->mn)) return ERM_E_NOMEM;
     return 0;
@@ -397,11 +412,11 @@ void free2(erm2d_ctx* c) {
     for (auto& m : c->mod) { F(m.vp); F(m.vs); }
     for (auto& e : c->ex) F(e);
     for (auto& e : c->img) F(e);
-    F(c->hist); F(c->dm); F(c->grad); F(c->illum);
+    F(c->hist); F(c->ls_strip); F(c->dm); F(c->grad); F(c->illum);
     for (int q = 0; q < 8; ++q) { F(c->bt.A[q]); F(c->bt.B[q]); }
     for (int l = 0; l < 3; ++l) for (int q = 0; q < 4; ++q) { F(c->bt.rA[l][q]); F(c->bt.rB[l][q]); }
     for (auto& w : c->bt.W) F(w);
-    F(c->bt.hist); F(c->bt.grad); F(c->bt.rec); F(c->bt.srcs);
+    F(c->bt.hist); F(c->bt.grad); F(c->bt.rec); F(c->bt.srcs); F(c->bt.strip);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -453,6 +468,14 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
     if (p.boundary != ERM_BOUNDARY_DAMP && p.boundary != ERM_BOUNDARY_CPML) { g_err2 = "boundary must be 0 (damping) or 1 (CPML)"; return ERM_E_PARAM; }
     if (p.inversion_para < 0 || p.inversion_para > 3) { g_err2 = "inversion_para must be 1, 2 or 3 (0: 2)"; return ERM_E_PARAM; }
     if (p.adjoint_mode != 0 && p.adjoint_mode != 1) { g_err2 = "adjoint_mode must be 0 (exact transpose) or 1 (reference)"; return ERM_E_PARAM; }
+    if (p.adjoint_source != ERM2D_SRC_HISTORY && p.adjoint_source != ERM2D_SRC_STRIPS) {
+        g_err2 = "adjoint_source must be 0 (history) or 2 (strips)";
+        return ERM_E_PARAM;
+    }
+    if (p.adjoint_source == ERM2D_SRC_STRIPS && (p.nsls != 0 || p.adjoint_mode != 0)) {
+        g_err2 = "adjoint_source 2 (strips) needs an elastic medium (nsls = 0) and adjoint_mode 0";
+        return ERM_E_PARAM;
+    }
     int nd = 0;
     if (hipGetDeviceCount(&nd) != hipSuccess || nd == 0) { g_err2 = "no HIP device"; return ERM_E_DEVICE; }
     if (device < 0 || device >= nd) { g_err2 = "device out of range"; return ERM_E_PARAM; }
@@ -800,11 +823,27 @@ std::vector<long> xcorr_ring(const erm2d_ctx* c) {
             if (i < i0 || i >= i1 || k < k0 || k >= k1) out.push_back((long)i * c->pz + k);
     return out;
 }
-void ring_io(erm2d_ctx* c, bool vside, float* buf, int save) {
+int ensure_ring(erm2d_ctx* c) {
+    if (c->xc_ring) return 0;
+    std::vector<long> rg = xcorr_ring(c);
+    c->xc_nr = (long)rg.size();
+    if (dmal(c, &c->xc_ring, std::max<size_t>(rg.size(), 1))) return ERM_E_NOMEM;
+    if (!rg.empty())
+        HC(c, hipMemcpyAsync(c->xc_ring, rg.data(), rg.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
+    HC(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+// save or restore the ring of state A's stress side (sxx, szz, sxz, tp) or
+// velocity side (vx, vz, vxp, vzp) for every shot of the running operator;
+// buf is shot 0's [4][nr] block, shot z's sits bst floats further (restore
+// with a null buf writes zeros)
+void ring_io(erm2d_ctx* c, bool vside, float* buf, int save, long bst = 0) {
+    if (c->xc_nr == 0) return;  // the model region fills the grid: no ring
     const F2 f = state(c, false);
     float* a[4] = {vside ? f.vx : f.xx, vside ? f.vz : f.zz, vside ? f.vxp : f.xz, vside ? f.vzp : f.tp};
-    hipLaunchKernelGGL(k2x_ring, dim3((unsigned)std::min<long>((c->xc_nr + 255) / 256, 1024)), dim3(256), 0, c->stream,
-                       a[0], a[1], a[2], a[3], c->xc_ring, c->xc_nr, buf, save);
+    const dim3 grid((unsigned)std::min<long>((c->xc_nr + 255) / 256, 1024), nshot_of(c));
+    hipLaunchKernelGGL(k2x_ring, grid, dim3(256), 0, c->stream, a[0], a[1], a[2], a[3], c->xc_ring, c->xc_nr, buf, save,
+                       c->nb ? (long)c->ncell : 0L, buf ? bst : 0L);
 }
 }  // namespace
 
@@ -828,12 +867,7 @@ int erm2d_rtm_xcorr(erm2d_ctx* c, const float* rec, int strategy, float* images)
         c->xc_hist_n = (size_t)ltf * 3 * mn;
     }
     if (strategy == 2) {
-        if (!c->xc_ring) {
-            std::vector<long> rg = xcorr_ring(c);
-            c->xc_nr = (long)rg.size();
-            if (dmal(c, &c->xc_ring, rg.size())) return ERM_E_NOMEM;
-            HC(c, hipMemcpyAsync(c->xc_ring, rg.data(), rg.size() * sizeof(long), hipMemcpyHostToDevice, c->stream));
-        }
+        if (const int e = ensure_ring(c)) return e;
         const size_t need = (size_t)ltf * 8 * c->xc_nr;
         if (c->xc_strip_n < need) {
             if (c->xc_strip) { (void)hipFree(c->xc_strip); c->xc_strip = nullptr; c->xc_strip_n = 0; }
@@ -975,7 +1009,12 @@ int born_core(erm2d_ctx* c) {
 // background pass stores E and F of every step, then the transposed Born
 // step runs backward from the residual records in rec_of(c); per-shot
 // gradients in grad_of(c), summed into c->grad and mapped to the inverted
-// parameters.
+// parameters.  With adjoint_source strips the background pass stores only
+// the ring strips of every half step, and before each transposed step the
+// background steps back over that step (velocity, then stress, with the
+// derivative terms and the source negated, rfwd_..._old), the ring restored
+// from the strips: those reversed kernels recompute F and E of the step into
+// the one-step history slot.
 int adjoint_ref_core(erm2d_ctx* c, bool mapped);
 int adjoint_core(erm2d_ctx* c) {
     if (c->p.adjoint_mode == 1) return adjoint_ref_core(c, true);
@@ -991,13 +1030,26 @@ int adjoint_core(erm2d_ctx* c) {
     const long sst = c->nb ? (long)c->ncell : 0, rst = c->nb ? 2 * comp : 0;
     const F2 fb = state(c, true);
     const size_t step = 5 * c->mn;
-    for (int it = 0; it < c->p.lt; ++it) {  // background, storing E and F per step
-        float* h = hist + (size_t)it * step;
+    const bool strips = ls_strips(c);
+    float* strip = c->nb ? c->bt.strip : c->ls_strip;
+    const long nr4 = 4 * c->xc_nr, bst = (long)c->p.lt * 2 * nr4;  // per-shot strip stride
+    for (int it = 0; it < c->p.lt; ++it) {  // background, storing E and F per step, or the strips
+        float* h = strips ? nullptr : hist + (size_t)it * step;
         do_stress(c, false, 1, c->src, wavelet_at(c, it), h);
-        do_velocity(c, false, 1, 0, it, h + 3 * c->mn);
+        if (strips) ring_io(c, false, strip + 2 * it * nr4, 1, bst);
+        do_velocity(c, false, 1, 0, it, h ? h + 3 * c->mn : nullptr);
+        if (strips) ring_io(c, true, strip + (2 * it + 1) * nr4, 1, bst);
     }
     for (int it = c->p.lt - 1; it >= 0; --it) {
-        const float* h = hist + (size_t)it * step;
+        const float* h = strips ? hist : hist + (size_t)it * step;
+        if (strips) {  // background back over step it: F, then E of the step into the one-step slot
+            ring_io(c, false, strip + 2 * it * nr4, 0, bst);
+            c->dsgn = -1.f;
+            do_velocity(c, false, 1, 0, it, hist + 3 * c->mn);
+            ring_io(c, true, it > 0 ? strip + (2 * it - 1) * nr4 : nullptr, 0, bst);
+            do_stress(c, false, 1, c->src, -wavelet_at(c, it), hist);
+            c->dsgn = 1.f;
+        }
         if (c->p.nrec > 0)
             hipLaunchKernelGGL(k2_inject, dim3(rg, ns), dim3(256), 0, c->stream, fb.vx, fb.vz, c->rec_idx, c->p.nrec,
                                rec, comp, (long)it * c->p.nrec, sst, rst);
@@ -1022,6 +1074,7 @@ int adjoint_core(erm2d_ctx* c) {
 // c->grad.  mapped: the records are first scaled per receiver by dt / rho
 // and the sum is mapped to the inverted parameters (k2r_grad_map).
 int adjoint_ref_core(erm2d_ctx* c, bool mapped) {
+    if (ls_strips(c)) return c->fail(ERM_E_PARAM, "the reference-mode adjoint needs adjoint_source 0 (history)");
     int rc;
     const long comp = (long)c->p.lt * c->p.nrec;
     const int rg = (c->p.nrec + 255) / 256;
@@ -1089,7 +1142,7 @@ void free_batch(erm2d_ctx* c) {
     for (int q = 0; q < 8; ++q) { F(b.A[q]); F(b.B[q]); }
     for (int l = 0; l < 3; ++l) for (int q = 0; q < 4; ++q) { F(b.rA[l][q]); F(b.rB[l][q]); }
     for (auto& w : b.W) F(w);
-    F(b.hist); F(b.grad); F(b.rec); F(b.srcs);
+    F(b.hist); F(b.grad); F(b.rec); F(b.srcs); F(b.strip);
     b = erm2d_ctx::Batch{};
 }
 }  // namespace
@@ -1157,7 +1210,11 @@ int erm2d_set_batch(erm2d_ctx* c, int nb) {
     for (int l = 0; l < c->L && !rc; ++l)
         for (int q = 0; q < 4 && !rc; ++q) rc = dmal(c, &b.rA[l][q], n) || dmal(c, &b.rB[l][q], n);
     for (int q = 0; q < 3 && !rc; ++q) rc = dmal(c, &b.W[q], n);
-    if (!rc) rc = dmal(c, &b.hist, (size_t)c->p.lt * 5 * c->mn * nb);
+    if (!rc) rc = dmal(c, &b.hist, (size_t)hist_steps(c) * 5 * c->mn * nb);
+    if (!rc && ls_strips(c)) {
+        rc = ensure_ring(c);
+        if (!rc) rc = dmal(c, &b.strip, (size_t)c->p.lt * 8 * c->xc_nr * nb);
+    }
     if (!rc) rc = dmal(c, &b.grad, 3 * c->mn * nb);
     if (!rc) rc = dmal(c, &b.rec, std::max<size_t>((size_t)2 * c->p.lt * c->p.nrec * nb, 1));
     if (!rc) rc = dmal(c, &b.srcs, nb);

@@ -5,6 +5,9 @@ FIELDS2D = ["vx", "vz", "txx", "tzz", "txz", "tp", "vxp", "vzp", "div", "curl",
             "ex_time", "ex_amp", "img_tp", "img_pp", "img_ps", "ex_tp", "ex_vxp", "ex_vzp", "r_vxp", "r_vzp",
             "ex_tp_time", "ex_tp_amp", "pdiv", "scurl"]
 FIELD2D_ID = {n: i for i, n in enumerate(FIELDS2D)}
+# source-wavefield strategies (ERM2D_SRC_*): erm2d_rtm_xcorr's strategy and
+# erm2d_params.adjoint_source
+SRC_HISTORY, SRC_REVERSE, SRC_STRIPS = 0, 1, 2
 
 
 class Params2D(ctypes.Structure):
@@ -18,7 +21,8 @@ class Params2D(ctypes.Structure):
                                                 "rec_z"]]
                 + [("qcomp", ctypes.c_int), ("ex_tp_shift", ctypes.c_int), ("ex_vzp_avg", ctypes.c_int),
                    ("boundary", ctypes.c_int), ("cpml_r", ctypes.c_float), ("cpml_vmax", ctypes.c_float),
-                   ("inversion_para", ctypes.c_int), ("adjoint_mode", ctypes.c_int)])
+                   ("inversion_para", ctypes.c_int), ("adjoint_mode", ctypes.c_int),
+                   ("adjoint_source", ctypes.c_int)])
 
     def update(self, **kw):
         for k, v in kw.items():

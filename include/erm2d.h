@@ -56,6 +56,14 @@ typedef struct erm2d_params {
                           0 (default) = exact transpose of the discrete Born operator;
                           1 = the reference's continuous-adjoint equations and gradient
                           (erm2d_adjoint_ref), mapped to the inversion_para parameters */
+    int   adjoint_source; /* background wavefield of the exact-transpose adjoint (adjoint_mode 0):
+                          ERM2D_SRC_HISTORY (0, default) = the strain increments E and forces F of
+                          every step stored in HBM ([lt][5][nx*nz] floats per shot, as the
+                          reference's save_and_set_wavefiled history, viscoelastic_equation.cu:3-21);
+                          ERM2D_SRC_STRIPS (2) = only a ring of width R + 1 around the model region
+                          stored per half step ([lt][8][ring] floats per shot) and the background
+                          stepped backward, E and F recomputed on the way (elastic media, nsls = 0;
+                          equal to the history gradient up to FP32 rounding of the reversal) */
 } erm2d_params;
 
 /* fields for erm2d_get_field / erm2d_set_field (padded grid, z fastest) */

@@ -577,20 +577,29 @@ struct Oracle2D {
         return 0;
     }
 
-    /* g = L^T r: exact transpose of born(), stepping back from lt-1. */
-    int adjoint(const float* rr, float* gout) {
+    /* g = L^T r: exact transpose of born(), stepping back from lt-1.  EHin /
+     * FHin: the background's E and F histories computed elsewhere (the strips
+     * reconstruction, adjoint_strips); else the background pass stores them. */
+    int adjoint(const float* rr, float* gout, const std::vector<T>* EHin = nullptr,
+                const std::vector<T>* FHin = nullptr) {
         if (cpml) return -1;
         const Model& m = mod[1];
         if (!m.have) return -4;
         size_t mn = (size_t)p.nx * p.nz;
         /* background history: E(v0^n) and F(s0^{n+1}) over the model region */
-        std::vector<T> EH((size_t)p.lt * 3 * mn), FH((size_t)p.lt * 2 * mn);
-        zero();
-        for (int it = 0; it < p.lt; ++it) {
-            add_source(f, it);
-            stress(f, r, m, EH.data() + (size_t)it * 3 * mn);
-            velocity(f, m, 0, it, FH.data() + (size_t)it * 2 * mn);
+        std::vector<T> EHl, FHl;
+        if (!EHin || !FHin) {
+            EHl.assign((size_t)p.lt * 3 * mn, 0);
+            FHl.assign((size_t)p.lt * 2 * mn, 0);
+            zero();
+            for (int it = 0; it < p.lt; ++it) {
+                add_source(f, it);
+                stress(f, r, m, EHl.data() + (size_t)it * 3 * mn);
+                velocity(f, m, 0, it, FHl.data() + (size_t)it * 2 * mn);
+            }
         }
+        const std::vector<T>& EH = EHin && FHin ? *EHin : EHl;
+        const std::vector<T>& FH = EHin && FHin ? *FHin : FHl;
         std::vector<T> lv[2], ls[3], lr[3][4], W[3];
         for (auto& v : lv) v.assign(N, 0);
         for (auto& v : ls) v.assign(N, 0);
@@ -896,8 +905,11 @@ struct XcorrRTM {
     using Model = typename Oracle2D<T>::Model;
     enum { VX, VZ, XX, ZZ, XZ, TP, VXP, VZP };
 
-    void rvelocity(std::vector<T>* fv, const Model& m) {
+    /* Fh / Eh (optional): F / E of the step over the model region, as the
+     * forward kernels store them (stress() / velocity() of Oracle2D) */
+    void rvelocity(std::vector<T>* fv, const Model& m, T* Fh = nullptr) {
         const ptrdiff_t sX = o.nnz;
+        const size_t mn = (size_t)o.p.nx * o.p.nz;
 #pragma omp parallel for schedule(static)
         for (int i = o.R; i < o.nnx - o.R; ++i)
             for (int k = o.R; k < o.nnz - o.R; ++k) {
@@ -909,6 +921,8 @@ struct XcorrRTM {
                 T F1 = o.pml(6, n, 0, 0, i, f1x) + o.pml(7, n, 1, 1, k, f1z);
                 px = o.pml(8, n, 0, 1, i, px);
                 pz = o.pml(9, n, 1, 1, k, pz);
+                size_t q;
+                if (Fh && o.model_cell(i, k, &q)) { Fh[q] = F0; Fh[mn + q] = F1; }
                 T b, a = o.ab(n, &b), bo = m.B[n];
                 fv[VX][n] = a * (b * fv[VX][n] + bo * -F0);
                 fv[VZ][n] = a * (b * fv[VZ][n] + bo * -F1);
@@ -916,8 +930,9 @@ struct XcorrRTM {
                 fv[VZP][n] = a * (b * fv[VZP][n] + bo * -pz);
             }
     }
-    void rstress(std::vector<T>* fv, std::vector<T> (*rv)[4], const Model& m, int it) {
+    void rstress(std::vector<T>* fv, std::vector<T> (*rv)[4], const Model& m, int it, T* Eh = nullptr) {
         const ptrdiff_t sX = o.nnz;
+        const size_t mn = (size_t)o.p.nx * o.p.nz;
         /* the wavelet enters where the forward step adds it, with the opposite sign */
         if (it >= 0 && it < (int)o.wavelet.size()) { fv[XX][o.src] -= o.wavelet[it]; fv[ZZ][o.src] -= o.wavelet[it]; }
 #pragma omp parallel for schedule(static)
@@ -928,6 +943,8 @@ struct XcorrRTM {
                 T e2x = o.cx * o.dm(fv[VZ].data() + n, sX), e2z = o.cz * o.dp(fv[VX].data() + n, 1);
                 T E[3] = {o.pml(0, n, 0, 0, i, e0), o.pml(2, n, 1, 0, k, e1),
                           o.pml(1, n, 0, 1, i, e2x) + o.pml(3, n, 1, 1, k, e2z)};
+                size_t q;
+                if (Eh && o.model_cell(i, k, &q)) { Eh[q] = E[0]; Eh[mn + q] = E[1]; Eh[2 * mn + q] = E[2]; }
                 T b, a = o.ab(n, &b);
                 T M = m.M[n], U = m.U[n], La = M - 2 * U;
                 T sp = 0, sxx = 0, szz = 0, sxz = 0;
@@ -1101,6 +1118,50 @@ struct XcorrRTM {
     }
 };
 
+/* ---------------------------------------------------------------------------
+ * Exact-transpose adjoint with the background from boundary strips
+ * (adjoint_source = 2; no reference counterpart, elastic media only): the
+ * background pass keeps only the ring of XcorrRTM::ring() after every half
+ * step; stepping back from lt-1, the ring is restored before each reversed
+ * half step (stress side of step it, then velocity side of step it-1, zeros
+ * before step 0) and the reversed steps store F and E of step it, which
+ * then drive the same transposed Born steps as the stored history.
+ * ------------------------------------------------------------------------- */
+template <typename T>
+int adjoint_strips(Oracle2D<T>& o, const float* rr, float* gout) {
+    if (o.cpml || o.L != 0) return -1;
+    const auto& m = o.mod[1];
+    if (!m.have) return -4;
+    XcorrRTM<T> x(o);
+    enum { VX, VZ, XX, ZZ, XZ, TP, VXP, VZP };
+    const int vfield[4] = {VX, VZ, VXP, VZP}, sfield[4] = {XX, ZZ, XZ, TP};
+    const auto& p = o.p;
+    const size_t mn = (size_t)p.nx * p.nz;
+    const std::vector<size_t> rg = x.ring();
+    const size_t nr = rg.size();
+    std::vector<T> rings((size_t)p.lt * 4 * nr), ringv((size_t)p.lt * 4 * nr);
+    o.zero();
+    for (int it = 0; it < p.lt; ++it) {
+        o.add_source(o.f, it);
+        o.stress(o.f, o.r, m);
+        for (int q = 0; q < 4; ++q)
+            for (size_t j = 0; j < nr; ++j) rings[((size_t)it * 4 + q) * nr + j] = o.f[sfield[q]][rg[j]];
+        o.velocity(o.f, m, 0, it);
+        for (int q = 0; q < 4; ++q)
+            for (size_t j = 0; j < nr; ++j) ringv[((size_t)it * 4 + q) * nr + j] = o.f[vfield[q]][rg[j]];
+    }
+    std::vector<T> EH((size_t)p.lt * 3 * mn, 0), FH((size_t)p.lt * 2 * mn, 0);
+    for (int it = p.lt - 1; it >= 0; --it) {
+        for (int q = 0; q < 4; ++q)
+            for (size_t j = 0; j < nr; ++j) o.f[sfield[q]][rg[j]] = rings[((size_t)it * 4 + q) * nr + j];
+        x.rvelocity(o.f, m, FH.data() + (size_t)it * 2 * mn);
+        for (int q = 0; q < 4; ++q)
+            for (size_t j = 0; j < nr; ++j) o.f[vfield[q]][rg[j]] = it > 0 ? ringv[((size_t)(it - 1) * 4 + q) * nr + j] : (T)0;
+        x.rstress(o.f, o.r, m, it, EH.data() + (size_t)it * 3 * mn);
+    }
+    return o.adjoint(rr, gout, &EH, &FH);
+}
+
 template <typename T>
 struct Box { Oracle2D<T> o; };
 
@@ -1202,6 +1263,10 @@ int oracle2d_image(void* h, int it) {
 }
 int oracle2d_born(void* h, const float* dm, float* d) { return disp(h, [&](auto& o) { return o.born(dm, d); }); }
 int oracle2d_adjoint(void* h, const float* r, float* g) { return disp(h, [&](auto& o) { return o.adjoint(r, g); }); }
+
+int oracle2d_adjoint_strips(void* h, const float* r, float* g) {
+    return disp(h, [&](auto& o) { return adjoint_strips(o, r, g); });
+}
 /* cross-correlation RTM (records as set by oracle2d_record); out [5][nx][nz] */
 int oracle2d_rtm_xcorr(void* h, int strategy, float* out) {
     return disp(h, [&](auto& o) {

@@ -35,6 +35,7 @@ for name, res, args in [
     ("oracle2d_image", ctypes.c_int, [_vp, ctypes.c_int]),
     ("oracle2d_born", ctypes.c_int, [_vp, _fp, _fp]),
     ("oracle2d_adjoint", ctypes.c_int, [_vp, _fp, _fp]),
+    ("oracle2d_adjoint_strips", ctypes.c_int, [_vp, _fp, _fp]),
     ("oracle2d_adjoint_ref", ctypes.c_int, [_vp, _fp, _fp]),
     ("oracle2d_rtm_xcorr", ctypes.c_int, [_vp, ctypes.c_int, _fp]),
     ("oracle2d_ref_adjoint_steps", ctypes.c_int, [_vp, _dp, _dp, ctypes.c_int, _dp, _dp]),
@@ -129,6 +130,15 @@ class Oracle2D:
         r = _f(r)
         g = np.zeros((3, p.nx, p.nz), np.float32)
         self._c(lib.oracle2d_adjoint(self._h, _ptr(r, ctypes.c_float), _ptr(g, ctypes.c_float)))
+        return g
+
+    def adjoint_strips(self, r):
+        """The exact-transpose adjoint with the background's E and F recomputed
+        by stepping it back from ring strips (elastic media, damping boundary)."""
+        p = self.p
+        r = _f(r)
+        g = np.zeros((3, p.nx, p.nz), np.float32)
+        self._c(lib.oracle2d_adjoint_strips(self._h, _ptr(r, ctypes.c_float), _ptr(g, ctypes.c_float)))
         return g
 
     def adjoint_ref(self, r):
