@@ -505,6 +505,70 @@ def run_rtm2d_xcorr(erm, args):
     print(json.dumps(line), flush=True)
 
 
+def ring_cells(p, R):
+    """Cells of the strips ring of a 2D context (the library's xcorr_ring):
+    outside the model region shrunk by one cell, within R + 1 of it."""
+    nnx, nnz = p.nx + p.bl + p.br, p.nz + p.bu + p.bd
+    i0, i1, k0, k1 = p.bl + 1, p.bl + p.nx - 1, p.bu + 1, p.bu + p.nz - 1
+    ia, ib = max(i0 - R - 1, 0), min(i1 + R + 1, nnx)
+    ka, kb = max(k0 - R - 1, 0), min(k1 + R + 1, nnz)
+    return (ib - ia) * (kb - ka) - (i1 - i0) * (k1 - k0)
+
+
+def run_gradient(erm, args):
+    """LSRTM gradient (the exact-transpose adjoint) with the background from
+    the stored E/F history against the background stepped back from boundary
+    strips (erm2d_params.adjoint_source), on the learned workload's held-out
+    problem: 800x300 model, d = 10 m, 12 shots in one batched launch per
+    kernel, nt = 2000 (1 ms), 15 Hz, Born data.  Times one adjoint
+    application (median of 3) and a few CG iterations with each; states the
+    HBM each holds for the background and the gradient difference."""
+    learned = importlib.import_module(PKG + ".learned")
+    nx, nz, nshots, lt, iters = 800, 300, 12, 2000, 10
+    bvp, bvs, brho, dm = learned.random_model(nx, nz, 5000)
+    shots = learned.shot_positions(nx, nshots)
+    out, grads, d = {}, {}, None
+    for name, src in (("history", erm.SRC_HISTORY), ("strips", erm.SRC_STRIPS)):
+        p = learned.survey_params(nx, nz, lt)
+        p.adjoint_source = src
+        g = erm.Elastic2D(p)
+        g.load_model(1, bvp, bvs, brho)
+        g.set_batch(nshots)
+        if d is None:
+            d = learned.born_data(g, shots, dm)
+        g.adjoint_shots(shots, d)  # warm-up
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            grads[name] = g.adjoint_shots(shots, d)
+            ts.append(time.perf_counter() - t0)
+        R = 4
+        held = (nshots * lt * 5 * nx * nz * 4.0 if name == "history"
+                else nshots * lt * 8 * ring_cells(p, R) * 4.0)
+        m, mis, sec, _ = g.lsrtm_cg(shots, d, iters, pmask=3)
+        g.close()
+        out[name] = {"adjoint_s": round(float(np.median(ts)), 4), "background_hbm_gb": round(held / 1e9, 3),
+                     "cg_s_per_iteration": round(float(sec[-1] / iters), 4),
+                     "misfit": [round(float(v), 6) for v in mis]}
+        log(f"bench gradient: {name}: adjoint {out[name]['adjoint_s']} s, background {out[name]['background_hbm_gb']} GB, "
+            f"CG misfit after {iters}: {mis[-1]:.4f}")
+    diff = [float(np.linalg.norm(grads["strips"][c] - grads["history"][c]) / np.linalg.norm(grads["history"][c]))
+            for c in range(3)]
+    line = {
+        "metric": "LSRTM gradient (exact-transpose adjoint) s per application, background from boundary strips",
+        "value": out["strips"]["adjoint_s"], "unit": "s", "n_gpus": 1, "steps": 3, "warmup": 1,
+        "ms_per_step": round(out["strips"]["adjoint_s"] * 1e3, 1), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded layered-plus-anomaly model, Born data)",
+        "config": {"workload": f"2D elastic LSRTM adjoint, {nx}x{nz}, d=10 m, {nshots} shots batched, nt={lt}, "
+                               f"15 Hz, 20-cell damping, radius 4; {iters} CG iterations per strategy",
+                   "parallelism": "single GPU"},
+        "history": out["history"], "strips": out["strips"],
+        "strips_vs_history_gradient_rel_l2": [round(v, 8) for v in diff],
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def run_visco(erm, args):
     """2D viscoelastic LSRTM with plain CG: 600x300 model, d = 8 m, 3 SLS
     mechanisms, 60 shots, 600 receivers, 20 Hz, nt = 3000 (dt 0.8 ms), 40-cell
@@ -637,10 +701,11 @@ def main():
     ap.add_argument("--workers", type=int, default=4, help="visco workload: concurrent shot contexts")
     ap.add_argument("--batch", type=int, default=0, help="visco workload: shots per batched launch (0: workers)")
     ap.add_argument("--oracle-check", action="store_true", help="visco workload: FP64 oracle CG check (1 shot)")
-    ap.add_argument("--workload", choices=["shot", "batch", "visco", "learned", "rtm2d"], default="shot",
+    ap.add_argument("--workload", choices=["shot", "batch", "visco", "learned", "rtm2d", "gradient"], default="shot",
                     help="shot: the one-shot 3D configuration (default); batch: 24 windowed shots over a "
                          "600x600x250 model in one batched, graph-captured launch per kernel per step; "
-                         "learned: LSRTM time to misfit, learned preconditioner vs plain CG")
+                         "learned: LSRTM time to misfit, learned preconditioner vs plain CG; gradient: the LSRTM "
+                         "adjoint from the stored E/F history vs from boundary strips")
     ap.add_argument("--quick", action="store_true", help="learned workload: the reduced test case")
     ap.add_argument("--learned-test-shape", type=int, nargs=2, default=None, metavar=("NX", "NZ"),
                     help="learned workload: held-out model size")
@@ -681,6 +746,8 @@ def main():
         return run_learned(erm, args)
     if args.workload == "rtm2d":
         return run_rtm2d_xcorr(erm, args) if args.imaging == "xcorr" else run_rtm2d(erm, args)
+    if args.workload == "gradient":
+        return run_gradient(erm, args)
     nt = args.nt
     p = make_params(erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
     p.tile_x, p.tile_y, p.zchunk = args.tile_x, args.tile_y, args.zchunk

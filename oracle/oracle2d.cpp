@@ -19,7 +19,11 @@
  * L = 3 uses the tau-method (Blanch et al. 1995): tau_eps_l = tau_sig_l (1+tau),
  * tau_sig_l = 1/(2 pi f_l), f_l = freq * {1/4, 1, 4}, tau fixed by Q(2 pi freq) = Q.
  * The adjoint is the exact transpose of the discrete Born operator (pinned
- * by the dot-product test in tests/test_oracle2d.py).
+ * by the dot-product test in tests/test_oracle2d.py); adjoint_strips takes the
+ * background's E and F from a reversal with ring strips instead of the
+ * stored history (save_and_set_wavefiled, viscoelastic_equation.cu:3-21;
+ * the reversal of rfwd_..._old, :1298-1435), pinned by equality with the
+ * history adjoint and the same dot-product test.
  * The imaging step also keeps the receiver vector P of each cell at its
  * imaging step (r_vxp, r_vzp) for the angle gathers that oracle2d.py restates
  * (pinned by a flat reflector); the source illumination is composed in
