@@ -446,6 +446,7 @@ int erm_batch_set_shots(erm_batch* b, const erm_shot_window* shots, int nshots) 
         q.tile_y = b->ty;
         q.zchunk = b->kchunk;
         q.xcd_map = P.xcd_map == 2 ? 2 : 0;  // band order of the window's own one-shot tile lists
+        q.poynting = 0;                      // the batched steps keep no Poynting vectors
         erm_ctx* c = nullptr;
         int rc = erm_create(&q, b->device, &c);
         if (rc) { free_shots(b); return b->fail(rc, "shot %d: %s", s, erm_last_error(nullptr)); }

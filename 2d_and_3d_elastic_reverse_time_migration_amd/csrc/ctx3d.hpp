@@ -39,6 +39,7 @@ struct erm_ctx {
     int* ex_time = nullptr;
     float* ex[5] = {nullptr};   // amp tp vxp vyp vzp
     float* img[3] = {nullptr};  // tp pp ps
+    float* poyn[6] = {nullptr}; // receiver Poynting rp (x y z), rs (x y z) at the imaging step (p.poynting)
     float* rec = nullptr;       // [3][lt][nr]
     long* rec_idx = nullptr;
     int nr = 0;

@@ -222,6 +222,7 @@ int parse_par(int argc, const char* const* argv, erm_params* p, std::string* err
     t.geti("tile_y", &p->tile_y);
     t.geti("zchunk", &p->zchunk);
     t.geti("xcd_map", &p->xcd_map);
+    t.geti("poynting", &p->poynting);
     if (p->fd_order != 8 && p->fd_order != 12) {
         if (err) *err = "fd_order must be 8 or 12";
         return ERM_E_PARAM;

@@ -839,6 +839,32 @@ void k_velocity_batch(const ShotStep* __restrict__ shots,
     }
 }
 
+// Receiver Poynting vectors at each cell's imaging step (erm_params.poynting),
+// the body of cuda_cal_receiver_poyn_3D, whose call is commented out at
+// 3D_forward_or_back_together.cu:432 and whose body is absent; the 3D form of
+// sum_poynting (2D_elastic_RTM/elastic_2D_kernel_2.cu:8-51):
+//   rp = -tp vP,   rs = -(sigma - tp I) vS,   vS = v - vP,
+// every product at the cell's own index, as sum_poynting.  Runs after the
+// velocity update of receiver step it, so the fields are those the imaging
+// condition of the same step reads.  Pointwise over the n allocated cells of
+// a volume; ex_time is -1 outside the updated cells.
+[[maybe_unused]] static __global__ void k_poynting(Wave w, const int* __restrict__ ex_time, float* __restrict__ rpx,
+                                                   float* __restrict__ rpy, float* __restrict__ rpz,
+                                                   float* __restrict__ rsx, float* __restrict__ rsy,
+                                                   float* __restrict__ rsz, long n, int it) {
+    for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+        if (ex_time[q] != it) continue;
+        const float tp = w.tp[q], px = w.vxp[q], py = w.vyp[q], pz = w.vzp[q];
+        const float sx = w.vx[q] - px, sy = w.vy[q] - py, sz = w.vz[q] - pz;
+        const float xx = w.txx[q] - tp, yy = w.tyy[q] - tp, zz = w.tzz[q] - tp;
+        const float xy = w.txy[q], xz = w.txz[q], yz = w.tyz[q];
+        rpx[q] = -tp * px; rpy[q] = -tp * py; rpz[q] = -tp * pz;
+        rsx[q] = -(xx * sx + xy * sy + xz * sz);
+        rsy[q] = -(xy * sx + yy * sy + yz * sz);
+        rsz[q] = -(xz * sx + yz * sy + zz * sz);
+    }
+}
+
 // Receivers of all shots: shot_of[r] names the shot of global receiver r,
 // RecShot holds that shot's fields, record and the first global index.
 struct RecShot {

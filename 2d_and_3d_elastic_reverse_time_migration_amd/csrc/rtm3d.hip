@@ -478,6 +478,7 @@ void free_all(erm_ctx* c) {
     for (auto& q : c->fr) vfree(c, q);
     F(c->strip_idx); F(c->strips); F(c->wavelet_neg); F(c->hist);
     for (auto& q : c->ximg) vfree(c, q);
+    for (auto& q : c->poyn) vfree(c, q);
     vfree(c, c->decbuf);
     for (auto& e : c->pass_ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->kev) (void)hipEventDestroy(e);
@@ -627,6 +628,10 @@ int run_pass(erm_ctx* c, int pass, int it_begin, int nsteps, bool strips = false
         if (strips && pass == ERM_PASS_SOURCE) strip_stress(c, c->f, it, 0);
         do_velocity(c, which, pass == ERM_PASS_SOURCE ? MODE_EXCITE : pass == ERM_PASS_RECEIVER ? MODE_IMAGE : MODE_PLAIN, it);
         if (c->profiling) HIPC(c, hipEventRecord(c->kev[3 * s + 2], c->stream));
+        if (pass == ERM_PASS_RECEIVER && c->poyn[0])
+            hipLaunchKernelGGL(k_poynting, dim3((unsigned)std::min<size_t>((c->ncell + 255) / 256, 16384)), dim3(256), 0,
+                               c->stream, make_wave(c), (const int*)c->ex_time, c->poyn[0], c->poyn[1], c->poyn[2],
+                               c->poyn[3], c->poyn[4], c->poyn[5], (long)c->ncell, it);
         if (strips && pass == ERM_PASS_SOURCE) strip_velocity(c, c->f, it + 1, 0);
         if (hist && (it + 1) % c->hist_decim == 0) {  // a stored step of the decimated history
             const erm_params& P = c->p;
@@ -779,6 +784,7 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
     std::string err;
     int rc = check_grid(pin, R, &err);
     if (rc) { set_global_error("%s", err.c_str()); return rc; }
+    if (pin->poynting != 0 && pin->poynting != 1) { set_global_error("poynting must be 0 or 1"); return ERM_E_PARAM; }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) { set_global_error("no HIP device"); return ERM_E_DEVICE; }
     if (device < 0 || device >= ndev) { set_global_error("device %d out of range (%d devices)", device, ndev); return ERM_E_PARAM; }
@@ -942,6 +948,8 @@ int erm_create(const erm_params* pin, int device, erm_ctx** out) {
         }
         CK(dmalloc(c, &c->rec, (size_t)3 * pin->lt * c->nr));
     }
+    if (pin->poynting == 1)
+        for (auto& q : c->poyn) CK(vmalloc(c, &q, c->ncell));
     CK(erm_zero_wavefield(c));
     CK(erm_zero_excitation(c));
     for (auto& p : c->img) CK(fill(c, p, c->ncell, 0.f));
@@ -1059,6 +1067,7 @@ int erm_zero_excitation(erm_ctx* c) {
     if (!c) return ERM_E_PARAM;
     if (filli(c, c->ex_time, c->ncell, -1)) return ERM_E_DEVICE;
     for (auto& p : c->ex) if (fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
+    for (auto& p : c->poyn) if (p && fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
     return 0;
 }
 
@@ -1093,6 +1102,7 @@ int erm_backward_smoothed(erm_ctx* c, const float* rec) {
     if (rec && (rc = erm_set_record(c, rec))) return rc;
     if ((rc = erm_zero_wavefield(c))) return rc;
     for (auto& p : c->img) if (fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
+    for (auto& p : c->poyn) if (p && fill(c, p, c->ncell, 0.f)) return ERM_E_DEVICE;
     return run_pass(c, ERM_PASS_RECEIVER, c->p.lt - 1, c->p.lt);
 }
 
@@ -1139,6 +1149,18 @@ int erm_get_field(erm_ctx* c, int field, float* dst) {
     }
     if (field >= ERM_F_EX_AMP && field <= ERM_F_EX_VZP) return download(c, c->ex[field - ERM_F_EX_AMP], dst);
     if (field >= ERM_F_IMG_TP && field <= ERM_F_IMG_PS) return download(c, c->img[field - ERM_F_IMG_TP], dst);
+    if (field >= ERM_F_POYN_SX && field <= ERM_F_POYN_SZ) {  // -ex_tp ex_vP
+        std::vector<float> tp(N);
+        int rc = download(c, c->ex[1], tp.data());
+        if (!rc) rc = download(c, c->ex[2 + field - ERM_F_POYN_SX], dst);
+        if (rc) return rc;
+        for (size_t n = 0; n < N; ++n) dst[n] = -tp[n] * dst[n];
+        return 0;
+    }
+    if (field >= ERM_F_POYN_RPX && field <= ERM_F_POYN_RSZ) {
+        if (!c->poyn[0]) return c->fail(ERM_E_STATE, "receiver Poynting vectors need poynting = 1 at erm_create");
+        return download(c, c->poyn[field - ERM_F_POYN_RPX], dst);
+    }
     if (field >= ERM_F_DIV && field <= ERM_F_CURLZ) {
         int rc = decompose(c, field - ERM_F_DIV);
         return rc ? rc : download(c, c->decbuf, dst);

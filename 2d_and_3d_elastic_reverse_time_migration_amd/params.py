@@ -12,6 +12,7 @@ FIELDS = [
     "vx", "vy", "vz", "txx", "tyy", "tzz", "txy", "txz", "tyz", "tp", "vxp", "vyp", "vzp",
     "vxs", "vys", "vzs", "ex_time", "ex_amp", "ex_tp", "ex_vxp", "ex_vyp", "ex_vzp",
     "img_tp", "img_pp", "img_ps", "div", "curlx", "curly", "curlz",
+    "poyn_sx", "poyn_sy", "poyn_sz", "poyn_rpx", "poyn_rpy", "poyn_rpz", "poyn_rsx", "poyn_rsy", "poyn_rsz",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
@@ -39,8 +40,8 @@ _PARAM_FIELDS = (
                                     "fd_order", "boundary", "lt_forward"]]
     + [("cpml_r", ctypes.c_float), ("cpml_vmax", ctypes.c_float)]
     + [(k, ctypes.c_int) for k in ["tile_x", "tile_y", "zchunk", "pitch_align", "xcd_map", "cpml_split",
-                                   "region_streams"]]
-    + [("reserved", ctypes.c_int * 4)]
+                                   "region_streams", "poynting"]]
+    + [("reserved", ctypes.c_int * 3)]
 )
 
 

@@ -106,7 +106,11 @@ typedef struct erm_params {
     int   region_streams; /* 1: the two region launches of a step run concurrently on two streams (they
                              write disjoint cells and neither writes what the other reads; same results);
                              0 (default, measured faster: 18.8 vs 18.3 Gcell/s): one stream */
-    int   reserved[4];
+    int   poynting;     /* 1: the receiver pass (erm_backward_smoothed, erm_rtm_step) also keeps, at each
+                           cell's imaging step, the receiver P and S Poynting vectors (ERM_F_POYN_R*): the
+                           cuda_cal_receiver_poyn_3D call of 3D_forward_or_back_together.cu:432, commented
+                           out in the reference; one-shot contexts only (0 in batched windows) */
+    int   reserved[3];
 } erm_params;
 
 /* field ids for erm_get_field / erm_set_field */
@@ -124,6 +128,17 @@ enum erm_field {
      * at the syz node, curl_y = D+z vx/dz - D-x vz/dx at the sxz node,
      * curl_z = D-x vy/dx - D-y vx/dy at the sxy node; 0 on the outer R cells. */
     ERM_F_DIV, ERM_F_CURLX, ERM_F_CURLY, ERM_F_CURLZ,
+    /* Poynting vectors (read only; the bodies of cuda_cal_source_poyn_3D and
+     * cuda_cal_receiver_poyn_3D, whose calls are commented out at
+     * 3D_forward_or_back_together.cu:283,432, are absent; this is the 3D form of
+     * sum_poynting, 2D_elastic_RTM/elastic_2D_kernel_2.cu:8-51):
+     * source P at the excitation step, -ex_tp ex_vP (derived from the excitation
+     * state); receiver P and S at the imaging step (poynting = 1), -tp vP and
+     * -(sigma - tp I) vS with vS = v - vP, each product at the cell's own index;
+     * 0 where the step was not reached. */
+    ERM_F_POYN_SX, ERM_F_POYN_SY, ERM_F_POYN_SZ,
+    ERM_F_POYN_RPX, ERM_F_POYN_RPY, ERM_F_POYN_RPZ,
+    ERM_F_POYN_RSX, ERM_F_POYN_RSY, ERM_F_POYN_RSZ,
     ERM_F_COUNT
 };
 

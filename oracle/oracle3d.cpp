@@ -104,6 +104,7 @@ struct Oracle3D {
     std::vector<int> ex_time;
     std::vector<T> ex_amp, ex_tp, ex_vxp, ex_vyp, ex_vzp;
     std::vector<T> img_tp, img_pp, img_ps;
+    std::vector<T> poyn[6];               /* receiver Poynting rp, rs at the imaging step (p.poynting) */
     std::vector<T> rec;                   /* [3][lt][ry][rx] */
     /* CPML (no reference counterpart): per axis a, b at integer ("full") and
      * staggered ("half": x-1/2, y-1/2, z+1/2) positions, and 21 memory
@@ -153,6 +154,7 @@ struct Oracle3D {
         ex_time.assign(N, -1);
         ex_amp.assign(N, 0); ex_tp.assign(N, 0); ex_vxp.assign(N, 0); ex_vyp.assign(N, 0); ex_vzp.assign(N, 0);
         img_tp.assign(N, 0); img_pp.assign(N, 0); img_ps.assign(N, 0);
+        for (auto& v : poyn) v.assign(p.poynting == 1 ? N : 0, 0);
         rec.assign((size_t)3 * p.lt * p.receiver_num_y * p.receiver_num_x, 0);
         return set_shot(0, 0, 0);
     }
@@ -382,6 +384,20 @@ struct Oracle3D {
         T ps = ex * (nvx - nxp) + ey * (nvy - nyp) + ez * (nvz - nzp);
         img_pp[n] = den > (T)0 ? pp / den : (T)0;
         img_ps[n] = den > (T)0 ? ps / den : (T)0;
+        if (!poyn[0].empty()) poynting_cell(n, tpv, nvx, nvy, nvz, nxp, nyp, nzp);
+    }
+    /* cuda_cal_receiver_poyn_3D (call commented out at :432, body absent) as
+     * the 3D form of sum_poynting (elastic_2D_kernel_2.cu:8-51) at the cell's
+     * imaging step: rp = -tp vP, rs = -(sigma - tp I) vS with vS = v - vP,
+     * every product at the cell's own index. */
+    void poynting_cell(size_t n, T tpv, T nvx, T nvy, T nvz, T nxp, T nyp, T nzp) {
+        const T sx = nvx - nxp, sy = nvy - nyp, sz = nvz - nzp;
+        const T xx = f[3][n] - tpv, yy = f[4][n] - tpv, zz = f[5][n] - tpv;
+        const T xy = f[6][n], xz = f[7][n], yz = f[8][n];
+        poyn[0][n] = -tpv * nxp; poyn[1][n] = -tpv * nyp; poyn[2][n] = -tpv * nzp;
+        poyn[3][n] = -(xx * sx + xy * sy + xz * sz);
+        poyn[4][n] = -(xy * sx + yy * sy + yz * sz);
+        poyn[5][n] = -(xz * sx + yz * sy + zz * sz);
     }
     /* The imaging condition alone, applied to the current fields at step it
      * (no propagation): pins the condition on prescribed receiver fields. */
@@ -461,8 +477,12 @@ struct Oracle3D {
     void zero_excitation() {
         std::fill(ex_time.begin(), ex_time.end(), -1);
         for (auto* v : {&ex_amp, &ex_tp, &ex_vxp, &ex_vyp, &ex_vzp}) std::fill(v->begin(), v->end(), (T)0);
+        for (auto& v : poyn) std::fill(v.begin(), v.end(), (T)0);
     }
-    void zero_images() { for (auto* v : {&img_tp, &img_pp, &img_ps}) std::fill(v->begin(), v->end(), (T)0); }
+    void zero_images() {
+        for (auto* v : {&img_tp, &img_pp, &img_ps}) std::fill(v->begin(), v->end(), (T)0);
+        for (auto& v : poyn) std::fill(v.begin(), v.end(), (T)0);
+    }
 
     /* P/S decomposition by divergence and curl (the 3D form of decom,
      * elastic_2D_kernel_2.cu:621-666, with the y positions of the 3D grid):
@@ -492,6 +512,17 @@ struct Oracle3D {
         if (fid >= ERM_F_VXS && fid <= ERM_F_VZS) {
             int c3 = fid - ERM_F_VXS;
             for (size_t n = 0; n < N; ++n) dst[n] = (double)(f[c3][n] - f[10 + c3][n]);
+            return 0;
+        }
+        if (fid >= ERM_F_POYN_SX && fid <= ERM_F_POYN_SZ) {  /* source P at the excitation step */
+            const std::vector<T>& v = fid == ERM_F_POYN_SX ? ex_vxp : fid == ERM_F_POYN_SY ? ex_vyp : ex_vzp;
+            for (size_t n = 0; n < N; ++n) dst[n] = (double)(-ex_tp[n] * v[n]);
+            return 0;
+        }
+        if (fid >= ERM_F_POYN_RPX && fid <= ERM_F_POYN_RSZ) {
+            const std::vector<T>& v = poyn[fid - ERM_F_POYN_RPX];
+            if (v.empty()) return ERM_E_STATE;
+            for (size_t n = 0; n < N; ++n) dst[n] = (double)v[n];
             return 0;
         }
         const std::vector<T>* src = nullptr;

@@ -573,6 +573,81 @@ def test_imaging_condition_closed_forms(oracle):
     o.close()
 
 
+def test_poynting_vectors_closed_forms(oracle):
+    """Receiver Poynting vectors at the imaging step (poynting = 1, the 3D form
+    of sum_poynting) on the local state of plane waves with unit direction k,
+    prescribed at every cell and imaged at each cell's excitation step:
+      P wave (strain f k k^T): tp = M f, v = vP = -c_p f k, vS = 0:
+        rp = M c_p f^2 k, rs = 0;
+      S wave (polarisation e, e.k = 0): sigma = mu g (e k^T + k e^T), tp = 0,
+        v = vS = -c_s g e:  rp = 0, rs = mu c_s g^2 k;
+      both together: rp as the P wave's, rs = mu c_s g^2 k + 2 mu f vS (the
+        P stress minus tp I, 2 mu f (k k^T - I), acting on vS).
+    The source vector is -ex_tp ex_vP of the excitation state.  A missing
+    '- tp I', a transposed stress index or a sign flip changes these values
+    (the first is checked below on the same inputs)."""
+    p = cases.params(nx=16, ny=14, nz=12, b=6, R=4, lt=30, dt=1.0, freq=25.0, src=(8, 7, 4))
+    p.lt_forward = p.lt
+    p.poynting = 1
+    vp, vs, rho = cases.layered(p, seed=5)
+    o, ex = _excited_state(oracle, p, vp, vs, rho)
+    et = ex["ex_time"]
+    hit = et >= 0
+    assert hit.sum() > 100
+    rng = np.random.default_rng(4)
+    lam, mu = 2.0e9, 1.5e9
+    M = lam + 2 * mu
+    cp, cs = 3000.0, 1800.0
+    names = ("vx", "vy", "vz")
+
+    def prescribed(f, g, k, e):
+        kk = np.outer(k, k)
+        eps_p = f * kk
+        sig = lam * np.trace(eps_p) * np.eye(3) + 2 * mu * eps_p + mu * g * (np.outer(e, k) + np.outer(k, e))
+        vP = -cp * f * k
+        vS = -cs * g * e
+        o.zero(wavefield=True, images=True)
+        shape = et.shape
+        for c in range(3):
+            o.set_field(names[c], np.full(shape, vP[c] + vS[c]))
+            o.set_field("v" + "xyz"[c] + "p", np.full(shape, vP[c]))
+        for name, (a, b) in zip(("txx", "tyy", "tzz", "txy", "txz", "tyz"),
+                                ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+            o.set_field(name, np.full(shape, sig[a, b]))
+        o.set_field("tp", np.full(shape, M * f))
+        for it in range(p.lt_forward):
+            o.image(it)
+        rp = np.stack([o.get_field(n)[hit] for n in ("poyn_rpx", "poyn_rpy", "poyn_rpz")])
+        rs = np.stack([o.get_field(n)[hit] for n in ("poyn_rsx", "poyn_rsy", "poyn_rsz")])
+        return rp, rs, sig, vS
+
+    for _ in range(3):
+        k = rng.standard_normal(3)
+        k /= np.linalg.norm(k)
+        e = np.cross(k, rng.standard_normal(3))
+        e /= np.linalg.norm(e)
+        f, g = 1e-6 * rng.uniform(0.5, 2), 1e-6 * rng.uniform(0.5, 2)
+        rp, rs, _, _ = prescribed(f, 0.0, k, e)
+        np.testing.assert_allclose(rp, np.broadcast_to((M * cp * f * f * k)[:, None], rp.shape), rtol=1e-10)
+        assert np.max(np.abs(rs)) <= 1e-12 * M * cp * f * f
+        rp, rs, _, _ = prescribed(0.0, g, k, e)
+        assert np.max(np.abs(rp)) == 0.0
+        np.testing.assert_allclose(rs, np.broadcast_to((mu * cs * g * g * k)[:, None], rs.shape),
+                                   rtol=1e-9, atol=1e-12 * mu * cs * g * g)
+        rp, rs, sig, vS = prescribed(f, g, k, e)
+        want = mu * cs * g * g * k + 2 * mu * f * vS
+        np.testing.assert_allclose(rp, np.broadcast_to((M * cp * f * f * k)[:, None], rp.shape), rtol=1e-10)
+        np.testing.assert_allclose(rs, np.broadcast_to(want[:, None], rs.shape), rtol=1e-9,
+                                   atol=1e-12 * np.abs(want).max())
+        wrong = -sig @ vS  # without '- tp I'
+        assert not np.allclose(wrong, want, rtol=1e-3)
+    # cells never imaged keep zero vectors; the source vector is -ex_tp ex_vP
+    assert np.all(o.get_field("poyn_rpx")[~hit] == 0)
+    for c, n in enumerate(("poyn_sx", "poyn_sy", "poyn_sz")):
+        np.testing.assert_array_equal(o.get_field(n), -ex["ex_tp"] * ex["ex_v" + "xyz"[c] + "p"])
+    o.close()
+
+
 def test_div_curl_discrete_identities_3d(oracle):
     """3D divergence / curl decomposition (the 3D form of decom,
     elastic_2D_kernel_2.cu:621-666).  Staggered fields from random
