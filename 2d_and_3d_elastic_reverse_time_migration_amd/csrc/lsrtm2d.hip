@@ -50,7 +50,8 @@ int resolve_scratch(int device) {
 const double kC6[7] = {0.0, 1.2213363647, -0.096931457519, 0.017447662353,
                        -0.0029672895159, 0.00035900539822, -0.00002184781161};
 const double kC4[5] = {0.0, 1225.0 / 1024.0, -245.0 / 3072.0, 49.0 / 5120.0, -5.0 / 7168.0};
-constexpr int TXR = 4;  // rows per workgroup (64 x 4 = 256 threads)
+constexpr int TXR = 4;       // rows per workgroup (64 x 4 = 256 threads)
+constexpr int TXR_TALL = 8;  // the taller stencil workgroups of large launches (erm2d_params.tile_rows)
 
 struct Sls {
     double fM, fU, gP[3], gS[3], taus[3];
@@ -248,7 +249,16 @@ Ex2 excite(const erm2d_ctx* c) {
     e.tp_time = c->ex_tp_time; e.tp_amp = c->ex_tp_amp; e.tp_shift = c->p.ex_tp_shift;
     return e;
 }
-dim3 grid2(const erm2d_ctx* c) { return dim3((c->nnz + TZ - 1) / TZ, (c->nnx + TXR - 1) / TXR, nshot_of(c)); }
+dim3 grid2(const erm2d_ctx* c, int rows = TXR) { return dim3((c->nnz + TZ - 1) / TZ, (c->nnx + rows - 1) / rows, nshot_of(c)); }
+// x rows per workgroup of the stress / velocity / adjoint stencil launches
+// (erm2d_params.tile_rows): the taller workgroups re-read fewer x-halo rows
+// per row but halve the workgroup count, so by default only launches that
+// still have >= 4096 of them (batched shots) use them
+int rows(const erm2d_ctx* c) {
+    if (c->p.tile_rows) return c->p.tile_rows;
+    const long wg = (long)((c->nnz + TZ - 1) / TZ) * ((c->nnx + TXR_TALL - 1) / TXR_TALL) * nshot_of(c);
+    return wg >= 4096 ? TXR_TALL : TXR;
+}
 dim3 grid_pt(const erm2d_ctx* c) { return dim3((c->nnz + 255) / 256, c->nnx, nshot_of(c)); }
 
 // geometry of a launch on state A (the C-PML memory variables belong to it) or B
@@ -259,8 +269,12 @@ G2 geo_of(const erm2d_ctx* c, bool B) {
 }
 template <int R, int L, bool SAVE>
 void stress_r(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
-    hipLaunchKernelGGL((k2_stress<R, L, TXR, SAVE>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, B), model(c, which),
-                       geo_of(c, B), c->coef, src, sv, Eh);
+    if (rows(c) == TXR_TALL)
+        hipLaunchKernelGGL((k2_stress<R, L, TXR_TALL, SAVE>), grid2(c, TXR_TALL), dim3(TZ, TXR_TALL), 0, c->stream,
+                           state(c, B), model(c, which), geo_of(c, B), c->coef, src, sv, Eh);
+    else
+        hipLaunchKernelGGL((k2_stress<R, L, TXR, SAVE>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, B),
+                           model(c, which), geo_of(c, B), c->coef, src, sv, Eh);
 }
 template <int R, bool SAVE>
 void stress_l(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
@@ -274,8 +288,12 @@ void do_stress(erm2d_ctx* c, bool B, int which, long src, float sv, float* Eh) {
 }
 template <int R, int MODE, bool SAVE>
 void vel_r(erm2d_ctx* c, bool B, int which, int it, float* Fh) {
-    hipLaunchKernelGGL((k2_velocity<R, TXR, MODE, SAVE>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, B),
-                       model(c, which), geo_of(c, B), c->coef, excite(c), it, Fh);
+    if (rows(c) == TXR_TALL)
+        hipLaunchKernelGGL((k2_velocity<R, TXR_TALL, MODE, SAVE>), grid2(c, TXR_TALL), dim3(TZ, TXR_TALL), 0, c->stream,
+                           state(c, B), model(c, which), geo_of(c, B), c->coef, excite(c), it, Fh);
+    else
+        hipLaunchKernelGGL((k2_velocity<R, TXR, MODE, SAVE>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, B),
+                           model(c, which), geo_of(c, B), c->coef, excite(c), it, Fh);
 }
 template <int R>
 void vel_m(erm2d_ctx* c, bool B, int which, int mode, int it, float* Fh) {
@@ -299,10 +317,15 @@ void born_stress_src(erm2d_ctx* c, const float* Eh) {
     else if (c->L == 1) hipLaunchKernelGGL((k2_born_stress_src<1>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh, para(c));
     else hipLaunchKernelGGL((k2_born_stress_src<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), c->dm, Eh, para(c));
 }
+template <int R, int TR>
+void adj_v_r(erm2d_ctx* c, const float* Fh) {
+    hipLaunchKernelGGL((k2a_v<R, TR>), grid2(c, TR), dim3(TZ, TR), 0, c->stream, state(c, true), model(c, 1), geo(c),
+                       c->coef, Fh, grad_of(c));
+}
 void adj_v(erm2d_ctx* c, const float* Fh) {
-    float* gr = grad_of(c);
-    if (c->R == 4) hipLaunchKernelGGL((k2a_v<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, gr);
-    else hipLaunchKernelGGL((k2a_v<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef, Fh, gr);
+    const bool t = rows(c) == TXR_TALL;
+    if (c->R == 4) { if (t) adj_v_r<4, TXR_TALL>(c, Fh); else adj_v_r<4, TXR>(c, Fh); }
+    else { if (t) adj_v_r<6, TXR_TALL>(c, Fh); else adj_v_r<6, TXR>(c, Fh); }
 }
 void adj_s1(erm2d_ctx* c, const float* Eh) {
     dim3 g = grid_pt(c);
@@ -312,18 +335,30 @@ void adj_s1(erm2d_ctx* c, const float* Eh) {
     else if (c->L == 1) hipLaunchKernelGGL((k2a_s1<1>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, gr, W[0], W[1], W[2], c->R);
     else hipLaunchKernelGGL((k2a_s1<3>), g, dim3(256), 0, c->stream, state(c, true), model(c, 1), geo(c), Eh, gr, W[0], W[1], W[2], c->R);
 }
-void adj_s2(erm2d_ctx* c) {
+template <int R, int TR>
+void adj_s2_r(erm2d_ctx* c) {
     float* const* W = w_of(c);
-    if (c->R == 4) hipLaunchKernelGGL((k2a_s2<4, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, W[0], W[1], W[2]);
-    else hipLaunchKernelGGL((k2a_s2<6, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), geo(c), c->coef, W[0], W[1], W[2]);
+    hipLaunchKernelGGL((k2a_s2<R, TR>), grid2(c, TR), dim3(TZ, TR), 0, c->stream, state(c, true), geo(c), c->coef, W[0],
+                       W[1], W[2]);
+}
+void adj_s2(erm2d_ctx* c) {
+    const bool t = rows(c) == TXR_TALL;
+    if (c->R == 4) { if (t) adj_s2_r<4, TXR_TALL>(c); else adj_s2_r<4, TXR>(c); }
+    else { if (t) adj_s2_r<6, TXR_TALL>(c); else adj_s2_r<6, TXR>(c); }
 }
 
 // reference-mode adjoint steps on state B (see k2r_s / k2r_v)
+template <int R, int L, int TR>
+void ref_step_rlt(erm2d_ctx* c, const float* Eh) {
+    hipLaunchKernelGGL((k2r_s<R, L, TR>), grid2(c, TR), dim3(TZ, TR), 0, c->stream, state(c, true), model(c, 1), geo(c),
+                       c->coef);
+    hipLaunchKernelGGL((k2r_v<R, L, TR>), grid2(c, TR), dim3(TZ, TR), 0, c->stream, state(c, true), model(c, 1), geo(c),
+                       c->coef, Eh, grad_of(c));
+}
 template <int R, int L>
 void ref_step_rl(erm2d_ctx* c, const float* Eh) {
-    hipLaunchKernelGGL((k2r_s<R, L, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef);
-    hipLaunchKernelGGL((k2r_v<R, L, TXR>), grid2(c), dim3(TZ, TXR), 0, c->stream, state(c, true), model(c, 1), geo(c), c->coef,
-                       Eh, grad_of(c));
+    if (rows(c) == TXR_TALL) ref_step_rlt<R, L, TXR_TALL>(c, Eh);
+    else ref_step_rlt<R, L, TXR>(c, Eh);
 }
 template <int R>
 void ref_step_r(erm2d_ctx* c, const float* Eh) {
@@ -468,6 +503,7 @@ int erm2d_create(const erm2d_params* pin, int device, erm2d_ctx** out) {
     if (p.boundary != ERM_BOUNDARY_DAMP && p.boundary != ERM_BOUNDARY_CPML) { g_err2 = "boundary must be 0 (damping) or 1 (CPML)"; return ERM_E_PARAM; }
     if (p.inversion_para < 0 || p.inversion_para > 3) { g_err2 = "inversion_para must be 1, 2 or 3 (0: 2)"; return ERM_E_PARAM; }
     if (p.adjoint_mode != 0 && p.adjoint_mode != 1) { g_err2 = "adjoint_mode must be 0 (exact transpose) or 1 (reference)"; return ERM_E_PARAM; }
+    if (p.tile_rows != 0 && p.tile_rows != TXR && p.tile_rows != TXR_TALL) { g_err2 = "tile_rows must be 0, 4 or 8"; return ERM_E_PARAM; }
     if (p.adjoint_source != ERM2D_SRC_HISTORY && p.adjoint_source != ERM2D_SRC_STRIPS) {
         g_err2 = "adjoint_source must be 0 (history) or 2 (strips)";
         return ERM_E_PARAM;

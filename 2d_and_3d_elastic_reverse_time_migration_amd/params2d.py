@@ -22,7 +22,7 @@ class Params2D(ctypes.Structure):
                 + [("qcomp", ctypes.c_int), ("ex_tp_shift", ctypes.c_int), ("ex_vzp_avg", ctypes.c_int),
                    ("boundary", ctypes.c_int), ("cpml_r", ctypes.c_float), ("cpml_vmax", ctypes.c_float),
                    ("inversion_para", ctypes.c_int), ("adjoint_mode", ctypes.c_int),
-                   ("adjoint_source", ctypes.c_int)])
+                   ("adjoint_source", ctypes.c_int), ("tile_rows", ctypes.c_int)])
 
     def update(self, **kw):
         for k, v in kw.items():

@@ -64,6 +64,11 @@ typedef struct erm2d_params {
                           stored per half step ([lt][8][ring] floats per shot) and the background
                           stepped backward, E and F recomputed on the way (elastic media, nsls = 0;
                           equal to the history gradient up to FP32 rounding of the reversal) */
+    int   tile_rows;      /* x rows per workgroup of the stencil launches (64 z lanes each): 0 (default) =
+                          8 when a launch of 8-row workgroups has at least 4096 of them (batched shots;
+                          measured 12.7-13.0 against 13.4 s per CG iteration at 600x300, 12 shots per launch),
+                          else 4 (a single small shot then still fills the GPU); or 4, 8.  Results do
+                          not depend on it */
 } erm2d_params;
 
 /* fields for erm2d_get_field / erm2d_set_field (padded grid, z fastest) */

@@ -50,6 +50,37 @@ def test_gpu2d_batched_born_adjoint_equal_one_shot(erm, L, R, para):
         g.set_batch(1)
 
 
+@pytest.mark.parametrize("L,R,mode", [(0, 4, 0), (3, 6, 0), (3, 4, 1)])
+def test_gpu2d_tall_workgroups_give_the_same_operators(erm, L, R, mode):
+    """tile_rows = 8 (the taller stencil workgroups that large batched
+    launches use by default) against tile_rows = 4: Born records bit-identical
+    (the same arithmetic per cell), adjoint gradients equal to the last bits,
+    batched and one shot, for the exact and the reference-mode adjoint."""
+    import test_oracle2d as T
+    p, bg = T._lsrtm_case(L, R=R, lt=120)
+    p.adjoint_mode = mode
+    rng = np.random.default_rng(17)
+    dm = (rng.standard_normal((3, p.nx, p.nz)) * cases2d.taper_mask(p)).astype(np.float32)
+    shots = [(6, 2), (14, 3), (20, 2), (27, 4)]
+    r = rng.standard_normal((len(shots), 2, p.lt, p.nrec)).astype(np.float32)
+    print(f"tile_rows 8 vs 4: L={L} R={R} adjoint_mode={mode}", flush=True)
+    out = {}
+    for rows in (4, 8):
+        p.tile_rows = rows
+        with erm.Elastic2D(p) as g:
+            g.load_model(1, *bg)
+            g.set_shot(*shots[0])
+            one = (g.born(dm), g.adjoint(r[0]))
+            g.set_batch(len(shots))
+            out[rows] = one + (g.born_shots(shots, dm), g.adjoint_shots(shots, r))
+    for k in (0, 2):
+        np.testing.assert_array_equal(out[8][k], out[4][k])
+    for k in (1, 3):
+        for c in range(3):
+            e = cases.rel_l2(out[8][k][c], out[4][k][c])
+            assert e < 1e-6, (k, c, e)
+
+
 def test_gpu2d_batched_cg_follows_one_shot_cg(erm):
     p = cases2d.params(nx=40, nz=24, b=10, R=4, L=0, lt=160, freq=25.0, src=(20, 1), rec_dx=1, rec_z=1, nrec=38,
                        rec_x0=1)
