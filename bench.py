@@ -131,9 +131,9 @@ def batch_windows(n=24, nx=600, ny=600, seed=33, lo=192, hi=320):
     return out
 
 
-def batch_setup(erm, nt, boundary):
+def batch_setup(erm, nt, boundary, lo=192, hi=320):
     """The batched workload's parameters, true and migration models and shot
-    windows: 600x600x250 model (seed 3), 24 windows of 192..320 cells."""
+    windows: 600x600x250 model (seed 3), 24 windows of lo..hi cells (192..320)."""
     p = erm.default_params()
     p.update(nx=600, ny=600, nz=250, dx=10.0, dy=10.0, dz=10.0, dt=0.8, lt=nt, lt_forward=nt, freq=15.0,
              fd_order=8, boundary=boundary, coe_attenuation=220.0, bu=20, bd=20, bl=20, br=20, bf=20, bb=20,
@@ -142,7 +142,7 @@ def batch_setup(erm, nt, boundary):
     vp_s = smooth_box(vp, 2)
     vs_s = (vp_s / 1.732).astype(np.float32)
     rho_s = (230.0 * vp_s.astype(np.float64) ** 0.25).astype(np.float32)
-    return p, (vp, vs, rho), (vp_s, vs_s, rho_s), batch_windows()
+    return p, (vp, vs, rho), (vp_s, vs_s, rho_s), batch_windows(lo=lo, hi=hi)
 
 
 def run_batch(erm, args):
@@ -152,8 +152,9 @@ def run_batch(erm, args):
     of all 24 shots (source pass + receiver pass, nt steps each); records are
     modelled once (batched) before timing.  The launch shape is autotuned."""
     nt = args.nt
+    lo, hi = args.batch_window
     p, (vp, vs, rho), (vp_s, vs_s, rho_s), wins = batch_setup(
-        erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP)
+        erm, nt, erm.BOUNDARY_CPML if args.boundary == "cpml" else erm.BOUNDARY_DAMP, lo, hi)
     log(f"bench batch: {len(wins)} shots, nt={nt}, creating window contexts")
     b = erm.Batch3D(p, 0)
     b.load_model(0, vp, vs, rho)
@@ -213,10 +214,10 @@ def run_batch(erm, args):
         "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 1), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded dipping-layer + anomaly model, random shot layout)",
-        "config": {"workload": "3D elastic RTM, 24 shots, 600x600x250 model, windows 192-320 cells laterally "
+        "config": {"workload": "3D elastic RTM, 24 shots, 600x600x250 model, windows %d-%d cells laterally "
                                "(full depth) + 20-cell %s, 8th-order FD, dt=0.8 ms, nt=%d per pass, 15 Hz; "
                                "step = source + receiver pass of all shots in one batched graph-captured launch "
-                               "per kernel per step" % ("C-PML" if args.boundary == "cpml" else "damping", nt),
+                               "per kernel per step" % (lo, hi, "C-PML" if args.boundary == "cpml" else "damping", nt),
                    "cells_per_time_step": cells, "autotuned": tune, "parallelism": "single GPU"},
         "batched_pass_Gcells": round(batched_pass_value, 3),
         "one_shot_per_launch_Gcells": round(seq_value, 3),
@@ -697,6 +698,8 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="only run the plain copy kernel over GIB GiB (HBM counter calibration) and exit")
     ap.add_argument("--shots", type=int, default=60, help="visco workload: shots")
+    ap.add_argument("--batch-window", type=int, nargs=2, default=(192, 320), metavar=("LO", "HI"),
+                    help="batch workload: lateral window sizes (cells; the configured 192 320)")
     ap.add_argument("--iters", type=int, default=20, help="visco workload: CG iterations")
     ap.add_argument("--workers", type=int, default=4, help="visco workload: concurrent shot contexts")
     ap.add_argument("--batch", type=int, default=0, help="visco workload: shots per batched launch (0: workers)")
