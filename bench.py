@@ -604,6 +604,10 @@ def run_visco(erm, args):
     oracle_check = None
     if args.oracle_check:
         oracle_check = visco_oracle_check(erm, learned, p, (bvp, bvs, brho, qp, qs), dm)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench visco: CPU baseline (FP32 oracle, one shot)")
+        cpu = visco_cpu_baseline(learned, p, (bvp, bvs, brho, qp, qs), dm, len(shots), args.cpu_budget * 3)
     # operator throughput: per shot and iteration one Born (2 states) and one adjoint (forward + backward)
     updates = float(nnx * nnz) * p.lt * 4 * len(shots) * it
     line = {
@@ -619,9 +623,45 @@ def run_visco(erm, args):
         "model_error_vs_true": round(err, 4),
         "operator_Gcell_updates_per_s": round(updates / el / 1e9, 3),
         "oracle_check": oracle_check,
-        "roofline": None, "cpu_baseline": None,
+        "roofline": None, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def visco_cpu_baseline(learned, p, bg, dm, nshots, budget_s, nt_sample=200):
+    """FP32 oracle on the host (OpenMP, threads as the 3D baseline): one Born
+    and one adjoint application for one shot of the same 600x300 L = 3 problem
+    over nt_sample steps, median of 3, projected to the full nt and to the
+    shots of one CG iteration (one Born + one adjoint per shot)."""
+    import ctypes
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle3d
+    import oracle2d
+    aff, nproc, model = host_cpu()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or aff)
+    oracle3d.set_threads(threads)
+    q = learned.Params2D()
+    ctypes.memmove(ctypes.addressof(q), ctypes.addressof(p), ctypes.sizeof(q))
+    q.lt = q.lt_forward = min(nt_sample, p.lt)
+    o = oracle2d.Oracle2D(q, 32)
+    o.load_model(1, *bg)
+    r = np.random.default_rng(3).standard_normal((2, q.lt, q.nrec)).astype(np.float32)
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.born(dm)
+        o.adjoint(r)
+        runs.append(time.perf_counter() - t0)
+        if sum(runs) > budget_s:
+            break
+    o.close()
+    per_shot = float(np.median(runs)) * p.lt / q.lt
+    return {"value": round(per_shot, 2), "unit": "s per shot (Born + adjoint, full nt)", "cores": threads,
+            "kind": "port", "nproc": nproc, "affinity_cores": aff, "cpu_model": model,
+            "runs_s": [round(v, 3) for v in runs], "projected_s_per_cg_iteration": round(per_shot * nshots, 1),
+            "sample": f"median of {len(runs)} runs of one Born + one adjoint application of one shot over "
+                      f"{q.lt} of the {p.lt} steps, the same 600x300 L = 3 grid, FP32 oracle, {threads} threads; "
+                      f"projected linearly to {p.lt} steps"}
 
 
 def visco_oracle_check(erm, learned, p, bg, dm, iters=2, nt=500):
