@@ -22,6 +22,7 @@ rocprofv3 PMC summary (profiles/), and an FP32 CPU baseline (the oracle's
 FP32 instantiation, OpenMP) on a bounded sample of the same grid.
 """
 import argparse
+import ctypes
 import importlib
 import json
 import os
@@ -200,6 +201,21 @@ def run_batch(erm, args):
     seq_value = seq_updates / (seq_ms * 1e-3) / 1e9
     log(f"bench batch: pass-event rates: batched {batched_pass_value:.3f}, one shot per launch {seq_value:.3f} "
         f"Gcell-updates/s")
+    # CPU baseline: the FP32 oracle on the window closest to 256x256 (a representative size)
+    cpu = None
+    if not args.no_cpu_baseline:
+        k = int(np.argmin([abs(w[5] - 256) + abs(w[6] - 256) for w in wins]))
+        sx, sy, sz, x0, y0, wx, wy = wins[k]
+        q = erm.default_params()
+        ctypes.memmove(ctypes.addressof(q), ctypes.addressof(p), ctypes.sizeof(q))
+        q.nx, q.ny = wx, wy
+        q.shot_start_x, q.shot_start_y, q.shot_start_z = sx - x0, sy - y0, sz
+        q.receiver_num_x = q.receiver_num_y = 0
+        cut = [np.ascontiguousarray(a[x0:x0 + wx, y0:y0 + wy, :]) for a in (vp_s, vs_s, rho_s)]
+        log(f"bench batch: CPU baseline (FP32 oracle) on window {k}, {wx}x{wy}x{p.nz}")
+        cpu = cpu_baseline(q, *cut, args.cpu_budget)
+        cpu["sample"] = f"window {k} ({wx}x{wy}x{p.nz} + 20-cell C-PML): " + cpu["sample"].replace(
+            "of the same 296x296x168 grid", "of that window's grid")
     # per-step roofline of the batched step (both kernels): algorithmic bytes / autotuned step time
     step_bytes = cells * (76.0 + 84.0)
     roof = {"bound": "hbm", "achieved": round(step_bytes / (tune["ms_per_step"] * 1e-3) / 1e9, 1),
@@ -222,7 +238,7 @@ def run_batch(erm, args):
         "batched_pass_Gcells": round(batched_pass_value, 3),
         "one_shot_per_launch_Gcells": round(seq_value, 3),
         "roofline": roof,
-        "cpu_baseline": None,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
